@@ -1,0 +1,146 @@
+/*
+ * rgl_hip.h — C ABI of the MI355X (gfx950) rgl rasteriser pipeline.
+ *
+ * Each entry point replaces one member of the reference's rglv::GPU /
+ * rglv::GL surface (usrlocalben/rsr, src/rgl/rglv); the reference
+ * declaration it stands for is cited beside it.  Plain pointers and sizes
+ * only.  Pointers named dev_* are device (HBM) addresses the caller owns;
+ * they must stay valid until the frame that uses them has completed
+ * (rgl_gpu_wait), as the reference requires of its borrowed buffers until
+ * the next Run() completes (rglv_gpu.hxx:176-192).
+ *
+ * Every call returns 0 on success and a negative RGL_E_* code on failure;
+ * rgl_gpu_last_error() then describes it.  The reference calls
+ * std::exit(1) in the same situations (rglv_gpu.cxx:199-202, 300-309).
+ */
+#ifndef RGL_HIP_H
+#define RGL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rgl_gpu rgl_gpu;
+
+enum {
+	RGL_OK = 0,
+	RGL_E_ARG = -1,          /* invalid argument */
+	RGL_E_HIP = -2,          /* a HIP runtime call failed */
+	RGL_E_DISPATCH = -3,     /* no installed program for a state key */
+	RGL_E_UNSUPPORTED = -4,  /* state this pipeline does not implement */
+	RGL_E_CAPACITY = -5,     /* a per-frame limit was exceeded */
+	RGL_E_STATE = -6         /* call out of order */
+};
+
+/* install kinds (the three Install overloads, rglv_gpu.hxx:154-156) */
+enum { RGL_BIN_PROGRAM = 0, RGL_DRAW_PROGRAM = 1, RGL_BLT_PROGRAM = 2 };
+
+/* pipeline counters (GPUStats, rglv_gpu.hxx:51-62, plus fragment and bin counts) */
+typedef struct rgl_stats {
+	int64_t tiles;
+	int64_t commands;
+	int64_t primitives_submitted;
+	int64_t primitives_clipped;
+	int64_t primitives_binned;   /* triangles accepted into at least one tile */
+	int64_t tile_pairs;          /* (triangle, tile) pairs in the tile lists */
+	int64_t clipped_tile_pairs;  /* (clipped triangle, tile) pairs */
+	int64_t fragments;           /* covered pixels that reached the depth test */
+	int64_t overflow;            /* nonzero: a capacity was exceeded, frame invalid */
+} rgl_stats;
+
+/* --- rglv::GPU ---------------------------------------------------------- */
+
+/* GPU(int concurrency, std::string guid)   rglv_gpu.hxx:153.
+   device_ordinal selects the HIP device; concurrency is accepted for API
+   compatibility (the device schedules its own tiles). */
+int rgl_gpu_create(int device_ordinal, int concurrency, const char* guid, rgl_gpu** out);
+int rgl_gpu_destroy(rgl_gpu* gpu);
+
+/* Install(programId, stateKey, Bin/Draw/BltProgramPtrs)   rglv_gpu.hxx:154-156,
+   keyed (programId << 24) | stateKey as in rglv_gpu.cxx:26-47. */
+int rgl_gpu_install(rgl_gpu* gpu, int program_id, uint32_t state_key, int kind);
+
+/* Reset(ivec2 bufferDimensionsInPixels, ivec2 tileDimensionsInBlocks)
+   rglv_gpu.hxx:159; tiles are tile_blocks * 8 pixels (rglv_gpu.hxx:27).
+   Also resets the recording context (GL::Reset, rglv_gl.cxx:440-445). */
+int rgl_gpu_reset(rgl_gpu* gpu, int width, int height, int tile_blocks_x, int tile_blocks_y);
+
+/* Run()   rglv_gpu.hxx:167-168: bin and draw everything recorded since
+   Reset, asynchronously on hip_stream (a hipStream_t; NULL = the default
+   stream).  Completion is observed with rgl_gpu_wait. */
+int rgl_gpu_run(rgl_gpu* gpu, void* hip_stream);
+
+/* Blocks until the last Run has completed and checks its device counters. */
+int rgl_gpu_wait(rgl_gpu* gpu);
+
+/* GPUStats of the last completed Run. */
+int rgl_gpu_stats(rgl_gpu* gpu, rgl_stats* out);
+
+/* Average duration in milliseconds of each pipeline kernel over the runs
+   since the last call (event-timed on the run's stream when timing is
+   enabled); names_out receives a comma-separated kernel list. */
+int rgl_gpu_set_timing(rgl_gpu* gpu, int enable);
+int rgl_gpu_kernel_times(rgl_gpu* gpu, double* ms_out, int max_kernels, char* names_out, int names_len);
+
+const char* rgl_gpu_last_error(rgl_gpu* gpu);
+
+/* --- rglv::GL (rglv_gl.hxx:186-331), recording into GPU::IC() ----------- */
+int rgl_gl_enable(rgl_gpu* gpu, int cap);                        /* GL::Enable          :186-197 */
+int rgl_gl_disable(rgl_gpu* gpu, int cap);                       /* GL::Disable         :199-210 */
+int rgl_gl_depth_func(rgl_gpu* gpu, int func);                   /* GL::DepthFunc       :212-214 */
+int rgl_gl_depth_write_mask(rgl_gpu* gpu, int enable);           /* GL::DepthWriteMask  :216-218 */
+int rgl_gl_color_write_mask(rgl_gpu* gpu, int enable);           /* GL::ColorWriteMask  :220-222 */
+int rgl_gl_cull_face(rgl_gpu* gpu, int face);                    /* GL::CullFace        :224-226 */
+int rgl_gl_scissor(rgl_gpu* gpu, int x, int y, int w, int h);    /* GL::Scissor         :228-231 */
+int rgl_gl_viewport(rgl_gpu* gpu, int x, int y, int w, int h);   /* GL::Viewport        :233-236 */
+int rgl_gl_use_program(rgl_gpu* gpu, int program_id);           /* GL::UseProgram      :238-240 */
+int rgl_gl_use_uniforms(rgl_gpu* gpu, int ofs);                  /* GL::UseUniforms     :242-244 */
+/* GL::AllocUniformBuffer :253-258 — returns the offset; the 32 floats are
+   copied from values (n <= 32), the caller may also rewrite them before Run
+   through rgl_gl_uniform_buffer_addr. */
+int rgl_gl_alloc_uniform_buffer(rgl_gpu* gpu, const float* values, int n);
+float* rgl_gl_uniform_buffer_addr(rgl_gpu* gpu, int ofs);        /* GL::GetUniformBufferAddr :260-263 */
+int rgl_gl_renderbuffer_type(rgl_gpu* gpu, int attachment, int type); /* GL::RenderbufferType :246-251 */
+/* GL::BindTexture :265-273; dev_texels = float RGBA, mip chain stacked
+   below the base level (N x 2N texels for power-of-two N). */
+int rgl_gl_bind_texture(rgl_gpu* gpu, int unit, const float* dev_texels, int width, int height, int stride, int mode);
+/* GL::UseBuffer :280-288; count = number of floats in the array (slot 0's
+   count is the vertex count of the draws that use it). */
+int rgl_gl_use_buffer(rgl_gpu* gpu, int slot, const float* dev_ptr, int count);
+int rgl_gl_clear_color(rgl_gpu* gpu, float r, float g, float b);  /* GL::ClearColor  :290-292 */
+int rgl_gl_clear_depth(rgl_gpu* gpu, float value);                /* GL::ClearDepth  :294-296 */
+int rgl_gl_view_matrix(rgl_gpu* gpu, const float* m16);           /* GL::ViewMatrix       :298-300, column-major */
+int rgl_gl_projection_matrix(rgl_gpu* gpu, const float* m16);     /* GL::ProjectionMatrix :302-304 */
+int rgl_gl_normal_matrix(rgl_gpu* gpu, const float* m16);         /* GL::NormalMatrix     :306-308 */
+/* GL::DrawElements rglv_gl.cxx:34-43; dev_indices = uint16 triangle list */
+int rgl_gl_draw_elements(rgl_gpu* gpu, int mode, int count, int type, const uint16_t* dev_indices, int hint);
+int rgl_gl_draw_arrays(rgl_gpu* gpu, int mode, int start, int count);                      /* rglv_gl.cxx:13-20 */
+int rgl_gl_draw_elements_instanced(rgl_gpu* gpu, int mode, int count, int type,
+                                   const uint16_t* dev_indices, int instance_count);       /* rglv_gl.cxx:46-56 */
+int rgl_gl_draw_arrays_instanced(rgl_gpu* gpu, int mode, int start, int count, int instance_count); /* :23-31 */
+int rgl_gl_clear(rgl_gpu* gpu, int bits);                                                  /* rglv_gl.cxx:58-61 */
+/* GL::StoreColor(TrueColorCanvas*, bool) rglv_gl.cxx:79-83: dev_pixels =
+   width*height uint32 0x00RRGGBB, row stride = width. */
+int rgl_gl_store_color_tc(rgl_gpu* gpu, uint32_t* dev_pixels, int enable_gamma);
+/* GL::StoreDepth rglv_gl.cxx:86-89: width*height float, row stride = width. */
+int rgl_gl_store_depth(rgl_gpu* gpu, float* dev_depth);
+/* No counterpart in the reference: stores, per pixel, the id of the
+   triangle that last passed the depth test (instance * triangles per
+   instance + triangle index, counted across the frame's draws in
+   submission order), 0xffffffff where nothing was drawn. */
+int rgl_gl_store_primitive_id(rgl_gpu* gpu, uint32_t* dev_ids);
+int rgl_gl_reset(rgl_gpu* gpu);                                                            /* GL::Reset rglv_gl.cxx:92-97 */
+
+/* --- texture preparation ------------------------------------------------ */
+/* Texture::maybe_make_mipmap (rglr_texture.cxx:33-81) on host memory:
+   texels = dim*dim*2 RGBA floats with the base level in the first dim rows;
+   fills the box-filtered mip chain below it. */
+int rgl_texture_make_mipmap(float* host_texels, int dim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RGL_HIP_H */

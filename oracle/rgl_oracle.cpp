@@ -1,0 +1,1115 @@
+/*
+ * rgl_oracle — CPU restatement of the rgl tile rasteriser (usrlocalben/rsr,
+ * src/rgl/rglv + src/rgl/rglr + src/rml), used ONLY as test infrastructure:
+ * the parity checker for the HIP pipeline and the timed CPU baseline.
+ * Nothing in the product links, loads or calls this file.
+ *
+ * It follows the reference's algorithm step by step, lane by lane, with
+ * the SSE arithmetic restated in scalar C++:
+ *   - cvttps2dq truncation with 0x80000000 for out-of-range/NaN;
+ *   - rcpps from a captured 2048-entry table (rcpps_table.inc), then the
+ *     one Newton step of rmlv::oneover (rmlv_mvec4.hxx:630-650);
+ *   - 32-bit wrapping integer products (sse2_mul32, rmlv_mmutil.hxx:22-34);
+ *   - maxps/minps operand order (NaN returns the second operand);
+ *   - every float expression in the reference's evaluation order, compiled
+ *     with -ffp-contract=off so no FMA is formed.
+ *
+ * Differences that do not change results: per-tile command lists are
+ * arrays of small structs instead of packed bytes, and every triangle entry
+ * also carries its primitive id (instance * triangles_per_instance + index
+ * within the draw + the draw's base), so a primitive-ID buffer can be
+ * written next to colour and depth.  The reference has no such buffer.
+ */
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+#include "rcpps_table.inc"
+#include "srgb8_table.inc"
+
+// ------------------------------------------------------------------------
+// SSE-faithful scalar arithmetic (rmlv_mvec4.hxx)
+// ------------------------------------------------------------------------
+inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+// _mm_cvttps_epi32 (rmlv_mvec4.hxx:552)
+inline int32_t cvtt(float f) {
+	if (f >= -2147483648.0F && f < 2147483648.0F) return static_cast<int32_t>(f);
+	return INT32_MIN; }
+
+// _mm_cvtepi32_ps
+inline float itof(int32_t i) { return static_cast<float>(i); }
+
+// _mm_rcp_ps, Intel table model (see gen_rcpps_table.c)
+inline float rcpps(float x) {
+	const uint32_t in = f2u(x);
+	const uint32_t e = (in >> 23) & 0xffu, s = in & 0x80000000u, m = in & 0x7fffffu;
+	if (e == 0xffu) return u2f(m ? (in | 0x400000u) : s);
+	if (e == 0u) return u2f(s | 0x7f800000u);
+	const int re = 126 + 127 - static_cast<int>(e);
+	if (re <= 0) return u2f(s);
+	return u2f(s | (static_cast<uint32_t>(re) << 23) | kRcpMantissa[m >> 12]); }
+
+// rmlv::oneover: rcp + one Newton-Raphson step (rmlv_mvec4.hxx:645-649)
+inline float oneover(float a) {
+	const float r = rcpps(a);
+	const float muls = a * (r * r);
+	return (r + r) - muls; }
+
+inline float sse_max(float a, float b) { return a > b ? a : b; }  // maxps
+inline float sse_min(float a, float b) { return a < b ? a : b; }  // minps
+
+inline int32_t wadd(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b)); }
+inline int32_t wsub(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) - static_cast<uint32_t>(b)); }
+inline int32_t wmul(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b)); }
+inline int32_t sar(int32_t a, int n) { return a >> n; }  // arithmetic on this target
+inline int32_t shl(int32_t a, int n) { return static_cast<int32_t>(static_cast<uint32_t>(a) << n); }
+
+// ryg float_to_srgb8_var2_SSE2 (3rdparty/ryg-srgb/ryg-srgb.h:183-219)
+inline uint32_t to_srgb8(float f) {
+	const float clamp1 = sse_max(f, u2f((127u - 13u) << 23));
+	const float clamp2 = sse_min(clamp1, u2f(0x3f7fffffu));
+	const uint32_t bits = f2u(clamp2);
+	const uint32_t tab = kSrgb8Tab[(bits >> 20) - (127u - 13u) * 8u];
+	const uint32_t t = (bits >> 12) & 0xffu;
+	const uint32_t prod = (tab & 0xffffu) * t + (tab >> 16) * 512u;  // pmaddwd of positive halves
+	return prod >> 16; }
+
+// to_tc_basic (rglr_canvas_util.hxx:40-48)
+inline uint32_t to_linear8(float a) {
+	float r0 = sse_min(a, 1.0F);
+	r0 = sse_max(r0, 0.0F);
+	r0 = r0 * 255.0F;
+	return static_cast<uint32_t>(cvtt(r0)); }
+
+// ------------------------------------------------------------------------
+// mat4, column-major storage (rmlm_mat4.hxx:157-207)
+// ------------------------------------------------------------------------
+struct mat4 { float ff[16]; };
+
+inline mat4 mul(const mat4& l, const mat4& r) {
+	mat4 out;
+	for (int row = 0; row < 4; ++row) {
+		for (int col = 0; col < 4; ++col) {
+			float ax = l.ff[0*4 + row] * r.ff[col*4 + 0];
+			ax += l.ff[1*4 + row] * r.ff[col*4 + 1];
+			ax += l.ff[2*4 + row] * r.ff[col*4 + 2];
+			ax += l.ff[3*4 + row] * r.ff[col*4 + 3];
+			out.ff[col*4 + row] = ax; }}
+	return out; }
+
+// qmat4 * qfloat4 (rmlm_soa.hxx:58-64)
+inline void xform4(const float* f, const float b[4], float o[4]) {
+	o[0] = ((f[0]*b[0] + f[4]*b[1]) + f[ 8]*b[2]) + f[12]*b[3];
+	o[1] = ((f[1]*b[0] + f[5]*b[1]) + f[ 9]*b[2]) + f[13]*b[3];
+	o[2] = ((f[2]*b[0] + f[6]*b[1]) + f[10]*b[2]) + f[14]*b[3];
+	o[3] = ((f[3]*b[0] + f[7]*b[1]) + f[11]*b[2]) + f[15]*b[3]; }
+
+// mul_w0 (rmlm_soa.hxx:42-47)
+inline void xform3w0(const float* f, const float b[3], float o[3]) {
+	o[0] = (f[0]*b[0] + f[4]*b[1]) + f[ 8]*b[2];
+	o[1] = (f[1]*b[0] + f[5]*b[1]) + f[ 9]*b[2];
+	o[2] = (f[2]*b[0] + f[6]*b[1]) + f[10]*b[2]; }
+
+// ------------------------------------------------------------------------
+// protocol and state (rglv_gpu_protocol.hxx, rglv_gl.hxx)
+// ------------------------------------------------------------------------
+constexpr int GL_CULL_FACE = 1, GL_SCISSOR_TEST = 2, GL_BLEND = 3, GL_DEPTH_TEST = 4;
+constexpr int GL_FRONT = 1, GL_BACK = 2;
+constexpr int GL_LESS = 0, GL_LEQUAL = 1, GL_EQUAL = 2;
+constexpr int GL_COLOR_BUFFER_BIT = 1, GL_DEPTH_BUFFER_BIT = 2;
+constexpr int GL_DEPTH_ATTACHMENT = 0, GL_COLOR_ATTACHMENT0 = 2;
+constexpr int RB_COLOR_DEPTH = 0;
+constexpr int UNIFORM_BUFFER_SIZE = 32;
+
+struct TextureState { const float* ptr; int width, height, stride, filter; };
+
+struct GLState {
+	float clearColor[4];
+	float clearDepth;
+	bool cullingEnabled;
+	int cullFace;
+	bool scissorEnabled;
+	int scissorOrigin[2], scissorSize[2];
+	int viewportOrigin[2];
+	bool hasViewportSize;
+	int viewportSize[2];
+	bool blendingEnabled, colorWriteMask, depthWriteMask, depthTestEnabled;
+	int depthFunc;
+	int programId;
+	int uniformsOfs;
+	int color0AttachmentType, depthAttachmentType;
+	const float* buffers[16];
+	TextureState tus[2];
+	mat4 viewMatrix, normalMatrix, projectionMatrix;
+
+	void reset() {  // GLState::reset (rglv_gl.hxx:152-179)
+		std::memset(this, 0, sizeof(*this));
+		clearColor[3] = 1.0F;
+		clearDepth = 1.0F;
+		cullFace = GL_BACK;
+		colorWriteMask = true;
+		depthWriteMask = true;
+		depthTestEnabled = true;
+		depthFunc = GL_LESS;
+		uniformsOfs = -1;
+		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }}};
+
+struct Cmd {
+	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
+	            STORE_TC, STORE_DEPTH, STORE_PRIM } kind;
+	int a, b, c;
+	const void* ptr; };
+
+// ------------------------------------------------------------------------
+// texture units (rglr_texture_sampler.cxx)
+// ------------------------------------------------------------------------
+inline int ilog2(unsigned x) { int p = 0; while (x != 0U) { x >>= 1; ++p; } return p - 1; }
+
+// LevelOfDetail (rglr_texture_sampler.cxx:25-42), lanes 0 and 1 only
+inline int level_of_detail(const float u[4], const float v[4]) {
+	const float dux = u[1] - u[0];
+	const float dvx = v[1] - v[0];
+	const float sqd = dux*dux + dvx*dvx;
+	return (static_cast<int32_t>(f2u(sqd)) - (127 << 23)) >> 24; }
+
+struct Sampler {
+	int kind = -1;  // 0 nearest-mip, 1 linear-mip, 2 npot nearest
+	int power = 0, width = 0, height = 0, stride = 0;
+	const float* buf = nullptr;
+
+	static Sampler Make(const TextureState& t, std::string& err) {
+		Sampler s;
+		s.buf = t.ptr; s.width = t.width; s.height = t.height; s.stride = t.stride;
+		const int power = ilog2(static_cast<unsigned>(t.width));
+		const bool isPow2 = (1 << power) == t.width && t.width == t.height && t.stride == t.width;
+		if (!isPow2) { s.kind = 2; return s; }
+		if (t.width < 4 || t.width > 1024) { err = "can't make TextureUnit for pow2 size"; return s; }
+		s.power = power;
+		s.kind = t.filter ? 1 : 0;
+		return s; }
+
+	// one quad, 4 lanes -> rgba[4][4] (channel, lane)
+	void sample(const float u[4], const float v[4], float out[4][4]) const {
+		if (buf == nullptr) { for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c][l] = 0.0F; return; }
+		const float almostone = u2f(0x3f7fffffu);
+		if (kind == 2) {  // TextureUnitRGBAF32_NM_ONEMAP_WRAP_NEAREST (:289-311)
+			for (int l = 0; l < 4; ++l) {
+				const float fu = u[l] + 100.0F, fv = v[l] + 100.0F;
+				const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+				const int32_t pu = cvtt(fru * static_cast<float>(width));
+				const int32_t pv = cvtt((almostone - frv) * static_cast<float>(height));
+				const int32_t ofs = shl(wadd(wmul(pv, stride), pu), 2);
+				for (int c = 0; c < 4; ++c) out[c][l] = buf[ofs + c]; }
+			return; }
+		const float baseDim = static_cast<float>(1 << power);
+		float bu[4], bv[4];
+		for (int l = 0; l < 4; ++l) { bu[l] = u[l] * baseDim; bv[l] = v[l] * baseDim; }
+		const int lod = std::clamp(level_of_detail(bu, bv), 0, power);
+		if (kind == 0) {  // ..._P2_MIPMAP_WRAP_NEAREST (:63-109)
+			const float levelDim = static_cast<float>(1 << (power - lod));
+			const int32_t levelBeginRow = static_cast<int32_t>((0xfffffffeu << (power - lod)) & ((1u << (power + 1)) - 1u));
+			for (int l = 0; l < 4; ++l) {
+				const float fu = u[l] + 10.0F, fv = v[l] + 10.0F;
+				const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+				const int32_t lx = cvtt(fru * levelDim);
+				const int32_t ly = cvtt((almostone - frv) * levelDim);
+				const int32_t by = wadd(levelBeginRow, ly);
+				const int32_t ofs = shl(wadd(shl(by, power), lx), 2);
+				for (int c = 0; c < 4; ++c) out[c][l] = buf[ofs + c]; }
+			return; }
+		// ..._P2_MIPMAP_WRAP_LINEAR (:172-286)
+		const int levelDim = 1 << (power - lod);
+		const int32_t wrapMask = levelDim - 1;
+		const int32_t levelLastRow = static_cast<int32_t>((0xffffffffu << (power - lod)) & ((1u << (power + 1)) - 1u)) - 1;
+		for (int l = 0; l < 4; ++l) {
+			const float lx = u[l] * static_cast<float>(levelDim);
+			const float ly = v[l] * static_cast<float>(levelDim);
+			int32_t tx0 = cvtt(lx - 0.5F), ty0 = cvtt(ly - 0.5F);
+			int32_t tx1 = wadd(tx0, 1), ty1 = wadd(ty0, 1);
+			const float fx = (lx - itof(tx0)) - 0.5F;
+			const float fy = (ly - itof(ty0)) - 0.5F;
+			const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
+			const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
+			tx0 &= wrapMask; ty0 &= wrapMask; tx1 &= wrapMask; ty1 &= wrapMask;
+			const int32_t by0 = wsub(levelLastRow, ty0), by1 = wsub(levelLastRow, ty1);
+			const int32_t o00 = shl(wadd(shl(by0, power), tx0), 2);
+			const int32_t o10 = shl(wadd(shl(by0, power), tx1), 2);
+			const int32_t o01 = shl(wadd(shl(by1, power), tx0), 2);
+			const int32_t o11 = shl(wadd(shl(by1, power), tx1), 2);
+			for (int c = 0; c < 4; ++c) {
+				float ax = buf[o00 + c] * w00;
+				ax = ax + buf[o10 + c] * w10;
+				ax = ax + buf[o01 + c] * w01;
+				ax = ax + buf[o11 + c] * w11;
+				out[c][l] = ax; }}}};
+
+// ------------------------------------------------------------------------
+// shader programs (rglv_gpu_shaders.hxx concept)
+// ------------------------------------------------------------------------
+constexpr int kMaxVary = 16;
+struct VOut { float f[kMaxVary]; };
+
+struct DrawEnv {
+	const GLState* st;
+	const float* uniforms;  // UniformsSD, may be null
+	mat4 vpm;
+	Sampler tu0, tu1; };
+
+inline void normalize3(float& x, float& y, float& z) {
+	const float len2 = (x*x + y*y) + z*z;
+	const float inv = 1.0F / std::sqrt(len2);
+	x = x * inv; y = y * inv; z = z * inv; }
+
+inline float interp(const float bp[3], float a, float b, float c) {
+	return (bp[0]*a + bp[1]*b) + bp[2]*c; }
+
+struct Program {
+	int id = 0;
+	bool earlyZ = true;
+	int nvary = 0;
+	bool instanced = false;
+	virtual ~Program() = default;
+	virtual void ShadeVertex(const DrawEnv& e, int iid, int idx, float pos[4], VOut& o) const = 0;
+	// one quad; vary[k][lane], colour[c][lane]; mask lanes may be cleared
+	virtual void ShadeFragment(const DrawEnv& e, const float bs[3][4], const float bp[3][4], const float vary[kMaxVary][4],
+	                           const float fragCoord[2][4], const float fragDepth[4], float color[4][4], int32_t mask[4]) const = 0;
+	void Mix(const VOut& a, const VOut& b, float t, VOut& o) const {  // mix() (rmlv_math.hxx:87-90)
+		for (int k = 0; k < nvary; ++k) o.f[k] = (1.0F - t) * a.f[k] + t * b.f[k]; }};
+
+inline float buf_at(const float* p, int idx, float dflt) { return p ? p[idx] : dflt; }
+
+inline void load_pos(const DrawEnv& e, int idx, float p[4]) {
+	const auto& b = e.st->buffers;
+	p[0] = buf_at(b[0], idx, 0.0F); p[1] = buf_at(b[1], idx, 0.0F); p[2] = buf_at(b[2], idx, 0.0F); p[3] = 1.0F; }
+
+// AmyProgram (viewer/shaders.hxx:69-161): uv varying, texture colour
+struct AmyProgram final : Program {
+	AmyProgram() { id = 4; nvary = 2; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		xform4(e.vpm.ff, p, pos);
+		o.f[0] = buf_at(e.st->buffers[9], idx, 0.0F);
+		o.f[1] = buf_at(e.st->buffers[10], idx, 0.0F); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		e.tu0.sample(vary[0], vary[1], color); }};
+
+// ManyProgram (viewer/shaders.hxx:482-583): instanced, colour = (u, v, magic, 1)
+struct ManyProgram final : Program {
+	ManyProgram() { id = 6; nvary = 2; instanced = true; }
+	void ShadeVertex(const DrawEnv& e, int iid, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		const float* imat = e.st->buffers[15] + 16 * iid;
+		float p1[4]; xform4(imat, p, p1);
+		xform4(e.vpm.ff, p1, pos);
+		o.f[0] = buf_at(e.st->buffers[9], idx, 0.0F);
+		o.f[1] = buf_at(e.st->buffers[10], idx, 0.0F); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float magic = e.uniforms ? e.uniforms[0] : 0.0F;
+		for (int l = 0; l < 4; ++l) { color[0][l] = vary[0][l]; color[1][l] = vary[1][l]; color[2][l] = magic; color[3][l] = 1.0F; }}};
+
+// LambertTexProgram (new): textured, one directional light.
+// uniforms: [0..2] unit direction toward the light, [4] ambient, [5] diffuse
+struct LambertTexProgram final : Program {
+	LambertTexProgram() { id = 80; nvary = 5; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		xform4(e.vpm.ff, p, pos);
+		const auto& b = e.st->buffers;
+		o.f[0] = buf_at(b[3], idx, 0.0F); o.f[1] = buf_at(b[4], idx, 0.0F); o.f[2] = buf_at(b[5], idx, 1.0F);
+		o.f[3] = buf_at(b[9], idx, 0.0F); o.f[4] = buf_at(b[10], idx, 0.0F); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float* u = e.uniforms;
+		float tex[4][4];
+		e.tu0.sample(vary[3], vary[4], tex);
+		for (int l = 0; l < 4; ++l) {
+			float nx = vary[0][l], ny = vary[1][l], nz = vary[2][l];
+			normalize3(nx, ny, nz);
+			float ndl = (nx*u[0] + ny*u[1]) + nz*u[2];
+			ndl = sse_max(ndl, 0.0F);
+			const float k = u[4] + u[5] * ndl;
+			color[0][l] = tex[0][l] * k; color[1][l] = tex[1][l] * k; color[2][l] = tex[2][l] * k;
+			color[3][l] = 1.0F; }}};
+
+// ManyBlinnPhongProgram (new): instanced, textured, one point light, Blinn-Phong.
+// uniforms: [0..2] eye, [4..6] light position, [8] ka, [9] kd, [10] ks
+struct ManyBlinnPhongProgram final : Program {
+	ManyBlinnPhongProgram() { id = 81; nvary = 8; instanced = true; }
+	void ShadeVertex(const DrawEnv& e, int iid, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		const auto& b = e.st->buffers;
+		const float* imat = b[15] + 16 * iid;
+		float p1[4]; xform4(imat, p, p1);
+		xform4(e.vpm.ff, p1, pos);
+		const float n[3] = { buf_at(b[3], idx, 0.0F), buf_at(b[4], idx, 0.0F), buf_at(b[5], idx, 1.0F) };
+		float wn[3]; xform3w0(imat, n, wn);
+		o.f[0] = p1[0]; o.f[1] = p1[1]; o.f[2] = p1[2];
+		o.f[3] = wn[0]; o.f[4] = wn[1]; o.f[5] = wn[2];
+		o.f[6] = buf_at(b[9], idx, 0.0F); o.f[7] = buf_at(b[10], idx, 0.0F); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float* u = e.uniforms;
+		float tex[4][4];
+		e.tu0.sample(vary[6], vary[7], tex);
+		for (int l = 0; l < 4; ++l) {
+			const float px = vary[0][l], py = vary[1][l], pz = vary[2][l];
+			float nx = vary[3][l], ny = vary[4][l], nz = vary[5][l];
+			normalize3(nx, ny, nz);
+			float vx = u[0] - px, vy = u[1] - py, vz = u[2] - pz;
+			normalize3(vx, vy, vz);
+			float lx = u[4] - px, ly = u[5] - py, lz = u[6] - pz;
+			normalize3(lx, ly, lz);
+			float hx = lx + vx, hy = ly + vy, hz = lz + vz;
+			normalize3(hx, hy, hz);
+			const float ndl = sse_max((nx*lx + ny*ly) + nz*lz, 0.0F);
+			const float ndh = sse_max((nx*hx + ny*hy) + nz*hz, 0.0F);
+			float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;  // ndh^32
+			const float kdiff = u[8] + u[9] * ndl;
+			const float kspec = u[10] * s;
+			color[0][l] = tex[0][l] * kdiff + kspec;
+			color[1][l] = tex[1][l] * kdiff + kspec;
+			color[2][l] = tex[2][l] * kdiff + kspec;
+			color[3][l] = 1.0F; }}};
+
+const Program* find_program(int id) {
+	static const AmyProgram amy; static const ManyProgram many;
+	static const LambertTexProgram lam; static const ManyBlinnPhongProgram mbp;
+	switch (id) {
+	case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp;
+	default: return nullptr; }}
+
+// post programs for StoreTrueColor (ShadeCanvas); 1 = Default, 2 = Exposure
+bool known_post_program(int id) { return id == 1 || id == 2; }
+
+// ------------------------------------------------------------------------
+// per-tile command lists
+// ------------------------------------------------------------------------
+struct TileOp {
+	enum Op : uint8_t { STATE, CLEAR, TRI, CLIPPED_TRI, STORE_TC, STORE_DEPTH, STORE_PRIM } op;
+	int32_t a;      // state index | clear bits | iid | gamma
+	uint16_t i0, i1, i2;
+	uint32_t prim;
+	const void* ptr; };
+
+struct ClippedVertex { float coord[4]; VOut data; };
+
+struct Rect { int x0, y0, x1, y1; };
+
+struct Frustum {  // ViewFrustum (rglv_view_frustum.hxx:257-315)
+	float factor;
+	explicit Frustum(int w, int h) {
+		const int longSide = std::max(w, h);
+		const int half = longSide / 2;
+		factor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half); }
+	uint32_t Test(const float p[4]) const {  // SIMD version, cmple
+		const float w = p[3] * factor;
+		uint32_t f = 0;
+		if (w + p[0] <= 0.0F) f |= 1u;
+		if (w + p[1] <= 0.0F) f |= 2u;
+		if (p[3] + p[2] <= 0.0F) f |= 4u;
+		if (w - p[0] <= 0.0F) f |= 8u;
+		if (w - p[1] <= 0.0F) f |= 16u;
+		return f; }};
+
+// clipping planes in reference order: Left, Bottom, Near, Right, Top (:245-251)
+inline float plane_dist(int plane, const float p[4]) {
+	switch (plane) {
+	case 0: return p[3] + p[0];
+	case 1: return p[3] + p[1];
+	case 2: return p[3] + p[2];
+	case 3: return p[3] - p[0];
+	default: return p[3] - p[1]; }}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// the oracle GPU
+// ------------------------------------------------------------------------
+struct rglo_gpu {
+	// recorder (rglv::GL)
+	GLState cs{};
+	bool dirty = true;
+	std::vector<float> ubuf;
+	std::deque<GLState> states;
+	std::vector<Cmd> commands;
+
+	// render target
+	int W = 0, H = 0, tileW = 64, tileH = 64, tilesX = 0, tilesY = 0;
+	std::vector<std::vector<TileOp>> tiles;
+	std::vector<ClippedVertex> clipped;
+	std::vector<uint32_t> installed;
+	std::string err;
+	int threads = 1;
+	// statistics
+	long long statSubmitted = 0, statClipped = 0, statBinned = 0, statFragments = 0, statClippedPairs = 0;
+
+	rglo_gpu() { cs.reset(); }
+
+	void MaybeUpdateState() {  // GL::MaybeUpdateState (rglv_gl.cxx:100-106)
+		if (!dirty) return;
+		states.push_back(cs);
+		commands.push_back({ Cmd::STATE, static_cast<int>(states.size()) - 1, 0, 0, nullptr });
+		dirty = false; }
+
+	const float* Uniforms(const GLState& s) const {
+		return s.uniformsOfs < 0 ? nullptr : ubuf.data() + s.uniformsOfs; }
+
+	Rect TileRect(int t) const {  // GPU::TileRect (rglv_gpu.hxx:291-299)
+		const int ty = t / tilesX, tx = t % tilesX;
+		return { tx * tileW, ty * tileH, std::min((tx + 1) * tileW, W), std::min((ty + 1) * tileH, H) }; }
+
+	Rect ScissorRect(const GLState& s) const {  // (rglv_gpu.hxx:272-276)
+		if (!s.scissorEnabled) return { 0, 0, W, H };
+		const int left = s.scissorOrigin[0], right = left + s.scissorSize[0];
+		const int bottom = H - s.scissorOrigin[1] - 1, top = bottom - s.scissorSize[1];
+		return { left, top, right, bottom }; }
+
+	void DSDO(const GLState& s, float DS[2], float DO[2]) const {  // (rglv_gpu.hxx:265-270)
+		const int vx = s.hasViewportSize ? s.viewportSize[0] : W;
+		const int vy = s.hasViewportSize ? s.viewportSize[1] : H;
+		DS[0] = static_cast<float>(vx / 2); DS[1] = static_cast<float>(-vy / 2);
+		DO[0] = static_cast<float>(vx / 2 + s.viewportOrigin[0]);
+		DO[1] = static_cast<float>(H - (vy / 2 + s.viewportOrigin[1])); }
+
+	bool KeyInstalled(uint32_t key) const {
+		return std::find(installed.begin(), installed.end(), key) != installed.end(); }
+
+	static uint32_t FragmentStateKey(const GLState& s) {  // (rglv_gl.hxx:136-146)
+		uint32_t key = static_cast<uint32_t>(s.programId) << 24;
+		key |= static_cast<uint32_t>(s.scissorEnabled);
+		key |= static_cast<uint32_t>(s.depthTestEnabled) << 1;
+		if (s.depthTestEnabled) key |= static_cast<uint32_t>(s.depthFunc) << 2;
+		key |= static_cast<uint32_t>(s.blendingEnabled) << 4;
+		key |= static_cast<uint32_t>(s.depthWriteMask) << 5;
+		key |= static_cast<uint32_t>(s.colorWriteMask) << 6;
+		key |= static_cast<uint32_t>(s.color0AttachmentType) << 7;
+		key |= static_cast<uint32_t>(s.depthAttachmentType) << 9;
+		return key; }
+
+	bool MakeEnv(const GLState& s, DrawEnv& e) {
+		e.st = &s;
+		e.uniforms = Uniforms(s);
+		e.vpm = mul(s.projectionMatrix, s.viewMatrix);  // MakeMatrices (rglv_gpu_impl.hxx:44-51)
+		std::string terr;
+		e.tu0 = Sampler::Make(s.tus[0], terr);
+		e.tu1 = Sampler::Make(s.tus[1], terr);
+		if (!terr.empty()) { err = terr; return false; }
+		return true; }
+
+	// ---- binning (GPU::BinImpl, rglv_gpu.cxx:119-260) ----
+	bool Bin() {
+		tiles.assign(static_cast<size_t>(tilesX) * tilesY, {});
+		clipped.clear();
+		const GLState* binState = nullptr;
+		int binStateIdx = -1;
+		uint32_t primBase = 0;
+		for (const Cmd& c : commands) {
+			switch (c.kind) {
+			case Cmd::STATE:
+				binStateIdx = c.a; binState = &states[c.a];
+				for (auto& t : tiles) t.push_back({ TileOp::STATE, c.a, 0, 0, 0, 0, nullptr });
+				break;
+			case Cmd::CLEAR:
+				for (auto& t : tiles) t.push_back({ TileOp::CLEAR, c.a, 0, 0, 0, 0, nullptr });
+				break;
+			case Cmd::STORE_TC:
+				if (binState == nullptr || !KeyInstalled((static_cast<uint32_t>(binState->programId) << 24) | 0x40000000u)) {
+					err = "STORE_COLOR: no blt dispatch entry"; return false; }
+				for (auto& t : tiles) t.push_back({ TileOp::STORE_TC, c.a, 0, 0, 0, 0, c.ptr });
+				break;
+			case Cmd::STORE_DEPTH:
+				for (auto& t : tiles) t.push_back({ TileOp::STORE_DEPTH, 0, 0, 0, 0, 0, c.ptr });
+				break;
+			case Cmd::STORE_PRIM:
+				for (auto& t : tiles) t.push_back({ TileOp::STORE_PRIM, 0, 0, 0, 0, 0, c.ptr });
+				break;
+			default: {
+				if (binState == nullptr) { err = "draw before state"; return false; }
+				const uint32_t vkey = static_cast<uint32_t>(binState->programId) << 24;
+				if (!KeyInstalled(vkey | 0x80000000u)) { err = "BIN: no dispatch entry"; return false; }
+				if (!KeyInstalled(FragmentStateKey(*binState))) { err = "DRAW: no dispatch entry"; return false; }
+				const bool elements = c.kind == Cmd::DRAW_ELEMENTS || c.kind == Cmd::DRAW_ELEMENTS_INSTANCED;
+				const bool inst = c.kind == Cmd::DRAW_ELEMENTS_INSTANCED || c.kind == Cmd::DRAW_ARRAYS_INSTANCED;
+				const int count = c.a;
+				const int instanceCnt = inst ? c.b : 1;
+				const auto* indices = static_cast<const uint16_t*>(c.ptr);
+				if (!BinTriangles(*binState, binStateIdx, count, elements ? indices : nullptr, instanceCnt, inst, primBase)) return false;
+				primBase += static_cast<uint32_t>(count / 3) * static_cast<uint32_t>(instanceCnt);
+				break; }}}
+		return true; }
+
+	// BinTriangles1P (rglv_gpu_impl.hxx:517-675); 2P gives the same lists
+	bool BinTriangles(const GLState& state, int stateIdx, int count, const uint16_t* indices, int instanceCnt,
+	                  bool instanced, uint32_t primBase) {
+		const Program* pgm = find_program(state.programId);
+		if (!pgm) { err = "unknown program"; return false; }
+		DrawEnv env;
+		if (!MakeEnv(state, env)) return false;
+		const Frustum frustum(W, H);
+		const Rect sr = ScissorRect(state);
+		float DS[2], DO[2]; DSDO(state, DS, DO);
+		const int tlx = sr.x0, tly = sr.y0, brx = sr.x1 - 1, bry = sr.y1 - 1;
+		const bool keepBacks = !(state.cullingEnabled && state.cullFace == GL_BACK);
+		const bool keepFronts = !(state.cullingEnabled && state.cullFace == GL_FRONT);
+		const int numPrims = count / 3;
+		std::vector<int> touched(tiles.size(), 0);
+		for (auto& t : tiles) t.push_back({ TileOp::TRI, -1, 0, 0, 0, 0, nullptr });  // draw marker
+		struct ClipItem { int iid; uint16_t i[3]; uint32_t prim; };
+		std::vector<ClipItem> clipQueue;
+		for (int iid = 0; iid < instanceCnt; ++iid) {
+			for (int pi = 0; pi < numPrims; ++pi) {
+				++statSubmitted;
+				uint16_t ii[3];
+				for (int k = 0; k < 3; ++k) ii[k] = indices ? indices[pi*3 + k] : static_cast<uint16_t>(pi*3 + k);
+				uint32_t cf[3]; float dcx[3], dcy[3];
+				for (int k = 0; k < 3; ++k) {
+					float pos[4]; VOut vo;
+					pgm->ShadeVertex(env, iid, ii[k], pos, vo);
+					cf[k] = frustum.Test(pos);
+					const float iw = oneover(pos[3]);  // pdiv (rglv_math.hxx:19-23)
+					dcx[k] = (pos[0] * iw) * DS[0] + DO[0];
+					dcy[k] = (pos[1] * iw) * DS[1] + DO[1]; }
+				const bool pointsOutside = (cf[0] | cf[1] | cf[2]) > 0;
+				const bool primsOutside = (cf[0] & cf[1] & cf[2]) > 0;
+				const uint32_t prim = primBase + static_cast<uint32_t>(iid) * numPrims + pi;
+				if (primsOutside) continue;
+				if (pointsOutside) clipQueue.push_back({ iid, { ii[0], ii[1], ii[2] }, prim });
+				// rmlg::Area (rmlg_triangle.hxx:17-21)
+				const float d31x = dcx[2] - dcx[0], d31y = dcy[2] - dcy[0];
+				const float d21x = dcx[1] - dcx[0], d21y = dcy[1] - dcy[0];
+				const float area = d31x*d21y - d31y*d21x;
+				const bool front = area > 0.0F;
+				const int32_t ix0 = cvtt(dcx[0]), iy0 = cvtt(dcy[0]);
+				const int32_t ix1 = cvtt(dcx[1]), iy1 = cvtt(dcy[1]);
+				const int32_t ix2 = cvtt(dcx[2]), iy2 = cvtt(dcy[2]);
+				const int32_t vminx = std::max(std::min(ix0, std::min(ix1, ix2)), tlx);
+				const int32_t vminy = std::max(std::min(iy0, std::min(iy1, iy2)), tly);
+				const int32_t vmaxx = std::min(wadd(std::max(ix0, std::max(ix1, ix2)), 1), brx);
+				const int32_t vmaxy = std::min(wadd(std::max(iy0, std::max(iy1, iy2)), 1), bry);
+				bool notCulled = (!front && keepBacks) || (front && keepFronts);
+				notCulled = notCulled && !pointsOutside;
+				const bool accept = notCulled && vmaxx > vminx && vmaxy > vminy;
+				if (!accept) continue;
+				uint16_t n0 = front ? ii[0] : ii[2];
+				const uint16_t n1 = ii[1];
+				const uint16_t n2 = front ? ii[2] : ii[0];
+				if (!front) n0 |= 0x8000;
+				const int tx0 = vminx / tileW, ty0 = vminy / tileH, tx1 = vmaxx / tileW, ty1 = vmaxy / tileH;
+				for (int ty = ty0; ty <= ty1; ++ty) {
+					for (int tx = tx0; tx <= tx1; ++tx) {
+						const int t = ty * tilesX + tx;
+						touched[t] = 1;
+						++statBinned;
+						tiles[t].push_back({ TileOp::TRI, iid, n0, n1, n2, prim, nullptr }); }}}}
+		for (size_t t = 0; t < tiles.size(); ++t) {
+			if (!touched[t]) {
+				tiles[t].pop_back(); }  // untouched: remove the marker (Unappend)
+			else {
+				tiles[t].push_back({ TileOp::TRI, -2, 0, 0, 0, 0, nullptr }); }}  // 0xffff terminator
+		statClipped += static_cast<long long>(clipQueue.size());
+		// ClipTriangles (rglv_gpu_impl.hxx:677-793)
+		for (const ClipItem& ci : clipQueue) {
+			std::vector<ClippedVertex> A, B;
+			for (int k = 0; k < 3; ++k) {
+				ClippedVertex cv{};
+				pgm->ShadeVertex(env, ci.iid, ci.i[k], cv.coord, cv.data);
+				A.push_back(cv); }
+			for (int plane = 0; plane < 5; ++plane) {
+				bool hereIsInside = plane_dist(plane, A[0].coord) >= 0.0F;
+				B.clear();
+				for (size_t h = 0; h < A.size(); ++h) {
+					const size_t n = (h + 1) % A.size();
+					const bool nextIsInside = plane_dist(plane, A[n].coord) >= 0.0F;
+					if (hereIsInside) B.push_back(A[h]);
+					if (hereIsInside != nextIsInside) {
+						const ClippedVertex& from = hereIsInside ? A[h] : A[n];
+						const ClippedVertex& to = hereIsInside ? A[n] : A[h];
+						const float fa = plane_dist(plane, from.coord), fb = plane_dist(plane, to.coord);
+						const float d = fa / (fa - fb);
+						ClippedVertex nv{};
+						for (int q = 0; q < 4; ++q) nv.coord[q] = (1.0F - d) * from.coord[q] + d * to.coord[q];
+						pgm->Mix(from.data, to.data, d, nv.data);
+						B.push_back(nv);
+						hereIsInside = !hereIsInside; }}
+				std::swap(A, B);
+				if (A.empty()) break; }
+			if (A.empty()) continue;
+			for (auto& v : A) {
+				const float iw = oneover(v.coord[3]);  // scalar pdiv (rglv_math.hxx:30-39)
+				v.coord[0] = v.coord[0] * iw; v.coord[1] = v.coord[1] * iw; v.coord[2] = v.coord[2] * iw; v.coord[3] = iw;
+				v.coord[0] = v.coord[0] * DS[0] + DO[0];
+				v.coord[1] = v.coord[1] * DS[1] + DO[1]; }
+			const float d31x = A[2].coord[0] - A[0].coord[0], d31y = A[2].coord[1] - A[0].coord[1];
+			const float d21x = A[1].coord[0] - A[0].coord[0], d21y = A[1].coord[1] - A[0].coord[1];
+			const bool backfacing = (d31x*d21y - d31y*d21x) < 0.0F;
+			bool willCull = true;
+			if (backfacing) { if (!state.cullingEnabled || (state.cullFace & GL_BACK) == 0) willCull = false; }
+			else { if (!state.cullingEnabled || (state.cullFace & GL_FRONT) == 0) willCull = false; }
+			if (willCull) continue;
+			if (backfacing) std::reverse(A.begin(), A.end());
+			const int bi = static_cast<int>(clipped.size());
+			clipped.insert(clipped.end(), A.begin(), A.end());
+			for (int vi = 1; vi < static_cast<int>(A.size()) - 1; ++vi) {
+				const int i0 = 0, i1 = vi, i2 = vi + 1;
+				// ForEachCoveredTile (:795-817)
+				int idx[3] = { cvtt(A[i0].coord[0]), cvtt(A[i1].coord[0]), cvtt(A[i2].coord[0]) };
+				int idy[3] = { cvtt(A[i0].coord[1]), cvtt(A[i1].coord[1]), cvtt(A[i2].coord[1]) };
+				const int vminx = std::max(std::min({ idx[0], idx[1], idx[2] }), sr.x0);
+				const int vminy = std::max(std::min({ idy[0], idy[1], idy[2] }), sr.y0);
+				const int vmaxx = std::min(wadd(std::max({ idx[0], idx[1], idx[2] }), 1), sr.x1 - 1);
+				const int vmaxy = std::min(wadd(std::max({ idy[0], idy[1], idy[2] }), 1), sr.y1 - 1);
+				const int tx0 = vminx / tileW, ty0 = vminy / tileH, tx1 = vmaxx / tileW, ty1 = vmaxy / tileH;
+				for (int ty = ty0; ty <= ty1; ++ty) {
+					for (int tx = tx0; tx <= tx1; ++tx) {
+						if (tx < 0 || ty < 0 || tx >= tilesX || ty >= tilesY) { err = "clipped triangle outside tile grid"; return false; }
+						++statClippedPairs;
+						tiles[ty * tilesX + tx].push_back({ TileOp::CLIPPED_TRI, stateIdx,
+							static_cast<uint16_t>((bi + i0) | (backfacing ? 0x8000 : 0)),
+							static_cast<uint16_t>(bi + i1), static_cast<uint16_t>(bi + i2), ci.prim, nullptr }); }}}}
+		return true; }
+
+	// ---- per-tile drawing (GPU::DrawImpl, rglv_gpu.cxx:263-432) ----
+	struct TileBuf {
+		std::vector<float> r, g, b, d;
+		std::vector<uint32_t> prim;
+		long long fragments = 0; };
+
+	struct TriCtx {
+		const Program* pgm;
+		const DrawEnv* env;
+		float w[3], z[3];
+		VOut vo[3];
+		uint32_t prim;
+		int depthFunc;
+		bool depthTest, depthWrite, colorWrite, blend; };
+
+	// TriangleProgram::Render (rglv_gpu_impl.hxx:166-222) for one quad
+	void Render(TileBuf& tb, const Rect& tr, const TriCtx& tc, int qx, int qy, const float fc[2][4],
+	            const int32_t trimask[4], const float bs[3][4]) const {
+		int pix[4];
+		for (int l = 0; l < 4; ++l) {
+			const int px = qx + (l & 1), py = qy + (l >> 1);
+			pix[l] = (py - tr.y0) * tileW + (px - tr.x0); }
+		float fragDepth[4];
+		for (int l = 0; l < 4; ++l) fragDepth[l] = (bs[0][l]*tc.z[0] + bs[1][l]*tc.z[1]) + bs[2][l]*tc.z[2];
+		for (int l = 0; l < 4; ++l) if (!trimask[l]) ++tb.fragments;  // covered lanes reaching the depth test
+		int32_t mask[4];
+		auto depthPass = [&](int l) {
+			const float dst = tb.d[pix[l]];
+			switch (tc.depthFunc) {
+			case GL_LESS: return fragDepth[l] < dst;
+			case GL_LEQUAL: return fragDepth[l] <= dst;
+			default: return fragDepth[l] == dst; }};
+		const bool earlyZ = tc.pgm->earlyZ;
+		if (tc.depthTest && earlyZ) {
+			bool any = false;
+			for (int l = 0; l < 4; ++l) { mask[l] = (!trimask[l] && depthPass(l)) ? -1 : 0; any |= mask[l] != 0; }
+			if (!any) return; }
+		else {
+			for (int l = 0; l < 4; ++l) mask[l] = trimask[l] ? 0 : -1; }
+		if (tc.depthWrite && earlyZ) {
+			for (int l = 0; l < 4; ++l) if (mask[l]) tb.d[pix[l]] = fragDepth[l]; }
+		// perspective-correct barycentrics
+		float bp[3][4];
+		for (int l = 0; l < 4; ++l) {
+			const float fragW = oneover((bs[0][l]*tc.w[0] + bs[1][l]*tc.w[1]) + bs[2][l]*tc.w[2]);
+			bp[0][l] = (tc.w[0] * bs[0][l]) * fragW;
+			bp[2][l] = (tc.w[2] * bs[2][l]) * fragW;
+			bp[1][l] = (1.0F - bp[0][l]) - bp[2][l]; }
+		float vary[kMaxVary][4];
+		for (int k = 0; k < tc.pgm->nvary; ++k) {
+			for (int l = 0; l < 4; ++l) {
+				const float b3[3] = { bp[0][l], bp[1][l], bp[2][l] };
+				vary[k][l] = interp(b3, tc.vo[0].f[k], tc.vo[1].f[k], tc.vo[2].f[k]); }}
+		float color[4][4];
+		tc.pgm->ShadeFragment(*tc.env, bs, bp, vary, fc, fragDepth, color, mask);
+		if (tc.depthTest && !earlyZ) {
+			bool any = false;
+			for (int l = 0; l < 4; ++l) { if (mask[l] && !depthPass(l)) mask[l] = 0; any |= mask[l] != 0; }
+			if (!any) return; }
+		if (tc.depthWrite && !earlyZ) {
+			for (int l = 0; l < 4; ++l) if (mask[l]) tb.d[pix[l]] = fragDepth[l]; }
+		if (tc.colorWrite) {
+			for (int l = 0; l < 4; ++l) {
+				if (!mask[l]) continue;
+				const int p = pix[l];
+				if (tc.blend) {  // BlendAlpha (:80-84)
+					const float a = color[3][l], oma = 1.0F - a;
+					tb.r[p] = color[0][l]*a + tb.r[p]*oma;
+					tb.g[p] = color[1][l]*a + tb.g[p]*oma;
+					tb.b[p] = color[2][l]*a + tb.b[p]*oma; }
+				else {
+					tb.r[p] = color[0][l]; tb.g[p] = color[1][l]; tb.b[p] = color[2][l]; }}}
+		for (int l = 0; l < 4; ++l) if (mask[l]) tb.prim[pix[l]] = tc.prim; }
+
+	// VTriangleRasterizer::Draw (rglv_triangle.hxx:193-304), one lane
+	void RasterVector(TileBuf& tb, const Rect& tr, const TriCtx& tc, const int32_t X[3], const int32_t Y[3]) const {
+		const int32_t x1 = X[0], x2 = X[1], x3 = X[2], y1 = Y[0], y2 = Y[1], y3 = Y[2];
+		int32_t minx = std::max(sar(std::min(std::min(x1, x2), x3), 4), tr.x0);
+		const int32_t maxx = std::min(sar(wadd(std::max(std::max(x1, x2), x3), 15), 4), tr.x1);
+		int32_t miny = std::max(sar(std::min(std::min(y1, y2), y3), 4), tr.y0);
+		const int32_t maxy = std::min(sar(wadd(std::max(std::max(y1, y2), y3), 15), 4), tr.y1);
+		minx &= ~1; miny &= ~1;
+		const int32_t dx12 = wsub(x1, x2), dy12 = wsub(y2, y1);
+		const int32_t dx23 = wsub(x2, x3), dy23 = wsub(y3, y2);
+		const int32_t dx31 = wsub(x3, x1), dy31 = wsub(y1, y3);
+		const int32_t sx = wadd(shl(minx, 4), 8), sy = wadd(shl(miny, 4), 8);
+		int32_t c1 = wadd(wmul(dy12, wsub(sx, x1)), wmul(dx12, wsub(sy, y1)));
+		int32_t c2 = wadd(wmul(dy23, wsub(sx, x2)), wmul(dx23, wsub(sy, y2)));
+		int32_t c3 = wadd(wmul(dy31, wsub(sx, x3)), wmul(dx31, wsub(sy, y3)));
+		auto tl = [](int32_t dy, int32_t dx) { return (dy > 0 || (dy == 0 && dx > 0)) ? 1 : 0; };
+		c1 = sar(wsub(wadd(c1, tl(dy12, dx12)), 1), 4);
+		c2 = sar(wsub(wadd(c2, tl(dy23, dx23)), 1), 4);
+		c3 = sar(wsub(wadd(c3, tl(dy31, dx31)), 1), 4);
+		const float scale = 1.0F / itof(wadd(wadd(c1, c2), c3));
+		const float topRow = static_cast<float>(H) - 0.5F;
+		int32_t r1 = c1, r2 = c2, r3 = c3;
+		for (int y = miny; y < maxy; y += 2, r1 = wadd(r1, wmul(dx12, 2)), r2 = wadd(r2, wmul(dx23, 2)), r3 = wadd(r3, wmul(dx31, 2))) {
+			int32_t q1 = r1, q2 = r2, q3 = r3;
+			for (int x = minx; x < maxx; x += 2, q1 = wadd(q1, wmul(dy12, 2)), q2 = wadd(q2, wmul(dy23, 2)), q3 = wadd(q3, wmul(dy31, 2))) {
+				int32_t e1[4] = { q1, wadd(q1, dy12), wadd(q1, dx12), wadd(wadd(q1, dx12), dy12) };
+				int32_t e2[4] = { q2, wadd(q2, dy23), wadd(q2, dx23), wadd(wadd(q2, dx23), dy23) };
+				int32_t e3[4] = { q3, wadd(q3, dy31), wadd(q3, dx31), wadd(wadd(q3, dx31), dy31) };
+				int32_t trimask[4]; int neg = 0;
+				for (int l = 0; l < 4; ++l) { trimask[l] = (e1[l] | e2[l] | e3[l]) >> 31; neg += trimask[l] ? 1 : 0; }
+				if (neg == 4) continue;
+				float fc[2][4], bs[3][4];
+				const float fx = static_cast<float>(x) + 0.5F;
+				const float fy = topRow - static_cast<float>(y);
+				for (int l = 0; l < 4; ++l) {
+					fc[0][l] = fx + static_cast<float>(l & 1);
+					fc[1][l] = fy - static_cast<float>(l >> 1);
+					bs[0][l] = itof(e2[l]) * scale;
+					bs[2][l] = itof(e1[l]) * scale;
+					bs[1][l] = (1.0F - bs[0][l]) - bs[2][l]; }
+				Render(tb, tr, tc, x, y, fc, trimask, bs); }}}
+
+	// TriangleRasterizer::Draw (rglv_triangle.hxx:79-167), clipped triangles
+	void RasterScalar(TileBuf& tb, const Rect& tr, const TriCtx& tc, const int32_t X[3], const int32_t Y[3]) const {
+		const int x1 = X[0], x2 = X[1], x3 = X[2], y1 = Y[0], y2 = Y[1], y3 = Y[2];
+		int minx = std::max(std::min({ x1, x2, x3 }) >> 4, tr.x0);
+		const int endx = std::min((std::max({ x1, x2, x3 }) + 15) >> 4, tr.x1);
+		int miny = std::max(std::min({ y1, y2, y3 }) >> 4, tr.y0);
+		const int endy = std::min((std::max({ y1, y2, y3 }) + 15) >> 4, tr.y1);
+		minx &= ~1; miny &= ~1;
+		const int64_t dx12 = x1 - x2, dy12 = y2 - y1;
+		const int64_t dx23 = x2 - x3, dy23 = y3 - y2;
+		const int64_t dx31 = x3 - x1, dy31 = y1 - y3;
+		const int64_t sx = (static_cast<int64_t>(minx) << 4) + 8, sy = (static_cast<int64_t>(miny) << 4) + 8;
+		int64_t c1 = dy12*(sx - x1) + dx12*(sy - y1);
+		int64_t c2 = dy23*(sx - x2) + dx23*(sy - y2);
+		int64_t c3 = dy31*(sx - x3) + dx31*(sy - y3);
+		if (dy12 > 0 || (dy12 == 0 && dx12 > 0)) { c1++; }
+		--c1;
+		if (dy23 > 0 || (dy23 == 0 && dx23 > 0)) { c2++; }
+		--c2;
+		if (dy31 > 0 || (dy31 == 0 && dx31 > 0)) { c3++; }
+		--c3;
+		c1 >>= 4; c2 >>= 4; c3 >>= 4;
+		int32_t cb1[4] = { static_cast<int32_t>(c1), static_cast<int32_t>(c1 + dy12), static_cast<int32_t>(c1 + dx12), static_cast<int32_t>(c1 + dx12 + dy12) };
+		int32_t cb2[4] = { static_cast<int32_t>(c2), static_cast<int32_t>(c2 + dy23), static_cast<int32_t>(c2 + dx23), static_cast<int32_t>(c2 + dx23 + dy23) };
+		int32_t cb3[4] = { static_cast<int32_t>(c3), static_cast<int32_t>(c3 + dy31), static_cast<int32_t>(c3 + dx31), static_cast<int32_t>(c3 + dx31 + dy31) };
+		const int32_t s1x = wmul(static_cast<int32_t>(dy12), 2), s1y = wmul(static_cast<int32_t>(dx12), 2);
+		const int32_t s2x = wmul(static_cast<int32_t>(dy23), 2), s2y = wmul(static_cast<int32_t>(dx23), 2);
+		const int32_t s3x = wmul(static_cast<int32_t>(dy31), 2), s3y = wmul(static_cast<int32_t>(dx31), 2);
+		const float scale = 1.0F / static_cast<float>(c1 + c2 + c3);
+		for (int y = miny; y < endy; y += 2) {
+			int32_t q1[4], q2[4], q3[4];
+			std::memcpy(q1, cb1, sizeof q1); std::memcpy(q2, cb2, sizeof q2); std::memcpy(q3, cb3, sizeof q3);
+			for (int x = minx; x < endx; x += 2) {
+				int32_t trimask[4]; int neg = 0;
+				for (int l = 0; l < 4; ++l) { trimask[l] = (q1[l] | q2[l] | q3[l]) >> 31; neg += trimask[l] ? 1 : 0; }
+				if (neg != 4) {
+					float fc[2][4], bs[3][4];
+					const float fx = static_cast<float>(x) + 0.5F;
+					const float fy = static_cast<float>(H - y) - 0.5F;
+					for (int l = 0; l < 4; ++l) {
+						fc[0][l] = fx + static_cast<float>(l & 1);
+						fc[1][l] = fy + static_cast<float>(l < 2 ? 1 : 0);
+						bs[0][l] = itof(q2[l]) * scale;
+						bs[2][l] = itof(q1[l]) * scale;
+						bs[1][l] = (1.0F - bs[0][l]) - bs[2][l]; }
+					Render(tb, tr, tc, x, y, fc, trimask, bs); }
+				for (int l = 0; l < 4; ++l) { q1[l] = wadd(q1[l], s1x); q2[l] = wadd(q2[l], s2x); q3[l] = wadd(q3[l], s3x); }}
+			for (int l = 0; l < 4; ++l) { cb1[l] = wadd(cb1[l], s1y); cb2[l] = wadd(cb2[l], s2y); cb3[l] = wadd(cb3[l], s3y); }}}
+
+	bool DrawTile(int t, TileBuf& tb) {
+		const Rect tr = TileRect(t);
+		const size_t area = static_cast<size_t>(tileW) * tileH;
+		tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F);
+		tb.d.assign(area, 0.0F); tb.prim.assign(area, 0xffffffffu);
+		const GLState* st = nullptr;
+		DrawEnv env{};
+		const Program* pgm = nullptr;
+		bool inDraw = false;
+		for (const TileOp& op : tiles[t]) {
+			switch (op.op) {
+			case TileOp::STATE:
+				st = &states[op.a];
+				if (!MakeEnv(*st, env)) return false;
+				pgm = find_program(st->programId);
+				break;
+			case TileOp::CLEAR:
+				if (op.a & GL_COLOR_BUFFER_BIT) {
+					std::fill(tb.r.begin(), tb.r.end(), st->clearColor[0]);
+					std::fill(tb.g.begin(), tb.g.end(), st->clearColor[1]);
+					std::fill(tb.b.begin(), tb.b.end(), st->clearColor[2]);
+					std::fill(tb.prim.begin(), tb.prim.end(), 0xffffffffu); }
+				if (op.a & GL_DEPTH_BUFFER_BIT) std::fill(tb.d.begin(), tb.d.end(), st->clearDepth);
+				break;
+			case TileOp::TRI: {
+				if (op.a == -1) { inDraw = true; break; }
+				if (op.a == -2) { inDraw = false; break; }
+				if (!inDraw || !pgm) { err = "triangle outside a draw"; return false; }
+				TriCtx tc{ pgm, &env, {}, {}, {}, op.prim, st->depthFunc, st->depthTestEnabled,
+				           st->depthWriteMask, st->colorWriteMask, st->blendingEnabled };
+				float DS[2], DO[2]; DSDO(*st, DS, DO);
+				const uint16_t idx[3] = { static_cast<uint16_t>(op.i0 & 0x7fff), op.i1, op.i2 };
+				int32_t X[3], Y[3];
+				for (int k = 0; k < 3; ++k) {  // GPUTileImpl::DrawTriangles flush (:880-886)
+					float pos[4];
+					pgm->ShadeVertex(env, op.a, idx[k], pos, tc.vo[k]);
+					const float iw = oneover(pos[3]);
+					const float dx = pos[0] * iw, dy = pos[1] * iw, dz = pos[2] * iw;
+					X[k] = cvtt(16.0F * (dx * DS[0] + DO[0]));
+					Y[k] = cvtt(16.0F * (dy * DS[1] + DO[1]));
+					tc.w[k] = iw; tc.z[k] = dz; }
+				RasterVector(tb, tr, tc, X, Y);
+				break; }
+			case TileOp::CLIPPED_TRI: {  // DrawClipped (:949-998)
+				if (!pgm) { err = "clipped triangle outside a draw"; return false; }
+				TriCtx tc{ pgm, &env, {}, {}, {}, op.prim, st->depthFunc, st->depthTestEnabled,
+				           st->depthWriteMask, st->colorWriteMask, st->blendingEnabled };
+				const int idx[3] = { op.i0 & 0x7fff, op.i1, op.i2 };
+				int32_t X[3], Y[3];
+				for (int k = 0; k < 3; ++k) {
+					const ClippedVertex& cv = clipped[idx[k]];
+					X[k] = static_cast<int32_t>(cvtt(cv.coord[0] * 16.0F));
+					Y[k] = static_cast<int32_t>(cvtt(cv.coord[1] * 16.0F));
+					tc.w[k] = cv.coord[3]; tc.z[k] = cv.coord[2]; tc.vo[k] = cv.data; }
+				RasterScalar(tb, tr, tc, X, Y);
+				break; }
+			case TileOp::STORE_TC: {  // GPUBltImpl::StoreTrueColor -> FilterTile (rglr_algorithm.hxx:31-104)
+				if (!known_post_program(st->programId)) { err = "no blt dispatch entry"; return false; }
+				const float* u = Uniforms(*st);
+				const float exposure = (st->programId == 2 && u) ? u[0] : 1.0F;
+				auto* out = static_cast<uint32_t*>(const_cast<void*>(op.ptr));
+				for (int y = tr.y0; y < tr.y1; ++y) {
+					for (int x = tr.x0; x < tr.x1; ++x) {
+						const int p = (y - tr.y0) * tileW + (x - tr.x0);
+						float r = tb.r[p], g = tb.g[p], b = tb.b[p];
+						if (st->programId == 2) { r = r * exposure; g = g * exposure; b = b * exposure; }
+						uint32_t px;
+						if (op.a) px = (to_srgb8(r) << 16) | (to_srgb8(g) << 8) | to_srgb8(b);
+						else px = (to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b);
+						out[static_cast<size_t>(y) * W + x] = px; }}
+				break; }
+			case TileOp::STORE_DEPTH: {
+				auto* out = static_cast<float*>(const_cast<void*>(op.ptr));
+				for (int y = tr.y0; y < tr.y1; ++y)
+					for (int x = tr.x0; x < tr.x1; ++x)
+						out[static_cast<size_t>(y) * W + x] = tb.d[(y - tr.y0) * tileW + (x - tr.x0)];
+				break; }
+			case TileOp::STORE_PRIM: {
+				auto* out = static_cast<uint32_t*>(const_cast<void*>(op.ptr));
+				for (int y = tr.y0; y < tr.y1; ++y)
+					for (int x = tr.x0; x < tr.x1; ++x)
+						out[static_cast<size_t>(y) * W + x] = tb.prim[(y - tr.y0) * tileW + (x - tr.x0)];
+				break; }}}
+		return true; }
+
+	bool Run() {
+		err.clear();
+		if (!Bin()) return false;
+		// cost-sorted tile order (GPU::RunImpl, rglv_gpu.cxx:100-104)
+		std::vector<std::pair<size_t, int>> order;
+		for (int t = 0; t < static_cast<int>(tiles.size()); ++t) order.push_back({ tiles[t].size(), t });
+		std::stable_sort(order.begin(), order.end(), [](auto a, auto b) { return a.first > b.first; });
+		std::atomic<int> next{ 0 };
+		std::atomic<bool> ok{ true };
+		std::atomic<long long> frags{ 0 };
+		std::vector<std::string> errs(static_cast<size_t>(std::max(threads, 1)));
+		auto worker = [&](int wid) {
+			TileBuf tb;
+			for (;;) {
+				const int i = next.fetch_add(1);
+				if (i >= static_cast<int>(order.size())) break;
+				if (!DrawTileSafe(order[i].second, tb, errs[wid])) { ok = false; break; }}
+			frags += tb.fragments; };
+		const int nt = std::max(threads, 1);
+		if (nt == 1) worker(0);
+		else {
+			std::vector<std::thread> pool;
+			for (int i = 0; i < nt; ++i) pool.emplace_back(worker, i);
+			for (auto& th : pool) th.join(); }
+		statFragments += frags.load();
+		if (!ok) { for (auto& e : errs) if (!e.empty()) err = e; return false; }
+		return true; }
+
+	bool DrawTileSafe(int t, TileBuf& tb, std::string& e) {
+		const bool r = DrawTile(t, tb);
+		if (!r) e = err.empty() ? std::string("draw failed") : err;
+		return r; }
+
+	void SetSize(int w, int h, int tbx, int tby) {  // GPU::Reset / Retile (rglv_gpu.cxx:50-87)
+		W = w; H = h; tileW = tbx * 8; tileH = tby * 8;
+		tilesX = (W + tileW - 1) / tileW; tilesY = (H + tileH - 1) / tileH; }
+
+	void ResetGL() {  // GL::Reset (rglv_gl.cxx:440-445)
+		dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); }
+};
+
+// ------------------------------------------------------------------------
+// C ABI, same shape as the product's rgl_* entry points
+// ------------------------------------------------------------------------
+extern "C" {
+
+rglo_gpu* rglo_gpu_create(int threads) { auto* g = new rglo_gpu(); g->threads = threads; return g; }
+void rglo_gpu_destroy(rglo_gpu* g) { delete g; }
+void rglo_gpu_set_threads(rglo_gpu* g, int threads) { g->threads = threads; }
+const char* rglo_gpu_last_error(rglo_gpu* g) { return g->err.c_str(); }
+int rglo_gpu_install(rglo_gpu* g, int programId, uint32_t stateKey, int kind) {
+	// kind 0 bin, 1 draw, 2 blt.  Bin entries are tagged with the top bit.
+	uint32_t key = (static_cast<uint32_t>(programId) << 24) | stateKey;
+	if (kind == 0) key |= 0x80000000u;
+	if (kind == 2) key |= 0x40000000u;
+	g->installed.push_back(key);
+	return 0; }
+int rglo_gpu_reset(rglo_gpu* g, int w, int h, int tbx, int tby) {
+	if (w <= 0 || h <= 0 || tbx <= 0 || tby <= 0 || tbx > 16 || tby > 16) { g->err = "bad size"; return -1; }
+	g->SetSize(w, h, tbx, tby); g->ResetGL(); return 0; }
+int rglo_gpu_run(rglo_gpu* g) { return g->Run() ? 0 : -1; }
+void rglo_gpu_stats(rglo_gpu* g, long long out[5]) {
+	out[0] = g->statSubmitted; out[1] = g->statClipped; out[2] = g->statBinned; out[3] = g->statFragments; out[4] = g->statClippedPairs;
+	g->statSubmitted = g->statClipped = g->statBinned = g->statFragments = g->statClippedPairs = 0; }
+
+void rglo_gl_enable(rglo_gpu* g, int v) {
+	g->dirty = true;
+	if (v == GL_CULL_FACE) g->cs.cullingEnabled = true;
+	else if (v == GL_SCISSOR_TEST) g->cs.scissorEnabled = true;
+	else if (v == GL_BLEND) g->cs.blendingEnabled = true;
+	else if (v == GL_DEPTH_TEST) g->cs.depthTestEnabled = true; }
+void rglo_gl_disable(rglo_gpu* g, int v) {
+	g->dirty = true;
+	if (v == GL_CULL_FACE) g->cs.cullingEnabled = false;
+	else if (v == GL_SCISSOR_TEST) g->cs.scissorEnabled = false;
+	else if (v == GL_BLEND) g->cs.blendingEnabled = false;
+	else if (v == GL_DEPTH_TEST) g->cs.depthTestEnabled = false; }
+void rglo_gl_depth_func(rglo_gpu* g, int v) { g->dirty = true; g->cs.depthFunc = v; }
+void rglo_gl_depth_write_mask(rglo_gpu* g, int v) { g->dirty = true; g->cs.depthWriteMask = v != 0; }
+void rglo_gl_color_write_mask(rglo_gpu* g, int v) { g->dirty = true; g->cs.colorWriteMask = v != 0; }
+void rglo_gl_cull_face(rglo_gpu* g, int v) { g->dirty = true; g->cs.cullFace = v; }
+void rglo_gl_scissor(rglo_gpu* g, int x, int y, int w, int h) {
+	g->dirty = true; g->cs.scissorOrigin[0] = x; g->cs.scissorOrigin[1] = y; g->cs.scissorSize[0] = w; g->cs.scissorSize[1] = h; }
+void rglo_gl_viewport(rglo_gpu* g, int x, int y, int w, int h) {
+	g->dirty = true; g->cs.viewportOrigin[0] = x; g->cs.viewportOrigin[1] = y;
+	g->cs.hasViewportSize = true; g->cs.viewportSize[0] = w; g->cs.viewportSize[1] = h; }
+void rglo_gl_use_program(rglo_gpu* g, int v) { g->dirty = true; g->cs.programId = v; }
+void rglo_gl_use_uniforms(rglo_gpu* g, int ofs) { g->dirty = true; g->cs.uniformsOfs = ofs; }
+int rglo_gl_alloc_uniform_buffer(rglo_gpu* g, const float* values, int n) {
+	const int idx = static_cast<int>(g->ubuf.size());
+	g->ubuf.resize(g->ubuf.size() + UNIFORM_BUFFER_SIZE, 0.0F);
+	for (int i = 0; i < n && i < UNIFORM_BUFFER_SIZE; ++i) g->ubuf[idx + i] = values[i];
+	return idx; }
+void rglo_gl_renderbuffer_type(rglo_gpu* g, int attachment, int type) {
+	g->dirty = true;
+	if (attachment == GL_DEPTH_ATTACHMENT) g->cs.depthAttachmentType = type;
+	else if (attachment == GL_COLOR_ATTACHMENT0) g->cs.color0AttachmentType = type; }
+void rglo_gl_bind_texture(rglo_gpu* g, int unit, const float* ptr, int w, int h, int stride, int mode) {
+	g->dirty = true; g->cs.tus[unit] = { ptr, w, h, stride, mode }; }
+void rglo_gl_use_buffer(rglo_gpu* g, int idx, const float* ptr) { g->dirty = true; g->cs.buffers[idx] = ptr; }
+void rglo_gl_clear_color(rglo_gpu* g, float r, float gg, float b) {
+	g->dirty = true; g->cs.clearColor[0] = r; g->cs.clearColor[1] = gg; g->cs.clearColor[2] = b; g->cs.clearColor[3] = 1.0F; }
+void rglo_gl_clear_depth(rglo_gpu* g, float v) { g->dirty = true; g->cs.clearDepth = v; }
+void rglo_gl_view_matrix(rglo_gpu* g, const float* m) { g->dirty = true; std::memcpy(g->cs.viewMatrix.ff, m, 64); }
+void rglo_gl_projection_matrix(rglo_gpu* g, const float* m) { g->dirty = true; std::memcpy(g->cs.projectionMatrix.ff, m, 64); }
+void rglo_gl_normal_matrix(rglo_gpu* g, const float* m) { g->dirty = true; std::memcpy(g->cs.normalMatrix.ff, m, 64); }
+void rglo_gl_draw_elements(rglo_gpu* g, int, int count, int, const uint16_t* indices, int) {
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ELEMENTS, count, 1, 0, indices }); }
+void rglo_gl_draw_elements_instanced(rglo_gpu* g, int, int count, int, const uint16_t* indices, int instanceCnt) {
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ELEMENTS_INSTANCED, count, instanceCnt, 0, indices }); }
+void rglo_gl_draw_arrays(rglo_gpu* g, int, int, int count) {
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ARRAYS, count, 1, 0, nullptr }); }
+void rglo_gl_draw_arrays_instanced(rglo_gpu* g, int, int, int count, int instanceCnt) {
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ARRAYS_INSTANCED, count, instanceCnt, 0, nullptr }); }
+void rglo_gl_clear(rglo_gpu* g, int bits) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::CLEAR, bits, 0, 0, nullptr }); }
+void rglo_gl_store_color_tc(rglo_gpu* g, uint32_t* dst, int gamma) {
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_TC, gamma ? 1 : 0, 0, 0, dst }); }
+void rglo_gl_store_depth(rglo_gpu* g, float* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_DEPTH, 0, 0, 0, dst }); }
+void rglo_gl_store_primitive_id(rglo_gpu* g, uint32_t* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_PRIM, 0, 0, 0, dst }); }
+void rglo_gl_reset(rglo_gpu* g) { g->ResetGL(); }
+
+// building blocks exposed for known-answer tests
+float rglo_oneover(float x) { return oneover(x); }
+float rglo_rcpps(float x) { return rcpps(x); }
+int32_t rglo_cvtt(float x) { return cvtt(x); }
+uint32_t rglo_to_srgb8(float x) { return to_srgb8(x); }
+
+// the reference triangle test harness (rglv_triangle.t.cxx:35-131, 251-377):
+// draws one triangle given in device coordinates with TriangleRasterizer into a
+// w x h target, LEQUAL against depth 1.0, and returns per pixel
+// [covered, BS.x, BS.y, BS.z, u, v] where (u, v) are the BP-interpolated
+// vertex uvs.  Uncovered pixels are all zero.
+int rglo_scalar_triangle(int w, int h, const float* pts /*3 x vec4*/, const float* uvs /*3 x vec2*/, float* out) {
+	struct Capture : Program {
+		Capture() { id = 1000; nvary = 2; earlyZ = true; }
+		void ShadeVertex(const DrawEnv&, int, int, float*, VOut&) const override {}
+		void ShadeFragment(const DrawEnv&, const float bs[3][4], const float[3][4], const float vary[kMaxVary][4],
+		                   const float[2][4], const float[4], float color[4][4], int32_t[4]) const override {
+			for (int l = 0; l < 4; ++l) { color[0][l] = bs[0][l]; color[1][l] = bs[1][l]; color[2][l] = bs[2][l];
+			                              color[3][l] = vary[0][l]; }
+			(void)vary; }};
+	static const Capture cap;
+	rglo_gpu g; g.SetSize(w, h, 1, 1);
+	g.tileW = w; g.tileH = h;
+	rglo_gpu::TileBuf tb;
+	const size_t area = static_cast<size_t>(w) * h;
+	tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F); tb.d.assign(area, 1.0F); tb.prim.assign(area, 0xffffffffu);
+	std::vector<float> vv(area, 0.0F);
+	DrawEnv env{};
+	rglo_gpu::TriCtx tc{ &cap, &env, {}, {}, {}, 0u, GL_LEQUAL, true, true, true, false };
+	int32_t X[3], Y[3];
+	for (int k = 0; k < 3; ++k) {
+		X[k] = static_cast<int32_t>(16.0F * pts[k*4 + 0]);
+		Y[k] = static_cast<int32_t>(16.0F * pts[k*4 + 1]);
+		tc.w[k] = pts[k*4 + 3]; tc.z[k] = pts[k*4 + 2];
+		tc.vo[k].f[0] = uvs[k*2 + 0]; tc.vo[k].f[1] = uvs[k*2 + 1]; }
+	const Rect r{ 0, 0, w, h };
+	// pass 1: BS and u; pass 2: v (the capture program has three colour channels)
+	g.RasterScalar(tb, r, tc, X, Y);
+	std::vector<float> bsx = tb.r, bsy = tb.g, bsz = tb.b;
+	std::vector<uint32_t> hit = tb.prim;
+	struct CaptureUV : Program {
+		CaptureUV() { id = 1001; nvary = 2; earlyZ = true; }
+		void ShadeVertex(const DrawEnv&, int, int, float*, VOut&) const override {}
+		void ShadeFragment(const DrawEnv&, const float[3][4], const float[3][4], const float vary[kMaxVary][4],
+		                   const float[2][4], const float[4], float color[4][4], int32_t[4]) const override {
+			for (int l = 0; l < 4; ++l) { color[0][l] = vary[0][l]; color[1][l] = vary[1][l]; color[2][l] = 0.0F; color[3][l] = 0.0F; }}};
+	static const CaptureUV capuv;
+	tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F); tb.d.assign(area, 1.0F); tb.prim.assign(area, 0xffffffffu);
+	tc.pgm = &capuv;
+	g.RasterScalar(tb, r, tc, X, Y);
+	for (size_t p = 0; p < area; ++p) {
+		const bool on = hit[p] != 0xffffffffu;
+		out[p*6 + 0] = on ? 1.0F : 0.0F;
+		out[p*6 + 1] = on ? bsx[p] : 0.0F;
+		out[p*6 + 2] = on ? bsy[p] : 0.0F;
+		out[p*6 + 3] = on ? bsz[p] : 0.0F;
+		out[p*6 + 4] = on ? tb.r[p] : 0.0F;
+		out[p*6 + 5] = on ? tb.g[p] : 0.0F; }
+	return 0; }
+
+}  // extern "C"

@@ -1,0 +1,338 @@
+// rgl_device.h — device-side arithmetic, frame descriptors and shader
+// programs of the gfx950 rgl pipeline.
+//
+// Bit-exact parity with the reference's SSE code rests on three rules that
+// every function here keeps:
+//   * float expressions are written in the reference's evaluation order and
+//     compiled with -ffp-contract=off (no FMA is ever formed);
+//   * division and sqrt are IEEE correctly rounded
+//     (-fhip-fp32-correctly-rounded-divide-sqrt), denormals are kept;
+//   * the x86 instructions whose results are not IEEE-defined are emulated
+//     exactly: cvttps2dq (0x80000000 on overflow/NaN), rcpps (captured
+//     table), maxps/minps (NaN returns the second operand), and 32-bit
+//     wrapping integer products.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rgl {
+
+#include "rcpps_table.inc"
+#include "srgb8_table.inc"
+
+constexpr int kMaxDraws = 64;
+constexpr int kMaxVary = 8;        // varyings per vertex (largest program)
+constexpr int kClipVerts = 8;      // 3 + one per clipping plane
+constexpr int kFanMax = 6;         // kClipVerts - 2
+constexpr int kSlotsPerTri = 1 + kFanMax;  // inline slot + fan slots
+
+// ---------------------------------------------------------------------------
+// SSE-faithful scalar arithmetic (reference src/rml/rmlv/rmlv_mvec4.hxx)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+__host__ __device__ inline float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+// _mm_cvttps_epi32 / cvttss2si (rmlv_mvec4.hxx:552)
+__host__ __device__ inline int32_t cvtt(float f) {
+	return (f >= -2147483648.0F && f < 2147483648.0F) ? static_cast<int32_t>(f) : INT32_MIN; }
+
+__host__ __device__ inline float itof(int32_t i) { return static_cast<float>(i); }
+
+// rcpps on the Intel host the reference output is defined on:
+// result mantissa from an 11-bit table, exponent 253 - e.
+__device__ inline float rcpps(float x) {
+	const uint32_t in = f2u(x);
+	const uint32_t e = (in >> 23) & 0xffu, s = in & 0x80000000u, m = in & 0x7fffffu;
+	if (e == 0xffu) return u2f(m ? (in | 0x400000u) : s);
+	if (e == 0u) return u2f(s | 0x7f800000u);
+	const int re = 253 - static_cast<int>(e);
+	if (re <= 0) return u2f(s);
+	return u2f(s | (static_cast<uint32_t>(re) << 23) | kRcpMantissa[m >> 12]); }
+
+// rmlv::oneover: rcpps + one Newton step (rmlv_mvec4.hxx:645-649)
+__device__ inline float oneover(float a) {
+	const float r = rcpps(a);
+	const float muls = a * (r * r);
+	return (r + r) - muls; }
+
+__host__ __device__ inline float sse_max(float a, float b) { return a > b ? a : b; }
+__host__ __device__ inline float sse_min(float a, float b) { return a < b ? a : b; }
+
+__host__ __device__ inline int32_t wadd(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b)); }
+__host__ __device__ inline int32_t wsub(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) - static_cast<uint32_t>(b)); }
+__host__ __device__ inline int32_t wmul(int32_t a, int32_t b) { return static_cast<int32_t>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b)); }
+__host__ __device__ inline int32_t shl(int32_t a, int n) { return static_cast<int32_t>(static_cast<uint32_t>(a) << n); }
+__host__ __device__ inline int32_t imin(int32_t a, int32_t b) { return a < b ? a : b; }
+__host__ __device__ inline int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+
+// ryg float_to_srgb8_var2_SSE2 (3rdparty/ryg-srgb/ryg-srgb.h:183-219)
+__device__ inline uint32_t to_srgb8(float f) {
+	const float clamp1 = sse_max(f, u2f((127u - 13u) << 23));
+	const float clamp2 = sse_min(clamp1, u2f(0x3f7fffffu));
+	const uint32_t bits = f2u(clamp2);
+	const uint32_t tab = kSrgb8Tab[(bits >> 20) - (127u - 13u) * 8u];
+	const uint32_t t = (bits >> 12) & 0xffu;
+	return ((tab & 0xffffu) * t + (tab >> 16) * 512u) >> 16; }
+
+// to_tc_basic (rglr_canvas_util.hxx:40-48)
+__device__ inline uint32_t to_linear8(float a) {
+	float r = sse_min(a, 1.0F);
+	r = sse_max(r, 0.0F);
+	return static_cast<uint32_t>(cvtt(r * 255.0F)); }
+
+// qmat4 * qfloat4 (rmlm_soa.hxx:58-64), column-major f
+__device__ inline void xform4(const float* f, const float b[4], float o[4]) {
+	o[0] = ((f[0]*b[0] + f[4]*b[1]) + f[ 8]*b[2]) + f[12]*b[3];
+	o[1] = ((f[1]*b[0] + f[5]*b[1]) + f[ 9]*b[2]) + f[13]*b[3];
+	o[2] = ((f[2]*b[0] + f[6]*b[1]) + f[10]*b[2]) + f[14]*b[3];
+	o[3] = ((f[3]*b[0] + f[7]*b[1]) + f[11]*b[2]) + f[15]*b[3]; }
+
+// mul_w0 (rmlm_soa.hxx:42-47)
+__device__ inline void xform3w0(const float* f, const float b[3], float o[3]) {
+	o[0] = (f[0]*b[0] + f[4]*b[1]) + f[ 8]*b[2];
+	o[1] = (f[1]*b[0] + f[5]*b[1]) + f[ 9]*b[2];
+	o[2] = (f[2]*b[0] + f[6]*b[1]) + f[10]*b[2]; }
+
+__device__ inline void normalize3(float& x, float& y, float& z) {
+	const float len2 = (x*x + y*y) + z*z;
+	const float inv = 1.0F / __builtin_sqrtf(len2);
+	x = x * inv; y = y * inv; z = z * inv; }
+
+// depth ordering key: float order as unsigned, -0 == +0
+__device__ inline uint32_t depth_order(float d) {
+	uint32_t u = f2u(d);
+	if (u == 0x80000000u) u = 0u;
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u); }
+
+// ---------------------------------------------------------------------------
+// frame descriptors (host fills, device reads)
+// ---------------------------------------------------------------------------
+struct SamplerDesc {
+	const float* buf;
+	int kind;      // -1 none, 0 nearest-mip, 1 linear-mip, 2 non-power-of-two nearest
+	int power, width, height, stride;
+	int texels;    // number of RGBA texels addressable (bounds for the UB cases)
+};
+
+struct DrawDesc {
+	int program;
+	int instances;           // I
+	int tris;                // T per instance
+	int verts;               // V: vertex-cache stride per instance
+	int nvary;               // varyings per vertex for this program
+	int elements;            // 1: indexed
+	long long triBase;       // first global triangle index of this draw
+	long long vBase;         // first vertex-cache entry
+	uint32_t slotBase;       // first visibility slot: 1 + 7 * triBase
+	uint32_t primBase;       // first primitive id (== triBase)
+	const uint16_t* indices;
+	const float* buffers[16];
+	float vpm[16];           // projection * view (MakeMatrices, rglv_gpu_impl.hxx:44-51)
+	float DS[2], DO[2];      // GPU::DSDO (rglv_gpu.hxx:265-270)
+	float gbFactor;          // CalcGuardBandFactor (rglv_view_frustum.hxx:257-260)
+	int scissor[4];          // x0, y0, x1, y1 (GPU::ScissorRect)
+	int keepBacks, keepFronts;
+	int cullingEnabled, cullFace;
+	float uniforms[32];      // UniformsSD (<= 128 bytes)
+	SamplerDesc tu0, tu1;
+};
+
+struct FrameDesc {
+	int W, H, tileW, tileH, tilesX, tilesY, nTiles;
+	int nDraws;
+	long long totalTris;     // S
+	int units;               // binning units (kUnitTris triangles each)
+	int depthFunc;           // GL_LESS or GL_LEQUAL for the whole frame
+	int drawEnabled;         // 0 when the clear depth makes every test fail
+	float clearColor[3];
+	float clearDepth;
+	int doClearColor, doClearDepth;
+	uint32_t* outColor; int gamma; int postProgram; float exposure;
+	float* outDepth;
+	uint32_t* outPrim;
+};
+
+// ---------------------------------------------------------------------------
+// texture units (rglr_texture_sampler.cxx)
+// ---------------------------------------------------------------------------
+// LevelOfDetail (rglr_texture_sampler.cxx:25-42): coarse, from lanes 0 and 1
+__device__ inline int level_of_detail(float u0, float u1, float v0, float v1) {
+	const float dux = u1 - u0;
+	const float dvx = v1 - v0;
+	const float sqd = dux*dux + dvx*dvx;
+	return (static_cast<int32_t>(f2u(sqd)) - (127 << 23)) >> 24; }
+
+__device__ inline int clamp_texel(int32_t ofs4, int texels) {
+	// offsets outside the buffer are undefined behaviour in the reference
+	// (negative wrap coordinates); clamp so the device never faults
+	const int32_t t = ofs4 >> 2;
+	return (t < 0 || t >= texels) ? 0 : ofs4; }
+
+// one quad: u[4], v[4] -> rgba[4][4] (channel, lane)
+__device__ inline void sample_quad(const SamplerDesc& s, const float u[4], const float v[4], float out[4][4]) {
+	if (s.kind < 0 || s.buf == nullptr) {
+		for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c][l] = 0.0F;
+		return; }
+	const float almostone = u2f(0x3f7fffffu);
+	if (s.kind == 2) {  // TextureUnitRGBAF32_NM_ONEMAP_WRAP_NEAREST (:289-311)
+		for (int l = 0; l < 4; ++l) {
+			const float fu = u[l] + 100.0F, fv = v[l] + 100.0F;
+			const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+			const int32_t pu = cvtt(fru * static_cast<float>(s.width));
+			const int32_t pv = cvtt((almostone - frv) * static_cast<float>(s.height));
+			const int ofs = clamp_texel(shl(wadd(wmul(pv, s.stride), pu), 2), s.texels);
+			const float4 t = *reinterpret_cast<const float4*>(s.buf + ofs);
+			out[0][l] = t.x; out[1][l] = t.y; out[2][l] = t.z; out[3][l] = t.w; }
+		return; }
+	const int power = s.power;
+	const float baseDim = static_cast<float>(1 << power);
+	int lod = level_of_detail(u[0] * baseDim, u[1] * baseDim, v[0] * baseDim, v[1] * baseDim);
+	lod = lod < 0 ? 0 : (lod > power ? power : lod);
+	if (s.kind == 0) {  // ..._P2_MIPMAP_WRAP_NEAREST (:63-109)
+		const float levelDim = static_cast<float>(1 << (power - lod));
+		const int32_t levelBeginRow = static_cast<int32_t>((0xfffffffeu << (power - lod)) & ((1u << (power + 1)) - 1u));
+		for (int l = 0; l < 4; ++l) {
+			const float fu = u[l] + 10.0F, fv = v[l] + 10.0F;
+			const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+			const int32_t lx = cvtt(fru * levelDim);
+			const int32_t ly = cvtt((almostone - frv) * levelDim);
+			const int ofs = clamp_texel(shl(wadd(shl(wadd(levelBeginRow, ly), power), lx), 2), s.texels);
+			const float4 t = *reinterpret_cast<const float4*>(s.buf + ofs);
+			out[0][l] = t.x; out[1][l] = t.y; out[2][l] = t.z; out[3][l] = t.w; }
+		return; }
+	// ..._P2_MIPMAP_WRAP_LINEAR (:172-286)
+	const int levelDim = 1 << (power - lod);
+	const int32_t wrapMask = levelDim - 1;
+	const int32_t levelLastRow = static_cast<int32_t>((0xffffffffu << (power - lod)) & ((1u << (power + 1)) - 1u)) - 1;
+	for (int l = 0; l < 4; ++l) {
+		const float lx = u[l] * static_cast<float>(levelDim);
+		const float ly = v[l] * static_cast<float>(levelDim);
+		int32_t tx0 = cvtt(lx - 0.5F), ty0 = cvtt(ly - 0.5F);
+		int32_t tx1 = wadd(tx0, 1), ty1 = wadd(ty0, 1);
+		const float fx = (lx - itof(tx0)) - 0.5F;
+		const float fy = (ly - itof(ty0)) - 0.5F;
+		const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
+		const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
+		tx0 &= wrapMask; ty0 &= wrapMask; tx1 &= wrapMask; ty1 &= wrapMask;
+		const int32_t by0 = levelLastRow - ty0, by1 = levelLastRow - ty1;
+		const float4 p00 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx0)) << 2));
+		const float4 p10 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx1)) << 2));
+		const float4 p01 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx0)) << 2));
+		const float4 p11 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx1)) << 2));
+		out[0][l] = ((p00.x * w00 + p10.x * w10) + p01.x * w01) + p11.x * w11;
+		out[1][l] = ((p00.y * w00 + p10.y * w10) + p01.y * w01) + p11.y * w11;
+		out[2][l] = ((p00.z * w00 + p10.z * w10) + p01.z * w01) + p11.z * w11;
+		out[3][l] = ((p00.w * w00 + p10.w * w10) + p01.w * w01) + p11.w * w11; }}
+
+// ---------------------------------------------------------------------------
+// shader programs, in the reference's program concept
+// (rglv_gpu_shaders.hxx:14-95): ShadeVertex per vertex, ShadeFragment per
+// 2x2 quad, varyings interpolated with the perspective-correct BP.
+// ---------------------------------------------------------------------------
+__device__ inline float buf_at(const float* p, int idx, float dflt) { return p ? p[idx] : dflt; }
+
+__device__ inline void load_pos(const DrawDesc& d, int idx, float p[4]) {
+	p[0] = buf_at(d.buffers[0], idx, 0.0F); p[1] = buf_at(d.buffers[1], idx, 0.0F);
+	p[2] = buf_at(d.buffers[2], idx, 0.0F); p[3] = 1.0F; }
+
+// AmyProgram (viewer/shaders.hxx:69-161)
+struct ProgAmy {
+	static constexpr int kId = 4, kVary = 2;
+	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		xform4(d.vpm, p, pos);
+		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
+		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
+	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+		float t[4][4];
+		sample_quad(d.tu0, vary[0], vary[1], t);
+		for (int c = 0; c < 3; ++c) for (int l = 0; l < 4; ++l) color[c][l] = t[c][l]; }};
+
+// ManyProgram (viewer/shaders.hxx:482-583)
+struct ProgMany {
+	static constexpr int kId = 6, kVary = 2;
+	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		const float* imat = d.buffers[15] + 16 * iid;
+		float m[16];
+		for (int k = 0; k < 16; ++k) m[k] = imat[k];
+		float p1[4]; xform4(m, p, p1);
+		xform4(d.vpm, p1, pos);
+		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
+		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
+	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+		for (int l = 0; l < 4; ++l) { color[0][l] = vary[0][l]; color[1][l] = vary[1][l]; color[2][l] = d.uniforms[0]; }}};
+
+// LambertTex (this project): textured, one directional light.
+// uniforms: [0..2] unit direction toward the light, [4] ambient, [5] diffuse
+struct ProgLambertTex {
+	static constexpr int kId = 80, kVary = 5;
+	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		xform4(d.vpm, p, pos);
+		vo[0] = buf_at(d.buffers[3], idx, 0.0F); vo[1] = buf_at(d.buffers[4], idx, 0.0F);
+		vo[2] = buf_at(d.buffers[5], idx, 1.0F);
+		vo[3] = buf_at(d.buffers[9], idx, 0.0F); vo[4] = buf_at(d.buffers[10], idx, 0.0F); }
+	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+		const float* u = d.uniforms;
+		float tex[4][4];
+		sample_quad(d.tu0, vary[3], vary[4], tex);
+		for (int l = 0; l < 4; ++l) {
+			float nx = vary[0][l], ny = vary[1][l], nz = vary[2][l];
+			normalize3(nx, ny, nz);
+			const float ndl = sse_max((nx*u[0] + ny*u[1]) + nz*u[2], 0.0F);
+			const float k = u[4] + u[5] * ndl;
+			color[0][l] = tex[0][l] * k; color[1][l] = tex[1][l] * k; color[2][l] = tex[2][l] * k; }}};
+
+// ManyBlinnPhong (this project): instanced, textured, one point light.
+// uniforms: [0..2] eye, [4..6] light position, [8] ka, [9] kd, [10] ks
+struct ProgManyBlinnPhong {
+	static constexpr int kId = 81, kVary = 8;
+	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		const float* imat = d.buffers[15] + 16 * iid;
+		float m[16];
+		for (int k = 0; k < 16; ++k) m[k] = imat[k];
+		float p1[4]; xform4(m, p, p1);
+		xform4(d.vpm, p1, pos);
+		const float n[3] = { buf_at(d.buffers[3], idx, 0.0F), buf_at(d.buffers[4], idx, 0.0F), buf_at(d.buffers[5], idx, 1.0F) };
+		float wn[3]; xform3w0(m, n, wn);
+		vo[0] = p1[0]; vo[1] = p1[1]; vo[2] = p1[2];
+		vo[3] = wn[0]; vo[4] = wn[1]; vo[5] = wn[2];
+		vo[6] = buf_at(d.buffers[9], idx, 0.0F); vo[7] = buf_at(d.buffers[10], idx, 0.0F); }
+	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+		const float* u = d.uniforms;
+		float tex[4][4];
+		sample_quad(d.tu0, vary[6], vary[7], tex);
+		for (int l = 0; l < 4; ++l) {
+			const float px = vary[0][l], py = vary[1][l], pz = vary[2][l];
+			float nx = vary[3][l], ny = vary[4][l], nz = vary[5][l];
+			normalize3(nx, ny, nz);
+			float vx = u[0] - px, vy = u[1] - py, vz = u[2] - pz;
+			normalize3(vx, vy, vz);
+			float lx = u[4] - px, ly = u[5] - py, lz = u[6] - pz;
+			normalize3(lx, ly, lz);
+			float hx = lx + vx, hy = ly + vy, hz = lz + vz;
+			normalize3(hx, hy, hz);
+			const float ndl = sse_max((nx*lx + ny*ly) + nz*lz, 0.0F);
+			const float ndh = sse_max((nx*hx + ny*hy) + nz*hz, 0.0F);
+			float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;
+			const float kdiff = u[8] + u[9] * ndl;
+			const float kspec = u[10] * s;
+			color[0][l] = tex[0][l] * kdiff + kspec;
+			color[1][l] = tex[1][l] * kdiff + kspec;
+			color[2][l] = tex[2][l] * kdiff + kspec; }}};
+
+__host__ __device__ inline int program_nvary(int pid) {
+	switch (pid) {
+	case ProgAmy::kId: return ProgAmy::kVary;
+	case ProgMany::kId: return ProgMany::kVary;
+	case ProgLambertTex::kId: return ProgLambertTex::kVary;
+	case ProgManyBlinnPhong::kId: return ProgManyBlinnPhong::kVary;
+	default: return -1; }}
+
+__host__ __device__ inline bool program_instanced(int pid) {
+	return pid == ProgMany::kId || pid == ProgManyBlinnPhong::kId; }
+
+}  // namespace rgl

@@ -1,0 +1,660 @@
+// rgl_hip.hip — host side of the gfx950 rgl pipeline: the rglv::GL command
+// recorder, frame planning (validation, draw descriptors, capacities),
+// kernel launches and the C ABI declared in include/rgl_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/rgl_hip.h"
+#include "rgl_kernels.h"
+
+using namespace rgl;
+
+namespace {
+
+// GL constants (rglv_gl.hxx:20-62)
+constexpr int GL_CULL_FACE = 1, GL_SCISSOR_TEST = 2, GL_BLEND = 3, GL_DEPTH_TEST = 4;
+constexpr int GL_BACK = 2, GL_FRONT = 1;
+constexpr int GL_LESS = 0, GL_LEQUAL = 1;
+constexpr int GL_COLOR_BUFFER_BIT = 1, GL_DEPTH_BUFFER_BIT = 2, GL_STENCIL_BUFFER_BIT = 4;
+constexpr int GL_DEPTH_ATTACHMENT = 0, GL_COLOR_ATTACHMENT0 = 2;
+constexpr int RB_COLOR_DEPTH = 0, RB_RGBF32 = 1, RB_RGBAF32 = 2, RB_F32 = 3;
+constexpr int UNIFORM_BUFFER_SIZE = 32;
+
+struct TexState { const float* ptr; int width, height, stride, filter; };
+
+// GLState (rglv_gl.hxx:80-179), with buffer lengths for bounds checks
+struct GLState {
+	float clearColor[4];
+	float clearDepth;
+	bool cullingEnabled; int cullFace;
+	bool scissorEnabled; int scissorOrigin[2], scissorSize[2];
+	int viewportOrigin[2]; bool hasViewportSize; int viewportSize[2];
+	bool blendingEnabled, colorWriteMask, depthWriteMask, depthTestEnabled;
+	int depthFunc;
+	int programId, uniformsOfs;
+	int color0AttachmentType, depthAttachmentType;
+	const float* buffers[16];
+	int bufferCounts[16];
+	TexState tus[2];
+	float viewMatrix[16], normalMatrix[16], projectionMatrix[16];
+
+	void reset() {
+		std::memset(this, 0, sizeof(*this));
+		clearColor[3] = 1.0F; clearDepth = 1.0F;
+		cullFace = GL_BACK;
+		colorWriteMask = true; depthWriteMask = true; depthTestEnabled = true;
+		depthFunc = GL_LESS;
+		uniformsOfs = -1;
+		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }}
+
+	uint32_t FragmentStateKey() const {  // rglv_gl.hxx:136-146
+		uint32_t key = static_cast<uint32_t>(programId) << 24;
+		key |= static_cast<uint32_t>(scissorEnabled);
+		key |= static_cast<uint32_t>(depthTestEnabled) << 1;
+		if (depthTestEnabled) key |= static_cast<uint32_t>(depthFunc) << 2;
+		key |= static_cast<uint32_t>(blendingEnabled) << 4;
+		key |= static_cast<uint32_t>(depthWriteMask) << 5;
+		key |= static_cast<uint32_t>(colorWriteMask) << 6;
+		key |= static_cast<uint32_t>(color0AttachmentType) << 7;
+		key |= static_cast<uint32_t>(depthAttachmentType) << 9;
+		return key; }};
+
+struct Cmd {
+	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
+	            STORE_TC, STORE_DEPTH, STORE_PRIM } kind;
+	int a, b;
+	const void* ptr; };
+
+// mat4 product in the reference's order (rmlm_mat4.hxx:196-207)
+void mat4_mul(const float* l, const float* r, float* out) {
+	for (int row = 0; row < 4; ++row)
+		for (int col = 0; col < 4; ++col) {
+			float ax = l[0*4 + row] * r[col*4 + 0];
+			ax += l[1*4 + row] * r[col*4 + 1];
+			ax += l[2*4 + row] * r[col*4 + 2];
+			ax += l[3*4 + row] * r[col*4 + 3];
+			out[col*4 + row] = ax; }}
+
+int ilog2(unsigned x) { int p = 0; while (x != 0U) { x >>= 1; ++p; } return p - 1; }
+
+template <class T>
+struct DevArray {
+	T* p = nullptr;
+	size_t cap = 0;
+	hipError_t ensure(size_t n) {
+		if (n <= cap) return hipSuccess;
+		if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; cap = 0; }
+		size_t want = std::max<size_t>(n + n / 4, 256);
+		hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+		if (e == hipSuccess) cap = want;
+		return e; }
+	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
+
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_raster" };
+constexpr int kNumTimed = 6;
+
+}  // namespace
+
+struct rgl_gpu {
+	int device = 0;
+	std::string guid;
+	std::string err;
+
+	// recorder (rglv::GL)
+	GLState cs{};
+	bool dirty = true;
+	std::vector<float> ubuf;
+	std::deque<GLState> states;
+	std::vector<Cmd> commands;
+	std::vector<uint32_t> installed;
+
+	// render target (GPU::Reset / Retile, rglv_gpu.cxx:50-87)
+	int W = 0, H = 0, tileW = 64, tileH = 64, tilesX = 0, tilesY = 0;
+
+	// device buffers
+	DevArray<float4> vpos; DevArray<uint8_t> vflags; DevArray<float> vvary;
+	DevArray<uint2> trect; DevArray<uint4> tvert;
+	DevArray<uint32_t> hist, bsum, list;
+	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
+	DevArray<uint8_t> fcount; DevArray<uint2> frect;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist;
+	DevArray<DrawDesc> dDraws;
+	Counters* dCtr = nullptr;
+	Counters* hCtr = nullptr;       // pinned
+	DrawDesc* hDraws = nullptr;     // pinned, kMaxDraws
+	uint32_t listCap = 0, clipCap = 0, clistCap = 0;
+	std::vector<uint8_t> lastTable;
+
+	// last frame
+	bool pending = false;
+	hipStream_t stream = nullptr;
+	hipEvent_t done = nullptr;
+	rgl_stats stats{};
+	long long submitted = 0;
+	int lastCommands = 0;
+
+	// timing
+	bool timing = false;
+	hipEvent_t ev[kNumTimed + 1] = {};
+	double acc[kNumTimed] = {};
+	int accRuns = 0;
+	bool evPending = false;
+
+	~rgl_gpu() {
+		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist }) a->release();
+		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
+		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
+		if (dCtr) (void)hipFree(dCtr);
+		if (hCtr) (void)hipHostFree(hCtr);
+		if (hDraws) (void)hipHostFree(hDraws);
+		if (done) (void)hipEventDestroy(done);
+		for (auto& e : ev) if (e) (void)hipEventDestroy(e); }
+
+	int fail(int code, const std::string& msg) { err = msg; return code; }
+	int hipfail(hipError_t e, const char* what) {
+		err = std::string(what) + ": " + hipGetErrorString(e);
+		return RGL_E_HIP; }
+
+	void MaybeUpdateState() {  // GL::MaybeUpdateState (rglv_gl.cxx:100-106)
+		if (!dirty) return;
+		states.push_back(cs);
+		commands.push_back({ Cmd::STATE, static_cast<int>(states.size()) - 1, 0, nullptr });
+		dirty = false; }
+
+	bool Installed(uint32_t key) const { return std::find(installed.begin(), installed.end(), key) != installed.end(); }
+
+	void ResetGL() { dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); }
+
+	// ---- frame planning ----
+	int Plan(FrameDesc& f, std::vector<DrawDesc>& draws) {
+		std::memset(&f, 0, sizeof(f));
+		f.W = W; f.H = H; f.tileW = tileW; f.tileH = tileH; f.tilesX = tilesX; f.tilesY = tilesY;
+		f.nTiles = tilesX * tilesY;
+		f.depthFunc = -1;
+		f.drawEnabled = 1;
+		const GLState* st = nullptr;
+		bool cleared = false, stored = false;
+		long long triBase = 0, vBase = 0;
+		for (const Cmd& c : commands) {
+			switch (c.kind) {
+			case Cmd::STATE: st = &states[c.a]; break;
+			case Cmd::CLEAR: {
+				if (!draws.empty() || stored) return fail(RGL_E_UNSUPPORTED, "CLEAR after a draw or store is not supported");
+				if (c.a & GL_STENCIL_BUFFER_BIT) return fail(RGL_E_UNSUPPORTED, "CLEAR on STENCIL not implemented");
+				if (st->color0AttachmentType == RB_COLOR_DEPTH && c.a != (GL_COLOR_BUFFER_BIT | GL_DEPTH_BUFFER_BIT))
+					return fail(RGL_E_STATE, "must clear color and depth when using RB_COLOR_DEPTH");
+				f.doClearColor = (c.a & GL_COLOR_BUFFER_BIT) != 0;
+				f.doClearDepth = (c.a & GL_DEPTH_BUFFER_BIT) != 0;
+				for (int k = 0; k < 3; ++k) f.clearColor[k] = st->clearColor[k];
+				f.clearDepth = st->clearDepth;
+				cleared = true;
+				break; }
+			case Cmd::STORE_TC:
+				if (!Installed((static_cast<uint32_t>(st->programId) << 24) | 0x40000000u))
+					return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_STORE_COLOR_FULL_LINEAR_TC");
+				if (st->programId != 1 && st->programId != 2) return fail(RGL_E_UNSUPPORTED, "post program not implemented");
+				f.outColor = static_cast<uint32_t*>(const_cast<void*>(c.ptr));
+				f.gamma = c.a; f.postProgram = st->programId;
+				f.exposure = (st->programId == 2 && st->uniformsOfs >= 0) ? ubuf[st->uniformsOfs] : 1.0F;
+				stored = true;
+				break;
+			case Cmd::STORE_DEPTH:
+				if (st->depthAttachmentType != RB_F32 && st->depthAttachmentType != RB_COLOR_DEPTH)
+					return fail(RGL_E_UNSUPPORTED, "STORE_DEPTH_FULL_LINEAR_FP not implemented for attachment type");
+				f.outDepth = static_cast<float*>(const_cast<void*>(c.ptr)); stored = true; break;
+			case Cmd::STORE_PRIM:
+				f.outPrim = static_cast<uint32_t*>(const_cast<void*>(c.ptr)); stored = true; break;
+			default: {
+				if (stored) return fail(RGL_E_UNSUPPORTED, "draw after a store is not supported");
+				if (!st) return fail(RGL_E_STATE, "draw without state");
+				if (static_cast<int>(draws.size()) >= kMaxDraws) return fail(RGL_E_CAPACITY, "too many draws in one frame");
+				const int r = PlanDraw(*st, c, triBase, vBase, f, draws);
+				if (r != 0) return r;
+				break; }}}
+		if (!cleared && !draws.empty()) return fail(RGL_E_UNSUPPORTED, "a frame must begin with Clear(COLOR|DEPTH)");
+		if (!f.doClearColor || !f.doClearDepth) {
+			if (!draws.empty() || f.outColor || f.outDepth) return fail(RGL_E_UNSUPPORTED, "partial clears are not supported"); }
+		if (f.depthFunc < 0) f.depthFunc = GL_LESS;
+		if (std::isnan(f.clearDepth)) f.drawEnabled = 0;
+		f.nDraws = static_cast<int>(draws.size());
+		f.totalTris = triBase;
+		f.units = static_cast<int>((triBase + kUnitTris - 1) / kUnitTris);
+		if (triBase * kSlotsPerTri + 1 >= 0xffffffffLL) return fail(RGL_E_CAPACITY, "too many triangles in one frame");
+		return 0; }
+
+	int PlanDraw(const GLState& s, const Cmd& c, long long& triBase, long long& vBase, FrameDesc& f, std::vector<DrawDesc>& draws) {
+		const uint32_t vkey = (static_cast<uint32_t>(s.programId) << 24) | 0x80000000u;
+		if (!Installed(vkey)) return fail(RGL_E_DISPATCH, "BIN: no dispatch entry for program");
+		if (!Installed(s.FragmentStateKey())) return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_DRAW_INLINE");
+		const int nv = program_nvary(s.programId);
+		if (nv < 0) return fail(RGL_E_UNSUPPORTED, "program not implemented on the device");
+		if (!s.depthTestEnabled || !s.depthWriteMask || !s.colorWriteMask || s.blendingEnabled || s.scissorEnabled)
+			return fail(RGL_E_UNSUPPORTED, "state needs the in-order raster path (blend, scissor, no depth test or write)");
+		if (s.depthFunc != GL_LESS && s.depthFunc != GL_LEQUAL) return fail(RGL_E_UNSUPPORTED, "depth func not implemented");
+		if (f.depthFunc >= 0 && f.depthFunc != s.depthFunc) return fail(RGL_E_UNSUPPORTED, "mixed depth funcs in one frame");
+		f.depthFunc = s.depthFunc;
+		const bool elements = c.kind == Cmd::DRAW_ELEMENTS || c.kind == Cmd::DRAW_ELEMENTS_INSTANCED;
+		const bool inst = c.kind == Cmd::DRAW_ELEMENTS_INSTANCED || c.kind == Cmd::DRAW_ARRAYS_INSTANCED;
+		const int count = c.a;
+		const int instances = inst ? c.b : 1;
+		if (count < 0 || instances < 0) return fail(RGL_E_ARG, "negative count");
+		if (instances > 65536) return fail(RGL_E_CAPACITY, "instance id must be < 65536");
+		const int tris = count / 3;
+		if (tris == 0 || instances == 0) return 0;
+		DrawDesc d;
+		std::memset(&d, 0, sizeof(d));
+		d.program = s.programId;
+		d.instances = instances;
+		d.tris = tris;
+		d.elements = elements ? 1 : 0;
+		d.indices = static_cast<const uint16_t*>(c.ptr);
+		if (elements && d.indices == nullptr) return fail(RGL_E_ARG, "null index buffer");
+		int verts;
+		if (elements) {
+			if (!s.buffers[0] || s.bufferCounts[0] <= 0) return fail(RGL_E_ARG, "indexed draw needs a position buffer with a count");
+			verts = s.bufferCounts[0]; }
+		else {
+			verts = count; }
+		if (verts > 32768) return fail(RGL_E_CAPACITY, "vertex index must be <= 32767");
+		for (int k = 0; k < 15; ++k)
+			if (s.buffers[k] && s.bufferCounts[k] < (elements ? verts : count))
+				return fail(RGL_E_ARG, "vertex buffer shorter than the position buffer");
+		if (program_instanced(s.programId)) {
+			if (!s.buffers[15] || s.bufferCounts[15] < 16 * instances) return fail(RGL_E_ARG, "instance matrix buffer too short"); }
+		else if (instances != 1) {
+			return fail(RGL_E_UNSUPPORTED, "instanced draw with a non-instanced program"); }
+		d.verts = verts;
+		d.nvary = nv;
+		d.triBase = triBase;
+		d.vBase = vBase;
+		d.primBase = static_cast<uint32_t>(triBase);
+		d.slotBase = static_cast<uint32_t>(1 + kSlotsPerTri * triBase);
+		for (int k = 0; k < 16; ++k) d.buffers[k] = s.buffers[k];
+		mat4_mul(s.projectionMatrix, s.viewMatrix, d.vpm);
+		// GPU::DSDO (rglv_gpu.hxx:265-270)
+		const int vx = s.hasViewportSize ? s.viewportSize[0] : W;
+		const int vy = s.hasViewportSize ? s.viewportSize[1] : H;
+		d.DS[0] = static_cast<float>(vx / 2); d.DS[1] = static_cast<float>(-vy / 2);
+		d.DO[0] = static_cast<float>(vx / 2 + s.viewportOrigin[0]);
+		d.DO[1] = static_cast<float>(H - (vy / 2 + s.viewportOrigin[1]));
+		// CalcGuardBandFactor (rglv_view_frustum.hxx:257-260)
+		const int half = std::max(W, H) / 2;
+		d.gbFactor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half);
+		d.scissor[0] = 0; d.scissor[1] = 0; d.scissor[2] = W; d.scissor[3] = H;
+		d.keepBacks = !(s.cullingEnabled && s.cullFace == GL_BACK);
+		d.keepFronts = !(s.cullingEnabled && s.cullFace == GL_FRONT);
+		d.cullingEnabled = s.cullingEnabled; d.cullFace = s.cullFace;
+		if (s.uniformsOfs >= 0) std::memcpy(d.uniforms, ubuf.data() + s.uniformsOfs, sizeof(d.uniforms));
+		for (int u = 0; u < 2; ++u) {
+			SamplerDesc& sd = u == 0 ? d.tu0 : d.tu1;
+			const TexState& t = s.tus[u];
+			sd.buf = t.ptr; sd.kind = -1;
+			if (!t.ptr) continue;
+			const int power = ilog2(static_cast<unsigned>(t.width));
+			const bool isPow2 = (1 << power) == t.width && t.width == t.height && t.stride == t.width;
+			sd.width = t.width; sd.height = t.height; sd.stride = t.stride;
+			if (!isPow2) { sd.kind = 2; sd.texels = t.stride * t.height; continue; }
+			if (t.width < 4 || t.width > 1024) return fail(RGL_E_UNSUPPORTED, "can't make TextureUnit for pow2 size");
+			sd.power = power; sd.kind = t.filter ? 1 : 0; sd.texels = 2 * t.width * t.width; }
+		draws.push_back(d);
+		triBase += static_cast<long long>(tris) * instances;
+		vBase += static_cast<long long>(verts) * instances;
+		submitted += static_cast<long long>(tris) * instances;
+		return 0; }
+
+	int Ensure(const FrameDesc& f, long long vtot) {
+		hipError_t e;
+		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
+		const long long S = std::max<long long>(f.totalTris, 1);
+		if ((e = vpos.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vpos");
+		if ((e = vflags.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vflags");
+		if ((e = vvary.ensure((vtot + 1) * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc vvary");
+		if ((e = trect.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
+		if ((e = tvert.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
+		const long long nh = static_cast<long long>(f.nTiles) * std::max(f.units, 1) + 1;
+		if ((e = hist.ensure(nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
+		if ((e = bsum.ensure(nh / kScanChunk + 2)) != hipSuccess) return hipfail(e, "hipMalloc bsum");
+		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
+		if ((e = list.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc list");
+		if (clipCap < std::min<long long>(S, 4096)) clipCap = static_cast<uint32_t>(std::min<long long>(S, 4096));
+		if ((e = clipq.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc clipq");
+		if ((e = clipqOf.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc clipqOf");
+		if ((e = cvpos.ensure(static_cast<size_t>(clipCap) * kClipVerts)) != hipSuccess) return hipfail(e, "hipMalloc cvpos");
+		if ((e = cvvary.ensure(static_cast<size_t>(clipCap) * kClipVerts * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc cvvary");
+		if ((e = fcount.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc fcount");
+		if ((e = frect.ensure(static_cast<size_t>(clipCap) * kFanMax)) != hipSuccess) return hipfail(e, "hipMalloc frect");
+		if ((e = clipHist.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipHist");
+		if ((e = clipOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
+		if ((e = clipCursor.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
+		if (clistCap < 4096) clistCap = 65536;
+		if ((e = clist.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clist");
+		if (!dDraws.p) {
+			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
+			lastTable.clear(); }
+		return 0; }
+
+	template <class P>
+	void LaunchVertex(const DrawDesc& d) {
+		const long long n = static_cast<long long>(d.instances) * d.verts;
+		const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+		k_vertex<P><<<blocks, 256, 0, stream>>>(d, vpos.p, vflags.p, vvary.p); }
+
+	template <class P>
+	void LaunchClip(int di, const FrameDesc& f) {
+		k_clip<P><<<64, 64, 0, stream>>>(dDraws.p, di, f, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr); }
+
+	void Mark(int i) { if (timing) (void)hipEventRecord(ev[i], stream); }
+
+	int Run(hipStream_t s) {
+		err.clear();
+		// with timing on, each frame is waited for so its events can be read
+		if (pending && timing) { const int r = Wait(); if (r != 0) return r; }
+		FrameDesc f;
+		std::vector<DrawDesc> draws;
+		submitted = 0;
+		int r = Plan(f, draws);
+		if (r != 0) return r;
+		long long vtot = 0;
+		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
+		if ((r = Ensure(f, vtot)) != 0) return r;
+		stream = s;
+		hipError_t e;
+		if (!dCtr) {
+			if ((e = hipMalloc(reinterpret_cast<void**>(&dCtr), sizeof(Counters))) != hipSuccess) return hipfail(e, "hipMalloc counters");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
+		if (timing && !ev[0]) for (auto& x : ev) (void)hipEventCreate(&x);
+		// the draw table is uploaded only when it changed; the pinned staging
+		// copy is rewritten only after the frame reading it has completed
+		const size_t tableBytes = sizeof(DrawDesc) * draws.size();
+		if (tableBytes != lastTable.size() || (tableBytes && std::memcmp(lastTable.data(), draws.data(), tableBytes) != 0)) {
+			if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
+			std::memcpy(hDraws, draws.data(), tableBytes);
+			if (tableBytes && (e = hipMemcpyAsync(dDraws.p, hDraws, tableBytes, hipMemcpyHostToDevice, stream)) != hipSuccess)
+				return hipfail(e, "hipMemcpyAsync draws");
+			lastTable.assign(reinterpret_cast<const uint8_t*>(draws.data()), reinterpret_cast<const uint8_t*>(draws.data()) + tableBytes); }
+		k_zero_counters<<<8, 256, 0, stream>>>(dCtr, clipHist.p, f.nTiles);
+		Mark(0);
+		for (const auto& d : draws) {
+			switch (d.program) {
+			case ProgAmy::kId: LaunchVertex<ProgAmy>(d); break;
+			case ProgMany::kId: LaunchVertex<ProgMany>(d); break;
+			case ProgLambertTex::kId: LaunchVertex<ProgLambertTex>(d); break;
+			default: LaunchVertex<ProgManyBlinnPhong>(d); break; }}
+		Mark(1);
+		const long long nh = static_cast<long long>(f.nTiles) * f.units;
+		if (f.units > 0) {
+			k_setup_count<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, f, vpos.p, vflags.p, trect.p, tvert.p,
+				hist.p, clipq.p, clipqOf.p, clipCap, dCtr); }
+		Mark(2);
+		{
+			const int nb = static_cast<int>((nh + kScanChunk - 1) / kScanChunk);
+			if (nb > 0) {
+				k_scan_reduce<<<nb, kScanThreads, 0, stream>>>(hist.p, nh, bsum.p);
+				k_scan_bsums<<<1, 1024, 0, stream>>>(bsum.p, nb, &dCtr->pairs);
+				k_scan_down<<<nb, kScanThreads, 0, stream>>>(hist.p, nh, bsum.p, hist.p, &dCtr->pairs); }
+			else {
+				(void)hipMemsetAsync(hist.p, 0, sizeof(uint32_t) * (nh + 1), stream); }
+		}
+		Mark(3);
+		if (f.units > 0) k_scatter<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(f, trect.p, hist.p, list.p, listCap, dCtr);
+		Mark(4);
+		for (int di = 0; di < static_cast<int>(draws.size()); ++di) {
+			switch (draws[di].program) {
+			case ProgAmy::kId: LaunchClip<ProgAmy>(di, f); break;
+			case ProgMany::kId: LaunchClip<ProgMany>(di, f); break;
+			case ProgLambertTex::kId: LaunchClip<ProgLambertTex>(di, f); break;
+			default: LaunchClip<ProgManyBlinnPhong>(di, f); break; }}
+		k_clip_scan<<<1, 1024, 0, stream>>>(f, clipHist.p, clipOff.p, clipCursor.p, dCtr);
+		k_clip_scatter<<<64, 256, 0, stream>>>(f, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
+		Mark(5);
+		RasterArgs ra;
+		ra.draws = dDraws.p; ra.off = hist.p; ra.list = list.p; ra.listCap = listCap;
+		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
+		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
+		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
+		if (f.nTiles > 0) k_raster<<<f.nTiles, kRasterThreads, 0, stream>>>(f, ra);
+		Mark(6);
+		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
+		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
+		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
+		stats = rgl_stats{};
+		stats.tiles = f.nTiles;
+		stats.commands = static_cast<int64_t>(commands.size());
+		stats.primitives_submitted = submitted;
+		pending = true;
+		evPending = timing;
+		return 0; }
+
+	int Wait() {
+		if (!pending) return 0;
+		pending = false;
+		const hipError_t e = hipEventSynchronize(done);
+		if (e != hipSuccess) return hipfail(e, "hipEventSynchronize");
+		stats.primitives_clipped = hCtr->clipCount;
+		stats.primitives_binned = hCtr->binned;
+		stats.tile_pairs = hCtr->pairs;
+		stats.clipped_tile_pairs = hCtr->clipPairs;
+		stats.fragments = static_cast<int64_t>(hCtr->fragments);
+		stats.overflow = hCtr->overflow;
+		if (evPending) {
+			for (int i = 0; i < kNumTimed; ++i) {
+				float ms = 0.0F;
+				if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) acc[i] += ms; }
+			++accRuns;
+			evPending = false; }
+		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
+		if (hCtr->overflow & 16u) return fail(RGL_E_STATE, "tile list held an invalid entry");
+		if (hCtr->overflow) {
+			// grow and report; the caller runs the frame again
+			if (hCtr->overflow & 1u) listCap = static_cast<uint32_t>(std::min<long long>(2LL * hCtr->pairs + 4096, 0x7fffffffLL));
+			if (hCtr->overflow & 2u) clipCap = static_cast<uint32_t>(std::min<long long>(2LL * hCtr->clipCount + 1024, 0x7fffffffLL));
+			if (hCtr->overflow & 4u) clistCap = static_cast<uint32_t>(std::min<long long>(2LL * hCtr->clipPairs + 4096, 0x7fffffffLL));
+			return fail(RGL_E_CAPACITY, "a per-frame buffer overflowed; capacities were grown, run the frame again"); }
+		return 0; }
+};
+
+namespace {
+int check(rgl_gpu* g) { return g ? 0 : RGL_E_ARG; }
+}
+
+extern "C" {
+
+int rgl_gpu_create(int device_ordinal, int concurrency, const char* guid, rgl_gpu** out) {
+	if (!out) return RGL_E_ARG;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) return RGL_E_HIP;
+	if (hipSetDevice(device_ordinal) != hipSuccess) return RGL_E_HIP;
+	auto* g = new rgl_gpu();
+	g->device = device_ordinal;
+	g->guid = guid ? guid : "unnamed";
+	(void)concurrency;
+	g->cs.reset();
+	*out = g;
+	return 0; }
+
+int rgl_gpu_destroy(rgl_gpu* g) {
+	if (!g) return RGL_E_ARG;
+	if (g->pending) (void)g->Wait();
+	delete g;
+	return 0; }
+
+int rgl_gpu_install(rgl_gpu* g, int program_id, uint32_t state_key, int kind) {
+	if (check(g)) return RGL_E_ARG;
+	if (kind < 0 || kind > 2) return g->fail(RGL_E_ARG, "bad install kind");
+	uint32_t key = (static_cast<uint32_t>(program_id) << 24) | state_key;
+	if (kind == RGL_BIN_PROGRAM) key |= 0x80000000u;
+	if (kind == RGL_BLT_PROGRAM) key |= 0x40000000u;
+	if (std::find(g->installed.begin(), g->installed.end(), key) == g->installed.end()) g->installed.push_back(key);
+	return 0; }
+
+int rgl_gpu_reset(rgl_gpu* g, int width, int height, int tbx, int tby) {
+	if (check(g)) return RGL_E_ARG;
+	if (width <= 0 || height <= 0 || tbx <= 0 || tby <= 0) return g->fail(RGL_E_ARG, "bad size");
+	if (tbx * 8 > kMaxTileDim || tby * 8 > kMaxTileDim) return g->fail(RGL_E_UNSUPPORTED, "tiles larger than 64x64 pixels are not supported");
+	if (width > 65535 * tbx * 8 || height > 65535 * tby * 8) return g->fail(RGL_E_ARG, "target too large");
+	if (g->pending) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
+	g->W = width; g->H = height; g->tileW = tbx * 8; g->tileH = tby * 8;
+	g->tilesX = (width + g->tileW - 1) / g->tileW; g->tilesY = (height + g->tileH - 1) / g->tileH;
+	g->ResetGL();
+	return 0; }
+
+int rgl_gpu_run(rgl_gpu* g, void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (g->W <= 0) return g->fail(RGL_E_STATE, "Run before Reset");
+	return g->Run(static_cast<hipStream_t>(hip_stream)); }
+
+int rgl_gpu_wait(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; return g->Wait(); }
+
+int rgl_gpu_stats(rgl_gpu* g, rgl_stats* out) {
+	if (check(g) || !out) return RGL_E_ARG;
+	*out = g->stats;
+	return 0; }
+
+int rgl_gpu_set_timing(rgl_gpu* g, int enable) {
+	if (check(g)) return RGL_E_ARG;
+	g->timing = enable != 0;
+	for (double& a : g->acc) a = 0.0;
+	g->accRuns = 0;
+	return 0; }
+
+int rgl_gpu_kernel_times(rgl_gpu* g, double* ms_out, int max_kernels, char* names_out, int names_len) {
+	if (check(g)) return RGL_E_ARG;
+	std::string names;
+	for (int i = 0; i < kNumTimed; ++i) {
+		if (i < max_kernels && ms_out) ms_out[i] = g->accRuns ? g->acc[i] / g->accRuns : 0.0;
+		names += (i ? "," : ""); names += kKernelNames[i]; }
+	if (names_out && names_len > 0) { std::strncpy(names_out, names.c_str(), names_len - 1); names_out[names_len - 1] = 0; }
+	for (double& a : g->acc) a = 0.0;
+	g->accRuns = 0;
+	return kNumTimed; }
+
+const char* rgl_gpu_last_error(rgl_gpu* g) { return g ? g->err.c_str() : "null gpu"; }
+
+#define RGL_STATE_SETTER(body) do { if (check(g)) return RGL_E_ARG; g->dirty = true; body; return 0; } while (0)
+
+int rgl_gl_enable(rgl_gpu* g, int v) {
+	if (check(g)) return RGL_E_ARG;
+	g->dirty = true;
+	if (v == GL_CULL_FACE) g->cs.cullingEnabled = true;
+	else if (v == GL_SCISSOR_TEST) g->cs.scissorEnabled = true;
+	else if (v == GL_BLEND) g->cs.blendingEnabled = true;
+	else if (v == GL_DEPTH_TEST) g->cs.depthTestEnabled = true;
+	else return g->fail(RGL_E_ARG, "unknown glEnable value");
+	return 0; }
+int rgl_gl_disable(rgl_gpu* g, int v) {
+	if (check(g)) return RGL_E_ARG;
+	g->dirty = true;
+	if (v == GL_CULL_FACE) g->cs.cullingEnabled = false;
+	else if (v == GL_SCISSOR_TEST) g->cs.scissorEnabled = false;
+	else if (v == GL_BLEND) g->cs.blendingEnabled = false;
+	else if (v == GL_DEPTH_TEST) g->cs.depthTestEnabled = false;
+	else return g->fail(RGL_E_ARG, "unknown glEnable value");
+	return 0; }
+int rgl_gl_depth_func(rgl_gpu* g, int v) { RGL_STATE_SETTER(g->cs.depthFunc = v); }
+int rgl_gl_depth_write_mask(rgl_gpu* g, int v) { RGL_STATE_SETTER(g->cs.depthWriteMask = v != 0); }
+int rgl_gl_color_write_mask(rgl_gpu* g, int v) { RGL_STATE_SETTER(g->cs.colorWriteMask = v != 0); }
+int rgl_gl_cull_face(rgl_gpu* g, int v) { RGL_STATE_SETTER(g->cs.cullFace = v); }
+int rgl_gl_scissor(rgl_gpu* g, int x, int y, int w, int h) {
+	RGL_STATE_SETTER(g->cs.scissorOrigin[0] = x; g->cs.scissorOrigin[1] = y; g->cs.scissorSize[0] = w; g->cs.scissorSize[1] = h); }
+int rgl_gl_viewport(rgl_gpu* g, int x, int y, int w, int h) {
+	RGL_STATE_SETTER(g->cs.viewportOrigin[0] = x; g->cs.viewportOrigin[1] = y; g->cs.hasViewportSize = true;
+	                 g->cs.viewportSize[0] = w; g->cs.viewportSize[1] = h); }
+int rgl_gl_use_program(rgl_gpu* g, int v) { RGL_STATE_SETTER(g->cs.programId = v); }
+int rgl_gl_use_uniforms(rgl_gpu* g, int ofs) { RGL_STATE_SETTER(g->cs.uniformsOfs = ofs); }
+int rgl_gl_alloc_uniform_buffer(rgl_gpu* g, const float* values, int n) {
+	if (check(g)) return RGL_E_ARG;
+	const int idx = static_cast<int>(g->ubuf.size());
+	g->ubuf.resize(g->ubuf.size() + UNIFORM_BUFFER_SIZE, 0.0F);
+	for (int i = 0; values && i < n && i < UNIFORM_BUFFER_SIZE; ++i) g->ubuf[idx + i] = values[i];
+	return idx; }
+float* rgl_gl_uniform_buffer_addr(rgl_gpu* g, int ofs) {
+	if (!g || ofs < 0 || ofs >= static_cast<int>(g->ubuf.size())) return nullptr;
+	return g->ubuf.data() + ofs; }
+int rgl_gl_renderbuffer_type(rgl_gpu* g, int attachment, int type) {
+	if (check(g)) return RGL_E_ARG;
+	g->dirty = true;
+	if (attachment == GL_DEPTH_ATTACHMENT) g->cs.depthAttachmentType = type;
+	else if (attachment == GL_COLOR_ATTACHMENT0) g->cs.color0AttachmentType = type;
+	return 0; }
+int rgl_gl_bind_texture(rgl_gpu* g, int unit, const float* ptr, int w, int h, int stride, int mode) {
+	if (check(g)) return RGL_E_ARG;
+	if (unit < 0 || unit > 1) return g->fail(RGL_E_ARG, "texture unit must be 0 or 1");
+	if (mode != 0 && mode != 1) return g->fail(RGL_E_ARG, "texture filter must be GL_NEAREST/LINEAR_MIPMAP_NEAREST");
+	g->dirty = true;
+	g->cs.tus[unit] = { ptr, w, h, stride, mode };
+	return 0; }
+int rgl_gl_use_buffer(rgl_gpu* g, int slot, const float* ptr, int count) {
+	if (check(g)) return RGL_E_ARG;
+	if (slot < 0 || slot > 15) return g->fail(RGL_E_ARG, "buffer slot must be 0..15");
+	g->dirty = true;
+	g->cs.buffers[slot] = ptr;
+	g->cs.bufferCounts[slot] = ptr ? count : 0;
+	return 0; }
+int rgl_gl_clear_color(rgl_gpu* g, float r, float gg, float b) {
+	RGL_STATE_SETTER(g->cs.clearColor[0] = r; g->cs.clearColor[1] = gg; g->cs.clearColor[2] = b; g->cs.clearColor[3] = 1.0F); }
+int rgl_gl_clear_depth(rgl_gpu* g, float v) { RGL_STATE_SETTER(g->cs.clearDepth = v); }
+int rgl_gl_view_matrix(rgl_gpu* g, const float* m) { if (!m) return RGL_E_ARG; RGL_STATE_SETTER(std::memcpy(g->cs.viewMatrix, m, 64)); }
+int rgl_gl_projection_matrix(rgl_gpu* g, const float* m) { if (!m) return RGL_E_ARG; RGL_STATE_SETTER(std::memcpy(g->cs.projectionMatrix, m, 64)); }
+int rgl_gl_normal_matrix(rgl_gpu* g, const float* m) { if (!m) return RGL_E_ARG; RGL_STATE_SETTER(std::memcpy(g->cs.normalMatrix, m, 64)); }
+
+int rgl_gl_draw_elements(rgl_gpu* g, int mode, int count, int type, const uint16_t* idx, int hint) {
+	if (check(g)) return RGL_E_ARG;
+	if (mode != 0 || type != 1) return g->fail(RGL_E_UNSUPPORTED, "only GL_TRIANGLES with GL_UNSIGNED_SHORT indices");
+	(void)hint;
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ELEMENTS, count, 1, idx }); return 0; }
+int rgl_gl_draw_arrays(rgl_gpu* g, int mode, int start, int count) {
+	if (check(g)) return RGL_E_ARG;
+	if (mode != 0 || start != 0) return g->fail(RGL_E_UNSUPPORTED, "only GL_TRIANGLES from 0");
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ARRAYS, count, 1, nullptr }); return 0; }
+int rgl_gl_draw_elements_instanced(rgl_gpu* g, int mode, int count, int type, const uint16_t* idx, int n) {
+	if (check(g)) return RGL_E_ARG;
+	if (mode != 0 || type != 1) return g->fail(RGL_E_UNSUPPORTED, "only GL_TRIANGLES with GL_UNSIGNED_SHORT indices");
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ELEMENTS_INSTANCED, count, n, idx }); return 0; }
+int rgl_gl_draw_arrays_instanced(rgl_gpu* g, int mode, int start, int count, int n) {
+	if (check(g)) return RGL_E_ARG;
+	if (mode != 0 || start != 0) return g->fail(RGL_E_UNSUPPORTED, "only GL_TRIANGLES from 0");
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::DRAW_ARRAYS_INSTANCED, count, n, nullptr }); return 0; }
+int rgl_gl_clear(rgl_gpu* g, int bits) {
+	if (check(g)) return RGL_E_ARG;
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::CLEAR, bits, 0, nullptr }); return 0; }
+int rgl_gl_store_color_tc(rgl_gpu* g, uint32_t* dst, int gamma) {
+	if (check(g)) return RGL_E_ARG;
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_TC, gamma ? 1 : 0, 0, dst }); return 0; }
+int rgl_gl_store_depth(rgl_gpu* g, float* dst) {
+	if (check(g)) return RGL_E_ARG;
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_DEPTH, 0, 0, dst }); return 0; }
+int rgl_gl_store_primitive_id(rgl_gpu* g, uint32_t* dst) {
+	if (check(g)) return RGL_E_ARG;
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_PRIM, 0, 0, dst }); return 0; }
+int rgl_gl_reset(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; g->ResetGL(); return 0; }
+
+// Texture::maybe_make_mipmap (rglr_texture.cxx:33-81)
+int rgl_texture_make_mipmap(float* buf, int dim) {
+	if (!buf || dim < 1 || (dim & (dim - 1)) != 0 || dim > 4096) return RGL_E_ARG;
+	const int stride = dim;
+	int srcSize = dim, src = 0, dst = dim * dim;
+	while (srcSize > 1) {
+		for (int sy = 0; sy < srcSize; sy += 2) {
+			int dstRow = dst;
+			for (int sx = 0; sx < srcSize; sx += 2) {
+				const int r1 = src + sy * stride + sx, r2 = src + (sy + 1) * stride + sx;
+				for (int c = 0; c < 4; ++c) {
+					const float sum = ((buf[r1 * 4 + c] + buf[(r1 + 1) * 4 + c]) + buf[r2 * 4 + c]) + buf[(r2 + 1) * 4 + c];
+					buf[dstRow * 4 + c] = sum / 4.0F; }
+				++dstRow; }
+			dst += stride; }
+		src += srcSize * stride;
+		srcSize >>= 1; }
+	return 0; }
+
+}  // extern "C"

@@ -1,0 +1,243 @@
+"""Python binding of the gfx950 rgl pipeline (librgl_hip.so) — the
+rglv::GPU / rglv::GL surface over the C ABI in include/rgl_hip.h.
+
+Device memory and streams come from PyTorch (plumbing only); every
+pipeline stage runs in the HIP library.  There is no CPU fallback: loading
+fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "librgl_hip.so")
+
+RGL_E_CAPACITY = -5
+
+
+class RglError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "tiles", "commands", "primitives_submitted", "primitives_clipped", "primitives_binned",
+        "tile_pairs", "clipped_tile_pairs", "fragments", "overflow")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RglError(f"{LIB_PATH} is missing: build it with `python -m rsr_amd.build` "
+                           "(the HIP pipeline has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, f, u32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_uint32
+        L.rgl_gpu_create.argtypes = [i, i, ctypes.c_char_p, ctypes.POINTER(vp)]
+        L.rgl_gpu_destroy.argtypes = [vp]
+        L.rgl_gpu_install.argtypes = [vp, i, u32, i]
+        L.rgl_gpu_reset.argtypes = [vp, i, i, i, i]
+        L.rgl_gpu_run.argtypes = [vp, vp]
+        L.rgl_gpu_wait.argtypes = [vp]
+        L.rgl_gpu_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+        L.rgl_gpu_set_timing.argtypes = [vp, i]
+        L.rgl_gpu_kernel_times.argtypes = [vp, ctypes.POINTER(ctypes.c_double), i, ctypes.c_char_p, i]
+        L.rgl_gpu_last_error.argtypes = [vp]
+        L.rgl_gpu_last_error.restype = ctypes.c_char_p
+        for n in ("enable", "disable", "depth_func", "depth_write_mask", "color_write_mask",
+                  "cull_face", "use_program", "use_uniforms", "clear"):
+            getattr(L, "rgl_gl_" + n).argtypes = [vp, i]
+        L.rgl_gl_scissor.argtypes = [vp, i, i, i, i]
+        L.rgl_gl_viewport.argtypes = [vp, i, i, i, i]
+        L.rgl_gl_alloc_uniform_buffer.argtypes = [vp, vp, i]
+        L.rgl_gl_uniform_buffer_addr.argtypes = [vp, i]
+        L.rgl_gl_uniform_buffer_addr.restype = vp
+        L.rgl_gl_renderbuffer_type.argtypes = [vp, i, i]
+        L.rgl_gl_bind_texture.argtypes = [vp, i, vp, i, i, i, i]
+        L.rgl_gl_use_buffer.argtypes = [vp, i, vp, i]
+        L.rgl_gl_clear_color.argtypes = [vp, f, f, f]
+        L.rgl_gl_clear_depth.argtypes = [vp, f]
+        for n in ("view_matrix", "projection_matrix", "normal_matrix"):
+            getattr(L, "rgl_gl_" + n).argtypes = [vp, vp]
+        L.rgl_gl_draw_elements.argtypes = [vp, i, i, i, vp, i]
+        L.rgl_gl_draw_arrays.argtypes = [vp, i, i, i]
+        L.rgl_gl_draw_elements_instanced.argtypes = [vp, i, i, i, vp, i]
+        L.rgl_gl_draw_arrays_instanced.argtypes = [vp, i, i, i, i]
+        L.rgl_gl_store_color_tc.argtypes = [vp, vp, i]
+        L.rgl_gl_store_depth.argtypes = [vp, vp]
+        L.rgl_gl_store_primitive_id.argtypes = [vp, vp]
+        L.rgl_gl_reset.argtypes = [vp]
+        L.rgl_texture_make_mipmap.argtypes = [vp, i]
+        _lib = L
+    return _lib
+
+
+# every symbol include/rgl_hip.h declares
+EXPORTS = [
+    "rgl_gpu_create", "rgl_gpu_destroy", "rgl_gpu_install", "rgl_gpu_reset", "rgl_gpu_run",
+    "rgl_gpu_wait", "rgl_gpu_stats", "rgl_gpu_set_timing", "rgl_gpu_kernel_times", "rgl_gpu_last_error",
+    "rgl_gl_enable", "rgl_gl_disable", "rgl_gl_depth_func", "rgl_gl_depth_write_mask",
+    "rgl_gl_color_write_mask", "rgl_gl_cull_face", "rgl_gl_scissor", "rgl_gl_viewport",
+    "rgl_gl_use_program", "rgl_gl_use_uniforms", "rgl_gl_alloc_uniform_buffer",
+    "rgl_gl_uniform_buffer_addr", "rgl_gl_renderbuffer_type", "rgl_gl_bind_texture",
+    "rgl_gl_use_buffer", "rgl_gl_clear_color", "rgl_gl_clear_depth", "rgl_gl_view_matrix",
+    "rgl_gl_projection_matrix", "rgl_gl_normal_matrix", "rgl_gl_draw_elements",
+    "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",
+    "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
+    "rgl_gl_reset", "rgl_texture_make_mipmap",
+]
+
+
+def make_mipmap(texels: np.ndarray, dim: int) -> np.ndarray:
+    """Texture::maybe_make_mipmap on a host float RGBA N x 2N buffer (in place)."""
+    assert texels.dtype == np.float32 and texels.flags["C_CONTIGUOUS"] and texels.size >= dim * dim * 8
+    rc = lib().rgl_texture_make_mipmap(texels.ctypes.data, dim)
+    if rc != 0:
+        raise RglError("rgl_texture_make_mipmap failed")
+    return texels
+
+
+class GPU:
+    """rglv::GPU on one HIP device (rglv_gpu.hxx:97-276)."""
+
+    def __init__(self, device: int = 0, concurrency: int = 1, guid: str = "unnamed"):
+        self.L = lib()
+        h = ctypes.c_void_p()
+        rc = self.L.rgl_gpu_create(device, concurrency, guid.encode(), ctypes.byref(h))
+        if rc != 0:
+            raise RglError(f"rgl_gpu_create failed ({rc})")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rgl_gpu_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc < 0:
+            raise RglError(self.L.rgl_gpu_last_error(self.h).decode() + f" ({rc})")
+        return rc
+
+    def install(self, program_id, state_key, kind):
+        self._check(self.L.rgl_gpu_install(self.h, program_id, state_key, kind))
+
+    def reset(self, w, h, tbx=8, tby=8):
+        self._keep = []
+        self._check(self.L.rgl_gpu_reset(self.h, w, h, tbx, tby))
+
+    def run(self, stream=None):
+        self._check(self.L.rgl_gpu_run(self.h, stream))
+
+    def wait(self):
+        return self._check(self.L.rgl_gpu_wait(self.h))
+
+    def run_and_wait(self, stream=None, retries: int = 3):
+        """Run; when a per-frame capacity overflowed the library has grown it,
+        so the frame is drawn again."""
+        for _ in range(retries + 1):
+            self.run(stream)
+            rc = self.L.rgl_gpu_wait(self.h)
+            if rc != RGL_E_CAPACITY:
+                return self._check(rc)
+        raise RglError("capacity overflow persisted")
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.L.rgl_gpu_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def set_timing(self, enable: bool):
+        self._check(self.L.rgl_gpu_set_timing(self.h, int(enable)))
+
+    def kernel_times(self) -> dict:
+        ms = (ctypes.c_double * 16)()
+        names = ctypes.create_string_buffer(512)
+        n = self._check(self.L.rgl_gpu_kernel_times(self.h, ms, 16, names, 512))
+        return dict(zip(names.value.decode().split(","), [ms[i] for i in range(n)]))
+
+    # --- rglv::GL -----------------------------------------------------------
+    def gl_alloc_uniform_buffer(self, values=None):
+        v = np.zeros(32, np.float32)
+        if values is not None:
+            v[:len(values)] = values
+        return self._check(self.L.rgl_gl_alloc_uniform_buffer(self.h, v.ctypes.data, 32))
+
+    def gl_use_buffer(self, slot, dev_ptr, count=0):
+        return self._check(self.L.rgl_gl_use_buffer(self.h, slot, dev_ptr, count if dev_ptr else 0))
+
+    def __getattr__(self, name):
+        if not name.startswith("gl_"):
+            raise AttributeError(name)
+        fn = getattr(self.L, "rgl_" + name)
+
+        def call(*args):
+            conv = []
+            for a in args:
+                if isinstance(a, np.ndarray):  # host data copied by the call (matrices)
+                    a = np.ascontiguousarray(a, dtype=np.float32)
+                    self._keep.append(a)
+                    conv.append(a.ctypes.data)
+                elif isinstance(a, bool):
+                    conv.append(int(a))
+                else:
+                    conv.append(a)
+            return self._check(fn(self.h, *conv))
+        return call
+
+
+class DeviceScene:
+    """A Scene's arrays resident in HBM (torch tensors) plus output targets."""
+
+    def __init__(self, scene, device="cuda:0", prim: bool = True):
+        import torch
+        from .scenes import scene_arrays
+        self.scene = scene
+        self.tensors = {}
+        for a in scene_arrays(scene):
+            t = torch.from_numpy(np.ascontiguousarray(a)).to(device)
+            self.tensors[id(a)] = t
+        self.ptrs = {k: t.data_ptr() for k, t in self.tensors.items()}
+        h, w = scene.height, scene.width
+        self.color = torch.zeros((h, w), dtype=torch.int32, device=device)
+        self.depth = torch.zeros((h, w), dtype=torch.float32, device=device)
+        self.prim = torch.zeros((h, w), dtype=torch.int32, device=device) if prim else None
+
+    def record(self, gpu: GPU):
+        from .scenes import render_scene
+        render_scene(gpu, self.scene, self.color.data_ptr(), self.depth.data_ptr(),
+                     self.prim.data_ptr() if self.prim is not None else None, buffers=self.ptrs)
+
+    def outputs(self):
+        c = self.color.cpu().numpy().view(np.uint32)
+        d = self.depth.cpu().numpy()
+        p = self.prim.cpu().numpy().view(np.uint32) if self.prim is not None else None
+        return c, d, p
+
+
+def render(scene, gpu: GPU | None = None, device="cuda:0", prim: bool = True):
+    """Draw a Scene on the GPU; returns (color u32, depth f32, prim u32, stats)."""
+    import torch
+    own = gpu is None
+    if own:
+        gpu = GPU(int(str(device).split(":")[-1]) if ":" in str(device) else 0)
+    ds = DeviceScene(scene, device, prim)
+    ds.record(gpu)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    gpu.run_and_wait(stream)
+    out = ds.outputs()
+    st = gpu.stats()
+    if own:
+        gpu.close()
+    return (*out, st)

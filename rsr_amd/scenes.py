@@ -1,0 +1,393 @@
+"""Seeded synthetic scenes for the rgl pipeline workloads.
+
+Every array is float32 / uint16 numpy data that both the HIP pipeline and
+any other backend receive unchanged, so the scene generator's own floating
+point (trigonometry, normalisation) never affects parity.
+
+Layouts follow the reference's conventions:
+  * vertex attributes are SoA float arrays padded to a multiple of 4
+    (rglv::Float3Array / VertexArray_F3F3F3::pad, rglv_vao.hxx:141-272);
+  * attribute slots: 0-2 position, 3-5 normal, 9-10 uv, 15 per-instance
+    mat4 array (viewer/shaders.hxx:83-112, 366-367);
+  * mat4 is column-major, built from row-major arguments
+    (rmlm_mat4.hxx:156-164);
+  * textures are float RGBA with the mip chain stacked below the base
+    level in an N x 2N buffer (rglr_texture.cxx:33-81).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# GL constants (rglv_gl.hxx:20-62)
+GL_TRIANGLES = 0
+GL_CULL_FACE, GL_SCISSOR_TEST, GL_BLEND, GL_DEPTH_TEST = 1, 2, 3, 4
+GL_FRONT, GL_BACK = 1, 2
+GL_UNSIGNED_SHORT = 1
+GL_NEAREST_MIPMAP_NEAREST, GL_LINEAR_MIPMAP_NEAREST = 0, 1
+GL_COLOR_BUFFER_BIT, GL_DEPTH_BUFFER_BIT = 1, 2
+GL_LESS, GL_LEQUAL, GL_EQUAL = 0, 1, 2
+GL_DEPTH_ATTACHMENT, GL_COLOR_ATTACHMENT0 = 0, 2
+RB_COLOR_DEPTH, RB_RGBF32, RB_RGBAF32, RB_F32 = 0, 1, 2, 3
+RGL_HINT_READ4, RGL_HINT_DENSE = 1, 2
+
+# program ids: 1 DefaultPost, 2 ExposurePost, 4 Amy, 6 Many are the
+# reference's (viewer/shaders.hxx); 80 and 81 are this project's shaders.
+PGM_DEFAULT_POST = 1
+PGM_EXPOSURE_POST = 2
+PGM_AMY = 4
+PGM_MANY = 6
+PGM_LAMBERT_TEX = 80
+PGM_MANY_BLINN_PHONG = 81
+
+# dispatch keys (viewer/shaders.cxx:48-52 and :71)
+KEY_RGBF32_F32_LESS = 0x6e2
+KEY_COLOR_DEPTH_LESS = 0x62
+
+
+def mat4_rows(*rows) -> np.ndarray:
+    """mat4 from row-major arguments, stored column-major (rmlm_mat4.hxx:156-164)."""
+    m = np.array(rows, dtype=np.float64).reshape(4, 4)
+    return np.ascontiguousarray(m.T.reshape(16)).astype(np.float32)
+
+
+def perspective2(fovy: float, aspect: float, znear: float, zfar: float) -> np.ndarray:
+    """gluPerspective (rglv_math.cxx:152-167), float32 like the reference."""
+    h = np.float32(math.tan((fovy / 2) / 180 * math.pi) * znear)
+    w = np.float32(h * np.float32(aspect))
+    l, r, b, t = -w, w, -h, h
+    n, f = np.float32(znear), np.float32(zfar)
+    two = np.float32(2)
+    return mat4_rows(
+        (two * n) / (r - l), 0, (r + l) / (r - l), 0,
+        0, (two * n) / (t - b), (t + b) / (t - b), 0,
+        0, 0, -((f + n) / (f - n)), -((two * f * n) / (f - n)),
+        0, 0, -1, 0)
+
+
+def look_at(eye, center, up) -> np.ndarray:
+    """gluLookAt (rglv_math.cxx:129-146)."""
+    eye, center, up = (np.asarray(v, np.float64) for v in (eye, center, up))
+    fwd = center - eye
+    fwd /= np.linalg.norm(fwd)
+    side = np.cross(fwd, up)
+    side /= np.linalg.norm(side)
+    up2 = np.cross(side, fwd)
+    rot = np.array([[*side, 0], [*up2, 0], [*(-fwd), 0], [0, 0, 0, 1]])
+    tr = np.eye(4)
+    tr[:3, 3] = -eye
+    m = rot @ tr
+    return np.ascontiguousarray(m.T.reshape(16)).astype(np.float32)
+
+
+def pad4(a: np.ndarray, fill: float = 0.0) -> np.ndarray:
+    n = len(a)
+    p = (n + 3) & ~3
+    out = np.full(p, fill, dtype=np.float32)
+    out[:n] = a
+    return out
+
+
+# ---------------------------------------------------------------------------
+# textures
+# ---------------------------------------------------------------------------
+def xorshift32_stream(seed: int, n: int) -> np.ndarray:
+    x = seed & 0xFFFFFFFF
+    if x == 0:
+        x = 0x9E3779B9
+    out = np.empty(n, dtype=np.uint32)
+    for i in range(n):
+        x ^= (x << 13) & 0xFFFFFFFF
+        x ^= x >> 17
+        x ^= (x << 5) & 0xFFFFFFFF
+        out[i] = x
+    return out
+
+
+def srgb8_to_float_table() -> np.ndarray:
+    """D3D10 sRGB8 -> linear conversion, float32 (ryg-srgb.h:33-56 pattern)."""
+    c = np.arange(256, dtype=np.float32) * np.float32(1.0 / 255.0)
+    lin = np.where(c <= np.float32(0.04045), c / np.float32(12.92),
+                   ((c + np.float32(0.055)) / np.float32(1.055)).astype(np.float64) ** 2.4)
+    return lin.astype(np.float32)
+
+
+def value_noise_rgba8(dim: int, seed: int, lattice: int = 16) -> np.ndarray:
+    """dim x dim RGBA8 value noise: a lattice x lattice grid of xorshift32
+    values, smoothstep-bilinear upsampled with wrap-around."""
+    vals = xorshift32_stream(seed, lattice * lattice * 3).reshape(lattice, lattice, 3)
+    vals = (vals >> 24).astype(np.float64) / 255.0
+    t = (np.arange(dim) + 0.5) * lattice / dim
+    i0 = np.floor(t).astype(np.int64) % lattice
+    i1 = (i0 + 1) % lattice
+    f = t - np.floor(t)
+    s = f * f * (3 - 2 * f)
+    rows0 = vals[i0][:, i0] * (1 - s)[None, :, None] + vals[i0][:, i1] * s[None, :, None]
+    rows1 = vals[i1][:, i0] * (1 - s)[None, :, None] + vals[i1][:, i1] * s[None, :, None]
+    img = rows0 * (1 - s)[:, None, None] + rows1 * s[:, None, None]
+    rgba = np.empty((dim, dim, 4), dtype=np.uint8)
+    rgba[..., :3] = np.clip(np.round(img * 255.0), 0, 255).astype(np.uint8)
+    rgba[..., 3] = 255
+    return rgba
+
+
+def make_mipmap_stack(rgba8: np.ndarray) -> np.ndarray:
+    """Float RGBA texture with the box-filtered mip chain stacked below the
+    base level: an N x 2N buffer of RGBA floats, row stride N
+    (Texture::maybe_make_mipmap, rglr_texture.cxx:33-81)."""
+    n = rgba8.shape[0]
+    assert rgba8.shape[:2] == (n, n) and n & (n - 1) == 0
+    table = srgb8_to_float_table()
+    buf = np.zeros((2 * n, n, 4), dtype=np.float32)
+    buf[:n] = table[rgba8]
+    buf[:n, :, 3] = rgba8[..., 3].astype(np.float32) / np.float32(255.0)
+    src_row, size, dst_row = 0, n, n
+    while size > 1:
+        lv = buf[src_row:src_row + size, :size]
+        a = lv[0::2, 0::2]
+        b = lv[0::2, 1::2]
+        c = lv[1::2, 0::2]
+        d = lv[1::2, 1::2]
+        s = ((a + b) + c) + d
+        half = size // 2
+        buf[dst_row:dst_row + half, :half] = s / np.float32(4.0)
+        src_row, dst_row, size = dst_row, dst_row + half, half
+    return np.ascontiguousarray(buf.reshape(-1))
+
+
+# ---------------------------------------------------------------------------
+# meshes
+# ---------------------------------------------------------------------------
+def uv_sphere(slices: int, stacks: int, radius: float, center):
+    """UV sphere with a seam column: (stacks+1)*(slices+1) vertices and
+    2*slices*(stacks-1) triangles (pole bands emit one triangle per slice)."""
+    cx, cy, cz = center
+    pos, nrm, uv = [], [], []
+    for i in range(stacks + 1):
+        phi = math.pi * i / stacks
+        for j in range(slices + 1):
+            th = 2 * math.pi * j / slices
+            nx, ny, nz = math.sin(phi) * math.cos(th), math.cos(phi), math.sin(phi) * math.sin(th)
+            pos.append((cx + radius * nx, cy + radius * ny, cz + radius * nz))
+            nrm.append((nx, ny, nz))
+            uv.append((j / slices, 1.0 - i / stacks))
+    idx = []
+    row = slices + 1
+    for i in range(stacks):
+        for j in range(slices):
+            a, b = i * row + j, i * row + j + 1
+            c, d = (i + 1) * row + j, (i + 1) * row + j + 1
+            if i != 0:
+                idx += [a, b, c]  # ccw seen from outside
+            if i != stacks - 1:
+                idx += [b, d, c]
+    return (np.array(pos, np.float32), np.array(nrm, np.float32),
+            np.array(uv, np.float32), np.array(idx, np.uint16))
+
+
+def icosphere(subdiv: int):
+    """Subdivided icosahedron on the unit sphere: 10*4^s+2 vertices,
+    20*4^s triangles, shared vertices."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    p = np.array(verts, np.float32)
+    u = (np.arctan2(p[:, 2], p[:, 0]) / (2 * math.pi) + 0.5).astype(np.float32)
+    v = (np.arcsin(np.clip(p[:, 1], -1, 1)) / math.pi + 0.5).astype(np.float32)
+    return p, p.copy(), np.stack([u, v], 1), np.array(faces, np.uint16).reshape(-1)
+
+
+def random_rotation(rng: np.random.RandomState) -> np.ndarray:
+    z = rng.uniform(-1, 1)
+    a = rng.uniform(0, 2 * math.pi)
+    r = math.sqrt(1 - z * z)
+    axis = np.array([r * math.cos(a), r * math.sin(a), z])
+    ang = rng.uniform(0, 2 * math.pi)
+    k = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(ang) * k + (1 - math.cos(ang)) * (k @ k)
+
+
+# ---------------------------------------------------------------------------
+# scene = arrays + the GL call list that draws them
+# ---------------------------------------------------------------------------
+@dataclass
+class Mesh:
+    buffers: dict          # slot -> float32 array (padded)
+    indices: np.ndarray    # uint16
+    instances: int = 1     # >1 -> DrawElementsInstanced
+
+
+@dataclass
+class Scene:
+    name: str
+    width: int
+    height: int
+    tile_blocks: tuple = (8, 8)
+    clear_color: tuple = (0.1, 0.1, 0.15)
+    view: np.ndarray = None
+    projection: np.ndarray = None
+    program: int = PGM_LAMBERT_TEX
+    uniforms: np.ndarray = None
+    texture: np.ndarray = None
+    texture_dim: int = 0
+    texture_filter: int = GL_LINEAR_MIPMAP_NEAREST
+    meshes: list = field(default_factory=list)
+    cull: bool = True
+    depth_func: int = GL_LESS
+    frag_key: int = KEY_RGBF32_F32_LESS
+    post_program: int = PGM_DEFAULT_POST
+
+    @property
+    def triangles(self) -> int:
+        return sum(len(m.indices) // 3 * m.instances for m in self.meshes)
+
+
+def scene_sphere_ground(width: int = 640, height: int = 360, seed: int = 1) -> Scene:
+    """UV sphere (32x16, r=1, centre (0,1,0)) above a 2-triangle ground
+    quad (+-10 at y=0); 60 degree camera; directional Lambert; 256^2
+    seeded value-noise texture with mips, bilinear sampling."""
+    sp_pos, sp_nrm, sp_uv, sp_idx = uv_sphere(32, 16, 1.0, (0.0, 1.0, 0.0))
+    g_pos = np.array([(-10, 0, -10), (10, 0, -10), (10, 0, 10), (-10, 0, 10)], np.float32)
+    g_nrm = np.tile(np.array([[0, 1, 0]], np.float32), (4, 1))
+    g_uv = np.array([(0, 0), (4, 0), (4, 4), (0, 4)], np.float32)
+    g_idx = np.array([0, 2, 1, 0, 3, 2], np.uint16)
+
+    def mesh(pos, nrm, uv, idx):
+        b = {}
+        for k in range(3):
+            b[k] = pad4(pos[:, k])
+            b[3 + k] = pad4(nrm[:, k])
+        b[9], b[10] = pad4(uv[:, 0]), pad4(uv[:, 1])
+        return Mesh(b, idx)
+
+    L = np.array([-1.0, -2.0, -1.0])
+    to_light = -L / np.linalg.norm(L)
+    uni = np.zeros(32, np.float32)
+    uni[0:3] = to_light
+    uni[4], uni[5] = 0.2, 0.8
+    tex = make_mipmap_stack(value_noise_rgba8(256, seed))
+    return Scene(
+        name=f"sphere_ground_{width}x{height}", width=width, height=height,
+        view=look_at((0, 2, 6), (0, 0.75, 0), (0, 1, 0)),
+        projection=perspective2(60.0, width / height, 0.1, 100.0),
+        program=PGM_LAMBERT_TEX, uniforms=uni, texture=tex, texture_dim=256,
+        meshes=[mesh(g_pos, g_nrm, g_uv, g_idx), mesh(sp_pos, sp_nrm, sp_uv, sp_idx)])
+
+
+def instance_matrices(n: int, seed: int, center, extent: float, smin: float, smax: float) -> np.ndarray:
+    rng = np.random.RandomState(seed)
+    out = np.empty((n, 16), np.float32)
+    for i in range(n):
+        r = random_rotation(rng)
+        s = rng.uniform(smin, smax)
+        p = np.asarray(center) + rng.uniform(-extent / 2, extent / 2, 3)
+        m = np.eye(4)
+        m[:3, :3] = r * s
+        m[:3, 3] = p
+        out[i] = m.T.reshape(16)
+    return out
+
+
+def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256, seed: int = 2,
+                     tex_seed: int = 3, tex_dim: int = 1024) -> Scene:
+    """instances x subdivision-3 icosphere (1280 triangles each) with seeded
+    rotation, uniform scale 0.2-2.0 and position uniform in a 40-unit cube
+    centred 25 units in front of a 60 degree camera; one instanced draw;
+    Blinn-Phong with one point light; mipmapped value-noise texture."""
+    pos, nrm, uv, idx = icosphere(3)
+    b = {}
+    for k in range(3):
+        b[k] = pad4(pos[:, k])
+        b[3 + k] = pad4(nrm[:, k])
+    b[9], b[10] = pad4(uv[:, 0]), pad4(uv[:, 1])
+    b[15] = instance_matrices(instances, seed, (0.0, 0.0, -25.0), 40.0, 0.2, 2.0).reshape(-1)
+    uni = np.zeros(32, np.float32)
+    uni[0:3] = (0.0, 0.0, 0.0)       # eye
+    uni[4:7] = (10.0, 15.0, 0.0)     # point light
+    uni[8], uni[9], uni[10] = 0.15, 0.85, 0.6
+    tex = make_mipmap_stack(value_noise_rgba8(tex_dim, tex_seed))
+    return Scene(
+        name=f"icospheres{instances}_{width}x{height}", width=width, height=height,
+        view=look_at((0, 0, 0), (0, 0, -1), (0, 1, 0)),
+        projection=perspective2(60.0, width / height, 0.1, 100.0),
+        program=PGM_MANY_BLINN_PHONG, uniforms=uni, texture=tex, texture_dim=tex_dim,
+        meshes=[Mesh(b, idx, instances)])
+
+
+def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers=None) -> None:
+    """Issue the GL calls that draw *scene* into the given targets.
+
+    *gpu* is any object with the rglv::GPU / rglv::GL method set (reset,
+    install, gl_*); *buffers* maps id(array) -> the pointer the backend
+    expects for that array (device pointer for the HIP pipeline)."""
+    ptr = (lambda a: buffers[id(a)]) if buffers is not None else (lambda a: a)
+    gpu.reset(scene.width, scene.height, *scene.tile_blocks)
+    gpu.install(scene.program, 0, 0)
+    gpu.install(scene.program, scene.frag_key, 1)
+    gpu.install(scene.post_program, 0, 2)
+    gpu.gl_renderbuffer_type(GL_COLOR_ATTACHMENT0, RB_RGBF32)
+    gpu.gl_renderbuffer_type(GL_DEPTH_ATTACHMENT, RB_F32)
+    gpu.gl_clear_color(*scene.clear_color)
+    gpu.gl_clear_depth(1.0)
+    gpu.gl_clear(GL_COLOR_BUFFER_BIT | GL_DEPTH_BUFFER_BIT)
+    gpu.gl_use_program(scene.program)
+    gpu.gl_use_uniforms(gpu.gl_alloc_uniform_buffer(scene.uniforms))
+    gpu.gl_view_matrix(scene.view)
+    gpu.gl_projection_matrix(scene.projection)
+    if scene.cull:
+        gpu.gl_enable(GL_CULL_FACE)
+        gpu.gl_cull_face(GL_BACK)
+    gpu.gl_depth_func(scene.depth_func)
+    if scene.texture is not None:
+        gpu.gl_bind_texture(0, ptr(scene.texture), scene.texture_dim, scene.texture_dim,
+                            scene.texture_dim, scene.texture_filter)
+    for m in scene.meshes:
+        for slot in range(16):
+            if slot in m.buffers:
+                gpu.gl_use_buffer(slot, ptr(m.buffers[slot]), len(m.buffers[slot]))
+            else:
+                gpu.gl_use_buffer(slot, None, 0)
+        if m.instances > 1:
+            gpu.gl_draw_elements_instanced(GL_TRIANGLES, len(m.indices), GL_UNSIGNED_SHORT,
+                                           ptr(m.indices), m.instances)
+        else:
+            gpu.gl_draw_elements(GL_TRIANGLES, len(m.indices), GL_UNSIGNED_SHORT, ptr(m.indices),
+                                 RGL_HINT_DENSE | RGL_HINT_READ4)
+    gpu.gl_use_program(scene.post_program)
+    gpu.gl_store_color_tc(color_out, True)
+    gpu.gl_store_depth(depth_out)
+    if prim_out is not None:
+        gpu.gl_store_primitive_id(prim_out)
+
+
+def scene_arrays(scene: Scene) -> list:
+    """Every array the scene's GL calls reference (to upload once)."""
+    arrs = []
+    if scene.texture is not None:
+        arrs.append(scene.texture)
+    for m in scene.meshes:
+        arrs.extend(m.buffers.values())
+        arrs.append(m.indices)
+    return arrs
