@@ -140,12 +140,24 @@ struct rgl_gpu {
 	long long submitted = 0;
 	int lastCommands = 0;
 
-	// timing
+	// timing: a ring of event sets so timed frames stay queued back to back
+	static constexpr int kRing = 64;
 	bool timing = false;
-	hipEvent_t ev[kNumTimed + 1] = {};
+	hipEvent_t ev[kRing][kNumTimed + 1] = {};
+	bool evUsed[kRing] = {};
+	int evNext = 0;
+	hipEvent_t* cur = nullptr;
 	double acc[kNumTimed] = {};
 	int accRuns = 0;
-	bool evPending = false;
+
+	void Accumulate(int slot) {
+		if (!evUsed[slot]) return;
+		if (hipEventSynchronize(ev[slot][kNumTimed]) == hipSuccess) {
+			for (int i = 0; i < kNumTimed; ++i) {
+				float ms = 0.0F;
+				if (hipEventElapsedTime(&ms, ev[slot][i], ev[slot][i + 1]) == hipSuccess) acc[i] += ms; }
+			++accRuns; }
+		evUsed[slot] = false; }
 
 	~rgl_gpu() {
 		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist }) a->release();
@@ -155,7 +167,7 @@ struct rgl_gpu {
 		if (hCtr) (void)hipHostFree(hCtr);
 		if (hDraws) (void)hipHostFree(hDraws);
 		if (done) (void)hipEventDestroy(done);
-		for (auto& e : ev) if (e) (void)hipEventDestroy(e); }
+		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
 
 	int fail(int code, const std::string& msg) { err = msg; return code; }
 	int hipfail(hipError_t e, const char* what) {
@@ -350,12 +362,10 @@ struct rgl_gpu {
 	void LaunchClip(int di, const FrameDesc& f) {
 		k_clip<P><<<64, 64, 0, stream>>>(dDraws.p, di, f, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr); }
 
-	void Mark(int i) { if (timing) (void)hipEventRecord(ev[i], stream); }
+	void Mark(int i) { if (cur) (void)hipEventRecord(cur[i], stream); }
 
 	int Run(hipStream_t s) {
 		err.clear();
-		// with timing on, each frame is waited for so its events can be read
-		if (pending && timing) { const int r = Wait(); if (r != 0) return r; }
 		FrameDesc f;
 		std::vector<DrawDesc> draws;
 		submitted = 0;
@@ -371,7 +381,13 @@ struct rgl_gpu {
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
-		if (timing && !ev[0]) for (auto& x : ev) (void)hipEventCreate(&x);
+		cur = nullptr;
+		if (timing) {
+			if (!ev[0][0]) for (auto& set : ev) for (auto& x : set) (void)hipEventCreate(&x);
+			Accumulate(evNext);
+			cur = ev[evNext];
+			evUsed[evNext] = true;
+			evNext = (evNext + 1) % kRing; }
 		// the draw table is uploaded only when it changed; the pinned staging
 		// copy is rewritten only after the frame reading it has completed
 		const size_t tableBytes = sizeof(DrawDesc) * draws.size();
@@ -431,7 +447,6 @@ struct rgl_gpu {
 		stats.commands = static_cast<int64_t>(commands.size());
 		stats.primitives_submitted = submitted;
 		pending = true;
-		evPending = timing;
 		return 0; }
 
 	int Wait() {
@@ -445,12 +460,7 @@ struct rgl_gpu {
 		stats.clipped_tile_pairs = hCtr->clipPairs;
 		stats.fragments = static_cast<int64_t>(hCtr->fragments);
 		stats.overflow = hCtr->overflow;
-		if (evPending) {
-			for (int i = 0; i < kNumTimed; ++i) {
-				float ms = 0.0F;
-				if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess) acc[i] += ms; }
-			++accRuns;
-			evPending = false; }
+		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
 		if (hCtr->overflow & 16u) return fail(RGL_E_STATE, "tile list held an invalid entry");
 		if (hCtr->overflow) {
@@ -522,6 +532,7 @@ int rgl_gpu_stats(rgl_gpu* g, rgl_stats* out) {
 
 int rgl_gpu_set_timing(rgl_gpu* g, int enable) {
 	if (check(g)) return RGL_E_ARG;
+	for (int k = 0; k < rgl_gpu::kRing; ++k) g->evUsed[k] = false;
 	g->timing = enable != 0;
 	for (double& a : g->acc) a = 0.0;
 	g->accRuns = 0;
@@ -529,6 +540,7 @@ int rgl_gpu_set_timing(rgl_gpu* g, int enable) {
 
 int rgl_gpu_kernel_times(rgl_gpu* g, double* ms_out, int max_kernels, char* names_out, int names_len) {
 	if (check(g)) return RGL_E_ARG;
+	for (int k = 0; k < rgl_gpu::kRing; ++k) g->Accumulate(k);
 	std::string names;
 	for (int i = 0; i < kNumTimed; ++i) {
 		if (i < max_kernels && ms_out) ms_out[i] = g->accRuns ? g->acc[i] / g->accRuns : 0.0;
