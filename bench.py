@@ -102,15 +102,20 @@ def main():
     mtris = tris / (ms * 1e-3) / 1e6
     gfrag = st["fragments"] / (ms * 1e-3) / 1e9
 
-    # dominant kernel: per-tile raster + resolve.  Algorithmic HBM bytes per
-    # launch: RGBA8 + depth f32 + primitive id u32 written once per pixel
-    # (12 B/px) plus one u32 tile-list entry per (triangle, tile) pair.
+    # roofline of the dominant kernel (the larger of k_vis and k_resolve).
+    # Algorithmic HBM bytes per launch:
+    #   k_resolve: RGBA8 + depth f32 + primitive id u32 written once per
+    #              pixel plus the u32 visibility slot read: 16 B/px;
+    #   k_vis:     one u32 tile-list entry per (triangle, tile) pair read
+    #              plus the u32 visibility slot written per pixel.
     W, H = scene.width, scene.height
-    raster_ms = kt.get("k_raster", 0.0)
-    raster_bytes = 12 * W * H + 4 * st["tile_pairs"]
+    kbytes = {"k_resolve": 16 * W * H, "k_vis": 4 * st["tile_pairs"] + 4 * W * H}
+    dom = max(kbytes, key=lambda k: kt.get(k, 0.0))
+    raster_ms = kt.get(dom, 0.0)
+    raster_bytes = kbytes[dom]
     achieved = raster_bytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_raster_icospheres256_1080p_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_icospheres256_1080p_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -136,7 +141,7 @@ def main():
         "fragments": st["fragments"],
         "tile_pairs": st["tile_pairs"],
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
-        "roofline": {"kernel": "k_raster", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
     }
     if not args.no_cpu:

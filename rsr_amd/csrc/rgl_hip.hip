@@ -97,8 +97,8 @@ struct DevArray {
 		return e; }
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
-const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_raster" };
-constexpr int kNumTimed = 6;
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_vis", "k_resolve" };
+constexpr int kNumTimed = 7;
 
 }  // namespace
 
@@ -124,7 +124,7 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, list;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist, vis;
 	DevArray<DrawDesc> dDraws;
 	Counters* dCtr = nullptr;
 	Counters* hCtr = nullptr;       // pinned
@@ -160,7 +160,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist }) a->release();
+		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &vis }) a->release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
 		if (dCtr) (void)hipFree(dCtr);
@@ -347,6 +347,7 @@ struct rgl_gpu {
 		if ((e = clipCursor.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
 		if ((e = clist.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clist");
+		if ((e = vis.ensure(static_cast<size_t>(f.W) * f.H)) != hipSuccess) return hipfail(e, "hipMalloc vis");
 		if (!dDraws.p) {
 			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
 			lastTable.clear(); }
@@ -437,8 +438,15 @@ struct rgl_gpu {
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
-		if (f.nTiles > 0) k_raster<<<f.nTiles, kRasterThreads, 0, stream>>>(f, ra);
+		ra.vis = vis.p;
+		if (f.nTiles > 0) k_vis<<<f.nTiles, kRasterThreads, 0, stream>>>(f, ra);
 		Mark(6);
+		{
+			const int qw = (f.W + 1) / 2, qh = (f.H + 1) / 2;
+			const dim3 grid((qw + kResolveQx - 1) / kResolveQx, (qh + kResolveQy - 1) / kResolveQy);
+			k_resolve<<<grid, 256, 0, stream>>>(f, ra);
+		}
+		Mark(7);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
 		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");

@@ -19,7 +19,7 @@
 
 namespace rgl {
 
-constexpr int kUnitTris = 1024;     // triangles per binning unit (one wave)
+constexpr int kUnitTris = 256;      // triangles per binning unit (one wave)
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // LDS key array is kMaxTileDim^2
 
@@ -328,6 +328,12 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 			float pos[4];
 			P::vertex(d, iid, idx, pos, A[k].v);
 			A[k].c = make_float4(pos[0], pos[1], pos[2], pos[3]); }
+		// all three vertices outside one clipping plane: Sutherland-Hodgman
+		// would emit an empty polygon for that plane
+		bool rejected = false;
+		for (int plane = 0; plane < 5; ++plane)
+			rejected |= plane_dist(plane, A[0].c) < 0.0F && plane_dist(plane, A[1].c) < 0.0F && plane_dist(plane, A[2].c) < 0.0F;
+		if (rejected) { fcount[q] = 0; continue; }
 		for (int plane = 0; plane < 5 && na > 0; ++plane) {
 			bool hereIsInside = plane_dist(plane, A[0].c) >= 0.0F;
 			int nb = 0;
@@ -555,6 +561,7 @@ struct RasterArgs {
 	uint32_t clistCap;
 	uint32_t clipCap;
 	Counters* ctr;
+	uint32_t* vis;              // per-pixel winning slot, 0 = none
 };
 
 template <class P>
@@ -589,7 +596,7 @@ __device__ inline void shade_dispatch(const DrawDesc& d, const TriSetup& t, cons
 	case ProgLambertTex::kId: shade_lanes<ProgLambertTex>(d, t, tr, vary, kx, ky, color, depth); break;
 	default: shade_lanes<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, color, depth); break; }}
 
-__global__ __launch_bounds__(kRasterThreads) void k_raster(FrameDesc f, RasterArgs a) {
+__global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs a) {
 	__shared__ unsigned long long keys[kMaxTileDim * kMaxTileDim];
 	__shared__ TriSetup ts[kRasterThreads];
 	__shared__ uint32_t qoff[kRasterThreads + 1];
@@ -671,70 +678,94 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(FrameDesc f, RasterAr
 					if (key < keys[li]) atomicMin(&keys[li], key); }}
 			__syncthreads(); }}
 
-	// ---- resolve: shade each pixel's winner once, write colour/depth/id ----
-	const int qw = f.tileW >> 1, qh = f.tileH >> 1;
-	for (int q = threadIdx.x; q < qw * qh; q += kRasterThreads) {
-		const int qy = q / qw, qx = q - qy * qw;
-		const int x = R.x0 + 2 * qx, y = R.y0 + 2 * qy;
-		if (x >= R.x1 || y >= R.y1) continue;
-		uint32_t win[4];
-		for (int l = 0; l < 4; ++l) {
-			const unsigned long long k = keys[(y + (l >> 1) - R.y0) * f.tileW + (x + (l & 1) - R.x0)];
-			const uint32_t lo32 = static_cast<uint32_t>(k);
-			win[l] = (k == initKey) ? 0u : (leq ? ~lo32 : lo32); }
-		float col[3][4], dep[4];
-		uint32_t prim[4];
-		for (int l = 0; l < 4; ++l) {
-			col[0][l] = f.clearColor[0]; col[1][l] = f.clearColor[1]; col[2][l] = f.clearColor[2];
-			dep[l] = f.clearDepth; prim[l] = 0xffffffffu; }
-		uint32_t done = 0;
-		for (int l = 0; l < 4; ++l) {
-			if (win[l] == 0u || (done & (1u << l))) continue;
-			const uint32_t slot = win[l];
-			const int di = find_draw_slot(dl, f.nDraws, slot);
-			const DrawLite& d = dl[di];
-			const uint32_t rel = slot - d.slotBase;
-			const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
-			TriVerts tr;
-			TriSetup t;
-			uint32_t srcTri;
-			const float* vary;
-			if (rel < nTri) {
-				srcTri = rel;
-				inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
-				setup_vector(R, tr.X, tr.Y, t);
-				vary = a.vvary; }
-			else {
-				const uint32_t fslot = rel - nTri;
-				srcTri = fslot / kFanMax;
-				const uint32_t j = fslot - srcTri * kFanMax;
-				const uint32_t cq = a.clipqOf[d.triBase + srcTri];
-				clipped_verts(cq * kFanMax + j, a.cvpos, tr);
-				setup_scalar(R, tr.X, tr.Y, t);
-				vary = a.cvvary; }
-			const int kx = (x - t.minx) >> 1, ky = (y - t.miny) >> 1;
-			float c4[3][4], d4[4];
-			shade_dispatch(a.draws[di], t, tr, vary, kx, ky, c4, d4);
-			for (int m = l; m < 4; ++m) {
-				if (win[m] != slot) continue;
-				done |= 1u << m;
-				col[0][m] = c4[0][m]; col[1][m] = c4[1][m]; col[2][m] = c4[2][m];
-				dep[m] = d4[m];
-				prim[m] = d.primBase + srcTri; }}
-		for (int l = 0; l < 4; ++l) {
-			const int px = x + (l & 1), py = y + (l >> 1);
-			if (px >= R.x1 || py >= R.y1) continue;
-			const long long o = static_cast<long long>(py) * f.W + px;
-			if (f.outColor) {
-				float r = col[0][l], g = col[1][l], b = col[2][l];
-				if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
-				f.outColor[o] = f.gamma ? ((to_srgb8(r) << 16) | (to_srgb8(g) << 8) | to_srgb8(b))
-				                        : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
-			if (f.outDepth) f.outDepth[o] = dep[l];
-			if (f.outPrim) f.outPrim[o] = prim[l]; }}
+	// ---- write each pixel's winning slot (0 = nothing drawn) ----
+	const int rw = R.x1 - R.x0, rh = R.y1 - R.y0;
+	for (int i = threadIdx.x; i < rw * rh; i += kRasterThreads) {
+		const int ly = i / rw, lx = i - ly * rw;
+		const unsigned long long k = keys[ly * f.tileW + lx];
+		const uint32_t lo32 = static_cast<uint32_t>(k);
+		a.vis[static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = (k == initKey) ? 0u : (leq ? ~lo32 : lo32); }
 
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
 	if ((threadIdx.x & 63) == 0 && frags) atomicAdd(&a.ctr->fragments, frags); }
+
+
+// ---------------------------------------------------------------------------
+// resolve: one thread per 2x2 quad of the screen.  Each pixel's winning slot
+// is shaded by recomputing its triangle's quad exactly as the reference's
+// rasteriser produced it in that pixel's tile (tile-clamped edge start,
+// barycentric scale, lanes 0/1 for the texture LOD), then post-shaded and
+// sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104), and colour, depth
+// and primitive id are written once.
+// ---------------------------------------------------------------------------
+constexpr int kResolveQx = 32, kResolveQy = 8;   // 256 threads = 32 x 8 quads
+
+__global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
+	__shared__ DrawLite dl[kMaxDraws];
+	load_lite(dl, a.draws, f.nDraws);
+	__syncthreads();
+	const int qx = blockIdx.x * kResolveQx + (threadIdx.x % kResolveQx);
+	const int qy = blockIdx.y * kResolveQy + (threadIdx.x / kResolveQx);
+	const int x = 2 * qx, y = 2 * qy;
+	if (x >= f.W || y >= f.H) return;
+	const int tx = x / f.tileW, ty = y / f.tileH;
+	Rect R;
+	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
+	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
+	uint32_t win[4];
+	for (int l = 0; l < 4; ++l) {
+		const int px = x + (l & 1), py = y + (l >> 1);
+		win[l] = (px < f.W && py < f.H) ? a.vis[static_cast<long long>(py) * f.W + px] : 0u; }
+	float col[3][4], dep[4];
+	uint32_t prim[4];
+	for (int l = 0; l < 4; ++l) {
+		col[0][l] = f.clearColor[0]; col[1][l] = f.clearColor[1]; col[2][l] = f.clearColor[2];
+		dep[l] = f.clearDepth; prim[l] = 0xffffffffu; }
+	uint32_t done = 0;
+	for (int l = 0; l < 4; ++l) {
+		if (win[l] == 0u || (done & (1u << l))) continue;
+		const uint32_t slot = win[l];
+		const int di = find_draw_slot(dl, f.nDraws, slot);
+		const DrawLite& d = dl[di];
+		const uint32_t rel = slot - d.slotBase;
+		const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
+		TriVerts tr;
+		TriSetup t;
+		uint32_t srcTri;
+		const float* vary;
+		if (rel < nTri) {
+			srcTri = rel;
+			inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
+			setup_vector(R, tr.X, tr.Y, t);
+			vary = a.vvary; }
+		else {
+			const uint32_t fslot = rel - nTri;
+			srcTri = fslot / kFanMax;
+			const uint32_t j = fslot - srcTri * kFanMax;
+			const uint32_t cq = a.clipqOf[d.triBase + srcTri];
+			clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+			setup_scalar(R, tr.X, tr.Y, t);
+			vary = a.cvvary; }
+		const int kx = (x - t.minx) >> 1, ky = (y - t.miny) >> 1;
+		float c4[3][4], d4[4];
+		shade_dispatch(a.draws[di], t, tr, vary, kx, ky, c4, d4);
+		for (int m = l; m < 4; ++m) {
+			if (win[m] != slot) continue;
+			done |= 1u << m;
+			col[0][m] = c4[0][m]; col[1][m] = c4[1][m]; col[2][m] = c4[2][m];
+			dep[m] = d4[m];
+			prim[m] = d.primBase + srcTri; }}
+	for (int l = 0; l < 4; ++l) {
+		const int px = x + (l & 1), py = y + (l >> 1);
+		if (px >= f.W || py >= f.H) continue;
+		const long long o = static_cast<long long>(py) * f.W + px;
+		if (f.outColor) {
+			float r = col[0][l], g = col[1][l], b = col[2][l];
+			if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
+			f.outColor[o] = f.gamma ? ((to_srgb8(r) << 16) | (to_srgb8(g) << 8) | to_srgb8(b))
+			                        : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+		if (f.outDepth) f.outDepth[o] = dep[l];
+		if (f.outPrim) f.outPrim[o] = prim[l]; }}
 
 __global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -255,8 +255,8 @@ class Scene:
     texture_filter: int = GL_LINEAR_MIPMAP_NEAREST
     meshes: list = field(default_factory=list)
     cull: bool = True
+    cull_face: int = GL_BACK
     depth_func: int = GL_LESS
-    frag_key: int = KEY_RGBF32_F32_LESS
     post_program: int = PGM_DEFAULT_POST
 
     @property
@@ -296,6 +296,41 @@ def scene_sphere_ground(width: int = 640, height: int = 360, seed: int = 1) -> S
         meshes=[mesh(g_pos, g_nrm, g_uv, g_idx), mesh(sp_pos, sp_nrm, sp_uv, sp_idx)])
 
 
+def scene_clip_floor(width: int = 640, height: int = 360, n: int = 32, seed: int = 9) -> Scene:
+    """A tessellated n x n floor (2 n^2 triangles) spanning x in [-20, 20],
+    z in [-40, 10] at y = -1 under a camera at the origin looking slightly
+    down: the triangles near the camera cross the near plane and the guard
+    band with parts on screen, so clipped fans cover many tiles."""
+    xs = np.linspace(-20.0, 20.0, n + 1)
+    zs = np.linspace(-40.0, 10.0, n + 1)
+    X, Z = np.meshgrid(xs, zs)
+    pos = np.stack([X.ravel(), np.full(X.size, -1.0), Z.ravel()], 1).astype(np.float32)
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (len(pos), 1))
+    uv = np.stack([(X.ravel() + 20) / 10, (Z.ravel() + 40) / 10], 1).astype(np.float32)
+    idx = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            b, c, d = a + 1, a + n + 1, a + n + 2
+            idx += [a, c, b, b, c, d]
+    b_ = {}
+    for k in range(3):
+        b_[k] = pad4(pos[:, k])
+        b_[3 + k] = pad4(nrm[:, k])
+    b_[9], b_[10] = pad4(uv[:, 0]), pad4(uv[:, 1])
+    uni = np.zeros(32, np.float32)
+    L = np.array([0.3, 1.0, 0.2])
+    uni[0:3] = L / np.linalg.norm(L)
+    uni[4], uni[5] = 0.25, 0.75
+    tex = make_mipmap_stack(value_noise_rgba8(256, seed))
+    return Scene(
+        name=f"clip_floor{n}_{width}x{height}", width=width, height=height,
+        view=look_at((0, 0, 0), (0, -0.35, -1), (0, 1, 0)),
+        projection=perspective2(60.0, width / height, 0.1, 100.0),
+        program=PGM_LAMBERT_TEX, uniforms=uni, texture=tex, texture_dim=256,
+        meshes=[Mesh(b_, np.array(idx, np.uint16))])
+
+
 def instance_matrices(n: int, seed: int, center, extent: float, smin: float, smax: float) -> np.ndarray:
     rng = np.random.RandomState(seed)
     out = np.empty((n, 16), np.float32)
@@ -311,7 +346,8 @@ def instance_matrices(n: int, seed: int, center, extent: float, smin: float, sma
 
 
 def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256, seed: int = 2,
-                     tex_seed: int = 3, tex_dim: int = 1024) -> Scene:
+                     tex_seed: int = 3, tex_dim: int = 1024, center=(0.0, 0.0, -25.0), extent: float = 40.0,
+                     smin: float = 0.2, smax: float = 2.0, program: int = PGM_MANY_BLINN_PHONG) -> Scene:
     """instances x subdivision-3 icosphere (1280 triangles each) with seeded
     rotation, uniform scale 0.2-2.0 and position uniform in a 40-unit cube
     centred 25 units in front of a 60 degree camera; one instanced draw;
@@ -322,7 +358,7 @@ def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256
         b[k] = pad4(pos[:, k])
         b[3 + k] = pad4(nrm[:, k])
     b[9], b[10] = pad4(uv[:, 0]), pad4(uv[:, 1])
-    b[15] = instance_matrices(instances, seed, (0.0, 0.0, -25.0), 40.0, 0.2, 2.0).reshape(-1)
+    b[15] = instance_matrices(instances, seed, center, extent, smin, smax).reshape(-1)
     uni = np.zeros(32, np.float32)
     uni[0:3] = (0.0, 0.0, 0.0)       # eye
     uni[4:7] = (10.0, 15.0, 0.0)     # point light
@@ -332,8 +368,15 @@ def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256
         name=f"icospheres{instances}_{width}x{height}", width=width, height=height,
         view=look_at((0, 0, 0), (0, 0, -1), (0, 1, 0)),
         projection=perspective2(60.0, width / height, 0.1, 100.0),
-        program=PGM_MANY_BLINN_PHONG, uniforms=uni, texture=tex, texture_dim=tex_dim,
+        program=program, uniforms=uni, texture=tex, texture_dim=tex_dim,
         meshes=[Mesh(b, idx, instances)])
+
+
+def fragment_state_key(scene: Scene) -> int:
+    """GLState::FragmentStateKey of the scene's draw state without the
+    program id (rglv_gl.hxx:136-146): depth test on, depth and colour
+    writes on, RB_RGBF32 colour, RB_F32 depth."""
+    return (1 << 1) | (scene.depth_func << 2) | (1 << 5) | (1 << 6) | (RB_RGBF32 << 7) | (RB_F32 << 9)
 
 
 def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers=None) -> None:
@@ -345,7 +388,7 @@ def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers
     ptr = (lambda a: buffers[id(a)]) if buffers is not None else (lambda a: a)
     gpu.reset(scene.width, scene.height, *scene.tile_blocks)
     gpu.install(scene.program, 0, 0)
-    gpu.install(scene.program, scene.frag_key, 1)
+    gpu.install(scene.program, fragment_state_key(scene), 1)
     gpu.install(scene.post_program, 0, 2)
     gpu.gl_renderbuffer_type(GL_COLOR_ATTACHMENT0, RB_RGBF32)
     gpu.gl_renderbuffer_type(GL_DEPTH_ATTACHMENT, RB_F32)
@@ -358,7 +401,7 @@ def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers
     gpu.gl_projection_matrix(scene.projection)
     if scene.cull:
         gpu.gl_enable(GL_CULL_FACE)
-        gpu.gl_cull_face(GL_BACK)
+        gpu.gl_cull_face(scene.cull_face)
     gpu.gl_depth_func(scene.depth_func)
     if scene.texture is not None:
         gpu.gl_bind_texture(0, ptr(scene.texture), scene.texture_dim, scene.texture_dim,
