@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run one 64 MiB device-to-device copy (a known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -74,6 +77,12 @@ def main():
     from rsr_amd import rgl, scenes
 
     dev = "cuda:0"
+    if args.copy_calib:
+        src = torch.ones(16 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     scene = scenes.scene_icospheres()
     gpu = rgl.GPU(0)
     ds = rgl.DeviceScene(scene, dev, prim=True)

@@ -685,7 +685,9 @@ struct rglo_gpu {
 	struct TileBuf {
 		std::vector<float> r, g, b, d;
 		std::vector<uint32_t> prim;
-		long long fragments = 0; };
+		long long fragments = 0;
+		long long quads = 0; };
+	std::vector<long long> tileQuads, tilePairs;
 
 	struct TriCtx {
 		const Program* pgm;
@@ -785,6 +787,7 @@ struct rglo_gpu {
 				int32_t e3[4] = { q3, wadd(q3, dy31), wadd(q3, dx31), wadd(wadd(q3, dx31), dy31) };
 				int32_t trimask[4]; int neg = 0;
 				for (int l = 0; l < 4; ++l) { trimask[l] = (e1[l] | e2[l] | e3[l]) >> 31; neg += trimask[l] ? 1 : 0; }
+				++tb.quads;
 				if (neg == 4) continue;
 				float fc[2][4], bs[3][4];
 				const float fx = static_cast<float>(x) + 0.5F;
@@ -848,6 +851,11 @@ struct rglo_gpu {
 
 	bool DrawTile(int t, TileBuf& tb) {
 		const Rect tr = TileRect(t);
+		tb.quads = 0;
+		long long pairs = 0;
+		for (const TileOp& op : tiles[t]) pairs += (op.op == TileOp::TRI && op.a >= 0) || op.op == TileOp::CLIPPED_TRI;
+		struct Rec { rglo_gpu* g; int t; TileBuf& tb; long long pairs;
+			~Rec() { g->tileQuads[t] = tb.quads; g->tilePairs[t] = pairs; } } rec{ this, t, tb, pairs };
 		const size_t area = static_cast<size_t>(tileW) * tileH;
 		tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F);
 		tb.d.assign(area, 0.0F); tb.prim.assign(area, 0xffffffffu);
@@ -934,6 +942,8 @@ struct rglo_gpu {
 	bool Run() {
 		err.clear();
 		if (!Bin()) return false;
+		tileQuads.assign(tiles.size(), 0);
+		tilePairs.assign(tiles.size(), 0);
 		// cost-sorted tile order (GPU::RunImpl, rglv_gpu.cxx:100-104)
 		std::vector<std::pair<size_t, int>> order;
 		for (int t = 0; t < static_cast<int>(tiles.size()); ++t) order.push_back({ tiles[t].size(), t });
@@ -1051,6 +1061,11 @@ void rglo_gl_store_color_tc(rglo_gpu* g, uint32_t* dst, int gamma) {
 void rglo_gl_store_depth(rglo_gpu* g, float* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_DEPTH, 0, 0, 0, dst }); }
 void rglo_gl_store_primitive_id(rglo_gpu* g, uint32_t* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_PRIM, 0, 0, 0, dst }); }
 void rglo_gl_reset(rglo_gpu* g) { g->ResetGL(); }
+
+int rglo_gpu_tile_costs(rglo_gpu* g, long long* pairs, long long* quads, int n) {
+	const int m = std::min(n, static_cast<int>(g->tileQuads.size()));
+	for (int i = 0; i < m; ++i) { pairs[i] = g->tilePairs[i]; quads[i] = g->tileQuads[i]; }
+	return m; }
 
 // building blocks exposed for known-answer tests
 float rglo_oneover(float x) { return oneover(x); }

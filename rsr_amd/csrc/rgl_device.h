@@ -56,6 +56,19 @@ __device__ inline float oneover(float a) {
 	const float muls = a * (r * r);
 	return (r + r) - muls; }
 
+// the same with the mantissa table in LDS
+__device__ inline float oneover_tab(float a, const uint32_t* tab) {
+	const uint32_t in = f2u(a);
+	const uint32_t e = (in >> 23) & 0xffu, s = in & 0x80000000u, m = in & 0x7fffffu;
+	float r;
+	if (e == 0xffu) r = u2f(m ? (in | 0x400000u) : s);
+	else if (e == 0u) r = u2f(s | 0x7f800000u);
+	else {
+		const int re = 253 - static_cast<int>(e);
+		r = re <= 0 ? u2f(s) : u2f(s | (static_cast<uint32_t>(re) << 23) | tab[m >> 12]); }
+	const float muls = a * (r * r);
+	return (r + r) - muls; }
+
 __host__ __device__ inline float sse_max(float a, float b) { return a > b ? a : b; }
 __host__ __device__ inline float sse_min(float a, float b) { return a < b ? a : b; }
 
@@ -72,6 +85,14 @@ __device__ inline uint32_t to_srgb8(float f) {
 	const float clamp2 = sse_min(clamp1, u2f(0x3f7fffffu));
 	const uint32_t bits = f2u(clamp2);
 	const uint32_t tab = kSrgb8Tab[(bits >> 20) - (127u - 13u) * 8u];
+	const uint32_t t = (bits >> 12) & 0xffu;
+	return ((tab & 0xffffu) * t + (tab >> 16) * 512u) >> 16; }
+
+__device__ inline uint32_t to_srgb8_tab(float f, const uint32_t* tab4) {
+	const float clamp1 = sse_max(f, u2f((127u - 13u) << 23));
+	const float clamp2 = sse_min(clamp1, u2f(0x3f7fffffu));
+	const uint32_t bits = f2u(clamp2);
+	const uint32_t tab = tab4[(bits >> 20) - (127u - 13u) * 8u];
 	const uint32_t t = (bits >> 12) & 0xffu;
 	return ((tab & 0xffffu) * t + (tab >> 16) * 512u) >> 16; }
 
@@ -169,66 +190,67 @@ __device__ inline int clamp_texel(int32_t ofs4, int texels) {
 	const int32_t t = ofs4 >> 2;
 	return (t < 0 || t >= texels) ? 0 : ofs4; }
 
-// one quad: u[4], v[4] -> rgba[4][4] (channel, lane)
-__device__ inline void sample_quad(const SamplerDesc& s, const float u[4], const float v[4], float out[4][4]) {
-	if (s.kind < 0 || s.buf == nullptr) {
-		for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c][l] = 0.0F;
-		return; }
+// texture LOD of a quad from the base-level coordinates of its lanes 0
+// and 1, clamped to the mip chain (LevelOfDetail and its clamp,
+// rglr_texture_sampler.cxx:25-42, :81, :188)
+__device__ inline int texture_lod(const SamplerDesc& s, float u0, float u1, float v0, float v1) {
+	if (s.kind < 0 || s.kind == 2) return 0;
+	const float baseDim = static_cast<float>(1 << s.power);
+	const int lod = level_of_detail(u0 * baseDim, u1 * baseDim, v0 * baseDim, v1 * baseDim);
+	return lod < 0 ? 0 : (lod > s.power ? s.power : lod); }
+
+// one lane of TextureUnit::sample (rglr_texture_sampler.cxx:63-311) at a
+// given LOD; every lane of a quad uses the quad's LOD
+__device__ inline float4 sample1(const SamplerDesc& s, float u, float v, int lod) {
+	if (s.kind < 0 || s.buf == nullptr) return make_float4(0.0F, 0.0F, 0.0F, 0.0F);
 	const float almostone = u2f(0x3f7fffffu);
 	if (s.kind == 2) {  // TextureUnitRGBAF32_NM_ONEMAP_WRAP_NEAREST (:289-311)
-		for (int l = 0; l < 4; ++l) {
-			const float fu = u[l] + 100.0F, fv = v[l] + 100.0F;
-			const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
-			const int32_t pu = cvtt(fru * static_cast<float>(s.width));
-			const int32_t pv = cvtt((almostone - frv) * static_cast<float>(s.height));
-			const int ofs = clamp_texel(shl(wadd(wmul(pv, s.stride), pu), 2), s.texels);
-			const float4 t = *reinterpret_cast<const float4*>(s.buf + ofs);
-			out[0][l] = t.x; out[1][l] = t.y; out[2][l] = t.z; out[3][l] = t.w; }
-		return; }
+		const float fu = u + 100.0F, fv = v + 100.0F;
+		const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+		const int32_t pu = cvtt(fru * static_cast<float>(s.width));
+		const int32_t pv = cvtt((almostone - frv) * static_cast<float>(s.height));
+		const int ofs = clamp_texel(shl(wadd(wmul(pv, s.stride), pu), 2), s.texels);
+		return *reinterpret_cast<const float4*>(s.buf + ofs); }
 	const int power = s.power;
-	const float baseDim = static_cast<float>(1 << power);
-	int lod = level_of_detail(u[0] * baseDim, u[1] * baseDim, v[0] * baseDim, v[1] * baseDim);
-	lod = lod < 0 ? 0 : (lod > power ? power : lod);
 	if (s.kind == 0) {  // ..._P2_MIPMAP_WRAP_NEAREST (:63-109)
 		const float levelDim = static_cast<float>(1 << (power - lod));
 		const int32_t levelBeginRow = static_cast<int32_t>((0xfffffffeu << (power - lod)) & ((1u << (power + 1)) - 1u));
-		for (int l = 0; l < 4; ++l) {
-			const float fu = u[l] + 10.0F, fv = v[l] + 10.0F;
-			const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
-			const int32_t lx = cvtt(fru * levelDim);
-			const int32_t ly = cvtt((almostone - frv) * levelDim);
-			const int ofs = clamp_texel(shl(wadd(shl(wadd(levelBeginRow, ly), power), lx), 2), s.texels);
-			const float4 t = *reinterpret_cast<const float4*>(s.buf + ofs);
-			out[0][l] = t.x; out[1][l] = t.y; out[2][l] = t.z; out[3][l] = t.w; }
-		return; }
+		const float fu = u + 10.0F, fv = v + 10.0F;
+		const float fru = fu - itof(cvtt(fu)), frv = fv - itof(cvtt(fv));
+		const int32_t lx = cvtt(fru * levelDim);
+		const int32_t ly = cvtt((almostone - frv) * levelDim);
+		const int ofs = clamp_texel(shl(wadd(shl(wadd(levelBeginRow, ly), power), lx), 2), s.texels);
+		return *reinterpret_cast<const float4*>(s.buf + ofs); }
 	// ..._P2_MIPMAP_WRAP_LINEAR (:172-286)
 	const int levelDim = 1 << (power - lod);
 	const int32_t wrapMask = levelDim - 1;
 	const int32_t levelLastRow = static_cast<int32_t>((0xffffffffu << (power - lod)) & ((1u << (power + 1)) - 1u)) - 1;
-	for (int l = 0; l < 4; ++l) {
-		const float lx = u[l] * static_cast<float>(levelDim);
-		const float ly = v[l] * static_cast<float>(levelDim);
-		int32_t tx0 = cvtt(lx - 0.5F), ty0 = cvtt(ly - 0.5F);
-		int32_t tx1 = wadd(tx0, 1), ty1 = wadd(ty0, 1);
-		const float fx = (lx - itof(tx0)) - 0.5F;
-		const float fy = (ly - itof(ty0)) - 0.5F;
-		const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
-		const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
-		tx0 &= wrapMask; ty0 &= wrapMask; tx1 &= wrapMask; ty1 &= wrapMask;
-		const int32_t by0 = levelLastRow - ty0, by1 = levelLastRow - ty1;
-		const float4 p00 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx0)) << 2));
-		const float4 p10 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx1)) << 2));
-		const float4 p01 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx0)) << 2));
-		const float4 p11 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx1)) << 2));
-		out[0][l] = ((p00.x * w00 + p10.x * w10) + p01.x * w01) + p11.x * w11;
-		out[1][l] = ((p00.y * w00 + p10.y * w10) + p01.y * w01) + p11.y * w11;
-		out[2][l] = ((p00.z * w00 + p10.z * w10) + p01.z * w01) + p11.z * w11;
-		out[3][l] = ((p00.w * w00 + p10.w * w10) + p01.w * w01) + p11.w * w11; }}
+	const float lx = u * static_cast<float>(levelDim);
+	const float ly = v * static_cast<float>(levelDim);
+	int32_t tx0 = cvtt(lx - 0.5F), ty0 = cvtt(ly - 0.5F);
+	int32_t tx1 = wadd(tx0, 1), ty1 = wadd(ty0, 1);
+	const float fx = (lx - itof(tx0)) - 0.5F;
+	const float fy = (ly - itof(ty0)) - 0.5F;
+	const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
+	const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
+	tx0 &= wrapMask; ty0 &= wrapMask; tx1 &= wrapMask; ty1 &= wrapMask;
+	const int32_t by0 = levelLastRow - ty0, by1 = levelLastRow - ty1;
+	const float4 p00 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx0)) << 2));
+	const float4 p10 = *reinterpret_cast<const float4*>(s.buf + ((((by0 << power) + tx1)) << 2));
+	const float4 p01 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx0)) << 2));
+	const float4 p11 = *reinterpret_cast<const float4*>(s.buf + ((((by1 << power) + tx1)) << 2));
+	return make_float4(((p00.x * w00 + p10.x * w10) + p01.x * w01) + p11.x * w11,
+	                   ((p00.y * w00 + p10.y * w10) + p01.y * w01) + p11.y * w11,
+	                   ((p00.z * w00 + p10.z * w10) + p01.z * w01) + p11.z * w11,
+	                   ((p00.w * w00 + p10.w * w10) + p01.w * w01) + p11.w * w11); }
 
 // ---------------------------------------------------------------------------
 // shader programs, in the reference's program concept
-// (rglv_gpu_shaders.hxx:14-95): ShadeVertex per vertex, ShadeFragment per
-// 2x2 quad, varyings interpolated with the perspective-correct BP.
+// (rglv_gpu_shaders.hxx:14-95): ShadeVertex per vertex; ShadeFragment
+// evaluated here one lane at a time (lanes of a quad are independent except
+// for the texture LOD, which the caller computes from lanes 0 and 1 of the
+// quad); varyings are interpolated with the perspective-correct BP.
+// kUV is the index of the first texture-coordinate varying (-1: none).
 // ---------------------------------------------------------------------------
 __device__ inline float buf_at(const float* p, int idx, float dflt) { return p ? p[idx] : dflt; }
 
@@ -238,61 +260,60 @@ __device__ inline void load_pos(const DrawDesc& d, int idx, float p[4]) {
 
 // AmyProgram (viewer/shaders.hxx:69-161)
 struct ProgAmy {
-	static constexpr int kId = 4, kVary = 2;
+	static constexpr int kId = 4, kVary = 2, kUV = 0;
 	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
 		float p[4]; load_pos(d, idx, p);
 		xform4(d.vpm, p, pos);
 		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
 		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
-		float t[4][4];
-		sample_quad(d.tu0, vary[0], vary[1], t);
-		for (int c = 0; c < 3; ++c) for (int l = 0; l < 4; ++l) color[c][l] = t[c][l]; }};
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+		const float4 t = sample1(d.tu0, v[0], v[1], lod);
+		color[0] = t.x; color[1] = t.y; color[2] = t.z; }};
 
 // ManyProgram (viewer/shaders.hxx:482-583)
 struct ProgMany {
-	static constexpr int kId = 6, kVary = 2;
+	static constexpr int kId = 6, kVary = 2, kUV = -1;
 	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
 		float p[4]; load_pos(d, idx, p);
 		const float* imat = d.buffers[15] + 16 * iid;
 		float m[16];
+#pragma unroll
 		for (int k = 0; k < 16; ++k) m[k] = imat[k];
 		float p1[4]; xform4(m, p, p1);
 		xform4(d.vpm, p1, pos);
 		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
 		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
-		for (int l = 0; l < 4; ++l) { color[0][l] = vary[0][l]; color[1][l] = vary[1][l]; color[2][l] = d.uniforms[0]; }}};
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int, float color[3]) {
+		color[0] = v[0]; color[1] = v[1]; color[2] = d.uniforms[0]; }};
 
 // LambertTex (this project): textured, one directional light.
 // uniforms: [0..2] unit direction toward the light, [4] ambient, [5] diffuse
 struct ProgLambertTex {
-	static constexpr int kId = 80, kVary = 5;
+	static constexpr int kId = 80, kVary = 5, kUV = 3;
 	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
 		float p[4]; load_pos(d, idx, p);
 		xform4(d.vpm, p, pos);
 		vo[0] = buf_at(d.buffers[3], idx, 0.0F); vo[1] = buf_at(d.buffers[4], idx, 0.0F);
 		vo[2] = buf_at(d.buffers[5], idx, 1.0F);
 		vo[3] = buf_at(d.buffers[9], idx, 0.0F); vo[4] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
 		const float* u = d.uniforms;
-		float tex[4][4];
-		sample_quad(d.tu0, vary[3], vary[4], tex);
-		for (int l = 0; l < 4; ++l) {
-			float nx = vary[0][l], ny = vary[1][l], nz = vary[2][l];
-			normalize3(nx, ny, nz);
-			const float ndl = sse_max((nx*u[0] + ny*u[1]) + nz*u[2], 0.0F);
-			const float k = u[4] + u[5] * ndl;
-			color[0][l] = tex[0][l] * k; color[1][l] = tex[1][l] * k; color[2][l] = tex[2][l] * k; }}};
+		const float4 tex = sample1(d.tu0, v[3], v[4], lod);
+		float nx = v[0], ny = v[1], nz = v[2];
+		normalize3(nx, ny, nz);
+		const float ndl = sse_max((nx*u[0] + ny*u[1]) + nz*u[2], 0.0F);
+		const float k = u[4] + u[5] * ndl;
+		color[0] = tex.x * k; color[1] = tex.y * k; color[2] = tex.z * k; }};
 
 // ManyBlinnPhong (this project): instanced, textured, one point light.
 // uniforms: [0..2] eye, [4..6] light position, [8] ka, [9] kd, [10] ks
 struct ProgManyBlinnPhong {
-	static constexpr int kId = 81, kVary = 8;
+	static constexpr int kId = 81, kVary = 8, kUV = 6;
 	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
 		float p[4]; load_pos(d, idx, p);
 		const float* imat = d.buffers[15] + 16 * iid;
 		float m[16];
+#pragma unroll
 		for (int k = 0; k < 16; ++k) m[k] = imat[k];
 		float p1[4]; xform4(m, p, p1);
 		xform4(d.vpm, p1, pos);
@@ -301,28 +322,26 @@ struct ProgManyBlinnPhong {
 		vo[0] = p1[0]; vo[1] = p1[1]; vo[2] = p1[2];
 		vo[3] = wn[0]; vo[4] = wn[1]; vo[5] = wn[2];
 		vo[6] = buf_at(d.buffers[9], idx, 0.0F); vo[7] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment(const DrawDesc& d, const float vary[][4], float color[3][4]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
 		const float* u = d.uniforms;
-		float tex[4][4];
-		sample_quad(d.tu0, vary[6], vary[7], tex);
-		for (int l = 0; l < 4; ++l) {
-			const float px = vary[0][l], py = vary[1][l], pz = vary[2][l];
-			float nx = vary[3][l], ny = vary[4][l], nz = vary[5][l];
-			normalize3(nx, ny, nz);
-			float vx = u[0] - px, vy = u[1] - py, vz = u[2] - pz;
-			normalize3(vx, vy, vz);
-			float lx = u[4] - px, ly = u[5] - py, lz = u[6] - pz;
-			normalize3(lx, ly, lz);
-			float hx = lx + vx, hy = ly + vy, hz = lz + vz;
-			normalize3(hx, hy, hz);
-			const float ndl = sse_max((nx*lx + ny*ly) + nz*lz, 0.0F);
-			const float ndh = sse_max((nx*hx + ny*hy) + nz*hz, 0.0F);
-			float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;
-			const float kdiff = u[8] + u[9] * ndl;
-			const float kspec = u[10] * s;
-			color[0][l] = tex[0][l] * kdiff + kspec;
-			color[1][l] = tex[1][l] * kdiff + kspec;
-			color[2][l] = tex[2][l] * kdiff + kspec; }}};
+		const float4 tex = sample1(d.tu0, v[6], v[7], lod);
+		const float px = v[0], py = v[1], pz = v[2];
+		float nx = v[3], ny = v[4], nz = v[5];
+		normalize3(nx, ny, nz);
+		float vx = u[0] - px, vy = u[1] - py, vz = u[2] - pz;
+		normalize3(vx, vy, vz);
+		float lx = u[4] - px, ly = u[5] - py, lz = u[6] - pz;
+		normalize3(lx, ly, lz);
+		float hx = lx + vx, hy = ly + vy, hz = lz + vz;
+		normalize3(hx, hy, hz);
+		const float ndl = sse_max((nx*lx + ny*ly) + nz*lz, 0.0F);
+		const float ndh = sse_max((nx*hx + ny*hy) + nz*hz, 0.0F);
+		float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;
+		const float kdiff = u[8] + u[9] * ndl;
+		const float kspec = u[10] * s;
+		color[0] = tex.x * kdiff + kspec;
+		color[1] = tex.y * kdiff + kspec;
+		color[2] = tex.z * kdiff + kspec; }};
 
 __host__ __device__ inline int program_nvary(int pid) {
 	switch (pid) {

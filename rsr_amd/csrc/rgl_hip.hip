@@ -97,13 +97,14 @@ struct DevArray {
 		return e; }
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
-const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_vis", "k_resolve" };
-constexpr int kNumTimed = 7;
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_pair_setup", "k_vis", "k_resolve" };
+constexpr int kNumTimed = 8;
 
 }  // namespace
 
 struct rgl_gpu {
 	int device = 0;
+	int lastWorkItems = 0;          // visibility work items of the last completed frame
 	std::string guid;
 	std::string err;
 
@@ -124,7 +125,10 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, list;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist, vis;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist, workOff;
+	DevArray<unsigned long long> keys;
+	DevArray<VisRec> rec, crec;
+	size_t keysReady = 0;           // pixels of keys holding the sentinel
 	DevArray<DrawDesc> dDraws;
 	Counters* dCtr = nullptr;
 	Counters* hCtr = nullptr;       // pinned
@@ -160,7 +164,8 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &vis }) a->release();
+		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
+		keys.release(); rec.release(); crec.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
 		if (dCtr) (void)hipFree(dCtr);
@@ -335,6 +340,7 @@ struct rgl_gpu {
 		if ((e = bsum.ensure(nh / kScanChunk + 2)) != hipSuccess) return hipfail(e, "hipMalloc bsum");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
 		if ((e = list.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc list");
+		if ((e = rec.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
 		if (clipCap < std::min<long long>(S, 4096)) clipCap = static_cast<uint32_t>(std::min<long long>(S, 4096));
 		if ((e = clipq.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc clipq");
 		if ((e = clipqOf.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc clipqOf");
@@ -347,7 +353,11 @@ struct rgl_gpu {
 		if ((e = clipCursor.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
 		if ((e = clist.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clist");
-		if ((e = vis.ensure(static_cast<size_t>(f.W) * f.H)) != hipSuccess) return hipfail(e, "hipMalloc vis");
+		if ((e = crec.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
+		if ((e = workOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
+		const size_t px = static_cast<size_t>(f.W) * f.H;
+		if (px > keys.cap) keysReady = 0;
+		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
 		if (!dDraws.p) {
 			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
 			lastTable.clear(); }
@@ -419,7 +429,7 @@ struct rgl_gpu {
 				k_scan_bsums<<<1, 1024, 0, stream>>>(bsum.p, nb, &dCtr->pairs);
 				k_scan_down<<<nb, kScanThreads, 0, stream>>>(hist.p, nh, bsum.p, hist.p, &dCtr->pairs); }
 			else {
-				(void)hipMemsetAsync(hist.p, 0, sizeof(uint32_t) * (nh + 1), stream); }
+				k_fill_u32<<<1, 64, 0, stream>>>(hist.p, nh + 1, 0u); }
 		}
 		Mark(3);
 		if (f.units > 0) k_scatter<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(f, trect.p, hist.p, list.p, listCap, dCtr);
@@ -430,7 +440,7 @@ struct rgl_gpu {
 			case ProgMany::kId: LaunchClip<ProgMany>(di, f); break;
 			case ProgLambertTex::kId: LaunchClip<ProgLambertTex>(di, f); break;
 			default: LaunchClip<ProgManyBlinnPhong>(di, f); break; }}
-		k_clip_scan<<<1, 1024, 0, stream>>>(f, clipHist.p, clipOff.p, clipCursor.p, dCtr);
+		k_tile_scan<<<1, 1024, 0, stream>>>(f, hist.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
 		k_clip_scatter<<<64, 256, 0, stream>>>(f, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
 		Mark(5);
 		RasterArgs ra;
@@ -438,15 +448,24 @@ struct rgl_gpu {
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
-		ra.vis = vis.p;
-		if (f.nTiles > 0) k_vis<<<f.nTiles, kRasterThreads, 0, stream>>>(f, ra);
-		Mark(6);
+		ra.keys = keys.p; ra.workOff = workOff.p;
+		if (keysReady < keys.cap) {
+			// the resolve leaves every key at the sentinel; fill once after allocation
+			k_fill_u64<<<1024, 256, 0, stream>>>(keys.p, static_cast<long long>(keys.cap), kKeySentinel);
+			keysReady = keys.cap; }
+		if (f.nTiles > 0) {
+			const int setupBlocks = static_cast<int>(std::min<long long>((2LL * f.totalTris + 255) / 256 + 1, 4096));
+			k_pair_setup<<<dim3(setupBlocks, 2), 256, 0, stream>>>(f, ra, rec.p, crec.p);
+			Mark(6);
+			// one workgroup per work item, sized from the last completed frame
+			const int items = std::max<int>(lastWorkItems, f.nTiles * kQuadrants);
+			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(f, ra, rec.p, crec.p); }
+		Mark(7);
 		{
-			const int qw = (f.W + 1) / 2, qh = (f.H + 1) / 2;
-			const dim3 grid((qw + kResolveQx - 1) / kResolveQx, (qh + kResolveQy - 1) / kResolveQy);
+			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
 			k_resolve<<<grid, 256, 0, stream>>>(f, ra);
 		}
-		Mark(7);
+		Mark(8);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
 		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
@@ -468,6 +487,7 @@ struct rgl_gpu {
 		stats.clipped_tile_pairs = hCtr->clipPairs;
 		stats.fragments = static_cast<int64_t>(hCtr->fragments);
 		stats.overflow = hCtr->overflow;
+		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));
 		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
 		if (hCtr->overflow & 16u) return fail(RGL_E_STATE, "tile list held an invalid entry");

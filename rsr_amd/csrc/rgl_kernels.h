@@ -29,8 +29,10 @@ struct Counters {
 	uint32_t pairs;          // inline (triangle, tile) pairs, from the scan
 	uint32_t clipPairs;
 	uint32_t binned;         // triangles accepted into >= 1 tile
-	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index
-	uint32_t pad[3];
+	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index, 4 bad list entry
+	uint32_t workItems;      // visibility work items of the frame
+	uint32_t workNext;       // next work item to hand out
+	uint32_t pad;
 };
 
 // compact per-draw view used by the multi-draw kernels
@@ -396,34 +398,6 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 				for (int tx = tx0; tx <= tx1; ++tx)
 					atomicAdd(&clipHist[ty * f.tilesX + tx], 1u); }}}
 
-__global__ __launch_bounds__(1024) void k_clip_scan(FrameDesc f, const uint32_t* __restrict__ clipHist,
-		uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor, Counters* __restrict__ ctr) {
-	__shared__ uint32_t sh[1024 / 64 + 1];
-	uint32_t carry = 0;
-	for (int base = 0; base < f.nTiles; base += 1024) {
-		const int i = base + threadIdx.x;
-		const uint32_t v = i < f.nTiles ? clipHist[i] : 0u;
-		uint32_t tot;
-		const uint32_t ex = block_exclusive_scan(v, sh, tot);
-		if (i < f.nTiles) { clipOff[i] = carry + ex; clipCursor[i] = carry + ex; }
-		carry += tot; }
-	if (threadIdx.x == 0) { clipOff[f.nTiles] = carry; ctr->clipPairs = carry; }}
-
-__global__ __launch_bounds__(256) void k_clip_scatter(FrameDesc f, const uint8_t* __restrict__ fcount,
-		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
-		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
-	const uint32_t nq = min(ctr->clipCount, clipCap);
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-		const int nf = fcount[q];
-		for (int j = 0; j < nf; ++j) {
-			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
-			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
-			for (int ty = ty0; ty <= ty1; ++ty)
-				for (int tx = tx0; tx <= tx1; ++tx) {
-					const uint32_t pos = atomicAdd(&clipCursor[ty * f.tilesX + tx], 1u);
-					if (pos < clistCap) clist[pos] = q * kFanMax + j;
-					else atomicOr(&ctr->overflow, 4u); }}}}
-
 // ---------------------------------------------------------------------------
 // per-tile raster + resolve
 // ---------------------------------------------------------------------------
@@ -561,174 +535,394 @@ struct RasterArgs {
 	uint32_t clistCap;
 	uint32_t clipCap;
 	Counters* ctr;
-	uint32_t* vis;              // per-pixel winning slot, 0 = none
+	unsigned long long* keys;   // per-pixel visibility key; kKeySentinel between frames
+	const uint32_t* workOff;    // per-tile first work item [nTiles + 1]
 };
 
-template <class P>
-__device__ inline void shade_lanes(const DrawDesc& d, const TriSetup& t, const TriVerts& tr,
-                                   const float* __restrict__ vary, int kx, int ky, float color[3][4], float depth[4]) {
-	int32_t e1[4], e2[4], e3[4];
-	quad_edges(t, kx, ky, e1, e2, e3);
-	float bs[3][4], bp[3][4];
-	for (int l = 0; l < 4; ++l) {
-		bs[0][l] = itof(e2[l]) * t.scale;
-		bs[2][l] = itof(e1[l]) * t.scale;
-		bs[1][l] = (1.0F - bs[0][l]) - bs[2][l];
-		depth[l] = (bs[0][l]*tr.z[0] + bs[1][l]*tr.z[1]) + bs[2][l]*tr.z[2];
-		const float fragW = oneover((bs[0][l]*tr.w[0] + bs[1][l]*tr.w[1]) + bs[2][l]*tr.w[2]);
-		bp[0][l] = (tr.w[0] * bs[0][l]) * fragW;
-		bp[2][l] = (tr.w[2] * bs[2][l]) * fragW;
-		bp[1][l] = (1.0F - bp[0][l]) - bp[2][l]; }
-	float vv[P::kVary][4];
-	const float* v0 = vary + tr.vary[0] * kMaxVary;
-	const float* v1 = vary + tr.vary[1] * kMaxVary;
-	const float* v2 = vary + tr.vary[2] * kMaxVary;
-	for (int k = 0; k < P::kVary; ++k) {
-		const float a = v0[k], b = v1[k], c = v2[k];
-		for (int l = 0; l < 4; ++l) vv[k][l] = (bp[0][l]*a + bp[1][l]*b) + bp[2][l]*c; }
-	P::fragment(d, vv, color); }
+constexpr unsigned long long kKeySentinel = ~0ull;
+constexpr int kSegPairs = 512;       // (triangle, tile) pairs per visibility work item
+constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
+constexpr int kQuadrants = 4;
 
-__device__ inline void shade_dispatch(const DrawDesc& d, const TriSetup& t, const TriVerts& tr,
-                                      const float* vary, int kx, int ky, float color[3][4], float depth[4]) {
-	switch (d.program) {
-	case ProgAmy::kId: shade_lanes<ProgAmy>(d, t, tr, vary, kx, ky, color, depth); break;
-	case ProgMany::kId: shade_lanes<ProgMany>(d, t, tr, vary, kx, ky, color, depth); break;
-	case ProgLambertTex::kId: shade_lanes<ProgLambertTex>(d, t, tr, vary, kx, ky, color, depth); break;
-	default: shade_lanes<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, color, depth); break; }}
+// visibility key of a covered fragment (see the header comment)
+__device__ inline unsigned long long frag_key(float dz, uint32_t slot, bool leq) {
+	return (static_cast<unsigned long long>(depth_order(dz)) << 32) |
+	       (leq ? static_cast<unsigned long long>(~slot) : static_cast<unsigned long long>(slot)); }
 
-__global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs a) {
-	__shared__ unsigned long long keys[kMaxTileDim * kMaxTileDim];
-	__shared__ TriSetup ts[kRasterThreads];
+// the key that a fragment must beat to pass the depth test against the
+// clear depth: LESS needs depth < clear, LEQUAL depth <= clear (slots >= 1)
+__device__ inline unsigned long long clear_key(const FrameDesc& f) {
+	return (static_cast<unsigned long long>(depth_order(f.clearDepth)) << 32) | (f.depthFunc == 1 ? 0xffffffffull : 0ull); }
+
+__device__ inline uint32_t key_slot(unsigned long long k, bool leq) {
+	const uint32_t lo32 = static_cast<uint32_t>(k);
+	return leq ? ~lo32 : lo32; }
+
+// per-tile work items: each 64x64 tile is rasterised as four 32x32
+// quadrants, and the tile's inline and clipped pairs are cut into segments
+// of kSegPairs; a work item is (tile, segment, quadrant).  Also the
+// clipped-list offsets.
+__global__ __launch_bounds__(1024) void k_tile_scan(FrameDesc f, const uint32_t* __restrict__ off, uint32_t listCap,
+		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor,
+		uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
+	__shared__ uint32_t sh[1024 / 64 + 1];
+	uint32_t carry = 0, wcarry = 0;
+	for (int base = 0; base < f.nTiles; base += 1024) {
+		const int i = base + threadIdx.x;
+		uint32_t v = 0, items = 0;
+		if (i < f.nTiles) {
+			v = clipHist[i];
+			const uint32_t b = min(off[static_cast<long long>(i) * f.units], listCap);
+			const uint32_t e = min(off[static_cast<long long>(i + 1) * f.units], listCap);
+			items = kQuadrants * max(1u, (e - b + v + kSegPairs - 1) / kSegPairs); }
+		uint32_t tot, wtot;
+		const uint32_t ex = block_exclusive_scan(v, sh, tot);
+		const uint32_t wex = block_exclusive_scan(items, sh, wtot);
+		if (i < f.nTiles) { clipOff[i] = carry + ex; clipCursor[i] = carry + ex; workOff[i] = wcarry + wex; }
+		carry += tot; wcarry += wtot; }
+	if (threadIdx.x == 0) {
+		clipOff[f.nTiles] = carry; workOff[f.nTiles] = wcarry;
+		ctr->clipPairs = carry; ctr->workItems = wcarry; ctr->workNext = 0; }}
+
+__global__ __launch_bounds__(256) void k_clip_scatter(FrameDesc f, const uint8_t* __restrict__ fcount,
+		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
+		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
+	const uint32_t nq = min(ctr->clipCount, clipCap);
+	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+		const int nf = fcount[q];
+		for (int j = 0; j < nf; ++j) {
+			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
+			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+			for (int ty = ty0; ty <= ty1; ++ty)
+				for (int tx = tx0; tx <= tx1; ++tx) {
+					const uint32_t pos = atomicAdd(&clipCursor[ty * f.tilesX + tx], 1u);
+					if (pos < clistCap) clist[pos] = q * kFanMax + j;
+					else atomicOr(&ctr->overflow, 4u); }}}}
+
+// ---------------------------------------------------------------------------
+// pair setup: one thread per (triangle, tile) pair of the inline and
+// clipped lists computes the rasteriser setup of that triangle against that
+// tile — VTriangleRasterizer (rglv_triangle.hxx:206-240) for inline
+// triangles, TriangleRasterizer (:84-131) for clipped fans — and stores it
+// as a 64-byte record at the pair's list position.  Records of a tile are
+// contiguous and in list order.
+// ---------------------------------------------------------------------------
+struct VisRec {
+	int32_t c1, c2, c3;
+	int32_t dx12, dy12, dx23, dy23, dx31, dy31;
+	uint32_t minxy;       // minx | miny << 16 (even, tile-clamped)
+	uint32_t nq;          // quads across | quads down << 16
+	float scale;
+	float z0, z1, z2;
+	uint32_t slot;
+};
+static_assert(sizeof(VisRec) == 64, "VisRec is four 16-byte loads");
+
+struct VisDraw { long long triBase; uint32_t slotBase, nTri; };
+
+__device__ inline void load_visdraws(VisDraw* vd, const DrawDesc* draws, int nDraws) {
+	for (int k = threadIdx.x; k < nDraws; k += blockDim.x) {
+		const DrawDesc& d = draws[k];
+		vd[k].triBase = d.triBase; vd[k].slotBase = d.slotBase;
+		vd[k].nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances); }}
+
+__device__ inline int find_visdraw(const VisDraw* vd, int nDraws, long long s) {
+	int d = 0;
+	for (int k = 1; k < nDraws; ++k) if (vd[k].triBase <= s) d = k;
+	return d; }
+
+__device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_t slot) {
+	VisRec v;
+	v.c1 = t.c1; v.c2 = t.c2; v.c3 = t.c3;
+	v.dx12 = t.dx12; v.dy12 = t.dy12; v.dx23 = t.dx23; v.dy23 = t.dy23; v.dx31 = t.dx31; v.dy31 = t.dy31;
+	v.minxy = static_cast<uint32_t>(t.minx & 0xffff) | (static_cast<uint32_t>(t.miny) << 16);
+	v.nq = static_cast<uint32_t>(t.nqx) | (static_cast<uint32_t>(t.nqy) << 16);
+	v.scale = t.scale; v.z0 = tr.z[0]; v.z1 = tr.z[1]; v.z2 = tr.z[2]; v.slot = slot;
+	return v; }
+
+constexpr int kMaxTilesLds = 2048;   // tile offset tables cached in LDS up to this many tiles
+
+__global__ __launch_bounds__(256) void k_pair_setup(FrameDesc f, RasterArgs a, VisRec* __restrict__ rec, VisRec* __restrict__ crec) {
+	__shared__ uint32_t toff[kMaxTilesLds + 1];
+	__shared__ VisDraw vd[kMaxDraws];
+	load_visdraws(vd, a.draws, f.nDraws);
+	const bool clippedPass = blockIdx.y == 1;
+	const uint32_t total = clippedPass ? min(a.clipOff[f.nTiles], a.clistCap)
+	                                   : min(a.off[static_cast<long long>(f.nTiles) * f.units], a.listCap);
+	const bool lds = f.nTiles <= kMaxTilesLds;
+	if (lds)
+		for (int t = threadIdx.x; t <= f.nTiles; t += 256)
+			toff[t] = clippedPass ? a.clipOff[t] : a.off[static_cast<long long>(t) * f.units];
+	__syncthreads();
+	for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
+		int lo = 0, hi = f.nTiles - 1;  // last tile whose list starts at or before p
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			const uint32_t o = lds ? toff[mid] : (clippedPass ? a.clipOff[mid] : a.off[static_cast<long long>(mid) * f.units]);
+			if (o <= p) lo = mid; else hi = mid - 1; }
+		const Rect R = tile_rect(f, lo);
+		TriVerts tr;
+		TriSetup t;
+		if (!clippedPass) {
+			uint32_t s = a.list[p];
+			if (s >= f.totalTris) { atomicOr(&a.ctr->overflow, 16u); s = 0; }
+			const int di = find_visdraw(vd, f.nDraws, s);
+			inline_verts(a.tvert[s], a.vpos, tr);
+			setup_vector(R, tr.X, tr.Y, t);
+			rec[p] = make_rec(t, tr, vd[di].slotBase + static_cast<uint32_t>(s - vd[di].triBase)); }
+		else {
+			uint32_t fan = a.clist[p];
+			if (fan / kFanMax >= a.clipCap) { atomicOr(&a.ctr->overflow, 16u); fan = 0; }
+			const uint32_t q = fan / kFanMax, j = fan - q * kFanMax;
+			const uint32_t s = a.clipq[q];
+			const int di = find_visdraw(vd, f.nDraws, s);
+			clipped_verts(fan, a.cvpos, tr);
+			setup_scalar(R, tr.X, tr.Y, t);
+			crec[p] = make_rec(t, tr, vd[di].slotBase + vd[di].nTri + static_cast<uint32_t>(s - vd[di].triBase) * kFanMax + j); }}}
+
+// ---------------------------------------------------------------------------
+// visibility: a persistent kernel over (tile, segment, quadrant) work
+// items.  Each item holds a 32x32 quadrant's keys in LDS, takes up to
+// kSegPairs of its tile's setup records (inline list in submission order,
+// then clipped fans; set up against the full tile rectangle, the
+// reference's edge start values), rasterises the triangles' quads that
+// fall in the quadrant with ds_min_u64, and merges the quadrant into the
+// per-pixel key buffer: a plain store when the tile is one segment, a
+// 64-bit atomic min otherwise.
+// ---------------------------------------------------------------------------
+struct VisSetup {
+	int32_t c1, c2, c3;   // edge values at the triangle's first quad in the quadrant
+	int32_t x0, y0;       // that quad's top-left pixel, quadrant-relative
+	int32_t dx12, dy12, dx23, dy23, dx31, dy31;
+	uint32_t nx;          // quads across inside the quadrant
+	float invNx;          // 1 / nx: row = floor((k + 0.5) / nx), exact for k < 2^16
+	float scale;
+	float z0, z1, z2;
+	uint32_t slot;
+	uint32_t small;       // 1: every quad step fits 23 bits, 24-bit multiplies are exact
+	uint32_t pad;
+};
+static_assert(sizeof(VisSetup) % 16 == 0, "VisSetup rows are read with aligned 16-byte loads");
+
+// a + k * s with 32-bit wrap; k < 2^8 here, so with |s| < 2^23 the
+// product fits the 24-bit multiplier exactly
+__device__ inline int32_t step_mad(int32_t a, int32_t k, int32_t s, bool small) {
+	return small ? wadd(a, __mul24(k, s)) : wadd(a, wmul(k, s)); }
+
+constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per pass
+
+// visibility work items run one per workgroup; the grid is sized from the
+// previous frame's item count and strides over any remainder, so no
+// work-queue counter is shared between workgroups
+__global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs a, const VisRec* __restrict__ rec,
+                                                       const VisRec* __restrict__ crec) {
+	__shared__ unsigned long long keys[kSubDim * kSubDim];
+	__shared__ VisSetup ts[kRasterThreads];
 	__shared__ uint32_t qoff[kRasterThreads + 1];
 	__shared__ uint32_t scan_sh[kRasterThreads / 64 + 1];
-	__shared__ DrawLite dl[kMaxDraws];
-	const int tile = blockIdx.x;
-	const Rect R = tile_rect(f, tile);
+	__shared__ uint32_t wofs[kMaxTilesLds + 1];
+	__shared__ uint8_t owner[kQuadWindow];
 	const bool leq = f.depthFunc == 1;
-	const unsigned long long initKey = (static_cast<unsigned long long>(depth_order(f.clearDepth)) << 32) |
-	                                   (leq ? 0xffffffffull : 0ull);
-	const int tileArea = f.tileW * f.tileH;
-	for (int i = threadIdx.x; i < tileArea; i += kRasterThreads) keys[i] = initKey;
-	load_lite(dl, a.draws, f.nDraws);
-	__syncthreads();
-	unsigned long long frags = 0;
-
-	for (int pass = 0; pass < 2 && f.drawEnabled; ++pass) {
-		uint32_t beg, end;
-		if (pass == 0) {
-			beg = a.off[static_cast<long long>(tile) * f.units];
-			end = a.off[static_cast<long long>(tile + 1) * f.units];
-			end = min(end, a.listCap); }
-		else {
-			beg = a.clipOff[tile]; end = min(a.clipOff[tile + 1], a.clistCap); }
-		for (uint32_t b0 = beg; b0 < end; b0 += kRasterThreads) {
-			const uint32_t e = b0 + threadIdx.x;
+	const bool lds = f.nTiles <= kMaxTilesLds;
+	if (lds) for (int t = threadIdx.x; t <= f.nTiles; t += kRasterThreads) wofs[t] = a.workOff[t];
+	uint32_t frags = 0;
+	const uint32_t nItems = a.ctr->workItems;
+	for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
+		__syncthreads();
+		int lo = 0, hi = f.nTiles - 1;  // last tile with workOff <= item
+		while (lo < hi) {
+			const int mid = (lo + hi + 1) >> 1;
+			if ((lds ? wofs[mid] : a.workOff[mid]) <= item) lo = mid; else hi = mid - 1; }
+		const int tile = lo;
+		const uint32_t w0 = lds ? wofs[tile] : a.workOff[tile];
+		const uint32_t w1 = lds ? wofs[tile + 1] : a.workOff[tile + 1];
+		const uint32_t local = item - w0;
+		const uint32_t nseg = (w1 - w0) / kQuadrants;
+		const uint32_t seg = local / kQuadrants, quad = local - seg * kQuadrants;
+		const Rect R = tile_rect(f, tile);
+		Rect S;
+		S.x0 = R.x0 + (quad & 1) * kSubDim; S.y0 = R.y0 + (quad >> 1) * kSubDim;
+		S.x1 = imin(S.x0 + kSubDim, R.x1); S.y1 = imin(S.y0 + kSubDim, R.y1);
+		if (S.x0 >= S.x1 || S.y0 >= S.y1) continue;   // quadrant outside the screen (uniform)
+		for (int i = threadIdx.x; i < kSubDim * kSubDim; i += kRasterThreads) keys[i] = kKeySentinel;
+		const uint32_t ib = min(a.off[static_cast<long long>(tile) * f.units], a.listCap);
+		const uint32_t ie = min(a.off[static_cast<long long>(tile + 1) * f.units], a.listCap);
+		const uint32_t cb = min(a.clipOff[tile], a.clistCap);
+		const uint32_t ce = min(a.clipOff[tile + 1], a.clistCap);
+		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
+		const uint32_t pbeg = seg * kSegPairs, pend = min(pbeg + kSegPairs, ntot);
+		__syncthreads();
+		for (uint32_t b0 = pbeg; f.drawEnabled && b0 < pend; b0 += kRasterThreads) {
+			const uint32_t p = b0 + threadIdx.x;
 			uint32_t nq = 0;
-			if (e < end) {
-				TriVerts tr;
-				TriSetup t;
-				if (pass == 0) {
-					uint32_t s = a.list[e];
-					if (s >= f.totalTris) { atomicOr(&a.ctr->overflow, 16u); s = 0; }
-					const int di = find_draw_ll(dl, f.nDraws, s);
-					inline_verts(a.tvert[s], a.vpos, tr);
-					setup_vector(R, tr.X, tr.Y, t);
-					t.slot = dl[di].slotBase + static_cast<uint32_t>(s - dl[di].triBase); }
-				else {
-					uint32_t fan = a.clist[e];
-					if (fan / kFanMax >= a.clipCap) { atomicOr(&a.ctr->overflow, 16u); fan = 0; }
-					const uint32_t q = fan / kFanMax, j = fan - q * kFanMax;
-					const uint32_t s = a.clipq[q];
-					const int di = find_draw_ll(dl, f.nDraws, s);
-					clipped_verts(fan, a.cvpos, tr);
-					setup_scalar(R, tr.X, tr.Y, t);
-					const uint32_t nTri = static_cast<uint32_t>(dl[di].tris) * static_cast<uint32_t>(dl[di].instances);
-					t.slot = dl[di].slotBase + nTri + static_cast<uint32_t>(s - dl[di].triBase) * kFanMax + j; }
-				t.z0 = tr.z[0]; t.z1 = tr.z[1]; t.z2 = tr.z[2];
-				ts[threadIdx.x] = t;
-				nq = static_cast<uint32_t>(t.nqx) * static_cast<uint32_t>(t.nqy); }
+			if (p < pend) {
+				const VisRec r = p < ninl ? rec[ib + p] : crec[cb + (p - ninl)];
+				const int minx = static_cast<int>(r.minxy & 0xffffu), miny = static_cast<int>(r.minxy >> 16);
+				const int nqx = static_cast<int>(r.nq & 0xffffu), nqy = static_cast<int>(r.nq >> 16);
+				// the triangle's quads inside the quadrant
+				const int kx0 = imax(0, (S.x0 - minx) >> 1), ky0 = imax(0, (S.y0 - miny) >> 1);
+				const int kx1 = imin(nqx, (S.x1 - minx + 1) >> 1), ky1 = imin(nqy, (S.y1 - miny + 1) >> 1);
+				const int nx = imax(0, kx1 - kx0), ny = imax(0, ky1 - ky0);
+				VisSetup v;
+				// edge values at the first quad, stepped exactly as the rasteriser loops do
+				v.c1 = wadd(wadd(r.c1, wmul(kx0, wmul(r.dy12, 2))), wmul(ky0, wmul(r.dx12, 2)));
+				v.c2 = wadd(wadd(r.c2, wmul(kx0, wmul(r.dy23, 2))), wmul(ky0, wmul(r.dx23, 2)));
+				v.c3 = wadd(wadd(r.c3, wmul(kx0, wmul(r.dy31, 2))), wmul(ky0, wmul(r.dx31, 2)));
+				v.dx12 = r.dx12; v.dy12 = r.dy12; v.dx23 = r.dx23; v.dy23 = r.dy23; v.dx31 = r.dx31; v.dy31 = r.dy31;
+				const int32_t lim = 1 << 21;   // 2 * |d| < 2^22
+				v.small = (r.dx12 > -lim && r.dx12 < lim && r.dy12 > -lim && r.dy12 < lim &&
+				           r.dx23 > -lim && r.dx23 < lim && r.dy23 > -lim && r.dy23 < lim &&
+				           r.dx31 > -lim && r.dx31 < lim && r.dy31 > -lim && r.dy31 < lim) ? 1u : 0u;
+				v.pad = 0u;
+				v.x0 = minx + 2 * kx0 - S.x0; v.y0 = miny + 2 * ky0 - S.y0;
+				v.nx = static_cast<uint32_t>(nx);
+				v.invNx = nx > 0 ? 1.0F / static_cast<float>(nx) : 0.0F;
+				v.scale = r.scale; v.z0 = r.z0; v.z1 = r.z1; v.z2 = r.z2; v.slot = r.slot;
+				ts[threadIdx.x] = v;
+				nq = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny); }
 			uint32_t total;
 			const uint32_t ex = block_exclusive_scan(nq, scan_sh, total);
 			qoff[threadIdx.x] = ex;
 			if (threadIdx.x == 0) qoff[kRasterThreads] = total;
-			__syncthreads();
-			for (uint32_t it = threadIdx.x; it < total; it += kRasterThreads) {
-				int lo = 0, hi = kRasterThreads - 1;  // last j with qoff[j] <= it
-				while (lo < hi) {
-					const int mid = (lo + hi + 1) >> 1;
-					if (qoff[mid] <= it) lo = mid; else hi = mid - 1; }
-				const TriSetup& t = ts[lo];
-				const uint32_t k = it - qoff[lo];
-				const int ky = static_cast<int>(k / static_cast<uint32_t>(t.nqx));
-				const int kx = static_cast<int>(k - static_cast<uint32_t>(ky) * static_cast<uint32_t>(t.nqx));
-				int32_t e1[4], e2[4], e3[4];
-				quad_edges(t, kx, ky, e1, e2, e3);
-				const int x = t.minx + 2 * kx, y = t.miny + 2 * ky;
-				for (int l = 0; l < 4; ++l) {
-					if ((e1[l] | e2[l] | e3[l]) < 0) continue;
-					++frags;
-					const float bx = itof(e2[l]) * t.scale;
-					const float bz = itof(e1[l]) * t.scale;
-					const float by = (1.0F - bx) - bz;
-					const float dz = (bx*t.z0 + by*t.z1) + bz*t.z2;
-					if (dz != dz) continue;
-					const unsigned long long key = (static_cast<unsigned long long>(depth_order(dz)) << 32) |
-					                               (leq ? static_cast<unsigned long long>(~t.slot) : static_cast<unsigned long long>(t.slot));
-					const int li = (y + (l >> 1) - R.y0) * f.tileW + (x + (l & 1) - R.x0);
-					if (key < keys[li]) atomicMin(&keys[li], key); }}
-			__syncthreads(); }}
-
-	// ---- write each pixel's winning slot (0 = nothing drawn) ----
-	const int rw = R.x1 - R.x0, rh = R.y1 - R.y0;
-	for (int i = threadIdx.x; i < rw * rh; i += kRasterThreads) {
-		const int ly = i / rw, lx = i - ly * rw;
-		const unsigned long long k = keys[ly * f.tileW + lx];
-		const uint32_t lo32 = static_cast<uint32_t>(k);
-		a.vis[static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = (k == initKey) ? 0u : (leq ? ~lo32 : lo32); }
-
+			for (uint32_t win0 = 0; win0 < total; win0 += kQuadWindow) {
+				// owner table: which triangle each quad item of the window belongs to
+				__syncthreads();
+				{
+					const uint32_t s0 = max(ex, win0), s1 = min(ex + nq, win0 + kQuadWindow);
+					for (uint32_t q = s0; q < s1; ++q) owner[q - win0] = static_cast<uint8_t>(threadIdx.x);
+				}
+				__syncthreads();
+				const uint32_t wend = min(total, win0 + kQuadWindow);
+				for (uint32_t it = win0 + threadIdx.x; it < wend; it += kRasterThreads) {
+					const int j = owner[it - win0];
+					const VisSetup& v = ts[j];
+					const uint32_t k = it - qoff[j];
+					const uint32_t kyr = static_cast<uint32_t>((static_cast<float>(k) + 0.5F) * v.invNx);
+					const uint32_t kxr = k - kyr * v.nx;
+					const bool small = v.small != 0u;
+					const int32_t kx = static_cast<int32_t>(kxr), ky = static_cast<int32_t>(kyr);
+					const int32_t q1 = step_mad(step_mad(v.c1, kx, shl(v.dy12, 1), small), ky, shl(v.dx12, 1), small);
+					const int32_t q2 = step_mad(step_mad(v.c2, kx, shl(v.dy23, 1), small), ky, shl(v.dx23, 1), small);
+					const int32_t q3 = step_mad(step_mad(v.c3, kx, shl(v.dy31, 1), small), ky, shl(v.dx31, 1), small);
+					const int lx = v.x0 + 2 * kx, ly = v.y0 + 2 * ky;
+#pragma unroll
+					for (int l = 0; l < 4; ++l) {
+						// lanes (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
+						const int32_t e1 = (l & 2) ? wadd((l & 1) ? wadd(q1, v.dx12) : q1, (l & 1) ? v.dy12 : v.dx12) : ((l & 1) ? wadd(q1, v.dy12) : q1);
+						const int32_t e2 = (l & 2) ? wadd((l & 1) ? wadd(q2, v.dx23) : q2, (l & 1) ? v.dy23 : v.dx23) : ((l & 1) ? wadd(q2, v.dy23) : q2);
+						const int32_t e3 = (l & 2) ? wadd((l & 1) ? wadd(q3, v.dx31) : q3, (l & 1) ? v.dy31 : v.dx31) : ((l & 1) ? wadd(q3, v.dy31) : q3);
+						if ((e1 | e2 | e3) < 0) continue;
+						++frags;
+						const float bx = itof(e2) * v.scale;
+						const float bz = itof(e1) * v.scale;
+						const float by = (1.0F - bx) - bz;
+						const float dz = (bx*v.z0 + by*v.z1) + bz*v.z2;
+						if (dz != dz) continue;
+						const unsigned long long key = frag_key(dz, v.slot, leq);
+						const int li = (ly + (l >> 1)) * kSubDim + (lx + (l & 1));
+						if (key < keys[li]) atomicMin(&keys[li], key); }}}
+			__syncthreads(); }
+		// merge the quadrant into the frame's key buffer
+		const int rw = S.x1 - S.x0, rh = S.y1 - S.y0;
+		for (int i = threadIdx.x; i < rw * rh; i += kRasterThreads) {
+			const int ly = i / rw, lx = i - ly * rw;
+			const unsigned long long k = keys[ly * kSubDim + lx];
+			unsigned long long* dst = a.keys + static_cast<long long>(S.y0 + ly) * f.W + S.x0 + lx;
+			if (nseg == 1) *dst = k;
+			else if (k != kKeySentinel) atomicMin(dst, k); }}
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
-	if ((threadIdx.x & 63) == 0 && frags) atomicAdd(&a.ctr->fragments, frags); }
+	if ((threadIdx.x & 63) == 0 && frags) atomicAdd(&a.ctr->fragments, static_cast<unsigned long long>(frags)); }
 
+// one lane of a quad of a triangle setup: perspective-correct barycentrics
+// BP and depth (TriangleProgram::Render, rglv_gpu_impl.hxx:169-191)
+struct LaneBary { float bp0, bp1, bp2, depth; };
+
+__device__ inline LaneBary lane_bary(const TriSetup& t, const TriVerts& tr, int32_t e1, int32_t e2, const uint32_t* rcpTab) {
+	LaneBary r;
+	const float b0 = itof(e2) * t.scale;
+	const float b2 = itof(e1) * t.scale;
+	const float b1 = (1.0F - b0) - b2;
+	r.depth = (b0*tr.z[0] + b1*tr.z[1]) + b2*tr.z[2];
+	const float fragW = oneover_tab((b0*tr.w[0] + b1*tr.w[1]) + b2*tr.w[2], rcpTab);
+	r.bp0 = (tr.w[0] * b0) * fragW;
+	r.bp2 = (tr.w[2] * b2) * fragW;
+	r.bp1 = (1.0F - r.bp0) - r.bp2;
+	return r; }
+
+__device__ inline float lane_interp(const LaneBary& b, const float* v0, const float* v1, const float* v2, int k) {
+	return (b.bp0*v0[k] + b.bp1*v1[k]) + b.bp2*v2[k]; }
+
+// shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle
+template <class P>
+__device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* __restrict__ vary,
+                                   int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
+	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
+	// lane offsets of the rasteriser's quad: (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
+	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
+	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
+	const float* v0 = vary + tr.vary[0] * kMaxVary;
+	const float* v1 = vary + tr.vary[1] * kMaxVary;
+	const float* v2 = vary + tr.vary[2] * kMaxVary;
+	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
+	depth = own.depth;
+	float vv[P::kVary];
+#pragma unroll
+	for (int k = 0; k < P::kVary; ++k) vv[k] = lane_interp(own, v0, v1, v2, k);
+	int lod = 0;
+	if constexpr (P::kUV >= 0) {
+		// the quad's LOD comes from the texture coordinates of lanes 0 and 1
+		const LaneBary l0 = lane_bary(t, tr, q1, q2, rcpTab);
+		const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
+		lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
+		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
+	P::fragment1(d, vv, lod, color); }
+
+__device__ inline void shade_dispatch(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+                                      int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+	switch (d.program) {
+	case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+	case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+	case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+	default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}
 
 // ---------------------------------------------------------------------------
-// resolve: one thread per 2x2 quad of the screen.  Each pixel's winning slot
-// is shaded by recomputing its triangle's quad exactly as the reference's
-// rasteriser produced it in that pixel's tile (tile-clamped edge start,
-// barycentric scale, lanes 0/1 for the texture LOD), then post-shaded and
-// sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104), and colour, depth
-// and primitive id are written once.
+// resolve: one thread per pixel.  The pixel's winning slot is shaded by
+// recomputing its triangle's quad exactly as the reference's rasteriser
+// produced it in the pixel's tile (tile-clamped edge start, barycentric
+// scale, texture LOD from lanes 0 and 1), then post-shaded and
+// sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104).  Colour, depth
+// and primitive id are written once; the key is reset for the next frame.
 // ---------------------------------------------------------------------------
-constexpr int kResolveQx = 32, kResolveQy = 8;   // 256 threads = 32 x 8 quads
+constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
 
 __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 	__shared__ DrawLite dl[kMaxDraws];
+	__shared__ uint32_t rcpTab[2048];
+	__shared__ uint32_t srgbTab[104];
 	load_lite(dl, a.draws, f.nDraws);
+	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
+	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
-	const int qx = blockIdx.x * kResolveQx + (threadIdx.x % kResolveQx);
-	const int qy = blockIdx.y * kResolveQy + (threadIdx.x / kResolveQx);
-	const int x = 2 * qx, y = 2 * qy;
+	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
+	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
 	if (x >= f.W || y >= f.H) return;
-	const int tx = x / f.tileW, ty = y / f.tileH;
-	Rect R;
-	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
-	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
-	uint32_t win[4];
-	for (int l = 0; l < 4; ++l) {
-		const int px = x + (l & 1), py = y + (l >> 1);
-		win[l] = (px < f.W && py < f.H) ? a.vis[static_cast<long long>(py) * f.W + px] : 0u; }
-	float col[3][4], dep[4];
-	uint32_t prim[4];
-	for (int l = 0; l < 4; ++l) {
-		col[0][l] = f.clearColor[0]; col[1][l] = f.clearColor[1]; col[2][l] = f.clearColor[2];
-		dep[l] = f.clearDepth; prim[l] = 0xffffffffu; }
-	uint32_t done = 0;
-	for (int l = 0; l < 4; ++l) {
-		if (win[l] == 0u || (done & (1u << l))) continue;
-		const uint32_t slot = win[l];
+	const long long o = static_cast<long long>(y) * f.W + x;
+	const unsigned long long k = a.keys[o];
+	a.keys[o] = kKeySentinel;
+	float col[3] = { f.clearColor[0], f.clearColor[1], f.clearColor[2] };
+	float dep = f.clearDepth;
+	uint32_t prim = 0xffffffffu;
+	if (k < clear_key(f)) {
+		const uint32_t slot = key_slot(k, f.depthFunc == 1);
 		const int di = find_draw_slot(dl, f.nDraws, slot);
 		const DrawLite& d = dl[di];
 		const uint32_t rel = slot - d.slotBase;
 		const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
+		const int tx = x / f.tileW, ty = y / f.tileH;
+		Rect R;
+		R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
+		R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
 		TriVerts tr;
 		TriSetup t;
 		uint32_t srcTri;
@@ -746,30 +940,28 @@ __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 			clipped_verts(cq * kFanMax + j, a.cvpos, tr);
 			setup_scalar(R, tr.X, tr.Y, t);
 			vary = a.cvvary; }
-		const int kx = (x - t.minx) >> 1, ky = (y - t.miny) >> 1;
-		float c4[3][4], d4[4];
-		shade_dispatch(a.draws[di], t, tr, vary, kx, ky, c4, d4);
-		for (int m = l; m < 4; ++m) {
-			if (win[m] != slot) continue;
-			done |= 1u << m;
-			col[0][m] = c4[0][m]; col[1][m] = c4[1][m]; col[2][m] = c4[2][m];
-			dep[m] = d4[m];
-			prim[m] = d.primBase + srcTri; }}
-	for (int l = 0; l < 4; ++l) {
-		const int px = x + (l & 1), py = y + (l >> 1);
-		if (px >= f.W || py >= f.H) continue;
-		const long long o = static_cast<long long>(py) * f.W + px;
-		if (f.outColor) {
-			float r = col[0][l], g = col[1][l], b = col[2][l];
-			if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
-			f.outColor[o] = f.gamma ? ((to_srgb8(r) << 16) | (to_srgb8(g) << 8) | to_srgb8(b))
-			                        : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
-		if (f.outDepth) f.outDepth[o] = dep[l];
-		if (f.outPrim) f.outPrim[o] = prim[l]; }}
+		const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
+		const int lane = (x & 1) | ((y & 1) << 1);
+		shade_dispatch(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+		prim = d.primBase + srcTri; }
+	if (f.outColor) {
+		float r = col[0], g = col[1], b = col[2];
+		if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
+		f.outColor[o] = f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
+		                        : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+	if (f.outDepth) f.outDepth[o] = dep;
+	if (f.outPrim) f.outPrim[o] = prim; }
+
+__global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
+	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
+
+__global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
+	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
 
 __global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i == 0) { ctr->fragments = 0; ctr->clipCount = 0; ctr->pairs = 0; ctr->clipPairs = 0; ctr->binned = 0; ctr->overflow = 0; }
+	if (i == 0) { ctr->fragments = 0; ctr->clipCount = 0; ctr->pairs = 0; ctr->clipPairs = 0; ctr->binned = 0; ctr->overflow = 0;
+	              ctr->workItems = 0; ctr->workNext = 0; }
 	for (int t = i; t < nTiles; t += gridDim.x * blockDim.x) clipHist[t] = 0u; }
 
 }  // namespace rgl
