@@ -111,23 +111,36 @@ def main():
     mtris = tris / (ms * 1e-3) / 1e6
     gfrag = st["fragments"] / (ms * 1e-3) / 1e9
 
-    # roofline of the dominant kernel (the larger of k_vis and k_resolve).
-    # Algorithmic HBM bytes per launch:
-    #   k_resolve: RGBA8 + depth f32 + primitive id u32 written once per
-    #              pixel plus the u32 visibility slot read: 16 B/px;
-    #   k_vis:     one u32 tile-list entry per (triangle, tile) pair read
-    #              plus the u32 visibility slot written per pixel.
+    # Algorithmic HBM bytes per launch (design-independent):
+    #   k_vis:     4 B per binned (triangle, tile) pair (the list entry read)
+    #              + 8 B per pixel (the visible depth and triangle id);
+    #   k_resolve: 12 B per pixel written (RGBA8 + depth f32 + id u32);
+    #   binning:   12 B per transformed vertex + 6 B per submitted triangle
+    #              + 64 B per instance + 4 B per pair + 8 B per tile.
     W, H = scene.width, scene.height
-    kbytes = {"k_resolve": 16 * W * H, "k_vis": 4 * st["tile_pairs"] + 4 * W * H}
+    P = st["tile_pairs"]
+    kbytes = {"k_vis": 4 * P + 8 * W * H, "k_resolve": 12 * W * H}
     dom = max(kbytes, key=lambda k: kt.get(k, 0.0))
-    raster_ms = kt.get(dom, 0.0)
-    raster_bytes = kbytes[dom]
-    achieved = raster_bytes / (raster_ms * 1e-3) / 1e9 if raster_ms > 0 else 0.0
+    dom_ms = kt.get(dom, 0.0)
+    achieved = kbytes[dom] / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"{dom}_icospheres256_1080p_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "kernels_icospheres256_1080p.md")
     if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+        for line in open(tpath):
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "hbm_bytes_per_launch" and parts[1] == dom:
+                traffic = int(parts[2])
+    v_xf = scene.transformed_vertices
+    b_bin = 12 * v_xf + 6 * tris + 64 * scene.instances + 4 * P + 8 * st["tiles"]
+    bin_ms = sum(kt.get(k, 0.0) for k in ("k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip"))
+    stages = {
+        "bin": {"kernels": "k_vertex+k_setup_count+k_scan+k_scatter+k_clip", "ms": round(bin_ms, 5),
+                "bytes": b_bin, "achieved_gbs": round(b_bin / (bin_ms * 1e-3) / 1e9, 2) if bin_ms else None},
+        "resolve": {"kernels": "k_resolve", "ms": round(kt.get("k_resolve", 0.0), 5), "bytes": 12 * W * H,
+                    "achieved_gbs": round(12 * W * H / (kt["k_resolve"] * 1e-3) / 1e9, 2) if kt.get("k_resolve") else None},
+    }
+    for v in stages.values():
+        v["frac"] = round(v["achieved_gbs"] / HBM_PEAK_GBS, 5) if v["achieved_gbs"] else None
 
     out = {
         "metric": "Mtris/s & Gfrag/s at 1080p; frame ms plain vs MLP-shaded at >=40dB PSNR",
@@ -152,6 +165,7 @@ def main():
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
+        "stages": stages,
     }
     if not args.no_cpu:
         cb = cpu_baseline(scene)

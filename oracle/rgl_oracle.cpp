@@ -1072,6 +1072,11 @@ float rglo_oneover(float x) { return oneover(x); }
 float rglo_rcpps(float x) { return rcpps(x); }
 int32_t rglo_cvtt(float x) { return cvtt(x); }
 uint32_t rglo_to_srgb8(float x) { return to_srgb8(x); }
+uint32_t rglo_to_linear8(float x) { return to_linear8(x); }
+void rglo_mat4_mul(const float* l, const float* r, float* out) {
+	mat4 a, b; std::memcpy(a.ff, l, 64); std::memcpy(b.ff, r, 64);
+	const mat4 m = mul(a, b); std::memcpy(out, m.ff, 64); }
+void rglo_xform4(const float* m, const float* v, float* out) { xform4(m, v, out); }
 
 // the reference triangle test harness (rglv_triangle.t.cxx:35-131, 251-377):
 // draws one triangle given in device coordinates with TriangleRasterizer into a

@@ -122,7 +122,7 @@ struct rgl_gpu {
 	// device buffers
 	DevArray<float4> vpos; DevArray<uint8_t> vflags; DevArray<float> vvary;
 	DevArray<uint2> trect; DevArray<uint4> tvert;
-	DevArray<uint32_t> hist, bsum, list;
+	DevArray<uint32_t> hist, bsum, tileOff, list;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
 	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist, workOff;
@@ -164,7 +164,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &hist, &bsum, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
+		for (auto* a : { &hist, &bsum, &tileOff, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
 		keys.release(); rec.release(); crec.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
@@ -336,8 +336,10 @@ struct rgl_gpu {
 		if ((e = trect.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
 		if ((e = tvert.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
 		const long long nh = static_cast<long long>(f.nTiles) * std::max(f.units, 1) + 1;
+		const long long nChunks = (std::max(f.units, 1) + kColChunk - 1) / kColChunk;
 		if ((e = hist.ensure(nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
-		if ((e = bsum.ensure(nh / kScanChunk + 2)) != hipSuccess) return hipfail(e, "hipMalloc bsum");
+		if ((e = bsum.ensure(nChunks * f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
+		if ((e = tileOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
 		if ((e = list.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc list");
 		if ((e = rec.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
@@ -408,7 +410,9 @@ struct rgl_gpu {
 			if (tableBytes && (e = hipMemcpyAsync(dDraws.p, hDraws, tableBytes, hipMemcpyHostToDevice, stream)) != hipSuccess)
 				return hipfail(e, "hipMemcpyAsync draws");
 			lastTable.assign(reinterpret_cast<const uint8_t*>(draws.data()), reinterpret_cast<const uint8_t*>(draws.data()) + tableBytes); }
-		k_zero_counters<<<8, 256, 0, stream>>>(dCtr, clipHist.p, f.nTiles);
+		const long long nCsum = static_cast<long long>((f.units + kColChunk - 1) / kColChunk) * f.nTiles;
+		k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(
+			dCtr, clipHist.p, f.nTiles, bsum.p, nCsum);
 		Mark(0);
 		for (const auto& d : draws) {
 			switch (d.program) {
@@ -417,20 +421,17 @@ struct rgl_gpu {
 			case ProgLambertTex::kId: LaunchVertex<ProgLambertTex>(d); break;
 			default: LaunchVertex<ProgManyBlinnPhong>(d); break; }}
 		Mark(1);
-		const long long nh = static_cast<long long>(f.nTiles) * f.units;
 		if (f.units > 0) {
 			k_setup_count<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, f, vpos.p, vflags.p, trect.p, tvert.p,
-				hist.p, clipq.p, clipqOf.p, clipCap, dCtr); }
+				hist.p, bsum.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
-		{
-			const int nb = static_cast<int>((nh + kScanChunk - 1) / kScanChunk);
-			if (nb > 0) {
-				k_scan_reduce<<<nb, kScanThreads, 0, stream>>>(hist.p, nh, bsum.p);
-				k_scan_bsums<<<1, 1024, 0, stream>>>(bsum.p, nb, &dCtr->pairs);
-				k_scan_down<<<nb, kScanThreads, 0, stream>>>(hist.p, nh, bsum.p, hist.p, &dCtr->pairs); }
-			else {
-				k_fill_u32<<<1, 64, 0, stream>>>(hist.p, nh + 1, 0u); }
-		}
+		if (f.units > 0) {
+			const int nChunks = (f.units + kColChunk - 1) / kColChunk;
+			const dim3 cg((f.nTiles + 63) / 64, nChunks);
+			k_col_scan<<<1, 1024, 0, stream>>>(bsum.p, f.nTiles, nChunks, tileOff.p, &dCtr->pairs);
+			k_col_down<<<cg, 64, 0, stream>>>(hist.p, f.nTiles, f.units, bsum.p, tileOff.p); }
+		else {
+			k_fill_u32<<<1, 64, 0, stream>>>(tileOff.p, f.nTiles + 1, 0u); }
 		Mark(3);
 		if (f.units > 0) k_scatter<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(f, trect.p, hist.p, list.p, listCap, dCtr);
 		Mark(4);
@@ -440,22 +441,23 @@ struct rgl_gpu {
 			case ProgMany::kId: LaunchClip<ProgMany>(di, f); break;
 			case ProgLambertTex::kId: LaunchClip<ProgLambertTex>(di, f); break;
 			default: LaunchClip<ProgManyBlinnPhong>(di, f); break; }}
-		k_tile_scan<<<1, 1024, 0, stream>>>(f, hist.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
+		k_tile_scan<<<1, 1024, 0, stream>>>(f, tileOff.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
 		k_clip_scatter<<<64, 256, 0, stream>>>(f, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
 		Mark(5);
 		RasterArgs ra;
-		ra.draws = dDraws.p; ra.off = hist.p; ra.list = list.p; ra.listCap = listCap;
+		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.list = list.p; ra.listCap = listCap;
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
 		ra.keys = keys.p; ra.workOff = workOff.p;
 		if (keysReady < keys.cap) {
-			// the resolve leaves every key at the sentinel; fill once after allocation
+			// defined contents after allocation; each frame then resets the
+			// keys of multi-segment tiles in k_pair_setup and overwrites the rest
 			k_fill_u64<<<1024, 256, 0, stream>>>(keys.p, static_cast<long long>(keys.cap), kKeySentinel);
 			keysReady = keys.cap; }
 		if (f.nTiles > 0) {
 			const int setupBlocks = static_cast<int>(std::min<long long>((2LL * f.totalTris + 255) / 256 + 1, 4096));
-			k_pair_setup<<<dim3(setupBlocks, 2), 256, 0, stream>>>(f, ra, rec.p, crec.p);
+			k_pair_setup<<<dim3(setupBlocks, 3), 256, 0, stream>>>(f, ra, rec.p, crec.p);
 			Mark(6);
 			// one workgroup per work item, sized from the last completed frame
 			const int items = std::max<int>(lastWorkItems, f.nTiles * kQuadrants);

@@ -20,6 +20,7 @@
 namespace rgl {
 
 constexpr int kUnitTris = 256;      // triangles per binning unit (one wave)
+constexpr int kColChunk = 32;       // binning units per column-scan chunk
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // LDS key array is kMaxTileDim^2
 
@@ -112,7 +113,7 @@ __device__ inline const uint2 empty_rect() { return make_uint2(0x0000ffffu, 0x00
 
 __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__ draws, FrameDesc f,
 		const float4* __restrict__ vpos, const uint8_t* __restrict__ vflags,
-		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ hist,
+		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
 		uint32_t* __restrict__ clipq, uint32_t* __restrict__ clipqOf, uint32_t clipCap,
 		Counters* __restrict__ ctr) {
 	extern __shared__ uint32_t lhist[];  // nTiles counters
@@ -181,7 +182,11 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 		trect[s] = rect;
 		tvert[s] = tv; }
 	__syncthreads();
-	for (int t = threadIdx.x; t < f.nTiles; t += 64) hist[static_cast<long long>(t) * f.units + u] = lhist[t];
+	uint32_t* cs = csum + static_cast<long long>(u / kColChunk) * f.nTiles;
+	for (int t = threadIdx.x; t < f.nTiles; t += 64) {
+		const uint32_t c = lhist[t];
+		hist[static_cast<long long>(u) * f.nTiles + t] = c;
+		if (c) atomicAdd(&cs[t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
 	if (threadIdx.x == 0 && binned) atomicAdd(&ctr->binned, binned); }
@@ -190,7 +195,6 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 // exclusive scan of uint32 (three passes); out may alias in.
 // n+1 entries are written: out[n] = total.
 // ---------------------------------------------------------------------------
-constexpr int kScanThreads = 256, kScanPer = 4, kScanChunk = kScanThreads * kScanPer;
 
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
 	// sh: blockDim.x + 32 entries; Hillis-Steele over waves then lanes
@@ -211,51 +215,54 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 	__syncthreads();
 	return r; }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const uint32_t* __restrict__ in, long long n, uint32_t* __restrict__ bsum) {
-	__shared__ uint32_t sh[kScanThreads / 64 + 1];
-	const long long base = static_cast<long long>(blockIdx.x) * kScanChunk;
-	uint32_t acc = 0;
-	for (int k = 0; k < kScanPer; ++k) {
-		const long long i = base + k * kScanThreads + threadIdx.x;
-		if (i < n) acc += in[i]; }
-	for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-	if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
-	__syncthreads();
-	if (threadIdx.x == 0) {
-		uint32_t t = 0;
-		for (int w = 0; w < kScanThreads / 64; ++w) t += sh[w];
-		bsum[blockIdx.x] = t; }}
+// The histogram is unit-major, hist[u * nTiles + t] = unit u's pairs in tile
+// t, so that each unit writes (and the scatter reads) one contiguous row.
+// The list is tile-major: tile t's pairs from units 0..U-1 in unit order.
+// Columns are cut into chunks of kColChunk units; k_setup_count also adds
+// its nonzero counts into csum[chunk * nTiles + t].  The column scan turns
+// those chunk sums into within-tile prefixes and writes tileOff[t] = tile
+// t's first list position (tileOff[nTiles] = the total); the down pass
+// turns the counts in place into list positions.  A thread owns one tile
+// column, so a wave's loads and stores cover contiguous bytes.
+constexpr int kColBatch = 16;   // independent loads in flight per thread
 
-__global__ __launch_bounds__(1024) void k_scan_bsums(uint32_t* __restrict__ bsum, int nb, uint32_t* __restrict__ total_out) {
+__global__ __launch_bounds__(1024) void k_col_scan(uint32_t* __restrict__ csum, int nT, int nChunks,
+                                                   uint32_t* __restrict__ tileOff, uint32_t* __restrict__ total_out) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
 	uint32_t carry = 0;
-	for (int base = 0; base < nb; base += 1024) {
-		const int i = base + threadIdx.x;
-		const uint32_t v = i < nb ? bsum[i] : 0u;
+	for (int base = 0; base < nT; base += 1024) {
+		const int t = base + threadIdx.x;
+		uint32_t col = 0;
+		if (t < nT)
+			for (int c0 = 0; c0 < nChunks; c0 += kColBatch) {
+				uint32_t v[kColBatch];
+#pragma unroll
+				for (int j = 0; j < kColBatch; ++j)
+					v[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
+#pragma unroll
+				for (int j = 0; j < kColBatch; ++j) {
+					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
+					col += v[j]; }}
 		uint32_t tot;
-		const uint32_t ex = block_exclusive_scan(v, sh, tot);
-		if (i < nb) bsum[i] = carry + ex;
+		const uint32_t ex = block_exclusive_scan(col, sh, tot);
+		if (t < nT) tileOff[t] = carry + ex;
 		carry += tot; }
-	if (threadIdx.x == 0) *total_out = carry; }
+	if (threadIdx.x == 0) { tileOff[nT] = carry; *total_out = carry; }}
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_down(const uint32_t* in, long long n, const uint32_t* __restrict__ bsum,
-                                                          uint32_t* out, const uint32_t* __restrict__ total) {
-	__shared__ uint32_t sh[kScanThreads / 64 + 1];
-	const long long base = static_cast<long long>(blockIdx.x) * kScanChunk;
-	// each thread owns kScanPer consecutive elements
-	uint32_t v[kScanPer];
-	uint32_t local = 0;
-	for (int k = 0; k < kScanPer; ++k) {
-		const long long i = base + static_cast<long long>(threadIdx.x) * kScanPer + k;
-		v[k] = i < n ? in[i] : 0u;
-		local += v[k]; }
-	uint32_t tot;
-	uint32_t run = bsum[blockIdx.x] + block_exclusive_scan(local, sh, tot);
-	for (int k = 0; k < kScanPer; ++k) {
-		const long long i = base + static_cast<long long>(threadIdx.x) * kScanPer + k;
-		if (i < n) out[i] = run;
-		run += v[k]; }
-	if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total; }
+__global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
+                                                 const uint32_t* __restrict__ tileOff) {
+	const int t = blockIdx.x * 64 + threadIdx.x, uc = blockIdx.y;
+	if (t >= nT) return;
+	const int u0 = uc * kColChunk, u1 = min(U, u0 + kColChunk);
+	uint32_t run = tileOff[t] + csum[static_cast<long long>(uc) * nT + t];
+	for (int b = u0; b < u1; b += kColBatch) {
+		uint32_t v[kColBatch];
+#pragma unroll
+		for (int j = 0; j < kColBatch; ++j) v[j] = b + j < u1 ? hist[static_cast<long long>(b + j) * nT + t] : 0u;
+#pragma unroll
+		for (int j = 0; j < kColBatch; ++j) {
+			if (b + j < u1) hist[static_cast<long long>(b + j) * nT + t] = run;
+			run += v[j]; }}}
 
 // ---------------------------------------------------------------------------
 // ordered scatter: the per-tile append loop of BinTriangles (:651-661),
@@ -266,7 +273,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameDesc f, const uint2* __rest
 		const uint32_t* __restrict__ off, uint32_t* __restrict__ list, uint32_t listCap, Counters* __restrict__ ctr) {
 	extern __shared__ uint32_t base[];  // nTiles write cursors
 	const int u = blockIdx.x;
-	for (int t = threadIdx.x; t < f.nTiles; t += 64) base[t] = off[static_cast<long long>(t) * f.units + u];
+	for (int t = threadIdx.x; t < f.nTiles; t += 64) base[t] = off[static_cast<long long>(u) * f.nTiles + t];
 	__syncthreads();
 	const int lane = threadIdx.x;
 	const unsigned long long lt = (1ull << lane) - 1ull;
@@ -520,7 +527,7 @@ __device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cv
 
 struct RasterArgs {
 	const DrawDesc* draws;
-	const uint32_t* off;        // inline list offsets, tile-major [nTiles * units + 1]
+	const uint32_t* tileOff;    // inline list: tile t's pairs are [tileOff[t], tileOff[t + 1])
 	const uint32_t* list;
 	uint32_t listCap;
 	const uint4* tvert;
@@ -535,7 +542,7 @@ struct RasterArgs {
 	uint32_t clistCap;
 	uint32_t clipCap;
 	Counters* ctr;
-	unsigned long long* keys;   // per-pixel visibility key; kKeySentinel between frames
+	unsigned long long* keys;   // per-pixel visibility key (smallest (depth, slot) of the frame)
 	const uint32_t* workOff;    // per-tile first work item [nTiles + 1]
 };
 
@@ -562,7 +569,7 @@ __device__ inline uint32_t key_slot(unsigned long long k, bool leq) {
 // quadrants, and the tile's inline and clipped pairs are cut into segments
 // of kSegPairs; a work item is (tile, segment, quadrant).  Also the
 // clipped-list offsets.
-__global__ __launch_bounds__(1024) void k_tile_scan(FrameDesc f, const uint32_t* __restrict__ off, uint32_t listCap,
+__global__ __launch_bounds__(1024) void k_tile_scan(FrameDesc f, const uint32_t* __restrict__ tileOff, uint32_t listCap,
 		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor,
 		uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
@@ -572,8 +579,8 @@ __global__ __launch_bounds__(1024) void k_tile_scan(FrameDesc f, const uint32_t*
 		uint32_t v = 0, items = 0;
 		if (i < f.nTiles) {
 			v = clipHist[i];
-			const uint32_t b = min(off[static_cast<long long>(i) * f.units], listCap);
-			const uint32_t e = min(off[static_cast<long long>(i + 1) * f.units], listCap);
+			const uint32_t b = min(tileOff[i], listCap);
+			const uint32_t e = min(tileOff[i + 1], listCap);
 			items = kQuadrants * max(1u, (e - b + v + kSegPairs - 1) / kSegPairs); }
 		uint32_t tot, wtot;
 		const uint32_t ex = block_exclusive_scan(v, sh, tot);
@@ -607,7 +614,7 @@ __global__ __launch_bounds__(256) void k_clip_scatter(FrameDesc f, const uint8_t
 // as a 64-byte record at the pair's list position.  Records of a tile are
 // contiguous and in list order.
 // ---------------------------------------------------------------------------
-struct VisRec {
+struct alignas(16) VisRec {
 	int32_t c1, c2, c3;
 	int32_t dx12, dy12, dx23, dy23, dx31, dy31;
 	uint32_t minxy;       // minx | miny << 16 (even, tile-clamped)
@@ -645,20 +652,32 @@ constexpr int kMaxTilesLds = 2048;   // tile offset tables cached in LDS up to t
 __global__ __launch_bounds__(256) void k_pair_setup(FrameDesc f, RasterArgs a, VisRec* __restrict__ rec, VisRec* __restrict__ crec) {
 	__shared__ uint32_t toff[kMaxTilesLds + 1];
 	__shared__ VisDraw vd[kMaxDraws];
+	if (blockIdx.y == 2) {
+		// tiles cut into several segments merge their quadrants with a 64-bit
+		// atomic min, so their keys start the frame at the sentinel; tiles of
+		// one segment are overwritten whole by plain stores
+		for (int tile = blockIdx.x; tile < f.nTiles; tile += gridDim.x) {
+			if (a.workOff[tile + 1] - a.workOff[tile] <= static_cast<uint32_t>(kQuadrants)) continue;
+			const Rect R = tile_rect(f, tile);
+			const int rw = R.x1 - R.x0, n = rw * (R.y1 - R.y0);
+			for (int i = threadIdx.x; i < n; i += 256) {
+				const int ly = i / rw, lx = i - ly * rw;
+				a.keys[static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}
+		return; }
 	load_visdraws(vd, a.draws, f.nDraws);
 	const bool clippedPass = blockIdx.y == 1;
 	const uint32_t total = clippedPass ? min(a.clipOff[f.nTiles], a.clistCap)
-	                                   : min(a.off[static_cast<long long>(f.nTiles) * f.units], a.listCap);
+	                                   : min(a.tileOff[f.nTiles], a.listCap);
 	const bool lds = f.nTiles <= kMaxTilesLds;
 	if (lds)
 		for (int t = threadIdx.x; t <= f.nTiles; t += 256)
-			toff[t] = clippedPass ? a.clipOff[t] : a.off[static_cast<long long>(t) * f.units];
+			toff[t] = clippedPass ? a.clipOff[t] : a.tileOff[t];
 	__syncthreads();
 	for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
 		int lo = 0, hi = f.nTiles - 1;  // last tile whose list starts at or before p
 		while (lo < hi) {
 			const int mid = (lo + hi + 1) >> 1;
-			const uint32_t o = lds ? toff[mid] : (clippedPass ? a.clipOff[mid] : a.off[static_cast<long long>(mid) * f.units]);
+			const uint32_t o = lds ? toff[mid] : (clippedPass ? a.clipOff[mid] : a.tileOff[mid]);
 			if (o <= p) lo = mid; else hi = mid - 1; }
 		const Rect R = tile_rect(f, lo);
 		TriVerts tr;
@@ -690,7 +709,7 @@ __global__ __launch_bounds__(256) void k_pair_setup(FrameDesc f, RasterArgs a, V
 // per-pixel key buffer: a plain store when the tile is one segment, a
 // 64-bit atomic min otherwise.
 // ---------------------------------------------------------------------------
-struct VisSetup {
+struct alignas(16) VisSetup {
 	int32_t c1, c2, c3;   // edge values at the triangle's first quad in the quadrant
 	int32_t x0, y0;       // that quad's top-left pixel, quadrant-relative
 	int32_t dx12, dy12, dx23, dy23, dx31, dy31;
@@ -745,8 +764,8 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs 
 		S.x1 = imin(S.x0 + kSubDim, R.x1); S.y1 = imin(S.y0 + kSubDim, R.y1);
 		if (S.x0 >= S.x1 || S.y0 >= S.y1) continue;   // quadrant outside the screen (uniform)
 		for (int i = threadIdx.x; i < kSubDim * kSubDim; i += kRasterThreads) keys[i] = kKeySentinel;
-		const uint32_t ib = min(a.off[static_cast<long long>(tile) * f.units], a.listCap);
-		const uint32_t ie = min(a.off[static_cast<long long>(tile + 1) * f.units], a.listCap);
+		const uint32_t ib = min(a.tileOff[tile], a.listCap);
+		const uint32_t ie = min(a.tileOff[tile + 1], a.listCap);
 		const uint32_t cb = min(a.clipOff[tile], a.clistCap);
 		const uint32_t ce = min(a.clipOff[tile + 1], a.clistCap);
 		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
@@ -909,7 +928,6 @@ __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 	if (x >= f.W || y >= f.H) return;
 	const long long o = static_cast<long long>(y) * f.W + x;
 	const unsigned long long k = a.keys[o];
-	a.keys[o] = kKeySentinel;
 	float col[3] = { f.clearColor[0], f.clearColor[1], f.clearColor[2] };
 	float dep = f.clearDepth;
 	uint32_t prim = 0xffffffffu;
@@ -958,8 +976,9 @@ __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
 __global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
 	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
 
-__global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles) {
+__global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles, uint32_t* csum, long long nCsum) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	for (long long c = i; c < nCsum; c += static_cast<long long>(gridDim.x) * blockDim.x) csum[c] = 0u;
 	if (i == 0) { ctr->fragments = 0; ctr->clipCount = 0; ctr->pairs = 0; ctr->clipPairs = 0; ctr->binned = 0; ctr->overflow = 0;
 	              ctr->workItems = 0; ctr->workNext = 0; }
 	for (int t = i; t < nTiles; t += gridDim.x * blockDim.x) clipHist[t] = 0u; }

@@ -107,10 +107,13 @@ def xorshift32_stream(seed: int, n: int) -> np.ndarray:
 
 
 def srgb8_to_float_table() -> np.ndarray:
-    """D3D10 sRGB8 -> linear conversion, float32 (ryg-srgb.h:33-56 pattern)."""
+    """D3D10 sRGB8 -> linear table of ryg::srgb8_to_float (ryg-srgb.h:33-56),
+    which rglr_texture_load.cxx:55-64 applies to texture texels: c = i * (1/255)
+    in float, c / 12.92 below 0.04045, else the double-precision pow of the
+    float base with the float exponent 2.4f, rounded to float."""
     c = np.arange(256, dtype=np.float32) * np.float32(1.0 / 255.0)
-    lin = np.where(c <= np.float32(0.04045), c / np.float32(12.92),
-                   ((c + np.float32(0.055)) / np.float32(1.055)).astype(np.float64) ** 2.4)
+    base = ((c + np.float32(0.055)) / np.float32(1.055)).astype(np.float64)
+    lin = np.where(c <= np.float32(0.04045), c / np.float32(12.92), base ** np.float64(np.float32(2.4)))
     return lin.astype(np.float32)
 
 
@@ -262,6 +265,15 @@ class Scene:
     @property
     def triangles(self) -> int:
         return sum(len(m.indices) // 3 * m.instances for m in self.meshes)
+
+    @property
+    def instances(self) -> int:
+        return sum(m.instances for m in self.meshes)
+
+    @property
+    def transformed_vertices(self) -> int:
+        """Vertices referenced by the draws, counted once per instance."""
+        return sum((int(m.indices.max()) + 1) * m.instances for m in self.meshes)
 
 
 def scene_sphere_ground(width: int = 640, height: int = 360, seed: int = 1) -> Scene:

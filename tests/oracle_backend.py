@@ -71,6 +71,10 @@ def lib():
         L.rglo_cvtt.restype = ctypes.c_int32
         L.rglo_to_srgb8.argtypes = [f]
         L.rglo_to_srgb8.restype = u32
+        L.rglo_to_linear8.argtypes = [f]
+        L.rglo_to_linear8.restype = u32
+        L.rglo_mat4_mul.argtypes = [vp, vp, vp]
+        L.rglo_xform4.argtypes = [vp, vp, vp]
         L.rglo_scalar_triangle.argtypes = [i, i, vp, vp, vp]
         _lib = L
     return _lib
