@@ -384,11 +384,128 @@ struct ManyBlinnPhongProgram final : Program {
 			color[2][l] = tex[2][l] * kdiff + kspec;
 			color[3][l] = 1.0F; }}};
 
+// GGXEnvProgram (new, the heavy shader): ManyBlinnPhong's vertex stage
+// plus per-instance roughness; 16 point lights with Cook-Torrance GGX
+// (Smith-Schlick G, Schlick F, F0 = 0.04) and an 8-tap environment cube
+// lookup around the reflection vector.  Only + - * / sqrt in a fixed order.
+//   buffers: 11 environment cube (6 faces of 16x16 RGBA floats), 12 taps
+//            (8 x {dx, dy, dz, weight}), 13 lights (16 x {x, y, z, 0, r, g,
+//            b, 0}), 14 per-instance roughness, 15 instance matrices
+//   uniforms: [0..2] eye, [3] ambient, [4] env specular, [5] env diffuse
+constexpr int kGGXLights = 16, kEnvTaps = 8, kEnvDim = 16;
+constexpr float kPi = 3.14159265F, kInvPi = 0.318309886F;
+
+struct ShadeInput { float n[3], v[3], p[3], albedo[3], rough; };
+
+void env_sample(const float* env, float dx, float dy, float dz, float out[3]) {
+	out[0] = 0.0F; out[1] = 0.0F; out[2] = 0.0F;
+	if (env == nullptr) return;
+	const float ax = dx < 0.0F ? -dx : dx, ay = dy < 0.0F ? -dy : dy, az = dz < 0.0F ? -dz : dz;
+	int face; float ma, sc, tc;
+	if (ax >= ay && ax >= az) { face = dx > 0.0F ? 0 : 1; ma = ax; sc = dx > 0.0F ? -dz : dz; tc = -dy; }
+	else if (ay >= az) { face = dy > 0.0F ? 2 : 3; ma = ay; sc = dx; tc = dy > 0.0F ? dz : -dz; }
+	else { face = dz > 0.0F ? 4 : 5; ma = az; sc = dz > 0.0F ? dx : -dx; tc = -dy; }
+	if (!(ma > 0.0F)) return;
+	const float E = static_cast<float>(kEnvDim);
+	float fs = ((sc / ma + 1.0F) * 0.5F) * E - 0.5F;
+	float ft = ((tc / ma + 1.0F) * 0.5F) * E - 0.5F;
+	fs = fs >= 0.0F ? fs : 0.0F; fs = fs <= E - 1.0F ? fs : E - 1.0F;
+	ft = ft >= 0.0F ? ft : 0.0F; ft = ft <= E - 1.0F ? ft : E - 1.0F;
+	const int s0 = static_cast<int>(fs), t0 = static_cast<int>(ft);
+	const int s1 = std::min(s0 + 1, kEnvDim - 1), t1 = std::min(t0 + 1, kEnvDim - 1);
+	const float fx = fs - static_cast<float>(s0), fy = ft - static_cast<float>(t0);
+	const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
+	const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
+	const float* f0 = env + face * kEnvDim * kEnvDim * 4;
+	const float* p00 = f0 + (t0 * kEnvDim + s0) * 4;
+	const float* p10 = f0 + (t0 * kEnvDim + s1) * 4;
+	const float* p01 = f0 + (t1 * kEnvDim + s0) * 4;
+	const float* p11 = f0 + (t1 * kEnvDim + s1) * 4;
+	for (int c = 0; c < 3; ++c) out[c] = ((p00[c] * w00 + p10[c] * w10) + p01[c] * w01) + p11[c] * w11; }
+
+void ggx_shade(const DrawEnv& e, const ShadeInput& s, float color[3]) {
+	const float* u = e.uniforms;
+	const auto& b = e.st->buffers;
+	const float* lights = b[13];
+	const float* taps = b[12];
+	const float rough = s.rough;
+	const float a = rough * rough, a2 = a * a;
+	const float k = ((rough + 1.0F) * (rough + 1.0F)) * 0.125F;
+	const float ndvRaw = (s.n[0]*s.v[0] + s.n[1]*s.v[1]) + s.n[2]*s.v[2];
+	const float ndv = ndvRaw > 1e-4F ? ndvRaw : 1e-4F;
+	const float gv = ndv / (ndv * (1.0F - k) + k);
+	float acc[3] = { 0.0F, 0.0F, 0.0F };
+	for (int i = 0; lights != nullptr && i < kGGXLights; ++i) {
+		const float* L = lights + 8 * i;
+		float lx = L[0] - s.p[0], ly = L[1] - s.p[1], lz = L[2] - s.p[2];
+		const float d2 = (lx*lx + ly*ly) + lz*lz;
+		const float inv = 1.0F / std::sqrt(d2);
+		lx = lx * inv; ly = ly * inv; lz = lz * inv;
+		const float ndl = (s.n[0]*lx + s.n[1]*ly) + s.n[2]*lz;
+		if (!(ndl > 0.0F)) continue;
+		float hx = lx + s.v[0], hy = ly + s.v[1], hz = lz + s.v[2];
+		normalize3(hx, hy, hz);
+		float ndh = (s.n[0]*hx + s.n[1]*hy) + s.n[2]*hz;
+		ndh = ndh > 0.0F ? ndh : 0.0F;
+		float vdh = (s.v[0]*hx + s.v[1]*hy) + s.v[2]*hz;
+		vdh = vdh > 0.0F ? vdh : 0.0F;
+		const float t = (ndh * ndh) * (a2 - 1.0F) + 1.0F;
+		const float D = a2 / ((kPi * t) * t);
+		const float gl = ndl / (ndl * (1.0F - k) + k);
+		const float f = 1.0F - vdh, f2 = f * f, f5 = (f2 * f2) * f;
+		const float F = 0.04F + 0.96F * f5;
+		const float spec = (((D * gl) * gv) * F) / ((4.0F * ndl) * ndv);
+		const float kd = (1.0F - F) * kInvPi;
+		const float id2 = 1.0F / d2;
+		for (int c = 0; c < 3; ++c) acc[c] = acc[c] + ((s.albedo[c] * kd + spec) * (L[4 + c] * id2)) * ndl; }
+	const float rx = (2.0F * ndvRaw) * s.n[0] - s.v[0];
+	const float ry = (2.0F * ndvRaw) * s.n[1] - s.v[1];
+	const float rz = (2.0F * ndvRaw) * s.n[2] - s.v[2];
+	float ev[3] = { 0.0F, 0.0F, 0.0F };
+	for (int j = 0; taps != nullptr && j < kEnvTaps; ++j) {
+		const float* T = taps + 4 * j;
+		float smp[3];
+		env_sample(b[11], rx + T[0] * rough, ry + T[1] * rough, rz + T[2] * rough, smp);
+		for (int c = 0; c < 3; ++c) ev[c] = ev[c] + T[3] * smp[c]; }
+	const float fe = 1.0F - ndv, fe2 = fe * fe, fe5 = (fe2 * fe2) * fe;
+	const float Fe = 0.04F + 0.96F * fe5;
+	const float ks = (u[4] * Fe) * (1.0F - rough);
+	for (int c = 0; c < 3; ++c) color[c] = (acc[c] + u[3] * s.albedo[c]) + ev[c] * (ks + u[5] * s.albedo[c]); }
+
+struct GGXEnvProgram final : Program {
+	GGXEnvProgram() { id = 82; nvary = 9; instanced = true; }
+	void ShadeVertex(const DrawEnv& e, int iid, int idx, float pos[4], VOut& o) const override {
+		ManyBlinnPhongProgram().ShadeVertex(e, iid, idx, pos, o);
+		const float* r = e.st->buffers[14];
+		o.f[8] = r ? r[iid] : 0.5F; }
+	static void Inputs(const DrawEnv& e, const float vary[kMaxVary][4], const float tex[4][4], int l, ShadeInput& s) {
+		const float* u = e.uniforms;
+		s.albedo[0] = tex[0][l]; s.albedo[1] = tex[1][l]; s.albedo[2] = tex[2][l];
+		s.p[0] = vary[0][l]; s.p[1] = vary[1][l]; s.p[2] = vary[2][l];
+		float nx = vary[3][l], ny = vary[4][l], nz = vary[5][l];
+		normalize3(nx, ny, nz);
+		s.n[0] = nx; s.n[1] = ny; s.n[2] = nz;
+		float vx = u[0] - vary[0][l], vy = u[1] - vary[1][l], vz = u[2] - vary[2][l];
+		normalize3(vx, vy, vz);
+		s.v[0] = vx; s.v[1] = vy; s.v[2] = vz;
+		s.rough = vary[8][l]; }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		float tex[4][4];
+		e.tu0.sample(vary[6], vary[7], tex);
+		for (int l = 0; l < 4; ++l) {
+			ShadeInput s;
+			Inputs(e, vary, tex, l, s);
+			float c[3];
+			ggx_shade(e, s, c);
+			color[0][l] = c[0]; color[1][l] = c[1]; color[2][l] = c[2]; color[3][l] = 1.0F; }}};
+
 const Program* find_program(int id) {
 	static const AmyProgram amy; static const ManyProgram many;
 	static const LambertTexProgram lam; static const ManyBlinnPhongProgram mbp;
+	static const GGXEnvProgram ggx;
 	switch (id) {
-	case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp;
+	case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp; case 82: return &ggx;
 	default: return nullptr; }}
 
 // post programs for StoreTrueColor (ShadeCanvas); 1 = Default, 2 = Exposure

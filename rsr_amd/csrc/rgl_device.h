@@ -22,7 +22,7 @@ namespace rgl {
 #include "srgb8_table.inc"
 
 constexpr int kMaxDraws = 64;
-constexpr int kMaxVary = 8;        // varyings per vertex (largest program)
+constexpr int kMaxVary = 12;       // varyings per vertex (largest program, padded to 16 bytes)
 constexpr int kClipVerts = 8;      // 3 + one per clipping plane
 constexpr int kFanMax = 6;         // kClipVerts - 2
 constexpr int kSlotsPerTri = 1 + kFanMax;  // inline slot + fan slots
@@ -343,15 +343,134 @@ struct ProgManyBlinnPhong {
 		color[1] = tex.y * kdiff + kspec;
 		color[2] = tex.z * kdiff + kspec; }};
 
+// GGXEnvProgram (new, the heavy shader): instanced and textured like
+// ManyBlinnPhong; 16 point lights with Cook-Torrance GGX specular (Smith-
+// Schlick G, Schlick F, F0 = 0.04) and per-instance roughness, plus an
+// 8-tap environment lookup around the reflection vector.  Only + - * /
+// and sqrt, in a fixed order, so the CPU oracle matches it bit for bit.
+//   buffers: 11 environment cube (6 faces of kEnvDim^2 RGBA floats),
+//            12 taps (8 x {dx, dy, dz, weight}), 13 lights (16 x {x, y, z,
+//            0, r, g, b, 0}), 14 per-instance roughness, 15 instance mats
+//   uniforms: [0..2] eye, [3] ambient, [4] env specular, [5] env diffuse
+constexpr int kGGXLights = 16, kEnvTaps = 8, kEnvDim = 16;
+constexpr float kPi = 3.14159265F, kInvPi = 0.318309886F;
+
+// the shading inputs of one fragment: unit normal and view vector, world
+// position, albedo, roughness (also the learned shader's 13 inputs)
+struct ShadeInput { float n[3], v[3], p[3], albedo[3], rough; };
+
+__device__ inline void env_sample(const float* env, float dx, float dy, float dz, float out[3]) {
+	out[0] = 0.0F; out[1] = 0.0F; out[2] = 0.0F;
+	if (env == nullptr) return;
+	const float ax = dx < 0.0F ? -dx : dx, ay = dy < 0.0F ? -dy : dy, az = dz < 0.0F ? -dz : dz;
+	int face; float ma, sc, tc;
+	if (ax >= ay && ax >= az) { face = dx > 0.0F ? 0 : 1; ma = ax; sc = dx > 0.0F ? -dz : dz; tc = -dy; }
+	else if (ay >= az) { face = dy > 0.0F ? 2 : 3; ma = ay; sc = dx; tc = dy > 0.0F ? dz : -dz; }
+	else { face = dz > 0.0F ? 4 : 5; ma = az; sc = dz > 0.0F ? dx : -dx; tc = -dy; }
+	if (!(ma > 0.0F)) return;
+	const float E = static_cast<float>(kEnvDim);
+	float fs = ((sc / ma + 1.0F) * 0.5F) * E - 0.5F;
+	float ft = ((tc / ma + 1.0F) * 0.5F) * E - 0.5F;
+	fs = fs >= 0.0F ? fs : 0.0F; fs = fs <= E - 1.0F ? fs : E - 1.0F;   // NaN -> 0
+	ft = ft >= 0.0F ? ft : 0.0F; ft = ft <= E - 1.0F ? ft : E - 1.0F;
+	const int s0 = static_cast<int>(fs), t0 = static_cast<int>(ft);
+	const int s1 = s0 + 1 < kEnvDim ? s0 + 1 : kEnvDim - 1, t1 = t0 + 1 < kEnvDim ? t0 + 1 : kEnvDim - 1;
+	const float fx = fs - static_cast<float>(s0), fy = ft - static_cast<float>(t0);
+	const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
+	const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
+	const float* f0 = env + static_cast<long long>(face) * kEnvDim * kEnvDim * 4;
+	const float* p00 = f0 + (t0 * kEnvDim + s0) * 4;
+	const float* p10 = f0 + (t0 * kEnvDim + s1) * 4;
+	const float* p01 = f0 + (t1 * kEnvDim + s0) * 4;
+	const float* p11 = f0 + (t1 * kEnvDim + s1) * 4;
+#pragma unroll
+	for (int c = 0; c < 3; ++c) out[c] = ((p00[c] * w00 + p10[c] * w10) + p01[c] * w01) + p11[c] * w11; }
+
+__device__ inline void ggx_shade(const DrawDesc& d, const ShadeInput& s, float color[3]) {
+	const float* u = d.uniforms;
+	const float* lights = d.buffers[13];
+	const float* taps = d.buffers[12];
+	const float rough = s.rough;
+	const float a = rough * rough, a2 = a * a;
+	const float k = ((rough + 1.0F) * (rough + 1.0F)) * 0.125F;
+	const float ndvRaw = (s.n[0]*s.v[0] + s.n[1]*s.v[1]) + s.n[2]*s.v[2];
+	const float ndv = ndvRaw > 1e-4F ? ndvRaw : 1e-4F;
+	const float gv = ndv / (ndv * (1.0F - k) + k);
+	float acc[3] = { 0.0F, 0.0F, 0.0F };
+	for (int i = 0; lights != nullptr && i < kGGXLights; ++i) {
+		const float* L = lights + 8 * i;
+		float lx = L[0] - s.p[0], ly = L[1] - s.p[1], lz = L[2] - s.p[2];
+		const float d2 = (lx*lx + ly*ly) + lz*lz;
+		const float inv = 1.0F / __builtin_sqrtf(d2);
+		lx = lx * inv; ly = ly * inv; lz = lz * inv;
+		const float ndl = (s.n[0]*lx + s.n[1]*ly) + s.n[2]*lz;
+		if (!(ndl > 0.0F)) continue;
+		float hx = lx + s.v[0], hy = ly + s.v[1], hz = lz + s.v[2];
+		normalize3(hx, hy, hz);
+		float ndh = (s.n[0]*hx + s.n[1]*hy) + s.n[2]*hz;
+		ndh = ndh > 0.0F ? ndh : 0.0F;
+		float vdh = (s.v[0]*hx + s.v[1]*hy) + s.v[2]*hz;
+		vdh = vdh > 0.0F ? vdh : 0.0F;
+		const float t = (ndh * ndh) * (a2 - 1.0F) + 1.0F;
+		const float D = a2 / ((kPi * t) * t);
+		const float gl = ndl / (ndl * (1.0F - k) + k);
+		const float f = 1.0F - vdh, f2 = f * f, f5 = (f2 * f2) * f;
+		const float F = 0.04F + 0.96F * f5;
+		const float spec = (((D * gl) * gv) * F) / ((4.0F * ndl) * ndv);
+		const float kd = (1.0F - F) * kInvPi;
+		const float id2 = 1.0F / d2;
+#pragma unroll
+		for (int c = 0; c < 3; ++c) acc[c] = acc[c] + ((s.albedo[c] * kd + spec) * (L[4 + c] * id2)) * ndl; }
+	// environment: 8 taps spread around the reflection vector by roughness
+	const float rx = (2.0F * ndvRaw) * s.n[0] - s.v[0];
+	const float ry = (2.0F * ndvRaw) * s.n[1] - s.v[1];
+	const float rz = (2.0F * ndvRaw) * s.n[2] - s.v[2];
+	float e[3] = { 0.0F, 0.0F, 0.0F };
+	for (int j = 0; taps != nullptr && j < kEnvTaps; ++j) {
+		const float* T = taps + 4 * j;
+		float smp[3];
+		env_sample(d.buffers[11], rx + T[0] * rough, ry + T[1] * rough, rz + T[2] * rough, smp);
+#pragma unroll
+		for (int c = 0; c < 3; ++c) e[c] = e[c] + T[3] * smp[c]; }
+	const float fe = 1.0F - ndv, fe2 = fe * fe, fe5 = (fe2 * fe2) * fe;
+	const float Fe = 0.04F + 0.96F * fe5;
+	const float ks = (u[4] * Fe) * (1.0F - rough);
+#pragma unroll
+	for (int c = 0; c < 3; ++c) color[c] = (acc[c] + u[3] * s.albedo[c]) + e[c] * (ks + u[5] * s.albedo[c]); }
+
+struct ProgGGXEnv {
+	static constexpr int kId = 82, kVary = 9, kUV = 6;
+	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
+		ProgManyBlinnPhong::vertex(d, iid, idx, pos, vo);
+		vo[8] = d.buffers[14] ? d.buffers[14][iid] : 0.5F; }
+	// varyings -> shading inputs (the G-buffer of the learned shader)
+	__device__ static void inputs(const DrawDesc& d, const float* v, int lod, ShadeInput& s) {
+		const float4 tex = sample1(d.tu0, v[6], v[7], lod);
+		s.albedo[0] = tex.x; s.albedo[1] = tex.y; s.albedo[2] = tex.z;
+		s.p[0] = v[0]; s.p[1] = v[1]; s.p[2] = v[2];
+		float nx = v[3], ny = v[4], nz = v[5];
+		normalize3(nx, ny, nz);
+		s.n[0] = nx; s.n[1] = ny; s.n[2] = nz;
+		const float* u = d.uniforms;
+		float vx = u[0] - v[0], vy = u[1] - v[1], vz = u[2] - v[2];
+		normalize3(vx, vy, vz);
+		s.v[0] = vx; s.v[1] = vy; s.v[2] = vz;
+		s.rough = v[8]; }
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+		ShadeInput s;
+		inputs(d, v, lod, s);
+		ggx_shade(d, s, color); }};
+
 __host__ __device__ inline int program_nvary(int pid) {
 	switch (pid) {
 	case ProgAmy::kId: return ProgAmy::kVary;
 	case ProgMany::kId: return ProgMany::kVary;
 	case ProgLambertTex::kId: return ProgLambertTex::kVary;
 	case ProgManyBlinnPhong::kId: return ProgManyBlinnPhong::kVary;
+	case ProgGGXEnv::kId: return ProgGGXEnv::kVary;
 	default: return -1; }}
 
 __host__ __device__ inline bool program_instanced(int pid) {
-	return pid == ProgMany::kId || pid == ProgManyBlinnPhong::kId; }
+	return pid == ProgMany::kId || pid == ProgManyBlinnPhong::kId || pid == ProgGGXEnv::kId; }
 
 }  // namespace rgl

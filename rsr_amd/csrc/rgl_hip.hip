@@ -280,9 +280,16 @@ struct rgl_gpu {
 		else {
 			verts = count; }
 		if (verts > 32768) return fail(RGL_E_CAPACITY, "vertex index must be <= 32767");
-		for (int k = 0; k < 15; ++k)
+		const bool ggx = s.programId == ProgGGXEnv::kId;
+		for (int k = 0; k < (ggx ? 11 : 15); ++k)
 			if (s.buffers[k] && s.bufferCounts[k] < (elements ? verts : count))
 				return fail(RGL_E_ARG, "vertex buffer shorter than the position buffer");
+		if (ggx) {
+			// heavy-shader tables: environment cube, taps, lights, per-instance roughness
+			const int need[4] = { 6 * kEnvDim * kEnvDim * 4, 4 * kEnvTaps, 8 * kGGXLights, instances };
+			for (int k = 0; k < 4; ++k)
+				if (s.buffers[11 + k] && s.bufferCounts[11 + k] < need[k])
+					return fail(RGL_E_ARG, "heavy-shader table buffer too short"); }
 		if (program_instanced(s.programId)) {
 			if (!s.buffers[15] || s.bufferCounts[15] < 16 * instances) return fail(RGL_E_ARG, "instance matrix buffer too short"); }
 		else if (instances != 1) {
@@ -419,6 +426,7 @@ struct rgl_gpu {
 			case ProgAmy::kId: LaunchVertex<ProgAmy>(d); break;
 			case ProgMany::kId: LaunchVertex<ProgMany>(d); break;
 			case ProgLambertTex::kId: LaunchVertex<ProgLambertTex>(d); break;
+			case ProgGGXEnv::kId: LaunchVertex<ProgGGXEnv>(d); break;
 			default: LaunchVertex<ProgManyBlinnPhong>(d); break; }}
 		Mark(1);
 		if (f.units > 0) {
@@ -440,6 +448,7 @@ struct rgl_gpu {
 			case ProgAmy::kId: LaunchClip<ProgAmy>(di, f); break;
 			case ProgMany::kId: LaunchClip<ProgMany>(di, f); break;
 			case ProgLambertTex::kId: LaunchClip<ProgLambertTex>(di, f); break;
+			case ProgGGXEnv::kId: LaunchClip<ProgGGXEnv>(di, f); break;
 			default: LaunchClip<ProgManyBlinnPhong>(di, f); break; }}
 		k_tile_scan<<<1, 1024, 0, stream>>>(f, tileOff.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
 		k_clip_scatter<<<64, 256, 0, stream>>>(f, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
@@ -465,7 +474,15 @@ struct rgl_gpu {
 		Mark(7);
 		{
 			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
-			k_resolve<<<grid, 256, 0, stream>>>(f, ra);
+			int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
+			for (const auto& d : draws) if (d.program != pid) pid = -1;
+			switch (pid) {
+			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<grid, 256, 0, stream>>>(f, ra); break;
+			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<grid, 256, 0, stream>>>(f, ra); break;
+			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<grid, 256, 0, stream>>>(f, ra); break;
+			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<grid, 256, 0, stream>>>(f, ra); break;
+			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<grid, 256, 0, stream>>>(f, ra); break;
+			default: k_resolve<ShadeAny><<<grid, 256, 0, stream>>>(f, ra); break; }
 		}
 		Mark(8);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");

@@ -897,13 +897,23 @@ __device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const T
 		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
 	P::fragment1(d, vv, lod, color); }
 
-__device__ inline void shade_dispatch(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
-                                      int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-	switch (d.program) {
-	case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-	case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-	case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-	default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}
+// resolve shaders: one program known at compile time, or a switch over
+// the installed programs for frames that mix them
+template <class P>
+struct ShadeOne {
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+		shade_pixel<P>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); }};
+
+struct ShadeAny {
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+		switch (d.program) {
+		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}};
 
 // ---------------------------------------------------------------------------
 // resolve: one thread per pixel.  The pixel's winning slot is shaded by
@@ -915,6 +925,7 @@ __device__ inline void shade_dispatch(const DrawDesc& d, const TriSetup& t, cons
 // ---------------------------------------------------------------------------
 constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
 
+template <class S>
 __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 	__shared__ DrawLite dl[kMaxDraws];
 	__shared__ uint32_t rcpTab[2048];
@@ -960,7 +971,7 @@ __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 			vary = a.cvvary; }
 		const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
 		const int lane = (x & 1) | ((y & 1) << 1);
-		shade_dispatch(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+		S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
 		prim = d.primBase + srcTri; }
 	if (f.outColor) {
 		float r = col[0], g = col[1], b = col[2];

@@ -41,6 +41,7 @@ PGM_AMY = 4
 PGM_MANY = 6
 PGM_LAMBERT_TEX = 80
 PGM_MANY_BLINN_PHONG = 81
+PGM_GGX_ENV = 82
 
 # dispatch keys (viewer/shaders.cxx:48-52 and :71)
 KEY_RGBF32_F32_LESS = 0x6e2
@@ -382,6 +383,68 @@ def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256
         projection=perspective2(60.0, width / height, 0.1, 100.0),
         program=program, uniforms=uni, texture=tex, texture_dim=tex_dim,
         meshes=[Mesh(b, idx, instances)])
+
+
+ENV_DIM, ENV_TAPS, GGX_LIGHTS = 16, 8, 16
+
+
+def env_cube(dim: int, seed: int) -> np.ndarray:
+    """6 faces of dim x dim RGBA floats: a sky gradient (bright +y, dark
+    -y) times smooth seeded colour noise, faces in +x -x +y -y +z -z order."""
+    rng = np.random.RandomState(seed)
+    out = np.zeros((6, dim, dim, 4), np.float32)
+    t = (np.arange(dim) + 0.5) / dim * 2 - 1
+    S, T = np.meshgrid(t, t)
+    for f in range(6):
+        # direction of each texel (inverse of the face selection in env_sample)
+        if f == 0: d = np.stack([np.ones_like(S), -T, -S], -1)
+        elif f == 1: d = np.stack([-np.ones_like(S), -T, S], -1)
+        elif f == 2: d = np.stack([S, np.ones_like(S), T], -1)
+        elif f == 3: d = np.stack([S, -np.ones_like(S), -T], -1)
+        elif f == 4: d = np.stack([S, -T, np.ones_like(S)], -1)
+        else: d = np.stack([-S, -T, -np.ones_like(S)], -1)
+        d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+        sky = 0.35 + 0.65 * np.clip(d[..., 1] * 0.5 + 0.5, 0, 1)
+        lat = rng.uniform(0.6, 1.0, (4, 4, 3))
+        xi = np.clip(((S + 1) / 2 * 3), 0, 2.999)
+        yi = np.clip(((T + 1) / 2 * 3), 0, 2.999)
+        i0, j0 = xi.astype(int), yi.astype(int)
+        fx, fy = xi - i0, yi - j0
+        c = (lat[j0, i0] * ((1 - fx) * (1 - fy))[..., None] + lat[j0, i0 + 1] * (fx * (1 - fy))[..., None]
+             + lat[j0 + 1, i0] * ((1 - fx) * fy)[..., None] + lat[j0 + 1, i0 + 1] * (fx * fy)[..., None])
+        out[f, ..., :3] = c * sky[..., None]
+        out[f, ..., 3] = 1.0
+    return out.reshape(-1)
+
+
+def scene_heavy(width: int = 1920, height: int = 1080, instances: int = 256, seed: int = 2,
+                light_seed: int = 5, env_seed: int = 6, rough_seed: int = 7, tex_seed: int = 3,
+                tex_dim: int = 1024) -> Scene:
+    """The config-2 geometry under the heavy shader: 16 seeded point lights,
+    GGX specular with per-instance roughness uniform in [0.05, 1], and an
+    8-tap seeded environment lookup (program GGXEnv, id 82)."""
+    sc = scene_icospheres(width, height, instances, seed, tex_seed=tex_seed, tex_dim=tex_dim, program=PGM_GGX_ENV)
+    b = sc.meshes[0].buffers
+    b[14] = np.random.RandomState(rough_seed).uniform(0.05, 1.0, instances).astype(np.float32)
+    rng = np.random.RandomState(light_seed)
+    lights = np.zeros((GGX_LIGHTS, 8), np.float32)
+    lights[:, 0:3] = np.array([0.0, 0.0, -25.0]) + rng.uniform(-30, 30, (GGX_LIGHTS, 3))
+    lights[:, 4:7] = rng.uniform(0.3, 1.0, (GGX_LIGHTS, 3)) * 60.0
+    b[13] = lights.reshape(-1)
+    rng = np.random.RandomState(env_seed + 1000)
+    taps = np.zeros((ENV_TAPS, 4), np.float32)
+    taps[:, 0:3] = rng.normal(0.0, 0.6, (ENV_TAPS, 3))
+    taps[:, 3] = 1.0 / ENV_TAPS
+    b[12] = taps.reshape(-1)
+    b[11] = env_cube(ENV_DIM, env_seed)
+    uni = np.zeros(32, np.float32)
+    uni[0:3] = (0.0, 0.0, 0.0)   # eye
+    uni[3] = 0.03                # ambient
+    uni[4] = 0.8                 # environment specular
+    uni[5] = 0.25                # environment diffuse
+    sc.uniforms = uni
+    sc.name = f"heavy_icospheres{instances}_{width}x{height}"
+    return sc
 
 
 def fragment_state_key(scene: Scene) -> int:

@@ -61,3 +61,17 @@ def test_reference_programs():
     s2 = scenes.scene_icospheres(640, 360, instances=48, seed=7, tex_dim=64, program=scenes.PGM_MANY)
     s2.uniforms[0] = 0.25
     _check(s2)
+
+
+def test_heavy_shader_1080p():
+    # the GGX + environment heavy shader on the 1080p icosphere geometry
+    # (the learned shader's reference): colour bit-exact like the plain path
+    _check(scenes.scene_heavy())
+
+
+def test_heavy_shader_clipped():
+    # heavy shader with near-plane clipping: per-instance roughness and the
+    # environment taps carried through clipped fans
+    s = scenes.scene_heavy(640, 360, instances=96, seed=11)
+    s.meshes[0].buffers[15] = scenes.instance_matrices(96, 11, (0.0, 0.0, -4.0), 12.0, 0.3, 1.5).reshape(-1)
+    _check(s)
