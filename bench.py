@@ -58,12 +58,86 @@ def cpu_baseline(scene, seconds: float = 12.0, max_frames: int = 200):
                       "worst 5% dropped"}
 
 
+MLP_FLOP_PER_FRAGMENT = 2 * (117 * 64 + 64 * 64 + 64 * 64 + 64 * 3)   # 31,744
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def timed_frames(gpu, stream, steps, warmup):
+    import torch
+    for _ in range(max(warmup, 1)):
+        gpu.run_and_wait(stream)
+    gpu.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.run(stream)
+    gpu.wait()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    kt = gpu.kernel_times()
+    gpu.set_timing(False)
+    return ms, kt
+
+
+def bench_learned(steps: int, warmup: int):
+    """Config 4: the 1080p icosphere geometry under the heavy GGX shader,
+    plain vs the learned shader (per-tile fallback at 2/255)."""
+    import torch
+    from rsr_amd import learned, rgl, scenes
+    dev = "cuda:0"
+    wpath = os.path.join(ROOT, "rsr_amd", "ggx_mlp_weights.npy")
+    if not os.path.exists(wpath):
+        return {"error": f"{wpath} missing: run python -m rsr_amd.train_learned"}
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(dev)
+    sc = scenes.scene_heavy()
+    gpu = rgl.GPU(0)
+    ds = rgl.DeviceScene(sc, dev, prim=True)
+    ds.record(gpu)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    gpu.set_learned_shading(None, 0.0)
+    plain_ms, plain_kt = timed_frames(gpu, stream, steps, warmup)
+    pc, pd, pp = ds.outputs()
+    gpu.set_learned_shading(params.data_ptr(), 1.0)          # the network alone
+    net_ms, net_kt = timed_frames(gpu, stream, steps, warmup)
+    nc, nd, npid = ds.outputs()
+    gpu.set_learned_shading(params.data_ptr(), 2.0 / 255.0)  # with the per-tile fallback
+    fb_ms, fb_kt = timed_frames(gpu, stream, steps, warmup)
+    fc, fd, fp = ds.outputs()
+    st = gpu.stats()
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
+    cov = pd < 1.0
+    same_geometry = bool(np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
+                         and np.array_equal(pd.view(np.uint32), nd.view(np.uint32)))
+    frags = int(cov.sum())
+    mlp_ms = net_kt.get("k_mlp_shade", 0.0)
+    achieved = frags * MLP_FLOP_PER_FRAGMENT / (mlp_ms * 1e-3) / 1e12 if mlp_ms else 0.0
+    return {
+        "workload": sc.name, "shaded_fragments": frags,
+        "plain_heavy_ms": round(plain_ms, 4), "learned_ms": round(fb_ms, 4), "network_only_ms": round(net_ms, 4),
+        "speedup": round(plain_ms / fb_ms, 3), "speedup_network_only": round(plain_ms / net_ms, 3),
+        "psnr_db_covered": round(learned.psnr(pc, fc, cov), 2),
+        "psnr_db_frame": round(learned.psnr(pc, fc), 2),
+        "psnr_db_network_only_covered": round(learned.psnr(pc, nc, cov), 2),
+        "fallback_tiles": st["fallback_tiles"], "tiles": st["tiles"],
+        "coverage_depth_ids_identical": same_geometry,
+        "resolve_ms": {"plain_heavy": round(plain_kt.get("k_resolve", 0.0), 5),
+                       "gather": round(net_kt.get("k_resolve", 0.0), 5),
+                       "network": round(mlp_ms, 5),
+                       "probe_and_fallback": round(fb_kt.get("k_mlp_check", 0.0), 5)},
+        "mfma": {"kernel": "k_mlp_shade", "flop_per_fragment": MLP_FLOP_PER_FRAGMENT,
+                 "achieved_tflops": round(achieved, 2), "peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                 "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 5)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-shading (heavy shader) measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -93,7 +167,7 @@ def main():
         gpu.run_and_wait(stream)
     st = gpu.stats()
 
-    gpu.set_timing(True)
+    # the timed region: K frames back to back, no per-kernel events
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -101,10 +175,11 @@ def main():
     gpu.wait()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kt = gpu.kernel_times()
-    gpu.set_timing(False)
     if gpu.stats()["overflow"]:
         raise SystemExit("capacity overflow inside the timed region")
+    # per-kernel durations: a second pass with HIP events recorded on the
+    # frame's stream between the kernels
+    _, kt = timed_frames(gpu, stream, max(20, args.steps // 2), 2)
 
     ms = (t1 - t0) * 1e3 / args.steps
     tris = scene.triangles
@@ -167,6 +242,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
         "stages": stages,
     }
+    if not args.no_learned:
+        out["learned"] = bench_learned(max(20, args.steps // 4), max(5, args.warmup // 4))
     if not args.no_cpu:
         cb = cpu_baseline(scene)
         out["cpu_baseline"] = cb

@@ -48,6 +48,7 @@ typedef struct rgl_stats {
 	int64_t clipped_tile_pairs;  /* (clipped triangle, tile) pairs */
 	int64_t fragments;           /* covered pixels that reached the depth test */
 	int64_t overflow;            /* nonzero: a capacity was exceeded, frame invalid */
+	int64_t fallback_tiles;      /* learned shading: tiles re-shaded by the plain shader */
 } rgl_stats;
 
 /* --- rglv::GPU ---------------------------------------------------------- */
@@ -138,6 +139,49 @@ int rgl_gl_reset(rgl_gpu* gpu);                                                 
    texels = dim*dim*2 RGBA floats with the base level in the first dim rows;
    fills the box-filtered mip chain below it. */
 int rgl_texture_make_mipmap(float* host_texels, int dim);
+
+/* --- learned shading --------------------------------------------------- */
+/* No counterpart in the reference (its shaders are analytic).  A small MLP
+   (13 shading inputs, 4-band frequency encoding to 117 values, 3 hidden
+   layers of 64, ReLU, sRGB colour out; BF16 weights and activations, FP32
+   accumulation on MFMA) stands in for the heavy GGX + environment program
+   (id 82).  Parameters are an fp32 block of rgl_mlp_param_count() floats
+   on the device: W1[64][128] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+   W4[3][64] b4[3], then 4 normalisation floats (position centre xyz,
+   1 / half-extent) that training leaves alone. */
+int rgl_mlp_param_count(void);
+int rgl_mlp_trainable_count(void);
+
+/* Frames drawn with program 82 shade with the network from the next Run on:
+   the plain shader checks 64 probe pixels per tile and re-shades every tile
+   whose RMS error (8-bit, over covered probes) exceeds max_rms_err (in
+   [0,1] units, e.g. 2/255).  Coverage, depth and primitive ids are never
+   affected.  dev_params = NULL switches back to the plain shader. */
+int rgl_gpu_set_learned_shading(rgl_gpu* gpu, const float* dev_params, float max_rms_err);
+
+/* Training data: covered pixels of the frame, each kept with probability
+   keep (a hash of pixel and seed), appended at dev_out[atomic(*dev_count)]
+   as 16 floats: 13 shading inputs (normal, view vector, world position,
+   albedo, roughness) and the plain shader's sRGB colour / 255.  Frames must
+   use program 82. */
+int rgl_gl_store_training_samples(rgl_gpu* gpu, float* dev_out, int capacity, uint32_t* dev_count, float keep, uint32_t seed);
+
+/* Network output (sRGB-space RGB, before clamping) for n samples in the
+   16-float layout above: dev_out = n x 3 floats. */
+int rgl_mlp_forward(rgl_gpu* gpu, const float* dev_samples, int n, const float* dev_params, float* dev_out, void* hip_stream);
+
+/* Gradient of the mean squared error over the first n samples (all 3
+   channels) with respect to the trainable parameters -> dev_grad
+   (rgl_mlp_trainable_count() floats); *loss_out (host) = the loss. */
+int rgl_mlp_gradient(rgl_gpu* gpu, const float* dev_samples, int n, float* dev_params, float* dev_grad, float* loss_out,
+                     void* hip_stream);
+
+/* steps Adam steps (beta 0.9 / 0.99) on batches of `batch` samples drawn by
+   hash from the n samples; dev_opt = 2 x rgl_mlp_trainable_count() floats of
+   optimiser state (zero before the first step); step0 = steps already taken.
+   *loss_out (host, may be NULL) = the last batch's loss. */
+int rgl_mlp_train(rgl_gpu* gpu, const float* dev_samples, long long n, float* dev_params, float* dev_opt, int steps, int batch,
+                  float lr, uint32_t seed, int step0, float* loss_out, void* hip_stream);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "../../include/rgl_hip.h"
-#include "rgl_kernels.h"
+#include "rgl_mlp.h"
 
 using namespace rgl;
 
@@ -68,7 +68,7 @@ struct GLState {
 
 struct Cmd {
 	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
-	            STORE_TC, STORE_DEPTH, STORE_PRIM } kind;
+	            STORE_TC, STORE_DEPTH, STORE_PRIM, STORE_SAMPLES } kind;
 	int a, b;
 	const void* ptr; };
 
@@ -97,8 +97,12 @@ struct DevArray {
 		return e; }
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
-const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_pair_setup", "k_vis", "k_resolve" };
-constexpr int kNumTimed = 8;
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_pair_setup", "k_vis", "k_resolve",
+                               "k_mlp_shade", "k_mlp_check" };
+constexpr int kNumTimed = 10;
+
+// a StoreTrainingSamples request (no reference counterpart)
+struct SampleReq { float* out; uint32_t cap; uint32_t* count; float keep; uint32_t seed; };
 
 }  // namespace
 
@@ -115,6 +119,19 @@ struct rgl_gpu {
 	std::deque<GLState> states;
 	std::vector<Cmd> commands;
 	std::vector<uint32_t> installed;
+	std::vector<SampleReq> sampleReqs;
+
+	// learned shading (rgl_gpu_set_learned_shading)
+	const float* mlpParams = nullptr;
+	float mlpMaxRms = 2.0F / 255.0F;
+	DevArray<uint32_t> mlpImg;
+	DevArray<float> tileErr, mlpQueue;
+	DevArray<uint32_t> qCount;      // network queue counters, kQueueSegs x kQueueStride
+	int mlpShadeBlocks = 256;       // persistent network workgroups: one per CU
+	bool mlpShadeBlocksSet = false;
+	uint32_t mlpSegCap = 0;         // queue entries per segment
+	// training scratch
+	DevArray<float> tX, tT, tH1, tH2, tH3, tY, tdY, tdA, tdB, tPart, tLoss;
 
 	// render target (GPU::Reset / Retile, rglv_gpu.cxx:50-87)
 	int W = 0, H = 0, tileW = 64, tileH = 64, tilesX = 0, tilesY = 0;
@@ -164,8 +181,10 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &hist, &bsum, &tileOff, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
 		keys.release(); rec.release(); crec.release();
+		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
+		mlpImg.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
 		if (dCtr) (void)hipFree(dCtr);
@@ -187,7 +206,7 @@ struct rgl_gpu {
 
 	bool Installed(uint32_t key) const { return std::find(installed.begin(), installed.end(), key) != installed.end(); }
 
-	void ResetGL() { dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); }
+	void ResetGL() { dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); sampleReqs.clear(); }
 
 	// ---- frame planning ----
 	int Plan(FrameDesc& f, std::vector<DrawDesc>& draws) {
@@ -228,6 +247,8 @@ struct rgl_gpu {
 				f.outDepth = static_cast<float*>(const_cast<void*>(c.ptr)); stored = true; break;
 			case Cmd::STORE_PRIM:
 				f.outPrim = static_cast<uint32_t*>(const_cast<void*>(c.ptr)); stored = true; break;
+			case Cmd::STORE_SAMPLES:
+				stored = true; break;
 			default: {
 				if (stored) return fail(RGL_E_UNSUPPORTED, "draw after a store is not supported");
 				if (!st) return fail(RGL_E_STATE, "draw without state");
@@ -367,6 +388,18 @@ struct rgl_gpu {
 		const size_t px = static_cast<size_t>(f.W) * f.H;
 		if (px > keys.cap) keysReady = 0;
 		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
+		if (mlpParams) {
+			if ((e = mlpImg.ensure(kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
+			if ((e = tileErr.ensure(2LL * f.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
+			// each queue segment takes the blocks b with b % kQueueSegs == seg
+			const long long blocks = static_cast<long long>((f.W + kResolveW - 1) / kResolveW) * ((f.H + kResolveH - 1) / kResolveH);
+			mlpSegCap = static_cast<uint32_t>((blocks + kQueueSegs - 1) / kQueueSegs * 256);
+			if ((e = mlpQueue.ensure(static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
+			if (!mlpShadeBlocksSet) {
+				int cus = 0;
+				if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) mlpShadeBlocks = cus;
+				mlpShadeBlocksSet = true; }}
+		if ((e = qCount.ensure(kQueueSegs * kQueueStride)) != hipSuccess) return hipfail(e, "hipMalloc queue counters");
 		if (!dDraws.p) {
 			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
 			lastTable.clear(); }
@@ -419,7 +452,7 @@ struct rgl_gpu {
 			lastTable.assign(reinterpret_cast<const uint8_t*>(draws.data()), reinterpret_cast<const uint8_t*>(draws.data()) + tableBytes); }
 		const long long nCsum = static_cast<long long>((f.units + kColChunk - 1) / kColChunk) * f.nTiles;
 		k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(
-			dCtr, clipHist.p, f.nTiles, bsum.p, nCsum);
+			dCtr, clipHist.p, f.nTiles, bsum.p, nCsum, qCount.p);
 		Mark(0);
 		for (const auto& d : draws) {
 			switch (d.program) {
@@ -472,19 +505,39 @@ struct rgl_gpu {
 			const int items = std::max<int>(lastWorkItems, f.nTiles * kQuadrants);
 			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(f, ra, rec.p, crec.p); }
 		Mark(7);
-		{
+		int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
+		for (const auto& d : draws) if (d.program != pid) pid = -1;
+		const bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
+		if (learned) {
+			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
 			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
-			int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
-			for (const auto& d : draws) if (d.program != pid) pid = -1;
+			k_mlp_gather<ProgGGXEnv><<<grid, 256, 0, stream>>>(f, ra, mlpQueue.p, qCount.p, mlpSegCap); }
+		else {
+			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
 			switch (pid) {
 			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<grid, 256, 0, stream>>>(f, ra); break;
 			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<grid, 256, 0, stream>>>(f, ra); break;
 			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<grid, 256, 0, stream>>>(f, ra); break;
 			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<grid, 256, 0, stream>>>(f, ra); break;
 			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<grid, 256, 0, stream>>>(f, ra); break;
-			default: k_resolve<ShadeAny><<<grid, 256, 0, stream>>>(f, ra); break; }
-		}
+			default: k_resolve<ShadeAny><<<grid, 256, 0, stream>>>(f, ra); break; }}
 		Mark(8);
+		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(f, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
+		Mark(9);
+		if (learned && f.nTiles > 0) {
+			// per-tile error estimate at probe pixels, then plain re-shading of
+			// the tiles over the limit
+			k_fill_u32<<<8, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(tileErr.p), 2LL * f.nTiles, 0u);
+			const long long probes = static_cast<long long>(f.nTiles) * kMlpProbes;
+			k_mlp_probe<ProgGGXEnv><<<static_cast<int>((probes + 255) / 256), 256, 0, stream>>>(f, ra, tileErr.p);
+			const float lim = mlpMaxRms * 255.0F;
+			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
+			k_mlp_fallback<ProgGGXEnv><<<grid, 256, 0, stream>>>(f, ra, tileErr.p, lim * lim, &dCtr->fallbackTiles); }
+		for (const SampleReq& q : sampleReqs) {
+			if (pid != ProgGGXEnv::kId) return fail(RGL_E_UNSUPPORTED, "training samples need the heavy-shader program");
+			const long long px = static_cast<long long>(f.W) * f.H;
+			k_mlp_samples<ProgGGXEnv><<<static_cast<int>((px + 255) / 256), 256, 0, stream>>>(f, ra, q.out, q.cap, q.count, q.keep, q.seed); }
+		Mark(10);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
 		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
@@ -493,6 +546,51 @@ struct rgl_gpu {
 		stats.commands = static_cast<int64_t>(commands.size());
 		stats.primitives_submitted = submitted;
 		pending = true;
+		return 0; }
+
+	// ---- learned shading: one forward + backward pass over a batch ----
+	void Gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb, int ldc, int tA, int tB,
+	          const float* bias, int relu, const float* mask, int ldm, int kChunk, int nz) {
+		GemmArgs g{ A, B, C, M, N, K, lda, ldb, ldc, tA, tB, bias, relu, mask, ldm, kChunk, kMlpTrainable };
+		k_gemm_bf16<<<dim3((N + 63) / 64, (M + 63) / 64, nz), 256, 0, stream>>>(g); }
+
+	// seed 0xffffffff: the first B samples in order.  lr > 0: Adam step on
+	// prm/opt; lr == 0: the summed gradient goes to gradOut.
+	int TrainPass(const float* samples, long long n, int B, float* prm, float* opt, float lr, float b1, float b2,
+	              int t, uint32_t seed, float* gradOut) {
+		hipError_t e;
+		const int kChunk = 256;
+		const int nz = (B + kChunk - 1) / kChunk;
+		for (auto* a : { &tX }) if ((e = a->ensure(static_cast<size_t>(B) * kMlpK1)) != hipSuccess) return hipfail(e, "hipMalloc training");
+		for (auto* a : { &tH1, &tH2, &tH3, &tdA, &tdB }) if ((e = a->ensure(static_cast<size_t>(B) * kMlpHid)) != hipSuccess) return hipfail(e, "hipMalloc training");
+		for (auto* a : { &tT, &tY, &tdY }) if ((e = a->ensure(static_cast<size_t>(B) * 3)) != hipSuccess) return hipfail(e, "hipMalloc training");
+		if ((e = tPart.ensure(static_cast<size_t>(nz) * kMlpTrainable)) != hipSuccess) return hipfail(e, "hipMalloc training");
+		if ((e = tLoss.ensure(1)) != hipSuccess) return hipfail(e, "hipMalloc training");
+		k_mlp_batch<<<(B + 255) / 256, 256, 0, stream>>>(samples, n, B, seed, static_cast<uint32_t>(t), prm, tX.p, tT.p);
+		// forward: H = relu(X W^T + b), Y = H3 W4^T + b4
+		Gemm(tX.p, prm + kW1, tH1.p, B, kMlpHid, kMlpK1, kMlpK1, kMlpK1, kMlpHid, 0, 1, prm + kB1, 1, nullptr, 0, kMlpK1, 1);
+		Gemm(tH1.p, prm + kW2, tH2.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB2, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(tH2.p, prm + kW3, tH3.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB3, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(tH3.p, prm + kW4, tY.p, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpOut, 0, 1, prm + kB4, 0, nullptr, 0, kMlpHid, 1);
+		k_fill_u32<<<1, 64, 0, stream>>>(reinterpret_cast<uint32_t*>(tLoss.p), 1, 0u);
+		k_mlp_loss<<<(3 * B + 255) / 256, 256, 0, stream>>>(tY.p, tT.p, B, tdY.p, tLoss.p);
+		// backward: weight gradients split over K = samples into nz partials
+		float* P = tPart.p;
+		Gemm(tdY.p, tH3.p, P + kW4, kMlpOut, kMlpHid, B, kMlpOut, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdY.p, B, kMlpOut, kChunk, P + kB4, kMlpTrainable);
+		Gemm(tdY.p, prm + kW4, tdA.p, B, kMlpHid, kMlpOut, kMlpOut, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH3.p, kMlpHid, kMlpOut, 1);
+		Gemm(tdA.p, tH2.p, P + kW3, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdA.p, B, kMlpHid, kChunk, P + kB3, kMlpTrainable);
+		Gemm(tdA.p, prm + kW3, tdB.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH2.p, kMlpHid, kMlpHid, 1);
+		Gemm(tdB.p, tH1.p, P + kW2, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdB.p, B, kMlpHid, kChunk, P + kB2, kMlpTrainable);
+		Gemm(tdB.p, prm + kW2, tdA.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH1.p, kMlpHid, kMlpHid, 1);
+		Gemm(tdA.p, tX.p, P + kW1, kMlpHid, kMlpK1, B, kMlpHid, kMlpK1, kMlpK1, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdA.p, B, kMlpHid, kChunk, P + kB1, kMlpTrainable);
+		const float bc1 = 1.0F - std::pow(b1, static_cast<float>(t)), bc2 = 1.0F - std::pow(b2, static_cast<float>(t));
+		k_mlp_adam<<<(kMlpTrainable + 255) / 256, 256, 0, stream>>>(P, nz, prm, opt, opt ? opt + kMlpTrainable : nullptr,
+			lr, b1, b2, 1e-8F, bc1 > 0.0F ? bc1 : 1.0F, bc2 > 0.0F ? bc2 : 1.0F, gradOut);
+		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "training launch");
 		return 0; }
 
 	int Wait() {
@@ -506,6 +604,7 @@ struct rgl_gpu {
 		stats.clipped_tile_pairs = hCtr->clipPairs;
 		stats.fragments = static_cast<int64_t>(hCtr->fragments);
 		stats.overflow = hCtr->overflow;
+		stats.fallback_tiles = hCtr->fallbackTiles;
 		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));
 		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
@@ -714,6 +813,71 @@ int rgl_texture_make_mipmap(float* buf, int dim) {
 			dst += stride; }
 		src += srcSize * stride;
 		srcSize >>= 1; }
+	return 0; }
+
+/* ---- learned shading (no reference counterpart) ---- */
+int rgl_mlp_param_count(void) { return kMlpParams; }
+int rgl_mlp_trainable_count(void) { return kMlpTrainable; }
+
+int rgl_gpu_set_learned_shading(rgl_gpu* g, const float* dev_params, float max_rms_err) {
+	if (check(g)) return RGL_E_ARG;
+	if (!(max_rms_err >= 0.0F)) return g->fail(RGL_E_ARG, "max_rms_err must be >= 0");
+	g->mlpParams = dev_params;
+	g->mlpMaxRms = max_rms_err;
+	return 0; }
+
+int rgl_gl_store_training_samples(rgl_gpu* g, float* dev_out, int capacity, uint32_t* dev_count, float keep, uint32_t seed) {
+	if (check(g)) return RGL_E_ARG;
+	if (!dev_out || !dev_count || capacity < 0 || !(keep >= 0.0F && keep <= 1.0F)) return g->fail(RGL_E_ARG, "bad sample request");
+	g->MaybeUpdateState();
+	g->sampleReqs.push_back({ dev_out, static_cast<uint32_t>(capacity), dev_count, keep, seed });
+	g->commands.push_back({ Cmd::STORE_SAMPLES, static_cast<int>(g->sampleReqs.size()) - 1, 0, nullptr });
+	return 0; }
+
+int rgl_mlp_forward(rgl_gpu* g, const float* dev_samples, int n, const float* dev_params, float* dev_out, void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (!dev_samples || !dev_params || !dev_out || n < 0) return g->fail(RGL_E_ARG, "bad forward arguments");
+	if (n == 0) return 0;
+	g->stream = static_cast<hipStream_t>(hip_stream);
+	hipError_t e;
+	if ((e = g->mlpImg.ensure(kImgFloats)) != hipSuccess) return g->hipfail(e, "hipMalloc network image");
+	k_mlp_pack<<<16, 256, 0, g->stream>>>(dev_params, g->mlpImg.p);
+	const int per = 64 * kMlpWaves;
+	k_mlp_forward<<<(n + per - 1) / per, per, 0, g->stream>>>(dev_samples, n, g->mlpImg.p, dev_out);
+	if ((e = hipGetLastError()) != hipSuccess) return g->hipfail(e, "forward launch");
+	return 0; }
+
+int rgl_mlp_gradient(rgl_gpu* g, const float* dev_samples, int n, float* dev_params, float* dev_grad, float* loss_out,
+                     void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (!dev_samples || !dev_params || !dev_grad || n <= 0) return g->fail(RGL_E_ARG, "bad gradient arguments");
+	g->stream = static_cast<hipStream_t>(hip_stream);
+	int r = g->TrainPass(dev_samples, n, n, dev_params, nullptr, 0.0F, 0.9F, 0.999F, 1, 0xffffffffu, dev_grad);
+	if (r != 0) return r;
+	if (loss_out) {
+		hipError_t e;
+		float s = 0.0F;
+		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream)) != hipSuccess) return g->hipfail(e, "loss copy");
+		if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hipfail(e, "loss sync");
+		*loss_out = s / (3.0F * static_cast<float>(n)); }
+	return 0; }
+
+int rgl_mlp_train(rgl_gpu* g, const float* dev_samples, long long n, float* dev_params, float* dev_opt, int steps, int batch,
+                  float lr, uint32_t seed, int step0, float* loss_out, void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (!dev_samples || !dev_params || !dev_opt || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.0F) || step0 < 0)
+		return g->fail(RGL_E_ARG, "bad training arguments");
+	g->stream = static_cast<hipStream_t>(hip_stream);
+	for (int s = 0; s < steps; ++s) {
+		const int r = g->TrainPass(dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
+		                           seed == 0xffffffffu ? 0u : seed, nullptr);
+		if (r != 0) return r; }
+	if (loss_out) {
+		hipError_t e;
+		float s = 0.0F;
+		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream)) != hipSuccess) return g->hipfail(e, "loss copy");
+		if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hipfail(e, "loss sync");
+		*loss_out = s / (3.0F * static_cast<float>(batch)); }
 	return 0; }
 
 }  // extern "C"

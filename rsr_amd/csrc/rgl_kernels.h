@@ -33,7 +33,8 @@ struct Counters {
 	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index, 4 bad list entry
 	uint32_t workItems;      // visibility work items of the frame
 	uint32_t workNext;       // next work item to hand out
-	uint32_t pad;
+	uint32_t fallbackTiles;  // learned shading: tiles re-shaded by the plain shader
+	uint32_t pad[3];
 };
 
 // compact per-draw view used by the multi-draw kernels
@@ -925,6 +926,57 @@ struct ShadeAny {
 // ---------------------------------------------------------------------------
 constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
 
+// the winner of pixel (x, y), shaded: colour before the post program,
+// depth and primitive id (clear values where nothing was drawn)
+template <class S>
+__device__ inline bool resolve_pixel(const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+                                     const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
+	const long long o = static_cast<long long>(y) * f.W + x;
+	const unsigned long long k = a.keys[o];
+	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
+	dep = f.clearDepth;
+	prim = 0xffffffffu;
+	if (!(k < clear_key(f))) return false;
+	const uint32_t slot = key_slot(k, f.depthFunc == 1);
+	const int di = find_draw_slot(dl, f.nDraws, slot);
+	const DrawLite& d = dl[di];
+	const uint32_t rel = slot - d.slotBase;
+	const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
+	const int tx = x / f.tileW, ty = y / f.tileH;
+	Rect R;
+	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
+	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
+	TriVerts tr;
+	TriSetup t;
+	uint32_t srcTri;
+	const float* vary;
+	if (rel < nTri) {
+		srcTri = rel;
+		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
+		setup_vector(R, tr.X, tr.Y, t);
+		vary = a.vvary; }
+	else {
+		const uint32_t fslot = rel - nTri;
+		srcTri = fslot / kFanMax;
+		const uint32_t j = fslot - srcTri * kFanMax;
+		const uint32_t cq = a.clipqOf[d.triBase + srcTri];
+		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+		setup_scalar(R, tr.X, tr.Y, t);
+		vary = a.cvvary; }
+	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
+	const int lane = (x & 1) | ((y & 1) << 1);
+	S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+	prim = d.primBase + srcTri;
+	return true; }
+
+// StoreTrueColor's ShadeCanvas + sRGB encode of one pixel (FilterTile,
+// rglr_algorithm.hxx:31-104)
+__device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], const uint32_t* srgbTab) {
+	float r = col[0], g = col[1], b = col[2];
+	if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
+	return f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
+	               : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+
 template <class S>
 __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 	__shared__ DrawLite dl[kMaxDraws];
@@ -938,46 +990,10 @@ __global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
 	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
 	if (x >= f.W || y >= f.H) return;
 	const long long o = static_cast<long long>(y) * f.W + x;
-	const unsigned long long k = a.keys[o];
-	float col[3] = { f.clearColor[0], f.clearColor[1], f.clearColor[2] };
-	float dep = f.clearDepth;
-	uint32_t prim = 0xffffffffu;
-	if (k < clear_key(f)) {
-		const uint32_t slot = key_slot(k, f.depthFunc == 1);
-		const int di = find_draw_slot(dl, f.nDraws, slot);
-		const DrawLite& d = dl[di];
-		const uint32_t rel = slot - d.slotBase;
-		const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
-		const int tx = x / f.tileW, ty = y / f.tileH;
-		Rect R;
-		R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
-		R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
-		TriVerts tr;
-		TriSetup t;
-		uint32_t srcTri;
-		const float* vary;
-		if (rel < nTri) {
-			srcTri = rel;
-			inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
-			setup_vector(R, tr.X, tr.Y, t);
-			vary = a.vvary; }
-		else {
-			const uint32_t fslot = rel - nTri;
-			srcTri = fslot / kFanMax;
-			const uint32_t j = fslot - srcTri * kFanMax;
-			const uint32_t cq = a.clipqOf[d.triBase + srcTri];
-			clipped_verts(cq * kFanMax + j, a.cvpos, tr);
-			setup_scalar(R, tr.X, tr.Y, t);
-			vary = a.cvvary; }
-		const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
-		const int lane = (x & 1) | ((y & 1) << 1);
-		S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
-		prim = d.primBase + srcTri; }
-	if (f.outColor) {
-		float r = col[0], g = col[1], b = col[2];
-		if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
-		f.outColor[o] = f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
-		                        : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+	float col[3], dep;
+	uint32_t prim;
+	resolve_pixel<S>(f, a, dl, x, y, rcpTab, col, dep, prim);
+	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
 	if (f.outDepth) f.outDepth[o] = dep;
 	if (f.outPrim) f.outPrim[o] = prim; }
 
@@ -987,11 +1003,12 @@ __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
 __global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
 	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
 
-__global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles, uint32_t* csum, long long nCsum) {
+__global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles, uint32_t* csum, long long nCsum, uint32_t* qcount) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	for (long long c = i; c < nCsum; c += static_cast<long long>(gridDim.x) * blockDim.x) csum[c] = 0u;
 	if (i == 0) { ctr->fragments = 0; ctr->clipCount = 0; ctr->pairs = 0; ctr->clipPairs = 0; ctr->binned = 0; ctr->overflow = 0;
-	              ctr->workItems = 0; ctr->workNext = 0; }
+	              ctr->workItems = 0; ctr->workNext = 0; ctr->fallbackTiles = 0; }
+	if (i < 8 * 32) qcount[i] = 0u;
 	for (int t = i; t < nTiles; t += gridDim.x * blockDim.x) clipHist[t] = 0u; }
 
 }  // namespace rgl

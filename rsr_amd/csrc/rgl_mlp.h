@@ -1,0 +1,636 @@
+// Learned shading on gfx950: a small MLP surrogate for the heavy shader
+// (ProgGGXEnv), BF16 weights and activations with FP32 accumulation on
+// MFMA (v_mfma_f32_16x16x32_bf16).
+//
+//   inputs   13 shading inputs of a fragment: unit normal, unit view
+//            vector, world position (centred and scaled), albedo, roughness
+//   encoding each input x -> x, sin(2^k pi x), cos(2^k pi x), k = 0..3
+//            (117 values, padded with zeros to 128)
+//   network  117 -> 64 -> 64 -> 64 -> 3, ReLU; output = sRGB-space colour
+//
+// Inference follows the resolve: a gather pass queues the shading inputs
+// of covered pixels, a dense persistent pass runs them through the network,
+// 64 per wave; layer activations go through LDS in the operand layouts MFMA
+// reads, weights sit in LDS for the whole workgroup.  Training uses a generic bf16 MFMA
+// GEMM (operands rounded to bf16 when staged, FP32 accumulation) for the
+// forward and backward passes, split-K weight gradients and an Adam update.
+// No part of this has a counterpart in the reference, whose shaders are
+// all analytic.
+#pragma once
+#include "rgl_kernels.h"
+
+namespace rgl {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMlpFeat = 13;
+constexpr int kMlpBands = 4;
+constexpr int kMlpEnc = kMlpFeat * (1 + 2 * kMlpBands);   // 117
+constexpr int kMlpK1 = 128;                                // layer-1 inputs padded to the MFMA K step
+constexpr int kMlpHid = 64, kMlpOut = 3;
+constexpr int kMlpSample = 16;                             // floats per training sample: 13 inputs, 3 targets
+
+// fp32 parameter block, offsets in floats.  W1 is [64][128] (columns of the
+// padding stay zero), W2 and W3 [64][64], W4 [3][64]; kNorm holds the
+// position centre (xyz) and 1 / half-extent, which are not trained.
+constexpr int kW1 = 0;
+constexpr int kB1 = kW1 + kMlpHid * kMlpK1;
+constexpr int kW2 = kB1 + kMlpHid;
+constexpr int kB2 = kW2 + kMlpHid * kMlpHid;
+constexpr int kW3 = kB2 + kMlpHid;
+constexpr int kB3 = kW3 + kMlpHid * kMlpHid;
+constexpr int kW4 = kB3 + kMlpHid;
+constexpr int kB4 = kW4 + kMlpOut * kMlpHid;
+constexpr int kNorm = kB4 + kMlpOut;
+constexpr int kMlpTrainable = kNorm;
+constexpr int kMlpParams = kNorm + 4;
+
+// bf16 image of the weights in the LDS layout the kernels read (rows padded
+// by 8 elements so that 16 lanes reading 16 rows hit distinct banks),
+// followed by the fp32 biases and the normalisation
+constexpr int kLd1 = kMlpK1 + 8;      // 136
+constexpr int kLdH = kMlpHid + 8;     // 72
+constexpr int kImgW1 = 0;                          // uint16 offsets
+constexpr int kImgW2 = kImgW1 + kMlpHid * kLd1;
+constexpr int kImgW3 = kImgW2 + kMlpHid * kLdH;
+constexpr int kImgW4 = kImgW3 + kMlpHid * kLdH;    // 16 rows, 3 used
+constexpr int kImgEnd = kImgW4 + 16 * kLdH;        // 19072 uint16 = 38144 bytes
+constexpr int kImgBiasF = kImgEnd / 2;             // float offset of b1 b2 b3 b4(4) norm(4)
+constexpr int kImgFloats = kImgBiasF + 3 * kMlpHid + 4 + 4;
+static_assert(kImgEnd % 8 == 0, "bias block must stay 16-byte aligned");
+
+__device__ inline uint16_t bf16_bits(float v) {   // round to nearest even
+	uint32_t u = f2u(v);
+	u += 0x7fffu + ((u >> 16) & 1u);
+	return static_cast<uint16_t>(u >> 16); }
+
+__device__ inline float bf16_float(uint16_t b) { return u2f(static_cast<uint32_t>(b) << 16); }
+
+__device__ inline f32x4_t mfma_bf16(const bf16x8_t& a, const bf16x8_t& b, const f32x4_t& c) {
+	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+
+__device__ inline bf16x8_t lds_bf16x8(const uint16_t* p) { return *reinterpret_cast<const bf16x8_t*>(p); }
+
+// fp32 parameters -> bf16 LDS image
+__global__ void k_mlp_pack(const float* __restrict__ prm, uint32_t* __restrict__ img) {
+	uint16_t* h = reinterpret_cast<uint16_t*>(img);
+	float* fl = reinterpret_cast<float*>(img);
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kImgFloats; i += gridDim.x * blockDim.x) {
+		if (i < kImgBiasF) {
+			for (int half = 0; half < 2; ++half) {
+				const int e = 2 * i + half;
+				float v = 0.0F;
+				if (e < kImgW2) { const int r = e / kLd1, c = e - r * kLd1; if (c < kMlpK1) v = prm[kW1 + r * kMlpK1 + c]; }
+				else if (e < kImgW3) { const int q = e - kImgW2, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW2 + r * kMlpHid + c]; }
+				else if (e < kImgW4) { const int q = e - kImgW3, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW3 + r * kMlpHid + c]; }
+				else { const int q = e - kImgW4, r = q / kLdH, c = q - r * kLdH; if (r < kMlpOut && c < kMlpHid) v = prm[kW4 + r * kMlpHid + c]; }
+				h[e] = bf16_bits(v); }}
+		else {
+			const int q = i - kImgBiasF;
+			float v = 0.0F;
+			if (q < 3 * kMlpHid) v = prm[kB1 + (q / kMlpHid) * (kMlpHid + kMlpHid * kMlpHid) + q % kMlpHid];
+			else if (q < 3 * kMlpHid + 4) { const int c = q - 3 * kMlpHid; v = c < kMlpOut ? prm[kB4 + c] : 0.0F; }
+			else v = prm[kNorm + (q - 3 * kMlpHid - 4)];
+			fl[i] = v; }}}
+
+// the 13 network inputs of a fragment
+__device__ inline void mlp_inputs(const ShadeInput& s, const float* norm, float x[kMlpFeat]) {
+	x[0] = s.n[0]; x[1] = s.n[1]; x[2] = s.n[2];
+	x[3] = s.v[0]; x[4] = s.v[1]; x[5] = s.v[2];
+	x[6] = (s.p[0] - norm[0]) * norm[3]; x[7] = (s.p[1] - norm[1]) * norm[3]; x[8] = (s.p[2] - norm[2]) * norm[3];
+	x[9] = s.albedo[0]; x[10] = s.albedo[1]; x[11] = s.albedo[2];
+	x[12] = s.rough; }
+
+// frequency encoding, 9 values per input: x, then (sin, cos)(2^k pi x) for
+// k = 0..3 by the double-angle recurrence from sin(pi x), cos(pi x)
+__device__ inline float mlp_enc_value(const float x[kMlpFeat], int e) {
+	if (e >= kMlpEnc) return 0.0F;
+	const int i = e / 9, j = e - 9 * (e / 9);
+	const float xi = x[i];
+	if (j == 0) return xi;
+	float sn = __builtin_amdgcn_sinf(0.5F * xi), cs = __builtin_amdgcn_cosf(0.5F * xi);   // revolutions
+	const int kk = (j - 1) >> 1;
+	for (int q = 0; q < kk; ++q) { const float s2 = 2.0F * sn * cs, c2 = (cs - sn) * (cs + sn); sn = s2; cs = c2; }
+	return (j & 1) ? sn : cs; }
+
+// one wave: 64 samples (one per lane) through the network.  xc: this
+// wave's [64][40] chunk of encoded inputs; hb: its [64][72] activations;
+// W: the LDS image.  Every wave of the workgroup calls it together (it
+// synchronises the workgroup between phases).
+__device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* xc, uint16_t* hb,
+                                const float x[kMlpFeat], float y[kMlpOut]) {
+	const int lane = threadIdx.x & 63, r16 = lane & 15, g = lane >> 4;
+	f32x4_t acc[4][4];
+#pragma unroll
+	for (int i = 0; i < 4; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+	// layer 1, K = 128 in four chunks of 32 encoded inputs
+#pragma unroll
+	for (int c = 0; c < kMlpK1 / 32; ++c) {
+		uint32_t w[16];
+#pragma unroll
+		for (int q = 0; q < 16; ++q)
+			w[q] = static_cast<uint32_t>(bf16_bits(mlp_enc_value(x, 32 * c + 2 * q))) |
+			       (static_cast<uint32_t>(bf16_bits(mlp_enc_value(x, 32 * c + 2 * q + 1))) << 16);
+		uint4* dst = reinterpret_cast<uint4*>(xc + lane * 40);
+#pragma unroll
+		for (int q = 0; q < 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+		__syncthreads();
+		bf16x8_t a[4], b[4];
+#pragma unroll
+		for (int i = 0; i < 4; ++i) a[i] = lds_bf16x8(W + kImgW1 + (16 * i + r16) * kLd1 + 32 * c + g * 8);
+#pragma unroll
+		for (int j = 0; j < 4; ++j) b[j] = lds_bf16x8(xc + (16 * j + r16) * 40 + g * 8);
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16(a[i], b[j], acc[i][j]);
+		__syncthreads(); }
+	// layers 2 and 3: transposed products H^T = W H_prev^T, samples along N
+#pragma unroll
+	for (int layer = 0; layer < 3; ++layer) {
+		const float* bl = bias + layer * kMlpHid;
+		// epilogue of the previous product: bias, ReLU, bf16, H[sample][neuron]
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				const int n0 = 16 * i + 4 * g;
+				uint16_t hv[4];
+#pragma unroll
+				for (int r = 0; r < 4; ++r) {
+					const float v = acc[i][j][r] + bl[n0 + r];
+					hv[r] = bf16_bits(v > 0.0F ? v : 0.0F); }
+				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) =
+					make_uint2(hv[0] | (static_cast<uint32_t>(hv[1]) << 16), hv[2] | (static_cast<uint32_t>(hv[3]) << 16)); }
+		__syncthreads();
+		if (layer == 2) break;
+		const uint16_t* Wl = W + (layer == 0 ? kImgW2 : kImgW3);
+		bf16x8_t a[2][4], b[2][4];
+#pragma unroll
+		for (int c = 0; c < 2; ++c) {
+#pragma unroll
+			for (int i = 0; i < 4; ++i) a[c][i] = lds_bf16x8(Wl + (16 * i + r16) * kLdH + 32 * c + g * 8);
+#pragma unroll
+			for (int j = 0; j < 4; ++j) b[c][j] = lds_bf16x8(hb + (16 * j + r16) * kLdH + 32 * c + g * 8); }
+#pragma unroll
+		for (int i = 0; i < 4; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				f32x4_t s = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+				s = mfma_bf16(a[0][i], b[0][j], s);
+				acc[i][j] = mfma_bf16(a[1][i], b[1][j], s); }
+		__syncthreads(); }
+	// layer 4: 3 outputs in rows 0..2 of one 16-row tile
+	f32x4_t o[4];
+	{
+		bf16x8_t a[2], b[2][4];
+#pragma unroll
+		for (int c = 0; c < 2; ++c) {
+			a[c] = lds_bf16x8(W + kImgW4 + r16 * kLdH + 32 * c + g * 8);
+#pragma unroll
+			for (int j = 0; j < 4; ++j) b[c][j] = lds_bf16x8(hb + (16 * j + r16) * kLdH + 32 * c + g * 8); }
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			f32x4_t s = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+			s = mfma_bf16(a[0], b[0][j], s);
+			o[j] = mfma_bf16(a[1], b[1][j], s); }
+	}
+	__syncthreads();
+	// lanes 0..15 hold rows 0..3 (channels) of samples 16 j + lane
+	float* yo = reinterpret_cast<float*>(xc);
+	if (g == 0) {
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) yo[(16 * j + r16) * 4 + r] = o[j][r]; }
+	__syncthreads();
+	const float* b4 = bias + 3 * kMlpHid;
+#pragma unroll
+	for (int c = 0; c < kMlpOut; ++c) y[c] = yo[lane * 4 + c] + b4[c];
+	__syncthreads(); }
+
+constexpr int kMlpWaves = 8;   // waves per inference workgroup (64 samples each)
+
+struct MlpShared {
+	uint16_t w[kImgEnd];
+	float bias[3 * kMlpHid + 4];
+	float norm[4];
+	uint16_t xc[kMlpWaves][64 * 40];
+	uint16_t hb[kMlpWaves][64 * kLdH];
+};
+
+__device__ inline void mlp_load_image(MlpShared& sh, const uint32_t* __restrict__ img) {
+	const uint4* src = reinterpret_cast<const uint4*>(img);
+	uint4* dst = reinterpret_cast<uint4*>(sh.w);
+	for (int i = threadIdx.x; i < kImgEnd / 8; i += blockDim.x) dst[i] = src[i];
+	const float* fb = reinterpret_cast<const float*>(img) + kImgBiasF;
+	for (int i = threadIdx.x; i < 3 * kMlpHid + 4; i += blockDim.x) sh.bias[i] = fb[i];
+	if (threadIdx.x < 4) sh.norm[threadIdx.x] = fb[3 * kMlpHid + 4 + threadIdx.x]; }
+
+__device__ inline uint32_t srgb_unit_to8(float v) {   // sRGB-space [0,1] -> byte
+	v = v > 0.0F ? v : 0.0F;
+	v = v < 1.0F ? v : 1.0F;
+	return static_cast<uint32_t>(v * 255.0F + 0.5F); }
+
+// dense inference: n samples of kMlpSample floats (13 inputs used) -> n x 3
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(const float* __restrict__ in, int n,
+		const uint32_t* __restrict__ img, float* __restrict__ out) {
+	__shared__ __attribute__((aligned(16))) MlpShared sh;
+	mlp_load_image(sh, img);
+	__syncthreads();
+	const int wv = threadIdx.x >> 6;
+	const long long i = static_cast<long long>(blockIdx.x) * (64 * kMlpWaves) + threadIdx.x;
+	float x[kMlpFeat];
+	ShadeInput s;
+	const bool ok = i < n;
+	const float* r = in + (ok ? i : 0) * kMlpSample;
+	s.n[0] = r[0]; s.n[1] = r[1]; s.n[2] = r[2]; s.v[0] = r[3]; s.v[1] = r[4]; s.v[2] = r[5];
+	s.p[0] = r[6]; s.p[1] = r[7]; s.p[2] = r[8]; s.albedo[0] = r[9]; s.albedo[1] = r[10]; s.albedo[2] = r[11];
+	s.rough = r[12];
+	mlp_inputs(s, sh.norm, x);
+	float y[kMlpOut];
+	mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
+	if (ok) { out[i * 3 + 0] = y[0]; out[i * 3 + 1] = y[1]; out[i * 3 + 2] = y[2]; }}
+
+// ---------------------------------------------------------------------------
+// learned resolve: like k_resolve, but the winner's shading inputs go
+// through the network; 64 x kMlpWaves pixels per workgroup (one row per
+// wave).  Depth and primitive id are the same stores as the plain resolve.
+// ---------------------------------------------------------------------------
+template <class P>
+__device__ inline bool resolve_inputs(const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+                                      const uint32_t* rcpTab, ShadeInput& s, float& dep, uint32_t& prim) {
+	const long long o = static_cast<long long>(y) * f.W + x;
+	const unsigned long long k = a.keys[o];
+	dep = f.clearDepth;
+	prim = 0xffffffffu;
+	if (!(k < clear_key(f))) return false;
+	const uint32_t slot = key_slot(k, f.depthFunc == 1);
+	const int di = find_draw_slot(dl, f.nDraws, slot);
+	const DrawLite& dd = dl[di];
+	const uint32_t rel = slot - dd.slotBase;
+	const uint32_t nTri = static_cast<uint32_t>(dd.tris) * static_cast<uint32_t>(dd.instances);
+	const int tx = x / f.tileW, ty = y / f.tileH;
+	Rect R;
+	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
+	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
+	TriVerts tr;
+	TriSetup t;
+	uint32_t srcTri;
+	const float* vary;
+	if (rel < nTri) {
+		srcTri = rel;
+		inline_verts(a.tvert[dd.triBase + rel], a.vpos, tr);
+		setup_vector(R, tr.X, tr.Y, t);
+		vary = a.vvary; }
+	else {
+		const uint32_t fslot = rel - nTri;
+		srcTri = fslot / kFanMax;
+		const uint32_t j = fslot - srcTri * kFanMax;
+		const uint32_t cq = a.clipqOf[dd.triBase + srcTri];
+		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+		setup_scalar(R, tr.X, tr.Y, t);
+		vary = a.cvvary; }
+	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
+	const int lane = (x & 1) | ((y & 1) << 1);
+	const DrawDesc& d = a.draws[di];
+	// the quad lane exactly as shade_pixel interpolates it
+	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
+	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
+	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
+	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
+	const float* v0 = vary + tr.vary[0] * kMaxVary;
+	const float* v1 = vary + tr.vary[1] * kMaxVary;
+	const float* v2 = vary + tr.vary[2] * kMaxVary;
+	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
+	dep = own.depth;
+	float vv[P::kVary];
+#pragma unroll
+	for (int k2 = 0; k2 < P::kVary; ++k2) vv[k2] = lane_interp(own, v0, v1, v2, k2);
+	const LaneBary l0 = lane_bary(t, tr, q1, q2, rcpTab);
+	const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
+	const int lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
+	                            lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1));
+	P::inputs(d, vv, lod, s);
+	prim = dd.primBase + srcTri;
+	return true; }
+
+// learned resolve, pass 1: like k_resolve for depth, ids and the clear
+// colour; each covered pixel's shading inputs are queued (16 floats: the 13
+// inputs, then the pixel index) for the dense network pass.  The queue is
+// cut into kQueueSegs segments with a counter each, picked by block index,
+// so that consecutive workgroups (dealt round-robin to the XCDs) append
+// through different counters.
+constexpr int kQueueSegs = 8;
+constexpr int kQueueStride = 32;   // counters 128 bytes apart
+
+template <class P>
+__global__ __launch_bounds__(256) void k_mlp_gather(FrameDesc f, RasterArgs a, float* __restrict__ queue, uint32_t* __restrict__ count,
+                                                    uint32_t segCap) {
+	__shared__ DrawLite dl[kMaxDraws];
+	__shared__ uint32_t rcpTab[2048];
+	load_lite(dl, a.draws, f.nDraws);
+	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
+	__syncthreads();
+	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
+	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
+	const bool on = x < f.W && y < f.H;
+	ShadeInput s;
+	float dep = f.clearDepth;
+	uint32_t prim = 0xffffffffu;
+	const bool hit = on && resolve_inputs<P>(f, a, dl, x, y, rcpTab, s, dep, prim);
+	const long long o = static_cast<long long>(y) * f.W + x;
+	// one queue reservation per workgroup
+	__shared__ uint32_t wcnt[5];
+	const unsigned long long m = __ballot(hit);
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	if (lane == 0) wcnt[wv] = static_cast<uint32_t>(__popcll(m));
+	__syncthreads();
+	const int seg = static_cast<int>((blockIdx.y * gridDim.x + blockIdx.x) % kQueueSegs);
+	if (threadIdx.x == 0) {
+		const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+		wcnt[4] = tot ? atomicAdd(&count[seg * kQueueStride], tot) : 0u; }
+	__syncthreads();
+	uint32_t base = wcnt[4];
+	for (int w = 0; w < wv; ++w) base += wcnt[w];
+	if (hit) {
+		const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
+		if (slot < segCap) {
+			float4* r = reinterpret_cast<float4*>(queue + (static_cast<long long>(seg) * segCap + slot) * kMlpSample);
+			r[0] = make_float4(s.n[0], s.n[1], s.n[2], s.v[0]);
+			r[1] = make_float4(s.v[1], s.v[2], s.p[0], s.p[1]);
+			r[2] = make_float4(s.p[2], s.albedo[0], s.albedo[1], s.albedo[2]);
+			r[3] = make_float4(s.rough, u2f(static_cast<uint32_t>(o)), 0.0F, 0.0F); }}
+	if (!on) return;
+	if (f.outColor && !hit) f.outColor[o] = post_pixel(f, f.clearColor, kSrgb8Tab);
+	if (f.outDepth) f.outDepth[o] = dep;
+	if (f.outPrim) f.outPrim[o] = prim; }
+
+// learned resolve, pass 2: the queued fragments through the network, 512
+// per round; persistent workgroups keep the weights in LDS across rounds
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(FrameDesc f, const float* __restrict__ queue,
+		const uint32_t* __restrict__ count, uint32_t segCap, const uint32_t* __restrict__ img) {
+	__shared__ __attribute__((aligned(16))) MlpShared sh;
+	mlp_load_image(sh, img);
+	__syncthreads();
+	constexpr uint32_t kRound = 64 * kMlpWaves;
+	uint32_t cnt[kQueueSegs], first[kQueueSegs + 1];
+	first[0] = 0;
+#pragma unroll
+	for (int q = 0; q < kQueueSegs; ++q) {
+		cnt[q] = min(count[q * kQueueStride], segCap);
+		first[q + 1] = first[q] + (cnt[q] + kRound - 1) / kRound; }
+	const int wv = threadIdx.x >> 6;
+	for (uint32_t item = blockIdx.x; item < first[kQueueSegs]; item += gridDim.x) {
+		int seg = 0;
+#pragma unroll
+		for (int q = 1; q < kQueueSegs; ++q) if (item >= first[q]) seg = q;
+		const uint32_t base = (item - first[seg]) * kRound;
+		const uint32_t i = base + threadIdx.x;
+		const bool ok = i < cnt[seg];
+		const float4* r = reinterpret_cast<const float4*>(queue + (static_cast<long long>(seg) * segCap + (ok ? i : base)) * kMlpSample);
+		const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+		ShadeInput s;
+		s.n[0] = r0.x; s.n[1] = r0.y; s.n[2] = r0.z; s.v[0] = r0.w; s.v[1] = r1.x; s.v[2] = r1.y;
+		s.p[0] = r1.z; s.p[1] = r1.w; s.p[2] = r2.x; s.albedo[0] = r2.y; s.albedo[1] = r2.z; s.albedo[2] = r2.w;
+		s.rough = r3.x;
+		const uint32_t o = f2u(r3.y);
+		float x[kMlpFeat];
+		mlp_inputs(s, sh.norm, x);
+		float y[kMlpOut];
+		mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
+		if (ok && f.outColor)
+			f.outColor[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}
+
+// per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
+// each tile, squared 8-bit error against the network's output
+constexpr int kMlpProbes = 64;
+
+__device__ inline uint32_t hash32(uint32_t v) {
+	v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16; return v; }
+
+template <class P>
+__global__ __launch_bounds__(256) void k_mlp_probe(FrameDesc f, RasterArgs a, float* __restrict__ tileErr) {
+	__shared__ DrawLite dl[kMaxDraws];
+	load_lite(dl, a.draws, f.nDraws);
+	__syncthreads();
+	const long long id = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+	if (id >= static_cast<long long>(f.nTiles) * kMlpProbes || f.outColor == nullptr) return;
+	const int tile = static_cast<int>(id / kMlpProbes), j = static_cast<int>(id % kMlpProbes);
+	const Rect R = tile_rect(f, tile);
+	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
+	const uint32_t h = hash32(static_cast<uint32_t>(id) * 0x9e3779b9u + 17u);
+	const int x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
+	const int y = R.y0 + (j >> 3) * ch + static_cast<int>((h >> 16) % static_cast<uint32_t>(ch));
+	if (x >= R.x1 || y >= R.y1) return;
+	float col[3], dep;
+	uint32_t prim;
+	if (!resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, kRcpMantissa, col, dep, prim)) return;
+	const uint32_t want = post_pixel(f, col, kSrgb8Tab);
+	const uint32_t got = f.outColor[static_cast<long long>(y) * f.W + x];
+	float se = 0.0F;
+	for (int c = 0; c < 3; ++c) {
+		const float dv = static_cast<float>(static_cast<int>((want >> (8 * c)) & 255u) - static_cast<int>((got >> (8 * c)) & 255u));
+		se += dv * dv; }
+	atomicAdd(&tileErr[2 * tile], se);
+	atomicAdd(&tileErr[2 * tile + 1], 3.0F); }
+
+// pixels of tiles whose estimated RMS error exceeds the limit are re-shaded
+// with the plain shader (colour only; depth and ids are already stored);
+// one thread per pixel, tiles under the limit exit at once
+template <class P>
+__global__ __launch_bounds__(256) void k_mlp_fallback(FrameDesc f, RasterArgs a, const float* __restrict__ tileErr,
+                                                      float maxMse, uint32_t* __restrict__ nFallback) {
+	__shared__ DrawLite dl[kMaxDraws];
+	__shared__ uint32_t rcpTab[2048];
+	__shared__ uint32_t srgbTab[104];
+	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
+	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
+	// the block's rows lie in one tile row and span at most two tile columns
+	const int ty = (blockIdx.y * kResolveH) / f.tileH;
+	const int tx0 = (blockIdx.x * kResolveW) / f.tileW, tx1 = min(blockIdx.x * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
+	bool any = false;
+	for (int tx = tx0; tx <= tx1; ++tx) {
+		const int t = ty * f.tilesX + tx;
+		const float c = tileErr[2 * t + 1];
+		any = any || (c > 0.0F && tileErr[2 * t] / c > maxMse); }
+	if (!any || f.outColor == nullptr) return;   // uniform over the block
+	load_lite(dl, a.draws, f.nDraws);
+	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
+	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
+	__syncthreads();
+	if (x >= f.W || y >= f.H) return;
+	const int tile = (y / f.tileH) * f.tilesX + x / f.tileW;
+	const float cnt = tileErr[2 * tile + 1];
+	if (!(cnt > 0.0F) || !(tileErr[2 * tile] / cnt > maxMse)) return;
+	if (x % f.tileW == 0 && y % f.tileH == 0) atomicAdd(nFallback, 1u);
+	float col[3], dep;
+	uint32_t prim;
+	if (resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, rcpTab, col, dep, prim))
+		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, srgbTab); }
+
+// training samples from a frame: covered pixels kept with probability
+// `keep` (hash of pixel and seed), appended as 13 inputs + the plain
+// shader's sRGB colour / 255
+template <class P>
+__global__ __launch_bounds__(256) void k_mlp_samples(FrameDesc f, RasterArgs a, float* __restrict__ out, uint32_t cap,
+                                                     uint32_t* __restrict__ count, float keep, uint32_t seed) {
+	__shared__ DrawLite dl[kMaxDraws];
+	load_lite(dl, a.draws, f.nDraws);
+	__syncthreads();
+	const long long id = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+	if (id >= static_cast<long long>(f.W) * f.H) return;
+	const int x = static_cast<int>(id % f.W), y = static_cast<int>(id / f.W);
+	const uint32_t h = hash32(static_cast<uint32_t>(id) ^ hash32(seed));
+	if (static_cast<float>(h >> 8) * (1.0F / 16777216.0F) >= keep) return;
+	ShadeInput s;
+	float dep;
+	uint32_t prim;
+	if (!resolve_inputs<P>(f, a, dl, x, y, kRcpMantissa, s, dep, prim)) return;
+	float col[3];
+	resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, kRcpMantissa, col, dep, prim);
+	const uint32_t px = post_pixel(f, col, kSrgb8Tab);
+	const uint32_t slot = atomicAdd(count, 1u);
+	if (slot >= cap) return;
+	float* r = out + static_cast<long long>(slot) * kMlpSample;
+	r[0] = s.n[0]; r[1] = s.n[1]; r[2] = s.n[2]; r[3] = s.v[0]; r[4] = s.v[1]; r[5] = s.v[2];
+	r[6] = s.p[0]; r[7] = s.p[1]; r[8] = s.p[2]; r[9] = s.albedo[0]; r[10] = s.albedo[1]; r[11] = s.albedo[2];
+	r[12] = s.rough;
+	r[13] = static_cast<float>((px >> 16) & 255u) * (1.0F / 255.0F);
+	r[14] = static_cast<float>((px >> 8) & 255u) * (1.0F / 255.0F);
+	r[15] = static_cast<float>(px & 255u) * (1.0F / 255.0F); }
+
+// ---------------------------------------------------------------------------
+// training
+// ---------------------------------------------------------------------------
+// batch: encoded inputs X [B][128] (fp32) and targets T [B][3] for samples
+// chosen by hash (or in order when seed == 0xffffffff)
+__global__ void k_mlp_batch(const float* __restrict__ samples, long long n, int B, uint32_t seed, uint32_t step,
+                            const float* __restrict__ prm, float* __restrict__ X, float* __restrict__ T) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= B) return;
+	long long src;
+	if (seed == 0xffffffffu) src = i;
+	else src = static_cast<long long>(hash32(hash32(seed ^ (step * 0x9e3779b9u)) + static_cast<uint32_t>(i)) % static_cast<uint64_t>(n));
+	const float* r = samples + src * kMlpSample;
+	ShadeInput s;
+	s.n[0] = r[0]; s.n[1] = r[1]; s.n[2] = r[2]; s.v[0] = r[3]; s.v[1] = r[4]; s.v[2] = r[5];
+	s.p[0] = r[6]; s.p[1] = r[7]; s.p[2] = r[8]; s.albedo[0] = r[9]; s.albedo[1] = r[10]; s.albedo[2] = r[11];
+	s.rough = r[12];
+	float x[kMlpFeat];
+	mlp_inputs(s, prm + kNorm, x);
+	float* xo = X + static_cast<long long>(i) * kMlpK1;
+	for (int e = 0; e < kMlpK1; ++e) xo[e] = mlp_enc_value(x, e);
+	T[i * 3 + 0] = r[13]; T[i * 3 + 1] = r[14]; T[i * 3 + 2] = r[15]; }
+
+// C = epilogue(op(A) op(B)) with op(A)(m,k) = transA ? A[k*lda+m] : A[m*lda+k]
+// and op(B)(k,n) = transB ? B[n*ldb+k] : B[k*ldb+n]; operands are rounded
+// to bf16 as they are staged in LDS, products accumulate in fp32.
+// blockIdx.z splits K into chunks of kChunk; chunk z writes C + z*zStride.
+struct GemmArgs {
+	const float* A; const float* B; float* C;
+	int M, N, K, lda, ldb, ldc, transA, transB;
+	const float* bias;     // + bias[n]
+	int relu;
+	const float* mask; int ldm;   // * (mask[m*ldm+n] > 0)
+	int kChunk; long long zStride;
+};
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+	__shared__ __attribute__((aligned(16))) uint16_t As[64 * 40];
+	__shared__ __attribute__((aligned(16))) uint16_t Bs[64 * 40];
+	const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+	const int kb = blockIdx.z * g.kChunk, ke = min(g.K, kb + g.kChunk);
+	const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, r16 = lane & 15, q = lane >> 4;
+	const int wm = (wv & 1) * 32, wn = (wv >> 1) * 32;
+	f32x4_t acc[2][2];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+	for (int k0 = kb; k0 < ke; k0 += 32) {
+		// stage 64 x 32 of op(A) and op(B) as [row][k] bf16
+#pragma unroll
+		for (int e = 0; e < 8; ++e) {
+			const int idx = e * 256 + tid;
+			int r, k;
+			if (g.transA) { k = idx >> 6; r = idx & 63; } else { r = idx >> 5; k = idx & 31; }
+			const int gm = m0 + r, gk = k0 + k;
+			float v = 0.0F;
+			if (gm < g.M && gk < ke) v = g.transA ? g.A[static_cast<long long>(gk) * g.lda + gm] : g.A[static_cast<long long>(gm) * g.lda + gk];
+			As[r * 40 + k] = bf16_bits(v); }
+#pragma unroll
+		for (int e = 0; e < 8; ++e) {
+			const int idx = e * 256 + tid;
+			int r, k;
+			if (g.transB) { r = idx >> 5; k = idx & 31; } else { k = idx >> 6; r = idx & 63; }
+			const int gn = n0 + r, gk = k0 + k;
+			float v = 0.0F;
+			if (gn < g.N && gk < ke) v = g.transB ? g.B[static_cast<long long>(gn) * g.ldb + gk] : g.B[static_cast<long long>(gk) * g.ldb + gn];
+			Bs[r * 40 + k] = bf16_bits(v); }
+		__syncthreads();
+		bf16x8_t a[2], b[2];
+#pragma unroll
+		for (int i = 0; i < 2; ++i) a[i] = lds_bf16x8(As + (wm + 16 * i + r16) * 40 + q * 8);
+#pragma unroll
+		for (int j = 0; j < 2; ++j) b[j] = lds_bf16x8(Bs + (wn + 16 * j + r16) * 40 + q * 8);
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+#pragma unroll
+			for (int j = 0; j < 2; ++j) acc[i][j] = mfma_bf16(a[i], b[j], acc[i][j]);
+		__syncthreads(); }
+	float* C = g.C + blockIdx.z * g.zStride;
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j)
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int m = m0 + wm + 16 * i + 4 * q + r, n = n0 + wn + 16 * j + r16;
+				if (m >= g.M || n >= g.N) continue;
+				float v = acc[i][j][r];
+				if (g.bias) v += g.bias[n];
+				if (g.relu) v = v > 0.0F ? v : 0.0F;
+				if (g.mask && !(g.mask[static_cast<long long>(m) * g.ldm + n] > 0.0F)) v = 0.0F;
+				C[static_cast<long long>(m) * g.ldc + n] = v; }}
+
+// dY = 2 (Y - T) / (3 B); loss partial sums of squared error
+__global__ void k_mlp_loss(const float* __restrict__ Y, const float* __restrict__ T, int B, float* __restrict__ dY,
+                           float* __restrict__ lossSum) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	float se = 0.0F;
+	if (i < 3 * B) {
+		const float d = Y[i] - T[i];
+		dY[i] = d * (2.0F / (3.0F * static_cast<float>(B)));
+		se = d * d; }
+	for (int o = 32; o > 0; o >>= 1) se += __shfl_xor(se, o, 64);
+	if ((threadIdx.x & 63) == 0 && se != 0.0F) atomicAdd(lossSum, se); }
+
+// column sums of dZ [rows][cols] over row chunks -> partial z of a bias gradient
+__global__ void k_mlp_colsum(const float* __restrict__ dZ, int rows, int cols, int rowChunk, float* __restrict__ out,
+                             long long zStride) {
+	const int c = threadIdx.x;
+	if (c >= cols) return;
+	const int r0 = blockIdx.x * rowChunk, r1 = min(rows, r0 + rowChunk);
+	float s = 0.0F;
+	for (int r = r0; r < r1; ++r) s += dZ[static_cast<long long>(r) * cols + c];
+	out[blockIdx.x * zStride + c] = s; }
+
+// sums the split partials; Adam update when lr > 0, else writes the gradient
+__global__ void k_mlp_adam(const float* __restrict__ part, int nz, float* __restrict__ prm, float* __restrict__ mom,
+                           float* __restrict__ vel, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                           float* __restrict__ gradOut) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= kMlpTrainable) return;
+	float gsum = 0.0F;
+	for (int z = 0; z < nz; ++z) gsum += part[static_cast<long long>(z) * kMlpTrainable + p];
+	if (gradOut) { gradOut[p] = gsum; return; }
+	const float m = b1 * mom[p] + (1.0F - b1) * gsum;
+	const float v = b2 * vel[p] + (1.0F - b2) * gsum * gsum;
+	mom[p] = m; vel[p] = v;
+	prm[p] -= lr * (m / bc1) / (__builtin_sqrtf(v / bc2) + eps); }
+
+}  // namespace rgl

@@ -25,7 +25,7 @@ class RglError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "tiles", "commands", "primitives_submitted", "primitives_clipped", "primitives_binned",
-        "tile_pairs", "clipped_tile_pairs", "fragments", "overflow")]
+        "tile_pairs", "clipped_tile_pairs", "fragments", "overflow", "fallback_tiles")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -77,6 +77,13 @@ def lib():
         L.rgl_gl_store_primitive_id.argtypes = [vp, vp]
         L.rgl_gl_reset.argtypes = [vp]
         L.rgl_texture_make_mipmap.argtypes = [vp, i]
+        L.rgl_mlp_param_count.argtypes = []
+        L.rgl_mlp_trainable_count.argtypes = []
+        L.rgl_gpu_set_learned_shading.argtypes = [vp, vp, f]
+        L.rgl_gl_store_training_samples.argtypes = [vp, vp, i, vp, f, u32]
+        L.rgl_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp]
+        L.rgl_mlp_gradient.argtypes = [vp, vp, i, vp, vp, ctypes.POINTER(f), vp]
+        L.rgl_mlp_train.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, i, f, u32, i, ctypes.POINTER(f), vp]
         _lib = L
     return _lib
 
@@ -94,6 +101,8 @@ EXPORTS = [
     "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",
     "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
     "rgl_gl_reset", "rgl_texture_make_mipmap",
+    "rgl_mlp_param_count", "rgl_mlp_trainable_count", "rgl_gpu_set_learned_shading",
+    "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
 ]
 
 
@@ -166,6 +175,27 @@ class GPU:
         names = ctypes.create_string_buffer(512)
         n = self._check(self.L.rgl_gpu_kernel_times(self.h, ms, 16, names, 512))
         return dict(zip(names.value.decode().split(","), [ms[i] for i in range(n)]))
+
+    # --- learned shading (no reference counterpart) ---------------------------
+    def set_learned_shading(self, dev_params, max_rms_err: float = 2.0 / 255.0):
+        self._check(self.L.rgl_gpu_set_learned_shading(self.h, dev_params, max_rms_err))
+
+    def gl_store_training_samples(self, dev_out, capacity, dev_count, keep, seed):
+        self._check(self.L.rgl_gl_store_training_samples(self.h, dev_out, capacity, dev_count, keep, seed))
+
+    def mlp_forward(self, dev_samples, n, dev_params, dev_out, stream=None):
+        self._check(self.L.rgl_mlp_forward(self.h, dev_samples, n, dev_params, dev_out, stream))
+
+    def mlp_gradient(self, dev_samples, n, dev_params, dev_grad, stream=None) -> float:
+        loss = ctypes.c_float()
+        self._check(self.L.rgl_mlp_gradient(self.h, dev_samples, n, dev_params, dev_grad, ctypes.byref(loss), stream))
+        return float(loss.value)
+
+    def mlp_train(self, dev_samples, n, dev_params, dev_opt, steps, batch, lr, seed, step0, stream=None) -> float:
+        loss = ctypes.c_float()
+        self._check(self.L.rgl_mlp_train(self.h, dev_samples, n, dev_params, dev_opt, steps, batch, lr, seed, step0,
+                                         ctypes.byref(loss), stream))
+        return float(loss.value)
 
     # --- rglv::GL -----------------------------------------------------------
     def gl_alloc_uniform_buffer(self, values=None):

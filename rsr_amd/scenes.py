@@ -428,8 +428,12 @@ def scene_heavy(width: int = 1920, height: int = 1080, instances: int = 256, see
     b[14] = np.random.RandomState(rough_seed).uniform(0.05, 1.0, instances).astype(np.float32)
     rng = np.random.RandomState(light_seed)
     lights = np.zeros((GGX_LIGHTS, 8), np.float32)
-    lights[:, 0:3] = np.array([0.0, 0.0, -25.0]) + rng.uniform(-30, 30, (GGX_LIGHTS, 3))
-    lights[:, 4:7] = rng.uniform(0.3, 1.0, (GGX_LIGHTS, 3)) * 60.0
+    # on a shell around the instance volume: directions uniform, 35-55 units
+    # from its centre
+    d = rng.normal(size=(GGX_LIGHTS, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    lights[:, 0:3] = np.array([0.0, 0.0, -25.0]) + d * rng.uniform(35.0, 55.0, (GGX_LIGHTS, 1))
+    lights[:, 4:7] = rng.uniform(0.3, 1.0, (GGX_LIGHTS, 3)) * 400.0
     b[13] = lights.reshape(-1)
     rng = np.random.RandomState(env_seed + 1000)
     taps = np.zeros((ENV_TAPS, 4), np.float32)

@@ -1,0 +1,132 @@
+"""GPU tests of the learned shader: the MFMA forward and the training
+gradients against the numpy restatement (bf16 rounding at the same
+operands, float64 accumulation), training on frames of seeded scenes, and
+the learned frame against the plain heavy shader (coverage, depth and ids
+untouched; PSNR; fallback)."""
+import numpy as np
+import pytest
+import torch
+
+from rsr_amd import learned, rgl, scenes
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _samples(n, seed):
+    rng = np.random.RandomState(seed)
+    s = np.zeros((n, learned.SAMPLE), np.float32)
+    nv = rng.normal(size=(n, 3)); nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+    vv = rng.normal(size=(n, 3)); vv /= np.linalg.norm(vv, axis=1, keepdims=True)
+    s[:, 0:3], s[:, 3:6] = nv, vv
+    s[:, 6:9] = np.array(learned.POS_CENTER) + rng.uniform(-25, 25, (n, 3))
+    s[:, 9:12] = rng.uniform(0, 1, (n, 3))
+    s[:, 12] = rng.uniform(0.05, 1, n)
+    s[:, 13:16] = rng.uniform(0, 1, (n, 3))
+    return s
+
+
+def _params(seed):
+    p = learned.init_params(seed)
+    rng = np.random.RandomState(seed + 100)
+    # nonzero biases so that every layer has both live and dead ReLUs
+    for a, b in ((learned.B1, learned.W2), (learned.B2, learned.W3), (learned.B3, learned.W4)):
+        p[a:b] = rng.uniform(-0.3, 0.3, b - a)
+    return p
+
+
+def test_forward_matches_reference():
+    s, p = _samples(3000, 1), _params(3)
+    gpu = rgl.GPU(0)
+    ds, dp = torch.from_numpy(s).to(DEV), torch.from_numpy(p).to(DEV)
+    out = torch.zeros((len(s), 3), dtype=torch.float32, device=DEV)
+    gpu.mlp_forward(ds.data_ptr(), len(s), dp.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want, _ = learned.ref_forward(s, p)
+    err = np.abs(got - want)
+    print(f"forward: max abs err {err.max():.3e}, mean {err.mean():.3e}, output scale {np.abs(want).mean():.3f}")
+    assert err.mean() < 2e-3 and err.max() < 3e-2
+    gpu.close()
+
+
+def test_gradient_matches_reference():
+    s, p = _samples(1024, 2), _params(4)
+    gpu = rgl.GPU(0)
+    ds, dp = torch.from_numpy(s).to(DEV), torch.from_numpy(p).to(DEV)
+    g = torch.zeros(learned.TRAINABLE, dtype=torch.float32, device=DEV)
+    loss = gpu.mlp_gradient(ds.data_ptr(), len(s), dp.data_ptr(), g.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    got = g.cpu().numpy().astype(np.float64)
+    ref_loss, want = learned.ref_gradient(s, p)
+    print(f"loss gpu {loss:.6f} ref {ref_loss:.6f}")
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    names = ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4")
+    for name, a, b in zip(names, learned.layers(got), learned.layers(want)):
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(f"{name}: |g| {np.linalg.norm(b):.4e} rel err {rel:.2e}")
+        assert rel < 2e-2, name
+    # the padded inputs of layer 1 get no gradient
+    assert np.all(learned.layers(got)[0][:, learned.ENC:] == 0)
+    # a training step moves the loss down on this batch
+    opt = torch.zeros(2 * learned.TRAINABLE, dtype=torch.float32, device=DEV)
+    for k in range(20):
+        gpu.mlp_train(ds.data_ptr(), len(s), dp.data_ptr(), opt.data_ptr(), 1, 1024, 3e-3, 7, k,
+                      torch.cuda.current_stream().cuda_stream)
+    loss2 = gpu.mlp_gradient(ds.data_ptr(), len(s), dp.data_ptr(), g.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    print(f"loss after 20 Adam steps {loss2:.6f}")
+    assert loss2 < 0.8 * loss
+    gpu.close()
+
+
+def _frame(gpu, sc, params=None, max_rms=2.0 / 255.0):
+    gpu.set_learned_shading(params.data_ptr() if params is not None else None, max_rms)
+    c, d, p, st = rgl.render(sc, gpu=gpu)
+    return c, d, p, st
+
+
+def test_learned_frame_and_training_gain():
+    gpu = rgl.GPU(0)
+    w, h = 640, 360
+    samples = learned.collect_samples(gpu, list(range(100, 108)), 30000, w, h, DEV)
+    n = samples.shape[0]
+    print(f"{n} training samples")
+    assert n > 100000
+    s_host = samples.cpu().numpy()
+    assert np.all(np.isfinite(s_host)) and s_host[:, 13:16].min() >= 0 and s_host[:, 13:16].max() <= 1
+    p0 = torch.from_numpy(learned.init_params(0)).to(DEV)
+    params = p0.clone()
+    opt = torch.zeros(2 * learned.TRAINABLE, dtype=torch.float32, device=DEV)
+    g = torch.zeros(learned.TRAINABLE, dtype=torch.float32, device=DEV)
+    stream = torch.cuda.current_stream().cuda_stream
+    loss0 = gpu.mlp_gradient(samples.data_ptr(), 8192, params.data_ptr(), g.data_ptr(), stream)
+    learned.train(gpu, samples, params, opt, epochs=6, batch=4096, lr=3e-3, lr_end=2e-4)
+    loss1 = gpu.mlp_gradient(samples.data_ptr(), 8192, params.data_ptr(), g.data_ptr(), stream)
+    print(f"training loss {loss0:.5f} -> {loss1:.5f}")
+    assert loss1 < loss0 / 10
+
+    sc = scenes.scene_heavy(w, h)                      # evaluation layout, not a training seed
+    pc, pd, pp, pst = _frame(gpu, sc)                   # plain heavy shader
+    cov = pd < 1.0
+    lc, ld, lp, lst = _frame(gpu, sc, params, max_rms=1.0)     # network everywhere (no fallback)
+    assert np.array_equal(pd.view(np.uint32), ld.view(np.uint32)) and np.array_equal(pp, lp)
+    assert lst["fallback_tiles"] == 0
+    uc, _, _, _ = _frame(gpu, sc, p0, max_rms=1.0)             # the untrained network
+    fc, fd, fp, fst = _frame(gpu, sc, params)                  # with the 2/255 per-tile fallback
+    assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
+    q_trained, q_untrained, q_fallback = learned.psnr(pc, lc, cov), learned.psnr(pc, uc, cov), learned.psnr(pc, fc, cov)
+    print(f"PSNR over covered pixels: untrained {q_untrained:.2f} dB, trained {q_trained:.2f} dB, "
+          f"trained + fallback {q_fallback:.2f} dB ({fst['fallback_tiles']} of {fst['tiles']} tiles re-shaded)")
+    assert q_trained > q_untrained + 10.0
+    assert q_fallback >= q_trained
+    # background pixels are the clear colour in every variant
+    assert np.array_equal(pc[~cov], lc[~cov]) and np.array_equal(pc[~cov], fc[~cov])
+    # a zero error budget re-shades every tile whose probes saw any error
+    zc, _, _, zst = _frame(gpu, sc, params, max_rms=0.0)
+    th, tw = 64, 64
+    same = sum(np.array_equal(zc[y:y + th, x:x + tw], pc[y:y + th, x:x + tw])
+               for y in range(0, h, th) for x in range(0, w, tw))
+    print(f"zero budget: {zst['fallback_tiles']} tiles re-shaded, {same} tiles equal to the plain frame")
+    assert zst["fallback_tiles"] > 0 and same >= zst["fallback_tiles"]
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
