@@ -131,6 +131,35 @@ def bench_learned(steps: int, warmup: int):
     }
 
 
+def bench_batch(steps: int):
+    """The batched workload: 64 independent scenes (128x128 to 1024x768,
+    500 to 50k triangles each) drawn by one launch sequence replayed from a
+    captured HIP graph; tile size and pairs per work item picked by the
+    autotuner on the batch itself."""
+    import torch
+    from rsr_amd import rgl, scenes
+    from rsr_amd.autotune import autotune, time_batch
+    dev = "cuda:0"
+    batch = scenes.scene_batch()
+    ex = rgl.GPU(0)
+    ex.set_graph(True)
+    frames, choice, table = autotune(ex, batch, dev, steps=5)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ms = time_batch(ex, frames, stream, steps)
+    st = ex.stats()
+    tris = sum(sc.triangles for sc in batch)
+    out = {"workload": "batch64 (scene_batch seed 4)", "scenes": len(batch), "triangles": tris,
+           "pixels": int(sum(sc.width * sc.height for sc in batch)), "batch_ms": round(ms, 4),
+           "mtris_per_s": round(tris / (ms * 1e-3) / 1e6, 2),
+           "gfrag_per_s": round(st["fragments"] / (ms * 1e-3) / 1e9, 4),
+           "graph_replay": True, "graph_captures": ex.graph_captures(),
+           "autotune": {"choice": choice, "table": table}}
+    ex.close()
+    for g, _ in frames:
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +167,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-shading (heavy shader) measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -244,6 +274,8 @@ def main():
     }
     if not args.no_learned:
         out["learned"] = bench_learned(max(20, args.steps // 4), max(5, args.warmup // 4))
+    if not args.no_batch:
+        out["batched"] = bench_batch(max(10, args.steps // 10))
     if not args.no_cpu:
         cb = cpu_baseline(scene)
         out["cpu_baseline"] = cb

@@ -140,6 +140,34 @@ int rgl_gl_reset(rgl_gpu* gpu);                                                 
    fills the box-filtered mip chain below it. */
 int rgl_texture_make_mipmap(float* host_texels, int dim);
 
+/* --- batches ----------------------------------------------------------- */
+/* No counterpart in the reference, which runs one GPU object per canvas
+   (several at once for shadow passes, node/gllayer.cxx:152-181).  Draws
+   the frames recorded in frames[0..n) (each its own rgl_gpu recorder with
+   its own Reset size, tile size, state and stores; 1 <= n <= 64) with ONE
+   launch sequence on gpu's buffers and hip_stream: triangles, tiles and
+   resolve blocks are numbered across the batch.  Every frame's outputs
+   are the ones it would get alone.  gpu may be one of the frames.  Wait
+   and stats on gpu cover the whole batch. */
+int rgl_gpu_run_batch(rgl_gpu* gpu, rgl_gpu* const* frames, int n, void* hip_stream);
+
+/* enable != 0: the launch sequence of a run is captured once into a HIP
+   graph and replayed while the batch's plan (draws, frames, buffers,
+   capacities) is unchanged; runs with kernel timing on are never captured. */
+int rgl_gpu_set_graph(rgl_gpu* gpu, int enable);
+int rgl_gpu_graph_captures(rgl_gpu* gpu);   /* graphs captured so far */
+
+/* Guard band: mode 0 (default) = the reference's CalcGuardBandFactor
+   (rglv_view_frustum.hxx:257-260), which drops below 1 for targets wider
+   or taller than 2048 pixels, so that on-screen triangles beyond +-factor
+   NDC are culled or clipped as in the reference; mode 1 = the factor raised
+   to at least 1 (a documented deviation for large targets). */
+int rgl_gpu_set_guard_band(rgl_gpu* gpu, int mode);
+
+/* (triangle, tile) pairs per visibility work item (default 512): the
+   autotuner's knob for how many workgroups share a dense tile. */
+int rgl_gpu_set_raster_params(rgl_gpu* gpu, int seg_pairs);
+
 /* --- learned shading --------------------------------------------------- */
 /* No counterpart in the reference (its shaders are analytic).  A small MLP
    (13 shading inputs, 4-band frequency encoding to 117 values, 3 hidden

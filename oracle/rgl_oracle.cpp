@@ -527,10 +527,14 @@ struct Rect { int x0, y0, x1, y1; };
 
 struct Frustum {  // ViewFrustum (rglv_view_frustum.hxx:257-315)
 	float factor;
-	explicit Frustum(int w, int h) {
+	// fixedBand: the factor is raised to at least 1 (a guard band no smaller
+	// than the viewport); the reference's formula falls below 1 above 2048
+	// pixels and then culls on-screen triangles
+	explicit Frustum(int w, int h, bool fixedBand = false) {
 		const int longSide = std::max(w, h);
 		const int half = longSide / 2;
-		factor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half); }
+		factor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half);
+		if (fixedBand && factor < 1.0F) factor = 1.0F; }
 	uint32_t Test(const float p[4]) const {  // SIMD version, cmple
 		const float w = p[3] * factor;
 		uint32_t f = 0;
@@ -565,6 +569,7 @@ struct rglo_gpu {
 
 	// render target
 	int W = 0, H = 0, tileW = 64, tileH = 64, tilesX = 0, tilesY = 0;
+	bool fixedGuardBand = false;
 	std::vector<std::vector<TileOp>> tiles;
 	std::vector<ClippedVertex> clipped;
 	std::vector<uint32_t> installed;
@@ -675,7 +680,7 @@ struct rglo_gpu {
 		if (!pgm) { err = "unknown program"; return false; }
 		DrawEnv env;
 		if (!MakeEnv(state, env)) return false;
-		const Frustum frustum(W, H);
+		const Frustum frustum(W, H, fixedGuardBand);
 		const Rect sr = ScissorRect(state);
 		float DS[2], DO[2]; DSDO(state, DS, DO);
 		const int tlx = sr.x0, tly = sr.y0, brx = sr.x1 - 1, bry = sr.y1 - 1;
@@ -1107,6 +1112,7 @@ extern "C" {
 rglo_gpu* rglo_gpu_create(int threads) { auto* g = new rglo_gpu(); g->threads = threads; return g; }
 void rglo_gpu_destroy(rglo_gpu* g) { delete g; }
 void rglo_gpu_set_threads(rglo_gpu* g, int threads) { g->threads = threads; }
+void rglo_gpu_set_guard_band(rglo_gpu* g, int fixed) { g->fixedGuardBand = fixed != 0; }
 const char* rglo_gpu_last_error(rglo_gpu* g) { return g->err.c_str(); }
 int rglo_gpu_install(rglo_gpu* g, int programId, uint32_t stateKey, int kind) {
 	// kind 0 bin, 1 draw, 2 blt.  Bin entries are tagged with the top bit.

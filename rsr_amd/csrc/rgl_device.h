@@ -21,7 +21,8 @@ namespace rgl {
 #include "rcpps_table.inc"
 #include "srgb8_table.inc"
 
-constexpr int kMaxDraws = 64;
+constexpr int kMaxDraws = 256;      // draws per batch
+constexpr int kMaxFrames = 64;      // frames (render targets) per batch
 constexpr int kMaxVary = 12;       // varyings per vertex (largest program, padded to 16 bytes)
 constexpr int kClipVerts = 8;      // 3 + one per clipping plane
 constexpr int kFanMax = 6;         // kClipVerts - 2
@@ -157,13 +158,19 @@ struct DrawDesc {
 	int cullingEnabled, cullFace;
 	float uniforms[32];      // UniformsSD (<= 128 bytes)
 	SamplerDesc tu0, tu1;
+	// the frame this draw renders into and its tile grid
+	int frame;
+	int tileW, tileH, tilesX, tilesY, tileBase;
 };
 
+// one render target of a batch (a GPU::Reset'd canvas with its clear,
+// depth function and stores)
 struct FrameDesc {
 	int W, H, tileW, tileH, tilesX, tilesY, nTiles;
-	int nDraws;
-	long long totalTris;     // S
-	int units;               // binning units (kUnitTris triangles each)
+	int tileBase;            // first batch tile of this frame
+	long long keyBase;       // first per-pixel key of this frame (row-major, stride W)
+	int blockBase;           // first resolve block of this frame
+	int blocksX;             // resolve blocks across (kResolveW pixels each)
 	int depthFunc;           // GL_LESS or GL_LEQUAL for the whole frame
 	int drawEnabled;         // 0 when the clear depth makes every test fail
 	float clearColor[3];
@@ -173,6 +180,26 @@ struct FrameDesc {
 	float* outDepth;
 	uint32_t* outPrim;
 };
+
+// a batch of frames drawn by one launch sequence: triangles, tiles and
+// resolve blocks are numbered across the batch
+struct BatchDesc {
+	int nFrames;
+	int nTiles;              // sum of the frames' tiles
+	int nDraws;
+	int units;               // binning units (unitTris triangles each)
+	int unitTris;            // triangles per binning unit, a multiple of 64
+	long long totalTris;     // S
+	int nBlocks;             // resolve blocks, all frames
+	int segPairs;            // (triangle, tile) pairs per visibility work item
+	const FrameDesc* frames; // device table, nFrames entries
+};
+
+// last k in [0, n) with base(k) <= v, for a nondecreasing table
+__device__ inline int find_base(const int* base, int n, int v) {
+	int lo = 0, hi = n - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (base[mid] <= v) lo = mid; else hi = mid - 1; }
+	return lo; }
 
 // ---------------------------------------------------------------------------
 // texture units (rglr_texture_sampler.cxx)

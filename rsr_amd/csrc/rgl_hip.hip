@@ -120,6 +120,8 @@ struct rgl_gpu {
 	std::vector<Cmd> commands;
 	std::vector<uint32_t> installed;
 	std::vector<SampleReq> sampleReqs;
+	long long frameTriStart = 0;    // batch triangle index of this recorder's first triangle (planning)
+	bool fixedGuardBand = false;    // rgl_gpu_set_guard_band
 
 	// learned shading (rgl_gpu_set_learned_shading)
 	const float* mlpParams = nullptr;
@@ -150,6 +152,16 @@ struct rgl_gpu {
 	Counters* dCtr = nullptr;
 	Counters* hCtr = nullptr;       // pinned
 	DrawDesc* hDraws = nullptr;     // pinned, kMaxDraws
+	FrameDesc* hFrames = nullptr;   // pinned, kMaxFrames
+	DevArray<FrameDesc> dFrames;
+	std::vector<uint8_t> lastFrames;
+	int segPairs = kSegPairs;       // (triangle, tile) pairs per visibility work item
+	// graph replay of the launch sequence (rgl_gpu_set_graph)
+	bool graphMode = false, graphValid = false;
+	uint64_t graphKey = 0;
+	hipGraphExec_t graphExec = nullptr;
+	hipStream_t capStream = nullptr;
+	int captures = 0;
 	uint32_t listCap = 0, clipCap = 0, clistCap = 0;
 	std::vector<uint8_t> lastTable;
 
@@ -190,6 +202,10 @@ struct rgl_gpu {
 		if (dCtr) (void)hipFree(dCtr);
 		if (hCtr) (void)hipHostFree(hCtr);
 		if (hDraws) (void)hipHostFree(hDraws);
+		if (hFrames) (void)hipHostFree(hFrames);
+		dFrames.release();
+		if (graphExec) (void)hipGraphExecDestroy(graphExec);
+		if (capStream) (void)hipStreamDestroy(capStream);
 		if (done) (void)hipEventDestroy(done);
 		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
 
@@ -208,21 +224,26 @@ struct rgl_gpu {
 
 	void ResetGL() { dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); sampleReqs.clear(); }
 
-	// ---- frame planning ----
-	int Plan(FrameDesc& f, std::vector<DrawDesc>& draws) {
+	// ---- frame planning: this recorder's commands as frame fi of a batch;
+	// draws are appended with batch-wide triangle and vertex numbering ----
+	int PlanFrame(FrameDesc& f, std::vector<DrawDesc>& draws, long long& triBase, long long& vBase, int fi, int tileBase) {
 		std::memset(&f, 0, sizeof(f));
 		f.W = W; f.H = H; f.tileW = tileW; f.tileH = tileH; f.tilesX = tilesX; f.tilesY = tilesY;
 		f.nTiles = tilesX * tilesY;
+		f.tileBase = tileBase;
 		f.depthFunc = -1;
 		f.drawEnabled = 1;
 		const GLState* st = nullptr;
 		bool cleared = false, stored = false;
-		long long triBase = 0, vBase = 0;
+		const size_t firstDraw = draws.size();
+		frameTriStart = triBase;
+		submitted = 0;
+		if (W <= 0 || H <= 0) return fail(RGL_E_STATE, "Reset() must set the target size before Run");
 		for (const Cmd& c : commands) {
 			switch (c.kind) {
 			case Cmd::STATE: st = &states[c.a]; break;
 			case Cmd::CLEAR: {
-				if (!draws.empty() || stored) return fail(RGL_E_UNSUPPORTED, "CLEAR after a draw or store is not supported");
+				if (draws.size() != firstDraw || stored) return fail(RGL_E_UNSUPPORTED, "CLEAR after a draw or store is not supported");
 				if (c.a & GL_STENCIL_BUFFER_BIT) return fail(RGL_E_UNSUPPORTED, "CLEAR on STENCIL not implemented");
 				if (st->color0AttachmentType == RB_COLOR_DEPTH && c.a != (GL_COLOR_BUFFER_BIT | GL_DEPTH_BUFFER_BIT))
 					return fail(RGL_E_STATE, "must clear color and depth when using RB_COLOR_DEPTH");
@@ -252,22 +273,19 @@ struct rgl_gpu {
 			default: {
 				if (stored) return fail(RGL_E_UNSUPPORTED, "draw after a store is not supported");
 				if (!st) return fail(RGL_E_STATE, "draw without state");
-				if (static_cast<int>(draws.size()) >= kMaxDraws) return fail(RGL_E_CAPACITY, "too many draws in one frame");
-				const int r = PlanDraw(*st, c, triBase, vBase, f, draws);
+				if (static_cast<int>(draws.size()) >= kMaxDraws) return fail(RGL_E_CAPACITY, "too many draws in one batch");
+				const int r = PlanDraw(*st, c, triBase, vBase, f, draws, fi);
 				if (r != 0) return r;
 				break; }}}
-		if (!cleared && !draws.empty()) return fail(RGL_E_UNSUPPORTED, "a frame must begin with Clear(COLOR|DEPTH)");
+		if (!cleared && draws.size() != firstDraw) return fail(RGL_E_UNSUPPORTED, "a frame must begin with Clear(COLOR|DEPTH)");
 		if (!f.doClearColor || !f.doClearDepth) {
-			if (!draws.empty() || f.outColor || f.outDepth) return fail(RGL_E_UNSUPPORTED, "partial clears are not supported"); }
+			if (draws.size() != firstDraw || f.outColor || f.outDepth) return fail(RGL_E_UNSUPPORTED, "partial clears are not supported"); }
 		if (f.depthFunc < 0) f.depthFunc = GL_LESS;
 		if (std::isnan(f.clearDepth)) f.drawEnabled = 0;
-		f.nDraws = static_cast<int>(draws.size());
-		f.totalTris = triBase;
-		f.units = static_cast<int>((triBase + kUnitTris - 1) / kUnitTris);
-		if (triBase * kSlotsPerTri + 1 >= 0xffffffffLL) return fail(RGL_E_CAPACITY, "too many triangles in one frame");
+		if (triBase * kSlotsPerTri + 1 >= 0xffffffffLL) return fail(RGL_E_CAPACITY, "too many triangles in one batch");
 		return 0; }
 
-	int PlanDraw(const GLState& s, const Cmd& c, long long& triBase, long long& vBase, FrameDesc& f, std::vector<DrawDesc>& draws) {
+	int PlanDraw(const GLState& s, const Cmd& c, long long& triBase, long long& vBase, FrameDesc& f, std::vector<DrawDesc>& draws, int fi) {
 		const uint32_t vkey = (static_cast<uint32_t>(s.programId) << 24) | 0x80000000u;
 		if (!Installed(vkey)) return fail(RGL_E_DISPATCH, "BIN: no dispatch entry for program");
 		if (!Installed(s.FragmentStateKey())) return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_DRAW_INLINE");
@@ -319,7 +337,7 @@ struct rgl_gpu {
 		d.nvary = nv;
 		d.triBase = triBase;
 		d.vBase = vBase;
-		d.primBase = static_cast<uint32_t>(triBase);
+		d.primBase = static_cast<uint32_t>(triBase - frameTriStart);   // ids count from the frame's first triangle
 		d.slotBase = static_cast<uint32_t>(1 + kSlotsPerTri * triBase);
 		for (int k = 0; k < 16; ++k) d.buffers[k] = s.buffers[k];
 		mat4_mul(s.projectionMatrix, s.viewMatrix, d.vpm);
@@ -332,7 +350,10 @@ struct rgl_gpu {
 		// CalcGuardBandFactor (rglv_view_frustum.hxx:257-260)
 		const int half = std::max(W, H) / 2;
 		d.gbFactor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half);
+		if (fixedGuardBand && d.gbFactor < 1.0F) d.gbFactor = 1.0F;
 		d.scissor[0] = 0; d.scissor[1] = 0; d.scissor[2] = W; d.scissor[3] = H;
+		d.frame = fi;
+		d.tileW = tileW; d.tileH = tileH; d.tilesX = tilesX; d.tilesY = tilesY; d.tileBase = f.tileBase;
 		d.keepBacks = !(s.cullingEnabled && s.cullFace == GL_BACK);
 		d.keepFronts = !(s.cullingEnabled && s.cullFace == GL_FRONT);
 		d.cullingEnabled = s.cullingEnabled; d.cullFace = s.cullFace;
@@ -354,20 +375,20 @@ struct rgl_gpu {
 		submitted += static_cast<long long>(tris) * instances;
 		return 0; }
 
-	int Ensure(const FrameDesc& f, long long vtot) {
+	int Ensure(const BatchDesc& b, long long vtot, long long totalPx) {
 		hipError_t e;
 		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
-		const long long S = std::max<long long>(f.totalTris, 1);
+		const long long S = std::max<long long>(b.totalTris, 1);
 		if ((e = vpos.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vpos");
 		if ((e = vflags.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vflags");
 		if ((e = vvary.ensure((vtot + 1) * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc vvary");
 		if ((e = trect.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
 		if ((e = tvert.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
-		const long long nh = static_cast<long long>(f.nTiles) * std::max(f.units, 1) + 1;
-		const long long nChunks = (std::max(f.units, 1) + kColChunk - 1) / kColChunk;
+		const long long nh = static_cast<long long>(b.nTiles) * std::max(b.units, 1) + 1;
+		const long long nChunks = (std::max(b.units, 1) + kColChunk - 1) / kColChunk;
 		if ((e = hist.ensure(nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
-		if ((e = bsum.ensure(nChunks * f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
-		if ((e = tileOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
+		if ((e = bsum.ensure(nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
+		if ((e = tileOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
 		if ((e = list.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc list");
 		if ((e = rec.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
@@ -378,22 +399,21 @@ struct rgl_gpu {
 		if ((e = cvvary.ensure(static_cast<size_t>(clipCap) * kClipVerts * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc cvvary");
 		if ((e = fcount.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc fcount");
 		if ((e = frect.ensure(static_cast<size_t>(clipCap) * kFanMax)) != hipSuccess) return hipfail(e, "hipMalloc frect");
-		if ((e = clipHist.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipHist");
-		if ((e = clipOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
-		if ((e = clipCursor.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
+		if ((e = clipHist.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipHist");
+		if ((e = clipOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
+		if ((e = clipCursor.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
 		if ((e = clist.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clist");
 		if ((e = crec.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
-		if ((e = workOff.ensure(f.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
-		const size_t px = static_cast<size_t>(f.W) * f.H;
+		if ((e = workOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
+		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
 		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
 		if (mlpParams) {
 			if ((e = mlpImg.ensure(kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
-			if ((e = tileErr.ensure(2LL * f.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
-			// each queue segment takes the blocks b with b % kQueueSegs == seg
-			const long long blocks = static_cast<long long>((f.W + kResolveW - 1) / kResolveW) * ((f.H + kResolveH - 1) / kResolveH);
-			mlpSegCap = static_cast<uint32_t>((blocks + kQueueSegs - 1) / kQueueSegs * 256);
+			if ((e = tileErr.ensure(2LL * b.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
+			// each queue segment takes the resolve blocks with index % kQueueSegs == seg
+			mlpSegCap = static_cast<uint32_t>((static_cast<long long>(b.nBlocks) + kQueueSegs - 1) / kQueueSegs * 256);
 			if ((e = mlpQueue.ensure(static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
 			if (!mlpShadeBlocksSet) {
 				int cus = 0;
@@ -403,36 +423,75 @@ struct rgl_gpu {
 		if (!dDraws.p) {
 			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
 			lastTable.clear(); }
+		if (!dFrames.p) {
+			if ((e = dFrames.ensure(kMaxFrames)) != hipSuccess) return hipfail(e, "hipMalloc frames");
+			lastFrames.clear(); }
 		return 0; }
-
-	template <class P>
-	void LaunchVertex(const DrawDesc& d) {
-		const long long n = static_cast<long long>(d.instances) * d.verts;
-		const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
-		k_vertex<P><<<blocks, 256, 0, stream>>>(d, vpos.p, vflags.p, vvary.p); }
-
-	template <class P>
-	void LaunchClip(int di, const FrameDesc& f) {
-		k_clip<P><<<64, 64, 0, stream>>>(dDraws.p, di, f, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr); }
 
 	void Mark(int i) { if (cur) (void)hipEventRecord(cur[i], stream); }
 
-	int Run(hipStream_t s) {
+	// uploads a host table to its device copy through pinned staging when it
+	// changed; staging is rewritten only after the last frame reading it ended
+	template <class T>
+	int UploadTable(const std::vector<T>& v, std::vector<uint8_t>& last, T* pinned, T* dev) {
+		const size_t bytes = sizeof(T) * v.size();
+		if (bytes == last.size() && (bytes == 0 || std::memcmp(last.data(), v.data(), bytes) == 0)) return 0;
+		hipError_t e;
+		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
+		std::memcpy(pinned, v.data(), bytes);
+		if (bytes && (e = hipMemcpyAsync(dev, pinned, bytes, hipMemcpyHostToDevice, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync table");
+		last.assign(reinterpret_cast<const uint8_t*>(v.data()), reinterpret_cast<const uint8_t*>(v.data()) + bytes);
+		graphValid = false;
+		return 0; }
+
+	// Run of a batch: recs[i]'s recorded commands draw frame i; this object's
+	// buffers and stream execute all of them with one launch sequence
+	int RunBatch(rgl_gpu* const* recs, int n, hipStream_t s) {
 		err.clear();
-		FrameDesc f;
+		if (n < 1 || n > kMaxFrames) return fail(RGL_E_ARG, "a batch holds 1 to 64 frames");
+		std::vector<FrameDesc> frames(n);
 		std::vector<DrawDesc> draws;
+		long long triBase = 0, vBase = 0, keyBase = 0;
+		int tileBase = 0, blockBase = 0;
 		submitted = 0;
-		int r = Plan(f, draws);
-		if (r != 0) return r;
+		for (int i = 0; i < n; ++i) {
+			rgl_gpu* r = recs[i];
+			if (!r) return fail(RGL_E_ARG, "null frame in batch");
+			for (int j = 0; j < i; ++j) if (recs[j] == r) return fail(RGL_E_ARG, "a recorder appears twice in one batch");
+			const long long sub0 = submitted;
+			const int rc = r->PlanFrame(frames[i], draws, triBase, vBase, i, tileBase);
+			if (rc != 0) { err = r->err; return rc; }
+			submitted = sub0 + r->submitted;
+			FrameDesc& f = frames[i];
+			f.keyBase = keyBase; f.blockBase = blockBase;
+			f.blocksX = (f.W + kResolveW - 1) / kResolveW;
+			tileBase += f.nTiles;
+			keyBase += static_cast<long long>(f.W) * f.H;
+			blockBase += f.blocksX * ((f.H + kResolveH - 1) / kResolveH); }
+		if (n > 1) for (int i = 0; i < n; ++i)
+			if (!recs[i]->sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples are stored from single-frame runs");
+		BatchDesc b;
+		b.nFrames = n; b.nTiles = tileBase; b.nDraws = static_cast<int>(draws.size());
+		// binning units: kUnitTris triangles, grown (in multiples of 64) when
+		// the units x tiles histogram would exceed kHistBudget entries
+		long long ut = kUnitTris;
+		if (tileBase > 0 && (triBase + ut - 1) / ut * tileBase > kHistBudget)
+			ut = ((triBase * tileBase / kHistBudget + 1 + 63) / 64) * 64;
+		b.totalTris = triBase; b.unitTris = static_cast<int>(std::min<long long>(ut, 1 << 20));
+		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
+		b.nBlocks = blockBase; b.segPairs = segPairs; b.frames = nullptr;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
-		if ((r = Ensure(f, vtot)) != 0) return r;
+		int r = Ensure(b, vtot, keyBase);
+		if (r != 0) return r;
+		b.frames = dFrames.p;
 		stream = s;
 		hipError_t e;
 		if (!dCtr) {
 			if ((e = hipMalloc(reinterpret_cast<void**>(&dCtr), sizeof(Counters))) != hipSuccess) return hipfail(e, "hipMalloc counters");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hFrames), sizeof(FrameDesc) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 		cur = nullptr;
 		if (timing) {
@@ -441,50 +500,132 @@ struct rgl_gpu {
 			cur = ev[evNext];
 			evUsed[evNext] = true;
 			evNext = (evNext + 1) % kRing; }
-		// the draw table is uploaded only when it changed; the pinned staging
-		// copy is rewritten only after the frame reading it has completed
-		const size_t tableBytes = sizeof(DrawDesc) * draws.size();
-		if (tableBytes != lastTable.size() || (tableBytes && std::memcmp(lastTable.data(), draws.data(), tableBytes) != 0)) {
-			if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
-			std::memcpy(hDraws, draws.data(), tableBytes);
-			if (tableBytes && (e = hipMemcpyAsync(dDraws.p, hDraws, tableBytes, hipMemcpyHostToDevice, stream)) != hipSuccess)
-				return hipfail(e, "hipMemcpyAsync draws");
-			lastTable.assign(reinterpret_cast<const uint8_t*>(draws.data()), reinterpret_cast<const uint8_t*>(draws.data()) + tableBytes); }
-		const long long nCsum = static_cast<long long>((f.units + kColChunk - 1) / kColChunk) * f.nTiles;
+		if ((r = UploadTable(draws, lastTable, hDraws, dDraws.p)) != 0) return r;
+		if ((r = UploadTable(frames, lastFrames, hFrames, dFrames.p)) != 0) return r;
+		if (keysReady < keys.cap) {
+			// defined contents after allocation; each frame then resets the
+			// keys of multi-segment tiles in k_pair_setup and overwrites the rest
+			k_fill_u64<<<1024, 256, 0, stream>>>(keys.p, static_cast<long long>(keys.cap), kKeySentinel);
+			keysReady = keys.cap; }
+		int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
+		for (const auto& d : draws) if (d.program != pid) pid = -1;
+		const bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
+		const std::vector<SampleReq>& reqs = recs[0]->sampleReqs;
+		if (!reqs.empty() && pid != ProgGGXEnv::kId) return fail(RGL_E_UNSUPPORTED, "training samples need the heavy-shader program");
+		// the launch sequence, replayed from a captured graph when enabled and
+		// nothing it captured has changed
+		const bool useGraph = graphMode && !timing && reqs.empty();
+		const int items = std::max<int>(WorkItemsBucket(), b.nTiles * kQuadrants);
+		if (useGraph) {
+			const uint64_t key = GraphKey(b, learned, items);
+			if (!graphValid || key != graphKey) {
+				if ((r = Capture(b, draws, learned, pid, items, key)) != 0) return r; }
+			if ((e = hipGraphLaunch(graphExec, stream)) != hipSuccess) return hipfail(e, "hipGraphLaunch"); }
+		else {
+			if ((r = Launch(b, draws, learned, pid, items, reqs)) != 0) return r; }
+		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
+		stats = rgl_stats{};
+		stats.tiles = b.nTiles;
+		stats.commands = 0;
+		for (int i = 0; i < n; ++i) stats.commands += static_cast<int64_t>(recs[i]->commands.size());
+		stats.primitives_submitted = submitted;
+		pending = true;
+		return 0; }
+
+	int Run(hipStream_t s) { rgl_gpu* self = this; return RunBatch(&self, 1, s); }
+
+	// visibility grid: the last completed frame's item count, rounded up to a
+	// power of two so that a captured graph survives small changes
+	int WorkItemsBucket() const {
+		int v = 1;
+		while (v < lastWorkItems && v < (1 << 20)) v <<= 1;
+		return lastWorkItems > 0 ? v : 0; }
+
+	uint64_t GraphKey(const BatchDesc& b, bool learned, int items) const {
+		uint64_t h = 1469598103934665603ull;
+		auto mix = [&h](const void* p, size_t n) {
+			const uint8_t* c = static_cast<const uint8_t*>(p);
+			for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }};
+		mix(&b, sizeof(b));
+		mix(lastTable.data(), lastTable.size());
+		mix(lastFrames.data(), lastFrames.size());
+		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, list.p, clipq.p, clipqOf.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, clist.p, workOff.p,
+		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
+		mix(ptrs, sizeof(ptrs));
+		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
+		                          static_cast<uint32_t>(mlpShadeBlocks) };
+		mix(caps, sizeof(caps));
+		const float lim = mlpMaxRms;
+		mix(&lim, sizeof(lim));
+		return h; }
+
+	int Capture(const BatchDesc& b, const std::vector<DrawDesc>& draws, bool learned, int pid, int items, uint64_t key) {
+		hipError_t e;
+		if (graphExec) { (void)hipGraphExecDestroy(graphExec); graphExec = nullptr; }
+		if (!capStream && (e = hipStreamCreateWithFlags(&capStream, hipStreamNonBlocking)) != hipSuccess) return hipfail(e, "hipStreamCreate");
+		hipStream_t user = stream;
+		stream = capStream;
+		if ((e = hipStreamBeginCapture(capStream, hipStreamCaptureModeThreadLocal)) != hipSuccess) { stream = user; return hipfail(e, "hipStreamBeginCapture"); }
+		static const std::vector<SampleReq> none;
+		const int r = Launch(b, draws, learned, pid, items, none);
+		hipGraph_t graph = nullptr;
+		e = hipStreamEndCapture(capStream, &graph);
+		stream = user;
+		if (r != 0) { if (graph) (void)hipGraphDestroy(graph); return r; }
+		if (e != hipSuccess) return hipfail(e, "hipStreamEndCapture");
+		e = hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(graph);
+		if (e != hipSuccess) { graphExec = nullptr; return hipfail(e, "hipGraphInstantiate"); }
+		graphKey = key;
+		graphValid = true;
+		++captures;
+		return 0; }
+
+	// every kernel of one batch, on `stream`; ends with the counters' copy
+	// to pinned host memory
+	int Launch(const BatchDesc& b, const std::vector<DrawDesc>& draws, bool learned, int pid, int items,
+	           const std::vector<SampleReq>& reqs) {
+		hipError_t e;
+		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
+		const long long nCsum = static_cast<long long>((b.units + kColChunk - 1) / kColChunk) * b.nTiles;
 		k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(
-			dCtr, clipHist.p, f.nTiles, bsum.p, nCsum, qCount.p);
+			dCtr, clipHist.p, b.nTiles, bsum.p, nCsum, qCount.p);
 		Mark(0);
-		for (const auto& d : draws) {
-			switch (d.program) {
-			case ProgAmy::kId: LaunchVertex<ProgAmy>(d); break;
-			case ProgMany::kId: LaunchVertex<ProgMany>(d); break;
-			case ProgLambertTex::kId: LaunchVertex<ProgLambertTex>(d); break;
-			case ProgGGXEnv::kId: LaunchVertex<ProgGGXEnv>(d); break;
-			default: LaunchVertex<ProgManyBlinnPhong>(d); break; }}
+		// vertex and clip stages: one launch per program present in the batch
+		bool present[5] = { false, false, false, false, false };
+		for (const auto& d : draws) present[ProgramSlot(d.program)] = true;
+		long long vtot = 0;
+		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
+		const unsigned vblocks = static_cast<unsigned>((vtot + 255) / 256);
+		if (vtot > 0) {
+			if (present[0]) k_vertex<ProgAmy><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
+			if (present[1]) k_vertex<ProgMany><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
+			if (present[2]) k_vertex<ProgLambertTex><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
+			if (present[3]) k_vertex<ProgManyBlinnPhong><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
+			if (present[4]) k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p); }
 		Mark(1);
-		if (f.units > 0) {
-			k_setup_count<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, f, vpos.p, vflags.p, trect.p, tvert.p,
+		if (b.units > 0) {
+			k_setup_count<<<b.units, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
 				hist.p, bsum.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
-		if (f.units > 0) {
-			const int nChunks = (f.units + kColChunk - 1) / kColChunk;
-			const dim3 cg((f.nTiles + 63) / 64, nChunks);
-			k_col_scan<<<1, 1024, 0, stream>>>(bsum.p, f.nTiles, nChunks, tileOff.p, &dCtr->pairs);
-			k_col_down<<<cg, 64, 0, stream>>>(hist.p, f.nTiles, f.units, bsum.p, tileOff.p); }
+		if (b.units > 0) {
+			const int nChunks = (b.units + kColChunk - 1) / kColChunk;
+			const dim3 cg((b.nTiles + 63) / 64, nChunks);
+			k_col_scan<<<1, 1024, 0, stream>>>(bsum.p, b.nTiles, nChunks, tileOff.p, &dCtr->pairs);
+			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
 		else {
-			k_fill_u32<<<1, 64, 0, stream>>>(tileOff.p, f.nTiles + 1, 0u); }
+			k_fill_u32<<<1, 64, 0, stream>>>(tileOff.p, b.nTiles + 1, 0u); }
 		Mark(3);
-		if (f.units > 0) k_scatter<<<f.units, 64, f.nTiles * sizeof(uint32_t), stream>>>(f, trect.p, hist.p, list.p, listCap, dCtr);
+		if (b.units > 0) k_scatter<<<b.units, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, trect.p, hist.p, list.p, listCap, dCtr);
 		Mark(4);
-		for (int di = 0; di < static_cast<int>(draws.size()); ++di) {
-			switch (draws[di].program) {
-			case ProgAmy::kId: LaunchClip<ProgAmy>(di, f); break;
-			case ProgMany::kId: LaunchClip<ProgMany>(di, f); break;
-			case ProgLambertTex::kId: LaunchClip<ProgLambertTex>(di, f); break;
-			case ProgGGXEnv::kId: LaunchClip<ProgGGXEnv>(di, f); break;
-			default: LaunchClip<ProgManyBlinnPhong>(di, f); break; }}
-		k_tile_scan<<<1, 1024, 0, stream>>>(f, tileOff.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
-		k_clip_scatter<<<64, 256, 0, stream>>>(f, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
+		if (present[0]) k_clip<ProgAmy><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		if (present[1]) k_clip<ProgMany><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		if (present[2]) k_clip<ProgLambertTex><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		k_tile_scan<<<1, 1024, 0, stream>>>(b, tileOff.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
+		k_clip_scatter<<<64, 256, dlBytes, stream>>>(b, dDraws.p, clipq.p, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
 		Mark(5);
 		RasterArgs ra;
 		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.list = list.p; ra.listCap = listCap;
@@ -492,61 +633,52 @@ struct rgl_gpu {
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
 		ra.keys = keys.p; ra.workOff = workOff.p;
-		if (keysReady < keys.cap) {
-			// defined contents after allocation; each frame then resets the
-			// keys of multi-segment tiles in k_pair_setup and overwrites the rest
-			k_fill_u64<<<1024, 256, 0, stream>>>(keys.p, static_cast<long long>(keys.cap), kKeySentinel);
-			keysReady = keys.cap; }
-		if (f.nTiles > 0) {
-			const int setupBlocks = static_cast<int>(std::min<long long>((2LL * f.totalTris + 255) / 256 + 1, 4096));
-			k_pair_setup<<<dim3(setupBlocks, 3), 256, 0, stream>>>(f, ra, rec.p, crec.p);
+		if (b.nTiles > 0) {
+			const int setupBlocks = static_cast<int>(std::min<long long>((2LL * b.totalTris + 255) / 256 + 1, 4096));
+			k_pair_setup<<<dim3(setupBlocks, 3), 256, 0, stream>>>(b, ra, rec.p, crec.p);
 			Mark(6);
-			// one workgroup per work item, sized from the last completed frame
-			const int items = std::max<int>(lastWorkItems, f.nTiles * kQuadrants);
-			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(f, ra, rec.p, crec.p); }
+			// one workgroup per work item, sized from the last completed frame;
+			// the kernel strides over any remainder
+			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p); }
+		else Mark(6);
 		Mark(7);
-		int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
-		for (const auto& d : draws) if (d.program != pid) pid = -1;
-		const bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
+		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		if (learned) {
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
-			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
-			k_mlp_gather<ProgGGXEnv><<<grid, 256, 0, stream>>>(f, ra, mlpQueue.p, qCount.p, mlpSegCap); }
+			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
 		else {
-			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
 			switch (pid) {
-			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<grid, 256, 0, stream>>>(f, ra); break;
-			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<grid, 256, 0, stream>>>(f, ra); break;
-			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<grid, 256, 0, stream>>>(f, ra); break;
-			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<grid, 256, 0, stream>>>(f, ra); break;
-			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<grid, 256, 0, stream>>>(f, ra); break;
-			default: k_resolve<ShadeAny><<<grid, 256, 0, stream>>>(f, ra); break; }}
+			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
+			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
+			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
+			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
+			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
+			default: k_resolve<ShadeAny><<<rblocks, 256, dlBytes, stream>>>(b, ra); break; }}
 		Mark(8);
-		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(f, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
+		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(9);
-		if (learned && f.nTiles > 0) {
+		if (learned && b.nTiles > 0) {
 			// per-tile error estimate at probe pixels, then plain re-shading of
 			// the tiles over the limit
-			k_fill_u32<<<8, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(tileErr.p), 2LL * f.nTiles, 0u);
-			const long long probes = static_cast<long long>(f.nTiles) * kMlpProbes;
-			k_mlp_probe<ProgGGXEnv><<<static_cast<int>((probes + 255) / 256), 256, 0, stream>>>(f, ra, tileErr.p);
+			k_fill_u32<<<8, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(tileErr.p), 2LL * b.nTiles, 0u);
+			const long long probes = static_cast<long long>(b.nTiles) * kMlpProbes;
+			k_mlp_probe<ProgGGXEnv><<<static_cast<int>((probes + 255) / 256), 256, dlBytes, stream>>>(b, ra, tileErr.p);
 			const float lim = mlpMaxRms * 255.0F;
-			const dim3 grid((f.W + kResolveW - 1) / kResolveW, (f.H + kResolveH - 1) / kResolveH);
-			k_mlp_fallback<ProgGGXEnv><<<grid, 256, 0, stream>>>(f, ra, tileErr.p, lim * lim, &dCtr->fallbackTiles); }
-		for (const SampleReq& q : sampleReqs) {
-			if (pid != ProgGGXEnv::kId) return fail(RGL_E_UNSUPPORTED, "training samples need the heavy-shader program");
-			const long long px = static_cast<long long>(f.W) * f.H;
-			k_mlp_samples<ProgGGXEnv><<<static_cast<int>((px + 255) / 256), 256, 0, stream>>>(f, ra, q.out, q.cap, q.count, q.keep, q.seed); }
+			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, tileErr.p, lim * lim, &dCtr->fallbackTiles); }
+		for (const SampleReq& q : reqs)
+			k_mlp_samples<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, q.out, q.cap, q.count, q.keep, q.seed);
 		Mark(10);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
-		if ((e = hipEventRecord(done, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
-		stats = rgl_stats{};
-		stats.tiles = f.nTiles;
-		stats.commands = static_cast<int64_t>(commands.size());
-		stats.primitives_submitted = submitted;
-		pending = true;
 		return 0; }
+
+	static int ProgramSlot(int pid) {
+		switch (pid) {
+		case ProgAmy::kId: return 0;
+		case ProgMany::kId: return 1;
+		case ProgLambertTex::kId: return 2;
+		case ProgGGXEnv::kId: return 4;
+		default: return 3; }}
 
 	// ---- learned shading: one forward + backward pass over a batch ----
 	void Gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb, int ldc, int tA, int tB,
@@ -813,6 +945,34 @@ int rgl_texture_make_mipmap(float* buf, int dim) {
 			dst += stride; }
 		src += srcSize * stride;
 		srcSize >>= 1; }
+	return 0; }
+
+/* ---- batches, graph replay and raster parameters (no reference counterpart) ---- */
+int rgl_gpu_run_batch(rgl_gpu* g, rgl_gpu* const* frames, int n, void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (!frames) return g->fail(RGL_E_ARG, "null frame list");
+	return g->RunBatch(frames, n, static_cast<hipStream_t>(hip_stream)); }
+
+int rgl_gpu_set_graph(rgl_gpu* g, int enable) {
+	if (check(g)) return RGL_E_ARG;
+	if (g->pending) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
+	g->graphMode = enable != 0;
+	g->graphValid = false;
+	return 0; }
+
+int rgl_gpu_graph_captures(rgl_gpu* g) { return g ? g->captures : RGL_E_ARG; }
+
+int rgl_gpu_set_guard_band(rgl_gpu* g, int mode) {
+	if (check(g)) return RGL_E_ARG;
+	if (mode != 0 && mode != 1) return g->fail(RGL_E_ARG, "guard band mode is 0 (reference) or 1 (fixed)");
+	g->fixedGuardBand = mode == 1;
+	return 0; }
+
+int rgl_gpu_set_raster_params(rgl_gpu* g, int seg_pairs) {
+	if (check(g)) return RGL_E_ARG;
+	if (seg_pairs < 64 || seg_pairs > 4096) return g->fail(RGL_E_ARG, "seg_pairs must be in [64, 4096]");
+	g->segPairs = seg_pairs;
+	g->graphValid = false;
 	return 0; }
 
 /* ---- learned shading (no reference counterpart) ---- */

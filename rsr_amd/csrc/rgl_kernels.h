@@ -19,7 +19,8 @@
 
 namespace rgl {
 
-constexpr int kUnitTris = 256;      // triangles per binning unit (one wave)
+constexpr int kUnitTris = 256;      // triangles per binning unit (one wave), the minimum
+constexpr long long kHistBudget = 8LL << 20;   // units x tiles histogram entries before units grow
 constexpr int kColChunk = 32;       // binning units per column-scan chunk
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // LDS key array is kMaxTileDim^2
@@ -37,8 +38,9 @@ struct Counters {
 	uint32_t pad[3];
 };
 
-// compact per-draw view used by the multi-draw kernels
-struct DrawLite {
+// compact per-draw view used by the multi-draw kernels, kept in dynamic
+// LDS (nDraws entries, sized by the launch)
+struct alignas(16) DrawLite {
 	long long triBase;
 	long long vBase;
 	const uint16_t* indices;
@@ -47,17 +49,18 @@ struct DrawLite {
 	int scissor[4];
 	uint32_t slotBase;
 	uint32_t primBase;
+	int tileW, tileH, tilesX, tilesY, tileBase;   // the draw's frame tile grid
 };
 
 __device__ inline int find_draw_ll(const DrawLite* dl, int nDraws, long long s) {
-	int d = 0;
-	for (int k = 1; k < nDraws; ++k) if (dl[k].triBase <= s) d = k;
-	return d; }
+	int lo = 0, hi = nDraws - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (dl[mid].triBase <= s) lo = mid; else hi = mid - 1; }
+	return lo; }
 
 __device__ inline int find_draw_slot(const DrawLite* dl, int nDraws, uint32_t slot) {
-	int d = 0;
-	for (int k = 1; k < nDraws; ++k) if (dl[k].slotBase <= slot) d = k;
-	return d; }
+	int lo = 0, hi = nDraws - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (dl[mid].slotBase <= slot) lo = mid; else hi = mid - 1; }
+	return lo; }
 
 __device__ inline void load_lite(DrawLite* dl, const DrawDesc* draws, int nDraws) {
 	for (int k = threadIdx.x; k < nDraws; k += blockDim.x) {
@@ -68,18 +71,32 @@ __device__ inline void load_lite(DrawLite* dl, const DrawDesc* draws, int nDraws
 		l.keepBacks = d.keepBacks; l.keepFronts = d.keepFronts;
 		for (int i = 0; i < 4; ++i) l.scissor[i] = d.scissor[i];
 		l.slotBase = d.slotBase; l.primBase = d.primBase;
+		l.tileW = d.tileW; l.tileH = d.tileH; l.tilesX = d.tilesX; l.tilesY = d.tilesY; l.tileBase = d.tileBase;
 		dl[k] = l; }}
+
+// LDS table of the frames' first tiles (n + 1 entries, the last = total)
+__device__ inline void load_tile_bases(int* ftb, const BatchDesc& b) {
+	for (int k = threadIdx.x; k <= b.nFrames; k += blockDim.x) ftb[k] = k < b.nFrames ? b.frames[k].tileBase : b.nTiles; }
+
+__device__ inline void load_block_bases(int* fbb, const BatchDesc& b) {
+	for (int k = threadIdx.x; k <= b.nFrames; k += blockDim.x) fbb[k] = k < b.nFrames ? b.frames[k].blockBase : b.nBlocks; }
 
 // ---------------------------------------------------------------------------
 // vertex stage: GPUBinImpl::BinTriangles2P vertex loop (rglv_gpu_impl.hxx:364-385)
 // once per (instance, vertex) instead of once per tile (DrawTriangles, :880-886)
 // ---------------------------------------------------------------------------
+// one launch covers every draw of the batch that uses program P; a thread
+// finds its draw by the draws' first vertex-cache entries
 template <class P>
-__global__ __launch_bounds__(256) void k_vertex(DrawDesc d, float4* __restrict__ vpos,
-                                                 uint8_t* __restrict__ vflags, float* __restrict__ vvary) {
-	const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-	const long long total = static_cast<long long>(d.instances) * d.verts;
-	if (i >= total) return;
+__global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ draws, int nDraws, long long vtot,
+                                                 float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary) {
+	const long long gi = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (gi >= vtot) return;
+	int lo = 0, hi = nDraws - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (draws[mid].vBase <= gi) lo = mid; else hi = mid - 1; }
+	const DrawDesc& d = draws[lo];
+	if (d.program != P::kId) return;
+	const long long i = gi - d.vBase;
 	const int iid = static_cast<int>(i / d.verts);
 	const int v = static_cast<int>(i - static_cast<long long>(iid) * d.verts);
 	float pos[4], vo[P::kVary];
@@ -96,7 +113,7 @@ __global__ __launch_bounds__(256) void k_vertex(DrawDesc d, float4* __restrict__
 	const float iw = oneover(pos[3]);
 	const float dcx = (pos[0] * iw) * d.DS[0] + d.DO[0];
 	const float dcy = (pos[1] * iw) * d.DS[1] + d.DO[1];
-	const long long g = d.vBase + i;
+	const long long g = gi;
 	vpos[g] = make_float4(dcx, dcy, pos[2] * iw, iw);
 	vflags[g] = static_cast<uint8_t>(f);
 	float* out = vvary + g * kMaxVary;
@@ -112,23 +129,25 @@ __device__ inline uint2 pack_rect(int tx0, int ty0, int tx1, int ty1) {
 	                  static_cast<uint32_t>(ty0) | (static_cast<uint32_t>(ty1) << 16)); }
 __device__ inline const uint2 empty_rect() { return make_uint2(0x0000ffffu, 0x0000ffffu); }
 
-__global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__ draws, FrameDesc f,
+__global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__ draws, BatchDesc b,
 		const float4* __restrict__ vpos, const uint8_t* __restrict__ vflags,
 		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
 		uint32_t* __restrict__ clipq, uint32_t* __restrict__ clipqOf, uint32_t clipCap,
 		Counters* __restrict__ ctr) {
-	extern __shared__ uint32_t lhist[];  // nTiles counters
-	__shared__ DrawLite dl[kMaxDraws];
+	// dynamic LDS: the batch's draws, then nTiles counters
+	extern __shared__ uint4 dyn_lds[];
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	uint32_t* lhist = reinterpret_cast<uint32_t*>(dl + b.nDraws);
 	const int u = blockIdx.x;
-	load_lite(dl, draws, f.nDraws);
-	for (int t = threadIdx.x; t < f.nTiles; t += 64) lhist[t] = 0;
+	load_lite(dl, draws, b.nDraws);
+	for (int t = threadIdx.x; t < b.nTiles; t += 64) lhist[t] = 0;
 	__syncthreads();
 	uint32_t binned = 0;
-	const long long s0 = static_cast<long long>(u) * kUnitTris;
-	for (int r = 0; r < kUnitTris / 64; ++r) {
+	const long long s0 = static_cast<long long>(u) * b.unitTris;
+	for (int r = 0; r < b.unitTris / 64; ++r) {
 		const long long s = s0 + r * 64 + threadIdx.x;
-		if (s >= f.totalTris) break;
-		const int di = find_draw_ll(dl, f.nDraws, s);
+		if (s >= b.totalTris) break;
+		const int di = find_draw_ll(dl, b.nDraws, s);
 		const DrawLite& d = dl[di];
 		const long long n = s - d.triBase;
 		const int iid = static_cast<int>(n / d.tris);
@@ -169,8 +188,8 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 				const int32_t vmaxy = imin(wadd(imax(iy0, imax(iy1, iy2)), 1), d.scissor[3] - 1);
 				const bool notCulled = front ? d.keepFronts != 0 : d.keepBacks != 0;
 				if (notCulled && vmaxx > vminx && vmaxy > vminy) {
-					const int tx0 = vminx / f.tileW, ty0 = vminy / f.tileH;
-					const int tx1 = vmaxx / f.tileW, ty1 = vmaxy / f.tileH;
+					const int tx0 = vminx / d.tileW, ty0 = vminy / d.tileH;
+					const int tx1 = vmaxx / d.tileW, ty1 = vmaxy / d.tileH;
 					rect = pack_rect(tx0, ty0, tx1, ty1);
 					// backface reorder (:634-637); the 0x8000 flag is not needed
 					const uint32_t g0 = static_cast<uint32_t>(vb + (front ? ii[0] : ii[2]));
@@ -179,14 +198,14 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 					++binned;
 					for (int ty = ty0; ty <= ty1; ++ty)
 						for (int tx = tx0; tx <= tx1; ++tx)
-							atomicAdd(&lhist[ty * f.tilesX + tx], 1u); }}}
+							atomicAdd(&lhist[d.tileBase + ty * d.tilesX + tx], 1u); }}}
 		trect[s] = rect;
 		tvert[s] = tv; }
 	__syncthreads();
-	uint32_t* cs = csum + static_cast<long long>(u / kColChunk) * f.nTiles;
-	for (int t = threadIdx.x; t < f.nTiles; t += 64) {
+	uint32_t* cs = csum + static_cast<long long>(u / kColChunk) * b.nTiles;
+	for (int t = threadIdx.x; t < b.nTiles; t += 64) {
 		const uint32_t c = lhist[t];
-		hist[static_cast<long long>(u) * f.nTiles + t] = c;
+		hist[static_cast<long long>(u) * b.nTiles + t] = c;
 		if (c) atomicAdd(&cs[t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
@@ -270,25 +289,31 @@ __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, in
 // keeping submission order: units in order (scan offsets), rounds in
 // order, lanes in order (ballot rank), one wave per unit.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_scatter(FrameDesc f, const uint2* __restrict__ trect,
+__global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, const uint2* __restrict__ trect,
 		const uint32_t* __restrict__ off, uint32_t* __restrict__ list, uint32_t listCap, Counters* __restrict__ ctr) {
-	extern __shared__ uint32_t base[];  // nTiles write cursors
+	// dynamic LDS: the batch's draws, then nTiles write cursors
+	extern __shared__ uint4 dyn_lds[];
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	uint32_t* base = reinterpret_cast<uint32_t*>(dl + b.nDraws);
 	const int u = blockIdx.x;
-	for (int t = threadIdx.x; t < f.nTiles; t += 64) base[t] = off[static_cast<long long>(u) * f.nTiles + t];
+	load_lite(dl, draws, b.nDraws);
+	for (int t = threadIdx.x; t < b.nTiles; t += 64) base[t] = off[static_cast<long long>(u) * b.nTiles + t];
 	__syncthreads();
 	const int lane = threadIdx.x;
 	const unsigned long long lt = (1ull << lane) - 1ull;
 	bool ovf = false;
-	const long long s0 = static_cast<long long>(u) * kUnitTris;
-	for (int r = 0; r < kUnitTris / 64; ++r) {
+	const long long s0 = static_cast<long long>(u) * b.unitTris;
+	for (int r = 0; r < b.unitTris / 64; ++r) {
 		const long long s = s0 + r * 64 + lane;
-		int tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0;
-		if (s < f.totalTris) {
+		int tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0, tb = 0, tsx = 1;
+		if (s < b.totalTris) {
 			const uint2 rc = trect[s];
-			tx0 = rc.x & 0xffffu; tx1 = rc.x >> 16; ty0 = rc.y & 0xffffu; ty1 = rc.y >> 16; }
+			tx0 = rc.x & 0xffffu; tx1 = rc.x >> 16; ty0 = rc.y & 0xffffu; ty1 = rc.y >> 16;
+			const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
+			tb = d.tileBase; tsx = d.tilesX; }
 		const bool has = tx0 <= tx1 && ty0 <= ty1;
 		int cx = tx0, cy = ty0;
-		uint32_t cur = has ? static_cast<uint32_t>(cy * f.tilesX + cx) : 0xffffffffu;
+		uint32_t cur = has ? static_cast<uint32_t>(tb + cy * tsx + cx) : 0xffffffffu;
 		for (;;) {
 			uint32_t m = cur;
 			for (int o = 32; o > 0; o >>= 1) { const uint32_t y = __shfl_xor(m, o, 64); m = y < m ? y : m; }
@@ -299,7 +324,7 @@ __global__ __launch_bounds__(64) void k_scatter(FrameDesc f, const uint2* __rest
 				const uint32_t pos = b + static_cast<uint32_t>(__popcll(mask & lt));
 				if (pos < listCap) list[pos] = static_cast<uint32_t>(s); else ovf = true;
 				if (++cx > tx1) { cx = tx0; ++cy; }
-				cur = cy <= ty1 ? static_cast<uint32_t>(cy * f.tilesX + cx) : 0xffffffffu; }
+				cur = cy <= ty1 ? static_cast<uint32_t>(tb + cy * tsx + cx) : 0xffffffffu; }
 			__syncthreads();
 			if (lane == 0) base[m] = b + static_cast<uint32_t>(__popcll(mask));
 			__syncthreads(); }}
@@ -319,14 +344,16 @@ __device__ inline float plane_dist(int plane, const float4& p) {
 	default: return p.w - p.y; }}
 
 template <class P>
-__global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws, int drawIdx, FrameDesc f,
+__global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws, int nDraws,
 		const uint32_t* __restrict__ clipq, uint32_t clipCap, float4* __restrict__ cvpos, float* __restrict__ cvvary,
 		uint8_t* __restrict__ fcount, uint2* __restrict__ frect, uint32_t* __restrict__ clipHist, Counters* __restrict__ ctr) {
-	const DrawDesc& d = draws[drawIdx];
 	const uint32_t nq = min(ctr->clipCount, clipCap);
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
 		const long long s = clipq[q];
-		if (s < d.triBase || s >= d.triBase + static_cast<long long>(d.tris) * d.instances) continue;
+		int lo = 0, hi = nDraws - 1;
+		while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (draws[mid].triBase <= s) lo = mid; else hi = mid - 1; }
+		const DrawDesc& d = draws[lo];
+		if (d.program != P::kId) continue;   // another program's launch clips it
 		const long long n = s - d.triBase;
 		const int iid = static_cast<int>(n / d.tris);
 		const int t = static_cast<int>(n - static_cast<long long>(iid) * d.tris);
@@ -396,15 +423,15 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 			const int vminy = imax(imin(iy0, imin(iy1, iy2)), d.scissor[1]);
 			const int vmaxx = imin(wadd(imax(ix0, imax(ix1, ix2)), 1), d.scissor[2] - 1);
 			const int vmaxy = imin(wadd(imax(iy0, imax(iy1, iy2)), 1), d.scissor[3] - 1);
-			int tx0 = vminx / f.tileW, ty0 = vminy / f.tileH;
-			int tx1 = vmaxx / f.tileW, ty1 = vmaxy / f.tileH;
-			tx1 = imin(tx1, f.tilesX - 1); ty1 = imin(ty1, f.tilesY - 1);
+			int tx0 = vminx / d.tileW, ty0 = vminy / d.tileH;
+			int tx1 = vmaxx / d.tileW, ty1 = vmaxy / d.tileH;
+			tx1 = imin(tx1, d.tilesX - 1); ty1 = imin(ty1, d.tilesY - 1);
 			tx0 = imax(tx0, 0); ty0 = imax(ty0, 0);
 			if (tx0 > tx1 || ty0 > ty1) { frect[static_cast<long long>(q) * kFanMax + j] = empty_rect(); continue; }
 			frect[static_cast<long long>(q) * kFanMax + j] = pack_rect(tx0, ty0, tx1, ty1);
 			for (int ty = ty0; ty <= ty1; ++ty)
 				for (int tx = tx0; tx <= tx1; ++tx)
-					atomicAdd(&clipHist[ty * f.tilesX + tx], 1u); }}}
+					atomicAdd(&clipHist[d.tileBase + ty * d.tilesX + tx], 1u); }}}
 
 // ---------------------------------------------------------------------------
 // per-tile raster + resolve
@@ -570,40 +597,47 @@ __device__ inline uint32_t key_slot(unsigned long long k, bool leq) {
 // quadrants, and the tile's inline and clipped pairs are cut into segments
 // of kSegPairs; a work item is (tile, segment, quadrant).  Also the
 // clipped-list offsets.
-__global__ __launch_bounds__(1024) void k_tile_scan(FrameDesc f, const uint32_t* __restrict__ tileOff, uint32_t listCap,
+__global__ __launch_bounds__(1024) void k_tile_scan(BatchDesc b, const uint32_t* __restrict__ tileOff, uint32_t listCap,
 		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor,
 		uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
 	uint32_t carry = 0, wcarry = 0;
-	for (int base = 0; base < f.nTiles; base += 1024) {
+	for (int base = 0; base < b.nTiles; base += 1024) {
 		const int i = base + threadIdx.x;
 		uint32_t v = 0, items = 0;
-		if (i < f.nTiles) {
+		if (i < b.nTiles) {
 			v = clipHist[i];
-			const uint32_t b = min(tileOff[i], listCap);
-			const uint32_t e = min(tileOff[i + 1], listCap);
-			items = kQuadrants * max(1u, (e - b + v + kSegPairs - 1) / kSegPairs); }
+			const uint32_t lb = min(tileOff[i], listCap);
+			const uint32_t le = min(tileOff[i + 1], listCap);
+			const uint32_t sp = static_cast<uint32_t>(b.segPairs);
+			items = kQuadrants * max(1u, (le - lb + v + sp - 1) / sp); }
 		uint32_t tot, wtot;
 		const uint32_t ex = block_exclusive_scan(v, sh, tot);
 		const uint32_t wex = block_exclusive_scan(items, sh, wtot);
-		if (i < f.nTiles) { clipOff[i] = carry + ex; clipCursor[i] = carry + ex; workOff[i] = wcarry + wex; }
+		if (i < b.nTiles) { clipOff[i] = carry + ex; clipCursor[i] = carry + ex; workOff[i] = wcarry + wex; }
 		carry += tot; wcarry += wtot; }
 	if (threadIdx.x == 0) {
-		clipOff[f.nTiles] = carry; workOff[f.nTiles] = wcarry;
+		clipOff[b.nTiles] = carry; workOff[b.nTiles] = wcarry;
 		ctr->clipPairs = carry; ctr->workItems = wcarry; ctr->workNext = 0; }}
 
-__global__ __launch_bounds__(256) void k_clip_scatter(FrameDesc f, const uint8_t* __restrict__ fcount,
+__global__ __launch_bounds__(256) void k_clip_scatter(BatchDesc b, const DrawDesc* __restrict__ draws,
+		const uint32_t* __restrict__ clipq, const uint8_t* __restrict__ fcount,
 		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
 		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	load_lite(dl, draws, b.nDraws);
+	__syncthreads();
 	const uint32_t nq = min(ctr->clipCount, clipCap);
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
 		const int nf = fcount[q];
+		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, clipq[q])];
 		for (int j = 0; j < nf; ++j) {
 			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
 			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
 			for (int ty = ty0; ty <= ty1; ++ty)
 				for (int tx = tx0; tx <= tx1; ++tx) {
-					const uint32_t pos = atomicAdd(&clipCursor[ty * f.tilesX + tx], 1u);
+					const uint32_t pos = atomicAdd(&clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
 					if (pos < clistCap) clist[pos] = q * kFanMax + j;
 					else atomicOr(&ctr->overflow, 4u); }}}}
 
@@ -635,9 +669,9 @@ __device__ inline void load_visdraws(VisDraw* vd, const DrawDesc* draws, int nDr
 		vd[k].nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances); }}
 
 __device__ inline int find_visdraw(const VisDraw* vd, int nDraws, long long s) {
-	int d = 0;
-	for (int k = 1; k < nDraws; ++k) if (vd[k].triBase <= s) d = k;
-	return d; }
+	int lo = 0, hi = nDraws - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (vd[mid].triBase <= s) lo = mid; else hi = mid - 1; }
+	return lo; }
 
 __device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_t slot) {
 	VisRec v;
@@ -650,43 +684,48 @@ __device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_
 
 constexpr int kMaxTilesLds = 2048;   // tile offset tables cached in LDS up to this many tiles
 
-__global__ __launch_bounds__(256) void k_pair_setup(FrameDesc f, RasterArgs a, VisRec* __restrict__ rec, VisRec* __restrict__ crec) {
+__global__ __launch_bounds__(256) void k_pair_setup(BatchDesc b, RasterArgs a, VisRec* __restrict__ rec, VisRec* __restrict__ crec) {
 	__shared__ uint32_t toff[kMaxTilesLds + 1];
 	__shared__ VisDraw vd[kMaxDraws];
+	__shared__ int ftb[kMaxFrames + 1];
+	load_tile_bases(ftb, b);
 	if (blockIdx.y == 2) {
 		// tiles cut into several segments merge their quadrants with a 64-bit
 		// atomic min, so their keys start the frame at the sentinel; tiles of
 		// one segment are overwritten whole by plain stores
-		for (int tile = blockIdx.x; tile < f.nTiles; tile += gridDim.x) {
+		__syncthreads();
+		for (int tile = blockIdx.x; tile < b.nTiles; tile += gridDim.x) {
 			if (a.workOff[tile + 1] - a.workOff[tile] <= static_cast<uint32_t>(kQuadrants)) continue;
-			const Rect R = tile_rect(f, tile);
+			const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+			const Rect R = tile_rect(f, tile - f.tileBase);
 			const int rw = R.x1 - R.x0, n = rw * (R.y1 - R.y0);
 			for (int i = threadIdx.x; i < n; i += 256) {
 				const int ly = i / rw, lx = i - ly * rw;
-				a.keys[static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}
+				a.keys[f.keyBase + static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}
 		return; }
-	load_visdraws(vd, a.draws, f.nDraws);
+	load_visdraws(vd, a.draws, b.nDraws);
 	const bool clippedPass = blockIdx.y == 1;
-	const uint32_t total = clippedPass ? min(a.clipOff[f.nTiles], a.clistCap)
-	                                   : min(a.tileOff[f.nTiles], a.listCap);
-	const bool lds = f.nTiles <= kMaxTilesLds;
+	const uint32_t total = clippedPass ? min(a.clipOff[b.nTiles], a.clistCap)
+	                                   : min(a.tileOff[b.nTiles], a.listCap);
+	const bool lds = b.nTiles <= kMaxTilesLds;
 	if (lds)
-		for (int t = threadIdx.x; t <= f.nTiles; t += 256)
+		for (int t = threadIdx.x; t <= b.nTiles; t += 256)
 			toff[t] = clippedPass ? a.clipOff[t] : a.tileOff[t];
 	__syncthreads();
 	for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-		int lo = 0, hi = f.nTiles - 1;  // last tile whose list starts at or before p
+		int lo = 0, hi = b.nTiles - 1;  // last tile whose list starts at or before p
 		while (lo < hi) {
 			const int mid = (lo + hi + 1) >> 1;
 			const uint32_t o = lds ? toff[mid] : (clippedPass ? a.clipOff[mid] : a.tileOff[mid]);
 			if (o <= p) lo = mid; else hi = mid - 1; }
-		const Rect R = tile_rect(f, lo);
+		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, lo)];
+		const Rect R = tile_rect(f, lo - f.tileBase);
 		TriVerts tr;
 		TriSetup t;
 		if (!clippedPass) {
 			uint32_t s = a.list[p];
-			if (s >= f.totalTris) { atomicOr(&a.ctr->overflow, 16u); s = 0; }
-			const int di = find_visdraw(vd, f.nDraws, s);
+			if (s >= b.totalTris) { atomicOr(&a.ctr->overflow, 16u); s = 0; }
+			const int di = find_visdraw(vd, b.nDraws, s);
 			inline_verts(a.tvert[s], a.vpos, tr);
 			setup_vector(R, tr.X, tr.Y, t);
 			rec[p] = make_rec(t, tr, vd[di].slotBase + static_cast<uint32_t>(s - vd[di].triBase)); }
@@ -695,7 +734,7 @@ __global__ __launch_bounds__(256) void k_pair_setup(FrameDesc f, RasterArgs a, V
 			if (fan / kFanMax >= a.clipCap) { atomicOr(&a.ctr->overflow, 16u); fan = 0; }
 			const uint32_t q = fan / kFanMax, j = fan - q * kFanMax;
 			const uint32_t s = a.clipq[q];
-			const int di = find_visdraw(vd, f.nDraws, s);
+			const int di = find_visdraw(vd, b.nDraws, s);
 			clipped_verts(fan, a.cvpos, tr);
 			setup_scalar(R, tr.X, tr.Y, t);
 			crec[p] = make_rec(t, tr, vd[di].slotBase + vd[di].nTri + static_cast<uint32_t>(s - vd[di].triBase) * kFanMax + j); }}}
@@ -734,7 +773,7 @@ constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per p
 // visibility work items run one per workgroup; the grid is sized from the
 // previous frame's item count and strides over any remainder, so no
 // work-queue counter is shared between workgroups
-__global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs a, const VisRec* __restrict__ rec,
+__global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs a, const VisRec* __restrict__ rec,
                                                        const VisRec* __restrict__ crec) {
 	__shared__ unsigned long long keys[kSubDim * kSubDim];
 	__shared__ VisSetup ts[kRasterThreads];
@@ -742,14 +781,15 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs 
 	__shared__ uint32_t scan_sh[kRasterThreads / 64 + 1];
 	__shared__ uint32_t wofs[kMaxTilesLds + 1];
 	__shared__ uint8_t owner[kQuadWindow];
-	const bool leq = f.depthFunc == 1;
-	const bool lds = f.nTiles <= kMaxTilesLds;
-	if (lds) for (int t = threadIdx.x; t <= f.nTiles; t += kRasterThreads) wofs[t] = a.workOff[t];
+	__shared__ int ftb[kMaxFrames + 1];
+	load_tile_bases(ftb, b);
+	const bool lds = b.nTiles <= kMaxTilesLds;
+	if (lds) for (int t = threadIdx.x; t <= b.nTiles; t += kRasterThreads) wofs[t] = a.workOff[t];
 	uint32_t frags = 0;
 	const uint32_t nItems = a.ctr->workItems;
 	for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
 		__syncthreads();
-		int lo = 0, hi = f.nTiles - 1;  // last tile with workOff <= item
+		int lo = 0, hi = b.nTiles - 1;  // last tile with workOff <= item
 		while (lo < hi) {
 			const int mid = (lo + hi + 1) >> 1;
 			if ((lds ? wofs[mid] : a.workOff[mid]) <= item) lo = mid; else hi = mid - 1; }
@@ -759,7 +799,9 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs 
 		const uint32_t local = item - w0;
 		const uint32_t nseg = (w1 - w0) / kQuadrants;
 		const uint32_t seg = local / kQuadrants, quad = local - seg * kQuadrants;
-		const Rect R = tile_rect(f, tile);
+		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+		const bool leq = f.depthFunc == 1;
+		const Rect R = tile_rect(f, tile - f.tileBase);
 		Rect S;
 		S.x0 = R.x0 + (quad & 1) * kSubDim; S.y0 = R.y0 + (quad >> 1) * kSubDim;
 		S.x1 = imin(S.x0 + kSubDim, R.x1); S.y1 = imin(S.y0 + kSubDim, R.y1);
@@ -770,7 +812,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs 
 		const uint32_t cb = min(a.clipOff[tile], a.clistCap);
 		const uint32_t ce = min(a.clipOff[tile + 1], a.clistCap);
 		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
-		const uint32_t pbeg = seg * kSegPairs, pend = min(pbeg + kSegPairs, ntot);
+		const uint32_t pbeg = seg * static_cast<uint32_t>(b.segPairs), pend = min(pbeg + static_cast<uint32_t>(b.segPairs), ntot);
 		__syncthreads();
 		for (uint32_t b0 = pbeg; f.drawEnabled && b0 < pend; b0 += kRasterThreads) {
 			const uint32_t p = b0 + threadIdx.x;
@@ -847,7 +889,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(FrameDesc f, RasterArgs 
 		for (int i = threadIdx.x; i < rw * rh; i += kRasterThreads) {
 			const int ly = i / rw, lx = i - ly * rw;
 			const unsigned long long k = keys[ly * kSubDim + lx];
-			unsigned long long* dst = a.keys + static_cast<long long>(S.y0 + ly) * f.W + S.x0 + lx;
+			unsigned long long* dst = a.keys + f.keyBase + static_cast<long long>(S.y0 + ly) * f.W + S.x0 + lx;
 			if (nseg == 1) *dst = k;
 			else if (k != kKeySentinel) atomicMin(dst, k); }}
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
@@ -929,16 +971,15 @@ constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
 // the winner of pixel (x, y), shaded: colour before the post program,
 // depth and primitive id (clear values where nothing was drawn)
 template <class S>
-__device__ inline bool resolve_pixel(const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+__device__ inline bool resolve_pixel(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
                                      const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
-	const long long o = static_cast<long long>(y) * f.W + x;
-	const unsigned long long k = a.keys[o];
+	const unsigned long long k = a.keys[f.keyBase + static_cast<long long>(y) * f.W + x];
 	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
 	dep = f.clearDepth;
 	prim = 0xffffffffu;
 	if (!(k < clear_key(f))) return false;
 	const uint32_t slot = key_slot(k, f.depthFunc == 1);
-	const int di = find_draw_slot(dl, f.nDraws, slot);
+	const int di = find_draw_slot(dl, b.nDraws, slot);
 	const DrawLite& d = dl[di];
 	const uint32_t rel = slot - d.slotBase;
 	const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
@@ -977,22 +1018,40 @@ __device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], co
 	return f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
 	               : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
 
+// a resolve block's frame and pixel: blocks are numbered across the batch,
+// each frame's blocks row-major over kResolveW x kResolveH pixel strips
+struct BlockPixel { int frame, x, y; };
+
+__device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
+	BlockPixel r;
+	r.frame = find_base(fbb, b.nFrames, static_cast<int>(blockIdx.x));
+	const FrameDesc& f = b.frames[r.frame];
+	const int lb = static_cast<int>(blockIdx.x) - f.blockBase;
+	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
+	r.x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
+	r.y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
+	return r; }
+
 template <class S>
-__global__ __launch_bounds__(256) void k_resolve(FrameDesc f, RasterArgs a) {
-	__shared__ DrawLite dl[kMaxDraws];
+__global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	__shared__ uint32_t rcpTab[2048];
 	__shared__ uint32_t srgbTab[104];
-	load_lite(dl, a.draws, f.nDraws);
+	__shared__ int fbb[kMaxFrames + 1];
+	load_lite(dl, a.draws, b.nDraws);
+	load_block_bases(fbb, b);
 	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
 	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
-	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
-	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
+	const BlockPixel bp = block_pixel(b, fbb);
+	const FrameDesc& f = b.frames[bp.frame];
+	const int x = bp.x, y = bp.y;
 	if (x >= f.W || y >= f.H) return;
 	const long long o = static_cast<long long>(y) * f.W + x;
 	float col[3], dep;
 	uint32_t prim;
-	resolve_pixel<S>(f, a, dl, x, y, rcpTab, col, dep, prim);
+	resolve_pixel<S>(b, f, a, dl, x, y, rcpTab, col, dep, prim);
 	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
 	if (f.outDepth) f.outDepth[o] = dep;
 	if (f.outPrim) f.outPrim[o] = prim; }

@@ -261,15 +261,14 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(const float* __r
 // wave).  Depth and primitive id are the same stores as the plain resolve.
 // ---------------------------------------------------------------------------
 template <class P>
-__device__ inline bool resolve_inputs(const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+__device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
                                       const uint32_t* rcpTab, ShadeInput& s, float& dep, uint32_t& prim) {
-	const long long o = static_cast<long long>(y) * f.W + x;
-	const unsigned long long k = a.keys[o];
+	const unsigned long long k = a.keys[f.keyBase + static_cast<long long>(y) * f.W + x];
 	dep = f.clearDepth;
 	prim = 0xffffffffu;
 	if (!(k < clear_key(f))) return false;
 	const uint32_t slot = key_slot(k, f.depthFunc == 1);
-	const int di = find_draw_slot(dl, f.nDraws, slot);
+	const int di = find_draw_slot(dl, b.nDraws, slot);
 	const DrawLite& dd = dl[di];
 	const uint32_t rel = slot - dd.slotBase;
 	const uint32_t nTri = static_cast<uint32_t>(dd.tris) * static_cast<uint32_t>(dd.instances);
@@ -328,20 +327,24 @@ constexpr int kQueueSegs = 8;
 constexpr int kQueueStride = 32;   // counters 128 bytes apart
 
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_gather(FrameDesc f, RasterArgs a, float* __restrict__ queue, uint32_t* __restrict__ count,
+__global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, float* __restrict__ queue, uint32_t* __restrict__ count,
                                                     uint32_t segCap) {
-	__shared__ DrawLite dl[kMaxDraws];
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	__shared__ uint32_t rcpTab[2048];
-	load_lite(dl, a.draws, f.nDraws);
+	__shared__ int fbb[kMaxFrames + 1];
+	load_lite(dl, a.draws, b.nDraws);
+	load_block_bases(fbb, b);
 	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
 	__syncthreads();
-	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
-	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
+	const BlockPixel bp = block_pixel(b, fbb);
+	const FrameDesc& f = b.frames[bp.frame];
+	const int x = bp.x, y = bp.y;
 	const bool on = x < f.W && y < f.H;
 	ShadeInput s;
 	float dep = f.clearDepth;
 	uint32_t prim = 0xffffffffu;
-	const bool hit = on && resolve_inputs<P>(f, a, dl, x, y, rcpTab, s, dep, prim);
+	const bool hit = on && resolve_inputs<P>(b, f, a, dl, x, y, rcpTab, s, dep, prim);
 	const long long o = static_cast<long long>(y) * f.W + x;
 	// one queue reservation per workgroup
 	__shared__ uint32_t wcnt[5];
@@ -349,7 +352,7 @@ __global__ __launch_bounds__(256) void k_mlp_gather(FrameDesc f, RasterArgs a, f
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	if (lane == 0) wcnt[wv] = static_cast<uint32_t>(__popcll(m));
 	__syncthreads();
-	const int seg = static_cast<int>((blockIdx.y * gridDim.x + blockIdx.x) % kQueueSegs);
+	const int seg = static_cast<int>(blockIdx.x % kQueueSegs);
 	if (threadIdx.x == 0) {
 		const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
 		wcnt[4] = tot ? atomicAdd(&count[seg * kQueueStride], tot) : 0u; }
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(256) void k_mlp_gather(FrameDesc f, RasterArgs a, f
 			r[0] = make_float4(s.n[0], s.n[1], s.n[2], s.v[0]);
 			r[1] = make_float4(s.v[1], s.v[2], s.p[0], s.p[1]);
 			r[2] = make_float4(s.p[2], s.albedo[0], s.albedo[1], s.albedo[2]);
-			r[3] = make_float4(s.rough, u2f(static_cast<uint32_t>(o)), 0.0F, 0.0F); }}
+			r[3] = make_float4(s.rough, u2f(static_cast<uint32_t>(o)), u2f(static_cast<uint32_t>(bp.frame)), 0.0F); }}
 	if (!on) return;
 	if (f.outColor && !hit) f.outColor[o] = post_pixel(f, f.clearColor, kSrgb8Tab);
 	if (f.outDepth) f.outDepth[o] = dep;
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(256) void k_mlp_gather(FrameDesc f, RasterArgs a, f
 
 // learned resolve, pass 2: the queued fragments through the network, 512
 // per round; persistent workgroups keep the weights in LDS across rounds
-__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(FrameDesc f, const float* __restrict__ queue,
+__global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const float* __restrict__ queue,
 		const uint32_t* __restrict__ count, uint32_t segCap, const uint32_t* __restrict__ img) {
 	__shared__ __attribute__((aligned(16))) MlpShared sh;
 	mlp_load_image(sh, img);
@@ -402,8 +405,9 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(FrameDesc f, const
 		mlp_inputs(s, sh.norm, x);
 		float y[kMlpOut];
 		mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
-		if (ok && f.outColor)
-			f.outColor[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}
+		if (ok) {
+			uint32_t* out = b.frames[f2u(r3.z)].outColor;
+			if (out) out[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}}
 
 // per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
 // each tile, squared 8-bit error against the network's output
@@ -413,14 +417,19 @@ __device__ inline uint32_t hash32(uint32_t v) {
 	v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16; return v; }
 
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_probe(FrameDesc f, RasterArgs a, float* __restrict__ tileErr) {
-	__shared__ DrawLite dl[kMaxDraws];
-	load_lite(dl, a.draws, f.nDraws);
+__global__ __launch_bounds__(256) void k_mlp_probe(BatchDesc b, RasterArgs a, float* __restrict__ tileErr) {
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	__shared__ int ftb[kMaxFrames + 1];
+	load_lite(dl, a.draws, b.nDraws);
+	load_tile_bases(ftb, b);
 	__syncthreads();
 	const long long id = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-	if (id >= static_cast<long long>(f.nTiles) * kMlpProbes || f.outColor == nullptr) return;
+	if (id >= static_cast<long long>(b.nTiles) * kMlpProbes) return;
 	const int tile = static_cast<int>(id / kMlpProbes), j = static_cast<int>(id % kMlpProbes);
-	const Rect R = tile_rect(f, tile);
+	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+	if (f.outColor == nullptr) return;
+	const Rect R = tile_rect(f, tile - f.tileBase);
 	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
 	const uint32_t h = hash32(static_cast<uint32_t>(id) * 0x9e3779b9u + 17u);
 	const int x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(256) void k_mlp_probe(FrameDesc f, RasterArgs a, fl
 	if (x >= R.x1 || y >= R.y1) return;
 	float col[3], dep;
 	uint32_t prim;
-	if (!resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, kRcpMantissa, col, dep, prim)) return;
+	if (!resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim)) return;
 	const uint32_t want = post_pixel(f, col, kSrgb8Tab);
 	const uint32_t got = f.outColor[static_cast<long long>(y) * f.W + x];
 	float se = 0.0F;
@@ -442,56 +451,67 @@ __global__ __launch_bounds__(256) void k_mlp_probe(FrameDesc f, RasterArgs a, fl
 // with the plain shader (colour only; depth and ids are already stored);
 // one thread per pixel, tiles under the limit exit at once
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_fallback(FrameDesc f, RasterArgs a, const float* __restrict__ tileErr,
+__global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a, const float* __restrict__ tileErr,
                                                       float maxMse, uint32_t* __restrict__ nFallback) {
-	__shared__ DrawLite dl[kMaxDraws];
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	__shared__ uint32_t rcpTab[2048];
 	__shared__ uint32_t srgbTab[104];
-	const int x = blockIdx.x * kResolveW + (threadIdx.x % kResolveW);
-	const int y = blockIdx.y * kResolveH + (threadIdx.x / kResolveW);
-	// the block's rows lie in one tile row and span at most two tile columns
-	const int ty = (blockIdx.y * kResolveH) / f.tileH;
-	const int tx0 = (blockIdx.x * kResolveW) / f.tileW, tx1 = min(blockIdx.x * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
+	__shared__ int fbb[kMaxFrames + 1];
+	load_block_bases(fbb, b);
+	__syncthreads();
+	const BlockPixel bp = block_pixel(b, fbb);
+	const FrameDesc& f = b.frames[bp.frame];
+	const int x = bp.x, y = bp.y;
+	// the block's rows lie in one tile row and span tile columns tx0..tx1
+	const int lb = static_cast<int>(blockIdx.x) - f.blockBase, bx = lb % f.blocksX, by = lb / f.blocksX;
+	const int ty = (by * kResolveH) / f.tileH;
+	const int tx0 = (bx * kResolveW) / f.tileW, tx1 = min(bx * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
 	bool any = false;
 	for (int tx = tx0; tx <= tx1; ++tx) {
-		const int t = ty * f.tilesX + tx;
+		const int t = f.tileBase + ty * f.tilesX + tx;
 		const float c = tileErr[2 * t + 1];
 		any = any || (c > 0.0F && tileErr[2 * t] / c > maxMse); }
 	if (!any || f.outColor == nullptr) return;   // uniform over the block
-	load_lite(dl, a.draws, f.nDraws);
+	load_lite(dl, a.draws, b.nDraws);
 	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
 	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
 	if (x >= f.W || y >= f.H) return;
-	const int tile = (y / f.tileH) * f.tilesX + x / f.tileW;
+	const int tile = f.tileBase + (y / f.tileH) * f.tilesX + x / f.tileW;
 	const float cnt = tileErr[2 * tile + 1];
 	if (!(cnt > 0.0F) || !(tileErr[2 * tile] / cnt > maxMse)) return;
 	if (x % f.tileW == 0 && y % f.tileH == 0) atomicAdd(nFallback, 1u);
 	float col[3], dep;
 	uint32_t prim;
-	if (resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, rcpTab, col, dep, prim))
+	if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, rcpTab, col, dep, prim))
 		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, srgbTab); }
 
 // training samples from a frame: covered pixels kept with probability
 // `keep` (hash of pixel and seed), appended as 13 inputs + the plain
 // shader's sRGB colour / 255
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_samples(FrameDesc f, RasterArgs a, float* __restrict__ out, uint32_t cap,
+__global__ __launch_bounds__(256) void k_mlp_samples(BatchDesc b, RasterArgs a, float* __restrict__ out, uint32_t cap,
                                                      uint32_t* __restrict__ count, float keep, uint32_t seed) {
-	__shared__ DrawLite dl[kMaxDraws];
-	load_lite(dl, a.draws, f.nDraws);
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	__shared__ int fbb[kMaxFrames + 1];
+	load_lite(dl, a.draws, b.nDraws);
+	load_block_bases(fbb, b);
 	__syncthreads();
-	const long long id = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-	if (id >= static_cast<long long>(f.W) * f.H) return;
-	const int x = static_cast<int>(id % f.W), y = static_cast<int>(id / f.W);
+	const BlockPixel bp = block_pixel(b, fbb);
+	const FrameDesc& f = b.frames[bp.frame];
+	const int x = bp.x, y = bp.y;
+	if (x >= f.W || y >= f.H) return;
+	const long long id = f.keyBase + static_cast<long long>(y) * f.W + x;
 	const uint32_t h = hash32(static_cast<uint32_t>(id) ^ hash32(seed));
 	if (static_cast<float>(h >> 8) * (1.0F / 16777216.0F) >= keep) return;
 	ShadeInput s;
 	float dep;
 	uint32_t prim;
-	if (!resolve_inputs<P>(f, a, dl, x, y, kRcpMantissa, s, dep, prim)) return;
+	if (!resolve_inputs<P>(b, f, a, dl, x, y, kRcpMantissa, s, dep, prim)) return;
 	float col[3];
-	resolve_pixel<ShadeOne<P>>(f, a, dl, x, y, kRcpMantissa, col, dep, prim);
+	resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim);
 	const uint32_t px = post_pixel(f, col, kSrgb8Tab);
 	const uint32_t slot = atomicAdd(count, 1u);
 	if (slot >= cap) return;

@@ -77,6 +77,11 @@ def lib():
         L.rgl_gl_store_primitive_id.argtypes = [vp, vp]
         L.rgl_gl_reset.argtypes = [vp]
         L.rgl_texture_make_mipmap.argtypes = [vp, i]
+        L.rgl_gpu_run_batch.argtypes = [vp, ctypes.POINTER(vp), i, vp]
+        L.rgl_gpu_set_graph.argtypes = [vp, i]
+        L.rgl_gpu_graph_captures.argtypes = [vp]
+        L.rgl_gpu_set_raster_params.argtypes = [vp, i]
+        L.rgl_gpu_set_guard_band.argtypes = [vp, i]
         L.rgl_mlp_param_count.argtypes = []
         L.rgl_mlp_trainable_count.argtypes = []
         L.rgl_gpu_set_learned_shading.argtypes = [vp, vp, f]
@@ -101,6 +106,8 @@ EXPORTS = [
     "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",
     "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
     "rgl_gl_reset", "rgl_texture_make_mipmap",
+    "rgl_gpu_run_batch", "rgl_gpu_set_graph", "rgl_gpu_graph_captures", "rgl_gpu_set_raster_params",
+    "rgl_gpu_set_guard_band",
     "rgl_mlp_param_count", "rgl_mlp_trainable_count", "rgl_gpu_set_learned_shading",
     "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
 ]
@@ -175,6 +182,33 @@ class GPU:
         names = ctypes.create_string_buffer(512)
         n = self._check(self.L.rgl_gpu_kernel_times(self.h, ms, 16, names, 512))
         return dict(zip(names.value.decode().split(","), [ms[i] for i in range(n)]))
+
+    # --- batches (no reference counterpart) -----------------------------------
+    def run_batch(self, frames, stream=None):
+        """Draws the frames recorded in `frames` (GPU recorders, may include
+        self) with one launch sequence on this GPU's buffers."""
+        arr = (ctypes.c_void_p * len(frames))(*[f.h for f in frames])
+        self._check(self.L.rgl_gpu_run_batch(self.h, arr, len(frames), stream))
+
+    def run_batch_and_wait(self, frames, stream=None, retries: int = 3):
+        for _ in range(retries + 1):
+            self.run_batch(frames, stream)
+            rc = self.L.rgl_gpu_wait(self.h)
+            if rc != RGL_E_CAPACITY:
+                return self._check(rc)
+        raise RglError("capacity overflow persisted")
+
+    def set_graph(self, enable: bool):
+        self._check(self.L.rgl_gpu_set_graph(self.h, int(enable)))
+
+    def graph_captures(self) -> int:
+        return self._check(self.L.rgl_gpu_graph_captures(self.h))
+
+    def set_guard_band(self, fixed: bool):
+        self._check(self.L.rgl_gpu_set_guard_band(self.h, int(fixed)))
+
+    def set_raster_params(self, seg_pairs: int):
+        self._check(self.L.rgl_gpu_set_raster_params(self.h, seg_pairs))
 
     # --- learned shading (no reference counterpart) ---------------------------
     def set_learned_shading(self, dev_params, max_rms_err: float = 2.0 / 255.0):

@@ -223,6 +223,25 @@ def icosphere(subdiv: int):
     return p, p.copy(), np.stack([u, v], 1), np.array(faces, np.uint16).reshape(-1)
 
 
+def cube():
+    """Unit cube (half-extent 1) with per-face normals and uvs: 24 vertices,
+    12 triangles, counter-clockwise seen from outside."""
+    pos, nrm, uv, idx = [], [], [], []
+    for axis in range(3):
+        for sign in (1.0, -1.0):
+            n = np.zeros(3); n[axis] = sign
+            u_ax = np.zeros(3); u_ax[(axis + 1) % 3] = 1.0
+            v_ax = np.cross(n, u_ax)
+            base = len(pos)
+            for (a, b) in ((-1, -1), (1, -1), (1, 1), (-1, 1)):
+                pos.append(n + a * u_ax + b * v_ax)
+                nrm.append(n)
+                uv.append(((a + 1) / 2, (b + 1) / 2))
+            idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+    return (np.array(pos, np.float32), np.array(nrm, np.float32), np.array(uv, np.float32),
+            np.array(idx, np.uint16))
+
+
 def random_rotation(rng: np.random.RandomState) -> np.ndarray:
     z = rng.uniform(-1, 1)
     a = rng.uniform(0, 2 * math.pi)
@@ -262,6 +281,7 @@ class Scene:
     cull_face: int = GL_BACK
     depth_func: int = GL_LESS
     post_program: int = PGM_DEFAULT_POST
+    fixed_guard_band: bool = False   # guard band factor raised to >= 1 (large targets)
 
     @property
     def triangles(self) -> int:
@@ -385,6 +405,66 @@ def scene_icospheres(width: int = 1920, height: int = 1080, instances: int = 256
         meshes=[Mesh(b, idx, instances)])
 
 
+def _mesh_buffers(pos, nrm, uv):
+    b = {}
+    for k in range(3):
+        b[k] = pad4(pos[:, k])
+        b[3 + k] = pad4(nrm[:, k])
+    b[9], b[10] = pad4(uv[:, 0]), pad4(uv[:, 1])
+    return b
+
+
+_SHARED_TEXTURES = {}
+
+
+def shared_texture(dim: int, seed: int) -> np.ndarray:
+    """One mipmapped value-noise texture per (dim, seed), shared by scenes."""
+    key = (dim, seed)
+    if key not in _SHARED_TEXTURES:
+        _SHARED_TEXTURES[key] = make_mipmap_stack(value_noise_rgba8(dim, seed))
+    return _SHARED_TEXTURES[key]
+
+
+def scene_batch(n: int = 64, seed: int = 4, wmin: int = 128, wmax: int = 1024, hmin: int = 128, hmax: int = 768,
+                tmin: int = 500, tmax: int = 50000, tex_dim: int = 256) -> list:
+    """Batched workload: n independent scenes.  Framebuffers log-uniform from
+    wmin x hmin to wmax x hmax (multiples of 8), triangle counts log-uniform
+    in [tmin, tmax] made of seeded icosphere (1280 triangles) and cube (12
+    triangles) instances in front of a 60 degree camera; Blinn-Phong with one
+    point light; one shared mipmapped texture."""
+    rng = np.random.RandomState(seed)
+    ipos, inrm, iuv, iidx = icosphere(3)
+    cpos, cnrm, cuv, cidx = cube()
+    tex = shared_texture(tex_dim, seed)
+    out = []
+    for k in range(n):
+        w = int(round(math.exp(rng.uniform(math.log(wmin), math.log(wmax))) / 8)) * 8
+        h = int(round(math.exp(rng.uniform(math.log(hmin), math.log(hmax))) / 8)) * 8
+        tris = int(math.exp(rng.uniform(math.log(tmin), math.log(tmax))))
+        n_ico = int(tris * rng.uniform(0.3, 0.9)) // 1280
+        n_cube = max(1, (tris - n_ico * 1280) // 12)
+        meshes = []
+        sseed = int(rng.randint(1, 1 << 30))
+        depth = float(rng.uniform(8.0, 30.0))
+        extent = depth * 0.9
+        if n_ico > 0:
+            b = _mesh_buffers(ipos, inrm, iuv)
+            b[15] = instance_matrices(n_ico, sseed, (0.0, 0.0, -depth), extent, 0.2, 1.5).reshape(-1)
+            meshes.append(Mesh(b, iidx, n_ico))
+        b = _mesh_buffers(cpos, cnrm, cuv)
+        b[15] = instance_matrices(n_cube, sseed + 1, (0.0, 0.0, -depth), extent, 0.1, 0.8).reshape(-1)
+        meshes.append(Mesh(b, cidx, n_cube))
+        uni = np.zeros(32, np.float32)
+        uni[4:7] = (10.0, 15.0, 0.0)
+        uni[8], uni[9], uni[10] = 0.15, 0.85, 0.6
+        out.append(Scene(
+            name=f"batch{seed}_{k}_{w}x{h}", width=w, height=h,
+            view=look_at((0, 0, 0), (0, 0, -1), (0, 1, 0)),
+            projection=perspective2(60.0, w / h, 0.1, 100.0),
+            program=PGM_MANY_BLINN_PHONG, uniforms=uni, texture=tex, texture_dim=tex_dim, meshes=meshes))
+    return out
+
+
 ENV_DIM, ENV_TAPS, GGX_LIGHTS = 16, 8, 16
 
 
@@ -465,6 +545,7 @@ def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers
     install, gl_*); *buffers* maps id(array) -> the pointer the backend
     expects for that array (device pointer for the HIP pipeline)."""
     ptr = (lambda a: buffers[id(a)]) if buffers is not None else (lambda a: a)
+    gpu.set_guard_band(scene.fixed_guard_band)
     gpu.reset(scene.width, scene.height, *scene.tile_blocks)
     gpu.install(scene.program, 0, 0)
     gpu.install(scene.program, fragment_state_key(scene), 1)

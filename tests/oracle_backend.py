@@ -35,6 +35,7 @@ def lib():
         L.rglo_gpu_create.argtypes = [i]
         L.rglo_gpu_destroy.argtypes = [vp]
         L.rglo_gpu_set_threads.argtypes = [vp, i]
+        L.rglo_gpu_set_guard_band.argtypes = [vp, i]
         L.rglo_gpu_last_error.restype = ctypes.c_char_p
         L.rglo_gpu_last_error.argtypes = [vp]
         L.rglo_gpu_install.argtypes = [vp, i, u32, i]
@@ -106,6 +107,9 @@ class OracleGPU:
 
     def set_threads(self, n: int):
         self.L.rglo_gpu_set_threads(self.h, n)
+
+    def set_guard_band(self, fixed: bool):
+        self.L.rglo_gpu_set_guard_band(self.h, int(fixed))
 
     def check(self, rc: int):
         if rc != 0:

@@ -1,0 +1,106 @@
+"""GPU tests of batches (many frames, one launch sequence) and of graph
+replay: every frame of a batch must equal the frame drawn alone and the
+oracle's frame bit for bit; graph replays must equal eager runs."""
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+from rsr_amd import rgl, scenes
+from test_parity_gpu import _compare
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _recorders(scene_list):
+    out = []
+    for sc in scene_list:
+        g = rgl.GPU(0)
+        ds = rgl.DeviceScene(sc, DEV, prim=True)
+        ds.record(g)
+        out.append((g, ds))
+    return out
+
+
+def _same(name, a, b):
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), f"{name}: buffer {k} differs"
+
+
+def test_batch_matches_single_and_oracle():
+    from oracle_backend import render as oracle_render
+    mix = [
+        scenes.scene_sphere_ground(),
+        scenes.scene_icospheres(640, 360, instances=64, seed=11, tex_dim=256),
+        scenes.scene_clip_floor(320, 200, 16),
+        dataclasses.replace(scenes.scene_icospheres(488, 272, instances=40, seed=5, tex_dim=256),
+                            tile_blocks=(4, 4), name="icospheres40_488x272_tile32"),
+        dataclasses.replace(scenes.scene_sphere_ground(320, 176), cull=False, depth_func=scenes.GL_LEQUAL,
+                            name="sphere_ground_leq_320x176"),
+        scenes.scene_batch(3, seed=21)[2],
+    ]
+    stream = torch.cuda.current_stream().cuda_stream
+    recs = _recorders(mix)
+    ex = rgl.GPU(0)
+    ex.run_batch_and_wait([g for g, _ in recs], stream)
+    bst = ex.stats()
+    batched = [ds.outputs() for _, ds in recs]
+    frags = 0
+    for (g, ds), sc, out in zip(recs, mix, batched):
+        g.run_and_wait(stream)
+        alone = ds.outputs()
+        st = g.stats()
+        frags += st["fragments"]
+        _same(sc.name, out, alone)
+        _compare(sc.name, oracle_render(sc, threads=8), (*out, st))
+    assert bst["fragments"] == frags
+    assert bst["tiles"] == sum(g.stats()["tiles"] for g, _ in recs)
+    ex.close()
+    for g, _ in recs:
+        g.close()
+
+
+def test_batched_config3_64_scenes():
+    from oracle_backend import render as oracle_render
+    batch = scenes.scene_batch()
+    stream = torch.cuda.current_stream().cuda_stream
+    recs = _recorders(batch)
+    ex = rgl.GPU(0)
+    ex.set_graph(True)
+    for _ in range(3):           # capture, then replays
+        ex.run_batch_and_wait([g for g, _ in recs], stream)
+    st = ex.stats()
+    print(f"64-scene batch: {st}", flush=True)
+    assert st["primitives_submitted"] == sum(s.triangles for s in batch)
+    batched = [ds.outputs() for _, ds in recs]
+    for k, ((g, ds), sc, out) in enumerate(zip(recs, batch, batched)):
+        g.run_and_wait(stream)
+        _same(sc.name, out, ds.outputs())
+        if k % 8 == 0:
+            _compare(sc.name, oracle_render(sc, threads=8), (*out, g.stats()))
+    ex.close()
+    for g, _ in recs:
+        g.close()
+
+
+def test_graph_replay_matches_eager():
+    sc = scenes.scene_icospheres(960, 540, instances=96, seed=5, tex_dim=256)
+    other = scenes.scene_clip_floor(320, 200, 16)
+    stream = torch.cuda.current_stream().cuda_stream
+    (g, ds), (g2, ds2) = _recorders([sc, other])
+    g.run_and_wait(stream)
+    ref = ds.outputs()
+    g.set_graph(True)
+    for k in range(5):
+        ds.color.zero_(); ds.depth.zero_(); ds.prim.zero_()
+        torch.cuda.synchronize()
+        g.run_and_wait(stream)
+        _same(f"replay {k}", ds.outputs(), ref)
+        g2.run_and_wait(stream)          # eager work of the program between replays
+        torch.cuda.synchronize()
+    caps = g.graph_captures()
+    print(f"graph captures: {caps}")
+    assert 1 <= caps <= 2
+    g.close(); g2.close()
