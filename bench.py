@@ -160,6 +160,88 @@ def bench_batch(steps: int):
     return out
 
 
+def bench_stress(steps: int):
+    """The stress workload: 3840x2160, 2M triangles (6250 instanced
+    subdivision-2 icospheres, projected areas ~0.25-16 px^2, clustered to a
+    depth complexity of several layers) under the heavy GGX shader, four
+    camera views batched into one launch sequence (graph replay); plain and
+    learned shading."""
+    import torch
+    from rsr_amd import learned, rgl, scenes
+    dev = "cuda:0"
+    views = [scenes.scene_stress(v) for v in range(4)]
+    frames = []
+    for sc in views:
+        g = rgl.GPU(0)
+        ds = rgl.DeviceScene(sc, dev, prim=True)
+        ds.record(g)
+        frames.append((g, ds))
+    recs = [g for g, _ in frames]
+    ex = rgl.GPU(0)
+    ex.set_graph(True)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def run(n):
+        for _ in range(3):
+            ex.run_batch_and_wait(recs, stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            ex.run_batch(recs, stream)
+        ex.wait()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / n
+        ex.set_graph(False)
+        ex.set_timing(True)
+        for _ in range(max(3, n // 2)):
+            ex.run_batch(recs, stream)
+        ex.wait()
+        kt = ex.kernel_times()
+        ex.set_timing(False)
+        ex.set_graph(True)
+        return ms, kt
+
+    plain_ms, plain_kt = run(steps)
+    st = ex.stats()
+    plain = [ds.outputs() for _, ds in frames]
+    out = {"workload": "stress6250x4views_3840x2160", "triangles_per_view": views[0].triangles, "views": 4,
+           "batch_triangles": 4 * views[0].triangles, "fragments": st["fragments"],
+           "tile_pairs": st["tile_pairs"], "guard_band": "fixed (>= 1)",
+           "plain_heavy_ms": round(plain_ms, 4),
+           "mtris_per_s": round(4 * views[0].triangles / (plain_ms * 1e-3) / 1e6, 2),
+           "gfrag_per_s": round(st["fragments"] / (plain_ms * 1e-3) / 1e9, 4),
+           "kernel_ms_plain": {k: round(v, 5) for k, v in plain_kt.items()}}
+    v_xf = sum(sc.transformed_vertices for sc in views)
+    b_bin = 12 * v_xf + 6 * 4 * views[0].triangles + 64 * 4 * views[0].instances + 4 * st["tile_pairs"] + 8 * st["tiles"]
+    bin_ms = sum(plain_kt.get(k, 0.0) for k in ("k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip"))
+    out["binning"] = {"bytes": b_bin, "ms": round(bin_ms, 5),
+                      "achieved_gbs": round(b_bin / (bin_ms * 1e-3) / 1e9, 2) if bin_ms else None,
+                      "peak_gbs": HBM_PEAK_GBS}
+    wpath = os.path.join(ROOT, "rsr_amd", "ggx_mlp_weights.npy")
+    if os.path.exists(wpath):
+        params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(dev)
+        ex.set_learned_shading(params.data_ptr(), 2.0 / 255.0)
+        mlp_ms, mlp_kt = run(steps)
+        lst = ex.stats()
+        lrn = [ds.outputs() for _, ds in frames]
+        ex.set_learned_shading(None, 0.0)
+        covs = [p[1] < 1.0 for p in plain]
+        q = [learned.psnr(p[0], l[0], c) for p, l, c in zip(plain, lrn, covs)]
+        net_ms = mlp_kt.get("k_mlp_shade", 0.0)
+        frags = int(sum(c.sum() for c in covs))
+        tfl = frags * MLP_FLOP_PER_FRAGMENT / (net_ms * 1e-3) / 1e12 if net_ms else 0.0
+        out["learned"] = {"learned_ms": round(mlp_ms, 4), "speedup": round(plain_ms / mlp_ms, 3),
+                          "psnr_db_covered_per_view": [round(x, 2) for x in q],
+                          "fallback_tiles": lst["fallback_tiles"], "tiles": lst["tiles"],
+                          "mfma": {"kernel": "k_mlp_shade", "achieved_tflops": round(tfl, 2),
+                                   "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "frac": round(tfl / MFMA_BF16_PEAK_TFLOPS, 5)},
+                          "kernel_ms": {k: round(v, 5) for k, v in mlp_kt.items()}}
+    ex.close()
+    for g, _ in frames:
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +250,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-shading (heavy shader) measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
+    ap.add_argument("--no-stress", action="store_true", help="skip the 4K stress measurement")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -276,6 +359,8 @@ def main():
         out["learned"] = bench_learned(max(20, args.steps // 4), max(5, args.warmup // 4))
     if not args.no_batch:
         out["batched"] = bench_batch(max(10, args.steps // 10))
+    if not args.no_stress:
+        out["stress"] = bench_stress(max(5, args.steps // 20))
     if not args.no_cpu:
         cb = cpu_baseline(scene)
         out["cpu_baseline"] = cb

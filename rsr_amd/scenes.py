@@ -497,6 +497,29 @@ def env_cube(dim: int, seed: int) -> np.ndarray:
     return out.reshape(-1)
 
 
+def heavy_tables(light_seed: int = 5, env_seed: int = 6):
+    """The heavy shader's tables: environment cube (buffer 11), taps (12),
+    lights (13), and its uniforms with the eye at the origin."""
+    rng = np.random.RandomState(light_seed)
+    lights = np.zeros((GGX_LIGHTS, 8), np.float32)
+    # on a shell around the instance volume: directions uniform, 35-55 units
+    # from its centre
+    d = rng.normal(size=(GGX_LIGHTS, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    lights[:, 0:3] = np.array([0.0, 0.0, -25.0]) + d * rng.uniform(35.0, 55.0, (GGX_LIGHTS, 1))
+    lights[:, 4:7] = rng.uniform(0.3, 1.0, (GGX_LIGHTS, 3)) * 400.0
+    rng = np.random.RandomState(env_seed + 1000)
+    taps = np.zeros((ENV_TAPS, 4), np.float32)
+    taps[:, 0:3] = rng.normal(0.0, 0.6, (ENV_TAPS, 3))
+    taps[:, 3] = 1.0 / ENV_TAPS
+    uni = np.zeros(32, np.float32)
+    uni[0:3] = (0.0, 0.0, 0.0)   # eye
+    uni[3] = 0.03                # ambient
+    uni[4] = 0.8                 # environment specular
+    uni[5] = 0.25                # environment diffuse
+    return env_cube(ENV_DIM, env_seed), taps.reshape(-1), lights.reshape(-1), uni
+
+
 def scene_heavy(width: int = 1920, height: int = 1080, instances: int = 256, seed: int = 2,
                 light_seed: int = 5, env_seed: int = 6, rough_seed: int = 7, tex_seed: int = 3,
                 tex_dim: int = 1024) -> Scene:
@@ -506,29 +529,56 @@ def scene_heavy(width: int = 1920, height: int = 1080, instances: int = 256, see
     sc = scene_icospheres(width, height, instances, seed, tex_seed=tex_seed, tex_dim=tex_dim, program=PGM_GGX_ENV)
     b = sc.meshes[0].buffers
     b[14] = np.random.RandomState(rough_seed).uniform(0.05, 1.0, instances).astype(np.float32)
-    rng = np.random.RandomState(light_seed)
-    lights = np.zeros((GGX_LIGHTS, 8), np.float32)
-    # on a shell around the instance volume: directions uniform, 35-55 units
-    # from its centre
-    d = rng.normal(size=(GGX_LIGHTS, 3))
-    d /= np.linalg.norm(d, axis=1, keepdims=True)
-    lights[:, 0:3] = np.array([0.0, 0.0, -25.0]) + d * rng.uniform(35.0, 55.0, (GGX_LIGHTS, 1))
-    lights[:, 4:7] = rng.uniform(0.3, 1.0, (GGX_LIGHTS, 3)) * 400.0
-    b[13] = lights.reshape(-1)
-    rng = np.random.RandomState(env_seed + 1000)
-    taps = np.zeros((ENV_TAPS, 4), np.float32)
-    taps[:, 0:3] = rng.normal(0.0, 0.6, (ENV_TAPS, 3))
-    taps[:, 3] = 1.0 / ENV_TAPS
-    b[12] = taps.reshape(-1)
-    b[11] = env_cube(ENV_DIM, env_seed)
-    uni = np.zeros(32, np.float32)
-    uni[0:3] = (0.0, 0.0, 0.0)   # eye
-    uni[3] = 0.03                # ambient
-    uni[4] = 0.8                 # environment specular
-    uni[5] = 0.25                # environment diffuse
-    sc.uniforms = uni
+    b[11], b[12], b[13], sc.uniforms = heavy_tables(light_seed, env_seed)
     sc.name = f"heavy_icospheres{instances}_{width}x{height}"
     return sc
+
+
+STRESS_CENTER = (0.0, 0.0, -30.0)
+
+
+def scene_stress(view: int = 0, width: int = 3840, height: int = 2160, instances: int = 6250, seed: int = 8,
+                 cluster_radius: float = 6.0, smin: float = 0.06, smax: float = 0.46, tex_dim: int = 1024,
+                 fixed_guard_band: bool = True) -> Scene:
+    """Stress workload, one of four camera views: `instances` subdivision-2
+    icospheres (320 triangles each; 6250 -> 2,000,000 triangles) with scale
+    log-uniform in [smin, smax] (projected triangle areas about 0.25-16 px^2
+    at 3840x2160 from 30 units) and positions uniform in a ball of
+    `cluster_radius` around the centre, so that the covered region has a
+    depth complexity of several layers; the heavy GGX shader of the learned
+    shading workload.  The four views look at the centre from azimuths
+    0/90/180/270 degrees (+15 degrees) at 30 units."""
+    pos, nrm, uv, idx = icosphere(2)
+    rng = np.random.RandomState(seed)
+    # positions uniform in the ball, log-uniform scales, uniform random rotations
+    d = rng.normal(size=(instances, 3))
+    d *= (cluster_radius * rng.uniform(0, 1, (instances, 1)) ** (1.0 / 3.0)) / np.linalg.norm(d, axis=1, keepdims=True)
+    scl = np.exp(rng.uniform(math.log(smin), math.log(smax), instances))
+    q = rng.normal(size=(instances, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    R = np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w),
+                  2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w),
+                  2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], 1).reshape(instances, 3, 3)
+    M = np.zeros((instances, 4, 4))
+    M[:, :3, :3] = R * scl[:, None, None]
+    M[:, :3, 3] = np.asarray(STRESS_CENTER) + d
+    M[:, 3, 3] = 1.0
+    mats = np.ascontiguousarray(M.transpose(0, 2, 1)).reshape(instances, 16).astype(np.float32)
+    b = _mesh_buffers(pos, nrm, uv)
+    b[15] = mats.reshape(-1)
+    b[11], b[12], b[13], uni = heavy_tables()
+    b[14] = np.random.RandomState(seed + 1).uniform(0.05, 1.0, instances).astype(np.float32)
+    ang = math.radians(90.0 * view + 15.0)
+    c = np.asarray(STRESS_CENTER)
+    eye = c + 30.0 * np.array([math.sin(ang), 0.25, math.cos(ang)])
+    uni[0:3] = eye
+    return Scene(
+        name=f"stress{instances}_view{view}_{width}x{height}", width=width, height=height,
+        view=look_at(tuple(eye), tuple(c), (0, 1, 0)),
+        projection=perspective2(60.0, width / height, 0.1, 200.0),
+        program=PGM_GGX_ENV, uniforms=uni, texture=shared_texture(tex_dim, 3), texture_dim=tex_dim,
+        meshes=[Mesh(b, idx, instances)], fixed_guard_band=fixed_guard_band)
 
 
 def fragment_state_key(scene: Scene) -> int:

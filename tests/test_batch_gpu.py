@@ -104,3 +104,56 @@ def test_graph_replay_matches_eager():
     print(f"graph captures: {caps}")
     assert 1 <= caps <= 2
     g.close(); g2.close()
+
+
+def test_stress_4k_views_batched():
+    # the 4K stress geometry at 800 instances (256k triangles per view), four
+    # views in one batch, each against the oracle; the fixed guard band
+    from oracle_backend import render as oracle_render
+    views = [scenes.scene_stress(v, instances=800) for v in range(4)]
+    stream = torch.cuda.current_stream().cuda_stream
+    recs = _recorders(views)
+    ex = rgl.GPU(0)
+    ex.run_batch_and_wait([g for g, _ in recs], stream)
+    for (g, ds), sc in zip(recs, views):
+        out = ds.outputs()
+        g.run_and_wait(stream)
+        _same(sc.name, out, ds.outputs())
+        _compare(sc.name, oracle_render(sc, threads=8), (*out, g.stats()))
+    ex.close()
+    for g, _ in recs:
+        g.close()
+
+
+def test_stress_4k_reference_guard_band():
+    # the reference's guard band above 2048 pixels (factor 1/15 at 3840):
+    # on-screen triangles beyond it are culled or clipped, as in the oracle
+    from oracle_backend import render as oracle_render
+    sc = scenes.scene_stress(2, instances=800, fixed_guard_band=False)
+    ref = oracle_render(sc, threads=8)
+    got = rgl.render(sc)
+    _compare(sc.name, ref, got)
+    assert got[3]["primitives_clipped"] > 0
+    assert (got[1] < 1.0).mean() < 0.5 * (oracle_render(scenes.scene_stress(2, instances=800), threads=8)[1] < 1.0).mean()
+
+
+def test_stress_2m_batch_equals_single():
+    # full size (2M triangles per view): the batch of four views equals each
+    # view drawn alone
+    views = [scenes.scene_stress(v) for v in range(4)]
+    stream = torch.cuda.current_stream().cuda_stream
+    recs = _recorders(views)
+    ex = rgl.GPU(0)
+    ex.run_batch_and_wait([g for g, _ in recs], stream)
+    bst = ex.stats()
+    outs = [ds.outputs() for _, ds in recs]
+    frags = 0
+    for (g, ds), sc, out in zip(recs, views, outs):
+        g.run_and_wait(stream)
+        _same(sc.name, out, ds.outputs())
+        frags += g.stats()["fragments"]
+    print(f"stress batch: {bst}", flush=True)
+    assert bst["fragments"] == frags and bst["primitives_submitted"] == 8_000_000
+    ex.close()
+    for g, _ in recs:
+        g.close()
