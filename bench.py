@@ -16,6 +16,7 @@ roofline of the dominant kernel and the CPU oracle timed on the host.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -251,6 +252,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-shading (heavy shader) measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batch measurement")
     ap.add_argument("--no-stress", action="store_true", help="skip the 4K stress measurement")
+    ap.add_argument("--graph", action="store_true", help="time graph replay instead of eager launches")
+    ap.add_argument("--no-tune", action="store_true", help="default tile size and pairs per work item, no autotuner")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE readings)")
@@ -271,16 +274,47 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     scene = scenes.scene_icospheres()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    tune = None
+    if not args.no_tune:
+        # the autotuner picks the tile size and pairs per visibility work item
+        # on this workload (every choice draws identical frames)
+        from rsr_amd.autotune import autotune
+        tuner = rgl.GPU(0)
+        tuner.set_graph(True)
+        tframes, choice, table = autotune(tuner, [scene], dev, steps=20)
+        tuner.close()
+        for g, _ in tframes:
+            g.close()
+        tune = {"choice": choice, "table": table}
+        scene = dataclasses.replace(scene, tile_blocks=(choice["tile_px"] // 8, choice["tile_px"] // 8))
     gpu = rgl.GPU(0)
+    if tune:
+        gpu.set_raster_params(tune["choice"]["seg_pairs"])
     ds = rgl.DeviceScene(scene, dev, prim=True)
     ds.record(gpu)
-    stream = torch.cuda.current_stream(dev).cuda_stream
 
     for _ in range(max(args.warmup, 1)):
         gpu.run_and_wait(stream)
     st = gpu.stats()
 
-    # the timed region: K frames back to back, no per-kernel events
+    # replay from the captured launch graph (rgl_gpu_set_graph), for
+    # reference: K frames, each one hipGraphLaunch
+    gpu.set_graph(True)
+    for _ in range(3):
+        gpu.run_and_wait(stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gpu.run(stream)
+    gpu.wait()
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    gpu.set_graph(args.graph)
+    # the timed region: K frames back to back (eager launches unless
+    # --graph), no per-kernel events
+    for _ in range(3):
+        gpu.run_and_wait(stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -288,6 +322,7 @@ def main():
     gpu.wait()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    gpu.set_graph(False)
     if gpu.stats()["overflow"]:
         raise SystemExit("capacity overflow inside the timed region")
     # per-kernel durations: a second pass with HIP events recorded on the
@@ -345,9 +380,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": scene.name, "resolution": f"{W}x{H}", "triangles": tris,
                    "instances": 256, "shader": "Blinn-Phong, 1 point light, 1024^2 mip texture",
-                   "tile_px": 64, "path": "plain"},
+                   "tile_px": scene.tile_blocks[0] * 8, "path": "plain"},
         "gfrag_per_s": round(gfrag, 4),
         "frame_ms": round(ms, 4),
+        "launch": "graph replay" if args.graph else "eager",
+        "autotune": tune,
+        "frame_ms_graph_replay": round(graph_ms, 4),
         "fragments": st["fragments"],
         "tile_pairs": st["tile_pairs"],
         "kernel_ms": {k: round(v, 5) for k, v in kt.items()},
