@@ -97,7 +97,7 @@ struct DevArray {
 		return e; }
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
-const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip", "k_pair_setup", "k_vis", "k_resolve",
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_clip", "k_scan", "k_scatter", "k_pair_setup", "k_vis", "k_resolve",
                                "k_mlp_shade", "k_mlp_check" };
 constexpr int kNumTimed = 10;
 
@@ -609,23 +609,24 @@ struct rgl_gpu {
 			k_setup_count<<<b.units, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
 				hist.p, bsum.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
-		if (b.units > 0) {
-			const int nChunks = (b.units + kColChunk - 1) / kColChunk;
-			const dim3 cg((b.nTiles + 63) / 64, nChunks);
-			k_col_scan<<<1, 1024, 0, stream>>>(bsum.p, b.nTiles, nChunks, tileOff.p, &dCtr->pairs);
-			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
-		else {
-			k_fill_u32<<<1, 64, 0, stream>>>(tileOff.p, b.nTiles + 1, 0u); }
-		Mark(3);
-		if (b.units > 0) k_scatter<<<b.units, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, trect.p, hist.p, list.p, listCap, dCtr);
-		Mark(4);
+		// clipping: the clip queue is complete after setup; the clipped
+		// per-tile counts feed the scan below
 		if (present[0]) k_clip<ProgAmy><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[1]) k_clip<ProgMany><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[2]) k_clip<ProgLambertTex><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
-		k_tile_scan<<<1, 1024, 0, stream>>>(b, tileOff.p, listCap, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
-		k_clip_scatter<<<64, 256, dlBytes, stream>>>(b, dDraws.p, clipq.p, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, dCtr);
+		Mark(3);
+		const int nChunks = (b.units + kColChunk - 1) / kColChunk;
+		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
+		if (b.units > 0) {
+			const dim3 cg((b.nTiles + 63) / 64, nChunks);
+			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
+		Mark(4);
+		if (b.units > 0) {
+			const ClipScatterArgs csa{ clipq.p, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, kClipScatterBlocks };
+			k_scatter<<<b.units + kClipScatterBlocks, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(
+				b, dDraws.p, trect.p, hist.p, list.p, listCap, dCtr, csa); }
 		Mark(5);
 		RasterArgs ra;
 		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.list = list.p; ra.listCap = listCap;
@@ -671,6 +672,8 @@ struct rgl_gpu {
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
 		return 0; }
+
+	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
 
 	static int ProgramSlot(int pid) {
 		switch (pid) {

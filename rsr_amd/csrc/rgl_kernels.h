@@ -23,7 +23,10 @@ constexpr int kUnitTris = 256;      // triangles per binning unit (one wave), th
 constexpr long long kHistBudget = 8LL << 20;   // units x tiles histogram entries before units grow
 constexpr int kColChunk = 32;       // binning units per column-scan chunk
 constexpr int kRasterThreads = 256;
-constexpr int kMaxTileDim = 64;     // LDS key array is kMaxTileDim^2
+constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
+constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
+constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
+constexpr int kQuadrants = 4;
 
 struct Counters {
 	unsigned long long fragments;
@@ -246,28 +249,45 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 // column, so a wave's loads and stores cover contiguous bytes.
 constexpr int kColBatch = 16;   // independent loads in flight per thread
 
-__global__ __launch_bounds__(1024) void k_col_scan(uint32_t* __restrict__ csum, int nT, int nChunks,
-                                                   uint32_t* __restrict__ tileOff, uint32_t* __restrict__ total_out) {
+// One block: chunk prefixes within each column, then per tile the list
+// offset (exclusive scan of the column totals), the clipped-list offset
+// and cursor (scan of clipHist) and the visibility work items: each tile
+// is rasterised as kQuadrants quadrants and its inline + clipped pairs are
+// cut into segments of segPairs; a work item is (tile, segment, quadrant).
+__global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
+		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
+		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
-	uint32_t carry = 0;
+	const int nT = b.nTiles;
+	uint32_t carry = 0, ccarry = 0, wcarry = 0;
 	for (int base = 0; base < nT; base += 1024) {
 		const int t = base + threadIdx.x;
-		uint32_t col = 0;
-		if (t < nT)
+		uint32_t col = 0, v = 0, items = 0;
+		if (t < nT) {
 			for (int c0 = 0; c0 < nChunks; c0 += kColBatch) {
-				uint32_t v[kColBatch];
+				uint32_t w[kColBatch];
 #pragma unroll
 				for (int j = 0; j < kColBatch; ++j)
-					v[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
+					w[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
 #pragma unroll
 				for (int j = 0; j < kColBatch; ++j) {
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
-					col += v[j]; }}
-		uint32_t tot;
+					col += w[j]; }}
+			v = clipHist[t];
+			const uint32_t sp = static_cast<uint32_t>(b.segPairs);
+			items = kQuadrants * max(1u, (col + v + sp - 1) / sp); }
+		uint32_t tot, ctot, wtot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
-		if (t < nT) tileOff[t] = carry + ex;
-		carry += tot; }
-	if (threadIdx.x == 0) { tileOff[nT] = carry; *total_out = carry; }}
+		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
+		const uint32_t wex = block_exclusive_scan(items, sh, wtot);
+		if (t < nT) {
+			tileOff[t] = carry + ex;
+			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
+			workOff[t] = wcarry + wex; }
+		carry += tot; ccarry += ctot; wcarry += wtot; }
+	if (threadIdx.x == 0) {
+		tileOff[nT] = carry; clipOff[nT] = ccarry; workOff[nT] = wcarry;
+		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = wcarry; ctr->workNext = 0; }}
 
 __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
                                                  const uint32_t* __restrict__ tileOff) {
@@ -289,14 +309,48 @@ __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, in
 // keeping submission order: units in order (scan offsets), rounds in
 // order, lanes in order (ballot rank), one wave per unit.
 // ---------------------------------------------------------------------------
+// clipped fans into their tiles' clipped lists (ForEachCoveredTile,
+// rglv_gpu_impl.hxx:795-817); order within a tile's clipped list is free,
+// the visibility keys carry each fan's submission slot.  Run by the last
+// ncb blocks of k_scatter's grid.
+__device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, int ncb,
+		const uint32_t* __restrict__ clipq, const uint8_t* __restrict__ fcount,
+		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
+		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
+	const uint32_t nq = min(ctr->clipCount, clipCap);
+	for (uint32_t q = cb * blockDim.x + threadIdx.x; q < nq; q += ncb * blockDim.x) {
+		const int nf = fcount[q];
+		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, clipq[q])];
+		for (int j = 0; j < nf; ++j) {
+			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
+			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+			for (int ty = ty0; ty <= ty1; ++ty)
+				for (int tx = tx0; tx <= tx1; ++tx) {
+					const uint32_t pos = atomicAdd(&clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
+					if (pos < clistCap) clist[pos] = q * kFanMax + j;
+					else atomicOr(&ctr->overflow, 4u); }}}}
+
+// blocks [0, units): one unit each; blocks [units, units + nClipBlocks):
+// the clipped fans (clip_scatter)
+struct ClipScatterArgs {
+	const uint32_t* clipq; const uint8_t* fcount; const uint2* frect; uint32_t clipCap;
+	uint32_t* clipCursor; uint32_t* clist; uint32_t clistCap; int nBlocks;
+};
+
 __global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, const uint2* __restrict__ trect,
-		const uint32_t* __restrict__ off, uint32_t* __restrict__ list, uint32_t listCap, Counters* __restrict__ ctr) {
+		const uint32_t* __restrict__ off, uint32_t* __restrict__ list, uint32_t listCap, Counters* __restrict__ ctr,
+		ClipScatterArgs cs) {
 	// dynamic LDS: the batch's draws, then nTiles write cursors
 	extern __shared__ uint4 dyn_lds[];
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	uint32_t* base = reinterpret_cast<uint32_t*>(dl + b.nDraws);
 	const int u = blockIdx.x;
 	load_lite(dl, draws, b.nDraws);
+	if (u >= b.units) {
+		__syncthreads();
+		clip_scatter(b, dl, u - b.units, cs.nBlocks, cs.clipq, cs.fcount, cs.frect, cs.clipCap, cs.clipCursor, cs.clist,
+		             cs.clistCap, ctr);
+		return; }
 	for (int t = threadIdx.x; t < b.nTiles; t += 64) base[t] = off[static_cast<long long>(u) * b.nTiles + t];
 	__syncthreads();
 	const int lane = threadIdx.x;
@@ -575,9 +629,6 @@ struct RasterArgs {
 };
 
 constexpr unsigned long long kKeySentinel = ~0ull;
-constexpr int kSegPairs = 512;       // (triangle, tile) pairs per visibility work item
-constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
-constexpr int kQuadrants = 4;
 
 // visibility key of a covered fragment (see the header comment)
 __device__ inline unsigned long long frag_key(float dz, uint32_t slot, bool leq) {
@@ -593,53 +644,7 @@ __device__ inline uint32_t key_slot(unsigned long long k, bool leq) {
 	const uint32_t lo32 = static_cast<uint32_t>(k);
 	return leq ? ~lo32 : lo32; }
 
-// per-tile work items: each 64x64 tile is rasterised as four 32x32
-// quadrants, and the tile's inline and clipped pairs are cut into segments
-// of kSegPairs; a work item is (tile, segment, quadrant).  Also the
-// clipped-list offsets.
-__global__ __launch_bounds__(1024) void k_tile_scan(BatchDesc b, const uint32_t* __restrict__ tileOff, uint32_t listCap,
-		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor,
-		uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
-	__shared__ uint32_t sh[1024 / 64 + 1];
-	uint32_t carry = 0, wcarry = 0;
-	for (int base = 0; base < b.nTiles; base += 1024) {
-		const int i = base + threadIdx.x;
-		uint32_t v = 0, items = 0;
-		if (i < b.nTiles) {
-			v = clipHist[i];
-			const uint32_t lb = min(tileOff[i], listCap);
-			const uint32_t le = min(tileOff[i + 1], listCap);
-			const uint32_t sp = static_cast<uint32_t>(b.segPairs);
-			items = kQuadrants * max(1u, (le - lb + v + sp - 1) / sp); }
-		uint32_t tot, wtot;
-		const uint32_t ex = block_exclusive_scan(v, sh, tot);
-		const uint32_t wex = block_exclusive_scan(items, sh, wtot);
-		if (i < b.nTiles) { clipOff[i] = carry + ex; clipCursor[i] = carry + ex; workOff[i] = wcarry + wex; }
-		carry += tot; wcarry += wtot; }
-	if (threadIdx.x == 0) {
-		clipOff[b.nTiles] = carry; workOff[b.nTiles] = wcarry;
-		ctr->clipPairs = carry; ctr->workItems = wcarry; ctr->workNext = 0; }}
 
-__global__ __launch_bounds__(256) void k_clip_scatter(BatchDesc b, const DrawDesc* __restrict__ draws,
-		const uint32_t* __restrict__ clipq, const uint8_t* __restrict__ fcount,
-		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
-		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
-	extern __shared__ uint4 dyn_lds[];   // the batch's draws
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	load_lite(dl, draws, b.nDraws);
-	__syncthreads();
-	const uint32_t nq = min(ctr->clipCount, clipCap);
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
-		const int nf = fcount[q];
-		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, clipq[q])];
-		for (int j = 0; j < nf; ++j) {
-			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
-			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
-			for (int ty = ty0; ty <= ty1; ++ty)
-				for (int tx = tx0; tx <= tx1; ++tx) {
-					const uint32_t pos = atomicAdd(&clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
-					if (pos < clistCap) clist[pos] = q * kFanMax + j;
-					else atomicOr(&ctr->overflow, 4u); }}}}
 
 // ---------------------------------------------------------------------------
 // pair setup: one thread per (triangle, tile) pair of the inline and
