@@ -734,10 +734,13 @@ struct rgl_gpu {
 		const hipError_t e = hipEventSynchronize(done);
 		if (e != hipSuccess) return hipfail(e, "hipEventSynchronize");
 		stats.primitives_clipped = hCtr->clipCount;
-		stats.primitives_binned = hCtr->binned;
+		uint32_t binned = 0;
+		unsigned long long frags = 0;
+		for (const CtrSlot& c : hCtr->slot) { binned += c.binned; frags += c.fragments; }
+		stats.primitives_binned = binned;
 		stats.tile_pairs = hCtr->pairs;
 		stats.clipped_tile_pairs = hCtr->clipPairs;
-		stats.fragments = static_cast<int64_t>(hCtr->fragments);
+		stats.fragments = static_cast<int64_t>(frags);
 		stats.overflow = hCtr->overflow;
 		stats.fallback_tiles = hCtr->fallbackTiles;
 		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));

@@ -28,18 +28,32 @@ constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visib
 constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
 constexpr int kQuadrants = 4;
 
-struct Counters {
+// Tallies that many workgroups add to are split into kCtrSlots slots, one
+// 128-byte line each, and a workgroup adds into slot blockIdx % kCtrSlots:
+// workgroups are dealt round-robin over the eight XCDs, so each slot's
+// atomics come from one XCD's L2.  One device-wide address taking atomics
+// from every XCD costs tens of microseconds per kernel.
+constexpr int kCtrSlots = 8;
+
+struct alignas(128) CtrSlot {
 	unsigned long long fragments;
+	uint32_t binned;         // triangles accepted into >= 1 tile
+	uint32_t pad[29];
+};
+static_assert(sizeof(CtrSlot) == 128, "one line per slot");
+
+struct alignas(128) Counters {
 	uint32_t clipCount;
 	uint32_t pairs;          // inline (triangle, tile) pairs, from the scan
 	uint32_t clipPairs;
-	uint32_t binned;         // triangles accepted into >= 1 tile
 	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index, 4 bad list entry
 	uint32_t workItems;      // visibility work items of the frame
 	uint32_t workNext;       // next work item to hand out
 	uint32_t fallbackTiles;  // learned shading: tiles re-shaded by the plain shader
-	uint32_t pad[3];
+	uint32_t pad[25];
+	CtrSlot slot[kCtrSlots];
 };
+static_assert(sizeof(Counters) % 4 == 0, "zeroed as 32-bit words");
 
 // compact per-draw view used by the multi-draw kernels, kept in dynamic
 // LDS (nDraws entries, sized by the launch)
@@ -212,7 +226,7 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 		if (c) atomicAdd(&cs[t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
-	if (threadIdx.x == 0 && binned) atomicAdd(&ctr->binned, binned); }
+	if (threadIdx.x == 0 && binned) atomicAdd(&ctr->slot[blockIdx.x % kCtrSlots].binned, binned); }
 
 // ---------------------------------------------------------------------------
 // exclusive scan of uint32 (three passes); out may alias in.
@@ -774,6 +788,31 @@ __device__ inline int32_t step_mad(int32_t a, int32_t k, int32_t s, bool small) 
 	return small ? wadd(a, __mul24(k, s)) : wadd(a, wmul(k, s)); }
 
 constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per pass
+constexpr int kThreadQuads = 8;     // triangles with at most this many quads in the quadrant are walked by their own thread
+
+__device__ inline VisRec vis_record(const VisRec* __restrict__ rec, const VisRec* __restrict__ crec, uint32_t ib, uint32_t cb,
+                                    uint32_t ninl, uint32_t p) {
+	return p < ninl ? rec[ib + p] : crec[cb + (p - ninl)]; }
+
+// one quad of a triangle: the four lanes (c, c+dy, c+dx, c+dx+dy)
+// (rglv_triangle.hxx:253-258) with their depth keys merged by a 64-bit LDS
+// min; q1..q3 are the edge values at the quad's top-left pixel (lx, ly),
+// quadrant-relative
+__device__ inline void vis_quad(unsigned long long* keys, const VisSetup& v, int32_t q1, int32_t q2, int32_t q3,
+                                int lx, int ly, bool leq, uint32_t& frags) {
+#pragma unroll
+	for (int l = 0; l < 4; ++l) {
+		const int32_t e1 = (l & 2) ? wadd((l & 1) ? wadd(q1, v.dx12) : q1, (l & 1) ? v.dy12 : v.dx12) : ((l & 1) ? wadd(q1, v.dy12) : q1);
+		const int32_t e2 = (l & 2) ? wadd((l & 1) ? wadd(q2, v.dx23) : q2, (l & 1) ? v.dy23 : v.dx23) : ((l & 1) ? wadd(q2, v.dy23) : q2);
+		const int32_t e3 = (l & 2) ? wadd((l & 1) ? wadd(q3, v.dx31) : q3, (l & 1) ? v.dy31 : v.dx31) : ((l & 1) ? wadd(q3, v.dy31) : q3);
+		if ((e1 | e2 | e3) < 0) continue;
+		++frags;
+		const float bx = itof(e2) * v.scale;
+		const float bz = itof(e1) * v.scale;
+		const float by = (1.0F - bx) - bz;
+		const float dz = (bx*v.z0 + by*v.z1) + bz*v.z2;
+		if (dz != dz) continue;
+		atomicMin(&keys[(ly + (l >> 1)) * kSubDim + (lx + (l & 1))], frag_key(dz, v.slot, leq)); }}
 
 // visibility work items run one per workgroup; the grid is sized from the
 // previous frame's item count and strides over any remainder, so no
@@ -819,18 +858,23 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
 		const uint32_t pbeg = seg * static_cast<uint32_t>(b.segPairs), pend = min(pbeg + static_cast<uint32_t>(b.segPairs), ntot);
 		__syncthreads();
+		// each thread holds one pair's record; the next batch's record is
+		// loaded while this batch rasterises
+		VisRec rn;
+		if (f.drawEnabled && pbeg + threadIdx.x < pend) rn = vis_record(rec, crec, ib, cb, ninl, pbeg + threadIdx.x);
 		for (uint32_t b0 = pbeg; f.drawEnabled && b0 < pend; b0 += kRasterThreads) {
 			const uint32_t p = b0 + threadIdx.x;
+			const VisRec r = rn;
+			if (p + kRasterThreads < pend) rn = vis_record(rec, crec, ib, cb, ninl, p + kRasterThreads);
 			uint32_t nq = 0;
+			VisSetup v;
 			if (p < pend) {
-				const VisRec r = p < ninl ? rec[ib + p] : crec[cb + (p - ninl)];
 				const int minx = static_cast<int>(r.minxy & 0xffffu), miny = static_cast<int>(r.minxy >> 16);
 				const int nqx = static_cast<int>(r.nq & 0xffffu), nqy = static_cast<int>(r.nq >> 16);
 				// the triangle's quads inside the quadrant
 				const int kx0 = imax(0, (S.x0 - minx) >> 1), ky0 = imax(0, (S.y0 - miny) >> 1);
 				const int kx1 = imin(nqx, (S.x1 - minx + 1) >> 1), ky1 = imin(nqy, (S.y1 - miny + 1) >> 1);
 				const int nx = imax(0, kx1 - kx0), ny = imax(0, ky1 - ky0);
-				VisSetup v;
 				// edge values at the first quad, stepped exactly as the rasteriser loops do
 				v.c1 = wadd(wadd(r.c1, wmul(kx0, wmul(r.dy12, 2))), wmul(ky0, wmul(r.dx12, 2)));
 				v.c2 = wadd(wadd(r.c2, wmul(kx0, wmul(r.dy23, 2))), wmul(ky0, wmul(r.dx23, 2)));
@@ -845,49 +889,53 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 				v.nx = static_cast<uint32_t>(nx);
 				v.invNx = nx > 0 ? 1.0F / static_cast<float>(nx) : 0.0F;
 				v.scale = r.scale; v.z0 = r.z0; v.z1 = r.z1; v.z2 = r.z2; v.slot = r.slot;
-				ts[threadIdx.x] = v;
 				nq = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny); }
+			// small triangles: the thread walks its own quads row by row,
+			// stepping the edge values by wrapping adds (equal to the
+			// multiplied-out steps modulo 2^32)
+			const bool coop = nq > static_cast<uint32_t>(kThreadQuads);
+			if (!coop && nq != 0u) {
+				const int32_t sx1 = shl(v.dy12, 1), sx2 = shl(v.dy23, 1), sx3 = shl(v.dy31, 1);
+				const int32_t sy1 = shl(v.dx12, 1), sy2 = shl(v.dx23, 1), sy3 = shl(v.dx31, 1);
+				int32_t r1 = v.c1, r2 = v.c2, r3 = v.c3, q1 = r1, q2 = r2, q3 = r3;
+				int lx = v.x0, ly = v.y0;
+				uint32_t kx = 0;
+				for (uint32_t k = 0; k < nq; ++k) {
+					vis_quad(keys, v, q1, q2, q3, lx, ly, leq, frags);
+					if (++kx == v.nx) {
+						kx = 0; lx = v.x0; ly += 2;
+						r1 = wadd(r1, sy1); r2 = wadd(r2, sy2); r3 = wadd(r3, sy3);
+						q1 = r1; q2 = r2; q3 = r3; }
+					else {
+						lx += 2;
+						q1 = wadd(q1, sx1); q2 = wadd(q2, sx2); q3 = wadd(q3, sx3); }}}
+			// larger triangles: their quads are dealt out over the block
+			if (coop) ts[threadIdx.x] = v;
+			const uint32_t cq = coop ? nq : 0u;
 			uint32_t total;
-			const uint32_t ex = block_exclusive_scan(nq, scan_sh, total);
+			const uint32_t ex = block_exclusive_scan(cq, scan_sh, total);
 			qoff[threadIdx.x] = ex;
-			if (threadIdx.x == 0) qoff[kRasterThreads] = total;
 			for (uint32_t win0 = 0; win0 < total; win0 += kQuadWindow) {
 				// owner table: which triangle each quad item of the window belongs to
 				__syncthreads();
 				{
-					const uint32_t s0 = max(ex, win0), s1 = min(ex + nq, win0 + kQuadWindow);
+					const uint32_t s0 = max(ex, win0), s1 = min(ex + cq, win0 + kQuadWindow);
 					for (uint32_t q = s0; q < s1; ++q) owner[q - win0] = static_cast<uint8_t>(threadIdx.x);
 				}
 				__syncthreads();
 				const uint32_t wend = min(total, win0 + kQuadWindow);
 				for (uint32_t it = win0 + threadIdx.x; it < wend; it += kRasterThreads) {
 					const int j = owner[it - win0];
-					const VisSetup& v = ts[j];
+					const VisSetup& w = ts[j];
 					const uint32_t k = it - qoff[j];
-					const uint32_t kyr = static_cast<uint32_t>((static_cast<float>(k) + 0.5F) * v.invNx);
-					const uint32_t kxr = k - kyr * v.nx;
-					const bool small = v.small != 0u;
+					const uint32_t kyr = static_cast<uint32_t>((static_cast<float>(k) + 0.5F) * w.invNx);
+					const uint32_t kxr = k - kyr * w.nx;
+					const bool small = w.small != 0u;
 					const int32_t kx = static_cast<int32_t>(kxr), ky = static_cast<int32_t>(kyr);
-					const int32_t q1 = step_mad(step_mad(v.c1, kx, shl(v.dy12, 1), small), ky, shl(v.dx12, 1), small);
-					const int32_t q2 = step_mad(step_mad(v.c2, kx, shl(v.dy23, 1), small), ky, shl(v.dx23, 1), small);
-					const int32_t q3 = step_mad(step_mad(v.c3, kx, shl(v.dy31, 1), small), ky, shl(v.dx31, 1), small);
-					const int lx = v.x0 + 2 * kx, ly = v.y0 + 2 * ky;
-#pragma unroll
-					for (int l = 0; l < 4; ++l) {
-						// lanes (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
-						const int32_t e1 = (l & 2) ? wadd((l & 1) ? wadd(q1, v.dx12) : q1, (l & 1) ? v.dy12 : v.dx12) : ((l & 1) ? wadd(q1, v.dy12) : q1);
-						const int32_t e2 = (l & 2) ? wadd((l & 1) ? wadd(q2, v.dx23) : q2, (l & 1) ? v.dy23 : v.dx23) : ((l & 1) ? wadd(q2, v.dy23) : q2);
-						const int32_t e3 = (l & 2) ? wadd((l & 1) ? wadd(q3, v.dx31) : q3, (l & 1) ? v.dy31 : v.dx31) : ((l & 1) ? wadd(q3, v.dy31) : q3);
-						if ((e1 | e2 | e3) < 0) continue;
-						++frags;
-						const float bx = itof(e2) * v.scale;
-						const float bz = itof(e1) * v.scale;
-						const float by = (1.0F - bx) - bz;
-						const float dz = (bx*v.z0 + by*v.z1) + bz*v.z2;
-						if (dz != dz) continue;
-						const unsigned long long key = frag_key(dz, v.slot, leq);
-						const int li = (ly + (l >> 1)) * kSubDim + (lx + (l & 1));
-						if (key < keys[li]) atomicMin(&keys[li], key); }}}
+					const int32_t q1 = step_mad(step_mad(w.c1, kx, shl(w.dy12, 1), small), ky, shl(w.dx12, 1), small);
+					const int32_t q2 = step_mad(step_mad(w.c2, kx, shl(w.dy23, 1), small), ky, shl(w.dx23, 1), small);
+					const int32_t q3 = step_mad(step_mad(w.c3, kx, shl(w.dy31, 1), small), ky, shl(w.dx31, 1), small);
+					vis_quad(keys, w, q1, q2, q3, w.x0 + 2 * kx, w.y0 + 2 * ky, leq, frags); }}
 			__syncthreads(); }
 		// merge the quadrant into the frame's key buffer
 		const int rw = S.x1 - S.x0, rh = S.y1 - S.y0;
@@ -898,7 +946,13 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 			if (nseg == 1) *dst = k;
 			else if (k != kKeySentinel) atomicMin(dst, k); }}
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
-	if ((threadIdx.x & 63) == 0 && frags) atomicAdd(&a.ctr->fragments, static_cast<unsigned long long>(frags)); }
+	__syncthreads();
+	if ((threadIdx.x & 63) == 0) scan_sh[threadIdx.x >> 6] = frags;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		unsigned long long sum = 0;
+		for (int w = 0; w < kRasterThreads / 64; ++w) sum += scan_sh[w];
+		if (sum) atomicAdd(&a.ctr->slot[blockIdx.x % kCtrSlots].fragments, sum); }}
 
 // one lane of a quad of a triangle setup: perspective-correct barycentrics
 // BP and depth (TriangleProgram::Render, rglv_gpu_impl.hxx:169-191)
@@ -1070,8 +1124,7 @@ __global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long lon
 __global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles, uint32_t* csum, long long nCsum, uint32_t* qcount) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	for (long long c = i; c < nCsum; c += static_cast<long long>(gridDim.x) * blockDim.x) csum[c] = 0u;
-	if (i == 0) { ctr->fragments = 0; ctr->clipCount = 0; ctr->pairs = 0; ctr->clipPairs = 0; ctr->binned = 0; ctr->overflow = 0;
-	              ctr->workItems = 0; ctr->workNext = 0; ctr->fallbackTiles = 0; }
+	if (i < static_cast<int>(sizeof(Counters) / 4)) reinterpret_cast<uint32_t*>(ctr)[i] = 0u;
 	if (i < 8 * 32) qcount[i] = 0u;
 	for (int t = i; t < nTiles; t += gridDim.x * blockDim.x) clipHist[t] = 0u; }
 
