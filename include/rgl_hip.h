@@ -164,9 +164,12 @@ int rgl_gpu_graph_captures(rgl_gpu* gpu);   /* graphs captured so far */
    to at least 1 (a documented deviation for large targets). */
 int rgl_gpu_set_guard_band(rgl_gpu* gpu, int mode);
 
-/* (triangle, tile) pairs per visibility work item (default 512): the
-   autotuner's knob for how many workgroups share a dense tile. */
-int rgl_gpu_set_raster_params(rgl_gpu* gpu, int seg_pairs);
+/* The autotuner's knobs for how many workgroups share a dense tile:
+   seg_pairs = (triangle, tile) pairs per visibility work item (default
+   512); split_pairs = pairs per work item from which each 32 x 32
+   quadrant is cut in two, and from twice that in four (default 0 =
+   never).  Every setting draws identical frames. */
+int rgl_gpu_set_raster_params(rgl_gpu* gpu, int seg_pairs, int split_pairs);
 
 /* --- learned shading --------------------------------------------------- */
 /* No counterpart in the reference (its shaders are analytic).  A small MLP

@@ -1,6 +1,7 @@
-"""Autotuner for batched frames: picks the tile size and the number of
-(triangle, tile) pairs per visibility work item (how many workgroups share
-a dense tile) by timing the batch itself.
+"""Autotuner for batched frames: picks the tile size, the number of
+(triangle, tile) pairs per visibility work item and the pair count from
+which a work item's region is cut smaller (together: how many workgroups
+share a dense tile) by timing the batch itself.
 
 Every choice draws bit-identical frames (the rasteriser's arithmetic is
 invariant to the tile size and the visibility merge is order-independent),
@@ -13,7 +14,8 @@ import dataclasses
 import time
 
 TILE_CHOICES = ((8, 8), (4, 4))          # 64 and 32 pixel tiles (tile_blocks x 8)
-SEG_CHOICES = (128, 256, 512, 1024)
+SEG_CHOICES = (256, 512, 1024)
+SPLIT_CHOICES = (0, 64, 128, 256)
 
 
 def _record(ex_frames, scene_list, device, tile_blocks):
@@ -43,10 +45,11 @@ def time_batch(ex, frames, stream, steps: int, warmup: int = 3) -> float:
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def autotune(ex, scene_list, device="cuda:0", steps: int = 10, tiles=TILE_CHOICES, segs=SEG_CHOICES, log=None):
-    """Times every (tile size, pairs per item) choice on `ex` (graph replay
-    on) and leaves the fastest set: returns (frames, choice, table) where
-    frames are the recorders of the chosen tile size."""
+def autotune(ex, scene_list, device="cuda:0", steps: int = 10, tiles=TILE_CHOICES, segs=SEG_CHOICES,
+             splits=SPLIT_CHOICES, log=None):
+    """Times every (tile size, pairs per item, split threshold) choice on
+    `ex` (graph replay on) and leaves the fastest set: returns (frames,
+    choice, table) where frames are the recorders of the chosen tile size."""
     import torch
     stream = torch.cuda.current_stream(device).cuda_stream
     table = []
@@ -55,15 +58,16 @@ def autotune(ex, scene_list, device="cuda:0", steps: int = 10, tiles=TILE_CHOICE
     for tb in tiles:
         frames = _record(frames, scene_list, device, tb)
         for seg in segs:
-            ex.set_raster_params(seg)
-            ms = time_batch(ex, frames, stream, steps)
-            table.append({"tile_px": tb[0] * 8, "seg_pairs": seg, "ms": round(ms, 4)})
-            if log:
-                log(table[-1])
-            if best is None or ms < best[0]:
-                best = (ms, tb, seg)
-    _, tb, seg = best
+            for split in splits:
+                ex.set_raster_params(seg, split)
+                ms = time_batch(ex, frames, stream, steps)
+                table.append({"tile_px": tb[0] * 8, "seg_pairs": seg, "split_pairs": split, "ms": round(ms, 4)})
+                if log:
+                    log(table[-1])
+                if best is None or ms < best[0]:
+                    best = (ms, tb, seg, split)
+    _, tb, seg, split = best
     if frames is None or tb != tiles[-1]:
         frames = _record(frames, scene_list, device, tb)
-    ex.set_raster_params(seg)
-    return frames, {"tile_px": tb[0] * 8, "seg_pairs": seg}, table
+    ex.set_raster_params(seg, split)
+    return frames, {"tile_px": tb[0] * 8, "seg_pairs": seg, "split_pairs": split}, table

@@ -161,6 +161,7 @@ struct DrawDesc {
 	// the frame this draw renders into and its tile grid
 	int frame;
 	int tileW, tileH, tilesX, tilesY, tileBase;
+	int fW, fH;    // the frame's size
 };
 
 // one render target of a batch (a GPU::Reset'd canvas with its clear,
@@ -192,6 +193,7 @@ struct BatchDesc {
 	long long totalTris;     // S
 	int nBlocks;             // resolve blocks, all frames
 	int segPairs;            // (triangle, tile) pairs per visibility work item
+	int splitPairs;          // pairs per segment from which quadrants are cut in 2, and from twice this in 4 (0: never)
 	const FrameDesc* frames; // device table, nFrames entries
 };
 

@@ -97,9 +97,9 @@ struct DevArray {
 		return e; }
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
-const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_clip", "k_scan", "k_scatter", "k_pair_setup", "k_vis", "k_resolve",
+const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_clip", "k_scan", "k_scatter", "k_vis", "k_resolve",
                                "k_mlp_shade", "k_mlp_check" };
-constexpr int kNumTimed = 10;
+constexpr int kNumTimed = 9;
 
 // a StoreTrainingSamples request (no reference counterpart)
 struct SampleReq { float* out; uint32_t cap; uint32_t* count; float keep; uint32_t seed; };
@@ -141,10 +141,10 @@ struct rgl_gpu {
 	// device buffers
 	DevArray<float4> vpos; DevArray<uint8_t> vflags; DevArray<float> vvary;
 	DevArray<uint2> trect; DevArray<uint4> tvert;
-	DevArray<uint32_t> hist, bsum, tileOff, list;
+	DevArray<uint32_t> hist, bsum, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, clist, workOff;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff;
 	DevArray<unsigned long long> keys;
 	DevArray<VisRec> rec, crec;
 	size_t keysReady = 0;           // pixels of keys holding the sentinel
@@ -156,6 +156,7 @@ struct rgl_gpu {
 	DevArray<FrameDesc> dFrames;
 	std::vector<uint8_t> lastFrames;
 	int segPairs = kSegPairs;       // (triangle, tile) pairs per visibility work item
+	int splitPairs = kSplitPairs;   // pairs per segment from which visibility regions are halved
 	// graph replay of the launch sequence (rgl_gpu_set_graph)
 	bool graphMode = false, graphValid = false;
 	uint64_t graphKey = 0;
@@ -193,7 +194,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &list, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &clist, &workOff }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff }) a->release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
@@ -354,6 +355,7 @@ struct rgl_gpu {
 		d.scissor[0] = 0; d.scissor[1] = 0; d.scissor[2] = W; d.scissor[3] = H;
 		d.frame = fi;
 		d.tileW = tileW; d.tileH = tileH; d.tilesX = tilesX; d.tilesY = tilesY; d.tileBase = f.tileBase;
+		d.fW = W; d.fH = H;
 		d.keepBacks = !(s.cullingEnabled && s.cullFace == GL_BACK);
 		d.keepFronts = !(s.cullingEnabled && s.cullFace == GL_FRONT);
 		d.cullingEnabled = s.cullingEnabled; d.cullFace = s.cullFace;
@@ -390,7 +392,6 @@ struct rgl_gpu {
 		if ((e = bsum.ensure(nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
 		if ((e = tileOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
-		if ((e = list.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc list");
 		if ((e = rec.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
 		if (clipCap < std::min<long long>(S, 4096)) clipCap = static_cast<uint32_t>(std::min<long long>(S, 4096));
 		if ((e = clipq.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc clipq");
@@ -403,7 +404,6 @@ struct rgl_gpu {
 		if ((e = clipOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
 		if ((e = clipCursor.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
-		if ((e = clist.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clist");
 		if ((e = crec.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
 		if ((e = workOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
@@ -479,7 +479,7 @@ struct rgl_gpu {
 			ut = ((triBase * tileBase / kHistBudget + 1 + 63) / 64) * 64;
 		b.totalTris = triBase; b.unitTris = static_cast<int>(std::min<long long>(ut, 1 << 20));
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
-		b.nBlocks = blockBase; b.segPairs = segPairs; b.frames = nullptr;
+		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitPairs = splitPairs; b.frames = nullptr;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
 		int r = Ensure(b, vtot, keyBase);
@@ -549,8 +549,8 @@ struct rgl_gpu {
 		mix(&b, sizeof(b));
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
-		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, list.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, clist.p, workOff.p,
+		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -623,26 +623,25 @@ struct rgl_gpu {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
 		Mark(4);
-		if (b.units > 0) {
-			const ClipScatterArgs csa{ clipq.p, fcount.p, frect.p, clipCap, clipCursor.p, clist.p, clistCap, kClipScatterBlocks };
-			k_scatter<<<b.units + kClipScatterBlocks, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(
-				b, dDraws.p, trect.p, hist.p, list.p, listCap, dCtr, csa); }
-		Mark(5);
 		RasterArgs ra;
-		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.list = list.p; ra.listCap = listCap;
+		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
-		ra.clipOff = clipOff.p; ra.clist = clist.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
+		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
 		ra.keys = keys.p; ra.workOff = workOff.p;
+		if (b.units > 0) {
+			ScatterArgs sa;
+			sa.trect = trect.p; sa.tvert = tvert.p; sa.vpos = vpos.p; sa.off = hist.p; sa.rec = rec.p; sa.listCap = listCap;
+			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
+			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
+			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
+			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, sa); }
+		Mark(5);
 		if (b.nTiles > 0) {
-			const int setupBlocks = static_cast<int>(std::min<long long>((2LL * b.totalTris + 255) / 256 + 1, 4096));
-			k_pair_setup<<<dim3(setupBlocks, 3), 256, 0, stream>>>(b, ra, rec.p, crec.p);
-			Mark(6);
 			// one workgroup per work item, sized from the last completed frame;
 			// the kernel strides over any remainder
 			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p); }
-		else Mark(6);
-		Mark(7);
+		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		if (learned) {
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
@@ -655,9 +654,9 @@ struct rgl_gpu {
 			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
 			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
 			default: k_resolve<ShadeAny><<<rblocks, 256, dlBytes, stream>>>(b, ra); break; }}
-		Mark(8);
+		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
-		Mark(9);
+		Mark(8);
 		if (learned && b.nTiles > 0) {
 			// per-tile error estimate at probe pixels, then plain re-shading of
 			// the tiles over the limit
@@ -668,12 +667,13 @@ struct rgl_gpu {
 			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, tileErr.p, lim * lim, &dCtr->fallbackTiles); }
 		for (const SampleReq& q : reqs)
 			k_mlp_samples<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, q.out, q.cap, q.count, q.keep, q.seed);
-		Mark(10);
+		Mark(9);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
 		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
 		return 0; }
 
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
+	static constexpr int kResetBlocks = 128;        // k_scatter blocks that reset the keys of multi-segment tiles
 
 	static int ProgramSlot(int pid) {
 		switch (pid) {
@@ -746,7 +746,6 @@ struct rgl_gpu {
 		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));
 		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
-		if (hCtr->overflow & 16u) return fail(RGL_E_STATE, "tile list held an invalid entry");
 		if (hCtr->overflow) {
 			// grow and report; the caller runs the frame again
 			if (hCtr->overflow & 1u) listCap = static_cast<uint32_t>(std::min<long long>(2LL * hCtr->pairs + 4096, 0x7fffffffLL));
@@ -974,10 +973,12 @@ int rgl_gpu_set_guard_band(rgl_gpu* g, int mode) {
 	g->fixedGuardBand = mode == 1;
 	return 0; }
 
-int rgl_gpu_set_raster_params(rgl_gpu* g, int seg_pairs) {
+int rgl_gpu_set_raster_params(rgl_gpu* g, int seg_pairs, int split_pairs) {
 	if (check(g)) return RGL_E_ARG;
 	if (seg_pairs < 64 || seg_pairs > 4096) return g->fail(RGL_E_ARG, "seg_pairs must be in [64, 4096]");
+	if (split_pairs < 0 || split_pairs > 4096) return g->fail(RGL_E_ARG, "split_pairs must be in [0, 4096]");
 	g->segPairs = seg_pairs;
+	g->splitPairs = split_pairs;
 	g->graphValid = false;
 	return 0; }
 

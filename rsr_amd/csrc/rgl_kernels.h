@@ -25,6 +25,7 @@ constexpr int kColChunk = 32;       // binning units per column-scan chunk
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
 constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
+constexpr int kSplitPairs = 0;       // default pairs per segment from which visibility regions are halved
 constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
 constexpr int kQuadrants = 4;
 
@@ -46,7 +47,7 @@ struct alignas(128) Counters {
 	uint32_t clipCount;
 	uint32_t pairs;          // inline (triangle, tile) pairs, from the scan
 	uint32_t clipPairs;
-	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index, 4 bad list entry
+	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index
 	uint32_t workItems;      // visibility work items of the frame
 	uint32_t workNext;       // next work item to hand out
 	uint32_t fallbackTiles;  // learned shading: tiles re-shaded by the plain shader
@@ -67,6 +68,7 @@ struct alignas(16) DrawLite {
 	uint32_t slotBase;
 	uint32_t primBase;
 	int tileW, tileH, tilesX, tilesY, tileBase;   // the draw's frame tile grid
+	int fW, fH;                                  // and its size
 };
 
 __device__ inline int find_draw_ll(const DrawLite* dl, int nDraws, long long s) {
@@ -89,6 +91,7 @@ __device__ inline void load_lite(DrawLite* dl, const DrawDesc* draws, int nDraws
 		for (int i = 0; i < 4; ++i) l.scissor[i] = d.scissor[i];
 		l.slotBase = d.slotBase; l.primBase = d.primBase;
 		l.tileW = d.tileW; l.tileH = d.tileH; l.tilesX = d.tilesX; l.tilesY = d.tilesY; l.tileBase = d.tileBase;
+		l.fW = d.fW; l.fH = d.fH;
 		dl[k] = l; }}
 
 // LDS table of the frames' first tiles (n + 1 entries, the last = total)
@@ -263,11 +266,24 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 // column, so a wave's loads and stores cover contiguous bytes.
 constexpr int kColBatch = 16;   // independent loads in flight per thread
 
+// Visibility work items of a tile: its inline + clipped pairs are cut into
+// segments of segPairs, and the tile into regions: the four 32 x 32
+// quadrants, each cut in two (32 x 16) when a segment holds splitPairs
+// pairs or more and in four (16 x 16) from twice that.  A work item is
+// (tile, segment, region); the scan and the visibility kernel both derive
+// the counts from the tile's pair total.
+__device__ inline uint32_t vis_segments(const BatchDesc& b, uint32_t pairs) {
+	const uint32_t sp = static_cast<uint32_t>(b.segPairs);
+	return max(1u, (pairs + sp - 1) / sp); }
+
+__device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t pairs, uint32_t nseg) {
+	if (b.splitPairs <= 0) return kQuadrants;
+	const uint32_t pps = (pairs + nseg - 1) / nseg, s = static_cast<uint32_t>(b.splitPairs);
+	return pps >= 2 * s ? 4 * kQuadrants : pps >= s ? 2 * kQuadrants : kQuadrants; }
+
 // One block: chunk prefixes within each column, then per tile the list
 // offset (exclusive scan of the column totals), the clipped-list offset
-// and cursor (scan of clipHist) and the visibility work items: each tile
-// is rasterised as kQuadrants quadrants and its inline + clipped pairs are
-// cut into segments of segPairs; a work item is (tile, segment, quadrant).
+// and cursor (scan of clipHist) and the tile's visibility work items.
 __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
 		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
@@ -288,8 +304,8 @@ __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __rest
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
 					col += w[j]; }}
 			v = clipHist[t];
-			const uint32_t sp = static_cast<uint32_t>(b.segPairs);
-			items = kQuadrants * max(1u, (col + v + sp - 1) / sp); }
+			const uint32_t nseg = vis_segments(b, col + v);
+			items = nseg * vis_regions(b, col + v, nseg); }
 		uint32_t tot, ctot, wtot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
@@ -317,86 +333,6 @@ __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, in
 		for (int j = 0; j < kColBatch; ++j) {
 			if (b + j < u1) hist[static_cast<long long>(b + j) * nT + t] = run;
 			run += v[j]; }}}
-
-// ---------------------------------------------------------------------------
-// ordered scatter: the per-tile append loop of BinTriangles (:651-661),
-// keeping submission order: units in order (scan offsets), rounds in
-// order, lanes in order (ballot rank), one wave per unit.
-// ---------------------------------------------------------------------------
-// clipped fans into their tiles' clipped lists (ForEachCoveredTile,
-// rglv_gpu_impl.hxx:795-817); order within a tile's clipped list is free,
-// the visibility keys carry each fan's submission slot.  Run by the last
-// ncb blocks of k_scatter's grid.
-__device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, int ncb,
-		const uint32_t* __restrict__ clipq, const uint8_t* __restrict__ fcount,
-		const uint2* __restrict__ frect, uint32_t clipCap, uint32_t* __restrict__ clipCursor,
-		uint32_t* __restrict__ clist, uint32_t clistCap, Counters* __restrict__ ctr) {
-	const uint32_t nq = min(ctr->clipCount, clipCap);
-	for (uint32_t q = cb * blockDim.x + threadIdx.x; q < nq; q += ncb * blockDim.x) {
-		const int nf = fcount[q];
-		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, clipq[q])];
-		for (int j = 0; j < nf; ++j) {
-			const uint2 rc = frect[static_cast<long long>(q) * kFanMax + j];
-			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
-			for (int ty = ty0; ty <= ty1; ++ty)
-				for (int tx = tx0; tx <= tx1; ++tx) {
-					const uint32_t pos = atomicAdd(&clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
-					if (pos < clistCap) clist[pos] = q * kFanMax + j;
-					else atomicOr(&ctr->overflow, 4u); }}}}
-
-// blocks [0, units): one unit each; blocks [units, units + nClipBlocks):
-// the clipped fans (clip_scatter)
-struct ClipScatterArgs {
-	const uint32_t* clipq; const uint8_t* fcount; const uint2* frect; uint32_t clipCap;
-	uint32_t* clipCursor; uint32_t* clist; uint32_t clistCap; int nBlocks;
-};
-
-__global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, const uint2* __restrict__ trect,
-		const uint32_t* __restrict__ off, uint32_t* __restrict__ list, uint32_t listCap, Counters* __restrict__ ctr,
-		ClipScatterArgs cs) {
-	// dynamic LDS: the batch's draws, then nTiles write cursors
-	extern __shared__ uint4 dyn_lds[];
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	uint32_t* base = reinterpret_cast<uint32_t*>(dl + b.nDraws);
-	const int u = blockIdx.x;
-	load_lite(dl, draws, b.nDraws);
-	if (u >= b.units) {
-		__syncthreads();
-		clip_scatter(b, dl, u - b.units, cs.nBlocks, cs.clipq, cs.fcount, cs.frect, cs.clipCap, cs.clipCursor, cs.clist,
-		             cs.clistCap, ctr);
-		return; }
-	for (int t = threadIdx.x; t < b.nTiles; t += 64) base[t] = off[static_cast<long long>(u) * b.nTiles + t];
-	__syncthreads();
-	const int lane = threadIdx.x;
-	const unsigned long long lt = (1ull << lane) - 1ull;
-	bool ovf = false;
-	const long long s0 = static_cast<long long>(u) * b.unitTris;
-	for (int r = 0; r < b.unitTris / 64; ++r) {
-		const long long s = s0 + r * 64 + lane;
-		int tx0 = 1, tx1 = 0, ty0 = 1, ty1 = 0, tb = 0, tsx = 1;
-		if (s < b.totalTris) {
-			const uint2 rc = trect[s];
-			tx0 = rc.x & 0xffffu; tx1 = rc.x >> 16; ty0 = rc.y & 0xffffu; ty1 = rc.y >> 16;
-			const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
-			tb = d.tileBase; tsx = d.tilesX; }
-		const bool has = tx0 <= tx1 && ty0 <= ty1;
-		int cx = tx0, cy = ty0;
-		uint32_t cur = has ? static_cast<uint32_t>(tb + cy * tsx + cx) : 0xffffffffu;
-		for (;;) {
-			uint32_t m = cur;
-			for (int o = 32; o > 0; o >>= 1) { const uint32_t y = __shfl_xor(m, o, 64); m = y < m ? y : m; }
-			if (m == 0xffffffffu) break;
-			const unsigned long long mask = __ballot(cur == m);
-			const uint32_t b = base[m];
-			if (cur == m) {
-				const uint32_t pos = b + static_cast<uint32_t>(__popcll(mask & lt));
-				if (pos < listCap) list[pos] = static_cast<uint32_t>(s); else ovf = true;
-				if (++cx > tx1) { cx = tx0; ++cy; }
-				cur = cy <= ty1 ? static_cast<uint32_t>(tb + cy * tsx + cx) : 0xffffffffu; }
-			__syncthreads();
-			if (lane == 0) base[m] = b + static_cast<uint32_t>(__popcll(mask));
-			__syncthreads(); }}
-	if (ovf) atomicOr(&ctr->overflow, 1u); }
 
 // ---------------------------------------------------------------------------
 // clipping: GPUBinImpl::ClipTriangles (rglv_gpu_impl.hxx:677-793),
@@ -623,8 +559,7 @@ __device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cv
 
 struct RasterArgs {
 	const DrawDesc* draws;
-	const uint32_t* tileOff;    // inline list: tile t's pairs are [tileOff[t], tileOff[t + 1])
-	const uint32_t* list;
+	const uint32_t* tileOff;    // inline records: tile t's pairs are [tileOff[t], tileOff[t + 1])
 	uint32_t listCap;
 	const uint4* tvert;
 	const float4* vpos;
@@ -633,8 +568,7 @@ struct RasterArgs {
 	const uint32_t* clipqOf;
 	const float4* cvpos;
 	const float* cvvary;
-	const uint32_t* clipOff;
-	const uint32_t* clist;
+	const uint32_t* clipOff;    // clipped records: tile t's fans are [clipOff[t], clipOff[t + 1])
 	uint32_t clistCap;
 	uint32_t clipCap;
 	Counters* ctr;
@@ -661,12 +595,13 @@ __device__ inline uint32_t key_slot(unsigned long long k, bool leq) {
 
 
 // ---------------------------------------------------------------------------
-// pair setup: one thread per (triangle, tile) pair of the inline and
-// clipped lists computes the rasteriser setup of that triangle against that
-// tile — VTriangleRasterizer (rglv_triangle.hxx:206-240) for inline
-// triangles, TriangleRasterizer (:84-131) for clipped fans — and stores it
-// as a 64-byte record at the pair's list position.  Records of a tile are
-// contiguous and in list order.
+// scatter + pair setup: each (triangle, tile) pair's rasteriser setup
+// against the tile rect — VTriangleRasterizer (rglv_triangle.hxx:206-240)
+// for inline triangles, TriangleRasterizer (:84-131) for clipped fans —
+// stored as a 64-byte record at the pair's position in its tile's list
+// (the per-tile append of BinTriangles, rglv_gpu_impl.hxx:651-661).  A
+// tile's records are contiguous.  Their order within the tile is free: the
+// visibility keys carry each triangle's submission slot.
 // ---------------------------------------------------------------------------
 struct alignas(16) VisRec {
 	int32_t c1, c2, c3;
@@ -679,19 +614,6 @@ struct alignas(16) VisRec {
 };
 static_assert(sizeof(VisRec) == 64, "VisRec is four 16-byte loads");
 
-struct VisDraw { long long triBase; uint32_t slotBase, nTri; };
-
-__device__ inline void load_visdraws(VisDraw* vd, const DrawDesc* draws, int nDraws) {
-	for (int k = threadIdx.x; k < nDraws; k += blockDim.x) {
-		const DrawDesc& d = draws[k];
-		vd[k].triBase = d.triBase; vd[k].slotBase = d.slotBase;
-		vd[k].nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances); }}
-
-__device__ inline int find_visdraw(const VisDraw* vd, int nDraws, long long s) {
-	int lo = 0, hi = nDraws - 1;
-	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (vd[mid].triBase <= s) lo = mid; else hi = mid - 1; }
-	return lo; }
-
 __device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_t slot) {
 	VisRec v;
 	v.c1 = t.c1; v.c2 = t.c2; v.c3 = t.c3;
@@ -701,62 +623,111 @@ __device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_
 	v.scale = t.scale; v.z0 = tr.z[0]; v.z1 = tr.z[1]; v.z2 = tr.z[2]; v.slot = slot;
 	return v; }
 
+// (triangle, tile) pairs of a tile, inline and clipped, as the scan counted them
+__device__ inline uint32_t tile_pairs(const uint32_t* tileOff, const uint32_t* clipOff, int t) {
+	return (tileOff[t + 1] - tileOff[t]) + (clipOff[t + 1] - clipOff[t]); }
+
+__device__ inline Rect draw_tile_rect(const DrawLite& d, int tx, int ty) {
+	Rect r;
+	r.x0 = tx * d.tileW; r.y0 = ty * d.tileH;
+	r.x1 = imin(r.x0 + d.tileW, d.fW); r.y1 = imin(r.y0 + d.tileH, d.fH);
+	return r; }
+
 constexpr int kMaxTilesLds = 2048;   // tile offset tables cached in LDS up to this many tiles
 
-__global__ __launch_bounds__(256) void k_pair_setup(BatchDesc b, RasterArgs a, VisRec* __restrict__ rec, VisRec* __restrict__ crec) {
-	__shared__ uint32_t toff[kMaxTilesLds + 1];
-	__shared__ VisDraw vd[kMaxDraws];
+struct ScatterArgs {
+	const uint2* trect; const uint4* tvert; const float4* vpos;
+	const uint32_t* off;          // per (unit, tile) first list position, unit-major
+	VisRec* rec; uint32_t listCap;
+	// clipped fans (blocks [units, units + clipBlocks))
+	const uint32_t* clipq; const uint8_t* fcount; const uint2* frect; const float4* cvpos; uint32_t clipCap;
+	uint32_t* clipCursor; VisRec* crec; uint32_t clistCap; int clipBlocks;
+	// key reset of tiles cut into several segments (the remaining blocks)
+	const uint32_t* tileOff; const uint32_t* clipOff; unsigned long long* keys;
+	Counters* ctr;
+};
+
+// clipped fans into their tiles' clipped lists (ForEachCoveredTile,
+// rglv_gpu_impl.hxx:795-817), positions taken from per-tile cursors
+__device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, const ScatterArgs& a) {
+	const uint32_t nq = min(a.ctr->clipCount, a.clipCap);
+	for (uint32_t q = cb * blockDim.x + threadIdx.x; q < nq; q += a.clipBlocks * blockDim.x) {
+		const int nf = a.fcount[q];
+		const uint32_t s = a.clipq[q];
+		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
+		const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
+		for (int j = 0; j < nf; ++j) {
+			const uint2 rc = a.frect[static_cast<long long>(q) * kFanMax + j];
+			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+			TriVerts tr;
+			clipped_verts(q * kFanMax + j, a.cvpos, tr);
+			const uint32_t slot = d.slotBase + nTri + static_cast<uint32_t>(s - d.triBase) * kFanMax + static_cast<uint32_t>(j);
+			for (int ty = ty0; ty <= ty1; ++ty)
+				for (int tx = tx0; tx <= tx1; ++tx) {
+					const uint32_t pos = atomicAdd(&a.clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
+					if (pos < a.clistCap) {
+						TriSetup t;
+						setup_scalar(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
+						a.crec[pos] = make_rec(t, tr, slot); }
+					else atomicOr(&a.ctr->overflow, 4u); }}}}
+
+// tiles cut into several segments merge their regions with a 64-bit
+// atomic min, so their keys start the frame at the sentinel; tiles of one
+// segment are overwritten whole by plain stores
+__device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, int nrb, const ScatterArgs& a) {
+	for (int tile = rb; tile < b.nTiles; tile += nrb) {
+		if (vis_segments(b, tile_pairs(a.tileOff, a.clipOff, tile)) == 1u) continue;
+		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+		const Rect R = tile_rect(f, tile - f.tileBase);
+		const int rw = R.x1 - R.x0, n = rw * (R.y1 - R.y0);
+		for (int i = threadIdx.x; i < n; i += blockDim.x) {
+			const int ly = i / rw, lx = i - ly * rw;
+			a.keys[f.keyBase + static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}}
+
+// blocks [0, units): one binning unit each, one wave; each lane walks its
+// triangle's tiles, takes a list position from the unit's LDS cursor of
+// the tile and writes the pair's setup record.  Blocks [units, units +
+// clipBlocks): the clipped fans.  The rest: key resets.
+__global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, ScatterArgs a) {
+	// dynamic LDS: the batch's draws, then nTiles write cursors
+	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
-	load_tile_bases(ftb, b);
-	if (blockIdx.y == 2) {
-		// tiles cut into several segments merge their quadrants with a 64-bit
-		// atomic min, so their keys start the frame at the sentinel; tiles of
-		// one segment are overwritten whole by plain stores
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	uint32_t* base = reinterpret_cast<uint32_t*>(dl + b.nDraws);
+	const int u = blockIdx.x;
+	if (u >= b.units + a.clipBlocks) {
+		load_tile_bases(ftb, b);
 		__syncthreads();
-		for (int tile = blockIdx.x; tile < b.nTiles; tile += gridDim.x) {
-			if (a.workOff[tile + 1] - a.workOff[tile] <= static_cast<uint32_t>(kQuadrants)) continue;
-			const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
-			const Rect R = tile_rect(f, tile - f.tileBase);
-			const int rw = R.x1 - R.x0, n = rw * (R.y1 - R.y0);
-			for (int i = threadIdx.x; i < n; i += 256) {
-				const int ly = i / rw, lx = i - ly * rw;
-				a.keys[f.keyBase + static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}
+		reset_keys(b, ftb, u - b.units - a.clipBlocks, static_cast<int>(gridDim.x) - b.units - a.clipBlocks, a);
 		return; }
-	load_visdraws(vd, a.draws, b.nDraws);
-	const bool clippedPass = blockIdx.y == 1;
-	const uint32_t total = clippedPass ? min(a.clipOff[b.nTiles], a.clistCap)
-	                                   : min(a.tileOff[b.nTiles], a.listCap);
-	const bool lds = b.nTiles <= kMaxTilesLds;
-	if (lds)
-		for (int t = threadIdx.x; t <= b.nTiles; t += 256)
-			toff[t] = clippedPass ? a.clipOff[t] : a.tileOff[t];
+	load_lite(dl, draws, b.nDraws);
+	if (u >= b.units) {
+		__syncthreads();
+		clip_scatter(b, dl, u - b.units, a);
+		return; }
+	for (int t = threadIdx.x; t < b.nTiles; t += 64) base[t] = a.off[static_cast<long long>(u) * b.nTiles + t];
 	__syncthreads();
-	for (uint32_t p = blockIdx.x * 256 + threadIdx.x; p < total; p += gridDim.x * 256) {
-		int lo = 0, hi = b.nTiles - 1;  // last tile whose list starts at or before p
-		while (lo < hi) {
-			const int mid = (lo + hi + 1) >> 1;
-			const uint32_t o = lds ? toff[mid] : (clippedPass ? a.clipOff[mid] : a.tileOff[mid]);
-			if (o <= p) lo = mid; else hi = mid - 1; }
-		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, lo)];
-		const Rect R = tile_rect(f, lo - f.tileBase);
+	bool ovf = false;
+	const long long s0 = static_cast<long long>(u) * b.unitTris;
+	for (int r = 0; r < b.unitTris / 64; ++r) {
+		const long long s = s0 + r * 64 + threadIdx.x;
+		if (s >= b.totalTris) break;
+		const uint2 rc = a.trect[s];
+		const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+		if (tx0 > tx1 || ty0 > ty1) continue;
+		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
 		TriVerts tr;
-		TriSetup t;
-		if (!clippedPass) {
-			uint32_t s = a.list[p];
-			if (s >= b.totalTris) { atomicOr(&a.ctr->overflow, 16u); s = 0; }
-			const int di = find_visdraw(vd, b.nDraws, s);
-			inline_verts(a.tvert[s], a.vpos, tr);
-			setup_vector(R, tr.X, tr.Y, t);
-			rec[p] = make_rec(t, tr, vd[di].slotBase + static_cast<uint32_t>(s - vd[di].triBase)); }
-		else {
-			uint32_t fan = a.clist[p];
-			if (fan / kFanMax >= a.clipCap) { atomicOr(&a.ctr->overflow, 16u); fan = 0; }
-			const uint32_t q = fan / kFanMax, j = fan - q * kFanMax;
-			const uint32_t s = a.clipq[q];
-			const int di = find_visdraw(vd, b.nDraws, s);
-			clipped_verts(fan, a.cvpos, tr);
-			setup_scalar(R, tr.X, tr.Y, t);
-			crec[p] = make_rec(t, tr, vd[di].slotBase + vd[di].nTri + static_cast<uint32_t>(s - vd[di].triBase) * kFanMax + j); }}}
+		inline_verts(a.tvert[s], a.vpos, tr);
+		const uint32_t slot = d.slotBase + static_cast<uint32_t>(s - d.triBase);
+		for (int ty = ty0; ty <= ty1; ++ty)
+			for (int tx = tx0; tx <= tx1; ++tx) {
+				const uint32_t pos = atomicAdd(&base[d.tileBase + ty * d.tilesX + tx], 1u);
+				if (pos < a.listCap) {
+					TriSetup t;
+					setup_vector(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
+					a.rec[pos] = make_rec(t, tr, slot); }
+				else ovf = true; }}
+	if (ovf) atomicOr(&a.ctr->overflow, 1u); }
 
 // ---------------------------------------------------------------------------
 // visibility: a persistent kernel over (tile, segment, quadrant) work
@@ -788,7 +759,7 @@ __device__ inline int32_t step_mad(int32_t a, int32_t k, int32_t s, bool small) 
 	return small ? wadd(a, __mul24(k, s)) : wadd(a, wmul(k, s)); }
 
 constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per pass
-constexpr int kThreadQuads = 8;     // triangles with at most this many quads in the quadrant are walked by their own thread
+constexpr int kThreadQuads = 2;     // triangles with at most this many quads in the quadrant are walked by their own thread
 
 __device__ inline VisRec vis_record(const VisRec* __restrict__ rec, const VisRec* __restrict__ crec, uint32_t ib, uint32_t cb,
                                     uint32_t ninl, uint32_t p) {
@@ -841,15 +812,21 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		const uint32_t w0 = lds ? wofs[tile] : a.workOff[tile];
 		const uint32_t w1 = lds ? wofs[tile + 1] : a.workOff[tile + 1];
 		const uint32_t local = item - w0;
-		const uint32_t nseg = (w1 - w0) / kQuadrants;
-		const uint32_t seg = local / kQuadrants, quad = local - seg * kQuadrants;
+		const uint32_t nseg = vis_segments(b, tile_pairs(a.tileOff, a.clipOff, tile));
+		const uint32_t nreg = (w1 - w0) / nseg;
+		const uint32_t seg = local / nreg, reg = local - seg * nreg;
 		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
 		const bool leq = f.depthFunc == 1;
 		const Rect R = tile_rect(f, tile - f.tileBase);
+		// the item's region: a quadrant, or its half or quarter
+		const uint32_t per = max(1u, nreg / kQuadrants), quad = reg / per, sub = reg - quad * per;
 		Rect S;
-		S.x0 = R.x0 + (quad & 1) * kSubDim; S.y0 = R.y0 + (quad >> 1) * kSubDim;
-		S.x1 = imin(S.x0 + kSubDim, R.x1); S.y1 = imin(S.y0 + kSubDim, R.y1);
-		if (S.x0 >= S.x1 || S.y0 >= S.y1) continue;   // quadrant outside the screen (uniform)
+		S.x0 = R.x0 + static_cast<int>(quad & 1) * kSubDim; S.y0 = R.y0 + static_cast<int>(quad >> 1) * kSubDim;
+		int sw = kSubDim, sh = kSubDim;
+		if (per == 2) { S.y0 += static_cast<int>(sub) * (kSubDim / 2); sh = kSubDim / 2; }
+		else if (per == 4) { S.x0 += static_cast<int>(sub & 1) * (kSubDim / 2); S.y0 += static_cast<int>(sub >> 1) * (kSubDim / 2); sw = sh = kSubDim / 2; }
+		S.x1 = imin(S.x0 + sw, R.x1); S.y1 = imin(S.y0 + sh, R.y1);
+		if (S.x0 >= S.x1 || S.y0 >= S.y1) continue;   // region outside the screen (uniform)
 		for (int i = threadIdx.x; i < kSubDim * kSubDim; i += kRasterThreads) keys[i] = kKeySentinel;
 		const uint32_t ib = min(a.tileOff[tile], a.listCap);
 		const uint32_t ie = min(a.tileOff[tile + 1], a.listCap);

@@ -80,7 +80,7 @@ def lib():
         L.rgl_gpu_run_batch.argtypes = [vp, ctypes.POINTER(vp), i, vp]
         L.rgl_gpu_set_graph.argtypes = [vp, i]
         L.rgl_gpu_graph_captures.argtypes = [vp]
-        L.rgl_gpu_set_raster_params.argtypes = [vp, i]
+        L.rgl_gpu_set_raster_params.argtypes = [vp, i, i]
         L.rgl_gpu_set_guard_band.argtypes = [vp, i]
         L.rgl_mlp_param_count.argtypes = []
         L.rgl_mlp_trainable_count.argtypes = []
@@ -207,8 +207,8 @@ class GPU:
     def set_guard_band(self, fixed: bool):
         self._check(self.L.rgl_gpu_set_guard_band(self.h, int(fixed)))
 
-    def set_raster_params(self, seg_pairs: int):
-        self._check(self.L.rgl_gpu_set_raster_params(self.h, seg_pairs))
+    def set_raster_params(self, seg_pairs: int, split_pairs: int = 0):
+        self._check(self.L.rgl_gpu_set_raster_params(self.h, seg_pairs, split_pairs))
 
     # --- learned shading (no reference counterpart) ---------------------------
     def set_learned_shading(self, dev_params, max_rms_err: float = 2.0 / 255.0):
