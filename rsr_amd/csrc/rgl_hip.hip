@@ -387,7 +387,7 @@ struct rgl_gpu {
 		if ((e = trect.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
 		if ((e = tvert.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
 		const long long nh = static_cast<long long>(b.nTiles) * std::max(b.units, 1) + 1;
-		const long long nChunks = (std::max(b.units, 1) + kColChunk - 1) / kColChunk;
+		const long long nChunks = unit_chunks(std::max(b.units, 1));
 		if ((e = hist.ensure(nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
 		if ((e = bsum.ensure(nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
 		if ((e = tileOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
@@ -588,7 +588,7 @@ struct rgl_gpu {
 	           const std::vector<SampleReq>& reqs) {
 		hipError_t e;
 		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
-		const long long nCsum = static_cast<long long>((b.units + kColChunk - 1) / kColChunk) * b.nTiles;
+		const long long nCsum = static_cast<long long>(unit_chunks(b.units)) * b.nTiles;
 		k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(
 			dCtr, clipHist.p, b.nTiles, bsum.p, nCsum, qCount.p);
 		Mark(0);
@@ -617,7 +617,7 @@ struct rgl_gpu {
 		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
-		const int nChunks = (b.units + kColChunk - 1) / kColChunk;
+		const int nChunks = unit_chunks(b.units);
 		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);

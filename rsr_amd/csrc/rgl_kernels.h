@@ -22,6 +22,14 @@ namespace rgl {
 constexpr int kUnitTris = 256;      // triangles per binning unit (one wave), the minimum
 constexpr long long kHistBudget = 8LL << 20;   // units x tiles histogram entries before units grow
 constexpr int kColChunk = 32;       // binning units per column-scan chunk
+constexpr int kXcds = 8;             // workgroups are dealt round-robin over the eight XCDs
+
+// Column-scan chunks group the binning units that run on one XCD: chunk c
+// holds units x + 8 (32 g + k), k < 32, with x = c % 8 and g = c / 8, so
+// that a chunk's count atomics all come from one XCD's L2.
+__host__ __device__ inline int unit_chunk(int u) { return (u / (kXcds * kColChunk)) * kXcds + u % kXcds; }
+__host__ __device__ inline int unit_chunks(int units) { return ((units + kXcds * kColChunk - 1) / (kXcds * kColChunk)) * kXcds; }
+__host__ __device__ inline int chunk_unit(int c, int k) { return (c % kXcds) + kXcds * ((c / kXcds) * kColChunk + k); }
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
 constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
@@ -179,6 +187,7 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 			bad |= ii[k] >= static_cast<uint32_t>(d.verts); }
 		uint2 rect = empty_rect();
 		uint4 tv = make_uint4(0, 0, 0, 0);
+		bool clips = false;
 		if (bad) {
 			atomicOr(&ctr->overflow, 8u); }
 		else {
@@ -190,10 +199,7 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 				cf[k] = vflags[vb + ii[k]]; }
 			const bool pointsOutside = (cf[0] | cf[1] | cf[2]) != 0u;
 			const bool primsOutside = (cf[0] & cf[1] & cf[2]) != 0u;
-			if (!primsOutside && pointsOutside) {
-				const uint32_t q = atomicAdd(&ctr->clipCount, 1u);
-				if (q < clipCap) { clipq[q] = static_cast<uint32_t>(s); clipqOf[s] = q; }
-				else atomicOr(&ctr->overflow, 2u); }
+			clips = !primsOutside && pointsOutside;
 			if (!pointsOutside) {
 				const float d31x = dcx[2] - dcx[0], d31y = dcy[2] - dcy[0];
 				const float d21x = dcx[1] - dcx[0], d21y = dcy[1] - dcy[0];
@@ -220,9 +226,21 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 						for (int tx = tx0; tx <= tx1; ++tx)
 							atomicAdd(&lhist[d.tileBase + ty * d.tilesX + tx], 1u); }}}
 		trect[s] = rect;
-		tvert[s] = tv; }
+		tvert[s] = tv;
+		// the wave's clipped triangles take consecutive clip-queue entries
+		// from one atomic
+		const unsigned long long cm = __ballot(clips);
+		if (cm) {
+			uint32_t q0 = 0;
+			if (threadIdx.x == static_cast<unsigned>(__ffsll(static_cast<long long>(cm)) - 1))
+				q0 = atomicAdd(&ctr->clipCount, static_cast<uint32_t>(__popcll(cm)));
+			q0 = __shfl(q0, __ffsll(static_cast<long long>(cm)) - 1, 64);
+			if (clips) {
+				const uint32_t q = q0 + static_cast<uint32_t>(__popcll(cm & ((1ull << threadIdx.x) - 1ull)));
+				if (q < clipCap) { clipq[q] = static_cast<uint32_t>(s); clipqOf[s] = q; }
+				else atomicOr(&ctr->overflow, 2u); }}}
 	__syncthreads();
-	uint32_t* cs = csum + static_cast<long long>(u / kColChunk) * b.nTiles;
+	uint32_t* cs = csum + static_cast<long long>(unit_chunk(u)) * b.nTiles;
 	for (int t = threadIdx.x; t < b.nTiles; t += 64) {
 		const uint32_t c = lhist[t];
 		hist[static_cast<long long>(u) * b.nTiles + t] = c;
@@ -257,9 +275,9 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 
 // The histogram is unit-major, hist[u * nTiles + t] = unit u's pairs in tile
 // t, so that each unit writes (and the scatter reads) one contiguous row.
-// The list is tile-major: tile t's pairs from units 0..U-1 in unit order.
-// Columns are cut into chunks of kColChunk units; k_setup_count also adds
-// its nonzero counts into csum[chunk * nTiles + t].  The column scan turns
+// The list is tile-major: tile t's pairs grouped by unit, in chunk order.
+// Columns are cut into chunks of kColChunk units (unit_chunk); k_setup_count
+// also adds its nonzero counts into csum[chunk * nTiles + t].  The column scan turns
 // those chunk sums into within-tile prefixes and writes tileOff[t] = tile
 // t's first list position (tileOff[nTiles] = the total); the down pass
 // turns the counts in place into list positions.  A thread owns one tile
@@ -323,15 +341,17 @@ __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, in
                                                  const uint32_t* __restrict__ tileOff) {
 	const int t = blockIdx.x * 64 + threadIdx.x, uc = blockIdx.y;
 	if (t >= nT) return;
-	const int u0 = uc * kColChunk, u1 = min(U, u0 + kColChunk);
 	uint32_t run = tileOff[t] + csum[static_cast<long long>(uc) * nT + t];
-	for (int b = u0; b < u1; b += kColBatch) {
+	for (int k0 = 0; k0 < kColChunk; k0 += kColBatch) {
 		uint32_t v[kColBatch];
 #pragma unroll
-		for (int j = 0; j < kColBatch; ++j) v[j] = b + j < u1 ? hist[static_cast<long long>(b + j) * nT + t] : 0u;
+		for (int j = 0; j < kColBatch; ++j) {
+			const int u = chunk_unit(uc, k0 + j);
+			v[j] = u < U ? hist[static_cast<long long>(u) * nT + t] : 0u; }
 #pragma unroll
 		for (int j = 0; j < kColBatch; ++j) {
-			if (b + j < u1) hist[static_cast<long long>(b + j) * nT + t] = run;
+			const int u = chunk_unit(uc, k0 + j);
+			if (u < U) hist[static_cast<long long>(u) * nT + t] = run;
 			run += v[j]; }}}
 
 // ---------------------------------------------------------------------------
