@@ -144,7 +144,8 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff, itemTile;
+	uint32_t itemCap = 0;
 	DevArray<unsigned long long> keys;
 	DevArray<VisRec> rec, crec;
 	size_t keysReady = 0;           // pixels of keys holding the sentinel
@@ -194,7 +195,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff, &itemTile }) a->release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
@@ -406,6 +407,11 @@ struct rgl_gpu {
 		if (clistCap < 4096) clistCap = 65536;
 		if ((e = crec.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
 		if ((e = workOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
+		// work items: at most kMaxRegions per segment, and a tile's segments
+		// number at most its pairs / segPairs + 1
+		itemCap = static_cast<uint32_t>(std::min<long long>(
+			static_cast<long long>(kMaxRegions) * (b.nTiles + (static_cast<long long>(listCap) + clistCap) / b.segPairs + 1), 1LL << 30));
+		if ((e = itemTile.ensure(itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
 		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
@@ -550,7 +556,7 @@ struct rgl_gpu {
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -618,7 +624,7 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, dCtr);
+		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
@@ -640,7 +646,7 @@ struct rgl_gpu {
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;
 			// the kernel strides over any remainder
-			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p); }
+			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		if (learned) {

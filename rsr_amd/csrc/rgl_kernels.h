@@ -36,6 +36,7 @@ constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visib
 constexpr int kSplitPairs = 0;       // default pairs per segment from which visibility regions are halved
 constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
 constexpr int kQuadrants = 4;
+constexpr int kMaxRegions = 16;      // visibility regions per tile (quadrants cut in four)
 
 // Tallies that many workgroups add to are split into kCtrSlots slots, one
 // 128-byte line each, and a workgroup adds into slot blockIdx % kCtrSlots:
@@ -304,7 +305,8 @@ __device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t pairs, uint3
 // and cursor (scan of clipHist) and the tile's visibility work items.
 __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
-		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, Counters* __restrict__ ctr) {
+		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, uint32_t* __restrict__ itemTile, uint32_t itemCap,
+		Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
 	const int nT = b.nTiles;
 	uint32_t carry = 0, ccarry = 0, wcarry = 0;
@@ -331,11 +333,13 @@ __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __rest
 		if (t < nT) {
 			tileOff[t] = carry + ex;
 			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
-			workOff[t] = wcarry + wex; }
+			workOff[t] = wcarry + wex;
+			// each work item's tile
+			for (uint32_t j = wcarry + wex, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t); }
 		carry += tot; ccarry += ctot; wcarry += wtot; }
 	if (threadIdx.x == 0) {
 		tileOff[nT] = carry; clipOff[nT] = ccarry; workOff[nT] = wcarry;
-		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = wcarry; ctr->workNext = 0; }}
+		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
 
 __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
                                                  const uint32_t* __restrict__ tileOff) {
@@ -653,7 +657,6 @@ __device__ inline Rect draw_tile_rect(const DrawLite& d, int tx, int ty) {
 	r.x1 = imin(r.x0 + d.tileW, d.fW); r.y1 = imin(r.y0 + d.tileH, d.fH);
 	return r; }
 
-constexpr int kMaxTilesLds = 2048;   // tile offset tables cached in LDS up to this many tiles
 
 struct ScatterArgs {
 	const uint2* trect; const uint4* tvert; const float4* vpos;
@@ -805,32 +808,196 @@ __device__ inline void vis_quad(unsigned long long* keys, const VisSetup& v, int
 		if (dz != dz) continue;
 		atomicMin(&keys[(ly + (l >> 1)) * kSubDim + (lx + (l & 1))], frag_key(dz, v.slot, leq)); }}
 
+// one lane of a quad of a triangle setup: perspective-correct barycentrics
+// BP and depth (TriangleProgram::Render, rglv_gpu_impl.hxx:169-191)
+struct LaneBary { float bp0, bp1, bp2, depth; };
+
+__device__ inline LaneBary lane_bary(const TriSetup& t, const TriVerts& tr, int32_t e1, int32_t e2, const uint32_t* rcpTab) {
+	LaneBary r;
+	const float b0 = itof(e2) * t.scale;
+	const float b2 = itof(e1) * t.scale;
+	const float b1 = (1.0F - b0) - b2;
+	r.depth = (b0*tr.z[0] + b1*tr.z[1]) + b2*tr.z[2];
+	const float fragW = oneover_tab((b0*tr.w[0] + b1*tr.w[1]) + b2*tr.w[2], rcpTab);
+	r.bp0 = (tr.w[0] * b0) * fragW;
+	r.bp2 = (tr.w[2] * b2) * fragW;
+	r.bp1 = (1.0F - r.bp0) - r.bp2;
+	return r; }
+
+__device__ inline float lane_interp(const LaneBary& b, const float* v0, const float* v1, const float* v2, int k) {
+	return (b.bp0*v0[k] + b.bp1*v1[k]) + b.bp2*v2[k]; }
+
+// shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle
+template <class P>
+__device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* __restrict__ vary,
+                                   int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
+	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
+	// lane offsets of the rasteriser's quad: (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
+	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
+	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
+	const float* v0 = vary + tr.vary[0] * kMaxVary;
+	const float* v1 = vary + tr.vary[1] * kMaxVary;
+	const float* v2 = vary + tr.vary[2] * kMaxVary;
+	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
+	depth = own.depth;
+	float vv[P::kVary];
+#pragma unroll
+	for (int k = 0; k < P::kVary; ++k) vv[k] = lane_interp(own, v0, v1, v2, k);
+	int lod = 0;
+	if constexpr (P::kUV >= 0) {
+		// the quad's LOD comes from the texture coordinates of lanes 0 and 1
+		const LaneBary l0 = lane_bary(t, tr, q1, q2, rcpTab);
+		const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
+		lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
+		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
+	P::fragment1(d, vv, lod, color); }
+
+// resolve shaders: one program known at compile time, or a switch over
+// the installed programs for frames that mix them
+template <class P>
+struct ShadeOne {
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+		shade_pixel<P>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); }};
+
+struct ShadeAny {
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+		switch (d.program) {
+		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
+		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}};
+
+// ---------------------------------------------------------------------------
+// resolve: one thread per pixel.  The pixel's winning slot is shaded by
+// recomputing its triangle's quad exactly as the reference's rasteriser
+// produced it in the pixel's tile (tile-clamped edge start, barycentric
+// scale, texture LOD from lanes 0 and 1), then post-shaded and
+// sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104).  Colour, depth
+// and primitive id are written once; the key is reset for the next frame.
+// ---------------------------------------------------------------------------
+constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
+
+// the winner of pixel (x, y), shaded: colour before the post program,
+// depth and primitive id (clear values where nothing was drawn)
+template <class S>
+__device__ inline bool resolve_key(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+                                   unsigned long long k, const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
+	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
+	dep = f.clearDepth;
+	prim = 0xffffffffu;
+	if (!(k < clear_key(f))) return false;
+	const uint32_t slot = key_slot(k, f.depthFunc == 1);
+	const int di = find_draw_slot(dl, b.nDraws, slot);
+	const DrawLite& d = dl[di];
+	const uint32_t rel = slot - d.slotBase;
+	const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
+	const int tx = x / f.tileW, ty = y / f.tileH;
+	Rect R;
+	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
+	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
+	TriVerts tr;
+	TriSetup t;
+	uint32_t srcTri;
+	const float* vary;
+	if (rel < nTri) {
+		srcTri = rel;
+		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
+		setup_vector(R, tr.X, tr.Y, t);
+		vary = a.vvary; }
+	else {
+		const uint32_t fslot = rel - nTri;
+		srcTri = fslot / kFanMax;
+		const uint32_t j = fslot - srcTri * kFanMax;
+		const uint32_t cq = a.clipqOf[d.triBase + srcTri];
+		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+		setup_scalar(R, tr.X, tr.Y, t);
+		vary = a.cvvary; }
+	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
+	const int lane = (x & 1) | ((y & 1) << 1);
+	S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+	prim = d.primBase + srcTri;
+	return true; }
+
+template <class S>
+__device__ inline bool resolve_pixel(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+                                     const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
+	return resolve_key<S>(b, f, a, dl, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], rcpTab, col, dep, prim); }
+
+// StoreTrueColor's ShadeCanvas + sRGB encode of one pixel (FilterTile,
+// rglr_algorithm.hxx:31-104)
+__device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], const uint32_t* srgbTab) {
+	float r = col[0], g = col[1], b = col[2];
+	if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
+	return f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
+	               : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+
+// resolve one pixel from its key and store colour, depth and primitive id
+template <class S>
+__device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
+                                     unsigned long long k, const uint32_t* rcpTab, const uint32_t* srgbTab) {
+	float col[3], dep;
+	uint32_t prim;
+	resolve_key<S>(b, f, a, dl, x, y, k, rcpTab, col, dep, prim);
+	const long long o = static_cast<long long>(y) * f.W + x;
+	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
+	if (f.outDepth) f.outDepth[o] = dep;
+	if (f.outPrim) f.outPrim[o] = prim; }
+
+// a resolve block's frame and pixel: blocks are numbered across the batch,
+// each frame's blocks row-major over kResolveW x kResolveH pixel strips
+struct BlockPixel { int frame, x, y; };
+
+__device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
+	BlockPixel r;
+	r.frame = find_base(fbb, b.nFrames, static_cast<int>(blockIdx.x));
+	const FrameDesc& f = b.frames[r.frame];
+	const int lb = static_cast<int>(blockIdx.x) - f.blockBase;
+	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
+	r.x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
+	r.y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
+	return r; }
+
+template <class S>
+__global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
+	extern __shared__ uint4 dyn_lds[];   // the batch's draws
+	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
+	__shared__ uint32_t rcpTab[2048];
+	__shared__ uint32_t srgbTab[104];
+	__shared__ int fbb[kMaxFrames + 1];
+	load_lite(dl, a.draws, b.nDraws);
+	load_block_bases(fbb, b);
+	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
+	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
+	__syncthreads();
+	const BlockPixel bp = block_pixel(b, fbb);
+	const FrameDesc& f = b.frames[bp.frame];
+	const int x = bp.x, y = bp.y;
+	if (x >= f.W || y >= f.H) return;
+	resolve_store<S>(b, f, a, dl, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], rcpTab, srgbTab); }
+
 // visibility work items run one per workgroup; the grid is sized from the
 // previous frame's item count and strides over any remainder, so no
 // work-queue counter is shared between workgroups
 __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs a, const VisRec* __restrict__ rec,
-                                                       const VisRec* __restrict__ crec) {
+                                                       const VisRec* __restrict__ crec, const uint32_t* __restrict__ itemTile) {
 	__shared__ unsigned long long keys[kSubDim * kSubDim];
 	__shared__ VisSetup ts[kRasterThreads];
 	__shared__ uint32_t qoff[kRasterThreads + 1];
 	__shared__ uint32_t scan_sh[kRasterThreads / 64 + 1];
-	__shared__ uint32_t wofs[kMaxTilesLds + 1];
 	__shared__ uint8_t owner[kQuadWindow];
 	__shared__ int ftb[kMaxFrames + 1];
 	load_tile_bases(ftb, b);
-	const bool lds = b.nTiles <= kMaxTilesLds;
-	if (lds) for (int t = threadIdx.x; t <= b.nTiles; t += kRasterThreads) wofs[t] = a.workOff[t];
 	uint32_t frags = 0;
 	const uint32_t nItems = a.ctr->workItems;
 	for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
 		__syncthreads();
-		int lo = 0, hi = b.nTiles - 1;  // last tile with workOff <= item
-		while (lo < hi) {
-			const int mid = (lo + hi + 1) >> 1;
-			if ((lds ? wofs[mid] : a.workOff[mid]) <= item) lo = mid; else hi = mid - 1; }
-		const int tile = lo;
-		const uint32_t w0 = lds ? wofs[tile] : a.workOff[tile];
-		const uint32_t w1 = lds ? wofs[tile + 1] : a.workOff[tile + 1];
+		const int tile = static_cast<int>(itemTile[item]);
+		const uint32_t w0 = a.workOff[tile];
+		const uint32_t w1 = a.workOff[tile + 1];
 		const uint32_t local = item - w0;
 		const uint32_t nseg = vis_segments(b, tile_pairs(a.tileOff, a.clipOff, tile));
 		const uint32_t nreg = (w1 - w0) / nseg;
@@ -934,7 +1101,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 					const int32_t q3 = step_mad(step_mad(w.c3, kx, shl(w.dy31, 1), small), ky, shl(w.dx31, 1), small);
 					vis_quad(keys, w, q1, q2, q3, w.x0 + 2 * kx, w.y0 + 2 * ky, leq, frags); }}
 			__syncthreads(); }
-		// merge the quadrant into the frame's key buffer
+		// merge the region into the frame's key buffer
 		const int rw = S.x1 - S.x0, rh = S.y1 - S.y0;
 		for (int i = threadIdx.x; i < rw * rh; i += kRasterThreads) {
 			const int ly = i / rw, lx = i - ly * rw;
@@ -950,167 +1117,6 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		unsigned long long sum = 0;
 		for (int w = 0; w < kRasterThreads / 64; ++w) sum += scan_sh[w];
 		if (sum) atomicAdd(&a.ctr->slot[blockIdx.x % kCtrSlots].fragments, sum); }}
-
-// one lane of a quad of a triangle setup: perspective-correct barycentrics
-// BP and depth (TriangleProgram::Render, rglv_gpu_impl.hxx:169-191)
-struct LaneBary { float bp0, bp1, bp2, depth; };
-
-__device__ inline LaneBary lane_bary(const TriSetup& t, const TriVerts& tr, int32_t e1, int32_t e2, const uint32_t* rcpTab) {
-	LaneBary r;
-	const float b0 = itof(e2) * t.scale;
-	const float b2 = itof(e1) * t.scale;
-	const float b1 = (1.0F - b0) - b2;
-	r.depth = (b0*tr.z[0] + b1*tr.z[1]) + b2*tr.z[2];
-	const float fragW = oneover_tab((b0*tr.w[0] + b1*tr.w[1]) + b2*tr.w[2], rcpTab);
-	r.bp0 = (tr.w[0] * b0) * fragW;
-	r.bp2 = (tr.w[2] * b2) * fragW;
-	r.bp1 = (1.0F - r.bp0) - r.bp2;
-	return r; }
-
-__device__ inline float lane_interp(const LaneBary& b, const float* v0, const float* v1, const float* v2, int k) {
-	return (b.bp0*v0[k] + b.bp1*v1[k]) + b.bp2*v2[k]; }
-
-// shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle
-template <class P>
-__device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* __restrict__ vary,
-                                   int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
-	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
-	// lane offsets of the rasteriser's quad: (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
-	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
-	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
-	const float* v0 = vary + tr.vary[0] * kMaxVary;
-	const float* v1 = vary + tr.vary[1] * kMaxVary;
-	const float* v2 = vary + tr.vary[2] * kMaxVary;
-	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
-	depth = own.depth;
-	float vv[P::kVary];
-#pragma unroll
-	for (int k = 0; k < P::kVary; ++k) vv[k] = lane_interp(own, v0, v1, v2, k);
-	int lod = 0;
-	if constexpr (P::kUV >= 0) {
-		// the quad's LOD comes from the texture coordinates of lanes 0 and 1
-		const LaneBary l0 = lane_bary(t, tr, q1, q2, rcpTab);
-		const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
-		lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
-		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
-	P::fragment1(d, vv, lod, color); }
-
-// resolve shaders: one program known at compile time, or a switch over
-// the installed programs for frames that mix them
-template <class P>
-struct ShadeOne {
-	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
-	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-		shade_pixel<P>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); }};
-
-struct ShadeAny {
-	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
-	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-		switch (d.program) {
-		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}};
-
-// ---------------------------------------------------------------------------
-// resolve: one thread per pixel.  The pixel's winning slot is shaded by
-// recomputing its triangle's quad exactly as the reference's rasteriser
-// produced it in the pixel's tile (tile-clamped edge start, barycentric
-// scale, texture LOD from lanes 0 and 1), then post-shaded and
-// sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104).  Colour, depth
-// and primitive id are written once; the key is reset for the next frame.
-// ---------------------------------------------------------------------------
-constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
-
-// the winner of pixel (x, y), shaded: colour before the post program,
-// depth and primitive id (clear values where nothing was drawn)
-template <class S>
-__device__ inline bool resolve_pixel(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
-                                     const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
-	const unsigned long long k = a.keys[f.keyBase + static_cast<long long>(y) * f.W + x];
-	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
-	dep = f.clearDepth;
-	prim = 0xffffffffu;
-	if (!(k < clear_key(f))) return false;
-	const uint32_t slot = key_slot(k, f.depthFunc == 1);
-	const int di = find_draw_slot(dl, b.nDraws, slot);
-	const DrawLite& d = dl[di];
-	const uint32_t rel = slot - d.slotBase;
-	const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
-	const int tx = x / f.tileW, ty = y / f.tileH;
-	Rect R;
-	R.x0 = tx * f.tileW; R.y0 = ty * f.tileH;
-	R.x1 = imin(R.x0 + f.tileW, f.W); R.y1 = imin(R.y0 + f.tileH, f.H);
-	TriVerts tr;
-	TriSetup t;
-	uint32_t srcTri;
-	const float* vary;
-	if (rel < nTri) {
-		srcTri = rel;
-		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
-		setup_vector(R, tr.X, tr.Y, t);
-		vary = a.vvary; }
-	else {
-		const uint32_t fslot = rel - nTri;
-		srcTri = fslot / kFanMax;
-		const uint32_t j = fslot - srcTri * kFanMax;
-		const uint32_t cq = a.clipqOf[d.triBase + srcTri];
-		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
-		setup_scalar(R, tr.X, tr.Y, t);
-		vary = a.cvvary; }
-	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
-	const int lane = (x & 1) | ((y & 1) << 1);
-	S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
-	prim = d.primBase + srcTri;
-	return true; }
-
-// StoreTrueColor's ShadeCanvas + sRGB encode of one pixel (FilterTile,
-// rglr_algorithm.hxx:31-104)
-__device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], const uint32_t* srgbTab) {
-	float r = col[0], g = col[1], b = col[2];
-	if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
-	return f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
-	               : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
-
-// a resolve block's frame and pixel: blocks are numbered across the batch,
-// each frame's blocks row-major over kResolveW x kResolveH pixel strips
-struct BlockPixel { int frame, x, y; };
-
-__device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
-	BlockPixel r;
-	r.frame = find_base(fbb, b.nFrames, static_cast<int>(blockIdx.x));
-	const FrameDesc& f = b.frames[r.frame];
-	const int lb = static_cast<int>(blockIdx.x) - f.blockBase;
-	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
-	r.x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
-	r.y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
-	return r; }
-
-template <class S>
-__global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
-	extern __shared__ uint4 dyn_lds[];   // the batch's draws
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ uint32_t rcpTab[2048];
-	__shared__ uint32_t srgbTab[104];
-	__shared__ int fbb[kMaxFrames + 1];
-	load_lite(dl, a.draws, b.nDraws);
-	load_block_bases(fbb, b);
-	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
-	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
-	__syncthreads();
-	const BlockPixel bp = block_pixel(b, fbb);
-	const FrameDesc& f = b.frames[bp.frame];
-	const int x = bp.x, y = bp.y;
-	if (x >= f.W || y >= f.H) return;
-	const long long o = static_cast<long long>(y) * f.W + x;
-	float col[3], dep;
-	uint32_t prim;
-	resolve_pixel<S>(b, f, a, dl, x, y, rcpTab, col, dep, prim);
-	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
-	if (f.outDepth) f.outDepth[o] = dep;
-	if (f.outPrim) f.outPrim[o] = prim; }
 
 __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
 	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
