@@ -612,7 +612,7 @@ struct rgl_gpu {
 			if (present[4]) k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p); }
 		Mark(1);
 		if (b.units > 0) {
-			k_setup_count<<<b.units, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
+			k_setup_count<<<b.units, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
 				hist.p, bsum.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
 		// clipping: the clip queue is complete after setup; the clipped
@@ -641,7 +641,7 @@ struct rgl_gpu {
 			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
 			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
 			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
-			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, 64, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, sa); }
+			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, sa); }
 		Mark(5);
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;

@@ -19,7 +19,8 @@
 
 namespace rgl {
 
-constexpr int kUnitTris = 256;      // triangles per binning unit (one wave), the minimum
+constexpr int kUnitTris = 256;
+constexpr int kUnitThreads = 256;   // threads per binning-unit workgroup (setup/count and scatter)      // triangles per binning unit (one wave), the minimum
 constexpr long long kHistBudget = 8LL << 20;   // units x tiles histogram entries before units grow
 constexpr int kColChunk = 32;       // binning units per column-scan chunk
 constexpr int kXcds = 8;             // workgroups are dealt round-robin over the eight XCDs
@@ -158,7 +159,7 @@ __device__ inline uint2 pack_rect(int tx0, int ty0, int tx1, int ty1) {
 	                  static_cast<uint32_t>(ty0) | (static_cast<uint32_t>(ty1) << 16)); }
 __device__ inline const uint2 empty_rect() { return make_uint2(0x0000ffffu, 0x0000ffffu); }
 
-__global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__ draws, BatchDesc b,
+__global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __restrict__ draws, BatchDesc b,
 		const float4* __restrict__ vpos, const uint8_t* __restrict__ vflags,
 		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
 		uint32_t* __restrict__ clipq, uint32_t* __restrict__ clipqOf, uint32_t clipCap,
@@ -169,12 +170,12 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 	uint32_t* lhist = reinterpret_cast<uint32_t*>(dl + b.nDraws);
 	const int u = blockIdx.x;
 	load_lite(dl, draws, b.nDraws);
-	for (int t = threadIdx.x; t < b.nTiles; t += 64) lhist[t] = 0;
+	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) lhist[t] = 0;
 	__syncthreads();
 	uint32_t binned = 0;
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
-	for (int r = 0; r < b.unitTris / 64; ++r) {
-		const long long s = s0 + r * 64 + threadIdx.x;
+	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
+		const long long s = s0 + r;
 		if (s >= b.totalTris) break;
 		const int di = find_draw_ll(dl, b.nDraws, s);
 		const DrawLite& d = dl[di];
@@ -232,23 +233,24 @@ __global__ __launch_bounds__(64) void k_setup_count(const DrawDesc* __restrict__
 		// from one atomic
 		const unsigned long long cm = __ballot(clips);
 		if (cm) {
+			const unsigned lane = threadIdx.x & 63u;
+			const int leader = __ffsll(static_cast<long long>(cm)) - 1;
 			uint32_t q0 = 0;
-			if (threadIdx.x == static_cast<unsigned>(__ffsll(static_cast<long long>(cm)) - 1))
-				q0 = atomicAdd(&ctr->clipCount, static_cast<uint32_t>(__popcll(cm)));
-			q0 = __shfl(q0, __ffsll(static_cast<long long>(cm)) - 1, 64);
+			if (lane == static_cast<unsigned>(leader)) q0 = atomicAdd(&ctr->clipCount, static_cast<uint32_t>(__popcll(cm)));
+			q0 = __shfl(q0, leader, 64);
 			if (clips) {
-				const uint32_t q = q0 + static_cast<uint32_t>(__popcll(cm & ((1ull << threadIdx.x) - 1ull)));
+				const uint32_t q = q0 + static_cast<uint32_t>(__popcll(cm & ((1ull << lane) - 1ull)));
 				if (q < clipCap) { clipq[q] = static_cast<uint32_t>(s); clipqOf[s] = q; }
 				else atomicOr(&ctr->overflow, 2u); }}}
 	__syncthreads();
 	uint32_t* cs = csum + static_cast<long long>(unit_chunk(u)) * b.nTiles;
-	for (int t = threadIdx.x; t < b.nTiles; t += 64) {
+	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) {
 		const uint32_t c = lhist[t];
 		hist[static_cast<long long>(u) * b.nTiles + t] = c;
 		if (c) atomicAdd(&cs[t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
-	if (threadIdx.x == 0 && binned) atomicAdd(&ctr->slot[blockIdx.x % kCtrSlots].binned, binned); }
+	if ((threadIdx.x & 63) == 0 && binned) atomicAdd(&ctr->slot[blockIdx.x % kCtrSlots].binned, binned); }
 
 // ---------------------------------------------------------------------------
 // exclusive scan of uint32 (three passes); out may alias in.
@@ -711,7 +713,7 @@ __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, in
 // triangle's tiles, takes a list position from the unit's LDS cursor of
 // the tile and writes the pair's setup record.  Blocks [units, units +
 // clipBlocks): the clipped fans.  The rest: key resets.
-__global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, ScatterArgs a) {
+__global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, ScatterArgs a) {
 	// dynamic LDS: the batch's draws, then nTiles write cursors
 	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
@@ -728,12 +730,12 @@ __global__ __launch_bounds__(64) void k_scatter(BatchDesc b, const DrawDesc* __r
 		__syncthreads();
 		clip_scatter(b, dl, u - b.units, a);
 		return; }
-	for (int t = threadIdx.x; t < b.nTiles; t += 64) base[t] = a.off[static_cast<long long>(u) * b.nTiles + t];
+	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) base[t] = a.off[static_cast<long long>(u) * b.nTiles + t];
 	__syncthreads();
 	bool ovf = false;
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
-	for (int r = 0; r < b.unitTris / 64; ++r) {
-		const long long s = s0 + r * 64 + threadIdx.x;
+	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
+		const long long s = s0 + r;
 		if (s >= b.totalTris) break;
 		const uint2 rc = a.trect[s];
 		const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
