@@ -954,10 +954,11 @@ __device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, con
 struct BlockPixel { int frame, x, y; };
 
 __device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
+	const int blk = static_cast<int>(blockIdx.x);
 	BlockPixel r;
-	r.frame = find_base(fbb, b.nFrames, static_cast<int>(blockIdx.x));
+	r.frame = find_base(fbb, b.nFrames, blk);
 	const FrameDesc& f = b.frames[r.frame];
-	const int lb = static_cast<int>(blockIdx.x) - f.blockBase;
+	const int lb = blk - f.blockBase;
 	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
 	r.x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
 	r.y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
@@ -967,14 +968,15 @@ template <class S>
 __global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ uint32_t rcpTab[2048];
 	__shared__ uint32_t srgbTab[104];
 	__shared__ int fbb[kMaxFrames + 1];
 	load_lite(dl, a.draws, b.nDraws);
 	load_block_bases(fbb, b);
-	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
 	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
+	// the rcpps table is read through the vector L1 rather than copied into
+	// every workgroup's LDS
+	const uint32_t* rcpTab = kRcpMantissa;
 	const BlockPixel bp = block_pixel(b, fbb);
 	const FrameDesc& f = b.frames[bp.frame];
 	const int x = bp.x, y = bp.y;
