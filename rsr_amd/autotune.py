@@ -1,5 +1,5 @@
 """Autotuner for batched frames: picks the tile size, the number of
-(triangle, tile) pairs per visibility work item and the pair count from
+(triangle, tile) pairs per visibility work item and the quad work from
 which a work item's region is cut smaller (together: how many workgroups
 share a dense tile) by timing the batch itself.
 
@@ -15,7 +15,7 @@ import time
 
 TILE_CHOICES = ((8, 8), (4, 4))          # 64 and 32 pixel tiles (tile_blocks x 8)
 SEG_CHOICES = (256, 512, 1024)
-SPLIT_CHOICES = (0, 64, 128, 256)
+SPLIT_CHOICES = (0, 512, 1024, 2048)
 
 
 def _record(ex_frames, scene_list, device, tile_blocks):
@@ -61,7 +61,7 @@ def autotune(ex, scene_list, device="cuda:0", steps: int = 10, tiles=TILE_CHOICE
             for split in splits:
                 ex.set_raster_params(seg, split)
                 ms = time_batch(ex, frames, stream, steps)
-                table.append({"tile_px": tb[0] * 8, "seg_pairs": seg, "split_pairs": split, "ms": round(ms, 4)})
+                table.append({"tile_px": tb[0] * 8, "seg_pairs": seg, "split_quads": split, "ms": round(ms, 4)})
                 if log:
                     log(table[-1])
                 if best is None or ms < best[0]:
@@ -70,4 +70,4 @@ def autotune(ex, scene_list, device="cuda:0", steps: int = 10, tiles=TILE_CHOICE
     if frames is None or tb != tiles[-1]:
         frames = _record(frames, scene_list, device, tb)
     ex.set_raster_params(seg, split)
-    return frames, {"tile_px": tb[0] * 8, "seg_pairs": seg, "split_pairs": split}, table
+    return frames, {"tile_px": tb[0] * 8, "seg_pairs": seg, "split_quads": split}, table

@@ -193,7 +193,7 @@ struct BatchDesc {
 	long long totalTris;     // S
 	int nBlocks;             // resolve blocks, all frames
 	int segPairs;            // (triangle, tile) pairs per visibility work item
-	int splitPairs;          // pairs per segment from which quadrants are cut in 2, and from twice this in 4 (0: never)
+	int splitQuads;          // quad work per quadrant from which it is cut in 2, and from twice this in 4 (0: never)
 	const FrameDesc* frames; // device table, nFrames entries
 };
 

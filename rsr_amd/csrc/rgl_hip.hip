@@ -144,7 +144,8 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff, itemTile;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff, itemTile, tileCost;
+	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
 	DevArray<unsigned long long> keys;
 	DevArray<VisRec> rec, crec;
@@ -157,7 +158,7 @@ struct rgl_gpu {
 	DevArray<FrameDesc> dFrames;
 	std::vector<uint8_t> lastFrames;
 	int segPairs = kSegPairs;       // (triangle, tile) pairs per visibility work item
-	int splitPairs = kSplitPairs;   // pairs per segment from which visibility regions are halved
+	int splitQuads = kSplitQuads;   // quad work per quadrant from which visibility regions are halved
 	// graph replay of the launch sequence (rgl_gpu_set_graph)
 	bool graphMode = false, graphValid = false;
 	uint64_t graphKey = 0;
@@ -195,7 +196,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff, &itemTile }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff, &itemTile, &tileCost }) a->release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
@@ -412,6 +413,7 @@ struct rgl_gpu {
 		itemCap = static_cast<uint32_t>(std::min<long long>(
 			static_cast<long long>(kMaxRegions) * (b.nTiles + (static_cast<long long>(listCap) + clistCap) / b.segPairs + 1), 1LL << 30));
 		if ((e = itemTile.ensure(itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
+		if ((e = tileCost.ensure(b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
 		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
@@ -485,7 +487,7 @@ struct rgl_gpu {
 			ut = ((triBase * tileBase / kHistBudget + 1 + 63) / 64) * 64;
 		b.totalTris = triBase; b.unitTris = static_cast<int>(std::min<long long>(ut, 1 << 20));
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
-		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitPairs = splitPairs; b.frames = nullptr;
+		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
 		int r = Ensure(b, vtot, keyBase);
@@ -508,6 +510,9 @@ struct rgl_gpu {
 			evNext = (evNext + 1) % kRing; }
 		if ((r = UploadTable(draws, lastTable, hDraws, dDraws.p)) != 0) return r;
 		if ((r = UploadTable(frames, lastFrames, hFrames, dFrames.p)) != 0) return r;
+		if (costReady < tileCost.cap) {
+			k_fill_u32<<<8, 256, 0, stream>>>(tileCost.p, static_cast<long long>(tileCost.cap), 0u);
+			costReady = tileCost.cap; }
 		if (keysReady < keys.cap) {
 			// defined contents after allocation; each frame then resets the
 			// keys of multi-segment tiles in k_pair_setup and overwrites the rest
@@ -556,7 +561,7 @@ struct rgl_gpu {
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -624,7 +629,7 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, dCtr);
+		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileCost.p, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
@@ -634,7 +639,7 @@ struct rgl_gpu {
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
-		ra.keys = keys.p; ra.workOff = workOff.p;
+		ra.keys = keys.p; ra.workOff = workOff.p; ra.tileCost = tileCost.p;
 		if (b.units > 0) {
 			ScatterArgs sa;
 			sa.trect = trect.p; sa.tvert = tvert.p; sa.vpos = vpos.p; sa.off = hist.p; sa.rec = rec.p; sa.listCap = listCap;
@@ -979,12 +984,12 @@ int rgl_gpu_set_guard_band(rgl_gpu* g, int mode) {
 	g->fixedGuardBand = mode == 1;
 	return 0; }
 
-int rgl_gpu_set_raster_params(rgl_gpu* g, int seg_pairs, int split_pairs) {
+int rgl_gpu_set_raster_params(rgl_gpu* g, int seg_pairs, int split_quads) {
 	if (check(g)) return RGL_E_ARG;
 	if (seg_pairs < 64 || seg_pairs > 4096) return g->fail(RGL_E_ARG, "seg_pairs must be in [64, 4096]");
-	if (split_pairs < 0 || split_pairs > 4096) return g->fail(RGL_E_ARG, "split_pairs must be in [0, 4096]");
+	if (split_quads < 0 || split_quads > (1 << 20)) return g->fail(RGL_E_ARG, "split_quads must be in [0, 2^20]");
 	g->segPairs = seg_pairs;
-	g->splitPairs = split_pairs;
+	g->splitQuads = split_quads;
 	g->graphValid = false;
 	return 0; }
 

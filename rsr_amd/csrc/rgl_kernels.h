@@ -34,7 +34,7 @@ __host__ __device__ inline int chunk_unit(int c, int k) { return (c % kXcds) + k
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
 constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
-constexpr int kSplitPairs = 0;       // default pairs per segment from which visibility regions are halved
+constexpr int kSplitQuads = 0;       // default quad work per quadrant from which visibility regions are halved
 constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels
 constexpr int kQuadrants = 4;
 constexpr int kMaxRegions = 16;      // visibility regions per tile (quadrants cut in four)
@@ -289,18 +289,19 @@ constexpr int kColBatch = 16;   // independent loads in flight per thread
 
 // Visibility work items of a tile: its inline + clipped pairs are cut into
 // segments of segPairs, and the tile into regions: the four 32 x 32
-// quadrants, each cut in two (32 x 16) when a segment holds splitPairs
-// pairs or more and in four (16 x 16) from twice that.  A work item is
-// (tile, segment, region); the scan and the visibility kernel both derive
-// the counts from the tile's pair total.
+// quadrants, each cut in two (32 x 16) when its share of the tile's quad
+// work exceeds splitQuads and in four (16 x 16) above twice that.  The quad
+// work is the previous frame's (tileCost, accumulated by k_vis); every
+// split draws the same frame.  A work item is (tile, segment, region); the
+// visibility kernel recovers the region count from the tile's item count.
 __device__ inline uint32_t vis_segments(const BatchDesc& b, uint32_t pairs) {
 	const uint32_t sp = static_cast<uint32_t>(b.segPairs);
 	return max(1u, (pairs + sp - 1) / sp); }
 
-__device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t pairs, uint32_t nseg) {
-	if (b.splitPairs <= 0) return kQuadrants;
-	const uint32_t pps = (pairs + nseg - 1) / nseg, s = static_cast<uint32_t>(b.splitPairs);
-	return pps >= 2 * s ? 4 * kQuadrants : pps >= s ? 2 * kQuadrants : kQuadrants; }
+__device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t cost, uint32_t nseg) {
+	if (b.splitQuads <= 0) return kQuadrants;
+	const uint32_t per = cost / (nseg * kQuadrants), s = static_cast<uint32_t>(b.splitQuads);
+	return per > 2 * s ? 4 * kQuadrants : per > s ? 2 * kQuadrants : kQuadrants; }
 
 // One block: chunk prefixes within each column, then per tile the list
 // offset (exclusive scan of the column totals), the clipped-list offset
@@ -308,7 +309,7 @@ __device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t pairs, uint3
 __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
 		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, uint32_t* __restrict__ itemTile, uint32_t itemCap,
-		Counters* __restrict__ ctr) {
+		uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[1024 / 64 + 1];
 	const int nT = b.nTiles;
 	uint32_t carry = 0, ccarry = 0, wcarry = 0;
@@ -327,7 +328,8 @@ __global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __rest
 					col += w[j]; }}
 			v = clipHist[t];
 			const uint32_t nseg = vis_segments(b, col + v);
-			items = nseg * vis_regions(b, col + v, nseg); }
+			items = nseg * vis_regions(b, tileCost[t], nseg);
+			tileCost[t] = 0u; }
 		uint32_t tot, ctot, wtot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
@@ -600,6 +602,7 @@ struct RasterArgs {
 	Counters* ctr;
 	unsigned long long* keys;   // per-pixel visibility key (smallest (depth, slot) of the frame)
 	const uint32_t* workOff;    // per-tile first work item [nTiles + 1]
+	uint32_t* tileCost;         // per-tile quad work, for the next frame's region split
 };
 
 constexpr unsigned long long kKeySentinel = ~0ull;
@@ -778,10 +781,6 @@ struct alignas(16) VisSetup {
 };
 static_assert(sizeof(VisSetup) % 16 == 0, "VisSetup rows are read with aligned 16-byte loads");
 
-// a + k * s with 32-bit wrap; k < 2^8 here, so with |s| < 2^23 the
-// product fits the 24-bit multiplier exactly
-__device__ inline int32_t step_mad(int32_t a, int32_t k, int32_t s, bool small) {
-	return small ? wadd(a, __mul24(k, s)) : wadd(a, wmul(k, s)); }
 
 constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per pass
 constexpr int kThreadQuads = 2;     // triangles with at most this many quads in the quadrant are walked by their own thread
@@ -801,14 +800,13 @@ __device__ inline void vis_quad(unsigned long long* keys, const VisSetup& v, int
 		const int32_t e1 = (l & 2) ? wadd((l & 1) ? wadd(q1, v.dx12) : q1, (l & 1) ? v.dy12 : v.dx12) : ((l & 1) ? wadd(q1, v.dy12) : q1);
 		const int32_t e2 = (l & 2) ? wadd((l & 1) ? wadd(q2, v.dx23) : q2, (l & 1) ? v.dy23 : v.dx23) : ((l & 1) ? wadd(q2, v.dy23) : q2);
 		const int32_t e3 = (l & 2) ? wadd((l & 1) ? wadd(q3, v.dx31) : q3, (l & 1) ? v.dy31 : v.dx31) : ((l & 1) ? wadd(q3, v.dy31) : q3);
-		if ((e1 | e2 | e3) < 0) continue;
-		++frags;
+		const bool cov = (e1 | e2 | e3) >= 0;
+		frags += cov ? 1u : 0u;
 		const float bx = itof(e2) * v.scale;
 		const float bz = itof(e1) * v.scale;
 		const float by = (1.0F - bx) - bz;
 		const float dz = (bx*v.z0 + by*v.z1) + bz*v.z2;
-		if (dz != dz) continue;
-		atomicMin(&keys[(ly + (l >> 1)) * kSubDim + (lx + (l & 1))], frag_key(dz, v.slot, leq)); }}
+		if (cov && dz == dz) atomicMin(&keys[(ly + (l >> 1)) * kSubDim + (lx + (l & 1))], frag_key(dz, v.slot, leq)); }}
 
 // one lane of a quad of a triangle setup: perspective-correct barycentrics
 // BP and depth (TriangleProgram::Render, rglv_gpu_impl.hxx:169-191)
@@ -1025,6 +1023,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		const uint32_t ce = min(a.clipOff[tile + 1], a.clistCap);
 		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
 		const uint32_t pbeg = seg * static_cast<uint32_t>(b.segPairs), pend = min(pbeg + static_cast<uint32_t>(b.segPairs), ntot);
+		uint32_t itemQuads = 0;   // quads dealt out over the workgroup: the item's cost for the next frame's split
 		__syncthreads();
 		// each thread holds one pair's record; the next batch's record is
 		// loaded while this batch rasterises
@@ -1079,9 +1078,14 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 						q1 = wadd(q1, sx1); q2 = wadd(q2, sx2); q3 = wadd(q3, sx3); }}}
 			// larger triangles: their quads are dealt out over the block
 			if (coop) ts[threadIdx.x] = v;
+			// 24-bit step products are exact when every dealt-out triangle's
+			// steps fit 23 bits (nearly always); decided for the whole batch
+			// so that the quad loop does not diverge on it
+			const bool allSmall = __syncthreads_and(!coop || v.small != 0u) != 0;
 			const uint32_t cq = coop ? nq : 0u;
 			uint32_t total;
 			const uint32_t ex = block_exclusive_scan(cq, scan_sh, total);
+			itemQuads += total;
 			qoff[threadIdx.x] = ex;
 			for (uint32_t win0 = 0; win0 < total; win0 += kQuadWindow) {
 				// owner table: which triangle each quad item of the window belongs to
@@ -1094,15 +1098,25 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 				const uint32_t wend = min(total, win0 + kQuadWindow);
 				for (uint32_t it = win0 + threadIdx.x; it < wend; it += kRasterThreads) {
 					const int j = owner[it - win0];
-					const VisSetup& w = ts[j];
+					// the owner's setup row, read whole into registers
+					VisSetup w;
+					const uint4* src = reinterpret_cast<const uint4*>(&ts[j]);
+					uint4* dstw = reinterpret_cast<uint4*>(&w);
+#pragma unroll
+					for (int c = 0; c < static_cast<int>(sizeof(VisSetup) / 16); ++c) dstw[c] = src[c];
 					const uint32_t k = it - qoff[j];
 					const uint32_t kyr = static_cast<uint32_t>((static_cast<float>(k) + 0.5F) * w.invNx);
-					const uint32_t kxr = k - kyr * w.nx;
-					const bool small = w.small != 0u;
+					const uint32_t kxr = k - __umul24(kyr, w.nx);
 					const int32_t kx = static_cast<int32_t>(kxr), ky = static_cast<int32_t>(kyr);
-					const int32_t q1 = step_mad(step_mad(w.c1, kx, shl(w.dy12, 1), small), ky, shl(w.dx12, 1), small);
-					const int32_t q2 = step_mad(step_mad(w.c2, kx, shl(w.dy23, 1), small), ky, shl(w.dx23, 1), small);
-					const int32_t q3 = step_mad(step_mad(w.c3, kx, shl(w.dy31, 1), small), ky, shl(w.dx31, 1), small);
+					int32_t q1, q2, q3;
+					if (allSmall) {
+						q1 = wadd(wadd(w.c1, __mul24(kx, shl(w.dy12, 1))), __mul24(ky, shl(w.dx12, 1)));
+						q2 = wadd(wadd(w.c2, __mul24(kx, shl(w.dy23, 1))), __mul24(ky, shl(w.dx23, 1)));
+						q3 = wadd(wadd(w.c3, __mul24(kx, shl(w.dy31, 1))), __mul24(ky, shl(w.dx31, 1))); }
+					else {
+						q1 = wadd(wadd(w.c1, wmul(kx, shl(w.dy12, 1))), wmul(ky, shl(w.dx12, 1)));
+						q2 = wadd(wadd(w.c2, wmul(kx, shl(w.dy23, 1))), wmul(ky, shl(w.dx23, 1)));
+						q3 = wadd(wadd(w.c3, wmul(kx, shl(w.dy31, 1))), wmul(ky, shl(w.dx31, 1))); }
 					vis_quad(keys, w, q1, q2, q3, w.x0 + 2 * kx, w.y0 + 2 * ky, leq, frags); }}
 			__syncthreads(); }
 		// merge the region into the frame's key buffer
@@ -1112,7 +1126,8 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 			const unsigned long long k = keys[ly * kSubDim + lx];
 			unsigned long long* dst = a.keys + f.keyBase + static_cast<long long>(S.y0 + ly) * f.W + S.x0 + lx;
 			if (nseg == 1) *dst = k;
-			else if (k != kKeySentinel) atomicMin(dst, k); }}
+			else if (k != kKeySentinel) atomicMin(dst, k); }
+		if (threadIdx.x == 0 && itemQuads) atomicAdd(&a.tileCost[tile], itemQuads); }
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
 	__syncthreads();
 	if ((threadIdx.x & 63) == 0) scan_sh[threadIdx.x >> 6] = frags;

@@ -207,8 +207,8 @@ class GPU:
     def set_guard_band(self, fixed: bool):
         self._check(self.L.rgl_gpu_set_guard_band(self.h, int(fixed)))
 
-    def set_raster_params(self, seg_pairs: int, split_pairs: int = 0):
-        self._check(self.L.rgl_gpu_set_raster_params(self.h, seg_pairs, split_pairs))
+    def set_raster_params(self, seg_pairs: int, split_quads: int = 0):
+        self._check(self.L.rgl_gpu_set_raster_params(self.h, seg_pairs, split_quads))
 
     # --- learned shading (no reference counterpart) ---------------------------
     def set_learned_shading(self, dev_params, max_rms_err: float = 2.0 / 255.0):
