@@ -153,6 +153,7 @@ struct rgl_gpu {
 	DevArray<DrawDesc> dDraws;
 	Counters* dCtr = nullptr;
 	Counters* hCtr = nullptr;       // pinned
+	Counters* hCtrDev = nullptr;    // hCtr as the device sees it (null: the host copies the counters)
 	DrawDesc* hDraws = nullptr;     // pinned, kMaxDraws
 	FrameDesc* hFrames = nullptr;   // pinned, kMaxFrames
 	DevArray<FrameDesc> dFrames;
@@ -497,7 +498,8 @@ struct rgl_gpu {
 		hipError_t e;
 		if (!dCtr) {
 			if ((e = hipMalloc(reinterpret_cast<void**>(&dCtr), sizeof(Counters))) != hipSuccess) return hipfail(e, "hipMalloc counters");
-			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocMapped)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if (hipHostGetDevicePointer(reinterpret_cast<void**>(&hCtrDev), hCtr, 0) != hipSuccess) hCtrDev = nullptr;
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hFrames), sizeof(FrameDesc) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
@@ -600,8 +602,7 @@ struct rgl_gpu {
 		hipError_t e;
 		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
 		const long long nCsum = static_cast<long long>(unit_chunks(b.units)) * b.nTiles;
-		k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(
-			dCtr, clipHist.p, b.nTiles, bsum.p, nCsum, qCount.p);
+		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, bsum.p, nCsum, qCount.p };
 		Mark(0);
 		// vertex and clip stages: one launch per program present in the batch
 		bool present[5] = { false, false, false, false, false };
@@ -609,12 +610,15 @@ struct rgl_gpu {
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
 		const unsigned vblocks = static_cast<unsigned>((vtot + 255) / 256);
+		// the first vertex launch also clears the frame's counters
+		int zeroHere = 1;
 		if (vtot > 0) {
-			if (present[0]) k_vertex<ProgAmy><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
-			if (present[1]) k_vertex<ProgMany><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
-			if (present[2]) k_vertex<ProgLambertTex><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
-			if (present[3]) k_vertex<ProgManyBlinnPhong><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p);
-			if (present[4]) k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p); }
+			if (present[0]) { k_vertex<ProgAmy><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
+			if (present[1]) { k_vertex<ProgMany><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
+			if (present[2]) { k_vertex<ProgLambertTex><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
+			if (present[3]) { k_vertex<ProgManyBlinnPhong><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
+			if (present[4]) { k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }}
+		if (zeroHere) k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(za);
 		Mark(1);
 		if (b.units > 0) {
 			k_setup_count<<<b.units, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
@@ -640,6 +644,8 @@ struct rgl_gpu {
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
 		ra.keys = keys.p; ra.workOff = workOff.p; ra.tileCost = tileCost.p;
+		// plain frames end with k_resolve, which publishes the counters itself
+		ra.hostCtr = learned ? nullptr : hCtrDev;
 		if (b.units > 0) {
 			ScatterArgs sa;
 			sa.trect = trect.p; sa.tvert = tvert.p; sa.vpos = vpos.p; sa.off = hist.p; sa.rec = rec.p; sa.listCap = listCap;
@@ -680,7 +686,8 @@ struct rgl_gpu {
 			k_mlp_samples<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, q.out, q.cap, q.count, q.keep, q.seed);
 		Mark(9);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "kernel launch");
-		if ((e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess) return hipfail(e, "hipMemcpyAsync counters");
+		if (!ra.hostCtr && (e = hipMemcpyAsync(hCtr, dCtr, sizeof(Counters), hipMemcpyDeviceToHost, stream)) != hipSuccess)
+			return hipfail(e, "hipMemcpyAsync counters");
 		return 0; }
 
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans

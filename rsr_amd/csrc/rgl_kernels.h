@@ -117,9 +117,26 @@ __device__ inline void load_block_bases(int* fbb, const BatchDesc& b) {
 // ---------------------------------------------------------------------------
 // one launch covers every draw of the batch that uses program P; a thread
 // finds its draw by the draws' first vertex-cache entries
+// the frame's start-of-frame clears: counters, clip histogram, scan chunk
+// sums and network queue counters (run by the first vertex launch, or by
+// k_zero_counters when a frame has no vertices)
+struct ZeroArgs {
+	Counters* ctr; uint32_t* clipHist; int nTiles; uint32_t* csum; long long nCsum; uint32_t* qcount;
+};
+
+__device__ inline void zero_frame(const ZeroArgs& z) {
+	const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+	const long long n = static_cast<long long>(gridDim.x) * blockDim.x;
+	for (long long c = i; c < z.nCsum; c += n) z.csum[c] = 0u;
+	for (long long c = i; c < static_cast<long long>(sizeof(Counters) / 4); c += n) reinterpret_cast<uint32_t*>(z.ctr)[c] = 0u;
+	for (long long c = i; c < 8 * 32; c += n) z.qcount[c] = 0u;
+	for (long long t = i; t < z.nTiles; t += n) z.clipHist[t] = 0u; }
+
 template <class P>
 __global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ draws, int nDraws, long long vtot,
-                                                 float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary) {
+                                                 float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary,
+                                                 ZeroArgs z, int zeroHere) {
+	if (zeroHere) zero_frame(z);
 	const long long gi = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (gi >= vtot) return;
 	int lo = 0, hi = nDraws - 1;
@@ -603,6 +620,7 @@ struct RasterArgs {
 	unsigned long long* keys;   // per-pixel visibility key (smallest (depth, slot) of the frame)
 	const uint32_t* workOff;    // per-tile first work item [nTiles + 1]
 	uint32_t* tileCost;         // per-tile quad work, for the next frame's region split
+	Counters* hostCtr;          // pinned host copy of the counters, written by the last kernel (null: copied by the host)
 };
 
 constexpr unsigned long long kKeySentinel = ~0ull;
@@ -971,6 +989,11 @@ __global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
 	load_lite(dl, a.draws, b.nDraws);
 	load_block_bases(fbb, b);
 	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
+	// the frame's counters are final (no later kernel changes them): block 0
+	// publishes them to the pinned host copy
+	if (blockIdx.x == 0 && a.hostCtr)
+		for (int i = threadIdx.x; i < static_cast<int>(sizeof(Counters) / 4); i += 256)
+			reinterpret_cast<uint32_t*>(a.hostCtr)[i] = reinterpret_cast<const uint32_t*>(a.ctr)[i];
 	__syncthreads();
 	// the rcpps table is read through the vector L1 rather than copied into
 	// every workgroup's LDS
@@ -1143,11 +1166,6 @@ __global__ void k_fill_u32(uint32_t* p, long long n, uint32_t v) {
 __global__ void k_fill_u64(unsigned long long* p, long long n, unsigned long long v) {
 	for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<long long>(gridDim.x) * blockDim.x) p[i] = v; }
 
-__global__ void k_zero_counters(Counters* ctr, uint32_t* clipHist, int nTiles, uint32_t* csum, long long nCsum, uint32_t* qcount) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	for (long long c = i; c < nCsum; c += static_cast<long long>(gridDim.x) * blockDim.x) csum[c] = 0u;
-	if (i < static_cast<int>(sizeof(Counters) / 4)) reinterpret_cast<uint32_t*>(ctr)[i] = 0u;
-	if (i < 8 * 32) qcount[i] = 0u;
-	for (int t = i; t < nTiles; t += gridDim.x * blockDim.x) clipHist[t] = 0u; }
+__global__ void k_zero_counters(ZeroArgs z) { zero_frame(z); }
 
 }  // namespace rgl
