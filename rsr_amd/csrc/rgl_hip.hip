@@ -633,7 +633,7 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, 1024, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileCost.p, dCtr);
+		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileCost.p, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }

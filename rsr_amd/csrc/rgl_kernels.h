@@ -302,7 +302,9 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 // t's first list position (tileOff[nTiles] = the total); the down pass
 // turns the counts in place into list positions.  A thread owns one tile
 // column, so a wave's loads and stores cover contiguous bytes.
-constexpr int kColBatch = 16;   // independent loads in flight per thread
+constexpr int kColBatch = 32;   // independent loads in flight per thread (down pass: a whole chunk)
+constexpr int kScanBatch = 64;  // chunk sums loaded at once per tile column (column scan)
+constexpr int kScanThreads = 512; // column-scan block: one tile column per thread, registers for kScanBatch loads
 
 // Visibility work items of a tile: its inline + clipped pairs are cut into
 // segments of segPairs, and the tile into regions: the four 32 x 32
@@ -323,24 +325,24 @@ __device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t cost, uint32
 // One block: chunk prefixes within each column, then per tile the list
 // offset (exclusive scan of the column totals), the clipped-list offset
 // and cursor (scan of clipHist) and the tile's visibility work items.
-__global__ __launch_bounds__(1024) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
+__global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
 		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, uint32_t* __restrict__ itemTile, uint32_t itemCap,
 		uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
-	__shared__ uint32_t sh[1024 / 64 + 1];
+	__shared__ uint32_t sh[kScanThreads / 64 + 1];
 	const int nT = b.nTiles;
 	uint32_t carry = 0, ccarry = 0, wcarry = 0;
-	for (int base = 0; base < nT; base += 1024) {
+	for (int base = 0; base < nT; base += kScanThreads) {
 		const int t = base + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0;
 		if (t < nT) {
-			for (int c0 = 0; c0 < nChunks; c0 += kColBatch) {
-				uint32_t w[kColBatch];
+			for (int c0 = 0; c0 < nChunks; c0 += kScanBatch) {
+				uint32_t w[kScanBatch];
 #pragma unroll
-				for (int j = 0; j < kColBatch; ++j)
+				for (int j = 0; j < kScanBatch; ++j)
 					w[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
 #pragma unroll
-				for (int j = 0; j < kColBatch; ++j) {
+				for (int j = 0; j < kScanBatch; ++j) {
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
 					col += w[j]; }}
 			v = clipHist[t];
