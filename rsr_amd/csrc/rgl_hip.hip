@@ -145,6 +145,7 @@ struct rgl_gpu {
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
 	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff, itemTile, tileCost;
+	DevArray<VisTile> tileDesc;
 	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
 	DevArray<unsigned long long> keys;
@@ -198,6 +199,7 @@ struct rgl_gpu {
 
 	~rgl_gpu() {
 		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff, &itemTile, &tileCost }) a->release();
+		tileDesc.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
@@ -414,6 +416,7 @@ struct rgl_gpu {
 		itemCap = static_cast<uint32_t>(std::min<long long>(
 			static_cast<long long>(kMaxRegions) * (b.nTiles + (static_cast<long long>(listCap) + clistCap) / b.segPairs + 1), 1LL << 30));
 		if ((e = itemTile.ensure(itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
+		if ((e = tileDesc.ensure(b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
 		if ((e = tileCost.ensure(b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
@@ -563,7 +566,7 @@ struct rgl_gpu {
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, tileCost.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, tileDesc.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -633,7 +636,7 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileCost.p, dCtr);
+		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
@@ -657,7 +660,7 @@ struct rgl_gpu {
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;
 			// the kernel strides over any remainder
-			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p); }
+			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p, tileDesc.p); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		if (learned) {

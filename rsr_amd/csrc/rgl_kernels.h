@@ -322,19 +322,36 @@ __device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t cost, uint32
 	const uint32_t per = cost / (nseg * kQuadrants), s = static_cast<uint32_t>(b.splitQuads);
 	return per > 2 * s ? 4 * kQuadrants : per > s ? 2 * kQuadrants : kQuadrants; }
 
+// A tile's visibility work as the scan lays it out, so that a work item
+// reads its tile (itemTile) and then one 32-byte descriptor instead of a
+// chain of table lookups.
+struct alignas(16) VisTile {
+	uint32_t item0;        // the tile's first work item
+	uint32_t frame;
+	uint32_t ib, cb;       // the tile's first inline and clipped records (clamped to the capacities)
+	uint32_t ninl, ntot;   // the tile's inline records, and inline + clipped
+	uint32_t nreg;         // regions per segment
+	uint32_t multiSeg;     // 1: the tile has several segments (regions merge with atomic min)
+};
+static_assert(sizeof(VisTile) == 32, "two 16-byte loads");
+
 // One block: chunk prefixes within each column, then per tile the list
 // offset (exclusive scan of the column totals), the clipped-list offset
 // and cursor (scan of clipHist) and the tile's visibility work items.
 __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
 		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, uint32_t* __restrict__ itemTile, uint32_t itemCap,
-		uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
+		VisTile* __restrict__ tileDesc,
+		uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[kScanThreads / 64 + 1];
+	__shared__ int ftb[kMaxFrames + 1];
+	load_tile_bases(ftb, b);
+	__syncthreads();
 	const int nT = b.nTiles;
 	uint32_t carry = 0, ccarry = 0, wcarry = 0;
 	for (int base = 0; base < nT; base += kScanThreads) {
 		const int t = base + threadIdx.x;
-		uint32_t col = 0, v = 0, items = 0;
+		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants;
 		if (t < nT) {
 			for (int c0 = 0; c0 < nChunks; c0 += kScanBatch) {
 				uint32_t w[kScanBatch];
@@ -346,8 +363,9 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
 					col += w[j]; }}
 			v = clipHist[t];
-			const uint32_t nseg = vis_segments(b, col + v);
-			items = nseg * vis_regions(b, tileCost[t], nseg);
+			nseg = vis_segments(b, col + v);
+			nreg = vis_regions(b, tileCost[t], nseg);
+			items = nseg * nreg;
 			tileCost[t] = 0u; }
 		uint32_t tot, ctot, wtot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
@@ -357,8 +375,18 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 			tileOff[t] = carry + ex;
 			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
 			workOff[t] = wcarry + wex;
+			VisTile d;
+			d.item0 = wcarry + wex;
+			d.frame = static_cast<uint32_t>(find_base(ftb, b.nFrames, t));
+			d.ib = min(carry + ex, listCap);
+			d.cb = min(ccarry + cex, clistCap);
+			d.ninl = min(carry + ex + col, listCap) - d.ib;
+			d.ntot = d.ninl + (min(ccarry + cex + v, clistCap) - d.cb);
+			d.nreg = nreg;
+			d.multiSeg = nseg > 1 ? 1u : 0u;
+			tileDesc[t] = d;
 			// each work item's tile
-			for (uint32_t j = wcarry + wex, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t); }
+			for (uint32_t j = d.item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t); }
 		carry += tot; ccarry += ctot; wcarry += wtot; }
 	if (threadIdx.x == 0) {
 		tileOff[nT] = carry; clipOff[nT] = ccarry; workOff[nT] = wcarry;
@@ -1010,26 +1038,23 @@ __global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
 // previous frame's item count and strides over any remainder, so no
 // work-queue counter is shared between workgroups
 __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs a, const VisRec* __restrict__ rec,
-                                                       const VisRec* __restrict__ crec, const uint32_t* __restrict__ itemTile) {
+                                                       const VisRec* __restrict__ crec, const uint32_t* __restrict__ itemTile,
+                                                       const VisTile* __restrict__ tileDesc) {
 	__shared__ unsigned long long keys[kSubDim * kSubDim];
 	__shared__ VisSetup ts[kRasterThreads];
 	__shared__ uint32_t qoff[kRasterThreads + 1];
 	__shared__ uint32_t scan_sh[kRasterThreads / 64 + 1];
 	__shared__ uint8_t owner[kQuadWindow];
-	__shared__ int ftb[kMaxFrames + 1];
-	load_tile_bases(ftb, b);
 	uint32_t frags = 0;
 	const uint32_t nItems = a.ctr->workItems;
 	for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
 		__syncthreads();
 		const int tile = static_cast<int>(itemTile[item]);
-		const uint32_t w0 = a.workOff[tile];
-		const uint32_t w1 = a.workOff[tile + 1];
-		const uint32_t local = item - w0;
-		const uint32_t nseg = vis_segments(b, tile_pairs(a.tileOff, a.clipOff, tile));
-		const uint32_t nreg = (w1 - w0) / nseg;
-		const uint32_t seg = local / nreg, reg = local - seg * nreg;
-		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+		const VisTile it = tileDesc[tile];
+		const uint32_t local = item - it.item0;
+		const uint32_t nreg = it.nreg, seg = local / nreg, reg = local - seg * nreg;
+		const bool multiSeg = it.multiSeg != 0u;
+		const FrameDesc& f = b.frames[it.frame];
 		const bool leq = f.depthFunc == 1;
 		const Rect R = tile_rect(f, tile - f.tileBase);
 		// the item's region: a quadrant, or its half or quarter
@@ -1042,11 +1067,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		S.x1 = imin(S.x0 + sw, R.x1); S.y1 = imin(S.y0 + sh, R.y1);
 		if (S.x0 >= S.x1 || S.y0 >= S.y1) continue;   // region outside the screen (uniform)
 		for (int i = threadIdx.x; i < kSubDim * kSubDim; i += kRasterThreads) keys[i] = kKeySentinel;
-		const uint32_t ib = min(a.tileOff[tile], a.listCap);
-		const uint32_t ie = min(a.tileOff[tile + 1], a.listCap);
-		const uint32_t cb = min(a.clipOff[tile], a.clistCap);
-		const uint32_t ce = min(a.clipOff[tile + 1], a.clistCap);
-		const uint32_t ninl = ie - ib, ntot = ninl + (ce - cb);
+		const uint32_t ib = it.ib, cb = it.cb, ninl = it.ninl, ntot = it.ntot;
 		const uint32_t pbeg = seg * static_cast<uint32_t>(b.segPairs), pend = min(pbeg + static_cast<uint32_t>(b.segPairs), ntot);
 		uint32_t itemQuads = 0;   // quads dealt out over the workgroup: the item's cost for the next frame's split
 		__syncthreads();
@@ -1150,7 +1171,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 			const int ly = i / rw, lx = i - ly * rw;
 			const unsigned long long k = keys[ly * kSubDim + lx];
 			unsigned long long* dst = a.keys + f.keyBase + static_cast<long long>(S.y0 + ly) * f.W + S.x0 + lx;
-			if (nseg == 1) *dst = k;
+			if (!multiSeg) *dst = k;
 			else if (k != kKeySentinel) atomicMin(dst, k); }
 		if (threadIdx.x == 0 && itemQuads) atomicAdd(&a.tileCost[tile], itemQuads); }
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
