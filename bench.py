@@ -290,7 +290,7 @@ def main():
         scene = dataclasses.replace(scene, tile_blocks=(choice["tile_px"] // 8, choice["tile_px"] // 8))
     gpu = rgl.GPU(0)
     if tune:
-        gpu.set_raster_params(tune["choice"]["seg_pairs"], tune["choice"]["split_pairs"])
+        gpu.set_raster_params(tune["choice"]["seg_pairs"], tune["choice"]["split_quads"])
     ds = rgl.DeviceScene(scene, dev, prim=True)
     ds.record(gpu)
 

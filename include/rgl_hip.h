@@ -166,10 +166,11 @@ int rgl_gpu_set_guard_band(rgl_gpu* gpu, int mode);
 
 /* The autotuner's knobs for how many workgroups share a dense tile:
    seg_pairs = (triangle, tile) pairs per visibility work item (default
-   512); split_pairs = pairs per work item from which each 32 x 32
-   quadrant is cut in two, and from twice that in four (default 0 =
-   never).  Every setting draws identical frames. */
-int rgl_gpu_set_raster_params(rgl_gpu* gpu, int seg_pairs, int split_pairs);
+   512); split_quads = quads dealt out per 32 x 32 quadrant (measured on
+   the previous frame) above which the quadrant is cut in two, and above
+   twice that in four (default 0 = never).  Every setting draws identical
+   frames. */
+int rgl_gpu_set_raster_params(rgl_gpu* gpu, int seg_pairs, int split_quads);
 
 /* --- learned shading --------------------------------------------------- */
 /* No counterpart in the reference (its shaders are analytic).  A small MLP

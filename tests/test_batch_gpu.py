@@ -162,6 +162,8 @@ def test_stress_2m_batch_equals_single():
 def test_raster_params_draw_identical_frames():
     # every (pairs per work item, region split) setting draws the same
     # frame: single and multi-segment tiles, quadrants cut in two and four
+    # (the split follows the previous frame's quad work, so each setting
+    # runs twice)
     from oracle_backend import render as oracle_render
     mix = [scenes.scene_icospheres(960, 540, instances=128, seed=7, tex_dim=256),
            dataclasses.replace(scenes.scene_sphere_ground(320, 176), cull=False, depth_func=scenes.GL_LEQUAL,
@@ -170,20 +172,21 @@ def test_raster_params_draw_identical_frames():
     stream = torch.cuda.current_stream().cuda_stream
     recs = _recorders(mix)
     ref = None
-    for seg, split in ((512, 0), (64, 0), (64, 1), (128, 32), (256, 128), (4096, 1), (4096, 0)):
+    for seg, split in ((512, 0), (64, 0), (64, 1), (128, 16), (256, 64), (4096, 1), (4096, 0)):
         ex = rgl.GPU(0)
         ex.set_raster_params(seg, split)
-        ex.run_batch_and_wait([g for g, _ in recs], stream)
-        out = [ds.outputs() for _, ds in recs]
-        if ref is None:
-            ref = out
-            for sc, o in zip(mix, out):
-                g, ds = recs[mix.index(sc)]
-                g.run_and_wait(stream)
-                _compare(sc.name, oracle_render(sc, threads=8), (*o, g.stats()))
-        else:
-            for sc, a, b in zip(mix, out, ref):
-                _same(f"{sc.name} seg {seg} split {split}", a, b)
+        for rep in range(2):
+            ex.run_batch_and_wait([g for g, _ in recs], stream)
+            out = [ds.outputs() for _, ds in recs]
+            if ref is None:
+                ref = out
+                for sc, o in zip(mix, out):
+                    g, ds = recs[mix.index(sc)]
+                    g.run_and_wait(stream)
+                    _compare(sc.name, oracle_render(sc, threads=8), (*o, g.stats()))
+            else:
+                for sc, a, b in zip(mix, out, ref):
+                    _same(f"{sc.name} seg {seg} split {split} run {rep}", a, b)
         ex.close()
     for g, _ in recs:
         g.close()
