@@ -132,16 +132,10 @@ __device__ inline void zero_frame(const ZeroArgs& z) {
 	for (long long c = i; c < 8 * 32; c += n) z.qcount[c] = 0u;
 	for (long long t = i; t < z.nTiles; t += n) z.clipHist[t] = 0u; }
 
+// one transformed vertex: ShadeVertex, clip flags, pdiv, viewport
 template <class P>
-__global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ draws, int nDraws, long long vtot,
-                                                 float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary,
-                                                 ZeroArgs z, int zeroHere) {
-	if (zeroHere) zero_frame(z);
-	const long long gi = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-	if (gi >= vtot) return;
-	int lo = 0, hi = nDraws - 1;
-	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (draws[mid].vBase <= gi) lo = mid; else hi = mid - 1; }
-	const DrawDesc& d = draws[lo];
+__device__ inline void vertex_one(const DrawDesc& d, long long gi, float4* __restrict__ vpos, uint8_t* __restrict__ vflags,
+                                  float* __restrict__ vvary) {
 	if (d.program != P::kId) return;
 	const long long i = gi - d.vBase;
 	const int iid = static_cast<int>(i / d.verts);
@@ -160,11 +154,25 @@ __global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ dra
 	const float iw = oneover(pos[3]);
 	const float dcx = (pos[0] * iw) * d.DS[0] + d.DO[0];
 	const float dcy = (pos[1] * iw) * d.DS[1] + d.DO[1];
-	const long long g = gi;
-	vpos[g] = make_float4(dcx, dcy, pos[2] * iw, iw);
-	vflags[g] = static_cast<uint8_t>(f);
-	float* out = vvary + g * kMaxVary;
+	vpos[gi] = make_float4(dcx, dcy, pos[2] * iw, iw);
+	vflags[gi] = static_cast<uint8_t>(f);
+	float* out = vvary + gi * kMaxVary;
 	for (int k = 0; k < P::kVary; ++k) out[k] = vo[k]; }
+
+template <class P>
+__global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ draws, int nDraws, long long vtot,
+                                                 float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary,
+                                                 ZeroArgs z, int zeroHere) {
+	if (zeroHere) zero_frame(z);
+	const long long gi = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (gi >= vtot) return;
+	int lo = 0, hi = nDraws - 1;
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (draws[mid].vBase <= gi) lo = mid; else hi = mid - 1; }
+	// a wave whose vertices all belong to one draw reads that draw's
+	// descriptor through scalar loads
+	const int d0 = __builtin_amdgcn_readfirstlane(lo);
+	if (__all(lo == d0)) vertex_one<P>(draws[d0], gi, vpos, vflags, vvary);
+	else vertex_one<P>(draws[lo], gi, vpos, vflags, vvary); }
 
 // ---------------------------------------------------------------------------
 // triangle setup + per-unit tile counts: the primitive loop of
