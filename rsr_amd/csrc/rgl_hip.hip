@@ -382,61 +382,69 @@ struct rgl_gpu {
 		submitted += static_cast<long long>(tris) * instances;
 		return 0; }
 
+	// a buffer that must grow is reallocated only once the frame in flight
+	// (which may still use it) has finished; otherwise frames queue back to
+	// back on the stream without a host round trip between them
+	template <class T>
+	hipError_t Grow(DevArray<T>& a, size_t n) {
+		if (n <= a.cap) return hipSuccess;
+		if (pending) { const hipError_t e = hipEventSynchronize(done); if (e != hipSuccess) return e; }
+		return a.ensure(n); }
+
 	int Ensure(const BatchDesc& b, long long vtot, long long totalPx) {
 		hipError_t e;
-		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
 		const long long S = std::max<long long>(b.totalTris, 1);
-		if ((e = vpos.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vpos");
-		if ((e = vflags.ensure(vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vflags");
-		if ((e = vvary.ensure((vtot + 1) * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc vvary");
-		if ((e = trect.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
-		if ((e = tvert.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
+		if ((e = Grow(vpos, vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vpos");
+		if ((e = Grow(vflags, vtot + 1)) != hipSuccess) return hipfail(e, "hipMalloc vflags");
+		if ((e = Grow(vvary, (vtot + 1) * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc vvary");
+		if ((e = Grow(trect, S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
+		if ((e = Grow(tvert, S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
 		const long long nh = static_cast<long long>(b.nTiles) * std::max(b.units, 1) + 1;
 		const long long nChunks = unit_chunks(std::max(b.units, 1));
-		if ((e = hist.ensure(nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
-		if ((e = bsum.ensure(nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
-		if ((e = tileOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
+		if ((e = Grow(hist, nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
+		if ((e = Grow(bsum, nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
+		if ((e = Grow(tileOff, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
-		if ((e = rec.ensure(listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
+		if ((e = Grow(rec, listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
 		if (clipCap < std::min<long long>(S, 4096)) clipCap = static_cast<uint32_t>(std::min<long long>(S, 4096));
-		if ((e = clipq.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc clipq");
-		if ((e = clipqOf.ensure(S)) != hipSuccess) return hipfail(e, "hipMalloc clipqOf");
-		if ((e = cvpos.ensure(static_cast<size_t>(clipCap) * kClipVerts)) != hipSuccess) return hipfail(e, "hipMalloc cvpos");
-		if ((e = cvvary.ensure(static_cast<size_t>(clipCap) * kClipVerts * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc cvvary");
-		if ((e = fcount.ensure(clipCap)) != hipSuccess) return hipfail(e, "hipMalloc fcount");
-		if ((e = frect.ensure(static_cast<size_t>(clipCap) * kFanMax)) != hipSuccess) return hipfail(e, "hipMalloc frect");
-		if ((e = clipHist.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipHist");
-		if ((e = clipOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
-		if ((e = clipCursor.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
+		if ((e = Grow(clipq, clipCap)) != hipSuccess) return hipfail(e, "hipMalloc clipq");
+		if ((e = Grow(clipqOf, S)) != hipSuccess) return hipfail(e, "hipMalloc clipqOf");
+		if ((e = Grow(cvpos, static_cast<size_t>(clipCap) * kClipVerts)) != hipSuccess) return hipfail(e, "hipMalloc cvpos");
+		if ((e = Grow(cvvary, static_cast<size_t>(clipCap) * kClipVerts * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc cvvary");
+		if ((e = Grow(fcount, clipCap)) != hipSuccess) return hipfail(e, "hipMalloc fcount");
+		if ((e = Grow(frect, static_cast<size_t>(clipCap) * kFanMax)) != hipSuccess) return hipfail(e, "hipMalloc frect");
+		if ((e = Grow(clipHist, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipHist");
+		if ((e = Grow(clipOff, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipOff");
+		if ((e = Grow(clipCursor, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
-		if ((e = crec.ensure(clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
-		if ((e = workOff.ensure(b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
+		if ((e = Grow(crec, clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
+		if ((e = Grow(workOff, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
 		// work items: at most kMaxRegions per segment, and a tile's segments
 		// number at most its pairs / segPairs + 1
 		itemCap = static_cast<uint32_t>(std::min<long long>(
 			static_cast<long long>(kMaxRegions) * (b.nTiles + (static_cast<long long>(listCap) + clistCap) / b.segPairs + 1), 1LL << 30));
-		if ((e = itemTile.ensure(itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
-		if ((e = tileDesc.ensure(b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
-		if ((e = tileCost.ensure(b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
+		if ((e = Grow(itemTile, itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
+		if ((e = Grow(tileDesc, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
+		if ((e = Grow(tileCost, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
-		if ((e = keys.ensure(px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
+		if ((e = Grow(keys, px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
 		if (mlpParams) {
-			if ((e = mlpImg.ensure(kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
-			if ((e = tileErr.ensure(2LL * b.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
+			if ((e = Grow(mlpImg, kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
+			if ((e = Grow(tileErr, 2LL * b.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
 			// each queue segment takes the resolve blocks with index % kQueueSegs == seg
 			mlpSegCap = static_cast<uint32_t>((static_cast<long long>(b.nBlocks) + kQueueSegs - 1) / kQueueSegs * 256);
-			if ((e = mlpQueue.ensure(static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
+			if ((e = Grow(mlpQueue, static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
 			if (!mlpShadeBlocksSet) {
 				int cus = 0;
 				if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) mlpShadeBlocks = cus;
 				mlpShadeBlocksSet = true; }}
-		if ((e = qCount.ensure(kQueueSegs * kQueueStride)) != hipSuccess) return hipfail(e, "hipMalloc queue counters");
+		if ((e = Grow(qCount, kQueueSegs * kQueueStride)) != hipSuccess) return hipfail(e, "hipMalloc queue counters");
 		if (!dDraws.p) {
-			if ((e = dDraws.ensure(kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
+			if ((e = Grow(dDraws, kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
 			lastTable.clear(); }
 		if (!dFrames.p) {
-			if ((e = dFrames.ensure(kMaxFrames)) != hipSuccess) return hipfail(e, "hipMalloc frames");
+			if ((e = Grow(dFrames, kMaxFrames)) != hipSuccess) return hipfail(e, "hipMalloc frames");
 			lastFrames.clear(); }
 		return 0; }
 
@@ -494,11 +502,14 @@ struct rgl_gpu {
 		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
+		hipError_t e;
+		// frames of one rgl_gpu queue back to back on one stream; a frame on
+		// another stream first waits for the one in flight
+		if (pending && s != stream && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
 		int r = Ensure(b, vtot, keyBase);
 		if (r != 0) return r;
 		b.frames = dFrames.p;
 		stream = s;
-		hipError_t e;
 		if (!dCtr) {
 			if ((e = hipMalloc(reinterpret_cast<void**>(&dCtr), sizeof(Counters))) != hipSuccess) return hipfail(e, "hipMalloc counters");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocMapped)) != hipSuccess) return hipfail(e, "hipHostMalloc");
@@ -578,6 +589,7 @@ struct rgl_gpu {
 
 	int Capture(const BatchDesc& b, const std::vector<DrawDesc>& draws, bool learned, int pid, int items, uint64_t key) {
 		hipError_t e;
+		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
 		if (graphExec) { (void)hipGraphExecDestroy(graphExec); graphExec = nullptr; }
 		if (!capStream && (e = hipStreamCreateWithFlags(&capStream, hipStreamNonBlocking)) != hipSuccess) return hipfail(e, "hipStreamCreate");
 		hipStream_t user = stream;
