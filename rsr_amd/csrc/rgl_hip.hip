@@ -144,7 +144,7 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, workOff, itemTile, tileCost;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost;
 	DevArray<VisTile> tileDesc;
 	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
@@ -198,7 +198,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &workOff, &itemTile, &tileCost }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
 		tileDesc.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
@@ -418,7 +418,6 @@ struct rgl_gpu {
 		if ((e = Grow(clipCursor, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc clipCursor");
 		if (clistCap < 4096) clistCap = 65536;
 		if ((e = Grow(crec, clistCap)) != hipSuccess) return hipfail(e, "hipMalloc clipped setup records");
-		if ((e = Grow(workOff, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc workOff");
 		// work items: at most kMaxRegions per segment, and a tile's segments
 		// number at most its pairs / segPairs + 1
 		itemCap = static_cast<uint32_t>(std::min<long long>(
@@ -577,7 +576,7 @@ struct rgl_gpu {
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, tileDesc.p, tileCost.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -648,7 +647,7 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, workOff.p, itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
+		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
@@ -658,7 +657,7 @@ struct rgl_gpu {
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
-		ra.keys = keys.p; ra.workOff = workOff.p; ra.tileCost = tileCost.p;
+		ra.keys = keys.p; ra.tileCost = tileCost.p;
 		// plain frames end with k_resolve, which publishes the counters itself
 		ra.hostCtr = learned ? nullptr : hCtrDev;
 		if (b.units > 0) {

@@ -348,7 +348,7 @@ static_assert(sizeof(VisTile) == 32, "two 16-byte loads");
 // and cursor (scan of clipHist) and the tile's visibility work items.
 __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
-		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ workOff, uint32_t* __restrict__ itemTile, uint32_t itemCap,
+		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap,
 		VisTile* __restrict__ tileDesc,
 		uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[kScanThreads / 64 + 1];
@@ -356,10 +356,10 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 	load_tile_bases(ftb, b);
 	__syncthreads();
 	const int nT = b.nTiles;
-	uint32_t carry = 0, ccarry = 0, wcarry = 0;
+	uint32_t carry = 0, ccarry = 0;
 	for (int base = 0; base < nT; base += kScanThreads) {
 		const int t = base + threadIdx.x;
-		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants;
+		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
 		if (t < nT) {
 			for (int c0 = 0; c0 < nChunks; c0 += kScanBatch) {
 				uint32_t w[kScanBatch];
@@ -371,33 +371,55 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
 					col += w[j]; }}
 			v = clipHist[t];
+			cost = tileCost[t];
 			nseg = vis_segments(b, col + v);
-			nreg = vis_regions(b, tileCost[t], nseg);
+			nreg = vis_regions(b, cost, nseg);
 			items = nseg * nreg;
 			tileCost[t] = 0u; }
-		uint32_t tot, ctot, wtot;
+		uint32_t tot, ctot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
-		const uint32_t wex = block_exclusive_scan(items, sh, wtot);
 		if (t < nT) {
 			tileOff[t] = carry + ex;
 			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
-			workOff[t] = wcarry + wex;
 			VisTile d;
-			d.item0 = wcarry + wex;
+			d.item0 = items;   // the item count until the ordering below
 			d.frame = static_cast<uint32_t>(find_base(ftb, b.nFrames, t));
 			d.ib = min(carry + ex, listCap);
 			d.cb = min(ccarry + cex, clistCap);
 			d.ninl = min(carry + ex + col, listCap) - d.ib;
 			d.ntot = d.ninl + (min(ccarry + cex + v, clistCap) - d.cb);
 			d.nreg = nreg;
-			d.multiSeg = nseg > 1 ? 1u : 0u;
-			tileDesc[t] = d;
-			// each work item's tile
-			for (uint32_t j = d.item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t); }
-		carry += tot; ccarry += ctot; wcarry += wtot; }
+			d.multiSeg = (nseg > 1 ? 1u : 0u) | (cost << 1);   // the cost rides along until the ordering
+			tileDesc[t] = d; }
+		carry += tot; ccarry += ctot; }
+	// Work-item order: by the tiles' previous-frame quad work, heaviest
+	// first in power-of-two classes (a counting sort), so that long items
+	// start at once and short ones fill the end of the kernel.
+	__shared__ uint32_t classItems[33], classCursor[33];
+	for (int c = threadIdx.x; c < 33; c += kScanThreads) classItems[c] = 0u;
+	__syncthreads();
+	for (int t = threadIdx.x; t < nT; t += kScanThreads) {
+		const uint32_t cost = tileDesc[t].multiSeg >> 1;
+		const int cls = cost ? 32 - __clz(cost) : 0;   // 0 .. 31
+		atomicAdd(&classItems[cls], tileDesc[t].item0); }
+	__syncthreads();
 	if (threadIdx.x == 0) {
-		tileOff[nT] = carry; clipOff[nT] = ccarry; workOff[nT] = wcarry;
+		uint32_t run = 0;
+		for (int c = 32; c >= 0; --c) { classCursor[c] = run; run += classItems[c]; }
+		classItems[32] = run; }
+	__syncthreads();
+	const uint32_t wcarry = classItems[32];
+	for (int t = threadIdx.x; t < nT; t += kScanThreads) {
+		VisTile& d = tileDesc[t];
+		const uint32_t cost = d.multiSeg >> 1, items = d.item0;
+		const int cls = cost ? 32 - __clz(cost) : 0;
+		const uint32_t item0 = atomicAdd(&classCursor[cls], items);
+		for (uint32_t j = item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t);
+		d.item0 = item0;
+		d.multiSeg &= 1u; }
+	if (threadIdx.x == 0) {
+		tileOff[nT] = carry; clipOff[nT] = ccarry;
 		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
 
 __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
@@ -656,7 +678,6 @@ struct RasterArgs {
 	uint32_t clipCap;
 	Counters* ctr;
 	unsigned long long* keys;   // per-pixel visibility key (smallest (depth, slot) of the frame)
-	const uint32_t* workOff;    // per-tile first work item [nTiles + 1]
 	uint32_t* tileCost;         // per-tile quad work, for the next frame's region split
 	Counters* hostCtr;          // pinned host copy of the counters, written by the last kernel (null: copied by the host)
 };
