@@ -144,7 +144,7 @@ struct rgl_gpu {
 	DevArray<uint32_t> hist, bsum, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost, colTotal;
 	DevArray<VisTile> tileDesc;
 	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
@@ -198,7 +198,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
+		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost, &colTotal }) a->release();
 		tileDesc.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
@@ -425,6 +425,7 @@ struct rgl_gpu {
 		if ((e = Grow(itemTile, itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
 		if ((e = Grow(tileDesc, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
 		if ((e = Grow(tileCost, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
+		if ((e = Grow(colTotal, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc column totals");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
 		if ((e = Grow(keys, px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
@@ -647,7 +648,11 @@ struct rgl_gpu {
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
 		const int nChunks = unit_chunks(b.units);
-		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
+		// many (chunk, tile) sums: take the column prefixes over many workgroups
+		const bool colSplit = static_cast<long long>(nChunks) * b.nTiles > kColSplit;
+		if (colSplit) k_col_sum<<<(b.nTiles + 255) / 256, 256, 0, stream>>>(bsum.p, nChunks, b.nTiles, colTotal.p);
+		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, itemCap,
+			tileDesc.p, listCap, clistCap, tileCost.p, colSplit ? colTotal.p : nullptr, dCtr);
 		if (b.units > 0) {
 			const dim3 cg((b.nTiles + 63) / 64, nChunks);
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
@@ -706,6 +711,7 @@ struct rgl_gpu {
 
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
 	static constexpr int kResetBlocks = 128;        // k_scatter blocks that reset the keys of multi-segment tiles
+	static constexpr long long kColSplit = 100000;   // (chunk, tile) sums above which k_col_sum runs first
 
 	static int ProgramSlot(int pid) {
 		switch (pid) {

@@ -350,7 +350,8 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
 		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap,
 		VisTile* __restrict__ tileDesc,
-		uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost, Counters* __restrict__ ctr) {
+		uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost, const uint32_t* __restrict__ colTotal,
+		Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[kScanThreads / 64 + 1];
 	__shared__ int ftb[kMaxFrames + 1];
 	load_tile_bases(ftb, b);
@@ -360,7 +361,8 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 	for (int base = 0; base < nT; base += kScanThreads) {
 		const int t = base + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
-		if (t < nT) {
+		if (t < nT && colTotal) col = colTotal[t];   // chunk prefixes already taken by k_col_sum
+		else if (t < nT) {
 			for (int c0 = 0; c0 < nChunks; c0 += kScanBatch) {
 				uint32_t w[kScanBatch];
 #pragma unroll
@@ -369,7 +371,8 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 #pragma unroll
 				for (int j = 0; j < kScanBatch; ++j) {
 					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
-					col += w[j]; }}
+					col += w[j]; }}}
+		if (t < nT) {
 			v = clipHist[t];
 			cost = tileCost[t];
 			nseg = vis_segments(b, col + v);
@@ -421,6 +424,23 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 	if (threadIdx.x == 0) {
 		tileOff[nT] = carry; clipOff[nT] = ccarry;
 		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
+
+// Large batches take the chunk prefixes with one thread per tile column
+// over many workgroups before the single-block scan (which then reads each
+// column's total).
+__global__ __launch_bounds__(256) void k_col_sum(uint32_t* __restrict__ csum, int nChunks, int nT, uint32_t* __restrict__ colTotal) {
+	const int t = blockIdx.x * 256 + threadIdx.x;
+	if (t >= nT) return;
+	uint32_t col = 0;
+	for (int c0 = 0; c0 < nChunks; c0 += kColBatch) {
+		uint32_t w[kColBatch];
+#pragma unroll
+		for (int j = 0; j < kColBatch; ++j) w[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
+#pragma unroll
+		for (int j = 0; j < kColBatch; ++j) {
+			if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
+			col += w[j]; }}
+	colTotal[t] = col; }
 
 __global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
                                                  const uint32_t* __restrict__ tileOff) {
