@@ -152,10 +152,13 @@ struct rgl_gpu {
 	DevArray<VisRec> rec, crec;
 	size_t keysReady = 0;           // pixels of keys holding the sentinel
 	DevArray<DrawDesc> dDraws;
+	DevArray<DrawLite> dLite;
 	Counters* dCtr = nullptr;
 	Counters* hCtr = nullptr;       // pinned
 	Counters* hCtrDev = nullptr;    // hCtr as the device sees it (null: the host copies the counters)
 	DrawDesc* hDraws = nullptr;     // pinned, kMaxDraws
+	DrawLite* hLite = nullptr;      // pinned, kMaxDraws
+	std::vector<uint8_t> lastLite;
 	FrameDesc* hFrames = nullptr;   // pinned, kMaxFrames
 	DevArray<FrameDesc> dFrames;
 	std::vector<uint8_t> lastFrames;
@@ -204,10 +207,11 @@ struct rgl_gpu {
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
-		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release();
+		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release(); dLite.release();
 		if (dCtr) (void)hipFree(dCtr);
 		if (hCtr) (void)hipHostFree(hCtr);
 		if (hDraws) (void)hipHostFree(hDraws);
+		if (hLite) (void)hipHostFree(hLite);
 		if (hFrames) (void)hipHostFree(hFrames);
 		dFrames.release();
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
@@ -442,6 +446,7 @@ struct rgl_gpu {
 		if ((e = Grow(qCount, kQueueSegs * kQueueStride)) != hipSuccess) return hipfail(e, "hipMalloc queue counters");
 		if (!dDraws.p) {
 			if ((e = Grow(dDraws, kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draws");
+			if ((e = Grow(dLite, kMaxDraws)) != hipSuccess) return hipfail(e, "hipMalloc draw table");
 			lastTable.clear(); }
 		if (!dFrames.p) {
 			if ((e = Grow(dFrames, kMaxFrames)) != hipSuccess) return hipfail(e, "hipMalloc frames");
@@ -515,6 +520,7 @@ struct rgl_gpu {
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hCtr), sizeof(Counters), hipHostMallocMapped)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if (hipHostGetDevicePointer(reinterpret_cast<void**>(&hCtrDev), hCtr, 0) != hipSuccess) hCtrDev = nullptr;
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hLite), sizeof(DrawLite) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hFrames), sizeof(FrameDesc) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 		cur = nullptr;
@@ -525,6 +531,9 @@ struct rgl_gpu {
 			evUsed[evNext] = true;
 			evNext = (evNext + 1) % kRing; }
 		if ((r = UploadTable(draws, lastTable, hDraws, dDraws.p)) != 0) return r;
+		std::vector<DrawLite> lite(draws.size());
+		for (size_t i = 0; i < draws.size(); ++i) lite[i] = make_lite(draws[i]);
+		if ((r = UploadTable(lite, lastLite, hLite, dLite.p)) != 0) return r;
 		if ((r = UploadTable(frames, lastFrames, hFrames, dFrames.p)) != 0) return r;
 		if (costReady < tileCost.cap) {
 			k_fill_u32<<<8, 256, 0, stream>>>(tileCost.p, static_cast<long long>(tileCost.cap), 0u);
@@ -658,7 +667,7 @@ struct rgl_gpu {
 			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
 		Mark(4);
 		RasterArgs ra;
-		ra.draws = dDraws.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
+		ra.draws = dDraws.p; ra.lite = dLite.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
 		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
 		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
@@ -684,12 +693,12 @@ struct rgl_gpu {
 			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
 		else {
 			switch (pid) {
-			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
-			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
-			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
-			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
-			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, dlBytes, stream>>>(b, ra); break;
-			default: k_resolve<ShadeAny><<<rblocks, 256, dlBytes, stream>>>(b, ra); break; }}
+			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, 0, stream>>>(b, ra); break;
+			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, 0, stream>>>(b, ra); break;
+			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, 0, stream>>>(b, ra); break;
+			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, 0, stream>>>(b, ra); break;
+			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, 0, stream>>>(b, ra); break;
+			default: k_resolve<ShadeAny><<<rblocks, 256, 0, stream>>>(b, ra); break; }}
 		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);

@@ -91,18 +91,19 @@ __device__ inline int find_draw_slot(const DrawLite* dl, int nDraws, uint32_t sl
 	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (dl[mid].slotBase <= slot) lo = mid; else hi = mid - 1; }
 	return lo; }
 
+__host__ __device__ inline DrawLite make_lite(const DrawDesc& d) {
+	DrawLite l;
+	l.triBase = d.triBase; l.vBase = d.vBase; l.indices = d.indices;
+	l.tris = d.tris; l.instances = d.instances; l.verts = d.verts; l.elements = d.elements;
+	l.keepBacks = d.keepBacks; l.keepFronts = d.keepFronts;
+	for (int i = 0; i < 4; ++i) l.scissor[i] = d.scissor[i];
+	l.slotBase = d.slotBase; l.primBase = d.primBase;
+	l.tileW = d.tileW; l.tileH = d.tileH; l.tilesX = d.tilesX; l.tilesY = d.tilesY; l.tileBase = d.tileBase;
+	l.fW = d.fW; l.fH = d.fH;
+	return l; }
+
 __device__ inline void load_lite(DrawLite* dl, const DrawDesc* draws, int nDraws) {
-	for (int k = threadIdx.x; k < nDraws; k += blockDim.x) {
-		const DrawDesc& d = draws[k];
-		DrawLite l;
-		l.triBase = d.triBase; l.vBase = d.vBase; l.indices = d.indices;
-		l.tris = d.tris; l.instances = d.instances; l.verts = d.verts; l.elements = d.elements;
-		l.keepBacks = d.keepBacks; l.keepFronts = d.keepFronts;
-		for (int i = 0; i < 4; ++i) l.scissor[i] = d.scissor[i];
-		l.slotBase = d.slotBase; l.primBase = d.primBase;
-		l.tileW = d.tileW; l.tileH = d.tileH; l.tilesX = d.tilesX; l.tilesY = d.tilesY; l.tileBase = d.tileBase;
-		l.fW = d.fW; l.fH = d.fH;
-		dl[k] = l; }}
+	for (int k = threadIdx.x; k < nDraws; k += blockDim.x) dl[k] = make_lite(draws[k]); }
 
 // LDS table of the frames' first tiles (n + 1 entries, the last = total)
 __device__ inline void load_tile_bases(int* ftb, const BatchDesc& b) {
@@ -684,6 +685,7 @@ __device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cv
 
 struct RasterArgs {
 	const DrawDesc* draws;
+	const DrawLite* lite;       // the draws' compact table (the resolve reads it through the caches)
 	const uint32_t* tileOff;    // inline records: tile t's pairs are [tileOff[t], tileOff[t + 1])
 	uint32_t listCap;
 	const uint4* tvert;
@@ -1059,29 +1061,26 @@ __device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
 	r.y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
 	return r; }
 
+// The resolve keeps no LDS: the draw table, the rcpps and sRGB tables and
+// the frame table are read through the scalar and vector caches, so a
+// workgroup starts on its pixels at once (it runs for a few microseconds).
 template <class S>
 __global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
-	extern __shared__ uint4 dyn_lds[];   // the batch's draws
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ uint32_t srgbTab[104];
-	__shared__ int fbb[kMaxFrames + 1];
-	load_lite(dl, a.draws, b.nDraws);
-	load_block_bases(fbb, b);
-	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	// the frame's counters are final (no later kernel changes them): block 0
 	// publishes them to the pinned host copy
 	if (blockIdx.x == 0 && a.hostCtr)
 		for (int i = threadIdx.x; i < static_cast<int>(sizeof(Counters) / 4); i += 256)
 			reinterpret_cast<uint32_t*>(a.hostCtr)[i] = reinterpret_cast<const uint32_t*>(a.ctr)[i];
-	__syncthreads();
-	// the rcpps table is read through the vector L1 rather than copied into
-	// every workgroup's LDS
-	const uint32_t* rcpTab = kRcpMantissa;
-	const BlockPixel bp = block_pixel(b, fbb);
-	const FrameDesc& f = b.frames[bp.frame];
-	const int x = bp.x, y = bp.y;
+	const int blk = static_cast<int>(blockIdx.x);
+	int lo = 0, hi = b.nFrames - 1;   // last frame whose first block is at or before blk
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (b.frames[mid].blockBase <= blk) lo = mid; else hi = mid - 1; }
+	const FrameDesc& f = b.frames[lo];
+	const int lb = blk - f.blockBase;
+	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
+	const int x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
+	const int y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
 	if (x >= f.W || y >= f.H) return;
-	resolve_store<S>(b, f, a, dl, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], rcpTab, srgbTab); }
+	resolve_store<S>(b, f, a, a.lite, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], kRcpMantissa, kSrgb8Tab); }
 
 // visibility work items run one per workgroup; the grid is sized from the
 // previous frame's item count and strides over any remainder, so no
