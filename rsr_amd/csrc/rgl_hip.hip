@@ -680,7 +680,7 @@ struct rgl_gpu {
 			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
 			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
 			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
-			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(b, dDraws.p, sa); }
+			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, b.nTiles * sizeof(uint32_t), stream>>>(b, dLite.p, sa); }
 		Mark(5);
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;

@@ -815,21 +815,18 @@ __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, in
 // triangle's tiles, takes a list position from the unit's LDS cursor of
 // the tile and writes the pair's setup record.  Blocks [units, units +
 // clipBlocks): the clipped fans.  The rest: key resets.
-__global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const DrawDesc* __restrict__ draws, ScatterArgs a) {
-	// dynamic LDS: the batch's draws, then nTiles write cursors
+__global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const DrawLite* __restrict__ dl, ScatterArgs a) {
+	// dynamic LDS: nTiles write cursors; the draw table is read through the caches
 	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	uint32_t* base = reinterpret_cast<uint32_t*>(dl + b.nDraws);
+	uint32_t* base = reinterpret_cast<uint32_t*>(dyn_lds);
 	const int u = blockIdx.x;
 	if (u >= b.units + a.clipBlocks) {
 		load_tile_bases(ftb, b);
 		__syncthreads();
 		reset_keys(b, ftb, u - b.units - a.clipBlocks, static_cast<int>(gridDim.x) - b.units - a.clipBlocks, a);
 		return; }
-	load_lite(dl, draws, b.nDraws);
 	if (u >= b.units) {
-		__syncthreads();
 		clip_scatter(b, dl, u - b.units, a);
 		return; }
 	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) base[t] = a.off[static_cast<long long>(u) * b.nTiles + t];
