@@ -879,7 +879,6 @@ static_assert(sizeof(VisSetup) % 16 == 0, "VisSetup rows are read with aligned 1
 
 
 constexpr int kQuadWindow = 2048;   // quad items mapped to their triangle per pass
-constexpr int kThreadQuads = 2;     // triangles with at most this many quads in the quadrant are walked by their own thread
 
 __device__ inline VisRec vis_record(const VisRec* __restrict__ rec, const VisRec* __restrict__ crec, uint32_t ib, uint32_t cb,
                                     uint32_t ninl, uint32_t p) {
@@ -1148,26 +1147,8 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 				v.invNx = nx > 0 ? 1.0F / static_cast<float>(nx) : 0.0F;
 				v.scale = r.scale; v.z0 = r.z0; v.z1 = r.z1; v.z2 = r.z2; v.slot = r.slot;
 				nq = static_cast<uint32_t>(nx) * static_cast<uint32_t>(ny); }
-			// small triangles: the thread walks its own quads row by row,
-			// stepping the edge values by wrapping adds (equal to the
-			// multiplied-out steps modulo 2^32)
-			const bool coop = nq > static_cast<uint32_t>(kThreadQuads);
-			if (!coop && nq != 0u) {
-				const int32_t sx1 = shl(v.dy12, 1), sx2 = shl(v.dy23, 1), sx3 = shl(v.dy31, 1);
-				const int32_t sy1 = shl(v.dx12, 1), sy2 = shl(v.dx23, 1), sy3 = shl(v.dx31, 1);
-				int32_t r1 = v.c1, r2 = v.c2, r3 = v.c3, q1 = r1, q2 = r2, q3 = r3;
-				int lx = v.x0, ly = v.y0;
-				uint32_t kx = 0;
-				for (uint32_t k = 0; k < nq; ++k) {
-					vis_quad(keys, v, q1, q2, q3, lx, ly, leq, frags);
-					if (++kx == v.nx) {
-						kx = 0; lx = v.x0; ly += 2;
-						r1 = wadd(r1, sy1); r2 = wadd(r2, sy2); r3 = wadd(r3, sy3);
-						q1 = r1; q2 = r2; q3 = r3; }
-					else {
-						lx += 2;
-						q1 = wadd(q1, sx1); q2 = wadd(q2, sx2); q3 = wadd(q3, sx3); }}}
-			// larger triangles: their quads are dealt out over the block
+			// the triangles' quads are dealt out over the block
+			const bool coop = nq != 0u;
 			if (coop) ts[threadIdx.x] = v;
 			// 24-bit step products are exact when every dealt-out triangle's
 			// steps fit 23 bits (nearly always); decided for the whole batch
