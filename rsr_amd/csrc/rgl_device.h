@@ -189,7 +189,7 @@ struct BatchDesc {
 	int nTiles;              // sum of the frames' tiles
 	int nDraws;
 	int units;               // binning units (unitTris triangles each)
-	int unitTris;            // triangles per binning unit, a multiple of 64
+	int unitTris;            // triangles per binning unit (kUnitTris)
 	long long totalTris;     // S
 	int nBlocks;             // resolve blocks, all frames
 	int segPairs;            // (triangle, tile) pairs per visibility work item

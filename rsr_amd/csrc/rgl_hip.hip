@@ -141,10 +141,10 @@ struct rgl_gpu {
 	// device buffers
 	DevArray<float4> vpos; DevArray<uint8_t> vflags; DevArray<float> vvary;
 	DevArray<uint2> trect; DevArray<uint4> tvert;
-	DevArray<uint32_t> hist, bsum, tileOff;
+	DevArray<uint32_t> tileCount, tileCursor, tileOff;
 	DevArray<uint32_t> clipq, clipqOf; DevArray<float4> cvpos; DevArray<float> cvvary;
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
-	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost, colTotal;
+	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost;
 	DevArray<VisTile> tileDesc;
 	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
@@ -201,7 +201,7 @@ struct rgl_gpu {
 		evUsed[slot] = false; }
 
 	~rgl_gpu() {
-		for (auto* a : { &qCount, &hist, &bsum, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost, &colTotal }) a->release();
+		for (auto* a : { &qCount, &tileCount, &tileCursor, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
 		tileDesc.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
@@ -403,10 +403,8 @@ struct rgl_gpu {
 		if ((e = Grow(vvary, (vtot + 1) * kMaxVary)) != hipSuccess) return hipfail(e, "hipMalloc vvary");
 		if ((e = Grow(trect, S)) != hipSuccess) return hipfail(e, "hipMalloc trect");
 		if ((e = Grow(tvert, S)) != hipSuccess) return hipfail(e, "hipMalloc tvert");
-		const long long nh = static_cast<long long>(b.nTiles) * std::max(b.units, 1) + 1;
-		const long long nChunks = unit_chunks(std::max(b.units, 1));
-		if ((e = Grow(hist, nh)) != hipSuccess) return hipfail(e, "hipMalloc hist");
-		if ((e = Grow(bsum, nChunks * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc column sums");
+		if ((e = Grow(tileCount, static_cast<long long>(kCountGroups) * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile counts");
+		if ((e = Grow(tileCursor, static_cast<long long>(kCountGroups) * b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile cursors");
 		if ((e = Grow(tileOff, b.nTiles + 1)) != hipSuccess) return hipfail(e, "hipMalloc tile offsets");
 		if (listCap < 2 * S + 4096) listCap = static_cast<uint32_t>(std::min<long long>(2 * S + 4096, 0x7fffffffLL));
 		if ((e = Grow(rec, listCap)) != hipSuccess) return hipfail(e, "hipMalloc setup records");
@@ -429,7 +427,6 @@ struct rgl_gpu {
 		if ((e = Grow(itemTile, itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
 		if ((e = Grow(tileDesc, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
 		if ((e = Grow(tileCost, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
-		if ((e = Grow(colTotal, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc column totals");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
 		if ((e = Grow(keys, px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
@@ -497,12 +494,8 @@ struct rgl_gpu {
 			if (!recs[i]->sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples are stored from single-frame runs");
 		BatchDesc b;
 		b.nFrames = n; b.nTiles = tileBase; b.nDraws = static_cast<int>(draws.size());
-		// binning units: kUnitTris triangles, grown (in multiples of 64) when
-		// the units x tiles histogram would exceed kHistBudget entries
-		long long ut = kUnitTris;
-		if (tileBase > 0 && (triBase + ut - 1) / ut * tileBase > kHistBudget)
-			ut = ((triBase * tileBase / kHistBudget + 1 + 63) / 64) * 64;
-		b.totalTris = triBase; b.unitTris = static_cast<int>(std::min<long long>(ut, 1 << 20));
+		// binning units of kUnitTris triangles
+		b.totalTris = triBase; b.unitTris = kUnitTris;
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
 		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
 		long long vtot = 0;
@@ -585,7 +578,7 @@ struct rgl_gpu {
 		mix(&b, sizeof(b));
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
-		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, hist.p, bsum.p, tileOff.p, clipq.p, clipqOf.p,
+		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
 		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
@@ -625,8 +618,7 @@ struct rgl_gpu {
 	           const std::vector<SampleReq>& reqs) {
 		hipError_t e;
 		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
-		const long long nCsum = static_cast<long long>(unit_chunks(b.units)) * b.nTiles;
-		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, bsum.p, nCsum, qCount.p };
+		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, tileCount.p, qCount.p };
 		Mark(0);
 		// vertex and clip stages: one launch per program present in the batch
 		bool present[5] = { false, false, false, false, false };
@@ -642,11 +634,11 @@ struct rgl_gpu {
 			if (present[2]) { k_vertex<ProgLambertTex><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
 			if (present[3]) { k_vertex<ProgManyBlinnPhong><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
 			if (present[4]) { k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }}
-		if (zeroHere) k_zero_counters<<<std::max(8, static_cast<int>(std::min<long long>((nCsum + 255) / 256, 256))), 256, 0, stream>>>(za);
+		if (zeroHere) k_zero_counters<<<std::max(8, std::min((kCountGroups * b.nTiles + 255) / 256, 256)), 256, 0, stream>>>(za);
 		Mark(1);
 		if (b.units > 0) {
 			k_setup_count<<<b.units, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
-				hist.p, bsum.p, clipq.p, clipqOf.p, clipCap, dCtr); }
+				tileCount.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
 		// clipping: the clip queue is complete after setup; the clipped
 		// per-tile counts feed the scan below
@@ -656,15 +648,8 @@ struct rgl_gpu {
 		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
-		const int nChunks = unit_chunks(b.units);
-		// many (chunk, tile) sums: take the column prefixes over many workgroups
-		const bool colSplit = static_cast<long long>(nChunks) * b.nTiles > kColSplit;
-		if (colSplit) k_col_sum<<<(b.nTiles + 255) / 256, 256, 0, stream>>>(bsum.p, nChunks, b.nTiles, colTotal.p);
-		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, bsum.p, nChunks, tileOff.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, itemCap,
-			tileDesc.p, listCap, clistCap, tileCost.p, colSplit ? colTotal.p : nullptr, dCtr);
-		if (b.units > 0) {
-			const dim3 cg((b.nTiles + 63) / 64, nChunks);
-			k_col_down<<<cg, 64, 0, stream>>>(hist.p, b.nTiles, b.units, bsum.p, tileOff.p); }
+		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipHist.p, clipOff.p, clipCursor.p,
+			itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
 		Mark(4);
 		RasterArgs ra;
 		ra.draws = dDraws.p; ra.lite = dLite.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
@@ -676,7 +661,7 @@ struct rgl_gpu {
 		ra.hostCtr = learned ? nullptr : hCtrDev;
 		if (b.units > 0) {
 			ScatterArgs sa;
-			sa.trect = trect.p; sa.tvert = tvert.p; sa.vpos = vpos.p; sa.off = hist.p; sa.rec = rec.p; sa.listCap = listCap;
+			sa.trect = trect.p; sa.tvert = tvert.p; sa.vpos = vpos.p; sa.cursor = tileCursor.p; sa.rec = rec.p; sa.listCap = listCap;
 			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
 			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
 			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
@@ -720,7 +705,6 @@ struct rgl_gpu {
 
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
 	static constexpr int kResetBlocks = 128;        // k_scatter blocks that reset the keys of multi-segment tiles
-	static constexpr long long kColSplit = 100000;   // (chunk, tile) sums above which k_col_sum runs first
 
 	static int ProgramSlot(int pid) {
 		switch (pid) {

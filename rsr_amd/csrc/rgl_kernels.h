@@ -19,18 +19,16 @@
 
 namespace rgl {
 
-constexpr int kUnitTris = 256;
-constexpr int kUnitThreads = 256;   // threads per binning-unit workgroup (setup/count and scatter)      // triangles per binning unit (one wave), the minimum
-constexpr long long kHistBudget = 8LL << 20;   // units x tiles histogram entries before units grow
-constexpr int kColChunk = 32;       // binning units per column-scan chunk
+constexpr int kUnitTris = 256;      // triangles per binning unit
+constexpr int kUnitThreads = 256;   // threads per binning-unit workgroup (setup/count and scatter)
 constexpr int kXcds = 8;             // workgroups are dealt round-robin over the eight XCDs
 
-// Column-scan chunks group the binning units that run on one XCD: chunk c
-// holds units x + 8 (32 g + k), k < 32, with x = c % 8 and g = c / 8, so
-// that a chunk's count atomics all come from one XCD's L2.
-__host__ __device__ inline int unit_chunk(int u) { return (u / (kXcds * kColChunk)) * kXcds + u % kXcds; }
-__host__ __device__ inline int unit_chunks(int units) { return ((units + kXcds * kColChunk - 1) / (kXcds * kColChunk)) * kXcds; }
-__host__ __device__ inline int chunk_unit(int c, int k) { return (c % kXcds) + kXcds * ((c / kXcds) * kColChunk + k); }
+// Per-tile pair counts are kept in kCountGroups groups: unit u adds its
+// counts into group u % kCountGroups and later takes its list positions
+// from the same group's cursor, so a count address only takes atomics from
+// the units of one XCD.  The scan lays a tile's groups out one after the
+// other in its list.
+constexpr int kCountGroups = kXcds;
 constexpr int kRasterThreads = 256;
 constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
 constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
@@ -118,17 +116,17 @@ __device__ inline void load_block_bases(int* fbb, const BatchDesc& b) {
 // ---------------------------------------------------------------------------
 // one launch covers every draw of the batch that uses program P; a thread
 // finds its draw by the draws' first vertex-cache entries
-// the frame's start-of-frame clears: counters, clip histogram, scan chunk
-// sums and network queue counters (run by the first vertex launch, or by
-// k_zero_counters when a frame has no vertices)
+// the frame's start-of-frame clears: counters, clip histogram, per-tile
+// pair counts and network queue counters (run by the first vertex launch,
+// or by k_zero_counters when a frame has no vertices)
 struct ZeroArgs {
-	Counters* ctr; uint32_t* clipHist; int nTiles; uint32_t* csum; long long nCsum; uint32_t* qcount;
+	Counters* ctr; uint32_t* clipHist; int nTiles; uint32_t* tileCount; uint32_t* qcount;
 };
 
 __device__ inline void zero_frame(const ZeroArgs& z) {
 	const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
 	const long long n = static_cast<long long>(gridDim.x) * blockDim.x;
-	for (long long c = i; c < z.nCsum; c += n) z.csum[c] = 0u;
+	for (long long c = i; c < static_cast<long long>(kCountGroups) * z.nTiles; c += n) z.tileCount[c] = 0u;
 	for (long long c = i; c < static_cast<long long>(sizeof(Counters) / 4); c += n) reinterpret_cast<uint32_t*>(z.ctr)[c] = 0u;
 	for (long long c = i; c < 8 * 32; c += n) z.qcount[c] = 0u;
 	for (long long t = i; t < z.nTiles; t += n) z.clipHist[t] = 0u; }
@@ -187,7 +185,7 @@ __device__ inline const uint2 empty_rect() { return make_uint2(0x0000ffffu, 0x00
 
 __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __restrict__ draws, BatchDesc b,
 		const float4* __restrict__ vpos, const uint8_t* __restrict__ vflags,
-		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ hist, uint32_t* __restrict__ csum,
+		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ tileCount,
 		uint32_t* __restrict__ clipq, uint32_t* __restrict__ clipqOf, uint32_t clipCap,
 		Counters* __restrict__ ctr) {
 	// dynamic LDS: the batch's draws, then nTiles counters
@@ -269,11 +267,10 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 				if (q < clipCap) { clipq[q] = static_cast<uint32_t>(s); clipqOf[s] = q; }
 				else atomicOr(&ctr->overflow, 2u); }}}
 	__syncthreads();
-	uint32_t* cs = csum + static_cast<long long>(unit_chunk(u)) * b.nTiles;
+	uint32_t* tc = tileCount + static_cast<long long>(u % kCountGroups) * b.nTiles;
 	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) {
 		const uint32_t c = lhist[t];
-		hist[static_cast<long long>(u) * b.nTiles + t] = c;
-		if (c) atomicAdd(&cs[t], c); }
+		if (c) atomicAdd(&tc[t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
 	if ((threadIdx.x & 63) == 0 && binned) atomicAdd(&ctr->slot[blockIdx.x % kCtrSlots].binned, binned); }
@@ -302,18 +299,14 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32
 	__syncthreads();
 	return r; }
 
-// The histogram is unit-major, hist[u * nTiles + t] = unit u's pairs in tile
-// t, so that each unit writes (and the scatter reads) one contiguous row.
-// The list is tile-major: tile t's pairs grouped by unit, in chunk order.
-// Columns are cut into chunks of kColChunk units (unit_chunk); k_setup_count
-// also adds its nonzero counts into csum[chunk * nTiles + t].  The column scan turns
-// those chunk sums into within-tile prefixes and writes tileOff[t] = tile
-// t's first list position (tileOff[nTiles] = the total); the down pass
-// turns the counts in place into list positions.  A thread owns one tile
-// column, so a wave's loads and stores cover contiguous bytes.
-constexpr int kColBatch = 32;   // independent loads in flight per thread (down pass: a whole chunk)
-constexpr int kScanBatch = 64;  // chunk sums loaded at once per tile column (column scan)
-constexpr int kScanThreads = 512; // column-scan block: one tile column per thread, registers for kScanBatch loads
+// The list is tile-major: tile t's inline pairs take list positions
+// [tileOff[t], tileOff[t + 1]), laid out by count group (the units u with
+// u % kCountGroups == g take one contiguous block, tileCursor[g * nTiles + t]
+// being its start).  Within a group, each unit reserves one block per tile
+// with an atomic add on the group's cursor, so the order of units inside a
+// tile's list is not fixed; the visibility result does not depend on it
+// (every fragment competes by its depth and submission slot).
+constexpr int kScanThreads = 512; // scan block: one tile per thread
 
 // Visibility work items of a tile: its inline + clipped pairs are cut into
 // segments of segPairs, and the tile into regions: the four 32 x 32
@@ -344,14 +337,13 @@ struct alignas(16) VisTile {
 };
 static_assert(sizeof(VisTile) == 32, "two 16-byte loads");
 
-// One block: chunk prefixes within each column, then per tile the list
-// offset (exclusive scan of the column totals), the clipped-list offset
-// and cursor (scan of clipHist) and the tile's visibility work items.
-__global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t* __restrict__ csum, int nChunks,
-		uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ clipOff,
-		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap,
-		VisTile* __restrict__ tileDesc,
-		uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost, const uint32_t* __restrict__ colTotal,
+// One block: per tile the list offset (exclusive scan of the tiles' pair
+// counts) and the count groups' cursors, the clipped-list offset and cursor
+// (scan of clipHist) and the tile's visibility work items.
+__global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const uint32_t* __restrict__ tileCount,
+		uint32_t* __restrict__ tileCursor, uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist,
+		uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap,
+		VisTile* __restrict__ tileDesc, uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost,
 		Counters* __restrict__ ctr) {
 	__shared__ uint32_t sh[kScanThreads / 64 + 1];
 	__shared__ int ftb[kMaxFrames + 1];
@@ -362,17 +354,11 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 	for (int base = 0; base < nT; base += kScanThreads) {
 		const int t = base + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
-		if (t < nT && colTotal) col = colTotal[t];   // chunk prefixes already taken by k_col_sum
-		else if (t < nT) {
-			for (int c0 = 0; c0 < nChunks; c0 += kScanBatch) {
-				uint32_t w[kScanBatch];
+		uint32_t grp[kCountGroups];
 #pragma unroll
-				for (int j = 0; j < kScanBatch; ++j)
-					w[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
-#pragma unroll
-				for (int j = 0; j < kScanBatch; ++j) {
-					if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
-					col += w[j]; }}}
+		for (int g = 0; g < kCountGroups; ++g) {
+			grp[g] = t < nT ? tileCount[static_cast<long long>(g) * nT + t] : 0u;
+			col += grp[g]; }
 		if (t < nT) {
 			v = clipHist[t];
 			cost = tileCost[t];
@@ -385,6 +371,9 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
 		if (t < nT) {
 			tileOff[t] = carry + ex;
+			uint32_t run = carry + ex;
+#pragma unroll
+			for (int g = 0; g < kCountGroups; ++g) { tileCursor[static_cast<long long>(g) * nT + t] = run; run += grp[g]; }
 			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
 			VisTile d;
 			d.item0 = items;   // the item count until the ordering below
@@ -425,40 +414,6 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, uint32_t
 	if (threadIdx.x == 0) {
 		tileOff[nT] = carry; clipOff[nT] = ccarry;
 		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
-
-// Large batches take the chunk prefixes with one thread per tile column
-// over many workgroups before the single-block scan (which then reads each
-// column's total).
-__global__ __launch_bounds__(256) void k_col_sum(uint32_t* __restrict__ csum, int nChunks, int nT, uint32_t* __restrict__ colTotal) {
-	const int t = blockIdx.x * 256 + threadIdx.x;
-	if (t >= nT) return;
-	uint32_t col = 0;
-	for (int c0 = 0; c0 < nChunks; c0 += kColBatch) {
-		uint32_t w[kColBatch];
-#pragma unroll
-		for (int j = 0; j < kColBatch; ++j) w[j] = c0 + j < nChunks ? csum[static_cast<long long>(c0 + j) * nT + t] : 0u;
-#pragma unroll
-		for (int j = 0; j < kColBatch; ++j) {
-			if (c0 + j < nChunks) csum[static_cast<long long>(c0 + j) * nT + t] = col;
-			col += w[j]; }}
-	colTotal[t] = col; }
-
-__global__ __launch_bounds__(64) void k_col_down(uint32_t* __restrict__ hist, int nT, int U, const uint32_t* __restrict__ csum,
-                                                 const uint32_t* __restrict__ tileOff) {
-	const int t = blockIdx.x * 64 + threadIdx.x, uc = blockIdx.y;
-	if (t >= nT) return;
-	uint32_t run = tileOff[t] + csum[static_cast<long long>(uc) * nT + t];
-	for (int k0 = 0; k0 < kColChunk; k0 += kColBatch) {
-		uint32_t v[kColBatch];
-#pragma unroll
-		for (int j = 0; j < kColBatch; ++j) {
-			const int u = chunk_unit(uc, k0 + j);
-			v[j] = u < U ? hist[static_cast<long long>(u) * nT + t] : 0u; }
-#pragma unroll
-		for (int j = 0; j < kColBatch; ++j) {
-			const int u = chunk_unit(uc, k0 + j);
-			if (u < U) hist[static_cast<long long>(u) * nT + t] = run;
-			run += v[j]; }}}
 
 // ---------------------------------------------------------------------------
 // clipping: GPUBinImpl::ClipTriangles (rglv_gpu_impl.hxx:677-793),
@@ -764,7 +719,7 @@ __device__ inline Rect draw_tile_rect(const DrawLite& d, int tx, int ty) {
 
 struct ScatterArgs {
 	const uint2* trect; const uint4* tvert; const float4* vpos;
-	const uint32_t* off;          // per (unit, tile) first list position, unit-major
+	uint32_t* cursor;             // per (count group, tile) next free list position
 	VisRec* rec; uint32_t listCap;
 	// clipped fans (blocks [units, units + clipBlocks))
 	const uint32_t* clipq; const uint8_t* fcount; const uint2* frect; const float4* cvpos; uint32_t clipCap;
@@ -811,12 +766,14 @@ __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, in
 			const int ly = i / rw, lx = i - ly * rw;
 			a.keys[f.keyBase + static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}}
 
-// blocks [0, units): one binning unit each, one wave; each lane walks its
-// triangle's tiles, takes a list position from the unit's LDS cursor of
-// the tile and writes the pair's setup record.  Blocks [units, units +
-// clipBlocks): the clipped fans.  The rest: key resets.
+// blocks [0, units): one binning unit each.  The unit counts its pairs per
+// tile in LDS, reserves one block of list positions per tile it touches
+// from its count group's cursor, then each lane walks its triangle's
+// tiles, takes a position from the unit's LDS cursor of the tile and
+// writes the pair's setup record.  Blocks [units, units + clipBlocks): the
+// clipped fans.  The rest: key resets.
 __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const DrawLite* __restrict__ dl, ScatterArgs a) {
-	// dynamic LDS: nTiles write cursors; the draw table is read through the caches
+	// dynamic LDS: nTiles counts, then write cursors; the draw table is read through the caches
 	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
 	uint32_t* base = reinterpret_cast<uint32_t*>(dyn_lds);
@@ -829,10 +786,25 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 	if (u >= b.units) {
 		clip_scatter(b, dl, u - b.units, a);
 		return; }
-	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) base[t] = a.off[static_cast<long long>(u) * b.nTiles + t];
+	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) base[t] = 0u;
+	__syncthreads();
+	const long long s0 = static_cast<long long>(u) * b.unitTris;
+	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
+		const long long s = s0 + r;
+		if (s >= b.totalTris) break;
+		const uint2 rc = a.trect[s];
+		const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+		if (tx0 > tx1 || ty0 > ty1) continue;
+		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
+		for (int ty = ty0; ty <= ty1; ++ty)
+			for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&base[d.tileBase + ty * d.tilesX + tx], 1u); }
+	__syncthreads();
+	uint32_t* cur = a.cursor + static_cast<long long>(u % kCountGroups) * b.nTiles;
+	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) {
+		const uint32_t c = base[t];
+		if (c) base[t] = atomicAdd(&cur[t], c); }
 	__syncthreads();
 	bool ovf = false;
-	const long long s0 = static_cast<long long>(u) * b.unitTris;
 	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
 		const long long s = s0 + r;
 		if (s >= b.totalTris) break;
