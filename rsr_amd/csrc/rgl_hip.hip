@@ -670,7 +670,7 @@ struct rgl_gpu {
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;
 			// the kernel strides over any remainder
-			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p, tileDesc.p); }
+			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p, tileDesc.p, b.frames); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		if (learned) {
@@ -678,12 +678,12 @@ struct rgl_gpu {
 			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
 		else {
 			switch (pid) {
-			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, 0, stream>>>(b, ra); break;
-			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, 0, stream>>>(b, ra); break;
-			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, 0, stream>>>(b, ra); break;
-			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, 0, stream>>>(b, ra); break;
-			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, 0, stream>>>(b, ra); break;
-			default: k_resolve<ShadeAny><<<rblocks, 256, 0, stream>>>(b, ra); break; }}
+			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
+			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
+			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
+			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
+			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
+			default: k_resolve<ShadeAny><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break; }}
 		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);

@@ -1,7 +1,7 @@
 // rgl_kernels.h — the gfx950 kernels of one rgl frame.
 //
-// Frame = vertex -> setup/count -> scan -> ordered scatter -> clip ->
-// clip count/scan/scatter -> per-tile raster + resolve.
+// Frame = vertex -> setup/count -> clip -> scan -> scatter (pair setup
+// records, clipped fans, key resets) -> visibility per tile -> resolve.
 //
 // Visibility is computed order-independently: for the states this pipeline
 // accepts (depth test LESS or LEQUAL with depth writes, no blending,
@@ -948,18 +948,11 @@ struct ShadeAny {
 // ---------------------------------------------------------------------------
 constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
 
-// the winner of pixel (x, y), shaded: colour before the post program,
-// depth and primitive id (clear values where nothing was drawn)
+// the winner of pixel (x, y) drawn by draw d (table entry l), shaded:
+// colour before the post program, depth and primitive id
 template <class S>
-__device__ inline bool resolve_key(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
-                                   unsigned long long k, const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
-	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
-	dep = f.clearDepth;
-	prim = 0xffffffffu;
-	if (!(k < clear_key(f))) return false;
-	const uint32_t slot = key_slot(k, f.depthFunc == 1);
-	const int di = find_draw_slot(dl, b.nDraws, slot);
-	const DrawLite& d = dl[di];
+__device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, const DrawLite& d, const DrawDesc& dd,
+                                    int x, int y, uint32_t slot, const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
 	const uint32_t rel = slot - d.slotBase;
 	const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
 	const int tx = x / f.tileW, ty = y / f.tileH;
@@ -985,14 +978,36 @@ __device__ inline bool resolve_key(const BatchDesc& b, const FrameDesc& f, const
 		vary = a.cvvary; }
 	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
 	const int lane = (x & 1) | ((y & 1) << 1);
-	S::shade(a.draws[di], t, tr, vary, kx, ky, lane, rcpTab, col, dep);
-	prim = d.primBase + srcTri;
+	S::shade(dd, t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+	prim = d.primBase + srcTri; }
+
+// the winner of pixel (x, y), shaded (clear values where nothing was
+// drawn).  A batch of one draw reads that draw's table entries at a fixed
+// index: from restrict tables they are scalar loads, so the draw's
+// uniforms and texture descriptors sit in scalar registers before the
+// per-pixel gathers return.
+template <class S>
+__device__ inline bool resolve_key(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl,
+                                   const DrawDesc* draws, int x, int y, unsigned long long k, const uint32_t* rcpTab,
+                                   float col[3], float& dep, uint32_t& prim) {
+	col[0] = f.clearColor[0]; col[1] = f.clearColor[1]; col[2] = f.clearColor[2];
+	dep = f.clearDepth;
+	prim = 0xffffffffu;
+	if (!(k < clear_key(f))) return false;
+	const uint32_t slot = key_slot(k, f.depthFunc == 1);
+	if (b.nDraws == 1) {
+		const DrawLite d = dl[0];
+		const DrawDesc dd = draws[0];
+		resolve_draw<S>(f, a, d, dd, x, y, slot, rcpTab, col, dep, prim); }
+	else {
+		const int di = find_draw_slot(dl, b.nDraws, slot);
+		resolve_draw<S>(f, a, dl[di], draws[di], x, y, slot, rcpTab, col, dep, prim); }
 	return true; }
 
 template <class S>
 __device__ inline bool resolve_pixel(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
                                      const uint32_t* rcpTab, float col[3], float& dep, uint32_t& prim) {
-	return resolve_key<S>(b, f, a, dl, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], rcpTab, col, dep, prim); }
+	return resolve_key<S>(b, f, a, dl, a.draws, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], rcpTab, col, dep, prim); }
 
 // StoreTrueColor's ShadeCanvas + sRGB encode of one pixel (FilterTile,
 // rglr_algorithm.hxx:31-104)
@@ -1004,11 +1019,12 @@ __device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], co
 
 // resolve one pixel from its key and store colour, depth and primitive id
 template <class S>
-__device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
-                                     unsigned long long k, const uint32_t* rcpTab, const uint32_t* srgbTab) {
+__device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl,
+                                     const DrawDesc* draws, int x, int y, unsigned long long k, const uint32_t* rcpTab,
+                                     const uint32_t* srgbTab) {
 	float col[3], dep;
 	uint32_t prim;
-	resolve_key<S>(b, f, a, dl, x, y, k, rcpTab, col, dep, prim);
+	resolve_key<S>(b, f, a, dl, draws, x, y, k, rcpTab, col, dep, prim);
 	const long long o = static_cast<long long>(y) * f.W + x;
 	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
 	if (f.outDepth) f.outDepth[o] = dep;
@@ -1033,29 +1049,32 @@ __device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
 // the frame table are read through the scalar and vector caches, so a
 // workgroup starts on its pixels at once (it runs for a few microseconds).
 template <class S>
-__global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a) {
-	// the frame's counters are final (no later kernel changes them): block 0
-	// publishes them to the pinned host copy
-	if (blockIdx.x == 0 && a.hostCtr)
-		for (int i = threadIdx.x; i < static_cast<int>(sizeof(Counters) / 4); i += 256)
-			reinterpret_cast<uint32_t*>(a.hostCtr)[i] = reinterpret_cast<const uint32_t*>(a.ctr)[i];
+__global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a, const FrameDesc* __restrict__ frames,
+                                                  const DrawLite* __restrict__ lite, const DrawDesc* __restrict__ draws) {
+	// the frame and draw tables are read-only (restrict): uniform reads of
+	// them are scalar loads
 	const int blk = static_cast<int>(blockIdx.x);
 	int lo = 0, hi = b.nFrames - 1;   // last frame whose first block is at or before blk
-	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (b.frames[mid].blockBase <= blk) lo = mid; else hi = mid - 1; }
-	const FrameDesc& f = b.frames[lo];
+	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (frames[mid].blockBase <= blk) lo = mid; else hi = mid - 1; }
+	const FrameDesc f = frames[lo];
 	const int lb = blk - f.blockBase;
 	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
 	const int x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
 	const int y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
-	if (x >= f.W || y >= f.H) return;
-	resolve_store<S>(b, f, a, a.lite, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], kRcpMantissa, kSrgb8Tab); }
+	if (x < f.W && y < f.H)
+		resolve_store<S>(b, f, a, lite, draws, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], kRcpMantissa, kSrgb8Tab);
+	// the frame's counters are final (no later kernel changes them): block 0
+	// publishes them to the pinned host copy
+	if (blockIdx.x == 0 && a.hostCtr)
+		for (int i = threadIdx.x; i < static_cast<int>(sizeof(Counters) / 4); i += 256)
+			reinterpret_cast<uint32_t*>(a.hostCtr)[i] = reinterpret_cast<const uint32_t*>(a.ctr)[i]; }
 
 // visibility work items run one per workgroup; the grid is sized from the
 // previous frame's item count and strides over any remainder, so no
 // work-queue counter is shared between workgroups
 __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs a, const VisRec* __restrict__ rec,
                                                        const VisRec* __restrict__ crec, const uint32_t* __restrict__ itemTile,
-                                                       const VisTile* __restrict__ tileDesc) {
+                                                       const VisTile* __restrict__ tileDesc, const FrameDesc* __restrict__ frames) {
 	__shared__ unsigned long long keys[kSubDim * kSubDim];
 	__shared__ VisSetup ts[kRasterThreads];
 	__shared__ uint32_t qoff[kRasterThreads + 1];
@@ -1070,7 +1089,8 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		const uint32_t local = item - it.item0;
 		const uint32_t nreg = it.nreg, seg = local / nreg, reg = local - seg * nreg;
 		const bool multiSeg = it.multiSeg != 0u;
-		const FrameDesc& f = b.frames[it.frame];
+		// the frame table is read-only (restrict): its fields come by scalar loads
+		const FrameDesc& f = frames[it.frame];
 		const bool leq = f.depthFunc == 1;
 		const Rect R = tile_rect(f, tile - f.tileBase);
 		// the item's region: a quadrant, or its half or quarter
