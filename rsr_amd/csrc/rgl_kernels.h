@@ -983,7 +983,8 @@ __device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, con
 
 // the winner of pixel (x, y), shaded (clear values where nothing was
 // drawn).  A batch of one draw reads that draw's table entries at a fixed
-// index: from restrict tables they are scalar loads, so the draw's
+// index, and a wave whose pixels share one draw reads them by one lane's
+// index: from restrict tables these are scalar loads, so the draw's
 // uniforms and texture descriptors sit in scalar registers before the
 // per-pixel gathers return.
 template <class S>
@@ -1000,8 +1001,14 @@ __device__ inline bool resolve_key(const BatchDesc& b, const FrameDesc& f, const
 		const DrawDesc dd = draws[0];
 		resolve_draw<S>(f, a, d, dd, x, y, slot, rcpTab, col, dep, prim); }
 	else {
+		// a wave whose pixels share one draw reads it by one lane's index
 		const int di = find_draw_slot(dl, b.nDraws, slot);
-		resolve_draw<S>(f, a, dl[di], draws[di], x, y, slot, rcpTab, col, dep, prim); }
+		const int d0 = __builtin_amdgcn_readfirstlane(di);
+		if (__all(di == d0)) {
+			const DrawLite d = dl[d0];
+			const DrawDesc dd = draws[d0];
+			resolve_draw<S>(f, a, d, dd, x, y, slot, rcpTab, col, dep, prim); }
+		else resolve_draw<S>(f, a, dl[di], draws[di], x, y, slot, rcpTab, col, dep, prim); }
 	return true; }
 
 template <class S>
