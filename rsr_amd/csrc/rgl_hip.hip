@@ -146,6 +146,7 @@ struct rgl_gpu {
 	DevArray<uint8_t> fcount; DevArray<uint2> frect;
 	DevArray<uint32_t> clipHist, clipOff, clipCursor, itemTile, tileCost;
 	DevArray<VisTile> tileDesc;
+	DevArray<ScanPart> scanPart;
 	size_t costReady = 0;          // tileCost entries cleared since allocation
 	uint32_t itemCap = 0;
 	DevArray<unsigned long long> keys;
@@ -202,7 +203,7 @@ struct rgl_gpu {
 
 	~rgl_gpu() {
 		for (auto* a : { &qCount, &tileCount, &tileCursor, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
-		tileDesc.release();
+		tileDesc.release(); scanPart.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release();
@@ -426,6 +427,7 @@ struct rgl_gpu {
 			static_cast<long long>(kMaxRegions) * (b.nTiles + (static_cast<long long>(listCap) + clistCap) / b.segPairs + 1), 1LL << 30));
 		if ((e = Grow(itemTile, itemCap)) != hipSuccess) return hipfail(e, "hipMalloc item tiles");
 		if ((e = Grow(tileDesc, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile descriptors");
+		if ((e = Grow(scanPart, (b.nTiles + kScanThreads - 1) / kScanThreads + 1)) != hipSuccess) return hipfail(e, "hipMalloc scan parts");
 		if ((e = Grow(tileCost, b.nTiles)) != hipSuccess) return hipfail(e, "hipMalloc tile costs");
 		const size_t px = static_cast<size_t>(std::max<long long>(totalPx, 1));
 		if (px > keys.cap) keysReady = 0;
@@ -579,7 +581,7 @@ struct rgl_gpu {
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
-		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, tileCost.p,
+		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, scanPart.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -648,7 +650,13 @@ struct rgl_gpu {
 		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
 		Mark(3);
-		k_col_scan<<<1, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipHist.p, clipOff.p, clipCursor.p,
+		if (b.nTiles > kScanSplitTiles) {
+			const int nb = (b.nTiles + kScanThreads - 1) / kScanThreads;
+			k_scan_local<<<nb, kScanThreads, 0, stream>>>(b, tileCount.p, clipHist.p, tileCost.p, tileDesc.p, scanPart.p);
+			k_scan_parts<<<1, kScanThreads, 0, stream>>>(scanPart.p, nb, b.nTiles, tileOff.p, clipOff.p, itemCap, dCtr);
+			k_scan_final<<<nb, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipOff.p, clipCursor.p,
+				itemTile.p, itemCap, tileDesc.p, listCap, clistCap, scanPart.p, nb); }
+		else k_col_scan<<<1, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipHist.p, clipOff.p, clipCursor.p,
 			itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
 		Mark(4);
 		RasterArgs ra;

@@ -415,6 +415,119 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 		tileOff[nT] = carry; clipOff[nT] = ccarry;
 		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
 
+// Large batches scan in three launches instead of one block: per block of
+// kScanThreads tiles the local prefixes and per-class item counts
+// (k_scan_local), the blocks' bases, one wave per column (k_scan_parts),
+// then every tile's offsets, cursors and items (k_scan_final).  Every
+// choice of work-item order within a cost class is valid, so a tile's
+// first item is taken from its block's LDS cursor of its class.
+constexpr int kCostClasses = 33;   // cost 0, then [2^(c-1), 2^c) for c = 1 .. 32
+constexpr int kScanSplitTiles = 2 * kScanThreads;   // tiles above which the scan takes three launches
+
+struct ScanPart {
+	uint32_t col, clip;            // block totals, then the block's exclusive bases
+	uint32_t cls[kCostClasses];    // items per cost class, then the block's offset within the class
+	uint32_t pad[3];
+};
+
+__device__ inline int cost_class(uint32_t cost) { return cost ? 32 - __clz(cost) : 0; }
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_local(BatchDesc b, const uint32_t* __restrict__ tileCount,
+		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ tileCost, VisTile* __restrict__ tileDesc,
+		ScanPart* __restrict__ part) {
+	__shared__ uint32_t sh[kScanThreads / 64 + 1];
+	__shared__ uint32_t classItems[kCostClasses];
+	__shared__ int ftb[kMaxFrames + 1];
+	load_tile_bases(ftb, b);
+	for (int c = threadIdx.x; c < kCostClasses; c += kScanThreads) classItems[c] = 0u;
+	__syncthreads();
+	const int nT = b.nTiles;
+	const int t = blockIdx.x * kScanThreads + threadIdx.x;
+	uint32_t col = 0, v = 0;
+	if (t < nT) {
+#pragma unroll
+		for (int g = 0; g < kCountGroups; ++g) col += tileCount[static_cast<long long>(g) * nT + t];
+		v = clipHist[t]; }
+	uint32_t tot, ctot;
+	const uint32_t ex = block_exclusive_scan(col, sh, tot);
+	const uint32_t cex = block_exclusive_scan(v, sh, ctot);
+	if (t < nT) {
+		const uint32_t cost = tileCost[t];
+		tileCost[t] = 0u;
+		const uint32_t nseg = vis_segments(b, col + v), nreg = vis_regions(b, cost, nseg);
+		const int cls = cost_class(cost);
+		atomicAdd(&classItems[cls], nseg * nreg);
+		VisTile d;   // provisional: local prefixes, counts and the cost class
+		d.item0 = nseg * nreg;
+		d.frame = static_cast<uint32_t>(find_base(ftb, b.nFrames, t));
+		d.ib = ex; d.cb = cex; d.ninl = col; d.ntot = v;
+		d.nreg = nreg;
+		d.multiSeg = (nseg > 1 ? 1u : 0u) | (static_cast<uint32_t>(cls) << 1);
+		tileDesc[t] = d; }
+	__syncthreads();
+	ScanPart& P = part[blockIdx.x];
+	for (int c = threadIdx.x; c < kCostClasses; c += kScanThreads) P.cls[c] = classItems[c];
+	if (threadIdx.x == 0) { P.col = tot; P.clip = ctot; }}
+
+// one block: each wave scans whole columns over the blocks (the list and
+// clipped-list totals, and each cost class), heaviest class first; row nb
+// of part receives the class bases
+__global__ __launch_bounds__(kScanThreads) void k_scan_parts(ScanPart* __restrict__ part, int nb, int nT,
+		uint32_t* __restrict__ tileOff, uint32_t* __restrict__ clipOff, uint32_t itemCap, Counters* __restrict__ ctr) {
+	constexpr int nCols = 2 + kCostClasses;
+	__shared__ uint32_t colTot[nCols];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	for (int k = wave; k < nCols; k += kScanThreads / 64) {
+		uint32_t carry = 0;
+		for (int i0 = 0; i0 < nb; i0 += 64) {
+			const int i = i0 + lane;
+			uint32_t* cell = i < nb ? (k == 0 ? &part[i].col : k == 1 ? &part[i].clip : &part[i].cls[k - 2]) : nullptr;
+			const uint32_t v = cell ? *cell : 0u;
+			uint32_t x = v;
+			for (int o = 1; o < 64; o <<= 1) {
+				const uint32_t y = __shfl_up(x, o, 64);
+				if (lane >= o) x += y; }
+			if (cell) *cell = carry + x - v;
+			carry += __shfl(x, 63, 64); }
+		if (lane == 0) colTot[k] = carry; }
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t run = 0;
+		for (int c = kCostClasses - 1; c >= 0; --c) { part[nb].cls[c] = run; run += colTot[2 + c]; }
+		tileOff[nT] = colTot[0]; clipOff[nT] = colTot[1];
+		ctr->pairs = colTot[0]; ctr->clipPairs = colTot[1]; ctr->workItems = min(run, itemCap); ctr->workNext = 0; }}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_final(BatchDesc b, const uint32_t* __restrict__ tileCount,
+		uint32_t* __restrict__ tileCursor, uint32_t* __restrict__ tileOff, uint32_t* __restrict__ clipOff,
+		uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap, VisTile* __restrict__ tileDesc,
+		uint32_t listCap, uint32_t clistCap, const ScanPart* __restrict__ part, int nb) {
+	__shared__ uint32_t classCursor[kCostClasses];
+	const ScanPart& P = part[blockIdx.x];
+	for (int c = threadIdx.x; c < kCostClasses; c += kScanThreads) classCursor[c] = part[nb].cls[c] + P.cls[c];
+	__syncthreads();
+	const int nT = b.nTiles;
+	const int t = blockIdx.x * kScanThreads + threadIdx.x;
+	if (t >= nT) return;
+	VisTile d = tileDesc[t];
+	const uint32_t col = d.ninl, v = d.ntot, items = d.item0;
+	const uint32_t off = P.col + d.ib, coff = P.clip + d.cb;
+	tileOff[t] = off;
+	uint32_t run = off;
+#pragma unroll
+	for (int g = 0; g < kCountGroups; ++g) {
+		tileCursor[static_cast<long long>(g) * nT + t] = run;
+		run += tileCount[static_cast<long long>(g) * nT + t]; }
+	clipOff[t] = coff; clipCursor[t] = coff;
+	const uint32_t item0 = atomicAdd(&classCursor[d.multiSeg >> 1], items);
+	for (uint32_t j = item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t);
+	d.item0 = item0;
+	d.ib = min(off, listCap);
+	d.cb = min(coff, clistCap);
+	d.ninl = min(off + col, listCap) - d.ib;
+	d.ntot = d.ninl + (min(coff + v, clistCap) - d.cb);
+	d.multiSeg &= 1u;
+	tileDesc[t] = d; }
+
 // ---------------------------------------------------------------------------
 // clipping: GPUBinImpl::ClipTriangles (rglv_gpu_impl.hxx:677-793),
 // Sutherland-Hodgman against L, B, N, R, T, fan triangulation, per-tile
