@@ -2,6 +2,7 @@
 // recorder, frame planning (validation, draw descriptors, capacities),
 // kernel launches and the C ABI declared in include/rgl_hip.h.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -189,6 +190,14 @@ struct rgl_gpu {
 	bool evUsed[kRing] = {};
 	int evNext = 0;
 	hipEvent_t* cur = nullptr;
+	// the visibility and resolve kernels of a timed frame carry start and
+	// stop events recorded by their own dispatch, so their times are the
+	// kernels' execution alone (the other stages are timed between events)
+	static constexpr int kDispatchTimed = 2;   // k_vis, k_resolve
+	static constexpr int kFirstDispatchTimed = 5;
+	hipEvent_t kev[kRing][2 * kDispatchTimed] = {};
+	bool kevUsed[kRing][kDispatchTimed] = {};
+	int curSlot = -1;
 	double acc[kNumTimed] = {};
 	int accRuns = 0;
 
@@ -197,9 +206,22 @@ struct rgl_gpu {
 		if (hipEventSynchronize(ev[slot][kNumTimed]) == hipSuccess) {
 			for (int i = 0; i < kNumTimed; ++i) {
 				float ms = 0.0F;
-				if (hipEventElapsedTime(&ms, ev[slot][i], ev[slot][i + 1]) == hipSuccess) acc[i] += ms; }
+				const int k = i - kFirstDispatchTimed;
+				if (k >= 0 && k < kDispatchTimed && kevUsed[slot][k]) {
+					if (hipEventElapsedTime(&ms, kev[slot][2 * k], kev[slot][2 * k + 1]) == hipSuccess) acc[i] += ms; }
+				else if (hipEventElapsedTime(&ms, ev[slot][i], ev[slot][i + 1]) == hipSuccess) acc[i] += ms; }
 			++accRuns; }
-		evUsed[slot] = false; }
+		evUsed[slot] = false;
+		for (bool& u : kevUsed[slot]) u = false; }
+
+	// a kernel launch; in a timed frame the dispatch records start and stop
+	// events of dispatch-timed kernel k
+	template <typename... KArgs, typename... Args>
+	void Dispatch(int k, void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shm, Args... args) {
+		if (cur && curSlot >= 0 && k >= 0) {
+			hipExtLaunchKernelGGL(kernel, grid, block, shm, stream, kev[curSlot][2 * k], kev[curSlot][2 * k + 1], 0, args...);
+			kevUsed[curSlot][k] = true; }
+		else hipLaunchKernelGGL(kernel, grid, block, shm, stream, args...); }
 
 	~rgl_gpu() {
 		for (auto* a : { &qCount, &tileCount, &tileCursor, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
@@ -218,7 +240,8 @@ struct rgl_gpu {
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
 		if (done) (void)hipEventDestroy(done);
-		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
+		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e);
+		for (auto& set : kev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
 
 	int fail(int code, const std::string& msg) { err = msg; return code; }
 	int hipfail(hipError_t e, const char* what) {
@@ -519,10 +542,14 @@ struct rgl_gpu {
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hFrames), sizeof(FrameDesc) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 		cur = nullptr;
+		curSlot = -1;
 		if (timing) {
-			if (!ev[0][0]) for (auto& set : ev) for (auto& x : set) (void)hipEventCreate(&x);
+			if (!ev[0][0]) {
+				for (auto& set : ev) for (auto& x : set) (void)hipEventCreate(&x);
+				for (auto& set : kev) for (auto& x : set) (void)hipEventCreate(&x); }
 			Accumulate(evNext);
 			cur = ev[evNext];
+			curSlot = evNext;
 			evUsed[evNext] = true;
 			evNext = (evNext + 1) % kRing; }
 		if ((r = UploadTable(draws, lastTable, hDraws, dDraws.p)) != 0) return r;
@@ -678,20 +705,24 @@ struct rgl_gpu {
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;
 			// the kernel strides over any remainder
-			k_vis<<<std::min(items, 1 << 20), kRasterThreads, 0, stream>>>(b, ra, rec.p, crec.p, itemTile.p, tileDesc.p, b.frames); }
+			Dispatch(0, k_vis, dim3(std::min(items, 1 << 20)), dim3(kRasterThreads), 0, b, ra,
+			         static_cast<const VisRec*>(rec.p), static_cast<const VisRec*>(crec.p), static_cast<const uint32_t*>(itemTile.p),
+			         static_cast<const VisTile*>(tileDesc.p), b.frames); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
+		const DrawLite* cLite = dLite.p;
+		const DrawDesc* cDraws = dDraws.p;
 		if (learned) {
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
 			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
 		else {
 			switch (pid) {
-			case ProgAmy::kId: k_resolve<ShadeOne<ProgAmy>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
-			case ProgMany::kId: k_resolve<ShadeOne<ProgMany>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
-			case ProgLambertTex::kId: k_resolve<ShadeOne<ProgLambertTex>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
-			case ProgManyBlinnPhong::kId: k_resolve<ShadeOne<ProgManyBlinnPhong>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
-			case ProgGGXEnv::kId: k_resolve<ShadeOne<ProgGGXEnv>><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break;
-			default: k_resolve<ShadeAny><<<rblocks, 256, 0, stream>>>(b, ra, b.frames, dLite.p, dDraws.p); break; }}
+			case ProgAmy::kId: Dispatch(1, k_resolve<ShadeOne<ProgAmy>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
+			case ProgMany::kId: Dispatch(1, k_resolve<ShadeOne<ProgMany>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
+			case ProgLambertTex::kId: Dispatch(1, k_resolve<ShadeOne<ProgLambertTex>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
+			case ProgManyBlinnPhong::kId: Dispatch(1, k_resolve<ShadeOne<ProgManyBlinnPhong>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
+			case ProgGGXEnv::kId: Dispatch(1, k_resolve<ShadeOne<ProgGGXEnv>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
+			default: Dispatch(1, k_resolve<ShadeAny>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break; }}
 		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);
@@ -854,7 +885,7 @@ int rgl_gpu_stats(rgl_gpu* g, rgl_stats* out) {
 
 int rgl_gpu_set_timing(rgl_gpu* g, int enable) {
 	if (check(g)) return RGL_E_ARG;
-	for (int k = 0; k < rgl_gpu::kRing; ++k) g->evUsed[k] = false;
+	for (int k = 0; k < rgl_gpu::kRing; ++k) { g->evUsed[k] = false; for (bool& u : g->kevUsed[k]) u = false; }
 	g->timing = enable != 0;
 	for (double& a : g->acc) a = 0.0;
 	g->accRuns = 0;
