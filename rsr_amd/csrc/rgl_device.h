@@ -194,6 +194,7 @@ struct BatchDesc {
 	int nBlocks;             // resolve blocks, all frames
 	int segPairs;            // (triangle, tile) pairs per visibility work item
 	int splitQuads;          // quad work per quadrant from which it is cut in 2, and from twice this in 4 (0: never)
+	int unitWindow;          // most tiles one binning unit's frames span (its LDS tile table)
 	const FrameDesc* frames; // device table, nFrames entries
 };
 

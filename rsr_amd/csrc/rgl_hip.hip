@@ -519,8 +519,15 @@ struct rgl_gpu {
 			if (!recs[i]->sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples are stored from single-frame runs");
 		BatchDesc b;
 		b.nFrames = n; b.nTiles = tileBase; b.nDraws = static_cast<int>(draws.size());
-		// binning units of kUnitTris triangles
-		b.totalTris = triBase; b.unitTris = kUnitTris;
+		// binning units: kUnitTris triangles, grown in steps of kUnitTris
+		// when the tiles a unit's frames span (its LDS table, cleared and
+		// scanned once per unit) outnumber its triangles by more than 4 to 1
+		b.totalTris = triBase;
+		b.unitTris = kUnitTris;
+		b.unitWindow = UnitWindow(draws, triBase, kUnitTris);
+		if (b.unitWindow > 4 * kUnitTris) {
+			b.unitTris = kUnitTris * std::min(32, b.unitWindow / (4 * kUnitTris));
+			b.unitWindow = UnitWindow(draws, triBase, b.unitTris); }
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
 		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
 		long long vtot = 0;
@@ -591,6 +598,26 @@ struct rgl_gpu {
 		return 0; }
 
 	int Run(hipStream_t s) { rgl_gpu* self = this; return RunBatch(&self, 1, s); }
+
+	// the most tiles any binning unit of ut triangles can touch: from the
+	// first tile of its first triangle's frame to the last tile of its last
+	// triangle's frame, with the draw of a triangle chosen as the kernels'
+	// find_draw_ll does (the last draw whose first triangle is at or before it)
+	static int UnitWindow(const std::vector<DrawDesc>& draws, long long total, long long ut) {
+		if (draws.empty() || total <= 0) return 1;
+		const int nd = static_cast<int>(draws.size());
+		auto drawOf = [&](long long s, int from) {
+			int d = from;
+			while (d + 1 < nd && draws[d + 1].triBase <= s) ++d;
+			return d; };
+		int best = 1, d0 = 0, d1 = 0;
+		for (long long s0 = 0; s0 < total; s0 += ut) {
+			d0 = drawOf(s0, d0);
+			d1 = drawOf(std::min(s0 + ut, total) - 1, std::max(d1, d0));
+			const DrawDesc& a = draws[d0];
+			const DrawDesc& z = draws[d1];
+			best = std::max(best, z.tileBase + z.tilesX * z.tilesY - a.tileBase); }
+		return best; }
 
 	// visibility grid: the last completed frame's item count, rounded up to a
 	// power of two so that a captured graph survives small changes
@@ -666,7 +693,7 @@ struct rgl_gpu {
 		if (zeroHere) k_zero_counters<<<std::max(8, std::min((kCountGroups * b.nTiles + 255) / 256, 256)), 256, 0, stream>>>(za);
 		Mark(1);
 		if (b.units > 0) {
-			k_setup_count<<<b.units, kUnitThreads, dlBytes + b.nTiles * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
+			k_setup_count<<<b.units, kUnitThreads, dlBytes + b.unitWindow * sizeof(uint32_t), stream>>>(dDraws.p, b, vpos.p, vflags.p, trect.p, tvert.p,
 				tileCount.p, clipq.p, clipqOf.p, clipCap, dCtr); }
 		Mark(2);
 		// clipping: the clip queue is complete after setup; the clipped
@@ -700,7 +727,7 @@ struct rgl_gpu {
 			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
 			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
 			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
-			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, b.nTiles * sizeof(uint32_t), stream>>>(b, dLite.p, sa); }
+			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, b.unitWindow * sizeof(uint32_t), stream>>>(b, dLite.p, sa); }
 		Mark(5);
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;
@@ -816,6 +843,7 @@ struct rgl_gpu {
 		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));
 		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");
+		if (hCtr->overflow & 16u) return fail(RGL_E_STATE, "a binning unit spans more tiles than its table holds");
 		if (hCtr->overflow) {
 			// grow and report; the caller runs the frame again
 			if (hCtr->overflow & 1u) listCap = static_cast<uint32_t>(std::min<long long>(2LL * hCtr->pairs + 4096, 0x7fffffffLL));

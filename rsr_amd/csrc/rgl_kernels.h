@@ -84,6 +84,22 @@ __device__ inline int find_draw_ll(const DrawLite* dl, int nDraws, long long s) 
 	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (dl[mid].triBase <= s) lo = mid; else hi = mid - 1; }
 	return lo; }
 
+// The tiles binning unit u can touch: triangles are numbered frame by
+// frame, so they lie between the first tile of its first triangle's frame
+// and the last tile of its last triangle's frame.  The unit's LDS tile
+// table covers this window only (at most b.unitWindow entries, which the
+// host sizes from the same rule).
+struct TileWindow { int lo, n; };
+__device__ inline TileWindow unit_window(const DrawLite* dl, const BatchDesc& b, int u) {
+	const long long s0 = static_cast<long long>(u) * b.unitTris;
+	const long long s1 = min(s0 + b.unitTris, b.totalTris) - 1;
+	const DrawLite& d0 = dl[find_draw_ll(dl, b.nDraws, s0)];
+	const DrawLite& d1 = dl[find_draw_ll(dl, b.nDraws, s1)];
+	TileWindow w;
+	w.lo = d0.tileBase;
+	w.n = d1.tileBase + d1.tilesX * d1.tilesY - d0.tileBase;
+	return w; }
+
 __device__ inline int find_draw_slot(const DrawLite* dl, int nDraws, uint32_t slot) {
 	int lo = 0, hi = nDraws - 1;
 	while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (dl[mid].slotBase <= slot) lo = mid; else hi = mid - 1; }
@@ -188,13 +204,18 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 		uint2* __restrict__ trect, uint4* __restrict__ tvert, uint32_t* __restrict__ tileCount,
 		uint32_t* __restrict__ clipq, uint32_t* __restrict__ clipqOf, uint32_t clipCap,
 		Counters* __restrict__ ctr) {
-	// dynamic LDS: the batch's draws, then nTiles counters
+	// dynamic LDS: the batch's draws, then the unit's tile-window counters
 	extern __shared__ uint4 dyn_lds[];
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	uint32_t* lhist = reinterpret_cast<uint32_t*>(dl + b.nDraws);
 	const int u = blockIdx.x;
 	load_lite(dl, draws, b.nDraws);
-	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) lhist[t] = 0;
+	__syncthreads();
+	const TileWindow win = unit_window(dl, b, u);
+	if (win.n > b.unitWindow) {   // the host's window bound disagrees: flag it, touch nothing
+		if (threadIdx.x == 0) atomicOr(&ctr->overflow, 16u);
+		return; }
+	for (int t = threadIdx.x; t < win.n; t += blockDim.x) lhist[t] = 0;   // entry t: batch tile win.lo + t
 	__syncthreads();
 	uint32_t binned = 0;
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
@@ -250,7 +271,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 					++binned;
 					for (int ty = ty0; ty <= ty1; ++ty)
 						for (int tx = tx0; tx <= tx1; ++tx)
-							atomicAdd(&lhist[d.tileBase + ty * d.tilesX + tx], 1u); }}}
+							atomicAdd(&lhist[d.tileBase - win.lo + ty * d.tilesX + tx], 1u); }}}
 		trect[s] = rect;
 		tvert[s] = tv;
 		// the wave's clipped triangles take consecutive clip-queue entries
@@ -268,9 +289,9 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 				else atomicOr(&ctr->overflow, 2u); }}}
 	__syncthreads();
 	uint32_t* tc = tileCount + static_cast<long long>(u % kCountGroups) * b.nTiles;
-	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) {
+	for (int t = threadIdx.x; t < win.n; t += blockDim.x) {
 		const uint32_t c = lhist[t];
-		if (c) atomicAdd(&tc[t], c); }
+		if (c) atomicAdd(&tc[win.lo + t], c); }
 	// wave reduce of the binned count
 	for (int o = 32; o > 0; o >>= 1) binned += __shfl_xor(binned, o, 64);
 	if ((threadIdx.x & 63) == 0 && binned) atomicAdd(&ctr->slot[blockIdx.x % kCtrSlots].binned, binned); }
@@ -886,7 +907,8 @@ __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, in
 // writes the pair's setup record.  Blocks [units, units + clipBlocks): the
 // clipped fans.  The rest: key resets.
 __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const DrawLite* __restrict__ dl, ScatterArgs a) {
-	// dynamic LDS: nTiles counts, then write cursors; the draw table is read through the caches
+	// dynamic LDS: the unit's tile-window counts, then write cursors; the
+	// draw table is read through the caches
 	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
 	uint32_t* base = reinterpret_cast<uint32_t*>(dyn_lds);
@@ -899,7 +921,9 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 	if (u >= b.units) {
 		clip_scatter(b, dl, u - b.units, a);
 		return; }
-	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) base[t] = 0u;
+	const TileWindow win = unit_window(dl, b, u);
+	if (win.n > b.unitWindow) return;   // flagged by the setup
+	for (int t = threadIdx.x; t < win.n; t += blockDim.x) base[t] = 0u;   // entry t: batch tile win.lo + t
 	__syncthreads();
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
 	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
@@ -910,10 +934,10 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 		if (tx0 > tx1 || ty0 > ty1) continue;
 		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
 		for (int ty = ty0; ty <= ty1; ++ty)
-			for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&base[d.tileBase + ty * d.tilesX + tx], 1u); }
+			for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&base[d.tileBase - win.lo + ty * d.tilesX + tx], 1u); }
 	__syncthreads();
-	uint32_t* cur = a.cursor + static_cast<long long>(u % kCountGroups) * b.nTiles;
-	for (int t = threadIdx.x; t < b.nTiles; t += blockDim.x) {
+	uint32_t* cur = a.cursor + static_cast<long long>(u % kCountGroups) * b.nTiles + win.lo;
+	for (int t = threadIdx.x; t < win.n; t += blockDim.x) {
 		const uint32_t c = base[t];
 		if (c) base[t] = atomicAdd(&cur[t], c); }
 	__syncthreads();
@@ -930,7 +954,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 		const uint32_t slot = d.slotBase + static_cast<uint32_t>(s - d.triBase);
 		for (int ty = ty0; ty <= ty1; ++ty)
 			for (int tx = tx0; tx <= tx1; ++tx) {
-				const uint32_t pos = atomicAdd(&base[d.tileBase + ty * d.tilesX + tx], 1u);
+				const uint32_t pos = atomicAdd(&base[d.tileBase - win.lo + ty * d.tilesX + tx], 1u);
 				if (pos < a.listCap) {
 					TriSetup t;
 					setup_vector(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
