@@ -521,12 +521,12 @@ struct rgl_gpu {
 		b.nFrames = n; b.nTiles = tileBase; b.nDraws = static_cast<int>(draws.size());
 		// binning units: kUnitTris triangles, grown in steps of kUnitTris
 		// when the tiles a unit's frames span (its LDS table, cleared and
-		// scanned once per unit) outnumber its triangles by more than 4 to 1
+		// scanned once per unit) outnumber its triangles by more than 2 to 1
 		b.totalTris = triBase;
 		b.unitTris = kUnitTris;
 		b.unitWindow = UnitWindow(draws, triBase, kUnitTris);
-		if (b.unitWindow > 4 * kUnitTris) {
-			b.unitTris = kUnitTris * std::min(32, b.unitWindow / (4 * kUnitTris));
+		if (b.unitWindow > 2 * kUnitTris) {
+			b.unitTris = kUnitTris * std::min(32, b.unitWindow / (2 * kUnitTris));
 			b.unitWindow = UnitWindow(draws, triBase, b.unitTris); }
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
 		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
