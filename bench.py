@@ -170,16 +170,14 @@ def bench_stress(steps: int):
     import torch
     from rsr_amd import learned, rgl, scenes
     dev = "cuda:0"
+    from rsr_amd.autotune import autotune
     views = [scenes.scene_stress(v) for v in range(4)]
-    frames = []
-    for sc in views:
-        g = rgl.GPU(0)
-        ds = rgl.DeviceScene(sc, dev, prim=True)
-        ds.record(g)
-        frames.append((g, ds))
-    recs = [g for g, _ in frames]
     ex = rgl.GPU(0)
     ex.set_graph(True)
+    # pairs per visibility work item and region split picked by the
+    # autotuner on the batch itself (64-pixel tiles)
+    frames, choice, table = autotune(ex, views, dev, steps=5, tiles=((8, 8),), splits=(0, 2048))
+    recs = [g for g, _ in frames]
     stream = torch.cuda.current_stream(dev).cuda_stream
 
     def run(n):
@@ -211,7 +209,8 @@ def bench_stress(steps: int):
            "plain_heavy_ms": round(plain_ms, 4),
            "mtris_per_s": round(4 * views[0].triangles / (plain_ms * 1e-3) / 1e6, 2),
            "gfrag_per_s": round(st["fragments"] / (plain_ms * 1e-3) / 1e9, 4),
-           "kernel_ms_plain": {k: round(v, 5) for k, v in plain_kt.items()}}
+           "kernel_ms_plain": {k: round(v, 5) for k, v in plain_kt.items()},
+           "autotune": {"choice": choice, "table": table}}
     v_xf = sum(sc.transformed_vertices for sc in views)
     b_bin = 12 * v_xf + 6 * 4 * views[0].triangles + 64 * 4 * views[0].instances + 4 * st["tile_pairs"] + 8 * st["tiles"]
     bin_ms = sum(plain_kt.get(k, 0.0) for k in ("k_vertex", "k_setup_count", "k_scan", "k_scatter", "k_clip"))

@@ -14,7 +14,7 @@ import dataclasses
 import time
 
 TILE_CHOICES = ((8, 8), (4, 4))          # 64 and 32 pixel tiles (tile_blocks x 8)
-SEG_CHOICES = (256, 512, 1024)
+SEG_CHOICES = (256, 512, 1024, 2048, 4096)
 SPLIT_CHOICES = (0, 512, 1024, 2048)
 
 
