@@ -1033,16 +1033,13 @@ __device__ inline float lane_interp(const LaneBary& b, const float* v0, const fl
 
 // shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle
 template <class P>
-__device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* __restrict__ vary,
-                                   int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+__device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* v0, const float* v1,
+                                   const float* v2, int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
 	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
 	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
 	// lane offsets of the rasteriser's quad: (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
 	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
 	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
-	const float* v0 = vary + tr.vary[0] * kMaxVary;
-	const float* v1 = vary + tr.vary[1] * kMaxVary;
-	const float* v2 = vary + tr.vary[2] * kMaxVary;
 	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
 	depth = own.depth;
 	float vv[P::kVary];
@@ -1058,22 +1055,35 @@ __device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const T
 	P::fragment1(d, vv, lod, color); }
 
 // resolve shaders: one program known at compile time, or a switch over
-// the installed programs for frames that mix them
+// the installed programs for frames that mix them.  load() takes the
+// triangle's vertex varyings as soon as its vertex ids are known, so their
+// gathers overlap the triangle setup; shade() then runs the program.
 template <class P>
 struct ShadeOne {
-	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	struct Vary { float v[3][P::kVary]; };
+	__device__ static void load(const float* __restrict__ vary, const TriVerts& tr, Vary& out) {
+#pragma unroll
+		for (int i = 0; i < 3; ++i) {
+			const float* src = vary + tr.vary[i] * kMaxVary;
+#pragma unroll
+			for (int k = 0; k < P::kVary; ++k) out.v[i][k] = src[k]; }}
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const Vary& vy,
 	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-		shade_pixel<P>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); }};
+		shade_pixel<P>(d, t, tr, vy.v[0], vy.v[1], vy.v[2], kx, ky, lane, rcpTab, color, depth); }};
 
 struct ShadeAny {
-	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* vary,
+	struct Vary { const float* v[3]; };
+	__device__ static void load(const float* __restrict__ vary, const TriVerts& tr, Vary& out) {
+		for (int i = 0; i < 3; ++i) out.v[i] = vary + tr.vary[i] * kMaxVary; }
+	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const Vary& vy,
 	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+		const float *v0 = vy.v[0], *v1 = vy.v[1], *v2 = vy.v[2];
 		switch (d.program) {
-		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break;
-		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, vary, kx, ky, lane, rcpTab, color, depth); break; }}};
+		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
+		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
+		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break; }}};
 
 // ---------------------------------------------------------------------------
 // resolve: one thread per pixel.  The pixel's winning slot is shaded by
@@ -1099,23 +1109,23 @@ __device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, con
 	TriVerts tr;
 	TriSetup t;
 	uint32_t srcTri;
-	const float* vary;
+	typename S::Vary vy;
 	if (rel < nTri) {
 		srcTri = rel;
 		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
-		setup_vector(R, tr.X, tr.Y, t);
-		vary = a.vvary; }
+		S::load(a.vvary, tr, vy);
+		setup_vector(R, tr.X, tr.Y, t); }
 	else {
 		const uint32_t fslot = rel - nTri;
 		srcTri = fslot / kFanMax;
 		const uint32_t j = fslot - srcTri * kFanMax;
 		const uint32_t cq = a.clipqOf[d.triBase + srcTri];
 		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
-		setup_scalar(R, tr.X, tr.Y, t);
-		vary = a.cvvary; }
+		S::load(a.cvvary, tr, vy);
+		setup_scalar(R, tr.X, tr.Y, t); }
 	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
 	const int lane = (x & 1) | ((y & 1) << 1);
-	S::shade(dd, t, tr, vary, kx, ky, lane, rcpTab, col, dep);
+	S::shade(dd, t, tr, vy, kx, ky, lane, rcpTab, col, dep);
 	prim = d.primBase + srcTri; }
 
 // the winner of pixel (x, y), shaded (clear values where nothing was
