@@ -1,8 +1,8 @@
 """GPU parity on scene variants that exercise the paths the two headline
 scenes leave quiet: near-plane and guard-band clipping with visible
 clipped fans, LEQUAL with culling off (back faces drawn and reordered),
-front-face culling, a 32-pixel tile grid, and the reference's own Amy and
-Many programs.  Every buffer is compared bit-exactly with the oracle."""
+front-face culling, 32-pixel tile grids (one of them large enough for
+the three-launch scan), and the reference's own Amy and Many programs.  Every buffer is compared bit-exactly with the oracle."""
 import dataclasses
 
 import numpy as np
@@ -52,6 +52,14 @@ def test_front_face_cull():
 def test_tile_32():
     s = dataclasses.replace(scenes.scene_icospheres(960, 540, instances=96, seed=5, tex_dim=256),
                             tile_blocks=(4, 4), name="icospheres96_960x540_tile32")
+    _check(s)
+
+
+def test_tile_32_1080p_split_scan():
+    # 2040 tiles in one frame: the scan takes its three-launch form (local
+    # block scans, block bases, final offsets and work items)
+    s = dataclasses.replace(scenes.scene_icospheres(1920, 1080, instances=128, seed=9, tex_dim=256),
+                            tile_blocks=(4, 4), name="icospheres128_1920x1080_tile32")
     _check(s)
 
 
