@@ -1180,9 +1180,10 @@ __device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, con
 	uint32_t prim;
 	resolve_key<S>(b, f, a, dl, draws, x, y, k, rcpTab, col, dep, prim);
 	const long long o = static_cast<long long>(y) * f.W + x;
-	if (f.outColor) f.outColor[o] = post_pixel(f, col, srgbTab);
-	if (f.outDepth) f.outDepth[o] = dep;
-	if (f.outPrim) f.outPrim[o] = prim; }
+	// the outputs are not read again by this frame: streaming stores
+	if (f.outColor) __builtin_nontemporal_store(post_pixel(f, col, srgbTab), &f.outColor[o]);
+	if (f.outDepth) __builtin_nontemporal_store(dep, &f.outDepth[o]);
+	if (f.outPrim) __builtin_nontemporal_store(prim, &f.outPrim[o]); }
 
 // a resolve block's frame and pixel: blocks are numbered across the batch,
 // each frame's blocks row-major over kResolveW x kResolveH pixel strips
