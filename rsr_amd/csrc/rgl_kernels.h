@@ -345,6 +345,10 @@ __device__ inline uint32_t vis_regions(const BatchDesc& b, uint32_t cost, uint32
 	const uint32_t per = cost / (nseg * kQuadrants), s = static_cast<uint32_t>(b.splitQuads);
 	return per > 2 * s ? 4 * kQuadrants : per > s ? 2 * kQuadrants : kQuadrants; }
 
+constexpr int kCostClasses = 33;   // cost 0, then [2^(c-1), 2^c) for c = 1 .. 32
+constexpr int kScanSplitTiles = 2 * kScanThreads;   // tiles above which the scan takes three launches
+__device__ inline int cost_class(uint32_t cost) { return cost ? 32 - __clz(cost) : 0; }
+
 // A tile's visibility work as the scan lays it out, so that a work item
 // reads its tile (itemTile) and then one 32-byte descriptor instead of a
 // chain of table lookups.
@@ -370,10 +374,16 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 	__shared__ int ftb[kMaxFrames + 1];
 	load_tile_bases(ftb, b);
 	__syncthreads();
-	const int nT = b.nTiles;
+	const int nT = b.nTiles;   // at most kScanSplitTiles: each thread keeps its tiles' descriptors in registers
+	constexpr int kPer = kScanSplitTiles / kScanThreads;
+	VisTile mine[kPer];
+	int mcls[kPer];
+	__shared__ uint32_t classItems[kCostClasses], classCursor[kCostClasses];
+	for (int c = threadIdx.x; c < kCostClasses; c += kScanThreads) classItems[c] = 0u;
 	uint32_t carry = 0, ccarry = 0;
-	for (int base = 0; base < nT; base += kScanThreads) {
-		const int t = base + threadIdx.x;
+#pragma unroll
+	for (int it = 0; it < kPer; ++it) {
+		const int t = it * kScanThreads + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
 		uint32_t grp[kCountGroups];
 #pragma unroll
@@ -390,13 +400,14 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 		uint32_t tot, ctot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
+		mcls[it] = cost_class(cost);
 		if (t < nT) {
 			tileOff[t] = carry + ex;
 			uint32_t run = carry + ex;
 #pragma unroll
 			for (int g = 0; g < kCountGroups; ++g) { tileCursor[static_cast<long long>(g) * nT + t] = run; run += grp[g]; }
 			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
-			VisTile d;
+			VisTile& d = mine[it];
 			d.item0 = items;   // the item count until the ordering below
 			d.frame = static_cast<uint32_t>(find_base(ftb, b.nFrames, t));
 			d.ib = min(carry + ex, listCap);
@@ -404,34 +415,35 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 			d.ninl = min(carry + ex + col, listCap) - d.ib;
 			d.ntot = d.ninl + (min(ccarry + cex + v, clistCap) - d.cb);
 			d.nreg = nreg;
-			d.multiSeg = (nseg > 1 ? 1u : 0u) | (cost << 1);   // the cost rides along until the ordering
-			tileDesc[t] = d; }
+			d.multiSeg = nseg > 1 ? 1u : 0u;
+			atomicAdd(&classItems[mcls[it]], items); }
 		carry += tot; ccarry += ctot; }
 	// Work-item order: by the tiles' previous-frame quad work, heaviest
 	// first in power-of-two classes (a counting sort), so that long items
-	// start at once and short ones fill the end of the kernel.
-	__shared__ uint32_t classItems[33], classCursor[33];
-	for (int c = threadIdx.x; c < 33; c += kScanThreads) classItems[c] = 0u;
+	// start at once and short ones fill the end of the kernel.  Wave 0
+	// scans the class counts from the heaviest class down.
 	__syncthreads();
-	for (int t = threadIdx.x; t < nT; t += kScanThreads) {
-		const uint32_t cost = tileDesc[t].multiSeg >> 1;
-		const int cls = cost ? 32 - __clz(cost) : 0;   // 0 .. 31
-		atomicAdd(&classItems[cls], tileDesc[t].item0); }
+	if (threadIdx.x < 64) {
+		const int lane = threadIdx.x, c = kCostClasses - 1 - lane;   // lane 0: the heaviest class
+		const uint32_t n = c >= 0 ? classItems[c] : 0u;
+		uint32_t x = n;
+		for (int o = 1; o < 64; o <<= 1) {
+			const uint32_t y = __shfl_up(x, o, 64);
+			if (lane >= o) x += y; }
+		if (c >= 0) classCursor[c] = x - n;
+		if (lane == 63) classItems[0] = x; }   // the total, read after the barrier below
 	__syncthreads();
-	if (threadIdx.x == 0) {
-		uint32_t run = 0;
-		for (int c = 32; c >= 0; --c) { classCursor[c] = run; run += classItems[c]; }
-		classItems[32] = run; }
-	__syncthreads();
-	const uint32_t wcarry = classItems[32];
-	for (int t = threadIdx.x; t < nT; t += kScanThreads) {
-		VisTile& d = tileDesc[t];
-		const uint32_t cost = d.multiSeg >> 1, items = d.item0;
-		const int cls = cost ? 32 - __clz(cost) : 0;
-		const uint32_t item0 = atomicAdd(&classCursor[cls], items);
+	const uint32_t wcarry = classItems[0];
+#pragma unroll
+	for (int it = 0; it < kPer; ++it) {
+		const int t = it * kScanThreads + threadIdx.x;
+		if (t >= nT) continue;
+		VisTile& d = mine[it];
+		const uint32_t items = d.item0;
+		const uint32_t item0 = atomicAdd(&classCursor[mcls[it]], items);
 		for (uint32_t j = item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t);
 		d.item0 = item0;
-		d.multiSeg &= 1u; }
+		tileDesc[t] = d; }
 	if (threadIdx.x == 0) {
 		tileOff[nT] = carry; clipOff[nT] = ccarry;
 		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
@@ -442,16 +454,12 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 // then every tile's offsets, cursors and items (k_scan_final).  Every
 // choice of work-item order within a cost class is valid, so a tile's
 // first item is taken from its block's LDS cursor of its class.
-constexpr int kCostClasses = 33;   // cost 0, then [2^(c-1), 2^c) for c = 1 .. 32
-constexpr int kScanSplitTiles = 2 * kScanThreads;   // tiles above which the scan takes three launches
 
 struct ScanPart {
 	uint32_t col, clip;            // block totals, then the block's exclusive bases
 	uint32_t cls[kCostClasses];    // items per cost class, then the block's offset within the class
 	uint32_t pad[3];
 };
-
-__device__ inline int cost_class(uint32_t cost) { return cost ? 32 - __clz(cost) : 0; }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_local(BatchDesc b, const uint32_t* __restrict__ tileCount,
 		const uint32_t* __restrict__ clipHist, uint32_t* __restrict__ tileCost, VisTile* __restrict__ tileDesc,
