@@ -79,9 +79,12 @@ int rgl_gpu_wait(rgl_gpu* gpu);
 /* GPUStats of the last completed Run. */
 int rgl_gpu_stats(rgl_gpu* gpu, rgl_stats* out);
 
-/* Average duration in milliseconds of each pipeline kernel over the runs
-   since the last call (event-timed on the run's stream when timing is
-   enabled); names_out receives a comma-separated kernel list. */
+/* Average duration in milliseconds of each pipeline stage over the runs
+   since the last call, when timing is enabled: k_vis and k_resolve by
+   start and stop events recorded by their own dispatch (the kernel's
+   execution alone), the other stages between events on the run's stream;
+   names_out receives a comma-separated stage list.  No counterpart in the
+   reference (measurement only). */
 int rgl_gpu_set_timing(rgl_gpu* gpu, int enable);
 int rgl_gpu_kernel_times(rgl_gpu* gpu, double* ms_out, int max_kernels, char* names_out, int names_len);
 
