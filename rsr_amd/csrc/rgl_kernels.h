@@ -971,14 +971,14 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 	if (ovf) atomicOr(&a.ctr->overflow, 1u); }
 
 // ---------------------------------------------------------------------------
-// visibility: a persistent kernel over (tile, segment, quadrant) work
-// items.  Each item holds a 32x32 quadrant's keys in LDS, takes up to
-// kSegPairs of its tile's setup records (inline list in submission order,
-// then clipped fans; set up against the full tile rectangle, the
-// reference's edge start values), rasterises the triangles' quads that
-// fall in the quadrant with ds_min_u64, and merges the quadrant into the
-// per-pixel key buffer: a plain store when the tile is one segment, a
-// 64-bit atomic min otherwise.
+// visibility: one workgroup per (tile, segment, region) work item, the
+// region a 32x32 quadrant or its half or quarter.  Each item holds its
+// region's keys in LDS, takes up to segPairs of its tile's setup records
+// (inline records, then clipped fans, in any order; set up against the
+// full tile rectangle, the reference's edge start values), rasterises the
+// triangles' quads that fall in the region with ds_min_u64, and merges the
+// region into the per-pixel key buffer: a plain store when the tile is one
+// segment, a 64-bit atomic min otherwise.
 // ---------------------------------------------------------------------------
 struct alignas(16) VisSetup {
 	int32_t c1, c2, c3;   // edge values at the triangle's first quad in the quadrant
