@@ -529,6 +529,9 @@ struct rgl_gpu {
 			b.unitTris = kUnitTris * std::min(32, b.unitWindow / (2 * kUnitTris));
 			b.unitWindow = UnitWindow(draws, triBase, b.unitTris); }
 		b.units = static_cast<int>((triBase + b.unitTris - 1) / b.unitTris);
+		// the setup's LDS holds the draw table and the unit's tile window
+		if (sizeof(DrawLite) * draws.size() + sizeof(uint32_t) * static_cast<size_t>(b.unitWindow) > kUnitLdsBytes)
+			return fail(RGL_E_UNSUPPORTED, "a binning unit's frames span too many tiles for its LDS table; use larger tiles");
 		b.nBlocks = blockBase; b.segPairs = segPairs; b.splitQuads = splitQuads; b.frames = nullptr;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
@@ -770,6 +773,7 @@ struct rgl_gpu {
 		return 0; }
 
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
+	static constexpr size_t kUnitLdsBytes = 160 * 1024;   // LDS per workgroup on gfx950
 	static constexpr int kResetBlocks = 128;        // k_scatter blocks that reset the keys of multi-segment tiles
 
 	static int ProgramSlot(int pid) {
