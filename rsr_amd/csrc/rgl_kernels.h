@@ -1101,7 +1101,10 @@ struct ShadeAny {
 // sRGB-encoded (FilterTile, rglr_algorithm.hxx:31-104).  Colour, depth
 // and primitive id are written once; the key is reset for the next frame.
 // ---------------------------------------------------------------------------
-constexpr int kResolveW = 64, kResolveH = 4;   // 256 threads = 64 x 4 pixels
+// 256 threads = 16 x 16 pixels, a wave 16 x 4: compact blocks, whose
+// pixels share more of their winning triangles' vertices and texels than
+// 64 x 4 strips do (measured faster)
+constexpr int kResolveW = 16, kResolveH = 16;
 
 // the winner of pixel (x, y) drawn by draw d (table entry l), shaded:
 // colour before the post program, depth and primitive id
@@ -1194,7 +1197,7 @@ __device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, con
 	if (f.outPrim) __builtin_nontemporal_store(prim, &f.outPrim[o]); }
 
 // a resolve block's frame and pixel: blocks are numbered across the batch,
-// each frame's blocks row-major over kResolveW x kResolveH pixel strips
+// each frame's blocks row-major over kResolveW x kResolveH pixel blocks
 struct BlockPixel { int frame, x, y; };
 
 __device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {

@@ -463,15 +463,16 @@ __global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a,
 	const BlockPixel bp = block_pixel(b, fbb);
 	const FrameDesc& f = b.frames[bp.frame];
 	const int x = bp.x, y = bp.y;
-	// the block's rows lie in one tile row and span tile columns tx0..tx1
+	// the tiles the block's pixels lie in: rows ty0..ty1, columns tx0..tx1
 	const int lb = static_cast<int>(blockIdx.x) - f.blockBase, bx = lb % f.blocksX, by = lb / f.blocksX;
-	const int ty = (by * kResolveH) / f.tileH;
+	const int ty0 = (by * kResolveH) / f.tileH, ty1 = min(by * kResolveH + kResolveH - 1, f.H - 1) / f.tileH;
 	const int tx0 = (bx * kResolveW) / f.tileW, tx1 = min(bx * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
 	bool any = false;
-	for (int tx = tx0; tx <= tx1; ++tx) {
-		const int t = f.tileBase + ty * f.tilesX + tx;
-		const float c = tileErr[2 * t + 1];
-		any = any || (c > 0.0F && tileErr[2 * t] / c > maxMse); }
+	for (int ty = ty0; ty <= ty1; ++ty)
+		for (int tx = tx0; tx <= tx1; ++tx) {
+			const int t = f.tileBase + ty * f.tilesX + tx;
+			const float c = tileErr[2 * t + 1];
+			any = any || (c > 0.0F && tileErr[2 * t] / c > maxMse); }
 	if (!any || f.outColor == nullptr) return;   // uniform over the block
 	load_lite(dl, a.draws, b.nDraws);
 	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
