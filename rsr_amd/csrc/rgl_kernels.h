@@ -1248,8 +1248,15 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 	__shared__ uint8_t owner[kQuadWindow];
 	uint32_t frags = 0;
 	const uint32_t nItems = a.ctr->workItems;
-	for (uint32_t item = blockIdx.x; item < nItems; item += gridDim.x) {
+	// Workgroups are dealt round-robin over the eight XCDs.  A tile's items
+	// are consecutive and come in multiples of four (its regions), so each
+	// run of four items is taken by workgroups of one XCD: they read the
+	// tile's records through that XCD's L2 instead of four.
+	constexpr uint32_t kGroup = 4u * kXcds;
+	const uint32_t full = nItems / kGroup * kGroup;
+	for (uint32_t wi = blockIdx.x; wi < nItems; wi += gridDim.x) {
 		__syncthreads();
+		const uint32_t item = wi < full ? wi / kGroup * kGroup + (wi % kXcds) * 4u + (wi / kXcds) % 4u : wi;
 		const int tile = static_cast<int>(itemTile[item]);
 		const VisTile it = tileDesc[tile];
 		const uint32_t local = item - it.item0;
