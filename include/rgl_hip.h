@@ -161,7 +161,7 @@ int rgl_gpu_set_graph(rgl_gpu* gpu, int enable);
 int rgl_gpu_graph_captures(rgl_gpu* gpu);   /* graphs captured so far */
 
 /* Guard band: mode 0 (default) = the reference's CalcGuardBandFactor
-   (rglv_view_frustum.hxx:257-260), which drops below 1 for targets wider
+   (rglv_view_frustum.hxx:36-39), which drops below 1 for targets wider
    or taller than 2048 pixels, so that on-screen triangles beyond +-factor
    NDC are culled or clipped as in the reference; mode 1 = the factor raised
    to at least 1 (a documented deviation for large targets). */

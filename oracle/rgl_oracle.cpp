@@ -284,8 +284,7 @@ struct Program {
 	// one quad; vary[k][lane], colour[c][lane]; mask lanes may be cleared
 	virtual void ShadeFragment(const DrawEnv& e, const float bs[3][4], const float bp[3][4], const float vary[kMaxVary][4],
 	                           const float fragCoord[2][4], const float fragDepth[4], float color[4][4], int32_t mask[4]) const = 0;
-	void Mix(const VOut& a, const VOut& b, float t, VOut& o) const {  // mix() (rmlv_math.hxx:87-90)
-		for (int k = 0; k < nvary; ++k) o.f[k] = (1.0F - t) * a.f[k] + t * b.f[k]; }};
+};
 
 inline float buf_at(const float* p, int idx, float dflt) { return p ? p[idx] : dflt; }
 
@@ -525,7 +524,7 @@ struct ClippedVertex { float coord[4]; VOut data; };
 
 struct Rect { int x0, y0, x1, y1; };
 
-struct Frustum {  // ViewFrustum (rglv_view_frustum.hxx:257-315)
+struct Frustum {  // ViewFrustum (rglv_view_frustum.hxx:36-73)
 	float factor;
 	// fixedBand: the factor is raised to at least 1 (a guard band no smaller
 	// than the viewport); the reference's formula falls below 1 above 2048
@@ -553,6 +552,36 @@ inline float plane_dist(int plane, const float p[4]) {
 	case 2: return p[3] + p[2];
 	case 3: return p[3] - p[0];
 	default: return p[3] - p[1]; }}
+
+// Sutherland-Hodgman over L, B, N, R, T against the unscaled planes, each
+// new vertex interpolated from the inside vertex toward the outside one
+// (ClipTriangles phase 2, rglv_gpu_impl.hxx:727-762; IsInside and Distance,
+// rglv_view_frustum.hxx:75-94; mix, rmlv_math.hxx:87-90).  A is replaced by
+// the clipped polygon (empty when nothing is left).
+inline void mix_vout(const VOut& a, const VOut& b, float t, VOut& o, int nvary) {
+	for (int k = 0; k < nvary; ++k) o.f[k] = (1.0F - t) * a.f[k] + t * b.f[k]; }
+
+void clip_polygon(std::vector<ClippedVertex>& A, int nvary) {
+	std::vector<ClippedVertex> B;
+	for (int plane = 0; plane < 5; ++plane) {
+		bool hereIsInside = plane_dist(plane, A[0].coord) >= 0.0F;
+		B.clear();
+		for (size_t h = 0; h < A.size(); ++h) {
+			const size_t n = (h + 1) % A.size();
+			const bool nextIsInside = plane_dist(plane, A[n].coord) >= 0.0F;
+			if (hereIsInside) B.push_back(A[h]);
+			if (hereIsInside != nextIsInside) {
+				const ClippedVertex& from = hereIsInside ? A[h] : A[n];
+				const ClippedVertex& to = hereIsInside ? A[n] : A[h];
+				const float fa = plane_dist(plane, from.coord), fb = plane_dist(plane, to.coord);
+				const float d = fa / (fa - fb);
+				ClippedVertex nv{};
+				for (int q = 0; q < 4; ++q) nv.coord[q] = (1.0F - d) * from.coord[q] + d * to.coord[q];
+				mix_vout(from.data, to.data, d, nv.data, nvary);
+				B.push_back(nv);
+				hereIsInside = !hereIsInside; }}
+		std::swap(A, B);
+		if (A.empty()) break; }}
 
 }  // namespace
 
@@ -744,30 +773,12 @@ struct rglo_gpu {
 		statClipped += static_cast<long long>(clipQueue.size());
 		// ClipTriangles (rglv_gpu_impl.hxx:677-793)
 		for (const ClipItem& ci : clipQueue) {
-			std::vector<ClippedVertex> A, B;
+			std::vector<ClippedVertex> A;
 			for (int k = 0; k < 3; ++k) {
 				ClippedVertex cv{};
 				pgm->ShadeVertex(env, ci.iid, ci.i[k], cv.coord, cv.data);
 				A.push_back(cv); }
-			for (int plane = 0; plane < 5; ++plane) {
-				bool hereIsInside = plane_dist(plane, A[0].coord) >= 0.0F;
-				B.clear();
-				for (size_t h = 0; h < A.size(); ++h) {
-					const size_t n = (h + 1) % A.size();
-					const bool nextIsInside = plane_dist(plane, A[n].coord) >= 0.0F;
-					if (hereIsInside) B.push_back(A[h]);
-					if (hereIsInside != nextIsInside) {
-						const ClippedVertex& from = hereIsInside ? A[h] : A[n];
-						const ClippedVertex& to = hereIsInside ? A[n] : A[h];
-						const float fa = plane_dist(plane, from.coord), fb = plane_dist(plane, to.coord);
-						const float d = fa / (fa - fb);
-						ClippedVertex nv{};
-						for (int q = 0; q < 4; ++q) nv.coord[q] = (1.0F - d) * from.coord[q] + d * to.coord[q];
-						pgm->Mix(from.data, to.data, d, nv.data);
-						B.push_back(nv);
-						hereIsInside = !hereIsInside; }}
-				std::swap(A, B);
-				if (A.empty()) break; }
+			clip_polygon(A, pgm->nvary);
 			if (A.empty()) continue;
 			for (auto& v : A) {
 				const float iw = oneover(v.coord[3]);  // scalar pdiv (rglv_math.hxx:30-39)
@@ -1253,6 +1264,96 @@ int rglo_scalar_triangle(int w, int h, const float* pts /*3 x vec4*/, const floa
 		out[p*6 + 3] = on ? bsz[p] : 0.0F;
 		out[p*6 + 4] = on ? tb.r[p] : 0.0F;
 		out[p*6 + 5] = on ? tb.g[p] : 0.0F; }
+	return 0; }
+
+// ---- building blocks for the pinning tests -------------------------------
+// One triangle given by its fixed-point device vertices (1/16 pixel) drawn by
+// the vector (mode 0, VTriangleRasterizer) or scalar (mode 1,
+// TriangleRasterizer) rasteriser into the pixel rectangle [x0, x1) x [y0, y1)
+// of a target h pixels high, depth test off; out[4 per pixel of the rect]
+// = covered, BS.x, BS.y, BS.z.  Quad lanes past the rect's right or bottom
+// edge (odd rects) are not reported.  x0 and y0 must be even, as every
+// tile origin is.
+int rglo_raster_tri(int mode, int x0, int y0, int x1, int y1, int h, const int32_t* X, const int32_t* Y, float* out) {
+	struct CaptureBS : Program {
+		CaptureBS() { id = 1002; nvary = 0; earlyZ = true; }
+		void ShadeVertex(const DrawEnv&, int, int, float*, VOut&) const override {}
+		void ShadeFragment(const DrawEnv&, const float bs[3][4], const float[3][4], const float[kMaxVary][4],
+		                   const float[2][4], const float[4], float color[4][4], int32_t[4]) const override {
+			for (int l = 0; l < 4; ++l) { color[0][l] = bs[0][l]; color[1][l] = bs[1][l]; color[2][l] = bs[2][l]; color[3][l] = 1.0F; }}};
+	static const CaptureBS cap;
+	if (x1 <= x0 || y1 <= y0 || (x0 & 1) || (y0 & 1)) return -1;   // tile origins are even
+	const int rw = x1 - x0, rh = y1 - y0;
+	rglo_gpu g; g.W = x1 + 2; g.H = h;
+	g.tileW = rw + 2; g.tileH = rh + 2;   // the buffer stride: room for a quad's second lane past the rect
+	rglo_gpu::TileBuf tb;
+	const size_t area = static_cast<size_t>(g.tileW) * g.tileH;
+	tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F); tb.d.assign(area, 1.0F); tb.prim.assign(area, 0xffffffffu);
+	DrawEnv env{};
+	rglo_gpu::TriCtx tc{ &cap, &env, {}, {}, {}, 0u, GL_LESS, false, false, true, false };
+	for (int k = 0; k < 3; ++k) { tc.w[k] = 1.0F; tc.z[k] = 0.5F; }
+	const Rect r{ x0, y0, x1, y1 };
+	if (mode == 0) g.RasterVector(tb, r, tc, X, Y);
+	else g.RasterScalar(tb, r, tc, X, Y);
+	for (int y = 0; y < rh; ++y)
+		for (int x = 0; x < rw; ++x) {
+			const size_t p = static_cast<size_t>(y) * g.tileW + x;
+			float* o = out + (static_cast<size_t>(y) * rw + x) * 4;
+			const bool on = tb.prim[p] != 0xffffffffu;
+			o[0] = on ? 1.0F : 0.0F; o[1] = on ? tb.r[p] : 0.0F; o[2] = on ? tb.g[p] : 0.0F; o[3] = on ? tb.b[p] : 0.0F; }
+	return 0; }
+
+// ClipTriangles' polygon clip of one triangle in clip coordinates with nv
+// varyings per vertex: returns the clipped vertex count (0: nothing left),
+// coordinates to out_coord (<= 8 x 4) and varyings to out_vary (<= 8 x nv).
+int rglo_clip(const float* coord, const float* vary, int nv, float* out_coord, float* out_vary) {
+	if (nv < 0 || nv > kMaxVary) return -1;
+	std::vector<ClippedVertex> A(3);
+	for (int k = 0; k < 3; ++k) {
+		for (int q = 0; q < 4; ++q) A[k].coord[q] = coord[k * 4 + q];
+		for (int j = 0; j < nv; ++j) A[k].data.f[j] = vary[k * nv + j]; }
+	clip_polygon(A, nv);
+	for (size_t k = 0; k < A.size(); ++k) {
+		for (int q = 0; q < 4; ++q) out_coord[k * 4 + q] = A[k].coord[q];
+		for (int j = 0; j < nv; ++j) out_vary[k * nv + j] = A[k].data.f[j]; }
+	return static_cast<int>(A.size()); }
+
+// TextureUnit::sample of one quad (4 lanes) on a float RGBA texture bound
+// like GL::BindTexture: out[channel * 4 + lane]; returns -1 for a
+// power-of-two size the reference cannot make a unit for.
+int rglo_sample(const float* buf, int w, int h, int stride, int filter, const float* u, const float* v, float* out) {
+	std::string err;
+	const Sampler s = Sampler::Make(TextureState{ buf, w, h, stride, filter }, err);
+	if (!err.empty()) return -1;
+	float o[4][4];
+	s.sample(u, v, o);
+	for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c * 4 + l] = o[c][l];
+	return s.kind; }
+
+// LevelOfDetail from lanes 0 and 1 of base-level texel coordinates
+int rglo_level_of_detail(const float* u, const float* v) { return level_of_detail(u, v); }
+
+// ShadeFragment of program pid for one quad: vary[k * 4 + lane] (k < the
+// program's varyings), uniforms = 32 floats, buffers = 16 pointers (slots
+// 11-14 for the heavy shader), texture unit 0 as GL::BindTexture; colour
+// to out[channel * 4 + lane].
+int rglo_shade(int pid, const float* uniforms, const float* const* buffers, const float* tex, int texDim, int filter,
+               const float* vary, float* out) {
+	const Program* pgm = find_program(pid);
+	if (!pgm) return -1;
+	GLState st; st.reset();
+	for (int i = 0; i < 16; ++i) st.buffers[i] = buffers ? buffers[i] : nullptr;
+	st.tus[0] = { tex, texDim, texDim, texDim, filter };
+	DrawEnv env{};
+	env.st = &st; env.uniforms = uniforms;
+	std::string err;
+	env.tu0 = Sampler::Make(st.tus[0], err);
+	if (!err.empty()) return -1;
+	float bs[3][4] = {}, bp[3][4] = {}, fc[2][4] = {}, fd[4] = {}, vv[kMaxVary][4] = {}, color[4][4] = {};
+	for (int k = 0; k < pgm->nvary; ++k) for (int l = 0; l < 4; ++l) vv[k][l] = vary[k * 4 + l];
+	int32_t mask[4] = { -1, -1, -1, -1 };
+	pgm->ShadeFragment(env, bs, bp, vv, fc, fd, color, mask);
+	for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c * 4 + l] = color[c][l];
 	return 0; }
 
 }  // extern "C"

@@ -152,7 +152,7 @@ struct DrawDesc {
 	const float* buffers[16];
 	float vpm[16];           // projection * view (MakeMatrices, rglv_gpu_impl.hxx:44-51)
 	float DS[2], DO[2];      // GPU::DSDO (rglv_gpu.hxx:265-270)
-	float gbFactor;          // CalcGuardBandFactor (rglv_view_frustum.hxx:257-260)
+	float gbFactor;          // CalcGuardBandFactor (rglv_view_frustum.hxx:36-39)
 	int scissor[4];          // x0, y0, x1, y1 (GPU::ScissorRect)
 	int keepBacks, keepFronts;
 	int cullingEnabled, cullFace;

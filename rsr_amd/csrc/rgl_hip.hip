@@ -127,7 +127,8 @@ struct rgl_gpu {
 	// learned shading (rgl_gpu_set_learned_shading)
 	const float* mlpParams = nullptr;
 	float mlpMaxRms = 2.0F / 255.0F;
-	DevArray<uint32_t> mlpImg;
+	DevArray<uint32_t> mlpImg;      // the network image of learned frames
+	DevArray<uint32_t> fwdImg;      // the network image of rgl_mlp_forward
 	DevArray<float> tileErr, mlpQueue;
 	DevArray<uint32_t> qCount;      // network queue counters, kQueueSegs x kQueueStride
 	int mlpShadeBlocks = 256;       // persistent network workgroups: one per CU
@@ -179,6 +180,11 @@ struct rgl_gpu {
 	bool pending = false;
 	hipStream_t stream = nullptr;
 	hipEvent_t done = nullptr;
+	// the last network forward / gradient / training call: its own stream,
+	// ordered against frames by an event (frames never run on it implicitly)
+	bool auxPending = false;
+	hipStream_t auxStream = nullptr;
+	hipEvent_t auxDone = nullptr;
 	rgl_stats stats{};
 	long long submitted = 0;
 	int lastCommands = 0;
@@ -228,7 +234,7 @@ struct rgl_gpu {
 		tileDesc.release(); scanPart.release();
 		keys.release(); rec.release(); crec.release();
 		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
-		mlpImg.release();
+		mlpImg.release(); fwdImg.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release(); dLite.release();
 		if (dCtr) (void)hipFree(dCtr);
@@ -237,6 +243,7 @@ struct rgl_gpu {
 		if (hLite) (void)hipHostFree(hLite);
 		if (hFrames) (void)hipHostFree(hFrames);
 		dFrames.release();
+		if (auxDone) (void)hipEventDestroy(auxDone);
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
 		if (done) (void)hipEventDestroy(done);
@@ -381,7 +388,7 @@ struct rgl_gpu {
 		d.DS[0] = static_cast<float>(vx / 2); d.DS[1] = static_cast<float>(-vy / 2);
 		d.DO[0] = static_cast<float>(vx / 2 + s.viewportOrigin[0]);
 		d.DO[1] = static_cast<float>(H - (vy / 2 + s.viewportOrigin[1]));
-		// CalcGuardBandFactor (rglv_view_frustum.hxx:257-260)
+		// CalcGuardBandFactor (rglv_view_frustum.hxx:36-39)
 		const int half = std::max(W, H) / 2;
 		d.gbFactor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half);
 		if (fixedGuardBand && d.gbFactor < 1.0F) d.gbFactor = 1.0F;
@@ -539,6 +546,9 @@ struct rgl_gpu {
 		// frames of one rgl_gpu queue back to back on one stream; a frame on
 		// another stream first waits for the one in flight
 		if (pending && s != stream && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
+		// a network call on another stream (which may update the parameters a
+		// learned frame reads) completes before this frame's kernels start
+		if (auxPending && auxStream != s && (e = hipStreamWaitEvent(s, auxDone, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 		int r = Ensure(b, vtot, keyBase);
 		if (r != 0) return r;
 		b.frames = dFrames.p;
@@ -577,7 +587,11 @@ struct rgl_gpu {
 			keysReady = keys.cap; }
 		int pid = draws.empty() ? ProgManyBlinnPhong::kId : draws[0].program;
 		for (const auto& d : draws) if (d.program != pid) pid = -1;
-		const bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
+		// the network's output is the sRGB colour of the default post program:
+		// frames stored linear or through another post program shade plainly
+		bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
+		for (const FrameDesc& f : frames)
+			if (f.outColor && (f.gamma != 1 || f.postProgram != 1)) learned = false;
 		const std::vector<SampleReq>& reqs = recs[0]->sampleReqs;
 		if (!reqs.empty() && pid != ProgGGXEnv::kId) return fail(RGL_E_UNSUPPORTED, "training samples need the heavy-shader program");
 		// the launch sequence, replayed from a captured graph when enabled and
@@ -601,6 +615,27 @@ struct rgl_gpu {
 		return 0; }
 
 	int Run(hipStream_t s) { rgl_gpu* self = this; return RunBatch(&self, 1, s); }
+
+	// The network entry points run on their caller's stream ms and share the
+	// network image and training scratch with nothing a frame writes except
+	// the image, so: a frame still in flight is finished first, and a call on
+	// another stream than the previous network call's waits for that call
+	// (its scratch may be regrown).
+	// AuxEnd records the event that a later frame on another stream waits for.
+	int AuxBegin(hipStream_t ms) {
+		hipError_t e;
+		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
+		if (auxPending && auxStream != ms && (e = hipEventSynchronize(auxDone)) != hipSuccess)
+			return hipfail(e, "hipEventSynchronize");
+		if (!auxDone && (e = hipEventCreateWithFlags(&auxDone, hipEventDisableTiming)) != hipSuccess)
+			return hipfail(e, "hipEventCreate");
+		return 0; }
+	int AuxEnd(hipStream_t ms) {
+		hipError_t e;
+		if ((e = hipEventRecord(auxDone, ms)) != hipSuccess) return hipfail(e, "hipEventRecord");
+		auxPending = true;
+		auxStream = ms;
+		return 0; }
 
 	// the most tiles any binning unit of ut triangles can touch: from the
 	// first tile of its first triangle's frame to the last tile of its last
@@ -785,14 +820,14 @@ struct rgl_gpu {
 		default: return 3; }}
 
 	// ---- learned shading: one forward + backward pass over a batch ----
-	void Gemm(const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb, int ldc, int tA, int tB,
-	          const float* bias, int relu, const float* mask, int ldm, int kChunk, int nz) {
+	static void Gemm(hipStream_t ts, const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb, int ldc,
+	                 int tA, int tB, const float* bias, int relu, const float* mask, int ldm, int kChunk, int nz) {
 		GemmArgs g{ A, B, C, M, N, K, lda, ldb, ldc, tA, tB, bias, relu, mask, ldm, kChunk, kMlpTrainable };
-		k_gemm_bf16<<<dim3((N + 63) / 64, (M + 63) / 64, nz), 256, 0, stream>>>(g); }
+		k_gemm_bf16<<<dim3((N + 63) / 64, (M + 63) / 64, nz), 256, 0, ts>>>(g); }
 
 	// seed 0xffffffff: the first B samples in order.  lr > 0: Adam step on
 	// prm/opt; lr == 0: the summed gradient goes to gradOut.
-	int TrainPass(const float* samples, long long n, int B, float* prm, float* opt, float lr, float b1, float b2,
+	int TrainPass(hipStream_t ts, const float* samples, long long n, int B, float* prm, float* opt, float lr, float b1, float b2,
 	              int t, uint32_t seed, float* gradOut) {
 		hipError_t e;
 		const int kChunk = 256;
@@ -802,29 +837,29 @@ struct rgl_gpu {
 		for (auto* a : { &tT, &tY, &tdY }) if ((e = a->ensure(static_cast<size_t>(B) * 3)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		if ((e = tPart.ensure(static_cast<size_t>(nz) * kMlpTrainable)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		if ((e = tLoss.ensure(1)) != hipSuccess) return hipfail(e, "hipMalloc training");
-		k_mlp_batch<<<(B + 255) / 256, 256, 0, stream>>>(samples, n, B, seed, static_cast<uint32_t>(t), prm, tX.p, tT.p);
+		k_mlp_batch<<<(B + 255) / 256, 256, 0, ts>>>(samples, n, B, seed, static_cast<uint32_t>(t), prm, tX.p, tT.p);
 		// forward: H = relu(X W^T + b), Y = H3 W4^T + b4
-		Gemm(tX.p, prm + kW1, tH1.p, B, kMlpHid, kMlpK1, kMlpK1, kMlpK1, kMlpHid, 0, 1, prm + kB1, 1, nullptr, 0, kMlpK1, 1);
-		Gemm(tH1.p, prm + kW2, tH2.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB2, 1, nullptr, 0, kMlpHid, 1);
-		Gemm(tH2.p, prm + kW3, tH3.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB3, 1, nullptr, 0, kMlpHid, 1);
-		Gemm(tH3.p, prm + kW4, tY.p, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpOut, 0, 1, prm + kB4, 0, nullptr, 0, kMlpHid, 1);
-		k_fill_u32<<<1, 64, 0, stream>>>(reinterpret_cast<uint32_t*>(tLoss.p), 1, 0u);
-		k_mlp_loss<<<(3 * B + 255) / 256, 256, 0, stream>>>(tY.p, tT.p, B, tdY.p, tLoss.p);
+		Gemm(ts, tX.p, prm + kW1, tH1.p, B, kMlpHid, kMlpK1, kMlpK1, kMlpK1, kMlpHid, 0, 1, prm + kB1, 1, nullptr, 0, kMlpK1, 1);
+		Gemm(ts, tH1.p, prm + kW2, tH2.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB2, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(ts, tH2.p, prm + kW3, tH3.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB3, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(ts, tH3.p, prm + kW4, tY.p, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpOut, 0, 1, prm + kB4, 0, nullptr, 0, kMlpHid, 1);
+		k_fill_u32<<<1, 64, 0, ts>>>(reinterpret_cast<uint32_t*>(tLoss.p), 1, 0u);
+		k_mlp_loss<<<(3 * B + 255) / 256, 256, 0, ts>>>(tY.p, tT.p, B, tdY.p, tLoss.p);
 		// backward: weight gradients split over K = samples into nz partials
 		float* P = tPart.p;
-		Gemm(tdY.p, tH3.p, P + kW4, kMlpOut, kMlpHid, B, kMlpOut, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdY.p, B, kMlpOut, kChunk, P + kB4, kMlpTrainable);
-		Gemm(tdY.p, prm + kW4, tdA.p, B, kMlpHid, kMlpOut, kMlpOut, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH3.p, kMlpHid, kMlpOut, 1);
-		Gemm(tdA.p, tH2.p, P + kW3, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdA.p, B, kMlpHid, kChunk, P + kB3, kMlpTrainable);
-		Gemm(tdA.p, prm + kW3, tdB.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH2.p, kMlpHid, kMlpHid, 1);
-		Gemm(tdB.p, tH1.p, P + kW2, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdB.p, B, kMlpHid, kChunk, P + kB2, kMlpTrainable);
-		Gemm(tdB.p, prm + kW2, tdA.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH1.p, kMlpHid, kMlpHid, 1);
-		Gemm(tdA.p, tX.p, P + kW1, kMlpHid, kMlpK1, B, kMlpHid, kMlpK1, kMlpK1, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, stream>>>(tdA.p, B, kMlpHid, kChunk, P + kB1, kMlpTrainable);
+		Gemm(ts, tdY.p, tH3.p, P + kW4, kMlpOut, kMlpHid, B, kMlpOut, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdY.p, B, kMlpOut, kChunk, P + kB4, kMlpTrainable);
+		Gemm(ts, tdY.p, prm + kW4, tdA.p, B, kMlpHid, kMlpOut, kMlpOut, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH3.p, kMlpHid, kMlpOut, 1);
+		Gemm(ts, tdA.p, tH2.p, P + kW3, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdA.p, B, kMlpHid, kChunk, P + kB3, kMlpTrainable);
+		Gemm(ts, tdA.p, prm + kW3, tdB.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH2.p, kMlpHid, kMlpHid, 1);
+		Gemm(ts, tdB.p, tH1.p, P + kW2, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdB.p, B, kMlpHid, kChunk, P + kB2, kMlpTrainable);
+		Gemm(ts, tdB.p, prm + kW2, tdA.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH1.p, kMlpHid, kMlpHid, 1);
+		Gemm(ts, tdA.p, tX.p, P + kW1, kMlpHid, kMlpK1, B, kMlpHid, kMlpK1, kMlpK1, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdA.p, B, kMlpHid, kChunk, P + kB1, kMlpTrainable);
 		const float bc1 = 1.0F - std::pow(b1, static_cast<float>(t)), bc2 = 1.0F - std::pow(b2, static_cast<float>(t));
-		k_mlp_adam<<<(kMlpTrainable + 255) / 256, 256, 0, stream>>>(P, nz, prm, opt, opt ? opt + kMlpTrainable : nullptr,
+		k_mlp_adam<<<(kMlpTrainable + 255) / 256, 256, 0, ts>>>(P, nz, prm, opt, opt ? opt + kMlpTrainable : nullptr,
 			lr, b1, b2, 1e-8F, bc1 > 0.0F ? bc1 : 1.0F, bc2 > 0.0F ? bc2 : 1.0F, gradOut);
 		if ((e = hipGetLastError()) != hipSuccess) return hipfail(e, "training launch");
 		return 0; }
@@ -880,6 +915,7 @@ int rgl_gpu_create(int device_ordinal, int concurrency, const char* guid, rgl_gp
 int rgl_gpu_destroy(rgl_gpu* g) {
 	if (!g) return RGL_E_ARG;
 	if (g->pending) (void)g->Wait();
+	if (g->auxPending) (void)hipEventSynchronize(g->auxDone);
 	delete g;
 	return 0; }
 
@@ -1107,27 +1143,32 @@ int rgl_mlp_forward(rgl_gpu* g, const float* dev_samples, int n, const float* de
 	if (check(g)) return RGL_E_ARG;
 	if (!dev_samples || !dev_params || !dev_out || n < 0) return g->fail(RGL_E_ARG, "bad forward arguments");
 	if (n == 0) return 0;
-	g->stream = static_cast<hipStream_t>(hip_stream);
+	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
+	int r = g->AuxBegin(ms);
+	if (r != 0) return r;
 	hipError_t e;
-	if ((e = g->mlpImg.ensure(kImgFloats)) != hipSuccess) return g->hipfail(e, "hipMalloc network image");
-	k_mlp_pack<<<16, 256, 0, g->stream>>>(dev_params, g->mlpImg.p);
+	if ((e = g->fwdImg.ensure(kImgFloats)) != hipSuccess) return g->hipfail(e, "hipMalloc network image");
+	k_mlp_pack<<<16, 256, 0, ms>>>(dev_params, g->fwdImg.p);
 	const int per = 64 * kMlpWaves;
-	k_mlp_forward<<<(n + per - 1) / per, per, 0, g->stream>>>(dev_samples, n, g->mlpImg.p, dev_out);
+	k_mlp_forward<<<(n + per - 1) / per, per, 0, ms>>>(dev_samples, n, g->fwdImg.p, dev_out);
 	if ((e = hipGetLastError()) != hipSuccess) return g->hipfail(e, "forward launch");
-	return 0; }
+	return g->AuxEnd(ms); }
 
 int rgl_mlp_gradient(rgl_gpu* g, const float* dev_samples, int n, float* dev_params, float* dev_grad, float* loss_out,
                      void* hip_stream) {
 	if (check(g)) return RGL_E_ARG;
 	if (!dev_samples || !dev_params || !dev_grad || n <= 0) return g->fail(RGL_E_ARG, "bad gradient arguments");
-	g->stream = static_cast<hipStream_t>(hip_stream);
-	int r = g->TrainPass(dev_samples, n, n, dev_params, nullptr, 0.0F, 0.9F, 0.999F, 1, 0xffffffffu, dev_grad);
+	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
+	int r = g->AuxBegin(ms);
 	if (r != 0) return r;
+	r = g->TrainPass(ms, dev_samples, n, n, dev_params, nullptr, 0.0F, 0.9F, 0.999F, 1, 0xffffffffu, dev_grad);
+	if (r != 0) return r;
+	if ((r = g->AuxEnd(ms)) != 0) return r;
 	if (loss_out) {
 		hipError_t e;
 		float s = 0.0F;
-		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream)) != hipSuccess) return g->hipfail(e, "loss copy");
-		if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hipfail(e, "loss sync");
+		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, ms)) != hipSuccess) return g->hipfail(e, "loss copy");
+		if ((e = hipStreamSynchronize(ms)) != hipSuccess) return g->hipfail(e, "loss sync");
 		*loss_out = s / (3.0F * static_cast<float>(n)); }
 	return 0; }
 
@@ -1136,16 +1177,19 @@ int rgl_mlp_train(rgl_gpu* g, const float* dev_samples, long long n, float* dev_
 	if (check(g)) return RGL_E_ARG;
 	if (!dev_samples || !dev_params || !dev_opt || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.0F) || step0 < 0)
 		return g->fail(RGL_E_ARG, "bad training arguments");
-	g->stream = static_cast<hipStream_t>(hip_stream);
+	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
+	int r = g->AuxBegin(ms);
+	if (r != 0) return r;
 	for (int s = 0; s < steps; ++s) {
-		const int r = g->TrainPass(dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
-		                           seed == 0xffffffffu ? 0u : seed, nullptr);
+		r = g->TrainPass(ms, dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
+		                 seed == 0xffffffffu ? 0u : seed, nullptr);
 		if (r != 0) return r; }
+	if ((r = g->AuxEnd(ms)) != 0) return r;
 	if (loss_out) {
 		hipError_t e;
 		float s = 0.0F;
-		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, g->stream)) != hipSuccess) return g->hipfail(e, "loss copy");
-		if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hipfail(e, "loss sync");
+		if ((e = hipMemcpyAsync(&s, g->tLoss.p, sizeof(float), hipMemcpyDeviceToHost, ms)) != hipSuccess) return g->hipfail(e, "loss copy");
+		if ((e = hipStreamSynchronize(ms)) != hipSuccess) return g->hipfail(e, "loss sync");
 		*loss_out = s / (3.0F * static_cast<float>(batch)); }
 	return 0; }
 

@@ -157,7 +157,7 @@ __device__ inline void vertex_one(const DrawDesc& d, long long gi, float4* __res
 	const int v = static_cast<int>(i - static_cast<long long>(iid) * d.verts);
 	float pos[4], vo[P::kVary];
 	P::vertex(d, iid, v, pos, vo);
-	// ViewFrustum::Test, SIMD form with cmple (rglv_view_frustum.hxx:281-294)
+	// ViewFrustum::Test, SIMD form with cmple (rglv_view_frustum.hxx:60-73)
 	const float w = pos[3] * d.gbFactor;
 	uint32_t f = 0;
 	if (w + pos[0] <= 0.0F) f |= 1u;

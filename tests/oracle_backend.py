@@ -15,7 +15,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "librgl_oracle.so")
+# RGL_ORACLE_SO points the tests at another build of the oracle (the
+# mutation check in oracle/mutation_check.py uses it)
+ORACLE_SO = os.environ.get("RGL_ORACLE_SO", os.path.join(ORACLE_DIR, "librgl_oracle.so"))
 
 _lib = None
 
