@@ -622,11 +622,21 @@ struct rglo_gpu {
 		const int ty = t / tilesX, tx = t % tilesX;
 		return { tx * tileW, ty * tileH, std::min((tx + 1) * tileW, W), std::min((ty + 1) * tileH, H) }; }
 
-	Rect ScissorRect(const GLState& s) const {  // (rglv_gpu.hxx:272-276)
+	// GPU::ScissorRect (rglv_gpu.hxx:258-276), clamped to the target: a rect
+	// reaching past it makes the reference bin into tiles outside its grid
+	Rect ScissorRect(const GLState& s) const {
 		if (!s.scissorEnabled) return { 0, 0, W, H };
 		const int left = s.scissorOrigin[0], right = left + s.scissorSize[0];
 		const int bottom = H - s.scissorOrigin[1] - 1, top = bottom - s.scissorSize[1];
-		return { left, top, right, bottom }; }
+		auto cl = [](int v, int hi) { return std::min(std::max(v, 0), hi); };
+		return { cl(left, W), cl(top, H), cl(right, W), cl(bottom, H) }; }
+
+	// the rect a tile rasterises a draw in: the tile rect, intersected with
+	// the scissor rect when the scissor test is on (rglv_gpu_impl.hxx:848-849)
+	Rect RasterRect(const Rect& tr, const GLState& s) const {
+		if (!s.scissorEnabled) return tr;
+		const Rect sr = ScissorRect(s);
+		return { std::max(tr.x0, sr.x0), std::max(tr.y0, sr.y0), std::min(tr.x1, sr.x1), std::min(tr.y1, sr.y1) }; }
 
 	void DSDO(const GLState& s, float DS[2], float DO[2]) const {  // (rglv_gpu.hxx:265-270)
 		const int vx = s.hasViewportSize ? s.viewportSize[0] : W;
@@ -890,13 +900,22 @@ struct rglo_gpu {
 					tb.r[p] = color[0][l]; tb.g[p] = color[1][l]; tb.b[p] = color[2][l]; }}}
 		for (int l = 0; l < 4; ++l) if (mask[l]) tb.prim[pix[l]] = tc.prim; }
 
-	// VTriangleRasterizer::Draw (rglv_triangle.hxx:193-304), one lane
-	void RasterVector(TileBuf& tb, const Rect& tr, const TriCtx& tc, const int32_t X[3], const int32_t Y[3]) const {
+	// ScissorFlags (rglv_triangle.hxx:20-29): the raster rect's odd edges
+	static int ScissorFlags(const Rect& r) {
+		return ((r.y0 & 1) ? 1 : 0) | ((r.y1 & 1) ? 2 : 0) | ((r.x0 & 1) ? 4 : 0) | ((r.x1 & 1) ? 8 : 0); }
+
+	// VTriangleRasterizer::Draw (rglv_triangle.hxx:193-304), one lane; tr is
+	// the tile (buffer addressing), rr the raster rect, scissor the
+	// SCISSOR_TEST template argument (lanes of quads on the rect's odd edges
+	// masked, :270-300)
+	void RasterVector(TileBuf& tb, const Rect& tr, const Rect& rr, bool scissor, const TriCtx& tc, const int32_t X[3],
+	                  const int32_t Y[3]) const {
 		const int32_t x1 = X[0], x2 = X[1], x3 = X[2], y1 = Y[0], y2 = Y[1], y3 = Y[2];
-		int32_t minx = std::max(sar(std::min(std::min(x1, x2), x3), 4), tr.x0);
-		const int32_t maxx = std::min(sar(wadd(std::max(std::max(x1, x2), x3), 15), 4), tr.x1);
-		int32_t miny = std::max(sar(std::min(std::min(y1, y2), y3), 4), tr.y0);
-		const int32_t maxy = std::min(sar(wadd(std::max(std::max(y1, y2), y3), 15), 4), tr.y1);
+		int32_t minx = std::max(sar(std::min(std::min(x1, x2), x3), 4), rr.x0);
+		const int32_t maxx = std::min(sar(wadd(std::max(std::max(x1, x2), x3), 15), 4), rr.x1);
+		int32_t miny = std::max(sar(std::min(std::min(y1, y2), y3), 4), rr.y0);
+		const int32_t maxy = std::min(sar(wadd(std::max(std::max(y1, y2), y3), 15), 4), rr.y1);
+		const int sflags = scissor ? ScissorFlags(rr) : 0;
 		minx &= ~1; miny &= ~1;
 		const int32_t dx12 = wsub(x1, x2), dy12 = wsub(y2, y1);
 		const int32_t dx23 = wsub(x2, x3), dy23 = wsub(y3, y2);
@@ -913,6 +932,8 @@ struct rglo_gpu {
 		const float topRow = static_cast<float>(H) - 0.5F;
 		int32_t r1 = c1, r2 = c2, r3 = c3;
 		for (int y = miny; y < maxy; y += 2, r1 = wadd(r1, wmul(dx12, 2)), r2 = wadd(r2, wmul(dx23, 2)), r3 = wadd(r3, wmul(dx31, 2))) {
+			const int32_t sTop = (y == (rr.y0 & ~1) && (sflags & 1)) ? -1 : 0;
+			const int32_t sBot = (y >= (rr.y1 & ~1) && (sflags & 2)) ? -1 : 0;
 			int32_t q1 = r1, q2 = r2, q3 = r3;
 			for (int x = minx; x < maxx; x += 2, q1 = wadd(q1, wmul(dy12, 2)), q2 = wadd(q2, wmul(dy23, 2)), q3 = wadd(q3, wmul(dy31, 2))) {
 				int32_t e1[4] = { q1, wadd(q1, dy12), wadd(q1, dx12), wadd(wadd(q1, dx12), dy12) };
@@ -922,6 +943,10 @@ struct rglo_gpu {
 				for (int l = 0; l < 4; ++l) { trimask[l] = (e1[l] | e2[l] | e3[l]) >> 31; neg += trimask[l] ? 1 : 0; }
 				++tb.quads;
 				if (neg == 4) continue;
+				if (scissor) {
+					const int32_t sLeft = (x == (rr.x0 & ~1) && (sflags & 4)) ? -1 : 0;
+					const int32_t sRight = (x >= (rr.x1 & ~1) && (sflags & 8)) ? -1 : 0;
+					trimask[0] |= sLeft | sTop; trimask[1] |= sRight | sTop; trimask[2] |= sLeft | sBot; trimask[3] |= sRight | sBot; }
 				float fc[2][4], bs[3][4];
 				const float fx = static_cast<float>(x) + 0.5F;
 				const float fy = topRow - static_cast<float>(y);
@@ -933,13 +958,16 @@ struct rglo_gpu {
 					bs[1][l] = (1.0F - bs[0][l]) - bs[2][l]; }
 				Render(tb, tr, tc, x, y, fc, trimask, bs); }}}
 
-	// TriangleRasterizer::Draw (rglv_triangle.hxx:79-167), clipped triangles
-	void RasterScalar(TileBuf& tb, const Rect& tr, const TriCtx& tc, const int32_t X[3], const int32_t Y[3]) const {
+	// TriangleRasterizer::Draw (rglv_triangle.hxx:79-167), clipped triangles;
+	// tr, rr and scissor as for RasterVector
+	void RasterScalar(TileBuf& tb, const Rect& tr, const Rect& rr, bool scissor, const TriCtx& tc, const int32_t X[3],
+	                  const int32_t Y[3]) const {
 		const int x1 = X[0], x2 = X[1], x3 = X[2], y1 = Y[0], y2 = Y[1], y3 = Y[2];
-		int minx = std::max(std::min({ x1, x2, x3 }) >> 4, tr.x0);
-		const int endx = std::min((std::max({ x1, x2, x3 }) + 15) >> 4, tr.x1);
-		int miny = std::max(std::min({ y1, y2, y3 }) >> 4, tr.y0);
-		const int endy = std::min((std::max({ y1, y2, y3 }) + 15) >> 4, tr.y1);
+		int minx = std::max(std::min({ x1, x2, x3 }) >> 4, rr.x0);
+		const int endx = std::min((std::max({ x1, x2, x3 }) + 15) >> 4, rr.x1);
+		int miny = std::max(std::min({ y1, y2, y3 }) >> 4, rr.y0);
+		const int endy = std::min((std::max({ y1, y2, y3 }) + 15) >> 4, rr.y1);
+		const int sflags = scissor ? ScissorFlags(rr) : 0;
 		minx &= ~1; miny &= ~1;
 		const int64_t dx12 = x1 - x2, dy12 = y2 - y1;
 		const int64_t dx23 = x2 - x3, dy23 = y3 - y2;
@@ -963,11 +991,17 @@ struct rglo_gpu {
 		const int32_t s3x = wmul(static_cast<int32_t>(dy31), 2), s3y = wmul(static_cast<int32_t>(dx31), 2);
 		const float scale = 1.0F / static_cast<float>(c1 + c2 + c3);
 		for (int y = miny; y < endy; y += 2) {
+			const int32_t sTop = (y == (rr.y0 & ~1) && (sflags & 1)) ? -1 : 0;
+			const int32_t sBot = (y >= (rr.y1 & ~1) && (sflags & 2)) ? -1 : 0;
 			int32_t q1[4], q2[4], q3[4];
 			std::memcpy(q1, cb1, sizeof q1); std::memcpy(q2, cb2, sizeof q2); std::memcpy(q3, cb3, sizeof q3);
 			for (int x = minx; x < endx; x += 2) {
 				int32_t trimask[4]; int neg = 0;
 				for (int l = 0; l < 4; ++l) { trimask[l] = (q1[l] | q2[l] | q3[l]) >> 31; neg += trimask[l] ? 1 : 0; }
+				if (neg != 4 && scissor) {
+					const int32_t sLeft = (x == (rr.x0 & ~1) && (sflags & 4)) ? -1 : 0;
+					const int32_t sRight = (x >= (rr.x1 & ~1) && (sflags & 8)) ? -1 : 0;
+					trimask[0] |= sLeft | sTop; trimask[1] |= sRight | sTop; trimask[2] |= sLeft | sBot; trimask[3] |= sRight | sBot; }
 				if (neg != 4) {
 					float fc[2][4], bs[3][4];
 					const float fx = static_cast<float>(x) + 0.5F;
@@ -1028,7 +1062,7 @@ struct rglo_gpu {
 					X[k] = cvtt(16.0F * (dx * DS[0] + DO[0]));
 					Y[k] = cvtt(16.0F * (dy * DS[1] + DO[1]));
 					tc.w[k] = iw; tc.z[k] = dz; }
-				RasterVector(tb, tr, tc, X, Y);
+				RasterVector(tb, tr, RasterRect(tr, *st), st->scissorEnabled, tc, X, Y);
 				break; }
 			case TileOp::CLIPPED_TRI: {  // DrawClipped (:949-998)
 				if (!pgm) { err = "clipped triangle outside a draw"; return false; }
@@ -1041,7 +1075,7 @@ struct rglo_gpu {
 					X[k] = static_cast<int32_t>(cvtt(cv.coord[0] * 16.0F));
 					Y[k] = static_cast<int32_t>(cvtt(cv.coord[1] * 16.0F));
 					tc.w[k] = cv.coord[3]; tc.z[k] = cv.coord[2]; tc.vo[k] = cv.data; }
-				RasterScalar(tb, tr, tc, X, Y);
+				RasterScalar(tb, tr, RasterRect(tr, *st), st->scissorEnabled, tc, X, Y);
 				break; }
 			case TileOp::STORE_TC: {  // GPUBltImpl::StoreTrueColor -> FilterTile (rglr_algorithm.hxx:31-104)
 				if (!known_post_program(st->programId)) { err = "no blt dispatch entry"; return false; }
@@ -1243,7 +1277,7 @@ int rglo_scalar_triangle(int w, int h, const float* pts /*3 x vec4*/, const floa
 		tc.vo[k].f[0] = uvs[k*2 + 0]; tc.vo[k].f[1] = uvs[k*2 + 1]; }
 	const Rect r{ 0, 0, w, h };
 	// pass 1: BS and u; pass 2: v (the capture program has three colour channels)
-	g.RasterScalar(tb, r, tc, X, Y);
+	g.RasterScalar(tb, r, r, false, tc, X, Y);
 	std::vector<float> bsx = tb.r, bsy = tb.g, bsz = tb.b;
 	std::vector<uint32_t> hit = tb.prim;
 	struct CaptureUV : Program {
@@ -1255,7 +1289,7 @@ int rglo_scalar_triangle(int w, int h, const float* pts /*3 x vec4*/, const floa
 	static const CaptureUV capuv;
 	tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F); tb.d.assign(area, 1.0F); tb.prim.assign(area, 0xffffffffu);
 	tc.pgm = &capuv;
-	g.RasterScalar(tb, r, tc, X, Y);
+	g.RasterScalar(tb, r, r, false, tc, X, Y);
 	for (size_t p = 0; p < area; ++p) {
 		const bool on = hit[p] != 0xffffffffu;
 		out[p*6 + 0] = on ? 1.0F : 0.0F;
@@ -1293,8 +1327,8 @@ int rglo_raster_tri(int mode, int x0, int y0, int x1, int y1, int h, const int32
 	rglo_gpu::TriCtx tc{ &cap, &env, {}, {}, {}, 0u, GL_LESS, false, false, true, false };
 	for (int k = 0; k < 3; ++k) { tc.w[k] = 1.0F; tc.z[k] = 0.5F; }
 	const Rect r{ x0, y0, x1, y1 };
-	if (mode == 0) g.RasterVector(tb, r, tc, X, Y);
-	else g.RasterScalar(tb, r, tc, X, Y);
+	if (mode == 0) g.RasterVector(tb, r, r, false, tc, X, Y);
+	else g.RasterScalar(tb, r, r, false, tc, X, Y);
 	for (int y = 0; y < rh; ++y)
 		for (int x = 0; x < rw; ++x) {
 			const size_t p = static_cast<size_t>(y) * g.tileW + x;

@@ -156,6 +156,10 @@ struct DrawDesc {
 	int scissor[4];          // x0, y0, x1, y1 (GPU::ScissorRect)
 	int keepBacks, keepFronts;
 	int cullingEnabled, cullFace;
+	// fragment state of the in-order tile path (FragmentStateKey,
+	// rglv_gl.hxx:136-146): depth test and function, depth and colour write
+	// masks, alpha blending, scissor (scissor[] then clips the raster rect)
+	int depthTest, depthFunc, depthWrite, colorWrite, blend, scissorOn;
 	float uniforms[32];      // UniformsSD (<= 128 bytes)
 	SamplerDesc tu0, tu1;
 	// the frame this draw renders into and its tile grid
@@ -180,6 +184,9 @@ struct FrameDesc {
 	uint32_t* outColor; int gamma; int postProgram; float exposure;
 	float* outDepth;
 	uint32_t* outPrim;
+	// 1: drawn by the in-order tile path (its commands opBase .. opBase +
+	// nOps of the batch's frame-command table apply in submission order)
+	int inOrder, opBase, nOps;
 };
 
 // a batch of frames drawn by one launch sequence: triangles, tiles and
@@ -296,9 +303,9 @@ struct ProgAmy {
 		xform4(d.vpm, p, pos);
 		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
 		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
 		const float4 t = sample1(d.tu0, v[0], v[1], lod);
-		color[0] = t.x; color[1] = t.y; color[2] = t.z; }};
+		color[0] = t.x; color[1] = t.y; color[2] = t.z; color[3] = t.w; }};
 
 // ManyProgram (viewer/shaders.hxx:482-583)
 struct ProgMany {
@@ -313,8 +320,8 @@ struct ProgMany {
 		xform4(d.vpm, p1, pos);
 		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
 		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int, float color[3]) {
-		color[0] = v[0]; color[1] = v[1]; color[2] = d.uniforms[0]; }};
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int, float color[4]) {
+		color[0] = v[0]; color[1] = v[1]; color[2] = d.uniforms[0]; color[3] = 1.0F; }};
 
 // LambertTex (this project): textured, one directional light.
 // uniforms: [0..2] unit direction toward the light, [4] ambient, [5] diffuse
@@ -326,9 +333,10 @@ struct ProgLambertTex {
 		vo[0] = buf_at(d.buffers[3], idx, 0.0F); vo[1] = buf_at(d.buffers[4], idx, 0.0F);
 		vo[2] = buf_at(d.buffers[5], idx, 1.0F);
 		vo[3] = buf_at(d.buffers[9], idx, 0.0F); vo[4] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
 		const float* u = d.uniforms;
 		const float4 tex = sample1(d.tu0, v[3], v[4], lod);
+		color[3] = 1.0F;
 		float nx = v[0], ny = v[1], nz = v[2];
 		normalize3(nx, ny, nz);
 		const float ndl = sse_max((nx*u[0] + ny*u[1]) + nz*u[2], 0.0F);
@@ -352,9 +360,10 @@ struct ProgManyBlinnPhong {
 		vo[0] = p1[0]; vo[1] = p1[1]; vo[2] = p1[2];
 		vo[3] = wn[0]; vo[4] = wn[1]; vo[5] = wn[2];
 		vo[6] = buf_at(d.buffers[9], idx, 0.0F); vo[7] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
 		const float* u = d.uniforms;
 		const float4 tex = sample1(d.tu0, v[6], v[7], lod);
+		color[3] = 1.0F;
 		const float px = v[0], py = v[1], pz = v[2];
 		float nx = v[3], ny = v[4], nz = v[5];
 		normalize3(nx, ny, nz);
@@ -486,10 +495,11 @@ struct ProgGGXEnv {
 		normalize3(vx, vy, vz);
 		s.v[0] = vx; s.v[1] = vy; s.v[2] = vz;
 		s.rough = v[8]; }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[3]) {
+	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
 		ShadeInput s;
 		inputs(d, v, lod, s);
-		ggx_shade(d, s, color); }};
+		ggx_shade(d, s, color);
+		color[3] = 1.0F; }};
 
 __host__ __device__ inline int program_nvary(int pid) {
 	switch (pid) {

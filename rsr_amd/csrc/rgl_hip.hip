@@ -14,6 +14,7 @@
 
 #include "../../include/rgl_hip.h"
 #include "rgl_mlp.h"
+#include "rgl_inorder.h"
 
 using namespace rgl;
 
@@ -22,11 +23,12 @@ namespace {
 // GL constants (rglv_gl.hxx:20-62)
 constexpr int GL_CULL_FACE = 1, GL_SCISSOR_TEST = 2, GL_BLEND = 3, GL_DEPTH_TEST = 4;
 constexpr int GL_BACK = 2, GL_FRONT = 1;
-constexpr int GL_LESS = 0, GL_LEQUAL = 1;
+constexpr int GL_LESS = 0, GL_LEQUAL = 1, GL_EQUAL = 2;
 constexpr int GL_COLOR_BUFFER_BIT = 1, GL_DEPTH_BUFFER_BIT = 2, GL_STENCIL_BUFFER_BIT = 4;
 constexpr int GL_DEPTH_ATTACHMENT = 0, GL_COLOR_ATTACHMENT0 = 2;
 constexpr int RB_COLOR_DEPTH = 0, RB_RGBF32 = 1, RB_RGBAF32 = 2, RB_F32 = 3;
 constexpr int UNIFORM_BUFFER_SIZE = 32;
+constexpr int kMaxOps = 1024;     // clears and stores per batch
 
 struct TexState { const float* ptr; int width, height, stride, filter; };
 
@@ -165,6 +167,16 @@ struct rgl_gpu {
 	FrameDesc* hFrames = nullptr;   // pinned, kMaxFrames
 	DevArray<FrameDesc> dFrames;
 	std::vector<uint8_t> lastFrames;
+	// the in-order tile path: frame commands, in-order frames, sorted tile lists
+	FrameOp* hOps = nullptr;        // pinned, kMaxOps
+	DevArray<FrameOp> dOps;
+	std::vector<uint8_t> lastOps;
+	IoFrame* hIo = nullptr;         // pinned, kMaxFrames
+	DevArray<IoFrame> dIo;
+	std::vector<uint8_t> lastIo;
+	int nIo = 0, nIoTiles = 0, nIoBlocks = 0;
+	DevArray<uint32_t> order;
+	DevArray<unsigned long long> skeys0, skeys1;
 	int segPairs = kSegPairs;       // (triangle, tile) pairs per visibility work item
 	int splitQuads = kSplitQuads;   // quad work per quadrant from which visibility regions are halved
 	// graph replay of the launch sequence (rgl_gpu_set_graph)
@@ -243,6 +255,9 @@ struct rgl_gpu {
 		if (hLite) (void)hipHostFree(hLite);
 		if (hFrames) (void)hipHostFree(hFrames);
 		dFrames.release();
+		if (hOps) (void)hipHostFree(hOps);
+		if (hIo) (void)hipHostFree(hIo);
+		dOps.release(); dIo.release(); order.release(); skeys0.release(); skeys1.release();
 		if (auxDone) (void)hipEventDestroy(auxDone);
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
@@ -266,77 +281,122 @@ struct rgl_gpu {
 	void ResetGL() { dirty = true; states.clear(); cs.reset(); ubuf.clear(); commands.clear(); sampleReqs.clear(); }
 
 	// ---- frame planning: this recorder's commands as frame fi of a batch;
-	// draws are appended with batch-wide triangle and vertex numbering ----
-	int PlanFrame(FrameDesc& f, std::vector<DrawDesc>& draws, long long& triBase, long long& vBase, int fi, int tileBase) {
+	// draws are appended with batch-wide triangle and vertex numbering, the
+	// frame's clears and stores to ops (its tile commands, in order) ----
+	//
+	// A frame takes the order-independent key path when it is one full
+	// clear, then draws that all test depth LESS (or all LEQUAL) and write
+	// depth and colour without blending or scissor, then at most one store
+	// of each kind.  Any other command sequence is drawn in order per tile.
+	int PlanFrame(FrameDesc& f, std::vector<DrawDesc>& draws, std::vector<FrameOp>& ops, long long& triBase, long long& vBase,
+	              int fi, int tileBase) {
 		std::memset(&f, 0, sizeof(f));
 		f.W = W; f.H = H; f.tileW = tileW; f.tileH = tileH; f.tilesX = tilesX; f.tilesY = tilesY;
 		f.nTiles = tilesX * tilesY;
 		f.tileBase = tileBase;
 		f.depthFunc = -1;
 		f.drawEnabled = 1;
+		f.opBase = static_cast<int>(ops.size());
 		const GLState* st = nullptr;
-		bool cleared = false, stored = false;
-		const size_t firstDraw = draws.size();
+		bool ordered = false;             // the frame needs the in-order tile path
+		int clears = 0, stores[3] = { 0, 0, 0 };
+		bool stored = false, drawn = false;
 		frameTriStart = triBase;
 		submitted = 0;
 		if (W <= 0 || H <= 0) return fail(RGL_E_STATE, "Reset() must set the target size before Run");
 		for (const Cmd& c : commands) {
+			const uint32_t pos = static_cast<uint32_t>(1 + kSlotsPerTri * triBase);   // the next triangle's first slot
+			FrameOp op;
+			std::memset(&op, 0, sizeof(op));
+			op.pos = pos;
 			switch (c.kind) {
 			case Cmd::STATE: st = &states[c.a]; break;
-			case Cmd::CLEAR: {
-				if (draws.size() != firstDraw || stored) return fail(RGL_E_UNSUPPORTED, "CLEAR after a draw or store is not supported");
+			case Cmd::CLEAR: {   // GPU::DrawImpl CMD_CLEAR (rglv_gpu.cxx:292-324)
+				if (!st) return fail(RGL_E_STATE, "clear without state");
 				if (c.a & GL_STENCIL_BUFFER_BIT) return fail(RGL_E_UNSUPPORTED, "CLEAR on STENCIL not implemented");
 				if (st->color0AttachmentType == RB_COLOR_DEPTH && c.a != (GL_COLOR_BUFFER_BIT | GL_DEPTH_BUFFER_BIT))
 					return fail(RGL_E_STATE, "must clear color and depth when using RB_COLOR_DEPTH");
+				if ((c.a & GL_COLOR_BUFFER_BIT) && st->color0AttachmentType != RB_COLOR_DEPTH && st->color0AttachmentType != RB_RGBF32 &&
+				    st->color0AttachmentType != RB_RGBAF32) return fail(RGL_E_UNSUPPORTED, "can't clear color buffer of this type");
+				if ((c.a & GL_DEPTH_BUFFER_BIT) && st->color0AttachmentType != RB_COLOR_DEPTH && st->depthAttachmentType != RB_F32)
+					return fail(RGL_E_UNSUPPORTED, "can't clear depth buffer of this type");
+				op.kind = kOpClear; op.bits = c.a & 3;
+				for (int k = 0; k < 3; ++k) op.color[k] = st->clearColor[k];
+				op.depth = st->clearDepth;
+				if (drawn || stored || clears > 0 || op.bits != 3) ordered = true;
+				++clears;
 				f.doClearColor = (c.a & GL_COLOR_BUFFER_BIT) != 0;
 				f.doClearDepth = (c.a & GL_DEPTH_BUFFER_BIT) != 0;
 				for (int k = 0; k < 3; ++k) f.clearColor[k] = st->clearColor[k];
 				f.clearDepth = st->clearDepth;
-				cleared = true;
+				ops.push_back(op);
 				break; }
 			case Cmd::STORE_TC:
+				if (!st) return fail(RGL_E_STATE, "store without state");
 				if (!Installed((static_cast<uint32_t>(st->programId) << 24) | 0x40000000u))
 					return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_STORE_COLOR_FULL_LINEAR_TC");
 				if (st->programId != 1 && st->programId != 2) return fail(RGL_E_UNSUPPORTED, "post program not implemented");
+				op.kind = kOpStoreColor; op.ptr = const_cast<void*>(c.ptr); op.gamma = c.a; op.post = st->programId;
+				op.exposure = (st->programId == 2 && st->uniformsOfs >= 0) ? ubuf[st->uniformsOfs] : 1.0F;
 				f.outColor = static_cast<uint32_t*>(const_cast<void*>(c.ptr));
-				f.gamma = c.a; f.postProgram = st->programId;
-				f.exposure = (st->programId == 2 && st->uniformsOfs >= 0) ? ubuf[st->uniformsOfs] : 1.0F;
+				f.gamma = op.gamma; f.postProgram = op.post; f.exposure = op.exposure;
+				if (stores[0]++) ordered = true;
 				stored = true;
+				ops.push_back(op);
 				break;
 			case Cmd::STORE_DEPTH:
+				if (!st) return fail(RGL_E_STATE, "store without state");
 				if (st->depthAttachmentType != RB_F32 && st->depthAttachmentType != RB_COLOR_DEPTH)
 					return fail(RGL_E_UNSUPPORTED, "STORE_DEPTH_FULL_LINEAR_FP not implemented for attachment type");
-				f.outDepth = static_cast<float*>(const_cast<void*>(c.ptr)); stored = true; break;
+				op.kind = kOpStoreDepth; op.ptr = const_cast<void*>(c.ptr);
+				f.outDepth = static_cast<float*>(const_cast<void*>(c.ptr));
+				if (stores[1]++) ordered = true;
+				stored = true;
+				ops.push_back(op);
+				break;
 			case Cmd::STORE_PRIM:
-				f.outPrim = static_cast<uint32_t*>(const_cast<void*>(c.ptr)); stored = true; break;
+				op.kind = kOpStorePrim; op.ptr = const_cast<void*>(c.ptr);
+				f.outPrim = static_cast<uint32_t*>(const_cast<void*>(c.ptr));
+				if (stores[2]++) ordered = true;
+				stored = true;
+				ops.push_back(op);
+				break;
 			case Cmd::STORE_SAMPLES:
 				stored = true; break;
 			default: {
-				if (stored) return fail(RGL_E_UNSUPPORTED, "draw after a store is not supported");
 				if (!st) return fail(RGL_E_STATE, "draw without state");
 				if (static_cast<int>(draws.size()) >= kMaxDraws) return fail(RGL_E_CAPACITY, "too many draws in one batch");
-				const int r = PlanDraw(*st, c, triBase, vBase, f, draws, fi);
+				const size_t nd = draws.size();
+				const int r = PlanDraw(*st, c, triBase, vBase, f, draws, fi, ordered);
 				if (r != 0) return r;
+				if (draws.size() != nd) {
+					if (stored || clears == 0) ordered = true;
+					drawn = true; }
 				break; }}}
-		if (!cleared && draws.size() != firstDraw) return fail(RGL_E_UNSUPPORTED, "a frame must begin with Clear(COLOR|DEPTH)");
-		if (!f.doClearColor || !f.doClearDepth) {
-			if (draws.size() != firstDraw || f.outColor || f.outDepth) return fail(RGL_E_UNSUPPORTED, "partial clears are not supported"); }
-		if (f.depthFunc < 0) f.depthFunc = GL_LESS;
-		if (std::isnan(f.clearDepth)) f.drawEnabled = 0;
+		if (ops.size() > static_cast<size_t>(kMaxOps)) return fail(RGL_E_CAPACITY, "too many clears and stores in one batch");
+		f.nOps = static_cast<int>(ops.size()) - f.opBase;
+		f.inOrder = ordered ? 1 : 0;
+		if (!ordered) {
+			if (clears == 0) { f.doClearColor = 0; f.doClearDepth = 0; }
+			if (f.depthFunc < 0) f.depthFunc = GL_LESS;
+			if (std::isnan(f.clearDepth)) f.drawEnabled = 0; }
+		if (ordered && !sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples need a frame of the key path");
 		if (triBase * kSlotsPerTri + 1 >= 0xffffffffLL) return fail(RGL_E_CAPACITY, "too many triangles in one batch");
 		return 0; }
 
-	int PlanDraw(const GLState& s, const Cmd& c, long long& triBase, long long& vBase, FrameDesc& f, std::vector<DrawDesc>& draws, int fi) {
+	int PlanDraw(const GLState& s, const Cmd& c, long long& triBase, long long& vBase, FrameDesc& f, std::vector<DrawDesc>& draws, int fi,
+	             bool& ordered) {
 		const uint32_t vkey = (static_cast<uint32_t>(s.programId) << 24) | 0x80000000u;
 		if (!Installed(vkey)) return fail(RGL_E_DISPATCH, "BIN: no dispatch entry for program");
 		if (!Installed(s.FragmentStateKey())) return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_DRAW_INLINE");
 		const int nv = program_nvary(s.programId);
 		if (nv < 0) return fail(RGL_E_UNSUPPORTED, "program not implemented on the device");
-		if (!s.depthTestEnabled || !s.depthWriteMask || !s.colorWriteMask || s.blendingEnabled || s.scissorEnabled)
-			return fail(RGL_E_UNSUPPORTED, "state needs the in-order raster path (blend, scissor, no depth test or write)");
-		if (s.depthFunc != GL_LESS && s.depthFunc != GL_LEQUAL) return fail(RGL_E_UNSUPPORTED, "depth func not implemented");
-		if (f.depthFunc >= 0 && f.depthFunc != s.depthFunc) return fail(RGL_E_UNSUPPORTED, "mixed depth funcs in one frame");
-		f.depthFunc = s.depthFunc;
+		if (s.depthFunc != GL_LESS && s.depthFunc != GL_LEQUAL && s.depthFunc != GL_EQUAL)
+			return fail(RGL_E_UNSUPPORTED, "depth func not implemented");
+		if (s.color0AttachmentType != RB_COLOR_DEPTH && s.color0AttachmentType != RB_RGBF32 && s.color0AttachmentType != RB_RGBAF32)
+			return fail(RGL_E_UNSUPPORTED, "RDR:STATE unknown color0 Renderbuffer type");
+		if (s.depthAttachmentType != RB_COLOR_DEPTH && s.depthAttachmentType != RB_F32)
+			return fail(RGL_E_UNSUPPORTED, "RDR:STATE unknown depth Renderbuffer type");
 		const bool elements = c.kind == Cmd::DRAW_ELEMENTS || c.kind == Cmd::DRAW_ELEMENTS_INSTANCED;
 		const bool inst = c.kind == Cmd::DRAW_ELEMENTS_INSTANCED || c.kind == Cmd::DRAW_ARRAYS_INSTANCED;
 		const int count = c.a;
@@ -345,6 +405,11 @@ struct rgl_gpu {
 		if (instances > 65536) return fail(RGL_E_CAPACITY, "instance id must be < 65536");
 		const int tris = count / 3;
 		if (tris == 0 || instances == 0) return 0;
+		// the key path takes depth-tested, depth- and colour-writing draws of
+		// one depth function (LESS or LEQUAL) without blending or scissor
+		if (!s.depthTestEnabled || !s.depthWriteMask || !s.colorWriteMask || s.blendingEnabled || s.scissorEnabled ||
+		    s.depthFunc == GL_EQUAL || (f.depthFunc >= 0 && f.depthFunc != s.depthFunc)) ordered = true;
+		if (f.depthFunc < 0 && (s.depthFunc == GL_LESS || s.depthFunc == GL_LEQUAL)) f.depthFunc = s.depthFunc;
 		DrawDesc d;
 		std::memset(&d, 0, sizeof(d));
 		d.program = s.programId;
@@ -392,7 +457,21 @@ struct rgl_gpu {
 		const int half = std::max(W, H) / 2;
 		d.gbFactor = (2048.0F - static_cast<float>(half)) / static_cast<float>(half);
 		if (fixedGuardBand && d.gbFactor < 1.0F) d.gbFactor = 1.0F;
+		// GPU::ScissorRect (rglv_gpu.hxx:258-276): left, top, right, bottom
+		// with bottom = H - origin.y - 1; clamped to the target, where the
+		// reference would index tiles outside its grid
 		d.scissor[0] = 0; d.scissor[1] = 0; d.scissor[2] = W; d.scissor[3] = H;
+		d.scissorOn = s.scissorEnabled ? 1 : 0;
+		if (s.scissorEnabled) {
+			const int left = s.scissorOrigin[0], right = left + s.scissorSize[0];
+			const int bottom = H - s.scissorOrigin[1] - 1, top = bottom - s.scissorSize[1];
+			d.scissor[0] = std::min(std::max(left, 0), W); d.scissor[1] = std::min(std::max(top, 0), H);
+			d.scissor[2] = std::min(std::max(right, 0), W); d.scissor[3] = std::min(std::max(bottom, 0), H); }
+		d.depthTest = s.depthTestEnabled ? 1 : 0;
+		d.depthFunc = s.depthFunc;
+		d.depthWrite = s.depthWriteMask ? 1 : 0;
+		d.colorWrite = s.colorWriteMask ? 1 : 0;
+		d.blend = s.blendingEnabled ? 1 : 0;
 		d.frame = fi;
 		d.tileW = tileW; d.tileH = tileH; d.tilesX = tilesX; d.tilesY = tilesY; d.tileBase = f.tileBase;
 		d.fW = W; d.fH = H;
@@ -505,6 +584,7 @@ struct rgl_gpu {
 		if (n < 1 || n > kMaxFrames) return fail(RGL_E_ARG, "a batch holds 1 to 64 frames");
 		std::vector<FrameDesc> frames(n);
 		std::vector<DrawDesc> draws;
+		std::vector<FrameOp> ops;
 		long long triBase = 0, vBase = 0, keyBase = 0;
 		int tileBase = 0, blockBase = 0;
 		submitted = 0;
@@ -513,7 +593,7 @@ struct rgl_gpu {
 			if (!r) return fail(RGL_E_ARG, "null frame in batch");
 			for (int j = 0; j < i; ++j) if (recs[j] == r) return fail(RGL_E_ARG, "a recorder appears twice in one batch");
 			const long long sub0 = submitted;
-			const int rc = r->PlanFrame(frames[i], draws, triBase, vBase, i, tileBase);
+			const int rc = r->PlanFrame(frames[i], draws, ops, triBase, vBase, i, tileBase);
 			if (rc != 0) { err = r->err; return rc; }
 			submitted = sub0 + r->submitted;
 			FrameDesc& f = frames[i];
@@ -524,6 +604,20 @@ struct rgl_gpu {
 			blockBase += f.blocksX * ((f.H + kResolveH - 1) / kResolveH); }
 		if (n > 1) for (int i = 0; i < n; ++i)
 			if (!recs[i]->sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples are stored from single-frame runs");
+		// the in-order frames: one k_tile_order block per tile, one k_inorder
+		// block per 32 x 32 region of a tile
+		std::vector<IoFrame> io;
+		int ioTiles = 0, ioBlocks = 0;
+		for (int i = 0; i < n; ++i) {
+			if (!frames[i].inOrder) continue;
+			IoFrame e;
+			std::memset(&e, 0, sizeof(e));
+			e.frame = i; e.tileStart = ioTiles; e.blockStart = ioBlocks;
+			e.regionsX = (frames[i].tileW + kRegionDim - 1) / kRegionDim;
+			e.regionsY = (frames[i].tileH + kRegionDim - 1) / kRegionDim;
+			io.push_back(e);
+			ioTiles += frames[i].nTiles;
+			ioBlocks += frames[i].nTiles * e.regionsX * e.regionsY; }
 		BatchDesc b;
 		b.nFrames = n; b.nTiles = tileBase; b.nDraws = static_cast<int>(draws.size());
 		// binning units: kUnitTris triangles, grown in steps of kUnitTris
@@ -551,6 +645,15 @@ struct rgl_gpu {
 		if (auxPending && auxStream != s && (e = hipStreamWaitEvent(s, auxDone, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 		int r = Ensure(b, vtot, keyBase);
 		if (r != 0) return r;
+		if (!io.empty()) {
+			const size_t nl = static_cast<size_t>(listCap) + clistCap;
+			if ((e = Grow(order, nl)) != hipSuccess) return hipfail(e, "hipMalloc tile order");
+			if ((e = Grow(skeys0, nl)) != hipSuccess) return hipfail(e, "hipMalloc sort keys");
+			if ((e = Grow(skeys1, nl)) != hipSuccess) return hipfail(e, "hipMalloc sort keys"); }
+		if (!dOps.p) {
+			if ((e = Grow(dOps, kMaxOps)) != hipSuccess) return hipfail(e, "hipMalloc frame commands");
+			if ((e = Grow(dIo, kMaxFrames)) != hipSuccess) return hipfail(e, "hipMalloc in-order frames");
+			lastOps.clear(); lastIo.clear(); }
 		b.frames = dFrames.p;
 		stream = s;
 		if (!dCtr) {
@@ -560,6 +663,8 @@ struct rgl_gpu {
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hDraws), sizeof(DrawDesc) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hLite), sizeof(DrawLite) * kMaxDraws, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hFrames), sizeof(FrameDesc) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hOps), sizeof(FrameOp) * kMaxOps, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
+			if ((e = hipHostMalloc(reinterpret_cast<void**>(&hIo), sizeof(IoFrame) * kMaxFrames, hipHostMallocDefault)) != hipSuccess) return hipfail(e, "hipHostMalloc");
 			if ((e = hipEventCreateWithFlags(&done, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 		cur = nullptr;
 		curSlot = -1;
@@ -577,6 +682,9 @@ struct rgl_gpu {
 		for (size_t i = 0; i < draws.size(); ++i) lite[i] = make_lite(draws[i]);
 		if ((r = UploadTable(lite, lastLite, hLite, dLite.p)) != 0) return r;
 		if ((r = UploadTable(frames, lastFrames, hFrames, dFrames.p)) != 0) return r;
+		if ((r = UploadTable(ops, lastOps, hOps, dOps.p)) != 0) return r;
+		if ((r = UploadTable(io, lastIo, hIo, dIo.p)) != 0) return r;
+		nIo = static_cast<int>(io.size()); nIoTiles = ioTiles; nIoBlocks = ioBlocks;
 		if (costReady < tileCost.cap) {
 			k_fill_u32<<<8, 256, 0, stream>>>(tileCost.p, static_cast<long long>(tileCost.cap), 0u);
 			costReady = tileCost.cap; }
@@ -591,7 +699,7 @@ struct rgl_gpu {
 		// frames stored linear or through another post program shade plainly
 		bool learned = mlpParams != nullptr && pid == ProgGGXEnv::kId;
 		for (const FrameDesc& f : frames)
-			if (f.outColor && (f.gamma != 1 || f.postProgram != 1)) learned = false;
+			if (f.inOrder || (f.outColor && (f.gamma != 1 || f.postProgram != 1))) learned = false;
 		const std::vector<SampleReq>& reqs = recs[0]->sampleReqs;
 		if (!reqs.empty() && pid != ProgGGXEnv::kId) return fail(RGL_E_UNSUPPORTED, "training samples need the heavy-shader program");
 		// the launch sequence, replayed from a captured graph when enabled and
@@ -672,9 +780,12 @@ struct rgl_gpu {
 		mix(&b, sizeof(b));
 		mix(lastTable.data(), lastTable.size());
 		mix(lastFrames.data(), lastFrames.size());
+		mix(lastOps.data(), lastOps.size());
+		mix(lastIo.data(), lastIo.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
 		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, scanPart.p, tileCost.p,
-		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams };
+		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams,
+		                       dOps.p, dIo.p, order.p, skeys0.p, skeys1.p };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
 		                          static_cast<uint32_t>(mlpShadeBlocks) };
@@ -773,6 +884,13 @@ struct rgl_gpu {
 			Dispatch(0, k_vis, dim3(std::min(items, 1 << 20)), dim3(kRasterThreads), 0, b, ra,
 			         static_cast<const VisRec*>(rec.p), static_cast<const VisRec*>(crec.p), static_cast<const uint32_t*>(itemTile.p),
 			         static_cast<const VisTile*>(tileDesc.p), b.frames); }
+		if (nIo > 0) {
+			// frames drawn in order: the tiles' records sorted by slot, then
+			// every tile region walks them (before k_resolve, whose first
+			// block publishes the frame's counters)
+			k_tile_order<<<nIoTiles, 256, 0, stream>>>(b, dIo.p, nIo, b.frames, tileDesc.p, rec.p, crec.p, order.p, skeys0.p, skeys1.p);
+			k_inorder<<<nIoBlocks, 256, 0, stream>>>(b, ra, dIo.p, nIo, b.frames, dOps.p, dDraws.p, dLite.p, tileDesc.p, rec.p, crec.p,
+				order.p); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		const DrawLite* cLite = dLite.p;

@@ -1,0 +1,325 @@
+// rgl_inorder.h — the in-order tile path of the gfx950 rgl pipeline.
+//
+// Frames whose state the order-independent visibility keys cannot express
+// are drawn here as the reference draws every tile (GPU::DrawImpl,
+// rglv_gpu.cxx:263-432): each tile walks its commands in submission order.
+// That covers alpha blending, the scissor test, depth test or depth writes
+// off, GL_EQUAL, draws with different depth functions, and clears and stores
+// between draws (a depth-only clear for a shadow pass, a store followed by
+// more drawing).
+//
+//   k_tile_order  one workgroup per tile: the tile's (triangle, tile) setup
+//                 records, inline and clipped, sorted by submission slot (an
+//                 LDS bitonic sort; above kSortCap records, LDS-sorted runs
+//                 merged in global memory).
+//   k_inorder     one workgroup per 32 x 32-pixel region of a tile, one
+//                 thread per 2x2 quad: the quad's colour, depth and
+//                 primitive id stay in registers while the workgroup steps
+//                 through the tile's triangles (from LDS, 256 at a time) and
+//                 frame commands in slot order, applying
+//                 TriangleProgram::Render (rglv_gpu_impl.hxx:166-222) to the
+//                 quad: coverage, scissor, early or late depth test, depth
+//                 write, shading, alpha blend, masked colour store.
+#pragma once
+
+#include "rgl_kernels.h"
+
+namespace rgl {
+
+// A frame command that every tile executes at one point of the submission
+// order: CLEAR and the STORE_* commands that GPU::BinImpl broadcasts to all
+// tiles (rglv_gpu.cxx:139-191).  pos is the first visibility slot it
+// precedes: the triangles drawn before it have smaller slots.
+enum FrameOpKind { kOpClear = 0, kOpStoreColor = 1, kOpStoreDepth = 2, kOpStorePrim = 3 };
+struct alignas(16) FrameOp {
+	uint32_t pos;
+	int kind;
+	int bits;            // clear: GL_COLOR_BUFFER_BIT (1) | GL_DEPTH_BUFFER_BIT (2)
+	int gamma;           // store colour: sRGB (1) or to_tc_basic (0)
+	float color[3];      // clear colour
+	float depth;         // clear depth
+	int post;            // store colour: post program (1 default, 2 exposure)
+	float exposure;
+	void* ptr;           // store target, row stride = the frame's width
+};
+
+// the in-order frames of a batch: their first k_tile_order block (one per
+// tile) and first k_inorder block (one per region of a tile)
+struct alignas(16) IoFrame {
+	int frame;
+	int tileStart;
+	int blockStart;
+	int regionsX, regionsY;
+	int pad[3];
+};
+
+constexpr int kSortCap = 4096;       // records a tile sorts in LDS in one piece
+constexpr int kRegionDim = 32;       // k_inorder region: 32 x 32 pixels, 16 x 16 quads
+
+__device__ inline int find_io(const IoFrame* io, int n, int blk, bool tiles) {
+	int lo = 0, hi = n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if ((tiles ? io[mid].tileStart : io[mid].blockStart) <= blk) lo = mid; else hi = mid - 1; }
+	return lo; }
+
+// ascending bitonic sort of m (a power of two) keys in LDS by the workgroup
+__device__ inline void bitonic_sort(unsigned long long* s, uint32_t m) {
+	for (uint32_t k = 2; k <= m; k <<= 1)
+		for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+			for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+				const uint32_t ixj = i ^ j;
+				if (ixj > i) {
+					const unsigned long long x = s[i], y = s[ixj];
+					if ((x > y) == ((i & k) == 0)) { s[i] = y; s[ixj] = x; }}}
+			__syncthreads(); }}
+
+// A tile's records in submission order.  Keys are (slot << 32 | p), p the
+// record's position in the tile (inline records first, then clipped); every
+// (triangle, tile) pair has its own slot, so keys are distinct.  order[ob +
+// k] receives the k-th record's p, ob = the tile's first inline + first
+// clipped list position (the tiles' ranges are disjoint).
+__global__ __launch_bounds__(256) void k_tile_order(BatchDesc b, const IoFrame* __restrict__ io, int nIo,
+		const FrameDesc* __restrict__ frames, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
+		const VisRec* __restrict__ crec, uint32_t* __restrict__ order, unsigned long long* __restrict__ skeys0,
+		unsigned long long* __restrict__ skeys1) {
+	__shared__ unsigned long long s[kSortCap];
+	const int blk = static_cast<int>(blockIdx.x);
+	const IoFrame e = io[find_io(io, nIo, blk, true)];
+	const FrameDesc& f = frames[e.frame];
+	const int tile = f.tileBase + (blk - e.tileStart);
+	const VisTile d = tileDesc[tile];
+	const uint32_t n = d.ntot, ninl = d.ninl, ob = d.ib + d.cb;
+	const uint32_t t = threadIdx.x;
+	auto key_of = [&](uint32_t p) -> unsigned long long {
+		const uint32_t slot = p < ninl ? rec[d.ib + p].slot : crec[d.cb + (p - ninl)].slot;
+		return (static_cast<unsigned long long>(slot) << 32) | p; };
+	if (n <= static_cast<uint32_t>(kSortCap)) {
+		uint32_t m = 1;
+		while (m < n) m <<= 1;
+		for (uint32_t i = t; i < m; i += blockDim.x) s[i] = i < n ? key_of(i) : ~0ull;
+		__syncthreads();
+		bitonic_sort(s, m);
+		for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(s[i]);
+		return; }
+	// runs of kSortCap sorted in LDS, then merged pairwise: each key's place
+	// in the merged run is its index in its own run plus its rank (binary
+	// search) in the other
+	unsigned long long* src = skeys0 + ob;
+	unsigned long long* dst = skeys1 + ob;
+	for (uint32_t c0 = 0; c0 < n; c0 += kSortCap) {
+		const uint32_t cn = min(static_cast<uint32_t>(kSortCap), n - c0);
+		__syncthreads();
+		for (uint32_t i = t; i < static_cast<uint32_t>(kSortCap); i += blockDim.x) s[i] = i < cn ? key_of(c0 + i) : ~0ull;
+		__syncthreads();
+		bitonic_sort(s, kSortCap);
+		for (uint32_t i = t; i < cn; i += blockDim.x) src[c0 + i] = s[i]; }
+	__syncthreads();
+	for (uint32_t run = kSortCap; run < n; run <<= 1) {
+		for (uint32_t i = t; i < n; i += blockDim.x) {
+			const uint32_t a0 = i / (2 * run) * (2 * run);
+			const uint32_t a1 = min(a0 + run, n), b1 = min(a0 + 2 * run, n);
+			const unsigned long long key = src[i];
+			const bool inA = i < a1;
+			uint32_t lo = inA ? a1 : a0, hi = inA ? b1 : a1;
+			const uint32_t own = inA ? i - a0 : i - a1;
+			const uint32_t base = inA ? a1 : a0;
+			while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (src[mid] < key) lo = mid + 1; else hi = mid; }
+			dst[a0 + own + (lo - base)] = key; }
+		__syncthreads();
+		unsigned long long* x = src; src = dst; dst = x; }
+	for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(src[i]); }
+
+// ShadeFragment of program P for a quad: perspective-correct BP from BS
+// and the vertices' 1/w (Render, rglv_gpu_impl.hxx:187-191), varyings,
+// the quad's texture LOD from lanes 0 and 1; colour with alpha per lane
+template <class P>
+__device__ inline void shade_quad(const DrawDesc& d, const float w[3], const float* v0, const float* v1, const float* v2,
+                                  const float bx[4], const float by[4], const float bz[4], float col[4][4]) {
+	float vv[4][P::kVary];
+#pragma unroll
+	for (int l = 0; l < 4; ++l) {
+		const float fragW = oneover((bx[l] * w[0] + by[l] * w[1]) + bz[l] * w[2]);
+		const float p0 = (w[0] * bx[l]) * fragW;
+		const float p2 = (w[2] * bz[l]) * fragW;
+		const float p1 = (1.0F - p0) - p2;
+#pragma unroll
+		for (int k = 0; k < P::kVary; ++k) vv[l][k] = (p0 * v0[k] + p1 * v1[k]) + p2 * v2[k]; }
+	int lod = 0;
+	if constexpr (P::kUV >= 0) lod = texture_lod(d.tu0, vv[0][P::kUV], vv[1][P::kUV], vv[0][P::kUV + 1], vv[1][P::kUV + 1]);
+#pragma unroll
+	for (int l = 0; l < 4; ++l) P::fragment1(d, vv[l], lod, col[l]); }
+
+__device__ inline void shade_quad_any(const DrawDesc& d, const float w[3], const float* v0, const float* v1, const float* v2,
+                                      const float bx[4], const float by[4], const float bz[4], float col[4][4]) {
+	switch (d.program) {
+	case ProgAmy::kId: shade_quad<ProgAmy>(d, w, v0, v1, v2, bx, by, bz, col); break;
+	case ProgMany::kId: shade_quad<ProgMany>(d, w, v0, v1, v2, bx, by, bz, col); break;
+	case ProgLambertTex::kId: shade_quad<ProgLambertTex>(d, w, v0, v1, v2, bx, by, bz, col); break;
+	case ProgGGXEnv::kId: shade_quad<ProgGGXEnv>(d, w, v0, v1, v2, bx, by, bz, col); break;
+	default: shade_quad<ProgManyBlinnPhong>(d, w, v0, v1, v2, bx, by, bz, col); break; }}
+
+// depth functions of the fragment state (DepthLT / DepthLE / DepthEQ,
+// rglv_gpu_impl.hxx:64-74)
+__device__ inline bool depth_pass(int func, float frag, float dest) {
+	return func == 0 ? frag < dest : func == 1 ? frag <= dest : frag == dest; }
+
+__global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, const IoFrame* __restrict__ io, int nIo,
+		const FrameDesc* __restrict__ frames, const FrameOp* __restrict__ ops, const DrawDesc* __restrict__ draws,
+		const DrawLite* __restrict__ lite, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
+		const VisRec* __restrict__ crec, const uint32_t* __restrict__ order) {
+	__shared__ VisRec srec[256];
+	__shared__ int sdraw[256];
+	__shared__ uint32_t sfrag[4];
+	const int blk = static_cast<int>(blockIdx.x);
+	const IoFrame e = io[find_io(io, nIo, blk, false)];
+	const FrameDesc& f = frames[e.frame];
+	const int rpt = e.regionsX * e.regionsY;
+	const int lb = blk - e.blockStart;
+	const int lt = lb / rpt, reg = lb - lt * rpt;
+	const int tile = f.tileBase + lt;
+	const Rect R = tile_rect(f, lt);
+	const int rgy = reg / e.regionsX, rgx = reg - rgy * e.regionsX;
+	const int t = static_cast<int>(threadIdx.x);
+	// this thread's quad: top-left pixel (qx, qy); lane l = (l & 1, l >> 1)
+	const int qx = R.x0 + rgx * kRegionDim + 2 * (t & 15);
+	const int qy = R.y0 + rgy * kRegionDim + 2 * (t >> 4);
+	bool onScreen[4];
+#pragma unroll
+	for (int l = 0; l < 4; ++l) onScreen[l] = qx + (l & 1) < R.x1 && qy + (l >> 1) < R.y1;
+	// the tile buffers start zeroed (the oracle's convention for a tile not
+	// yet cleared; the reference leaves the previous tile's contents)
+	float cr[4] = { 0.0F, 0.0F, 0.0F, 0.0F }, cg[4] = { 0.0F, 0.0F, 0.0F, 0.0F }, cb[4] = { 0.0F, 0.0F, 0.0F, 0.0F };
+	float dz[4] = { 0.0F, 0.0F, 0.0F, 0.0F };
+	uint32_t pid[4] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu };
+	uint32_t frags = 0;
+	const VisTile vt = tileDesc[tile];
+	const uint32_t n = vt.ntot, ninl = vt.ninl, ob = vt.ib + vt.cb;
+	uint32_t opi = static_cast<uint32_t>(f.opBase);
+	const uint32_t ope = static_cast<uint32_t>(f.opBase + f.nOps);
+
+	auto apply_op = [&](const FrameOp& op) {
+		if (op.kind == kOpClear) {
+			if (op.bits & 1) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) { cr[l] = op.color[0]; cg[l] = op.color[1]; cb[l] = op.color[2]; pid[l] = 0xffffffffu; }}
+			if (op.bits & 2) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) dz[l] = op.depth; }}
+		else {
+#pragma unroll
+			for (int l = 0; l < 4; ++l) {
+				if (!onScreen[l]) continue;
+				const long long o = static_cast<long long>(qy + (l >> 1)) * f.W + qx + (l & 1);
+				if (op.kind == kOpStoreColor) {
+					float c[3] = { cr[l], cg[l], cb[l] };
+					if (op.post == 2) { c[0] = c[0] * op.exposure; c[1] = c[1] * op.exposure; c[2] = c[2] * op.exposure; }
+					const uint32_t px = op.gamma
+						? ((to_srgb8(c[0]) << 16) | (to_srgb8(c[1]) << 8) | to_srgb8(c[2]))
+						: ((to_linear8(c[0]) << 16) | (to_linear8(c[1]) << 8) | to_linear8(c[2]));
+					static_cast<uint32_t*>(op.ptr)[o] = px; }
+				else if (op.kind == kOpStoreDepth) static_cast<float*>(op.ptr)[o] = dz[l];
+				else static_cast<uint32_t*>(op.ptr)[o] = pid[l]; }}};
+
+	for (uint32_t base = 0; base < n; base += 256) {
+		__syncthreads();
+		if (base + static_cast<uint32_t>(t) < n) {
+			const uint32_t p = order[ob + base + t];
+			const VisRec r = p < ninl ? rec[vt.ib + p] : crec[vt.cb + (p - ninl)];
+			srec[t] = r;
+			sdraw[t] = find_draw_slot(lite, b.nDraws, r.slot); }
+		__syncthreads();
+		const uint32_t cnt = min(256u, n - base);
+		for (uint32_t j = 0; j < cnt; ++j) {
+			const uint32_t slot = srec[j].slot;
+			while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
+			const int minx = static_cast<int>(srec[j].minxy & 0xffffu), miny = static_cast<int>(srec[j].minxy >> 16);
+			const int nqx = static_cast<int>(srec[j].nq & 0xffffu), nqy = static_cast<int>(srec[j].nq >> 16);
+			if (qx < minx || qy < miny) continue;
+			const int kx = (qx - minx) >> 1, ky = (qy - miny) >> 1;
+			if (kx >= nqx || ky >= nqy) continue;
+			const VisRec& r = srec[j];
+			// the quad's lanes (c, c+dy, c+dx, c+dx+dy), stepped as the
+			// rasteriser loops step (rglv_triangle.hxx:253-304)
+			const int32_t q1 = wadd(wadd(r.c1, wmul(kx, wmul(r.dy12, 2))), wmul(ky, wmul(r.dx12, 2)));
+			const int32_t q2 = wadd(wadd(r.c2, wmul(kx, wmul(r.dy23, 2))), wmul(ky, wmul(r.dx23, 2)));
+			const int32_t q3 = wadd(wadd(r.c3, wmul(kx, wmul(r.dy31, 2))), wmul(ky, wmul(r.dx31, 2)));
+			const int32_t e1[4] = { q1, wadd(q1, r.dy12), wadd(q1, r.dx12), wadd(wadd(q1, r.dx12), r.dy12) };
+			const int32_t e2[4] = { q2, wadd(q2, r.dy23), wadd(q2, r.dx23), wadd(wadd(q2, r.dx23), r.dy23) };
+			const int32_t e3[4] = { q3, wadd(q3, r.dy31), wadd(q3, r.dx31), wadd(wadd(q3, r.dx31), r.dy31) };
+			uint32_t live = 0;
+#pragma unroll
+			for (int l = 0; l < 4; ++l) live |= ((e1[l] | e2[l] | e3[l]) >= 0 ? 1u : 0u) << l;
+			if (!live) continue;
+			const int di = __builtin_amdgcn_readfirstlane(sdraw[j]);
+			const DrawDesc& dd = draws[di];
+			const DrawLite& dl = lite[di];
+			if (dd.scissorOn) {
+				// the scissor flags of the rasteriser loops mask exactly the
+				// lanes outside the clipped raster rect (tile origins are even)
+				const Rect S = draw_tile_rect(dl, (R.x0 / f.tileW), (R.y0 / f.tileH));
+#pragma unroll
+				for (int l = 0; l < 4; ++l) {
+					const int x = qx + (l & 1), y = qy + (l >> 1);
+					if (x < S.x0 || x >= S.x1 || y < S.y0 || y >= S.y1) live &= ~(1u << l); }}
+			frags += static_cast<uint32_t>(__popc(live));
+			float bx[4], by[4], bz[4], fd[4];
+#pragma unroll
+			for (int l = 0; l < 4; ++l) {
+				bx[l] = itof(e2[l]) * r.scale;
+				bz[l] = itof(e1[l]) * r.scale;
+				by[l] = (1.0F - bx[l]) - bz[l];
+				fd[l] = (bx[l] * r.z0 + by[l] * r.z1) + bz[l] * r.z2; }
+			// Render: early-Z, depth write, shade, blend, masked store (every
+			// program here is early-Z)
+			uint32_t mask = live;
+			if (dd.depthTest) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+				if (!mask) continue; }
+			if (dd.depthWrite) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
+			// the winning triangle's vertices: 1/w and varyings
+			const uint32_t rel = slot - dl.slotBase;
+			const uint32_t nTri = static_cast<uint32_t>(dl.tris) * static_cast<uint32_t>(dl.instances);
+			TriVerts tr;
+			uint32_t srcTri;
+			const float* vary;
+			if (rel < nTri) {
+				srcTri = rel;
+				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr);
+				vary = a.vvary; }
+			else {
+				const uint32_t fslot = rel - nTri;
+				srcTri = fslot / kFanMax;
+				const uint32_t jf = fslot - srcTri * kFanMax;
+				const uint32_t cq = a.clipqOf[dl.triBase + srcTri];
+				clipped_verts(cq * kFanMax + jf, a.cvpos, tr);
+				vary = a.cvvary; }
+			float col[4][4];
+			shade_quad_any(dd, tr.w, vary + tr.vary[0] * kMaxVary, vary + tr.vary[1] * kMaxVary, vary + tr.vary[2] * kMaxVary,
+			               bx, by, bz, col);
+			if (dd.colorWrite) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) {
+					if (!(mask & (1u << l))) continue;
+					if (dd.blend) {   // BlendAlpha (rglv_gpu_impl.hxx:80-84)
+						const float al = col[l][3], om = 1.0F - al;
+						cr[l] = col[l][0] * al + cr[l] * om;
+						cg[l] = col[l][1] * al + cg[l] * om;
+						cb[l] = col[l][2] * al + cb[l] * om; }
+					else { cr[l] = col[l][0]; cg[l] = col[l][1]; cb[l] = col[l][2]; }}}
+			const uint32_t prim = dl.primBase + srcTri;
+#pragma unroll
+			for (int l = 0; l < 4; ++l) if (mask & (1u << l)) pid[l] = prim; }}
+	while (opi < ope) { apply_op(ops[opi]); ++opi; }
+	// fragments that reached the depth test, one atomic per workgroup
+	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
+	if ((t & 63) == 0) sfrag[t >> 6] = frags;
+	__syncthreads();
+	if (t == 0) {
+		const unsigned long long sum = static_cast<unsigned long long>(sfrag[0]) + sfrag[1] + sfrag[2] + sfrag[3];
+		if (sum) atomicAdd(&a.ctr->slot[blockIdx.x % kCtrSlots].fragments, sum); }}
+
+}  // namespace rgl

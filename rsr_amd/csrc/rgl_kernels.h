@@ -77,6 +77,7 @@ struct alignas(16) DrawLite {
 	uint32_t primBase;
 	int tileW, tileH, tilesX, tilesY, tileBase;   // the draw's frame tile grid
 	int fW, fH;                                  // and its size
+	int scissorOn;                               // 1: the raster rect is the tile rect clipped by scissor[]
 };
 
 __device__ inline int find_draw_ll(const DrawLite* dl, int nDraws, long long s) {
@@ -114,6 +115,7 @@ __host__ __device__ inline DrawLite make_lite(const DrawDesc& d) {
 	l.slotBase = d.slotBase; l.primBase = d.primBase;
 	l.tileW = d.tileW; l.tileH = d.tileH; l.tilesX = d.tilesX; l.tilesY = d.tilesY; l.tileBase = d.tileBase;
 	l.fW = d.fW; l.fH = d.fH;
+	l.scissorOn = d.scissorOn;
 	return l; }
 
 __device__ inline void load_lite(DrawLite* dl, const DrawDesc* draws, int nDraws) {
@@ -852,10 +854,17 @@ __device__ inline VisRec make_rec(const TriSetup& t, const TriVerts& tr, uint32_
 __device__ inline uint32_t tile_pairs(const uint32_t* tileOff, const uint32_t* clipOff, int t) {
 	return (tileOff[t + 1] - tileOff[t]) + (clipOff[t + 1] - clipOff[t]); }
 
+// the rect a tile rasterises a draw's triangles in: the tile rect
+// (GPU::TileRect, rglv_gpu.hxx:291-299), intersected with the scissor rect
+// when the scissor test is on (DrawTriangles / DrawClipped,
+// rglv_gpu_impl.hxx:848-849, 953-954)
 __device__ inline Rect draw_tile_rect(const DrawLite& d, int tx, int ty) {
 	Rect r;
 	r.x0 = tx * d.tileW; r.y0 = ty * d.tileH;
 	r.x1 = imin(r.x0 + d.tileW, d.fW); r.y1 = imin(r.y0 + d.tileH, d.fH);
+	if (d.scissorOn) {
+		r.x0 = imax(r.x0, d.scissor[0]); r.y0 = imax(r.y0, d.scissor[1]);
+		r.x1 = imin(r.x1, d.scissor[2]); r.y1 = imin(r.y1, d.scissor[3]); }
 	return r; }
 
 
@@ -1060,7 +1069,9 @@ __device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const T
 		const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
 		lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
 		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
-	P::fragment1(d, vv, lod, color); }
+	float c4[4];
+	P::fragment1(d, vv, lod, c4);
+	color[0] = c4[0]; color[1] = c4[1]; color[2] = c4[2]; }
 
 // resolve shaders: one program known at compile time, or a switch over
 // the installed programs for frames that mix them.  load() takes the
@@ -1227,7 +1238,8 @@ __global__ __launch_bounds__(256) void k_resolve(BatchDesc b, RasterArgs a, cons
 	const int by = lb / f.blocksX, bx = lb - by * f.blocksX;
 	const int x = bx * kResolveW + static_cast<int>(threadIdx.x % kResolveW);
 	const int y = by * kResolveH + static_cast<int>(threadIdx.x / kResolveW);
-	if (x < f.W && y < f.H)
+	// frames of the in-order tile path store their own pixels
+	if (x < f.W && y < f.H && !f.inOrder)
 		resolve_store<S>(b, f, a, lite, draws, x, y, a.keys[f.keyBase + static_cast<long long>(y) * f.W + x], kRcpMantissa, kSrgb8Tab);
 	// the frame's counters are final (no later kernel changes them): block 0
 	// publishes them to the pinned host copy
@@ -1264,6 +1276,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_vis(BatchDesc b, RasterArgs 
 		const bool multiSeg = it.multiSeg != 0u;
 		// the frame table is read-only (restrict): its fields come by scalar loads
 		const FrameDesc& f = frames[it.frame];
+		if (f.inOrder) continue;   // drawn by the in-order tile path (uniform)
 		const bool leq = f.depthFunc == 1;
 		const Rect R = tile_rect(f, tile - f.tileBase);
 		// the item's region: a quadrant, or its half or quarter

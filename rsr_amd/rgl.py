@@ -305,3 +305,28 @@ def render(scene, gpu: GPU | None = None, device="cuda:0", prim: bool = True):
     if own:
         gpu.close()
     return (*out, st)
+
+
+def render_calls(sc, gpu: GPU | None = None, device="cuda:0"):
+    """Draw a CallScene on the GPU; returns (outputs as numpy arrays, stats)."""
+    import torch
+    from .scenes import render_calls as issue
+    own = gpu is None
+    if own:
+        gpu = GPU(int(str(device).split(":")[-1]) if ":" in str(device) else 0)
+    dev = {k: torch.from_numpy(np.ascontiguousarray(a)).to(device) for k, a in sc.arrays.items()}
+    ptr_of = {id(sc.arrays[k]): t.data_ptr() for k, t in dev.items()}
+    outs = []
+    for kind in sc.outputs:
+        dt = torch.float32 if kind == "depth" else torch.int32
+        # outputs start as a pattern the frame must overwrite
+        outs.append(torch.full((sc.height, sc.width), 0x5A5A5A5A if dt == torch.int32 else float("nan"), dtype=dt,
+                               device=device))
+    issue(gpu, sc, [o.data_ptr() for o in outs], ptr=lambda a: ptr_of[id(a)])
+    stream = torch.cuda.current_stream(device).cuda_stream
+    gpu.run_and_wait(stream)
+    res = [o.cpu().numpy().view(np.float32 if k == "depth" else np.uint32) for o, k in zip(outs, sc.outputs)]
+    st = gpu.stats()
+    if own:
+        gpu.close()
+    return res, st

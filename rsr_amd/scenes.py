@@ -644,3 +644,115 @@ def scene_arrays(scene: Scene) -> list:
         arrs.extend(m.buffers.values())
         arrs.append(m.indices)
     return arrs
+
+
+# ---------------------------------------------------------------------------
+# scenes given as GL call lists (the form the viewer's scene-graph nodes
+# issue: several draws with state changes between them, clears and stores
+# between draws, blending, scissor, depth and colour write masks)
+# ---------------------------------------------------------------------------
+@dataclass
+class CallScene:
+    """A frame as a list of rglv::GL calls.  calls: (method, *args) with
+    method the GL member name in snake case ("enable", "draw_elements", ...);
+    an argument "@name" is the array arrays[name] (a device pointer on the
+    GPU), ("out", k) is output buffer k.  outputs: the kinds of the store
+    targets, "color" (u32 0x00RRGGBB), "depth" (f32) or "prim" (u32)."""
+    name: str
+    width: int
+    height: int
+    tile_blocks: tuple = (8, 8)
+    arrays: dict = field(default_factory=dict)
+    calls: list = field(default_factory=list)
+    outputs: list = field(default_factory=list)
+    fixed_guard_band: bool = False
+
+
+def fragment_state_key_of(st: dict) -> int:
+    """GLState::FragmentStateKey without the program id (rglv_gl.hxx:136-146)."""
+    key = int(st["scissor"])
+    key |= int(st["depth_test"]) << 1
+    if st["depth_test"]:
+        key |= st["depth_func"] << 2
+    key |= int(st["blend"]) << 4
+    key |= int(st["depth_write"]) << 5
+    key |= int(st["color_write"]) << 6
+    key |= st["color_type"] << 7
+    key |= st["depth_type"] << 9
+    return key
+
+
+def call_installs(sc: CallScene) -> list:
+    """The Install entries the scene's draws and stores need: (program,
+    state key, kind) with kind 0 bin, 1 draw, 2 blt -- what the viewer's
+    shaders.cxx installs up front for the states it uses."""
+    st = dict(scissor=False, depth_test=True, depth_func=GL_LESS, blend=False, depth_write=True, color_write=True,
+              color_type=0, depth_type=0, program=0)
+    out = []
+    for c in sc.calls:
+        m, a = c[0], c[1:]
+        if m in ("enable", "disable"):
+            name = {GL_SCISSOR_TEST: "scissor", GL_BLEND: "blend", GL_DEPTH_TEST: "depth_test"}.get(a[0])
+            if name:
+                st[name] = m == "enable"
+        elif m == "depth_func":
+            st["depth_func"] = a[0]
+        elif m == "depth_write_mask":
+            st["depth_write"] = bool(a[0])
+        elif m == "color_write_mask":
+            st["color_write"] = bool(a[0])
+        elif m == "renderbuffer_type":
+            st["color_type" if a[0] == GL_COLOR_ATTACHMENT0 else "depth_type"] = a[1]
+        elif m == "use_program":
+            st["program"] = a[0]
+        elif m.startswith("draw_"):
+            out += [(st["program"], 0, 0), (st["program"], fragment_state_key_of(st), 1)]
+        elif m == "store_color_tc":
+            out.append((st["program"], 0, 2))
+    return sorted(set(out))
+
+
+def render_calls(gpu, sc: CallScene, outs: list, ptr=lambda a: a) -> None:
+    """Issue the scene's calls on *gpu* (any backend with the rglv::GPU /
+    rglv::GL method names); outs[k] is the backend's handle of output k,
+    ptr maps an array to the backend's handle of it."""
+    gpu.set_guard_band(sc.fixed_guard_band)
+    gpu.reset(sc.width, sc.height, *sc.tile_blocks)
+    for pid, key, kind in call_installs(sc):
+        gpu.install(pid, key, kind)
+
+    def arg(a):
+        if isinstance(a, str) and a.startswith("@"):
+            return ptr(sc.arrays[a[1:]])
+        if isinstance(a, tuple) and len(a) == 2 and a[0] == "out":
+            return outs[a[1]]
+        return a
+
+    for c in sc.calls:
+        m, a = c[0], [arg(x) for x in c[1:]]
+        if m == "use_buffer":
+            arr = sc.arrays[c[2][1:]] if isinstance(c[2], str) else None
+            gpu.gl_use_buffer(a[0], a[1], len(arr) if arr is not None else 0)
+        elif m == "alloc_uniforms":
+            gpu.gl_use_uniforms(gpu.gl_alloc_uniform_buffer(sc.arrays[c[1][1:]]))
+        else:
+            getattr(gpu, "gl_" + m)(*a)
+
+
+def mesh_calls(mesh_name: str, mesh: Mesh, instanced_arrays: bool = False) -> list:
+    """use_buffer calls for a mesh's slots and its draw, naming its arrays
+    "<mesh_name>.<slot>" and "<mesh_name>.idx" (see mesh_arrays)."""
+    calls = [("use_buffer", s, f"@{mesh_name}.{s}" if s in mesh.buffers else None) for s in range(16)]
+    if mesh.instances > 1:
+        calls.append(("draw_elements_instanced", GL_TRIANGLES, len(mesh.indices), GL_UNSIGNED_SHORT,
+                      f"@{mesh_name}.idx", mesh.instances))
+    else:
+        calls.append(("draw_elements", GL_TRIANGLES, len(mesh.indices), GL_UNSIGNED_SHORT, f"@{mesh_name}.idx",
+                      RGL_HINT_DENSE | RGL_HINT_READ4))
+    return calls
+
+
+def mesh_arrays(mesh_name: str, mesh: Mesh) -> dict:
+    out = {f"{mesh_name}.{s}": a for s, a in mesh.buffers.items()}
+    out[f"{mesh_name}.idx"] = mesh.indices
+    return out

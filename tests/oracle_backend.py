@@ -176,3 +176,20 @@ def render(scene, threads: int = 1, prim: bool = True):
     st = g.stats()
     g.close()
     return color, depth, primid, st
+
+
+def render_calls(sc, threads: int = 4):
+    """Draw a CallScene with the oracle; returns (outputs, stats)."""
+    from rsr_amd.scenes import render_calls as issue
+    g = OracleGPU(threads)
+    outs = []
+    for kind in sc.outputs:
+        if kind == "depth":
+            outs.append(np.full((sc.height, sc.width), np.nan, np.float32))
+        else:
+            outs.append(np.full((sc.height, sc.width), 0x5A5A5A5A, np.uint32))
+    issue(g, sc, outs)
+    g.run()
+    st = g.stats()
+    g.close()
+    return outs, st
