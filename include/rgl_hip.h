@@ -218,6 +218,19 @@ int rgl_mlp_gradient(rgl_gpu* gpu, const float* dev_samples, int n, float* dev_p
 int rgl_mlp_train(rgl_gpu* gpu, const float* dev_samples, long long n, float* dev_params, float* dev_opt, int steps, int batch,
                   float lr, uint32_t seed, int step0, float* loss_out, void* hip_stream);
 
+/* --- no allocation ------------------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving or training loops,
+   harnesses that forbid hipMalloc): rgl_mlp_train with every training buffer
+   in one file-scope device array of 8.6 MB that each GPU holds until the
+   process ends, so the call makes no hipMalloc, hipFree or other allocation.
+   batch <= 4096 (RGL_E_CAPACITY beyond).  The last batch's loss goes to the
+   device float *dev_loss_out (may be NULL) instead of host memory.  All
+   calls on one GPU share that array: calls that overlap on one GPU (other
+   streams, other rgl_gpu objects) race. */
+int rgl_mlp_train_noalloc(rgl_gpu* gpu, const float* dev_samples, long long n, float* dev_params, float* dev_opt, int steps,
+                          int batch, float lr, uint32_t seed, int step0, float* dev_loss_out, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,54 @@ constexpr int kNumTimed = 9;
 // a StoreTrainingSamples request (no reference counterpart)
 struct SampleReq { float* out; uint32_t cap; uint32_t* count; float keep; uint32_t seed; };
 
+// device buffers of one training pass over a batch of B samples
+struct TrainBufs { float *X, *T, *H1, *H2, *H3, *Y, *dY, *dA, *dB, *Part, *Loss; };
+
+// rgl_mlp_train_noalloc keeps its training buffers in one file-scope device
+// array, laid out for batches of at most kNoAllocMaxBatch samples; every
+// sub-buffer starts on a 256-byte boundary
+constexpr int kNoAllocMaxBatch = 4096;
+constexpr size_t kTrainChunk = 256;   // samples per split-K partial of the weight gradients
+constexpr size_t align64(size_t n) { return (n + 63) / 64 * 64; }
+constexpr size_t kScrX = 0;
+constexpr size_t kScrT = kScrX + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpK1);
+constexpr size_t kScrH1 = kScrT + align64(static_cast<size_t>(kNoAllocMaxBatch) * 3);
+constexpr size_t kScrH2 = kScrH1 + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpHid);
+constexpr size_t kScrH3 = kScrH2 + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpHid);
+constexpr size_t kScrY = kScrH3 + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpHid);
+constexpr size_t kScrdY = kScrY + align64(static_cast<size_t>(kNoAllocMaxBatch) * 3);
+constexpr size_t kScrdA = kScrdY + align64(static_cast<size_t>(kNoAllocMaxBatch) * 3);
+constexpr size_t kScrdB = kScrdA + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpHid);
+constexpr size_t kScrPart = kScrdB + align64(static_cast<size_t>(kNoAllocMaxBatch) * kMlpHid);
+constexpr size_t kScrLoss = kScrPart + align64((kNoAllocMaxBatch + kTrainChunk - 1) / kTrainChunk * static_cast<size_t>(kMlpTrainable));
+constexpr size_t kScratchFloats = kScrLoss + 64;   // 8.6 MB
+
+}  // namespace
+
+// The array as the host sees it is a stand-in symbol, not a device address:
+// host code uses the per-device address that hipGetSymbolAddress returns
+// (rgl_gpu::scratchDev), never this name.
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+// the device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array)
+hipError_t scratch_address(int device, float** out) {
+	static std::mutex m;
+	static float* addr[64] = {};
+	if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lock(m);
+	if (!addr[device]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess) return e;
+		addr[device] = static_cast<float*>(p); }
+	*out = addr[device];
+	return hipSuccess; }
+
+// dst[0] = src[0] * scale (the loss read-back of the no-allocation training)
+__global__ void k_scale_copy(const float* __restrict__ src, float scale, float* __restrict__ dst) {
+	if (threadIdx.x == 0) dst[0] = src[0] * scale; }
 }  // namespace
 
 struct rgl_gpu {
@@ -197,6 +246,7 @@ struct rgl_gpu {
 	bool auxPending = false;
 	hipStream_t auxStream = nullptr;
 	hipEvent_t auxDone = nullptr;
+	float* scratchDev = nullptr;    // this device's address of `scratch` (set up by rgl_gpu_create)
 	rgl_stats stats{};
 	long long submitted = 0;
 	int lastCommands = 0;
@@ -735,8 +785,7 @@ struct rgl_gpu {
 		if (pending && (e = hipEventSynchronize(done)) != hipSuccess) return hipfail(e, "hipEventSynchronize");
 		if (auxPending && auxStream != ms && (e = hipEventSynchronize(auxDone)) != hipSuccess)
 			return hipfail(e, "hipEventSynchronize");
-		if (!auxDone && (e = hipEventCreateWithFlags(&auxDone, hipEventDisableTiming)) != hipSuccess)
-			return hipfail(e, "hipEventCreate");
+		if (!auxDone) return fail(RGL_E_STATE, "network calls need the event rgl_gpu_create makes");
 		return 0; }
 	int AuxEnd(hipStream_t ms) {
 		hipError_t e;
@@ -945,37 +994,52 @@ struct rgl_gpu {
 
 	// seed 0xffffffff: the first B samples in order.  lr > 0: Adam step on
 	// prm/opt; lr == 0: the summed gradient goes to gradOut.
-	int TrainPass(hipStream_t ts, const float* samples, long long n, int B, float* prm, float* opt, float lr, float b1, float b2,
-	              int t, uint32_t seed, float* gradOut) {
+	// the training buffers of a batch of B samples, allocated on demand
+	int TrainBuffers(int B, TrainBufs& tb) {
 		hipError_t e;
-		const int kChunk = 256;
-		const int nz = (B + kChunk - 1) / kChunk;
+		const int nz = (B + static_cast<int>(kTrainChunk) - 1) / static_cast<int>(kTrainChunk);
 		for (auto* a : { &tX }) if ((e = a->ensure(static_cast<size_t>(B) * kMlpK1)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		for (auto* a : { &tH1, &tH2, &tH3, &tdA, &tdB }) if ((e = a->ensure(static_cast<size_t>(B) * kMlpHid)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		for (auto* a : { &tT, &tY, &tdY }) if ((e = a->ensure(static_cast<size_t>(B) * 3)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		if ((e = tPart.ensure(static_cast<size_t>(nz) * kMlpTrainable)) != hipSuccess) return hipfail(e, "hipMalloc training");
 		if ((e = tLoss.ensure(1)) != hipSuccess) return hipfail(e, "hipMalloc training");
-		k_mlp_batch<<<(B + 255) / 256, 256, 0, ts>>>(samples, n, B, seed, static_cast<uint32_t>(t), prm, tX.p, tT.p);
+		tb = TrainBufs{ tX.p, tT.p, tH1.p, tH2.p, tH3.p, tY.p, tdY.p, tdA.p, tdB.p, tPart.p, tLoss.p };
+		return 0; }
+
+	// the same buffers inside this device's `scratch` array (no allocation)
+	TrainBufs ScratchBuffers() const {
+		float* b = scratchDev;
+		return TrainBufs{ b + kScrX, b + kScrT, b + kScrH1, b + kScrH2, b + kScrH3, b + kScrY, b + kScrdY, b + kScrdA, b + kScrdB,
+		                  b + kScrPart, b + kScrLoss }; }
+
+	// seed 0xffffffff: the first B samples in order.  lr > 0: Adam step on
+	// prm/opt; lr == 0: the summed gradient goes to gradOut.
+	int TrainPass(hipStream_t ts, const TrainBufs& tb, const float* samples, long long n, int B, float* prm, float* opt, float lr,
+	              float b1, float b2, int t, uint32_t seed, float* gradOut) {
+		hipError_t e;
+		const int kChunk = static_cast<int>(kTrainChunk);
+		const int nz = (B + kChunk - 1) / kChunk;
+		k_mlp_batch<<<(B + 255) / 256, 256, 0, ts>>>(samples, n, B, seed, static_cast<uint32_t>(t), prm, tb.X, tb.T);
 		// forward: H = relu(X W^T + b), Y = H3 W4^T + b4
-		Gemm(ts, tX.p, prm + kW1, tH1.p, B, kMlpHid, kMlpK1, kMlpK1, kMlpK1, kMlpHid, 0, 1, prm + kB1, 1, nullptr, 0, kMlpK1, 1);
-		Gemm(ts, tH1.p, prm + kW2, tH2.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB2, 1, nullptr, 0, kMlpHid, 1);
-		Gemm(ts, tH2.p, prm + kW3, tH3.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB3, 1, nullptr, 0, kMlpHid, 1);
-		Gemm(ts, tH3.p, prm + kW4, tY.p, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpOut, 0, 1, prm + kB4, 0, nullptr, 0, kMlpHid, 1);
-		k_fill_u32<<<1, 64, 0, ts>>>(reinterpret_cast<uint32_t*>(tLoss.p), 1, 0u);
-		k_mlp_loss<<<(3 * B + 255) / 256, 256, 0, ts>>>(tY.p, tT.p, B, tdY.p, tLoss.p);
+		Gemm(ts, tb.X, prm + kW1, tb.H1, B, kMlpHid, kMlpK1, kMlpK1, kMlpK1, kMlpHid, 0, 1, prm + kB1, 1, nullptr, 0, kMlpK1, 1);
+		Gemm(ts, tb.H1, prm + kW2, tb.H2, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB2, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(ts, tb.H2, prm + kW3, tb.H3, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 1, prm + kB3, 1, nullptr, 0, kMlpHid, 1);
+		Gemm(ts, tb.H3, prm + kW4, tb.Y, B, kMlpOut, kMlpHid, kMlpHid, kMlpHid, kMlpOut, 0, 1, prm + kB4, 0, nullptr, 0, kMlpHid, 1);
+		k_fill_u32<<<1, 64, 0, ts>>>(reinterpret_cast<uint32_t*>(tb.Loss), 1, 0u);
+		k_mlp_loss<<<(3 * B + 255) / 256, 256, 0, ts>>>(tb.Y, tb.T, B, tb.dY, tb.Loss);
 		// backward: weight gradients split over K = samples into nz partials
-		float* P = tPart.p;
-		Gemm(ts, tdY.p, tH3.p, P + kW4, kMlpOut, kMlpHid, B, kMlpOut, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdY.p, B, kMlpOut, kChunk, P + kB4, kMlpTrainable);
-		Gemm(ts, tdY.p, prm + kW4, tdA.p, B, kMlpHid, kMlpOut, kMlpOut, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH3.p, kMlpHid, kMlpOut, 1);
-		Gemm(ts, tdA.p, tH2.p, P + kW3, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdA.p, B, kMlpHid, kChunk, P + kB3, kMlpTrainable);
-		Gemm(ts, tdA.p, prm + kW3, tdB.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH2.p, kMlpHid, kMlpHid, 1);
-		Gemm(ts, tdB.p, tH1.p, P + kW2, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdB.p, B, kMlpHid, kChunk, P + kB2, kMlpTrainable);
-		Gemm(ts, tdB.p, prm + kW2, tdA.p, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tH1.p, kMlpHid, kMlpHid, 1);
-		Gemm(ts, tdA.p, tX.p, P + kW1, kMlpHid, kMlpK1, B, kMlpHid, kMlpK1, kMlpK1, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
-		k_mlp_colsum<<<nz, 64, 0, ts>>>(tdA.p, B, kMlpHid, kChunk, P + kB1, kMlpTrainable);
+		float* P = tb.Part;
+		Gemm(ts, tb.dY, tb.H3, P + kW4, kMlpOut, kMlpHid, B, kMlpOut, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tb.dY, B, kMlpOut, kChunk, P + kB4, kMlpTrainable);
+		Gemm(ts, tb.dY, prm + kW4, tb.dA, B, kMlpHid, kMlpOut, kMlpOut, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tb.H3, kMlpHid, kMlpOut, 1);
+		Gemm(ts, tb.dA, tb.H2, P + kW3, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tb.dA, B, kMlpHid, kChunk, P + kB3, kMlpTrainable);
+		Gemm(ts, tb.dA, prm + kW3, tb.dB, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tb.H2, kMlpHid, kMlpHid, 1);
+		Gemm(ts, tb.dB, tb.H1, P + kW2, kMlpHid, kMlpHid, B, kMlpHid, kMlpHid, kMlpHid, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tb.dB, B, kMlpHid, kChunk, P + kB2, kMlpTrainable);
+		Gemm(ts, tb.dB, prm + kW2, tb.dA, B, kMlpHid, kMlpHid, kMlpHid, kMlpHid, kMlpHid, 0, 0, nullptr, 0, tb.H1, kMlpHid, kMlpHid, 1);
+		Gemm(ts, tb.dA, tb.X, P + kW1, kMlpHid, kMlpK1, B, kMlpHid, kMlpK1, kMlpK1, 1, 0, nullptr, 0, nullptr, 0, kChunk, nz);
+		k_mlp_colsum<<<nz, 64, 0, ts>>>(tb.dA, B, kMlpHid, kChunk, P + kB1, kMlpTrainable);
 		const float bc1 = 1.0F - std::pow(b1, static_cast<float>(t)), bc2 = 1.0F - std::pow(b2, static_cast<float>(t));
 		k_mlp_adam<<<(kMlpTrainable + 255) / 256, 256, 0, ts>>>(P, nz, prm, opt, opt ? opt + kMlpTrainable : nullptr,
 			lr, b1, b2, 1e-8F, bc1 > 0.0F ? bc1 : 1.0F, bc2 > 0.0F ? bc2 : 1.0F, gradOut);
@@ -1022,9 +1086,17 @@ int rgl_gpu_create(int device_ordinal, int concurrency, const char* guid, rgl_gp
 	int n = 0;
 	if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) return RGL_E_HIP;
 	if (hipSetDevice(device_ordinal) != hipSuccess) return RGL_E_HIP;
+	// set-up that later calls rely on without allocating: the network calls'
+	// event, and this device's address of the no-allocation scratch array
+	float* scr = nullptr;
+	if (scratch_address(device_ordinal, &scr) != hipSuccess) return RGL_E_HIP;
+	hipEvent_t aux = nullptr;
+	if (hipEventCreateWithFlags(&aux, hipEventDisableTiming) != hipSuccess) return RGL_E_HIP;
 	auto* g = new rgl_gpu();
 	g->device = device_ordinal;
 	g->guid = guid ? guid : "unnamed";
+	g->scratchDev = scr;
+	g->auxDone = aux;
 	(void)concurrency;
 	g->cs.reset();
 	*out = g;
@@ -1279,7 +1351,9 @@ int rgl_mlp_gradient(rgl_gpu* g, const float* dev_samples, int n, float* dev_par
 	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
 	int r = g->AuxBegin(ms);
 	if (r != 0) return r;
-	r = g->TrainPass(ms, dev_samples, n, n, dev_params, nullptr, 0.0F, 0.9F, 0.999F, 1, 0xffffffffu, dev_grad);
+	TrainBufs tb;
+	if ((r = g->TrainBuffers(n, tb)) != 0) return r;
+	r = g->TrainPass(ms, tb, dev_samples, n, n, dev_params, nullptr, 0.0F, 0.9F, 0.999F, 1, 0xffffffffu, dev_grad);
 	if (r != 0) return r;
 	if ((r = g->AuxEnd(ms)) != 0) return r;
 	if (loss_out) {
@@ -1298,8 +1372,10 @@ int rgl_mlp_train(rgl_gpu* g, const float* dev_samples, long long n, float* dev_
 	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
 	int r = g->AuxBegin(ms);
 	if (r != 0) return r;
+	TrainBufs tb;
+	if ((r = g->TrainBuffers(batch, tb)) != 0) return r;
 	for (int s = 0; s < steps; ++s) {
-		r = g->TrainPass(ms, dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
+		r = g->TrainPass(ms, tb, dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
 		                 seed == 0xffffffffu ? 0u : seed, nullptr);
 		if (r != 0) return r; }
 	if ((r = g->AuxEnd(ms)) != 0) return r;
@@ -1310,5 +1386,26 @@ int rgl_mlp_train(rgl_gpu* g, const float* dev_samples, long long n, float* dev_
 		if ((e = hipStreamSynchronize(ms)) != hipSuccess) return g->hipfail(e, "loss sync");
 		*loss_out = s / (3.0F * static_cast<float>(batch)); }
 	return 0; }
+
+/* ---- no allocation (no reference counterpart) ---- */
+int rgl_mlp_train_noalloc(rgl_gpu* g, const float* dev_samples, long long n, float* dev_params, float* dev_opt, int steps,
+                          int batch, float lr, uint32_t seed, int step0, float* dev_loss_out, void* hip_stream) {
+	if (check(g)) return RGL_E_ARG;
+	if (!dev_samples || !dev_params || !dev_opt || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.0F) || step0 < 0)
+		return g->fail(RGL_E_ARG, "bad training arguments");
+	if (batch > kNoAllocMaxBatch) return g->fail(RGL_E_CAPACITY, "rgl_mlp_train_noalloc: batch exceeds its scratch limit of 4096 samples");
+	if (!g->scratchDev) return g->fail(RGL_E_STATE, "no scratch address for this device");
+	const hipStream_t ms = static_cast<hipStream_t>(hip_stream);
+	int r = g->AuxBegin(ms);
+	if (r != 0) return r;
+	const TrainBufs tb = g->ScratchBuffers();
+	for (int s = 0; s < steps; ++s) {
+		r = g->TrainPass(ms, tb, dev_samples, n, batch, dev_params, dev_opt, lr, 0.9F, 0.99F, step0 + s + 1,
+		                 seed == 0xffffffffu ? 0u : seed, nullptr);
+		if (r != 0) return r; }
+	if (dev_loss_out && steps > 0) k_scale_copy<<<1, 64, 0, ms>>>(tb.Loss, 1.0F / (3.0F * static_cast<float>(batch)), dev_loss_out);
+	hipError_t e;
+	if ((e = hipGetLastError()) != hipSuccess) return g->hipfail(e, "training launch");
+	return g->AuxEnd(ms); }
 
 }  // extern "C"

@@ -89,6 +89,7 @@ def lib():
         L.rgl_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp]
         L.rgl_mlp_gradient.argtypes = [vp, vp, i, vp, vp, ctypes.POINTER(f), vp]
         L.rgl_mlp_train.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, i, f, u32, i, ctypes.POINTER(f), vp]
+        L.rgl_mlp_train_noalloc.argtypes = [vp, vp, ctypes.c_longlong, vp, vp, i, i, f, u32, i, vp, vp]
         _lib = L
     return _lib
 
@@ -110,6 +111,7 @@ EXPORTS = [
     "rgl_gpu_set_guard_band",
     "rgl_mlp_param_count", "rgl_mlp_trainable_count", "rgl_gpu_set_learned_shading",
     "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
+    "rgl_mlp_train_noalloc",
 ]
 
 
@@ -230,6 +232,12 @@ class GPU:
         self._check(self.L.rgl_mlp_train(self.h, dev_samples, n, dev_params, dev_opt, steps, batch, lr, seed, step0,
                                          ctypes.byref(loss), stream))
         return float(loss.value)
+
+    def mlp_train_noalloc(self, dev_samples, n, dev_params, dev_opt, steps, batch, lr, seed, step0, dev_loss=None,
+                          stream=None):
+        """rgl_mlp_train without allocation: the loss goes to a device float."""
+        self._check(self.L.rgl_mlp_train_noalloc(self.h, dev_samples, n, dev_params, dev_opt, steps, batch, lr, seed, step0,
+                                                 dev_loss, stream))
 
     # --- rglv::GL -----------------------------------------------------------
     def gl_alloc_uniform_buffer(self, values=None):
