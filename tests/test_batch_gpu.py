@@ -78,8 +78,7 @@ def test_batched_config3_64_scenes():
     for k, ((g, ds), sc, out) in enumerate(zip(recs, batch, batched)):
         g.run_and_wait(stream)
         _same(sc.name, out, ds.outputs())
-        if k % 8 == 0:
-            _compare(sc.name, oracle_render(sc, threads=8), (*out, g.stats()))
+        _compare(sc.name, oracle_render(sc, threads=8), (*out, g.stats()))
     ex.close()
     for g, _ in recs:
         g.close()
@@ -135,6 +134,16 @@ def test_stress_4k_reference_guard_band():
     _compare(sc.name, ref, got)
     assert got[3]["primitives_clipped"] > 0
     assert (got[1] < 1.0).mean() < 0.5 * (oracle_render(scenes.scene_stress(2, instances=800), threads=8)[1] < 1.0).mean()
+
+
+def test_stress_2m_view_matches_oracle():
+    # one full-size stress view (2M triangles, 3840x2160, fixed guard band)
+    # against the oracle
+    from oracle_backend import render as oracle_render
+    sc = scenes.scene_stress(1)
+    got = rgl.render(sc)
+    _compare(sc.name, oracle_render(sc, threads=16), got)
+    assert got[3]["primitives_submitted"] == 2_000_000
 
 
 def test_stress_2m_batch_equals_single():

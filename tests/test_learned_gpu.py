@@ -130,3 +130,75 @@ def test_learned_frame_and_training_gain():
     assert zst["fallback_tiles"] > 0 and same >= zst["fallback_tiles"]
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
+
+
+def test_config4_1080p_shipped_weights():
+    # the learned-shading workload at its full size with the shipped
+    # network: coverage, depth and ids identical to the plain heavy frame,
+    # and >= 40 dB over covered pixels with the 2/255 per-tile fallback
+    import os
+    wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    gpu = rgl.GPU(0)
+    sc = scenes.scene_heavy()
+    pc, pd, pp, pst = _frame(gpu, sc)
+    fc, fd, fp, fst = _frame(gpu, sc, params)
+    nc, nd, npid, nst = _frame(gpu, sc, params, max_rms=1.0)
+    cov = pd < 1.0
+    assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
+    assert np.array_equal(pd.view(np.uint32), nd.view(np.uint32)) and np.array_equal(pp, npid)
+    q, qn = learned.psnr(pc, fc, cov), learned.psnr(pc, nc, cov)
+    print(f"1080p heavy: PSNR covered {q:.2f} dB with fallback ({fst['fallback_tiles']} of {fst['tiles']} tiles), "
+          f"network alone {qn:.2f} dB")
+    assert q >= 40.0
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
+
+
+def test_learned_shading_keeps_linear_and_exposure_stores_plain():
+    # the network predicts the default post program's sRGB colour: frames
+    # stored linear or through the Exposure post shade with the plain
+    # shader even when learned shading is on
+    import dataclasses
+    import os
+    wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    gpu = rgl.GPU(0)
+    sc = dataclasses.replace(scenes.scene_heavy(640, 360), post_program=scenes.PGM_EXPOSURE_POST)
+    sc.uniforms = sc.uniforms.copy()
+    sc.uniforms[0] = 1.3          # the Exposure post reads uniforms[0] (here also the eye's x)
+    plain = _frame(gpu, sc)
+    assert (plain[1] < 1.0).sum() > 1000 and len(np.unique(plain[0])) > 100
+    lrn = _frame(gpu, sc, params, max_rms=1.0)
+    assert lrn[3]["fallback_tiles"] == 0
+    assert np.array_equal(plain[0], lrn[0])
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
+
+
+def test_network_call_on_another_stream_orders_against_frames():
+    # a frame on stream A, a network forward on stream B, the frame again on
+    # stream B: both frames equal the oracle's
+    from oracle_backend import render as oracle_render
+    sc = scenes.scene_heavy(640, 360)
+    ref = oracle_render(sc, threads=8)
+    gpu = rgl.GPU(0)
+    ds = rgl.DeviceScene(sc, DEV, prim=True)
+    ds.record(gpu)
+    a, b = torch.cuda.Stream(), torch.cuda.Stream()
+    gpu.run(a.cuda_stream)
+    s, p = _samples(5000, 5), _params(6)
+    dsm, dp = torch.from_numpy(s).to(DEV), torch.from_numpy(p).to(DEV)
+    out = torch.zeros((len(s), 3), dtype=torch.float32, device=DEV)
+    gpu.mlp_forward(dsm.data_ptr(), len(s), dp.data_ptr(), out.data_ptr(), b.cuda_stream)
+    a.synchronize()
+    first = [t.cpu().clone() for t in (ds.color, ds.depth, ds.prim)]
+    gpu.run(b.cuda_stream)
+    gpu.wait()
+    torch.cuda.synchronize()
+    for k, (x, y) in enumerate(zip(first, (ds.color, ds.depth, ds.prim))):
+        assert np.array_equal(x.numpy().view(np.uint32), ref[k].view(np.uint32)), f"frame on stream A, buffer {k}"
+        assert np.array_equal(y.cpu().numpy().view(np.uint32), ref[k].view(np.uint32)), f"frame on stream B, buffer {k}"
+    want, _ = learned.ref_forward(s, p)
+    assert np.abs(out.cpu().numpy() - want).mean() < 2e-3
+    gpu.close()
