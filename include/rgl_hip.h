@@ -110,6 +110,11 @@ int rgl_gl_renderbuffer_type(rgl_gpu* gpu, int attachment, int type); /* GL::Ren
 /* GL::BindTexture :265-273; dev_texels = float RGBA, mip chain stacked
    below the base level (N x 2N texels for power-of-two N). */
 int rgl_gl_bind_texture(rgl_gpu* gpu, int unit, const float* dev_texels, int width, int height, int stride, int mode);
+/* GL::BindTexture3 :275-278 — the depth texture unit (DepthTextureUnit,
+   rglr_texture_sampler.hxx:33-79): dev_depth = dim x dim floats, dim a
+   power of two <= 8192, nearest, border -1 outside [0,1)^2; null unbinds (lookups read 0).  The
+   array must not be a store target of the same run. */
+int rgl_gl_bind_texture3(rgl_gpu* gpu, const float* dev_depth, int dim);
 /* GL::UseBuffer :280-288; count = number of floats in the array (slot 0's
    count is the vertex count of the draws that use it). */
 int rgl_gl_use_buffer(rgl_gpu* gpu, int slot, const float* dev_ptr, int count);

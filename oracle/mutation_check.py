@@ -44,6 +44,19 @@ MUTANTS = [
     ("environment face order", "face = dx > 0.0F ? 0 : 1;", "face = dx > 0.0F ? 1 : 0;"),
     ("Blinn-Phong exponent 16", "float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;",
      "float s = ndh * ndh; s = s * s; s = s * s; s = s * s;"),
+    ("eye light difference with w from p.w", "lightPos[2] - p[2], lightPos[3] - p[2] };",
+     "lightPos[2] - p[2], lightPos[3] - p[3] };"),
+    ("clipping rasteriser gl_FragCoord row", "fc[1][l] = fy + static_cast<float>(l < 2 ? 1 : 0);",
+     "fc[1][l] = fy - static_cast<float>(l >> 1);"),
+    ("alpha discard keeps alpha 0", "for (int l = 0; l < 4; ++l) mask[l] &= (color[3][l] > 0.0F) ? -1 : 0; }};",
+     "for (int l = 0; l < 4; ++l) mask[l] &= (color[3][l] >= 0.0F) ? -1 : 0; }};"),
+    ("depth unit v not flipped", "const int32_t py = cvtt((1.0F - y[l]) * dimf);", "const int32_t py = cvtt(y[l] * dimf);"),
+    ("normal matrix not transposed", "mat4_transpose(inv.ff, env.nm.ff);", "env.nm = inv;"),
+    ("spot cone test inverted", "if (ang < u[22]) att[l] = 0.111F;", "if (ang > u[22]) att[l] = 0.111F;"),
+    ("text colour slots swapped", "o.f[0] = b[6][idx]; o.f[1] = b[7][idx]; o.f[2] = b[8][idx];",
+     "o.f[0] = b[8][idx]; o.f[1] = b[7][idx]; o.f[2] = b[6][idx];"),
+    ("pattern divided by dim.x", "uu[l] = fc[0][l] / u[5]; vv[l] = fc[1][l] / u[5];",
+     "uu[l] = fc[0][l] / u[4]; vv[l] = fc[1][l] / u[4];"),
 ]
 
 
@@ -51,7 +64,7 @@ def main() -> int:
     src = open(os.path.join(HERE, "rgl_oracle.cpp")).read()
     missed = 0
     with tempfile.TemporaryDirectory() as tmp:
-        for inc in ("rcpps_table.inc", "srgb8_table.inc"):
+        for inc in ("rcpps_table.inc", "rsqrtps_table.inc", "srgb8_table.inc"):
             os.symlink(os.path.join(HERE, inc), os.path.join(tmp, inc))
         for name, good, bad in MUTANTS:
             assert src.count(good) == 1, name
@@ -62,7 +75,7 @@ def main() -> int:
                             "-lpthread"], check=True)
             env = dict(os.environ, RGL_ORACLE_SO=so)
             r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "tests/test_oracle_pins.py",
-                                "tests/test_oracle_cpu.py", "-k", "not host_hardware and not reference_header"],
+                                "tests/test_program_pins.py", "tests/test_oracle_cpu.py", "-k", "not host_hardware and not reference_header"],
                                cwd=ROOT, env=env, capture_output=True, text=True)
             caught = r.returncode != 0
             failed = [l for l in r.stdout.splitlines() if l.startswith("FAILED")]

@@ -34,6 +34,7 @@
 namespace {
 
 #include "rcpps_table.inc"
+#include "rsqrtps_table.inc"
 #include "srgb8_table.inc"
 
 // ------------------------------------------------------------------------
@@ -59,6 +60,18 @@ inline float rcpps(float x) {
 	const int re = 126 + 127 - static_cast<int>(e);
 	if (re <= 0) return u2f(s);
 	return u2f(s | (static_cast<uint32_t>(re) << 23) | kRcpMantissa[m >> 12]); }
+
+// _mm_rsqrt_ps, Intel table model (see gen_rsqrtps_table.c)
+inline float rsqrtps(float x) {
+	const uint32_t in = f2u(x);
+	const uint32_t e = (in >> 23) & 0xffu, s = in & 0x80000000u, m = in & 0x7fffffu;
+	if (e == 0xffu) return u2f(m ? (in | 0x400000u) : (s ? 0xffc00000u : 0u));
+	if (e == 0u) return u2f(s | 0x7f800000u);
+	if (s) return u2f(0xffc00000u);
+	const bool even = (e & 1u) == 0u;
+	const uint32_t t = kRsqrtTab[(even ? 2048u : 0u) + (m >> 12)];
+	const int re = static_cast<int>((t >> 23) & 0xffu) - (even ? (static_cast<int>(e) - 128) / 2 : (static_cast<int>(e) - 127) / 2);
+	return u2f((static_cast<uint32_t>(re) << 23) | (t & 0x7fffffu)); }
 
 // rmlv::oneover: rcp + one Newton-Raphson step (rmlv_mvec4.hxx:645-649)
 inline float oneover(float a) {
@@ -121,6 +134,45 @@ inline void xform3w0(const float* f, const float b[3], float o[3]) {
 	o[1] = (f[1]*b[0] + f[5]*b[1]) + f[ 9]*b[2];
 	o[2] = (f[2]*b[0] + f[6]*b[1]) + f[10]*b[2]; }
 
+// rmlm::inverse (rmlm_mat4.cxx:23-167): each element a signed sum of six
+// triple products of the column-major elements, summed left to right, then
+// scaled by 1 / det with det = m0 inv0 + m1 inv4 + m2 inv8 + m3 inv12
+struct InvTerm { signed char s; unsigned char a, b, c; };
+const InvTerm kInvTerms[16][6] = {
+	{ { 1, 5, 10, 15 }, { -1, 5, 11, 14 }, { -1, 9, 6, 15 }, { 1, 9, 7, 14 }, { 1, 13, 6, 11 }, { -1, 13, 7, 10 } },   // 0
+	{ { -1, 1, 10, 15 }, { 1, 1, 11, 14 }, { 1, 9, 2, 15 }, { -1, 9, 3, 14 }, { -1, 13, 2, 11 }, { 1, 13, 3, 10 } },   // 1
+	{ { 1, 1, 6, 15 }, { -1, 1, 7, 14 }, { -1, 5, 2, 15 }, { 1, 5, 3, 14 }, { 1, 13, 2, 7 }, { -1, 13, 3, 6 } },        // 2
+	{ { -1, 1, 6, 11 }, { 1, 1, 7, 10 }, { 1, 5, 2, 11 }, { -1, 5, 3, 10 }, { -1, 9, 2, 7 }, { 1, 9, 3, 6 } },          // 3
+	{ { -1, 4, 10, 15 }, { 1, 4, 11, 14 }, { 1, 8, 6, 15 }, { -1, 8, 7, 14 }, { -1, 12, 6, 11 }, { 1, 12, 7, 10 } },   // 4
+	{ { 1, 0, 10, 15 }, { -1, 0, 11, 14 }, { -1, 8, 2, 15 }, { 1, 8, 3, 14 }, { 1, 12, 2, 11 }, { -1, 12, 3, 10 } },   // 5
+	{ { -1, 0, 6, 15 }, { 1, 0, 7, 14 }, { 1, 4, 2, 15 }, { -1, 4, 3, 14 }, { -1, 12, 2, 7 }, { 1, 12, 3, 6 } },        // 6
+	{ { 1, 0, 6, 11 }, { -1, 0, 7, 10 }, { -1, 4, 2, 11 }, { 1, 4, 3, 10 }, { 1, 8, 2, 7 }, { -1, 8, 3, 6 } },          // 7
+	{ { 1, 4, 9, 15 }, { -1, 4, 11, 13 }, { -1, 8, 5, 15 }, { 1, 8, 7, 13 }, { 1, 12, 5, 11 }, { -1, 12, 7, 9 } },      // 8
+	{ { -1, 0, 9, 15 }, { 1, 0, 11, 13 }, { 1, 8, 1, 15 }, { -1, 8, 3, 13 }, { -1, 12, 1, 11 }, { 1, 12, 3, 9 } },      // 9
+	{ { 1, 0, 5, 15 }, { -1, 0, 7, 13 }, { -1, 4, 1, 15 }, { 1, 4, 3, 13 }, { 1, 12, 1, 7 }, { -1, 12, 3, 5 } },        // 10
+	{ { -1, 0, 5, 11 }, { 1, 0, 7, 9 }, { 1, 4, 1, 11 }, { -1, 4, 3, 9 }, { -1, 8, 1, 7 }, { 1, 8, 3, 5 } },            // 11
+	{ { -1, 4, 9, 14 }, { 1, 4, 10, 13 }, { 1, 8, 5, 14 }, { -1, 8, 6, 13 }, { -1, 12, 5, 10 }, { 1, 12, 6, 9 } },      // 12
+	{ { 1, 0, 9, 14 }, { -1, 0, 10, 13 }, { -1, 8, 1, 14 }, { 1, 8, 2, 13 }, { 1, 12, 1, 10 }, { -1, 12, 2, 9 } },      // 13
+	{ { -1, 0, 5, 14 }, { 1, 0, 6, 13 }, { 1, 4, 1, 14 }, { -1, 4, 2, 13 }, { -1, 12, 1, 6 }, { 1, 12, 2, 5 } },        // 14
+	{ { 1, 0, 5, 10 }, { -1, 0, 6, 9 }, { -1, 4, 1, 10 }, { 1, 4, 2, 9 }, { 1, 8, 1, 6 }, { -1, 8, 2, 5 } } };          // 15
+
+inline void mat4_inverse(const float* m, float* inv) {
+	for (int i = 0; i < 16; ++i) {
+		const InvTerm* t = kInvTerms[i];
+		const float a0 = t[0].s < 0 ? -m[t[0].a] : m[t[0].a];
+		float acc = (a0 * m[t[0].b]) * m[t[0].c];
+		for (int k = 1; k < 6; ++k) {
+			const float p = (m[t[k].a] * m[t[k].b]) * m[t[k].c];
+			acc = t[k].s < 0 ? acc - p : acc + p; }
+		inv[i] = acc; }
+	const float det = ((m[0] * inv[0] + m[1] * inv[4]) + m[2] * inv[8]) + m[3] * inv[12];
+	const float invdet = 1.0F / det;
+	for (int i = 0; i < 16; ++i) inv[i] = inv[i] * invdet; }
+
+// transpose (rmlm_mat4.hxx:211-216)
+inline void mat4_transpose(const float* m, float* out) {
+	for (int r = 0; r < 4; ++r) for (int c = 0; c < 4; ++c) out[c * 4 + r] = m[r * 4 + c]; }
+
 // ------------------------------------------------------------------------
 // protocol and state (rglv_gpu_protocol.hxx, rglv_gl.hxx)
 // ------------------------------------------------------------------------
@@ -151,6 +203,7 @@ struct GLState {
 	int color0AttachmentType, depthAttachmentType;
 	const float* buffers[16];
 	TextureState tus[2];
+	const float* tu3ptr; int tu3dim;   // GL::BindTexture3 (rglv_gl.hxx:275-278)
 	mat4 viewMatrix, normalMatrix, projectionMatrix;
 
 	void reset() {  // GLState::reset (rglv_gl.hxx:152-179)
@@ -163,7 +216,8 @@ struct GLState {
 		depthTestEnabled = true;
 		depthFunc = GL_LESS;
 		uniformsOfs = -1;
-		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }}};
+		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }
+		tu3ptr = nullptr; tu3dim = 256; }};
 
 struct Cmd {
 	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
@@ -260,11 +314,27 @@ struct Sampler {
 constexpr int kMaxVary = 16;
 struct VOut { float f[kMaxVary]; };
 
+// DepthTextureUnit (rglr_texture_sampler.hxx:33-79): nearest, border -1
+// outside [0, 1)^2, offset wrapped by dim^2 - 1
+struct DepthUnit {
+	const float* buf = nullptr;
+	int dim = 256;
+	void sample(const float x[4], const float y[4], float out[4]) const {
+		const float dimf = static_cast<float>(dim);
+		for (int l = 0; l < 4; ++l) {
+			const bool hit = (x[l] >= 0.0F) && (x[l] < 1.0F) && (y[l] >= 0.0F) && (y[l] < 1.0F);
+			const int32_t px = cvtt(x[l] * dimf);
+			const int32_t py = cvtt((1.0F - y[l]) * dimf);
+			const int32_t ofs = wadd(wmul(py, dim), px) & wsub(wmul(dim, dim), 1);
+			const float c = buf ? buf[ofs] : 0.0F;
+			out[l] = hit ? c : -1.0F; }}};
+
 struct DrawEnv {
 	const GLState* st;
 	const float* uniforms;  // UniformsSD, may be null
-	mat4 vpm;
-	Sampler tu0, tu1; };
+	mat4 vpm, vm, nm;       // MakeMatrices (rglv_gpu_impl.hxx:44-51)
+	Sampler tu0, tu1;
+	DepthUnit tu3; };
 
 inline void normalize3(float& x, float& y, float& z) {
 	const float len2 = (x*x + y*y) + z*z;
@@ -499,12 +569,184 @@ struct GGXEnvProgram final : Program {
 			ggx_shade(e, s, c);
 			color[0][l] = c[0]; color[1][l] = c[1]; color[2][l] = c[2]; color[3][l] = 1.0F; }}};
 
+// ---- the viewer's other programs (viewer/shaders.hxx) ----
+inline float dot4(const float a[4], const float b[4]) { return ((a[0]*b[0] + a[1]*b[1]) + a[2]*b[2]) + a[3]*b[3]; }
+
+// AlphaTextureProgram (viewer/shaders.hxx:164-230): late depth test,
+// gl_triMask &= alpha > 0
+struct AlphaTextureProgram final : Program {
+	AlphaTextureProgram() { id = 65; nvary = 2; earlyZ = false; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		o.f[0] = buf_at(e.st->buffers[9], idx, 0.0F);
+		o.f[1] = buf_at(e.st->buffers[10], idx, 0.0F);
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t mask[4]) const override {
+		e.tu0.sample(vary[0], vary[1], color);
+		for (int l = 0; l < 4; ++l) mask[l] &= (color[3][l] > 0.0F) ? -1 : 0; }};
+
+// TextProgram (viewer/shaders.hxx:232-318): texture times vertex colour
+// (slots 6-8, default white), late depth test, alpha discard
+struct TextProgram final : Program {
+	TextProgram() { id = 26; nvary = 5; earlyZ = false; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		const auto& b = e.st->buffers;
+		if (b[6]) { o.f[0] = b[6][idx]; o.f[1] = b[7][idx]; o.f[2] = b[8][idx]; }
+		else { o.f[0] = o.f[1] = o.f[2] = 1.0F; }
+		o.f[3] = buf_at(b[9], idx, 0.0F);
+		o.f[4] = buf_at(b[10], idx, 0.0F);
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t mask[4]) const override {
+		e.tu0.sample(vary[3], vary[4], color);
+		for (int l = 0; l < 4; ++l) {
+			color[0][l] *= vary[0][l]; color[1][l] *= vary[1][l]; color[2][l] *= vary[2][l];
+			mask[l] &= (color[3][l] > 0.0F) ? -1 : 0; }}};
+
+// DepthProgram (viewer/shaders.hxx:321-420): tu3 at the uv, linearised
+struct DepthProgram final : Program {
+	DepthProgram() { id = 5; nvary = 2; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		o.f[0] = buf_at(e.st->buffers[9], idx, 0.0F);
+		o.f[1] = buf_at(e.st->buffers[10], idx, 0.0F);
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float zNear = 10.0F, zFar = 1000.0F;
+		float tmp[4];
+		e.tu3.sample(vary[0], vary[1], tmp);
+		for (int l = 0; l < 4; ++l) {
+			const float t = (2.0F * zNear) / ((zFar + zNear) - tmp[l] * (zFar - zNear));
+			color[0][l] = t; color[1][l] = t; color[2][l] = t; color[3][l] = 1.0F; }}};
+
+// PatternProgram (viewer/shaders.hxx:422-480): tu0 at gl_FragCoord / dim.y
+// + offset.xy; uniforms offset (vec4), dim (vec4)
+struct PatternProgram final : Program {
+	PatternProgram() { id = 41; nvary = 1; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		o.f[0] = 0.0F;
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float[kMaxVary][4], const float fc[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float* u = e.uniforms;
+		float uu[4], vv[4];
+		for (int l = 0; l < 4; ++l) { uu[l] = fc[0][l] / u[5]; vv[l] = fc[1][l] / u[5]; uu[l] = uu[l] + u[0]; vv[l] = vv[l] + u[1]; }
+		e.tu0.sample(uu, vv, color); }};
+
+inline void obj_normal(const DrawEnv& e, int idx, float n[3]) {
+	const auto& b = e.st->buffers;
+	if (b[3]) { n[0] = b[3][idx]; n[1] = b[4][idx]; n[2] = b[5][idx]; } else { n[0] = 0.0F; n[1] = 0.0F; n[2] = 1.0F; }}
+inline void obj_kd(const DrawEnv& e, int idx, float kd[3]) {
+	const auto& b = e.st->buffers;
+	if (b[6]) { kd[0] = b[6][idx]; kd[1] = b[7][idx]; kd[2] = b[8][idx]; } else { kd[0] = kd[1] = kd[2] = 1.0F; }}
+
+// the eye-light diffuse of OBJ1 (per vertex) and OBJ2 (per fragment): the
+// qfloat4 difference lightPos - p has w = lightPos.w - p.z
+// (rmlv_soa.hxx:163), length uses sqrt, normalize rsqrtps
+inline float obj_diffuse(const float p[4], const float n[4]) {
+	const float lightPos[4] = { 0.0F, 0.0F, 0.0F, 1.0F };
+	float d[4] = { lightPos[0] - p[0], lightPos[1] - p[1], lightPos[2] - p[2], lightPos[3] - p[2] };
+	const float distance = std::sqrt(dot4(d, d));
+	const float scale = rsqrtps(dot4(d, d));
+	const float lv[4] = { d[0] * scale, d[1] * scale, d[2] * scale, d[3] * scale };
+	float diffuse = sse_max(dot4(n, lv), 0.1F);
+	diffuse = diffuse * (100.0F / distance);
+	return diffuse; }
+
+// OBJ1Program (viewer/shaders.hxx:584-681)
+struct OBJ1Program final : Program {
+	OBJ1Program() { id = 7; nvary = 3; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		float n[3], kd[3]; obj_normal(e, idx, n); obj_kd(e, idx, kd);
+		float mvv[4], mvn[4];
+		xform4(e.vm.ff, p, mvv);
+		const float n4[4] = { n[0], n[1], n[2], 0.0F };
+		xform4(e.vm.ff, n4, mvn);
+		const float diffuse = obj_diffuse(mvv, mvn);
+		o.f[0] = kd[0] * diffuse; o.f[1] = kd[1] * diffuse; o.f[2] = kd[2] * diffuse;
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv&, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		for (int l = 0; l < 4; ++l) { color[0][l] = vary[0][l]; color[1][l] = vary[1][l]; color[2][l] = vary[2][l]; color[3][l] = 1.0F; }}};
+
+// OBJ2Program (viewer/shaders.hxx:701-818): varyings sp (4), sn (4), kd (3)
+struct OBJ2Program final : Program {
+	OBJ2Program() { id = 8; nvary = 11; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		float n[3]; obj_normal(e, idx, n);
+		xform4(e.vm.ff, p, o.f);
+		const float n4[4] = { n[0], n[1], n[2], 0.0F };
+		xform4(e.vm.ff, n4, o.f + 4);
+		obj_kd(e, idx, o.f + 8);
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv&, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		for (int l = 0; l < 4; ++l) {
+			const float sp[4] = { vary[0][l], vary[1][l], vary[2][l], vary[3][l] };
+			const float sn[4] = { vary[4][l], vary[5][l], vary[6][l], vary[7][l] };
+			const float diffuse = obj_diffuse(sp, sn);
+			color[0][l] = vary[8][l] * diffuse; color[1][l] = vary[9][l] * diffuse; color[2][l] = vary[10][l] * diffuse;
+			color[3][l] = 1.0F; }}};
+
+// OBJ2SProgram (viewer/shaders.hxx:821-978): varyings sp (4), sn (4), kd
+// (3), lp (4); uniforms modelToShadow (mat4), lpos, ldir (vec3), lcos
+struct OBJ2SProgram final : Program {
+	OBJ2SProgram() { id = 9; nvary = 15; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		float n[3]; obj_normal(e, idx, n);
+		xform4(e.vm.ff, p, o.f);
+		const float n4[4] = { n[0], n[1], n[2], 1.0F };
+		xform4(e.nm.ff, n4, o.f + 4);
+		obj_kd(e, idx, o.f + 8);
+		xform4(e.uniforms, p, o.f + 11);
+		xform4(e.vpm.ff, p, pos); }
+	static void normalize3r(float v[3]) {
+		const float s = rsqrtps((v[0]*v[0] + v[1]*v[1]) + v[2]*v[2]);
+		v[0] = v[0] * s; v[1] = v[1] * s; v[2] = v[2] * s; }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		const float* u = e.uniforms;
+		float att[4], diff[4], lx[4], ly[4], lz[4];
+		for (int l = 0; l < 4; ++l) {
+			float stl[3] = { u[16] - vary[0][l], u[17] - vary[1][l], u[18] - vary[2][l] };
+			const float dist = std::sqrt((stl[0]*stl[0] + stl[1]*stl[1]) + stl[2]*stl[2]);
+			normalize3r(stl);
+			att[l] = 100.0F / dist;
+			float ld[3] = { u[19], u[20], u[21] };
+			normalize3r(ld);
+			const float ang = ((-stl[0])*ld[0] + (-stl[1])*ld[1]) + (-stl[2])*ld[2];
+			if (ang < u[22]) att[l] = 0.111F;
+			float sn[3] = { vary[4][l], vary[5][l], vary[6][l] };
+			normalize3r(sn);
+			diff[l] = sse_max(0.0F, (sn[0]*stl[0] + sn[1]*stl[1]) + sn[2]*stl[2]);
+			lx[l] = vary[11][l] / vary[14][l]; ly[l] = vary[12][l] / vary[14][l]; lz[l] = vary[13][l] / vary[14][l]; }
+		float sx[4], sy[4], depth[4];
+		for (int l = 0; l < 4; ++l) { sx[l] = lx[l] * 0.5F + 0.5F; sy[l] = ly[l] * 0.5F + 0.5F; }
+		e.tu3.sample(sx, sy, depth);
+		for (int l = 0; l < 4; ++l) {
+			if (lz[l] - 0.0005F > depth[l]) att[l] = 0.111F;
+			color[0][l] = (att[l] * vary[8][l]) * diff[l];
+			color[1][l] = (att[l] * vary[9][l]) * diff[l];
+			color[2][l] = (att[l] * vary[10][l]) * diff[l];
+			color[3][l] = 1.0F; }}};
+
 const Program* find_program(int id) {
 	static const AmyProgram amy; static const ManyProgram many;
 	static const LambertTexProgram lam; static const ManyBlinnPhongProgram mbp;
 	static const GGXEnvProgram ggx;
+	static const AlphaTextureProgram alpha; static const TextProgram text; static const DepthProgram depth;
+	static const PatternProgram pattern; static const OBJ1Program obj1; static const OBJ2Program obj2; static const OBJ2SProgram obj2s;
 	switch (id) {
 	case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp; case 82: return &ggx;
+	case 65: return &alpha; case 26: return &text; case 5: return &depth; case 41: return &pattern;
+	case 7: return &obj1; case 8: return &obj2; case 9: return &obj2s;
 	default: return nullptr; }}
 
 // post programs for StoreTrueColor (ShadeCanvas); 1 = Default, 2 = Exposure
@@ -662,8 +904,14 @@ struct rglo_gpu {
 
 	bool MakeEnv(const GLState& s, DrawEnv& e) {
 		e.st = &s;
-		e.uniforms = Uniforms(s);
+		static const float kNoUniforms[UNIFORM_BUFFER_SIZE] = {};  // a draw without uniforms reads zeros
+		e.uniforms = Uniforms(s) ? Uniforms(s) : kNoUniforms;
 		e.vpm = mul(s.projectionMatrix, s.viewMatrix);  // MakeMatrices (rglv_gpu_impl.hxx:44-51)
+		e.vm = s.viewMatrix;
+		mat4 inv;
+		mat4_inverse(s.viewMatrix.ff, inv.ff);
+		mat4_transpose(inv.ff, e.nm.ff);
+		e.tu3.buf = s.tu3ptr; e.tu3.dim = s.tu3dim;
 		std::string terr;
 		e.tu0 = Sampler::Make(s.tus[0], terr);
 		e.tu1 = Sampler::Make(s.tus[1], terr);
@@ -1209,6 +1457,7 @@ void rglo_gl_renderbuffer_type(rglo_gpu* g, int attachment, int type) {
 void rglo_gl_bind_texture(rglo_gpu* g, int unit, const float* ptr, int w, int h, int stride, int mode) {
 	g->dirty = true; g->cs.tus[unit] = { ptr, w, h, stride, mode }; }
 void rglo_gl_use_buffer(rglo_gpu* g, int idx, const float* ptr) { g->dirty = true; g->cs.buffers[idx] = ptr; }
+void rglo_gl_bind_texture3(rglo_gpu* g, const float* ptr, int dim) { g->dirty = true; g->cs.tu3ptr = ptr; g->cs.tu3dim = dim; }
 void rglo_gl_clear_color(rglo_gpu* g, float r, float gg, float b) {
 	g->dirty = true; g->cs.clearColor[0] = r; g->cs.clearColor[1] = gg; g->cs.clearColor[2] = b; g->cs.clearColor[3] = 1.0F; }
 void rglo_gl_clear_depth(rglo_gpu* g, float v) { g->dirty = true; g->cs.clearDepth = v; }
@@ -1365,6 +1614,9 @@ int rglo_sample(const float* buf, int w, int h, int stride, int filter, const fl
 	return s.kind; }
 
 // LevelOfDetail from lanes 0 and 1 of base-level texel coordinates
+// the rsqrtps model (rmlv_soa.hxx:243-248 normalize)
+float rglo_rsqrtps(float x) { return rsqrtps(x); }
+
 int rglo_level_of_detail(const float* u, const float* v) { return level_of_detail(u, v); }
 
 // ShadeFragment of program pid for one quad: vary[k * 4 + lane] (k < the
@@ -1389,5 +1641,56 @@ int rglo_shade(int pid, const float* uniforms, const float* const* buffers, cons
 	pgm->ShadeFragment(env, bs, bp, vv, fc, fd, color, mask);
 	for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c * 4 + l] = color[c][l];
 	return 0; }
+
+// the draw environment of the two calls below: view matrix vm (column-major,
+// null = identity), the normal matrix from it as MakeMatrices forms it, the
+// depth texture unit (tu3, dim)
+static bool shade_env(int pid, const float* uniforms, const float* const* buffers, const float* tex, int texDim, int filter,
+                      const float* vm, const float* tu3, int tu3dim, GLState& st, DrawEnv& env, const Program*& pgm) {
+	pgm = find_program(pid);
+	if (!pgm) return false;
+	st.reset();
+	for (int i = 0; i < 16; ++i) st.buffers[i] = buffers ? buffers[i] : nullptr;
+	st.tus[0] = { tex, texDim, texDim, texDim, filter };
+	env = DrawEnv{};
+	static const float kNoUniforms[UNIFORM_BUFFER_SIZE] = {};
+	env.st = &st; env.uniforms = uniforms ? uniforms : kNoUniforms;
+	for (int i = 0; i < 16; ++i) env.vm.ff[i] = vm ? vm[i] : ((i % 5 == 0) ? 1.0F : 0.0F);
+	mat4 inv;
+	mat4_inverse(env.vm.ff, inv.ff);
+	mat4_transpose(inv.ff, env.nm.ff);
+	env.tu3.buf = tu3; env.tu3.dim = tu3dim;
+	std::string err;
+	if (tex) env.tu0 = Sampler::Make(st.tus[0], err);
+	return err.empty(); }
+
+// ShadeFragment with gl_FragCoord (fc[axis * 4 + lane], may be null), the
+// view/normal matrices and the depth unit; colour to out[channel * 4 +
+// lane], gl_triMask after the shader to mask_out[lane]
+int rglo_shade_ex(int pid, const float* uniforms, const float* const* buffers, const float* tex, int texDim, int filter,
+                  const float* vm, const float* tu3, int tu3dim, const float* fcIn, const float* vary, float* out,
+                  int32_t* mask_out) {
+	GLState st; DrawEnv env; const Program* pgm;
+	if (!shade_env(pid, uniforms, buffers, tex, texDim, filter, vm, tu3, tu3dim, st, env, pgm)) return -1;
+	float bs[3][4] = {}, bp[3][4] = {}, fc[2][4] = {}, fd[4] = {}, vv[kMaxVary][4] = {}, color[4][4] = {};
+	for (int k = 0; k < pgm->nvary; ++k) for (int l = 0; l < 4; ++l) vv[k][l] = vary[k * 4 + l];
+	if (fcIn) for (int a = 0; a < 2; ++a) for (int l = 0; l < 4; ++l) fc[a][l] = fcIn[a * 4 + l];
+	int32_t mask[4] = { -1, -1, -1, -1 };
+	pgm->ShadeFragment(env, bs, bp, vv, fc, fd, color, mask);
+	for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) out[c * 4 + l] = color[c][l];
+	for (int l = 0; l < 4; ++l) mask_out[l] = mask[l];
+	return pgm->nvary; }
+
+// ShadeVertex of vertex idx (instance iid): clip position to pos[4],
+// varyings to vary[kMaxVary]; vpm = projection * view (column-major)
+int rglo_shade_vertex(int pid, const float* uniforms, const float* const* buffers, const float* vpm, const float* vm,
+                      int iid, int idx, float* pos, float* vary) {
+	GLState st; DrawEnv env; const Program* pgm;
+	if (!shade_env(pid, uniforms, buffers, nullptr, 0, 0, vm, nullptr, 0, st, env, pgm)) return -1;
+	for (int i = 0; i < 16; ++i) env.vpm.ff[i] = vpm[i];
+	VOut o{};
+	pgm->ShadeVertex(env, iid, idx, pos, o);
+	for (int k = 0; k < kMaxVary; ++k) vary[k] = o.f[k];
+	return pgm->nvary; }
 
 }  // extern "C"

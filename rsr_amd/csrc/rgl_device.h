@@ -19,11 +19,12 @@
 namespace rgl {
 
 #include "rcpps_table.inc"
+#include "rsqrtps_table.inc"
 #include "srgb8_table.inc"
 
 constexpr int kMaxDraws = 256;      // draws per batch
 constexpr int kMaxFrames = 64;      // frames (render targets) per batch
-constexpr int kMaxVary = 12;       // varyings per vertex (largest program, padded to 16 bytes)
+constexpr int kMaxVary = 16;       // varyings per vertex (the largest program's 15, padded to 16 bytes)
 constexpr int kClipVerts = 8;      // 3 + one per clipping plane
 constexpr int kFanMax = 6;         // kClipVerts - 2
 constexpr int kSlotsPerTri = 1 + kFanMax;  // inline slot + fan slots
@@ -50,6 +51,20 @@ __device__ inline float rcpps(float x) {
 	const int re = 253 - static_cast<int>(e);
 	if (re <= 0) return u2f(s);
 	return u2f(s | (static_cast<uint32_t>(re) << 23) | kRcpMantissa[m >> 12]); }
+
+// rsqrtps on the same host (rmlv::rsqrt, rmlv_mvec4.hxx:499-500): the result
+// of a table entry chosen by the exponent's parity and the top 11 mantissa
+// bits, its exponent shifted by half the input's
+__device__ inline float rsqrtps(float x) {
+	const uint32_t in = f2u(x);
+	const uint32_t e = (in >> 23) & 0xffu, s = in & 0x80000000u, m = in & 0x7fffffu;
+	if (e == 0xffu) return u2f(m ? (in | 0x400000u) : (s ? 0xffc00000u : 0u));
+	if (e == 0u) return u2f(s | 0x7f800000u);
+	if (s) return u2f(0xffc00000u);
+	const bool even = (e & 1u) == 0u;
+	const uint32_t t = kRsqrtTab[(even ? 2048u : 0u) + (m >> 12)];
+	const int re = static_cast<int>((t >> 23) & 0xffu) - (even ? (static_cast<int>(e) - 128) / 2 : (static_cast<int>(e) - 127) / 2);
+	return u2f((static_cast<uint32_t>(re) << 23) | (t & 0x7fffffu)); }
 
 // rmlv::oneover: rcpps + one Newton step (rmlv_mvec4.hxx:645-649)
 __device__ inline float oneover(float a) {
@@ -151,6 +166,8 @@ struct DrawDesc {
 	const uint16_t* indices;
 	const float* buffers[16];
 	float vpm[16];           // projection * view (MakeMatrices, rglv_gpu_impl.hxx:44-51)
+	float vm[16];            // view
+	float nm[16];            // transpose(inverse(view))
 	float DS[2], DO[2];      // GPU::DSDO (rglv_gpu.hxx:265-270)
 	float gbFactor;          // CalcGuardBandFactor (rglv_view_frustum.hxx:36-39)
 	int scissor[4];          // x0, y0, x1, y1 (GPU::ScissorRect)
@@ -162,6 +179,8 @@ struct DrawDesc {
 	int depthTest, depthFunc, depthWrite, colorWrite, blend, scissorOn;
 	float uniforms[32];      // UniformsSD (<= 128 bytes)
 	SamplerDesc tu0, tu1;
+	const float* tu3;        // DepthTextureUnit (rglr_texture_sampler.hxx:33-79): dim x dim floats
+	int tu3dim;
 	// the frame this draw renders into and its tile grid
 	int frame;
 	int tileW, tileH, tilesX, tilesY, tileBase;
@@ -280,237 +299,5 @@ __device__ inline float4 sample1(const SamplerDesc& s, float u, float v, int lod
 	                   ((p00.y * w00 + p10.y * w10) + p01.y * w01) + p11.y * w11,
 	                   ((p00.z * w00 + p10.z * w10) + p01.z * w01) + p11.z * w11,
 	                   ((p00.w * w00 + p10.w * w10) + p01.w * w01) + p11.w * w11); }
-
-// ---------------------------------------------------------------------------
-// shader programs, in the reference's program concept
-// (rglv_gpu_shaders.hxx:14-95): ShadeVertex per vertex; ShadeFragment
-// evaluated here one lane at a time (lanes of a quad are independent except
-// for the texture LOD, which the caller computes from lanes 0 and 1 of the
-// quad); varyings are interpolated with the perspective-correct BP.
-// kUV is the index of the first texture-coordinate varying (-1: none).
-// ---------------------------------------------------------------------------
-__device__ inline float buf_at(const float* p, int idx, float dflt) { return p ? p[idx] : dflt; }
-
-__device__ inline void load_pos(const DrawDesc& d, int idx, float p[4]) {
-	p[0] = buf_at(d.buffers[0], idx, 0.0F); p[1] = buf_at(d.buffers[1], idx, 0.0F);
-	p[2] = buf_at(d.buffers[2], idx, 0.0F); p[3] = 1.0F; }
-
-// AmyProgram (viewer/shaders.hxx:69-161)
-struct ProgAmy {
-	static constexpr int kId = 4, kVary = 2, kUV = 0;
-	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
-		float p[4]; load_pos(d, idx, p);
-		xform4(d.vpm, p, pos);
-		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
-		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
-		const float4 t = sample1(d.tu0, v[0], v[1], lod);
-		color[0] = t.x; color[1] = t.y; color[2] = t.z; color[3] = t.w; }};
-
-// ManyProgram (viewer/shaders.hxx:482-583)
-struct ProgMany {
-	static constexpr int kId = 6, kVary = 2, kUV = -1;
-	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
-		float p[4]; load_pos(d, idx, p);
-		const float* imat = d.buffers[15] + 16 * iid;
-		float m[16];
-#pragma unroll
-		for (int k = 0; k < 16; ++k) m[k] = imat[k];
-		float p1[4]; xform4(m, p, p1);
-		xform4(d.vpm, p1, pos);
-		vo[0] = buf_at(d.buffers[9], idx, 0.0F);
-		vo[1] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int, float color[4]) {
-		color[0] = v[0]; color[1] = v[1]; color[2] = d.uniforms[0]; color[3] = 1.0F; }};
-
-// LambertTex (this project): textured, one directional light.
-// uniforms: [0..2] unit direction toward the light, [4] ambient, [5] diffuse
-struct ProgLambertTex {
-	static constexpr int kId = 80, kVary = 5, kUV = 3;
-	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
-		float p[4]; load_pos(d, idx, p);
-		xform4(d.vpm, p, pos);
-		vo[0] = buf_at(d.buffers[3], idx, 0.0F); vo[1] = buf_at(d.buffers[4], idx, 0.0F);
-		vo[2] = buf_at(d.buffers[5], idx, 1.0F);
-		vo[3] = buf_at(d.buffers[9], idx, 0.0F); vo[4] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
-		const float* u = d.uniforms;
-		const float4 tex = sample1(d.tu0, v[3], v[4], lod);
-		color[3] = 1.0F;
-		float nx = v[0], ny = v[1], nz = v[2];
-		normalize3(nx, ny, nz);
-		const float ndl = sse_max((nx*u[0] + ny*u[1]) + nz*u[2], 0.0F);
-		const float k = u[4] + u[5] * ndl;
-		color[0] = tex.x * k; color[1] = tex.y * k; color[2] = tex.z * k; }};
-
-// ManyBlinnPhong (this project): instanced, textured, one point light.
-// uniforms: [0..2] eye, [4..6] light position, [8] ka, [9] kd, [10] ks
-struct ProgManyBlinnPhong {
-	static constexpr int kId = 81, kVary = 8, kUV = 6;
-	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
-		float p[4]; load_pos(d, idx, p);
-		const float* imat = d.buffers[15] + 16 * iid;
-		float m[16];
-#pragma unroll
-		for (int k = 0; k < 16; ++k) m[k] = imat[k];
-		float p1[4]; xform4(m, p, p1);
-		xform4(d.vpm, p1, pos);
-		const float n[3] = { buf_at(d.buffers[3], idx, 0.0F), buf_at(d.buffers[4], idx, 0.0F), buf_at(d.buffers[5], idx, 1.0F) };
-		float wn[3]; xform3w0(m, n, wn);
-		vo[0] = p1[0]; vo[1] = p1[1]; vo[2] = p1[2];
-		vo[3] = wn[0]; vo[4] = wn[1]; vo[5] = wn[2];
-		vo[6] = buf_at(d.buffers[9], idx, 0.0F); vo[7] = buf_at(d.buffers[10], idx, 0.0F); }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
-		const float* u = d.uniforms;
-		const float4 tex = sample1(d.tu0, v[6], v[7], lod);
-		color[3] = 1.0F;
-		const float px = v[0], py = v[1], pz = v[2];
-		float nx = v[3], ny = v[4], nz = v[5];
-		normalize3(nx, ny, nz);
-		float vx = u[0] - px, vy = u[1] - py, vz = u[2] - pz;
-		normalize3(vx, vy, vz);
-		float lx = u[4] - px, ly = u[5] - py, lz = u[6] - pz;
-		normalize3(lx, ly, lz);
-		float hx = lx + vx, hy = ly + vy, hz = lz + vz;
-		normalize3(hx, hy, hz);
-		const float ndl = sse_max((nx*lx + ny*ly) + nz*lz, 0.0F);
-		const float ndh = sse_max((nx*hx + ny*hy) + nz*hz, 0.0F);
-		float s = ndh * ndh; s = s * s; s = s * s; s = s * s; s = s * s;
-		const float kdiff = u[8] + u[9] * ndl;
-		const float kspec = u[10] * s;
-		color[0] = tex.x * kdiff + kspec;
-		color[1] = tex.y * kdiff + kspec;
-		color[2] = tex.z * kdiff + kspec; }};
-
-// GGXEnvProgram (new, the heavy shader): instanced and textured like
-// ManyBlinnPhong; 16 point lights with Cook-Torrance GGX specular (Smith-
-// Schlick G, Schlick F, F0 = 0.04) and per-instance roughness, plus an
-// 8-tap environment lookup around the reflection vector.  Only + - * /
-// and sqrt, in a fixed order, so the CPU oracle matches it bit for bit.
-//   buffers: 11 environment cube (6 faces of kEnvDim^2 RGBA floats),
-//            12 taps (8 x {dx, dy, dz, weight}), 13 lights (16 x {x, y, z,
-//            0, r, g, b, 0}), 14 per-instance roughness, 15 instance mats
-//   uniforms: [0..2] eye, [3] ambient, [4] env specular, [5] env diffuse
-constexpr int kGGXLights = 16, kEnvTaps = 8, kEnvDim = 16;
-constexpr float kPi = 3.14159265F, kInvPi = 0.318309886F;
-
-// the shading inputs of one fragment: unit normal and view vector, world
-// position, albedo, roughness (also the learned shader's 13 inputs)
-struct ShadeInput { float n[3], v[3], p[3], albedo[3], rough; };
-
-__device__ inline void env_sample(const float* env, float dx, float dy, float dz, float out[3]) {
-	out[0] = 0.0F; out[1] = 0.0F; out[2] = 0.0F;
-	if (env == nullptr) return;
-	const float ax = dx < 0.0F ? -dx : dx, ay = dy < 0.0F ? -dy : dy, az = dz < 0.0F ? -dz : dz;
-	int face; float ma, sc, tc;
-	if (ax >= ay && ax >= az) { face = dx > 0.0F ? 0 : 1; ma = ax; sc = dx > 0.0F ? -dz : dz; tc = -dy; }
-	else if (ay >= az) { face = dy > 0.0F ? 2 : 3; ma = ay; sc = dx; tc = dy > 0.0F ? dz : -dz; }
-	else { face = dz > 0.0F ? 4 : 5; ma = az; sc = dz > 0.0F ? dx : -dx; tc = -dy; }
-	if (!(ma > 0.0F)) return;
-	const float E = static_cast<float>(kEnvDim);
-	float fs = ((sc / ma + 1.0F) * 0.5F) * E - 0.5F;
-	float ft = ((tc / ma + 1.0F) * 0.5F) * E - 0.5F;
-	fs = fs >= 0.0F ? fs : 0.0F; fs = fs <= E - 1.0F ? fs : E - 1.0F;   // NaN -> 0
-	ft = ft >= 0.0F ? ft : 0.0F; ft = ft <= E - 1.0F ? ft : E - 1.0F;
-	const int s0 = static_cast<int>(fs), t0 = static_cast<int>(ft);
-	const int s1 = s0 + 1 < kEnvDim ? s0 + 1 : kEnvDim - 1, t1 = t0 + 1 < kEnvDim ? t0 + 1 : kEnvDim - 1;
-	const float fx = fs - static_cast<float>(s0), fy = ft - static_cast<float>(t0);
-	const float fx1 = 1.0F - fx, fy1 = 1.0F - fy;
-	const float w00 = fx1 * fy1, w10 = fx * fy1, w01 = fx1 * fy, w11 = fx * fy;
-	const float* f0 = env + static_cast<long long>(face) * kEnvDim * kEnvDim * 4;
-	const float* p00 = f0 + (t0 * kEnvDim + s0) * 4;
-	const float* p10 = f0 + (t0 * kEnvDim + s1) * 4;
-	const float* p01 = f0 + (t1 * kEnvDim + s0) * 4;
-	const float* p11 = f0 + (t1 * kEnvDim + s1) * 4;
-#pragma unroll
-	for (int c = 0; c < 3; ++c) out[c] = ((p00[c] * w00 + p10[c] * w10) + p01[c] * w01) + p11[c] * w11; }
-
-__device__ inline void ggx_shade(const DrawDesc& d, const ShadeInput& s, float color[3]) {
-	const float* u = d.uniforms;
-	const float* lights = d.buffers[13];
-	const float* taps = d.buffers[12];
-	const float rough = s.rough;
-	const float a = rough * rough, a2 = a * a;
-	const float k = ((rough + 1.0F) * (rough + 1.0F)) * 0.125F;
-	const float ndvRaw = (s.n[0]*s.v[0] + s.n[1]*s.v[1]) + s.n[2]*s.v[2];
-	const float ndv = ndvRaw > 1e-4F ? ndvRaw : 1e-4F;
-	const float gv = ndv / (ndv * (1.0F - k) + k);
-	float acc[3] = { 0.0F, 0.0F, 0.0F };
-	for (int i = 0; lights != nullptr && i < kGGXLights; ++i) {
-		const float* L = lights + 8 * i;
-		float lx = L[0] - s.p[0], ly = L[1] - s.p[1], lz = L[2] - s.p[2];
-		const float d2 = (lx*lx + ly*ly) + lz*lz;
-		const float inv = 1.0F / __builtin_sqrtf(d2);
-		lx = lx * inv; ly = ly * inv; lz = lz * inv;
-		const float ndl = (s.n[0]*lx + s.n[1]*ly) + s.n[2]*lz;
-		if (!(ndl > 0.0F)) continue;
-		float hx = lx + s.v[0], hy = ly + s.v[1], hz = lz + s.v[2];
-		normalize3(hx, hy, hz);
-		float ndh = (s.n[0]*hx + s.n[1]*hy) + s.n[2]*hz;
-		ndh = ndh > 0.0F ? ndh : 0.0F;
-		float vdh = (s.v[0]*hx + s.v[1]*hy) + s.v[2]*hz;
-		vdh = vdh > 0.0F ? vdh : 0.0F;
-		const float t = (ndh * ndh) * (a2 - 1.0F) + 1.0F;
-		const float D = a2 / ((kPi * t) * t);
-		const float gl = ndl / (ndl * (1.0F - k) + k);
-		const float f = 1.0F - vdh, f2 = f * f, f5 = (f2 * f2) * f;
-		const float F = 0.04F + 0.96F * f5;
-		const float spec = (((D * gl) * gv) * F) / ((4.0F * ndl) * ndv);
-		const float kd = (1.0F - F) * kInvPi;
-		const float id2 = 1.0F / d2;
-#pragma unroll
-		for (int c = 0; c < 3; ++c) acc[c] = acc[c] + ((s.albedo[c] * kd + spec) * (L[4 + c] * id2)) * ndl; }
-	// environment: 8 taps spread around the reflection vector by roughness
-	const float rx = (2.0F * ndvRaw) * s.n[0] - s.v[0];
-	const float ry = (2.0F * ndvRaw) * s.n[1] - s.v[1];
-	const float rz = (2.0F * ndvRaw) * s.n[2] - s.v[2];
-	float e[3] = { 0.0F, 0.0F, 0.0F };
-	for (int j = 0; taps != nullptr && j < kEnvTaps; ++j) {
-		const float* T = taps + 4 * j;
-		float smp[3];
-		env_sample(d.buffers[11], rx + T[0] * rough, ry + T[1] * rough, rz + T[2] * rough, smp);
-#pragma unroll
-		for (int c = 0; c < 3; ++c) e[c] = e[c] + T[3] * smp[c]; }
-	const float fe = 1.0F - ndv, fe2 = fe * fe, fe5 = (fe2 * fe2) * fe;
-	const float Fe = 0.04F + 0.96F * fe5;
-	const float ks = (u[4] * Fe) * (1.0F - rough);
-#pragma unroll
-	for (int c = 0; c < 3; ++c) color[c] = (acc[c] + u[3] * s.albedo[c]) + e[c] * (ks + u[5] * s.albedo[c]); }
-
-struct ProgGGXEnv {
-	static constexpr int kId = 82, kVary = 9, kUV = 6;
-	__device__ static void vertex(const DrawDesc& d, int iid, int idx, float pos[4], float* vo) {
-		ProgManyBlinnPhong::vertex(d, iid, idx, pos, vo);
-		vo[8] = d.buffers[14] ? d.buffers[14][iid] : 0.5F; }
-	// varyings -> shading inputs (the G-buffer of the learned shader)
-	__device__ static void inputs(const DrawDesc& d, const float* v, int lod, ShadeInput& s) {
-		const float4 tex = sample1(d.tu0, v[6], v[7], lod);
-		s.albedo[0] = tex.x; s.albedo[1] = tex.y; s.albedo[2] = tex.z;
-		s.p[0] = v[0]; s.p[1] = v[1]; s.p[2] = v[2];
-		float nx = v[3], ny = v[4], nz = v[5];
-		normalize3(nx, ny, nz);
-		s.n[0] = nx; s.n[1] = ny; s.n[2] = nz;
-		const float* u = d.uniforms;
-		float vx = u[0] - v[0], vy = u[1] - v[1], vz = u[2] - v[2];
-		normalize3(vx, vy, vz);
-		s.v[0] = vx; s.v[1] = vy; s.v[2] = vz;
-		s.rough = v[8]; }
-	__device__ static void fragment1(const DrawDesc& d, const float* v, int lod, float color[4]) {
-		ShadeInput s;
-		inputs(d, v, lod, s);
-		ggx_shade(d, s, color);
-		color[3] = 1.0F; }};
-
-__host__ __device__ inline int program_nvary(int pid) {
-	switch (pid) {
-	case ProgAmy::kId: return ProgAmy::kVary;
-	case ProgMany::kId: return ProgMany::kVary;
-	case ProgLambertTex::kId: return ProgLambertTex::kVary;
-	case ProgManyBlinnPhong::kId: return ProgManyBlinnPhong::kVary;
-	case ProgGGXEnv::kId: return ProgGGXEnv::kVary;
-	default: return -1; }}
-
-__host__ __device__ inline bool program_instanced(int pid) {
-	return pid == ProgMany::kId || pid == ProgManyBlinnPhong::kId || pid == ProgGGXEnv::kId; }
 
 }  // namespace rgl

@@ -47,6 +47,7 @@ struct GLState {
 	const float* buffers[16];
 	int bufferCounts[16];
 	TexState tus[2];
+	const float* tu3ptr; int tu3dim;   // GL::BindTexture3 (rglv_gl.hxx:275-278)
 	float viewMatrix[16], normalMatrix[16], projectionMatrix[16];
 
 	void reset() {
@@ -56,7 +57,8 @@ struct GLState {
 		colorWriteMask = true; depthWriteMask = true; depthTestEnabled = true;
 		depthFunc = GL_LESS;
 		uniformsOfs = -1;
-		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }}
+		for (int i = 0; i < 2; ++i) { tus[i].ptr = nullptr; tus[i].width = tus[i].height = tus[i].stride = 8; tus[i].filter = 0; }
+		tu3ptr = nullptr; tu3dim = 256; }
 
 	uint32_t FragmentStateKey() const {  // rglv_gl.hxx:136-146
 		uint32_t key = static_cast<uint32_t>(programId) << 24;
@@ -87,6 +89,46 @@ void mat4_mul(const float* l, const float* r, float* out) {
 			out[col*4 + row] = ax; }}
 
 int ilog2(unsigned x) { int p = 0; while (x != 0U) { x >>= 1; ++p; } return p - 1; }
+
+// rmlm::inverse (rmlm_mat4.cxx:23-167): each element a signed sum of six
+// triple products of the column-major elements, summed left to right, then
+// scaled by 1 / det with det = m0 inv0 + m1 inv4 + m2 inv8 + m3 inv12
+struct InvTerm { signed char s; unsigned char a, b, c; };
+static const InvTerm kInvTerms[16][6] = {
+	{ { 1, 5, 10, 15 }, { -1, 5, 11, 14 }, { -1, 9, 6, 15 }, { 1, 9, 7, 14 }, { 1, 13, 6, 11 }, { -1, 13, 7, 10 } },   // 0
+	{ { -1, 1, 10, 15 }, { 1, 1, 11, 14 }, { 1, 9, 2, 15 }, { -1, 9, 3, 14 }, { -1, 13, 2, 11 }, { 1, 13, 3, 10 } },   // 1
+	{ { 1, 1, 6, 15 }, { -1, 1, 7, 14 }, { -1, 5, 2, 15 }, { 1, 5, 3, 14 }, { 1, 13, 2, 7 }, { -1, 13, 3, 6 } },        // 2
+	{ { -1, 1, 6, 11 }, { 1, 1, 7, 10 }, { 1, 5, 2, 11 }, { -1, 5, 3, 10 }, { -1, 9, 2, 7 }, { 1, 9, 3, 6 } },          // 3
+	{ { -1, 4, 10, 15 }, { 1, 4, 11, 14 }, { 1, 8, 6, 15 }, { -1, 8, 7, 14 }, { -1, 12, 6, 11 }, { 1, 12, 7, 10 } },   // 4
+	{ { 1, 0, 10, 15 }, { -1, 0, 11, 14 }, { -1, 8, 2, 15 }, { 1, 8, 3, 14 }, { 1, 12, 2, 11 }, { -1, 12, 3, 10 } },   // 5
+	{ { -1, 0, 6, 15 }, { 1, 0, 7, 14 }, { 1, 4, 2, 15 }, { -1, 4, 3, 14 }, { -1, 12, 2, 7 }, { 1, 12, 3, 6 } },        // 6
+	{ { 1, 0, 6, 11 }, { -1, 0, 7, 10 }, { -1, 4, 2, 11 }, { 1, 4, 3, 10 }, { 1, 8, 2, 7 }, { -1, 8, 3, 6 } },          // 7
+	{ { 1, 4, 9, 15 }, { -1, 4, 11, 13 }, { -1, 8, 5, 15 }, { 1, 8, 7, 13 }, { 1, 12, 5, 11 }, { -1, 12, 7, 9 } },      // 8
+	{ { -1, 0, 9, 15 }, { 1, 0, 11, 13 }, { 1, 8, 1, 15 }, { -1, 8, 3, 13 }, { -1, 12, 1, 11 }, { 1, 12, 3, 9 } },      // 9
+	{ { 1, 0, 5, 15 }, { -1, 0, 7, 13 }, { -1, 4, 1, 15 }, { 1, 4, 3, 13 }, { 1, 12, 1, 7 }, { -1, 12, 3, 5 } },        // 10
+	{ { -1, 0, 5, 11 }, { 1, 0, 7, 9 }, { 1, 4, 1, 11 }, { -1, 4, 3, 9 }, { -1, 8, 1, 7 }, { 1, 8, 3, 5 } },            // 11
+	{ { -1, 4, 9, 14 }, { 1, 4, 10, 13 }, { 1, 8, 5, 14 }, { -1, 8, 6, 13 }, { -1, 12, 5, 10 }, { 1, 12, 6, 9 } },      // 12
+	{ { 1, 0, 9, 14 }, { -1, 0, 10, 13 }, { -1, 8, 1, 14 }, { 1, 8, 2, 13 }, { 1, 12, 1, 10 }, { -1, 12, 2, 9 } },      // 13
+	{ { -1, 0, 5, 14 }, { 1, 0, 6, 13 }, { 1, 4, 1, 14 }, { -1, 4, 2, 13 }, { -1, 12, 1, 6 }, { 1, 12, 2, 5 } },        // 14
+	{ { 1, 0, 5, 10 }, { -1, 0, 6, 9 }, { -1, 4, 1, 10 }, { 1, 4, 2, 9 }, { 1, 8, 1, 6 }, { -1, 8, 2, 5 } } };          // 15
+
+void mat4_inverse(const float* m, float* inv) {
+	for (int i = 0; i < 16; ++i) {
+		const InvTerm* t = kInvTerms[i];
+		const float a0 = t[0].s < 0 ? -m[t[0].a] : m[t[0].a];
+		float acc = (a0 * m[t[0].b]) * m[t[0].c];
+		for (int k = 1; k < 6; ++k) {
+			const float p = (m[t[k].a] * m[t[k].b]) * m[t[k].c];
+			acc = t[k].s < 0 ? acc - p : acc + p; }
+		inv[i] = acc; }
+	const float det = ((m[0] * inv[0] + m[1] * inv[4]) + m[2] * inv[8]) + m[3] * inv[12];
+	const float invdet = 1.0F / det;
+	for (int i = 0; i < 16; ++i) inv[i] = inv[i] * invdet; }
+
+// transpose (rmlm_mat4.hxx:211-216)
+void mat4_transpose(const float* m, float* out) {
+	for (int r = 0; r < 4; ++r) for (int c = 0; c < 4; ++c) out[c * 4 + r] = m[r * 4 + c]; }
+
 
 template <class T>
 struct DevArray {
@@ -439,8 +481,11 @@ struct rgl_gpu {
 		const uint32_t vkey = (static_cast<uint32_t>(s.programId) << 24) | 0x80000000u;
 		if (!Installed(vkey)) return fail(RGL_E_DISPATCH, "BIN: no dispatch entry for program");
 		if (!Installed(s.FragmentStateKey())) return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_DRAW_INLINE");
-		const int nv = program_nvary(s.programId);
+		const ProgramInfo info = program_info(s.programId);
+		const int nv = info.nvary;
 		if (nv < 0) return fail(RGL_E_UNSUPPORTED, "program not implemented on the device");
+		if (s.tu3ptr && (s.tu3dim <= 0 || (s.tu3dim & (s.tu3dim - 1)) != 0 || s.tu3dim > 8192))
+			return fail(RGL_E_ARG, "depth texture dimension must be a power of two");
 		if (s.depthFunc != GL_LESS && s.depthFunc != GL_LEQUAL && s.depthFunc != GL_EQUAL)
 			return fail(RGL_E_UNSUPPORTED, "depth func not implemented");
 		if (s.color0AttachmentType != RB_COLOR_DEPTH && s.color0AttachmentType != RB_RGBF32 && s.color0AttachmentType != RB_RGBAF32)
@@ -459,6 +504,8 @@ struct rgl_gpu {
 		// one depth function (LESS or LEQUAL) without blending or scissor
 		if (!s.depthTestEnabled || !s.depthWriteMask || !s.colorWriteMask || s.blendingEnabled || s.scissorEnabled ||
 		    s.depthFunc == GL_EQUAL || (f.depthFunc >= 0 && f.depthFunc != s.depthFunc)) ordered = true;
+		// programs that test depth after shading or discard fragments
+		if (!info.earlyZ || info.discard) ordered = true;
 		if (f.depthFunc < 0 && (s.depthFunc == GL_LESS || s.depthFunc == GL_LEQUAL)) f.depthFunc = s.depthFunc;
 		DrawDesc d;
 		std::memset(&d, 0, sizeof(d));
@@ -485,7 +532,7 @@ struct rgl_gpu {
 			for (int k = 0; k < 4; ++k)
 				if (s.buffers[11 + k] && s.bufferCounts[11 + k] < need[k])
 					return fail(RGL_E_ARG, "heavy-shader table buffer too short"); }
-		if (program_instanced(s.programId)) {
+		if (info.instanced) {
 			if (!s.buffers[15] || s.bufferCounts[15] < 16 * instances) return fail(RGL_E_ARG, "instance matrix buffer too short"); }
 		else if (instances != 1) {
 			return fail(RGL_E_UNSUPPORTED, "instanced draw with a non-instanced program"); }
@@ -496,7 +543,13 @@ struct rgl_gpu {
 		d.primBase = static_cast<uint32_t>(triBase - frameTriStart);   // ids count from the frame's first triangle
 		d.slotBase = static_cast<uint32_t>(1 + kSlotsPerTri * triBase);
 		for (int k = 0; k < 16; ++k) d.buffers[k] = s.buffers[k];
+		// MakeMatrices (rglv_gpu_impl.hxx:44-51): vm, nm = transpose(inverse(vm)), vpm = pm * vm
 		mat4_mul(s.projectionMatrix, s.viewMatrix, d.vpm);
+		std::memcpy(d.vm, s.viewMatrix, sizeof(d.vm));
+		float inv[16];
+		mat4_inverse(s.viewMatrix, inv);
+		mat4_transpose(inv, d.nm);
+		d.tu3 = s.tu3ptr; d.tu3dim = s.tu3dim;
 		// GPU::DSDO (rglv_gpu.hxx:265-270)
 		const int vx = s.hasViewportSize ? s.viewportSize[0] : W;
 		const int vy = s.hasViewportSize ? s.viewportSize[1] : H;
@@ -866,6 +919,29 @@ struct rgl_gpu {
 		++captures;
 		return 0; }
 
+	// per-program launches, generated from the program list
+	struct VertexLaunch {
+		rgl_gpu& g; const std::vector<DrawDesc>& draws; const BatchDesc& b; const ZeroArgs& za; unsigned vblocks; long long vtot;
+		int zeroHere;
+		template <class P> void run() {
+			bool present = false;
+			for (const auto& d : draws) present |= d.program == P::kId;
+			if (!present || vtot <= 0) return;
+			k_vertex<P><<<vblocks, 256, 0, g.stream>>>(g.dDraws.p, b.nDraws, vtot, g.vpos.p, g.vflags.p, g.vvary.p, za, zeroHere);
+			zeroHere = 0; }};
+	struct ClipLaunch {
+		rgl_gpu& g; const std::vector<DrawDesc>& draws; const BatchDesc& b;
+		template <class P> void run() {
+			bool present = false;
+			for (const auto& d : draws) present |= d.program == P::kId;
+			if (!present) return;
+			k_clip<P><<<64, 64, 0, g.stream>>>(g.dDraws.p, b.nDraws, g.clipq.p, g.clipCap, g.cvpos.p, g.cvvary.p, g.fcount.p,
+				g.frect.p, g.clipHist.p, g.dCtr); }};
+	struct ResolveLaunch {
+		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;
+		template <class P> void run() {
+			g.Dispatch(1, k_resolve<ShadeOne<P>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, lite, dd); }};
+
 	// every kernel of one batch, on `stream`; ends with the counters' copy
 	// to pinned host memory
 	int Launch(const BatchDesc& b, const std::vector<DrawDesc>& draws, bool learned, int pid, int items,
@@ -875,19 +951,13 @@ struct rgl_gpu {
 		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, tileCount.p, qCount.p };
 		Mark(0);
 		// vertex and clip stages: one launch per program present in the batch
-		bool present[5] = { false, false, false, false, false };
-		for (const auto& d : draws) present[ProgramSlot(d.program)] = true;
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
 		const unsigned vblocks = static_cast<unsigned>((vtot + 255) / 256);
 		// the first vertex launch also clears the frame's counters
-		int zeroHere = 1;
-		if (vtot > 0) {
-			if (present[0]) { k_vertex<ProgAmy><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
-			if (present[1]) { k_vertex<ProgMany><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
-			if (present[2]) { k_vertex<ProgLambertTex><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
-			if (present[3]) { k_vertex<ProgManyBlinnPhong><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }
-			if (present[4]) { k_vertex<ProgGGXEnv><<<vblocks, 256, 0, stream>>>(dDraws.p, b.nDraws, vtot, vpos.p, vflags.p, vvary.p, za, zeroHere); zeroHere = 0; }}
+		VertexLaunch vl{ *this, draws, b, za, vblocks, vtot, 1 };
+		for_each_program(vl, AllPrograms{});
+		int zeroHere = vl.zeroHere;
 		if (zeroHere) k_zero_counters<<<std::max(8, std::min((kCountGroups * b.nTiles + 255) / 256, 256)), 256, 0, stream>>>(za);
 		Mark(1);
 		if (b.units > 0) {
@@ -896,11 +966,8 @@ struct rgl_gpu {
 		Mark(2);
 		// clipping: the clip queue is complete after setup; the clipped
 		// per-tile counts feed the scan below
-		if (present[0]) k_clip<ProgAmy><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
-		if (present[1]) k_clip<ProgMany><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
-		if (present[2]) k_clip<ProgLambertTex><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
-		if (present[3]) k_clip<ProgManyBlinnPhong><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
-		if (present[4]) k_clip<ProgGGXEnv><<<64, 64, 0, stream>>>(dDraws.p, b.nDraws, clipq.p, clipCap, cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, dCtr);
+		ClipLaunch cl{ *this, draws, b };
+		for_each_program(cl, AllPrograms{});
 		Mark(3);
 		if (b.nTiles > kScanSplitTiles) {
 			const int nb = (b.nTiles + kScanThreads - 1) / kScanThreads;
@@ -948,13 +1015,11 @@ struct rgl_gpu {
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
 			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
 		else {
-			switch (pid) {
-			case ProgAmy::kId: Dispatch(1, k_resolve<ShadeOne<ProgAmy>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
-			case ProgMany::kId: Dispatch(1, k_resolve<ShadeOne<ProgMany>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
-			case ProgLambertTex::kId: Dispatch(1, k_resolve<ShadeOne<ProgLambertTex>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
-			case ProgManyBlinnPhong::kId: Dispatch(1, k_resolve<ShadeOne<ProgManyBlinnPhong>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
-			case ProgGGXEnv::kId: Dispatch(1, k_resolve<ShadeOne<ProgGGXEnv>>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break;
-			default: Dispatch(1, k_resolve<ShadeAny>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); break; }}
+			// a batch of one program resolves with it compiled in, others
+			// dispatch over the program list per pixel
+			ResolveLaunch rl{ *this, b, ra, rblocks, cLite, cDraws };
+			if (pid < 0 || !dispatch_program(pid, rl, AllPrograms{}))
+				Dispatch(1, k_resolve<ShadeAny>, dim3(rblocks), dim3(256), 0, b, ra, b.frames, cLite, cDraws); }
 		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);
@@ -977,14 +1042,6 @@ struct rgl_gpu {
 	static constexpr int kClipScatterBlocks = 32;   // k_scatter blocks that place the clipped fans
 	static constexpr size_t kUnitLdsBytes = 160 * 1024;   // LDS per workgroup on gfx950
 	static constexpr int kResetBlocks = 128;        // k_scatter blocks that reset the keys of multi-segment tiles
-
-	static int ProgramSlot(int pid) {
-		switch (pid) {
-		case ProgAmy::kId: return 0;
-		case ProgMany::kId: return 1;
-		case ProgLambertTex::kId: return 2;
-		case ProgGGXEnv::kId: return 4;
-		default: return 3; }}
 
 	// ---- learned shading: one forward + backward pass over a batch ----
 	static void Gemm(hipStream_t ts, const float* A, const float* B, float* C, int M, int N, int K, int lda, int ldb, int ldc,
@@ -1215,6 +1272,15 @@ int rgl_gl_bind_texture(rgl_gpu* g, int unit, const float* ptr, int w, int h, in
 	if (mode != 0 && mode != 1) return g->fail(RGL_E_ARG, "texture filter must be GL_NEAREST/LINEAR_MIPMAP_NEAREST");
 	g->dirty = true;
 	g->cs.tus[unit] = { ptr, w, h, stride, mode };
+	return 0; }
+int rgl_gl_bind_texture3(rgl_gpu* g, const float* ptr, int dim) {
+	if (check(g)) return RGL_E_ARG;
+	// the unit addresses (row * dim + column) & (dim^2 - 1), which wraps
+	// around the map for a power-of-two dim
+	if (ptr && (dim <= 0 || (dim & (dim - 1)) != 0 || dim > 8192))
+		return g->fail(RGL_E_ARG, "depth texture dimension must be a power of two <= 8192");
+	g->dirty = true;
+	g->cs.tu3ptr = ptr; g->cs.tu3dim = dim;
 	return 0; }
 int rgl_gl_use_buffer(rgl_gpu* g, int slot, const float* ptr, int count) {
 	if (check(g)) return RGL_E_ARG;

@@ -130,13 +130,17 @@ __global__ __launch_bounds__(256) void k_tile_order(BatchDesc b, const IoFrame* 
 		unsigned long long* x = src; src = dst; dst = x; }
 	for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(src[i]); }
 
-// ShadeFragment of program P for a quad: perspective-correct BP from BS
-// and the vertices' 1/w (Render, rglv_gpu_impl.hxx:187-191), varyings,
-// the quad's texture LOD from lanes 0 and 1; colour with alpha per lane
+// ShadeFragment of program P for a quad at (qx, qy): perspective-correct BP
+// from BS and the vertices' 1/w (Render, rglv_gpu_impl.hxx:187-191),
+// varyings, gl_FragCoord and gl_FragDepth, the quad's texture LOD from
+// lanes 0 and 1; colour with alpha per lane, and the lanes the program
+// keeps (discard clears them)
 template <class P>
-__device__ inline void shade_quad(const DrawDesc& d, const float w[3], const float* v0, const float* v1, const float* v2,
-                                  const float bx[4], const float by[4], const float bz[4], float col[4][4]) {
+__device__ inline uint32_t shade_quad(const DrawDesc& d, const float w[3], const float* v0, const float* v1, const float* v2,
+                                      const float bx[4], const float by[4], const float bz[4], const float fd[4], int qx, int qy,
+                                      int h, bool clipped, float col[4][4]) {
 	float vv[4][P::kVary];
+	FragCoord fc[4];
 #pragma unroll
 	for (int l = 0; l < 4; ++l) {
 		const float fragW = oneover((bx[l] * w[0] + by[l] * w[1]) + bz[l] * w[2]);
@@ -144,20 +148,24 @@ __device__ inline void shade_quad(const DrawDesc& d, const float w[3], const flo
 		const float p2 = (w[2] * bz[l]) * fragW;
 		const float p1 = (1.0F - p0) - p2;
 #pragma unroll
-		for (int k = 0; k < P::kVary; ++k) vv[l][k] = (p0 * v0[k] + p1 * v1[k]) + p2 * v2[k]; }
+		for (int k = 0; k < P::kVary; ++k) vv[l][k] = (p0 * v0[k] + p1 * v1[k]) + p2 * v2[k];
+		fc[l] = frag_coord(qx, qy, l, h, clipped, fd[l]); }
 	int lod = 0;
-	if constexpr (P::kUV >= 0) lod = texture_lod(d.tu0, vv[0][P::kUV], vv[1][P::kUV], vv[0][P::kUV + 1], vv[1][P::kUV + 1]);
+	float u0, w0, u1, w1;
+	if (lod_texcoord<P>(d, vv[0], fc[0], u0, w0) && lod_texcoord<P>(d, vv[1], fc[1], u1, w1))
+		lod = texture_lod(d.tu0, u0, u1, w0, w1);
+	uint32_t keep = 0;
 #pragma unroll
-	for (int l = 0; l < 4; ++l) P::fragment1(d, vv[l], lod, col[l]); }
+	for (int l = 0; l < 4; ++l) keep |= (P::fragment(d, vv[l], lod, fc[l], col[l]) ? 1u : 0u) << l;
+	return keep; }
 
-__device__ inline void shade_quad_any(const DrawDesc& d, const float w[3], const float* v0, const float* v1, const float* v2,
-                                      const float bx[4], const float by[4], const float bz[4], float col[4][4]) {
-	switch (d.program) {
-	case ProgAmy::kId: shade_quad<ProgAmy>(d, w, v0, v1, v2, bx, by, bz, col); break;
-	case ProgMany::kId: shade_quad<ProgMany>(d, w, v0, v1, v2, bx, by, bz, col); break;
-	case ProgLambertTex::kId: shade_quad<ProgLambertTex>(d, w, v0, v1, v2, bx, by, bz, col); break;
-	case ProgGGXEnv::kId: shade_quad<ProgGGXEnv>(d, w, v0, v1, v2, bx, by, bz, col); break;
-	default: shade_quad<ProgManyBlinnPhong>(d, w, v0, v1, v2, bx, by, bz, col); break; }}
+struct ShadeQuad {
+	const DrawDesc& d; const float* w; const float* v0; const float* v1; const float* v2;
+	const float* bx; const float* by; const float* bz; const float* fd; int qx, qy, h; bool clipped;
+	float (*col)[4]; uint32_t keep; bool earlyZ;
+	template <class P> __device__ void run() {
+		keep = shade_quad<P>(d, w, v0, v1, v2, bx, by, bz, fd, qx, qy, h, clipped, col);
+		earlyZ = P::kEarlyZ; }};
 
 // depth functions of the fragment state (DepthLT / DepthLE / DepthEQ,
 // rglv_gpu_impl.hxx:64-74)
@@ -270,14 +278,15 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 				bz[l] = itof(e1[l]) * r.scale;
 				by[l] = (1.0F - bx[l]) - bz[l];
 				fd[l] = (bx[l] * r.z0 + by[l] * r.z1) + bz[l] * r.z2; }
-			// Render: early-Z, depth write, shade, blend, masked store (every
-			// program here is early-Z)
+			// Render: early or late depth test and depth write around the
+			// program, then blend and the masked colour store
+			const bool earlyZ = program_info(dd.program).earlyZ;
 			uint32_t mask = live;
-			if (dd.depthTest) {
+			if (dd.depthTest && earlyZ) {
 #pragma unroll
 				for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
 				if (!mask) continue; }
-			if (dd.depthWrite) {
+			if (dd.depthWrite && earlyZ) {
 #pragma unroll
 				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
 			// the winning triangle's vertices: 1/w and varyings
@@ -298,8 +307,17 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 				clipped_verts(cq * kFanMax + jf, a.cvpos, tr);
 				vary = a.cvvary; }
 			float col[4][4];
-			shade_quad_any(dd, tr.w, vary + tr.vary[0] * kMaxVary, vary + tr.vary[1] * kMaxVary, vary + tr.vary[2] * kMaxVary,
-			               bx, by, bz, col);
+			ShadeQuad sq{ dd, tr.w, vary + tr.vary[0] * kMaxVary, vary + tr.vary[1] * kMaxVary, vary + tr.vary[2] * kMaxVary,
+			              bx, by, bz, fd, qx, qy, f.H, tr.clipped, col, 0xfu, true };
+			dispatch_program(dd.program, sq, AllPrograms{});
+			mask &= sq.keep;   // gl_triMask after ShadeFragment
+			if (dd.depthTest && !earlyZ) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+				if (!mask) continue; }
+			if (dd.depthWrite && !earlyZ) {
+#pragma unroll
+				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
 			if (dd.colorWrite) {
 #pragma unroll
 				for (int l = 0; l < 4; ++l) {

@@ -15,7 +15,7 @@
 // have (same tile-clamped edge start values, same barycentric scale).
 #pragma once
 
-#include "rgl_device.h"
+#include "rgl_programs.h"
 
 namespace rgl {
 
@@ -1048,10 +1048,24 @@ __device__ inline LaneBary lane_bary(const TriSetup& t, const TriVerts& tr, int3
 __device__ inline float lane_interp(const LaneBary& b, const float* v0, const float* v1, const float* v2, int k) {
 	return (b.bp0*v0[k] + b.bp1*v1[k]) + b.bp2*v2[k]; }
 
-// shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle
+// gl_FragCoord of lane l of the quad at (qx, qy) in a target h pixels
+// high: the vector rasteriser's form (rglv_triangle.hxx:297) for inline
+// triangles, the scalar one's (:155) for clipped ones (its y is one higher
+// on the quad's top row)
+__device__ inline FragCoord frag_coord(int qx, int qy, int l, int h, bool clipped, float depth) {
+	FragCoord fc;
+	fc.x = (static_cast<float>(qx) + 0.5F) + ((l & 1) ? 1.0F : 0.0F);
+	if (clipped) fc.y = (static_cast<float>(h - qy) - 0.5F) + ((l >> 1) ? 0.0F : 1.0F);
+	else fc.y = ((static_cast<float>(h) - 0.5F) - static_cast<float>(qy)) - ((l >> 1) ? 1.0F : 0.0F);
+	fc.depth = depth;
+	return fc; }
+
+// shade one pixel (lane `lane` of quad (kx, ky)) of a winning triangle;
+// h = the target's height (gl_FragCoord)
 template <class P>
 __device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const float* v0, const float* v1,
-                                   const float* v2, int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
+                                   const float* v2, int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth,
+                                   int h = 0) {
 	const int32_t q1 = wadd(wadd(t.c1, wmul(kx, wmul(t.dy12, 2))), wmul(ky, wmul(t.dx12, 2)));
 	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
 	// lane offsets of the rasteriser's quad: (c, c+dy, c+dx, c+dx+dy) (rglv_triangle.hxx:253-258)
@@ -1062,15 +1076,22 @@ __device__ inline void shade_pixel(const DrawDesc& d, const TriSetup& t, const T
 	float vv[P::kVary];
 #pragma unroll
 	for (int k = 0; k < P::kVary; ++k) vv[k] = lane_interp(own, v0, v1, v2, k);
+	const int qx = t.minx + 2 * kx, qy = t.miny + 2 * ky;
+	const FragCoord fc = frag_coord(qx, qy, lane, h, tr.clipped, own.depth);
 	int lod = 0;
-	if constexpr (P::kUV >= 0) {
+	if constexpr (P::kUV >= 0 || P::kUV == kUVFromFragCoord) {
 		// the quad's LOD comes from the texture coordinates of lanes 0 and 1
 		const LaneBary l0 = lane_bary(t, tr, q1, q2, rcpTab);
 		const LaneBary l1 = lane_bary(t, tr, wadd(q1, t.dy12), wadd(q2, t.dy23), rcpTab);
-		lod = texture_lod(d.tu0, lane_interp(l0, v0, v1, v2, P::kUV), lane_interp(l1, v0, v1, v2, P::kUV),
-		                  lane_interp(l0, v0, v1, v2, P::kUV + 1), lane_interp(l1, v0, v1, v2, P::kUV + 1)); }
+		float a0[P::kVary], a1[P::kVary];
+#pragma unroll
+		for (int k = 0; k < P::kVary; ++k) { a0[k] = lane_interp(l0, v0, v1, v2, k); a1[k] = lane_interp(l1, v0, v1, v2, k); }
+		float u0, w0, u1, w1;
+		lod_texcoord<P>(d, a0, frag_coord(qx, qy, 0, h, tr.clipped, l0.depth), u0, w0);
+		lod_texcoord<P>(d, a1, frag_coord(qx, qy, 1, h, tr.clipped, l1.depth), u1, w1);
+		lod = texture_lod(d.tu0, u0, u1, w0, w1); }
 	float c4[4];
-	P::fragment1(d, vv, lod, c4);
+	(void)P::fragment(d, vv, lod, fc, c4);
 	color[0] = c4[0]; color[1] = c4[1]; color[2] = c4[2]; }
 
 // resolve shaders: one program known at compile time, or a switch over
@@ -1087,22 +1108,23 @@ struct ShadeOne {
 #pragma unroll
 			for (int k = 0; k < P::kVary; ++k) out.v[i][k] = src[k]; }}
 	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const Vary& vy,
-	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-		shade_pixel<P>(d, t, tr, vy.v[0], vy.v[1], vy.v[2], kx, ky, lane, rcpTab, color, depth); }};
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth, int h) {
+		shade_pixel<P>(d, t, tr, vy.v[0], vy.v[1], vy.v[2], kx, ky, lane, rcpTab, color, depth, h); }};
 
+// the program of the draw, chosen at run time over the program list
 struct ShadeAny {
 	struct Vary { const float* v[3]; };
 	__device__ static void load(const float* __restrict__ vary, const TriVerts& tr, Vary& out) {
 		for (int i = 0; i < 3; ++i) out.v[i] = vary + tr.vary[i] * kMaxVary; }
+	struct Call {
+		const DrawDesc& d; const TriSetup& t; const TriVerts& tr; const Vary& vy;
+		int kx, ky, lane; const uint32_t* rcpTab; float* color; float& depth; int h;
+		template <class P> __device__ void run() {
+			shade_pixel<P>(d, t, tr, vy.v[0], vy.v[1], vy.v[2], kx, ky, lane, rcpTab, color, depth, h); }};
 	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const Vary& vy,
-	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth) {
-		const float *v0 = vy.v[0], *v1 = vy.v[1], *v2 = vy.v[2];
-		switch (d.program) {
-		case ProgAmy::kId: shade_pixel<ProgAmy>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgMany::kId: shade_pixel<ProgMany>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgLambertTex::kId: shade_pixel<ProgLambertTex>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
-		case ProgGGXEnv::kId: shade_pixel<ProgGGXEnv>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break;
-		default: shade_pixel<ProgManyBlinnPhong>(d, t, tr, v0, v1, v2, kx, ky, lane, rcpTab, color, depth); break; }}};
+	                             int kx, int ky, int lane, const uint32_t* rcpTab, float color[3], float& depth, int h) {
+		Call c{ d, t, tr, vy, kx, ky, lane, rcpTab, color, depth, h };
+		dispatch_program(d.program, c, AllPrograms{}); }};
 
 // ---------------------------------------------------------------------------
 // resolve: one thread per pixel.  The pixel's winning slot is shaded by
@@ -1147,7 +1169,7 @@ __device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, con
 		setup_scalar(R, tr.X, tr.Y, t); }
 	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
 	const int lane = (x & 1) | ((y & 1) << 1);
-	S::shade(dd, t, tr, vy, kx, ky, lane, rcpTab, col, dep);
+	S::shade(dd, t, tr, vy, kx, ky, lane, rcpTab, col, dep, f.H);
 	prim = d.primBase + srcTri; }
 
 // the winner of pixel (x, y), shaded (clear values where nothing was

@@ -64,6 +64,7 @@ def lib():
         L.rgl_gl_renderbuffer_type.argtypes = [vp, i, i]
         L.rgl_gl_bind_texture.argtypes = [vp, i, vp, i, i, i, i]
         L.rgl_gl_use_buffer.argtypes = [vp, i, vp, i]
+        L.rgl_gl_bind_texture3.argtypes = [vp, vp, i]
         L.rgl_gl_clear_color.argtypes = [vp, f, f, f]
         L.rgl_gl_clear_depth.argtypes = [vp, f]
         for n in ("view_matrix", "projection_matrix", "normal_matrix"):
@@ -101,7 +102,7 @@ EXPORTS = [
     "rgl_gl_enable", "rgl_gl_disable", "rgl_gl_depth_func", "rgl_gl_depth_write_mask",
     "rgl_gl_color_write_mask", "rgl_gl_cull_face", "rgl_gl_scissor", "rgl_gl_viewport",
     "rgl_gl_use_program", "rgl_gl_use_uniforms", "rgl_gl_alloc_uniform_buffer",
-    "rgl_gl_uniform_buffer_addr", "rgl_gl_renderbuffer_type", "rgl_gl_bind_texture",
+    "rgl_gl_uniform_buffer_addr", "rgl_gl_renderbuffer_type", "rgl_gl_bind_texture", "rgl_gl_bind_texture3",
     "rgl_gl_use_buffer", "rgl_gl_clear_color", "rgl_gl_clear_depth", "rgl_gl_view_matrix",
     "rgl_gl_projection_matrix", "rgl_gl_normal_matrix", "rgl_gl_draw_elements",
     "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",

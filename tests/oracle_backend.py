@@ -54,6 +54,7 @@ def lib():
         L.rglo_gl_renderbuffer_type.argtypes = [vp, i, i]
         L.rglo_gl_bind_texture.argtypes = [vp, i, vp, i, i, i, i]
         L.rglo_gl_use_buffer.argtypes = [vp, i, vp]
+        L.rglo_gl_bind_texture3.argtypes = [vp, vp, i]
         L.rglo_gl_clear_color.argtypes = [vp, f, f, f]
         L.rglo_gl_clear_depth.argtypes = [vp, f]
         for name in ("view_matrix", "projection_matrix", "normal_matrix"):
@@ -70,6 +71,8 @@ def lib():
         L.rglo_oneover.restype = f
         L.rglo_rcpps.argtypes = [f]
         L.rglo_rcpps.restype = f
+        L.rglo_rsqrtps.argtypes = [f]
+        L.rglo_rsqrtps.restype = f
         L.rglo_cvtt.argtypes = [f]
         L.rglo_cvtt.restype = ctypes.c_int32
         L.rglo_to_srgb8.argtypes = [f]

@@ -16,6 +16,7 @@ than a restatement of its own formula:
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 import re
 import struct
@@ -92,6 +93,47 @@ def test_rcpps_closed_forms():
     assert L.rglo_rcpps(u2f(0x00000001)) == float("inf")     # denormal input
     assert L.rglo_rcpps(float("inf")) == 0.0
     assert np.isnan(L.rglo_rcpps(float("nan")))
+
+
+# ---------------------------------------------------------------------------
+# rsqrtps (rmlv_soa.hxx:243-248 normalize; hardware reciprocal square root
+# estimate)
+# ---------------------------------------------------------------------------
+def test_rsqrtps_tables_agree():
+    fixture = np.fromfile(os.path.join(GOLDEN, "rsqrtps_table.bin"), dtype="<u4").astype(np.uint64)
+    assert fixture.size == 4096
+    # rsqrtps of [1, 4) lies in (0.5, 1): every entry has biased exponent 126
+    assert np.all(fixture >> 23 == 126)
+    for path in (os.path.join(ORACLE_DIR, "rsqrtps_table.inc"), os.path.join(ROOT, "rsr_amd", "csrc", "rsqrtps_table.inc")):
+        assert np.array_equal(read_inc_table(path), fixture), path
+
+
+@pytest.mark.skipif(not _host_is_intel(), reason="the table is the Intel rsqrtps; other vendors' estimates differ")
+def test_rsqrtps_model_matches_host_hardware():
+    exe = os.path.join(ORACLE_DIR, "gen_rsqrtps_table")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    # every 97th float bit pattern through the model vs _mm_rsqrt_ps
+    r = subprocess.run([exe, "check", os.path.join(GOLDEN, "rsqrtps_table.bin"), "97"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_rsqrtps_closed_forms():
+    L = olib()
+    rng = np.random.default_rng(9)
+    for x in rng.uniform(1.0, 4.0, 3000).astype(np.float32):
+        r = L.rglo_rsqrtps(float(x))
+        assert abs(r * math.sqrt(float(x)) - 1.0) <= 1.5 * 2.0 ** -12   # the documented |rel err| bound
+        for k in (-30, -2, 3, 25):                                      # x * 4^k -> r * 2^-k exactly
+            assert L.rglo_rsqrtps(float(x) * 4.0 ** k) == r * 2.0 ** -k
+    assert L.rglo_rsqrtps(1.0) == L.rglo_rsqrtps(4.0) * 2.0
+    assert L.rglo_rsqrtps(0.0) == float("inf")
+    assert L.rglo_rsqrtps(-0.0) == float("-inf")
+    assert L.rglo_rsqrtps(u2f(0x00000001)) == float("inf")          # denormal input
+    assert L.rglo_rsqrtps(float("inf")) == 0.0
+    assert np.isnan(L.rglo_rsqrtps(-1.0)) and np.isnan(L.rglo_rsqrtps(float("-inf")))
+    assert np.isnan(L.rglo_rsqrtps(float("nan")))
 
 
 def test_oneover_newton_step():
