@@ -22,6 +22,7 @@
  */
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -569,6 +570,16 @@ struct GGXEnvProgram final : Program {
 			ggx_shade(e, s, c);
 			color[0][l] = c[0]; color[1][l] = c[1]; color[2][l] = c[2]; color[3][l] = 1.0F; }}};
 
+// rglv::BaseProgram (rglv_gpu_shaders.hxx:21-95): position only, colour 1
+struct BaseProgram final : Program {
+	BaseProgram() { id = 0; nvary = 0; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut&) const override {
+		float p[4]; load_pos(e, idx, p);
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv&, const float[3][4], const float[3][4], const float[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		for (int c = 0; c < 4; ++c) for (int l = 0; l < 4; ++l) color[c][l] = 1.0F; }};
+
 // ---- the viewer's other programs (viewer/shaders.hxx) ----
 inline float dot4(const float a[4], const float b[4]) { return ((a[0]*b[0] + a[1]*b[1]) + a[2]*b[2]) + a[3]*b[3]; }
 
@@ -737,20 +748,125 @@ struct OBJ2SProgram final : Program {
 			color[2][l] = (att[l] * vary[10][l]) * diff[l];
 			color[3][l] = 1.0F; }}};
 
+// EnvmapProgram (viewer/shaders_envmap.hxx:26-127): uv of the sphere map
+// at reflect(e, n), e the unit view-space position, n the unit normal
+// matrix image of mix(smooth, face, 0)
+struct EnvmapProgram final : Program {
+	EnvmapProgram() { id = 10; nvary = 2; }
+	static void normalize3r(float v[3]) {
+		const float sc = rsqrtps((v[0]*v[0] + v[1]*v[1]) + v[2]*v[2]);
+		v[0] = v[0] * sc; v[1] = v[1] * sc; v[2] = v[2] * sc; }
+	void ShadeVertex(const DrawEnv& e, int, int idx, float pos[4], VOut& o) const override {
+		float p[4]; load_pos(e, idx, p);
+		float smooth[3]; obj_normal(e, idx, smooth);
+		const auto& b = e.st->buffers;
+		float face[3] = { 1.0F, 1.0F, 1.0F };
+		if (b[6]) { face[0] = b[6][idx]; face[1] = b[7][idx]; face[2] = b[8][idx]; }
+		float position[4];
+		xform4(e.vm.ff, p, position);
+		float ev[3] = { position[0], position[1], position[2] };
+		normalize3r(ev);
+		float vn[3];
+		for (int k = 0; k < 3; ++k) { const float diff = face[k] - smooth[k]; vn[k] = smooth[k] + 0.0F * diff; }
+		const float* nm = e.nm.ff;
+		float n[3] = { (nm[0]*vn[0] + nm[4]*vn[1]) + nm[8]*vn[2],
+		               (nm[1]*vn[0] + nm[5]*vn[1]) + nm[9]*vn[2],
+		               (nm[2]*vn[0] + nm[6]*vn[1]) + nm[10]*vn[2] };
+		normalize3r(n);
+		const float d = (n[0]*ev[0] + n[1]*ev[1]) + n[2]*ev[2];
+		const float twod = 2.0F * d;
+		float r[3];
+		for (int k = 0; k < 3; ++k) r[k] = ev[k] - twod * n[k];
+		const float zp = r[2] + 1.0F;
+		const float m = 2.0F * std::sqrt(((r[0]*r[0]) + (r[1]*r[1])) + (zp*zp));
+		o.f[0] = r[0] / m + 0.5F;
+		o.f[1] = r[1] / m + 0.5F;
+		xform4(e.vpm.ff, p, pos); }
+	void ShadeFragment(const DrawEnv& e, const float[3][4], const float[3][4], const float vary[kMaxVary][4], const float[2][4], const float[4],
+	                   float color[4][4], int32_t[4]) const override {
+		e.tu0.sample(vary[0], vary[1], color);
+		for (int l = 0; l < 4; ++l) color[3][l] = 0.5F; }};
+
 const Program* find_program(int id) {
 	static const AmyProgram amy; static const ManyProgram many;
 	static const LambertTexProgram lam; static const ManyBlinnPhongProgram mbp;
 	static const GGXEnvProgram ggx;
 	static const AlphaTextureProgram alpha; static const TextProgram text; static const DepthProgram depth;
 	static const PatternProgram pattern; static const OBJ1Program obj1; static const OBJ2Program obj2; static const OBJ2SProgram obj2s;
+	static const EnvmapProgram envmap; static const BaseProgram base;
 	switch (id) {
-	case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp; case 82: return &ggx;
+	case 0: return &base; case 4: return &amy; case 6: return &many; case 80: return &lam; case 81: return &mbp; case 82: return &ggx;
 	case 65: return &alpha; case 26: return &text; case 5: return &depth; case 41: return &pattern;
-	case 7: return &obj1; case 8: return &obj2; case 9: return &obj2s;
+	case 7: return &obj1; case 8: return &obj2; case 9: return &obj2s; case 10: return &envmap;
 	default: return nullptr; }}
 
-// post programs for StoreTrueColor (ShadeCanvas); 1 = Default, 2 = Exposure
-bool known_post_program(int id) { return id == 1 || id == 2; }
+// post programs for StoreTrueColor (ShadeCanvas); 1 = Default, 2 = Exposure,
+// 3 = IQ (viewer/shaders.hxx:32-68)
+bool known_post_program(int id) { return id == 1 || id == 2 || id == 3; }
+
+// _mm_cvtps_epi32 under the default rounding mode (nearest, ties to even)
+inline int32_t cvtn(float f) {
+	if (f >= -2147483648.0F && f < 2147483648.0F) return static_cast<int32_t>(std::nearbyint(f));
+	return INT32_MIN; }
+
+// J. R. Fonseca's SSE2 exp2 / log2 (3rdparty/sse-pow/sse_pow.h:19-87, as
+// the reference configures it: EXP_POLY_DEGREE 3, LOG_POLY_DEGREE 5),
+// one lane; the polynomial coefficients are that method's minimax fits
+inline float sse_exp2(float x) {
+	x = sse_min(x, 129.00000F);
+	x = sse_max(x, -126.99999F);
+	const int32_t ipart = cvtn(x - 0.5F);
+	const float fpart = x - itof(ipart);
+	const float expipart = u2f(static_cast<uint32_t>(shl(wadd(ipart, 127), 23)));
+	float p = 7.8024521e-2F;
+	p = p * fpart + 2.2606716e-1F;
+	p = p * fpart + 6.9583356e-1F;
+	p = p * fpart + 9.9992520e-1F;
+	return expipart * p; }
+
+inline float sse_log2(float x) {
+	const uint32_t i = f2u(x);
+	const float e = itof(static_cast<int32_t>((i & 0x7F800000u) >> 23) - 127);
+	const float m = u2f((i & 0x007FFFFFu) | f2u(1.0F));
+	float p = static_cast<float>(0.0596515482674574969533);
+	p = p * m + static_cast<float>(-0.465725644288844778798);
+	p = p * m + static_cast<float>(1.48116647521213171641);
+	p = p * m + static_cast<float>(-2.52074962577807006663);
+	p = p * m + static_cast<float>(2.8882704548164776201);
+	p = p * (m - 1.0F);
+	return p + e; }
+
+// rmlv pow (rmlv_mvec4.hxx:652-654)
+inline float sse_pow(float x, float y) { return sse_exp2(sse_log2(x) * y); }
+
+// rmlv sin (rmlv_mvec4.hxx:700-760): scale by 1/pi, wrap to [-1, 1] with
+// the sign flipped for odd periods, then the parabola y (3.1 + 3.6 |y|)
+inline float approx_sin(float x) {
+	float M = x * static_cast<float>(1.0 / 3.14159265358979323846);
+	const int32_t whole = cvtt(M);
+	M = M - itof(whole);
+	M = u2f(f2u(M) ^ static_cast<uint32_t>(shl(whole, 31)));
+	const float y = M - (M * std::fabs(M));
+	return y * (3.1F + 3.6F * std::fabs(y)); }
+
+// rmlv fract (rmlv_mvec4.hxx:576-580)
+inline float fract_t(float a) { return a - itof(cvtt(a)); }
+
+// IQPostProgram::ShadeCanvas for one pixel at q (viewer/shaders.hxx:56-67;
+// rglv_fragment.hxx:34-36 hash3)
+inline void iq_post(float c[3], float qx, float qy) {
+	c[0] = sse_pow(c[0], 0.45F); c[1] = sse_pow(c[1], 0.5F); c[2] = sse_pow(c[2], 0.55F);
+	float t = 16.0F * qx;
+	t = t * qy;
+	t = t * (1.0F - qx);
+	t = t * (1.0F - qy);
+	const float v = 0.2F + 0.8F * sse_pow(t, 0.2F);
+	const float n = qx + 13.0F * qy;
+	const float k = static_cast<float>(1.0 / 255.0);
+	for (int ch = 0; ch < 3; ++ch) {
+		c[ch] = c[ch] * v;
+		const float h = fract_t(approx_sin(n + static_cast<float>(ch)) * 43758.5453123F);
+		c[ch] = c[ch] + k * h; }}
 
 // ------------------------------------------------------------------------
 // per-tile command lists
@@ -1330,15 +1446,33 @@ struct rglo_gpu {
 				const float* u = Uniforms(*st);
 				const float exposure = (st->programId == 2 && u) ? u[0] : 1.0F;
 				auto* out = static_cast<uint32_t*>(const_cast<void*>(op.ptr));
-				for (int y = tr.y0; y < tr.y1; ++y) {
-					for (int x = tr.x0; x < tr.x1; ++x) {
-						const int p = (y - tr.y0) * tileW + (x - tr.x0);
-						float r = tb.r[p], g = tb.g[p], b = tb.b[p];
-						if (st->programId == 2) { r = r * exposure; g = g * exposure; b = b * exposure; }
-						uint32_t px;
-						if (op.a) px = (to_srgb8(r) << 16) | (to_srgb8(g) << 8) | to_srgb8(b);
-						else px = (to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b);
-						out[static_cast<size_t>(y) * W + x] = px; }}
+				// the canvas coordinate q as FilterTile steps it: per 2 x 2
+				// quad, fcx += 2 / W along the row, fcy += -2 / H per row pair,
+				// both restarting at the tile's corner
+				const float iw = 1.0F / static_cast<float>(W), ih = 1.0F / static_cast<float>(H);
+				const float fcdx = iw * 2.0F, fcdy = -ih * 2.0F;
+				const float fcxLeft = (static_cast<float>(tr.x0) + 0.5F) / static_cast<float>(W);
+				float fcy[4];
+				for (int l = 0; l < 4; ++l) {
+					fcy[l] = (static_cast<float>(H - tr.y0) - 0.5F) / static_cast<float>(H);
+					if (l >= 2) fcy[l] = fcy[l] - ih; }
+				for (int y = tr.y0; y < tr.y1; y += 2) {
+					float fcx[4];
+					for (int l = 0; l < 4; ++l) fcx[l] = (l & 1) ? fcxLeft + iw : fcxLeft;
+					for (int x = tr.x0; x < tr.x1; x += 2) {
+						for (int l = 0; l < 4; ++l) {
+							const int px_ = x + (l & 1), py_ = y + (l >> 1);
+							if (px_ >= tr.x1 || py_ >= tr.y1) continue;
+							const int p = (py_ - tr.y0) * tileW + (px_ - tr.x0);
+							float c[3] = { tb.r[p], tb.g[p], tb.b[p] };
+							if (st->programId == 2) { c[0] = c[0] * exposure; c[1] = c[1] * exposure; c[2] = c[2] * exposure; }
+							if (st->programId == 3) iq_post(c, fcx[l], fcy[l]);
+							uint32_t px;
+							if (op.a) px = (to_srgb8(c[0]) << 16) | (to_srgb8(c[1]) << 8) | to_srgb8(c[2]);
+							else px = (to_linear8(c[0]) << 16) | (to_linear8(c[1]) << 8) | to_linear8(c[2]);
+							out[static_cast<size_t>(py_) * W + px_] = px; }
+						for (int l = 0; l < 4; ++l) fcx[l] = fcx[l] + fcdx; }
+					for (int l = 0; l < 4; ++l) fcy[l] = fcy[l] + fcdy; }
 				break; }
 			case TileOp::STORE_DEPTH: {
 				auto* out = static_cast<float*>(const_cast<void*>(op.ptr));
@@ -1354,9 +1488,14 @@ struct rglo_gpu {
 				break; }}}
 		return true; }
 
+	double lastBinSec = 0.0, lastDrawSec = 0.0;  // the last Run's serial bin and parallel draw
+
 	bool Run() {
 		err.clear();
+		const auto t0 = std::chrono::steady_clock::now();
 		if (!Bin()) return false;
+		const auto t1 = std::chrono::steady_clock::now();
+		lastBinSec = std::chrono::duration<double>(t1 - t0).count();
 		tileQuads.assign(tiles.size(), 0);
 		tilePairs.assign(tiles.size(), 0);
 		// cost-sorted tile order (GPU::RunImpl, rglv_gpu.cxx:100-104)
@@ -1380,6 +1519,7 @@ struct rglo_gpu {
 			std::vector<std::thread> pool;
 			for (int i = 0; i < nt; ++i) pool.emplace_back(worker, i);
 			for (auto& th : pool) th.join(); }
+		lastDrawSec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
 		statFragments += frags.load();
 		if (!ok) { for (auto& e : errs) if (!e.empty()) err = e; return false; }
 		return true; }
@@ -1418,6 +1558,9 @@ int rglo_gpu_reset(rglo_gpu* g, int w, int h, int tbx, int tby) {
 	if (w <= 0 || h <= 0 || tbx <= 0 || tby <= 0 || tbx > 16 || tby > 16) { g->err = "bad size"; return -1; }
 	g->SetSize(w, h, tbx, tby); g->ResetGL(); return 0; }
 int rglo_gpu_run(rglo_gpu* g) { return g->Run() ? 0 : -1; }
+// the last run's wall time: out[0] bin (one thread), out[1] draw + stores
+void rglo_gpu_times(rglo_gpu* g, double out[2]) { out[0] = g->lastBinSec; out[1] = g->lastDrawSec; }
+
 void rglo_gpu_stats(rglo_gpu* g, long long out[5]) {
 	out[0] = g->statSubmitted; out[1] = g->statClipped; out[2] = g->statBinned; out[3] = g->statFragments; out[4] = g->statClippedPairs;
 	g->statSubmitted = g->statClipped = g->statBinned = g->statFragments = g->statClippedPairs = 0; }
@@ -1616,6 +1759,15 @@ int rglo_sample(const float* buf, int w, int h, int stride, int filter, const fl
 // LevelOfDetail from lanes 0 and 1 of base-level texel coordinates
 // the rsqrtps model (rmlv_soa.hxx:243-248 normalize)
 float rglo_rsqrtps(float x) { return rsqrtps(x); }
+
+// the IQ post program's building blocks
+float rglo_sse_pow(float x, float y) { return sse_pow(x, y); }
+float rglo_approx_sin(float x) { return approx_sin(x); }
+float rglo_fract(float x) { return fract_t(x); }
+void rglo_iq_post(const float* in, float qx, float qy, float* out) {
+	float c[3] = { in[0], in[1], in[2] };
+	iq_post(c, qx, qy);
+	for (int k = 0; k < 3; ++k) out[k] = c[k]; }
 
 int rglo_level_of_detail(const float* u, const float* v) { return level_of_detail(u, v); }
 

@@ -118,6 +118,92 @@ __device__ inline uint32_t to_linear8(float a) {
 	r = sse_max(r, 0.0F);
 	return static_cast<uint32_t>(cvtt(r * 255.0F)); }
 
+// ---- the IQ post program's arithmetic (viewer/shaders.hxx:56-67) ----
+// _mm_cvtps_epi32 under the default rounding mode: nearest, ties to even
+__device__ inline int32_t cvtn(float f) {
+	if (f >= -2147483648.0F && f < 2147483648.0F) return static_cast<int32_t>(__builtin_rintf(f));
+	return INT32_MIN; }
+
+// exp2 and log2 of J. R. Fonseca's SSE2 pow (3rdparty/sse-pow/sse_pow.h:
+// 19-87 with EXP_POLY_DEGREE 3, LOG_POLY_DEGREE 5): the method's minimax
+// coefficients, evaluated in Horner order as its POLYn macros nest
+__device__ inline float sse_exp2(float x) {
+	x = sse_min(x, 129.00000F);
+	x = sse_max(x, -126.99999F);
+	const int32_t ip = cvtn(x - 0.5F);
+	const float fp = x - static_cast<float>(ip);
+	const float scale = u2f(static_cast<uint32_t>(shl(wadd(ip, 127), 23)));
+	float p = 7.8024521e-2F;
+	p = p * fp + 2.2606716e-1F;
+	p = p * fp + 6.9583356e-1F;
+	p = p * fp + 9.9992520e-1F;
+	return scale * p; }
+
+__device__ inline float sse_log2(float x) {
+	const uint32_t i = f2u(x);
+	const float e = static_cast<float>(static_cast<int32_t>((i & 0x7F800000u) >> 23) - 127);
+	const float m = u2f((i & 0x007FFFFFu) | 0x3F800000u);
+	float p = static_cast<float>(0.0596515482674574969533);
+	p = p * m + static_cast<float>(-0.465725644288844778798);
+	p = p * m + static_cast<float>(1.48116647521213171641);
+	p = p * m + static_cast<float>(-2.52074962577807006663);
+	p = p * m + static_cast<float>(2.8882704548164776201);
+	return p * (m - 1.0F) + e; }
+
+// rmlv pow (rmlv_mvec4.hxx:652-654)
+__device__ inline float sse_pow(float x, float y) { return sse_exp2(sse_log2(x) * y); }
+
+// rmlv sin (rmlv_mvec4.hxx:730-746): x / pi wrapped to [-1, 1] (sign
+// flipped on odd periods, wrap1), then y = M - M |M|, y (3.1 + 3.6 |y|)
+__device__ inline float approx_sin(float x) {
+	float m = x * static_cast<float>(1.0 / 3.14159265358979323846);
+	const int32_t whole = cvtt(m);
+	m = m - static_cast<float>(whole);
+	m = u2f(f2u(m) ^ static_cast<uint32_t>(shl(whole, 31)));
+	const float y = m - m * __builtin_fabsf(m);
+	return y * (3.1F + 3.6F * __builtin_fabsf(y)); }
+
+// rmlv fract (rmlv_mvec4.hxx:576-580): a - trunc(a) through cvttps2dq
+__device__ inline float fract_t(float a) { return a - static_cast<float>(cvtt(a)); }
+
+// IQPostProgram::ShadeCanvas of one pixel at canvas coordinate q:
+// per-channel gamma, vignette, and 1/255 of iq's hash3 of q.x + 13 q.y
+// (rglv_fragment.hxx:34-36)
+__device__ inline void iq_post(float c[3], float qx, float qy) {
+	c[0] = sse_pow(c[0], 0.45F); c[1] = sse_pow(c[1], 0.5F); c[2] = sse_pow(c[2], 0.55F);
+	const float t = (((16.0F * qx) * qy) * (1.0F - qx)) * (1.0F - qy);
+	const float v = 0.2F + 0.8F * sse_pow(t, 0.2F);
+	const float n = qx + 13.0F * qy;
+	const float k = static_cast<float>(1.0 / 255.0);
+#pragma unroll
+	for (int ch = 0; ch < 3; ++ch) {
+		const float h = fract_t(approx_sin(n + static_cast<float>(ch)) * 43758.5453123F);
+		c[ch] = c[ch] * v;
+		c[ch] = c[ch] + k * h; }}
+
+// FilterTile's canvas coordinate of pixel (x, y) in the tile whose corner
+// is (left, top) (rglr_algorithm.hxx:36-62): the reference steps it per
+// 2 x 2 quad, +2/W along a row and -2/H per row pair from the tile corner,
+// so the same float additions are repeated here
+__device__ inline void filter_q(int W, int H, int left, int top, int x, int y, float& qx, float& qy) {
+	const float iw = 1.0F / static_cast<float>(W), ih = 1.0F / static_cast<float>(H);
+	const float dx = iw * 2.0F, dy = -ih * 2.0F;
+	qx = (static_cast<float>(left) + 0.5F) / static_cast<float>(W);
+	if ((x - left) & 1) qx = qx + iw;
+	for (int k = (x - left) >> 1; k > 0; --k) qx = qx + dx;
+	qy = (static_cast<float>(H - top) - 0.5F) / static_cast<float>(H);
+	if ((y - top) & 1) qy = qy - ih;
+	for (int k = (y - top) >> 1; k > 0; --k) qy = qy + dy; }
+
+// StoreTrueColor's ShadeCanvas of a pixel: post program 1 (identity),
+// 2 (exposure) or 3 (IQ) (viewer/shaders.hxx:32-68)
+__device__ inline void post_shade(int post, float exposure, float c[3], int W, int H, int left, int top, int x, int y) {
+	if (post == 2) { c[0] = c[0] * exposure; c[1] = c[1] * exposure; c[2] = c[2] * exposure; }
+	else if (post == 3) {
+		float qx, qy;
+		filter_q(W, H, left, top, x, y, qx, qy);
+		iq_post(c, qx, qy); }}
+
 // qmat4 * qfloat4 (rmlm_soa.hxx:58-64), column-major f
 __device__ inline void xform4(const float* f, const float b[4], float o[4]) {
 	o[0] = ((f[0]*b[0] + f[4]*b[1]) + f[ 8]*b[2]) + f[12]*b[3];

@@ -427,7 +427,7 @@ struct rgl_gpu {
 				if (!st) return fail(RGL_E_STATE, "store without state");
 				if (!Installed((static_cast<uint32_t>(st->programId) << 24) | 0x40000000u))
 					return fail(RGL_E_DISPATCH, "no dispatch entry found for CMD_STORE_COLOR_FULL_LINEAR_TC");
-				if (st->programId != 1 && st->programId != 2) return fail(RGL_E_UNSUPPORTED, "post program not implemented");
+				if (st->programId < 1 || st->programId > 3) return fail(RGL_E_UNSUPPORTED, "post program not implemented");
 				op.kind = kOpStoreColor; op.ptr = const_cast<void*>(c.ptr); op.gamma = c.a; op.post = st->programId;
 				op.exposure = (st->programId == 2 && st->uniformsOfs >= 0) ? ubuf[st->uniformsOfs] : 1.0F;
 				f.outColor = static_cast<uint32_t*>(const_cast<void*>(c.ptr));

@@ -38,7 +38,7 @@ struct alignas(16) FrameOp {
 	int gamma;           // store colour: sRGB (1) or to_tc_basic (0)
 	float color[3];      // clear colour
 	float depth;         // clear depth
-	int post;            // store colour: post program (1 default, 2 exposure)
+	int post;            // store colour: post program (1 default, 2 exposure, 3 IQ)
 	float exposure;
 	void* ptr;           // store target, row stride = the frame's width
 };
@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 				const long long o = static_cast<long long>(qy + (l >> 1)) * f.W + qx + (l & 1);
 				if (op.kind == kOpStoreColor) {
 					float c[3] = { cr[l], cg[l], cb[l] };
-					if (op.post == 2) { c[0] = c[0] * op.exposure; c[1] = c[1] * op.exposure; c[2] = c[2] * op.exposure; }
+					if (op.post != 1)
+						post_shade(op.post, op.exposure, c, f.W, f.H, R.x0, R.y0, qx + (l & 1), qy + (l >> 1));
 					const uint32_t px = op.gamma
 						? ((to_srgb8(c[0]) << 16) | (to_srgb8(c[1]) << 8) | to_srgb8(c[2]))
 						: ((to_linear8(c[0]) << 16) | (to_linear8(c[1]) << 8) | to_linear8(c[2]));

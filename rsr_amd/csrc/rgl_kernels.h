@@ -1209,11 +1209,12 @@ __device__ inline bool resolve_pixel(const BatchDesc& b, const FrameDesc& f, con
 
 // StoreTrueColor's ShadeCanvas + sRGB encode of one pixel (FilterTile,
 // rglr_algorithm.hxx:31-104)
-__device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], const uint32_t* srgbTab) {
-	float r = col[0], g = col[1], b = col[2];
-	if (f.postProgram == 2) { r = r * f.exposure; g = g * f.exposure; b = b * f.exposure; }
-	return f.gamma ? ((to_srgb8_tab(r, srgbTab) << 16) | (to_srgb8_tab(g, srgbTab) << 8) | to_srgb8_tab(b, srgbTab))
-	               : ((to_linear8(r) << 16) | (to_linear8(g) << 8) | to_linear8(b)); }
+__device__ inline uint32_t post_pixel(const FrameDesc& f, const float col[3], int x, int y, const uint32_t* srgbTab) {
+	float c[3] = { col[0], col[1], col[2] };
+	if (f.postProgram != 1)
+		post_shade(f.postProgram, f.exposure, c, f.W, f.H, (x / f.tileW) * f.tileW, (y / f.tileH) * f.tileH, x, y);
+	return f.gamma ? ((to_srgb8_tab(c[0], srgbTab) << 16) | (to_srgb8_tab(c[1], srgbTab) << 8) | to_srgb8_tab(c[2], srgbTab))
+	               : ((to_linear8(c[0]) << 16) | (to_linear8(c[1]) << 8) | to_linear8(c[2])); }
 
 // resolve one pixel from its key and store colour, depth and primitive id
 template <class S>
@@ -1225,7 +1226,7 @@ __device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, con
 	resolve_key<S>(b, f, a, dl, draws, x, y, k, rcpTab, col, dep, prim);
 	const long long o = static_cast<long long>(y) * f.W + x;
 	// the outputs are not read again by this frame: streaming stores
-	if (f.outColor) __builtin_nontemporal_store(post_pixel(f, col, srgbTab), &f.outColor[o]);
+	if (f.outColor) __builtin_nontemporal_store(post_pixel(f, col, x, y, srgbTab), &f.outColor[o]);
 	if (f.outDepth) __builtin_nontemporal_store(dep, &f.outDepth[o]);
 	if (f.outPrim) __builtin_nontemporal_store(prim, &f.outPrim[o]); }
 

@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, f
 			r[2] = make_float4(s.p[2], s.albedo[0], s.albedo[1], s.albedo[2]);
 			r[3] = make_float4(s.rough, u2f(static_cast<uint32_t>(o)), u2f(static_cast<uint32_t>(bp.frame)), 0.0F); }}
 	if (!on) return;
-	if (f.outColor && !hit) f.outColor[o] = post_pixel(f, f.clearColor, kSrgb8Tab);
+	if (f.outColor && !hit) f.outColor[o] = post_pixel(f, f.clearColor, x, y, kSrgb8Tab);
 	if (f.outDepth) f.outDepth[o] = dep;
 	if (f.outPrim) f.outPrim[o] = prim; }
 
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(256) void k_mlp_probe(BatchDesc b, RasterArgs a, fl
 	float col[3], dep;
 	uint32_t prim;
 	if (!resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim)) return;
-	const uint32_t want = post_pixel(f, col, kSrgb8Tab);
+	const uint32_t want = post_pixel(f, col, x, y, kSrgb8Tab);
 	const uint32_t got = f.outColor[static_cast<long long>(y) * f.W + x];
 	float se = 0.0F;
 	for (int c = 0; c < 3; ++c) {
@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a,
 	float col[3], dep;
 	uint32_t prim;
 	if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, rcpTab, col, dep, prim))
-		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, srgbTab); }
+		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, x, y, srgbTab); }
 
 // training samples from a frame: covered pixels kept with probability
 // `keep` (hash of pixel and seed), appended as 13 inputs + the plain
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(256) void k_mlp_samples(BatchDesc b, RasterArgs a, 
 	if (!resolve_inputs<P>(b, f, a, dl, x, y, kRcpMantissa, s, dep, prim)) return;
 	float col[3];
 	resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim);
-	const uint32_t px = post_pixel(f, col, kSrgb8Tab);
+	const uint32_t px = post_pixel(f, col, x, y, kSrgb8Tab);
 	const uint32_t slot = atomicAdd(count, 1u);
 	if (slot >= cap) return;
 	float* r = out + static_cast<long long>(slot) * kMlpSample;

@@ -64,6 +64,21 @@ __device__ inline float sample_depth(const float* buf, int dim, float x, float y
 	const float c = buf ? buf[ofs] : 0.0F;
 	return hit ? c : -1.0F; }
 
+// rglv::BaseProgram (rglv_gpu_shaders.hxx:21-95): the position alone, colour
+// 1; the $layer shadow-map pass draws with it (node/gllayer.cxx:44-45).  It
+// has no varyings; the one here is unused (the kernels' per-vertex arrays
+// are sized by kVary)
+struct ProgBase {
+	static constexpr int kId = 0, kVary = 1, kUV = -1;
+	static constexpr bool kEarlyZ = true, kDiscard = false, kInstanced = false;
+	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		xform4(d.vpm, p, pos);
+		vo[0] = 0.0F; }
+	__device__ static bool fragment(const DrawDesc&, const float*, int, const FragCoord&, float color[4]) {
+		color[0] = 1.0F; color[1] = 1.0F; color[2] = 1.0F; color[3] = 1.0F;
+		return true; }};
+
 // AmyProgram (viewer/shaders.hxx:69-161)
 struct ProgAmy {
 	static constexpr int kId = 4, kVary = 2, kUV = 0;
@@ -440,6 +455,38 @@ struct ProgOBJ2S {
 		color[2] = (attenuation * v[10]) * diffuse; color[3] = 1.0F;
 		return true; }};
 
+// EnvmapProgram (viewer/shaders_envmap.hxx:26-127): a sphere-map lookup
+// at the reflection of the view vector about the smooth normal (slots 3-5,
+// default (0, 0, 1)); slots 6-8 hold the face normal (default (1, 1, 1)),
+// mixed in with weight 0; alpha 0.5 for the blended pass
+struct ProgEnvmap {
+	static constexpr int kId = 10, kVary = 2, kUV = 0;
+	static constexpr bool kEarlyZ = true, kDiscard = false, kInstanced = false;
+	__device__ static void vertex(const DrawDesc& d, int, int idx, float pos[4], float* vo) {
+		float p[4]; load_pos(d, idx, p);
+		float sn[3]; obj_normal(d, idx, sn);
+		const bool c = d.buffers[6] != nullptr;
+		const float fn[3] = { c ? d.buffers[6][idx] : 1.0F, c ? d.buffers[7][idx] : 1.0F, c ? d.buffers[8][idx] : 1.0F };
+		float mv[4]; xform4(d.vm, p, mv);
+		float e[3] = { mv[0], mv[1], mv[2] };
+		normalize3_rsqrt(e);
+		float vn[3];
+#pragma unroll
+		for (int k = 0; k < 3; ++k) vn[k] = sn[k] + 0.0F * (fn[k] - sn[k]);   // mix(a, b, 0) = a + t (b - a)
+		float n[3]; xform3w0(d.nm, vn, n);
+		normalize3_rsqrt(n);
+		const float dn2 = 2.0F * ((n[0]*e[0] + n[1]*e[1]) + n[2]*e[2]);
+		const float r[3] = { e[0] - dn2 * n[0], e[1] - dn2 * n[1], e[2] - dn2 * n[2] };   // reflect (rglv_math.hxx:45-46)
+		const float rz1 = r[2] + 1.0F;
+		const float m = 2.0F * __builtin_sqrtf(((r[0]*r[0]) + (r[1]*r[1])) + (rz1*rz1));
+		vo[0] = r[0] / m + 0.5F;
+		vo[1] = r[1] / m + 0.5F;
+		xform4(d.vpm, p, pos); }
+	__device__ static bool fragment(const DrawDesc& d, const float* v, int lod, const FragCoord&, float color[4]) {
+		const float4 t = sample1(d.tu0, v[0], v[1], lod);
+		color[0] = t.x; color[1] = t.y; color[2] = t.z; color[3] = 0.5F;
+		return true; }};
+
 // ---------------------------------------------------------------------------
 // the program list and what is generated from it
 // ---------------------------------------------------------------------------
@@ -449,8 +496,8 @@ template <class A, class B> struct ConcatPrograms;
 template <class... As, class... Bs> struct ConcatPrograms<ProgramList<As...>, ProgramList<Bs...>> {
 	using type = ProgramList<As..., Bs...>; };
 
-using BuiltinPrograms = ProgramList<ProgAmy, ProgMany, ProgLambertTex, ProgManyBlinnPhong, ProgGGXEnv, ProgAlphaTexture,
-                                    ProgText, ProgDepth, ProgPattern, ProgOBJ1, ProgOBJ2, ProgOBJ2S>;
+using BuiltinPrograms = ProgramList<ProgBase, ProgAmy, ProgMany, ProgLambertTex, ProgManyBlinnPhong, ProgGGXEnv, ProgAlphaTexture,
+                                    ProgText, ProgDepth, ProgPattern, ProgOBJ1, ProgOBJ2, ProgOBJ2S, ProgEnvmap>;
 
 }  // namespace rgl
 

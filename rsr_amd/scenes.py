@@ -33,17 +33,20 @@ GL_DEPTH_ATTACHMENT, GL_COLOR_ATTACHMENT0 = 0, 2
 RB_COLOR_DEPTH, RB_RGBF32, RB_RGBAF32, RB_F32 = 0, 1, 2, 3
 RGL_HINT_READ4, RGL_HINT_DENSE = 1, 2
 
-# program ids: 1 DefaultPost, 2 ExposurePost, 4 Amy, 5 Depth, 6 Many,
-# 7 OBJ1, 8 OBJ2, 9 OBJ2S, 26 Text, 41 Pattern and 65 AlphaTexture are the
-# reference's (viewer/shaders.hxx); 80, 81 and 82 are this project's shaders.
+# program ids: 0 rglv::BaseProgram, 1 DefaultPost, 2 ExposurePost, 3 IQPost, 4 Amy, 5 Depth, 6 Many,
+# 7 OBJ1, 8 OBJ2, 9 OBJ2S, 10 Envmap, 26 Text, 41 Pattern and 65 AlphaTexture
+# are the reference's (viewer/shaders.hxx, shaders_envmap.hxx); 80, 81 and 82 are this project's shaders.
+PGM_BASE = 0
 PGM_DEFAULT_POST = 1
 PGM_EXPOSURE_POST = 2
+PGM_IQ_POST = 3
 PGM_AMY = 4
 PGM_DEPTH = 5
 PGM_MANY = 6
 PGM_OBJ1 = 7
 PGM_OBJ2 = 8
 PGM_OBJ2S = 9
+PGM_ENVMAP = 10
 PGM_TEXT = 26
 PGM_PATTERN = 41
 PGM_ALPHA_TEXTURE = 65

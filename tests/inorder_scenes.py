@@ -129,12 +129,14 @@ def scene_mid_frame_clears():
 
 
 def scene_shadow_pass():
-    # the $layer shadow-map pass (node/gllayer.cxx:152-181): colour writes
-    # off, depth-only clear, front faces culled, depth stored
+    # the $layer shadow-map pass (node/gllayer.cxx:44-45, 152-181):
+    # BaseProgram, colour writes off, depth-only clear, front faces culled,
+    # depth stored
     sc, pro, _ = base(256, 256, name="shadow_pass")
     sc.calls = pro + [("color_write_mask", False), ("depth_write_mask", True), ("depth_func", GL_LESS),
-                      ("enable", GL_CULL_FACE), ("cull_face", GL_FRONT), ("clear_depth", 1.0), ("clear", DB)] + \
-        lambert(sc) + draw(sc, "ground") + draw(sc, "sphere") + [("store_depth", ("out", 0)), ("store_primitive_id", ("out", 1))]
+                      ("enable", GL_CULL_FACE), ("cull_face", GL_FRONT), ("clear_depth", 1.0), ("clear", DB),
+                      ("use_program", scenes.PGM_BASE)] + \
+        draw(sc, "ground") + draw(sc, "sphere") + [("store_depth", ("out", 0)), ("store_primitive_id", ("out", 1))]
     sc.outputs = ["depth", "prim"]
     return sc
 

@@ -73,6 +73,11 @@ def lib():
         L.rglo_rcpps.restype = f
         L.rglo_rsqrtps.argtypes = [f]
         L.rglo_rsqrtps.restype = f
+        for name in ("sse_pow", "approx_sin", "fract"):
+            fn = getattr(L, "rglo_" + name)
+            fn.argtypes = [f] * (2 if name == "sse_pow" else 1)
+            fn.restype = f
+        L.rglo_iq_post.argtypes = [vp, f, f, vp]
         L.rglo_cvtt.argtypes = [f]
         L.rglo_cvtt.restype = ctypes.c_int32
         L.rglo_to_srgb8.argtypes = [f]

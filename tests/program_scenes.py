@@ -46,7 +46,7 @@ def _inorder(calls, sc):
 def _finish(sc, pro, body, inorder, post=None):
     sc.calls = pro + [("clear", CB | DB)] + (_inorder(body, sc) if inorder else body)
     sc.calls += post if post is not None else S.stores()
-    sc.outputs = S.OUT3 if post is None else sc.outputs
+    sc.outputs = sc.outputs or S.OUT3
     if inorder:
         sc.name += "_inorder"
     return sc
@@ -176,14 +176,16 @@ def _col(m):
 
 
 def scene_shadow_map():
-    # the light's depth pass: colour writes off, front faces culled, the
+    # the light's depth pass as $layer draws it (node/gllayer.cxx:44-45,
+    # 152-181): BaseProgram, colour writes off, front faces culled, the
     # depth stored as the OBJ2S frame's shadow map
     sc, pro, _ = S.base(SHADOW_DIM, SHADOW_DIM, name="obj2s_shadow_map")
     view, proj = _light_matrices()
     pro = [p for p in pro if p[0] not in ("view_matrix", "projection_matrix")] + \
         [("view_matrix", view), ("projection_matrix", proj)]
-    sc.calls = pro + [("color_write_mask", False), ("cull_face", GL_FRONT), ("clear", DB)] + S.lambert(sc) + \
-        S.draw(sc, "ground") + S.draw(sc, "sphere") + [("store_depth", ("out", 0))]
+    sc.calls = pro + [("color_write_mask", False), ("cull_face", GL_FRONT), ("clear", DB),
+                      ("use_program", scenes.PGM_BASE)] + S.draw(sc, "ground") + S.draw(sc, "sphere") + \
+        [("store_depth", ("out", 0))]
     sc.outputs = ["depth"]
     return sc
 
@@ -211,8 +213,39 @@ def scene_obj2s(shadow_map, inorder=False, cone_deg=22.0):
     return _finish(sc, pro, body, inorder)
 
 
+def scene_envmap(two_pass=False):
+    # EnvmapProgram: the sphere-map uv per vertex from the reflected view
+    # vector; alone (the LESS install) or as the blended second pass of
+    # shaders_envmap.cxx:22-25 (GL_EQUAL, no depth write, BlendAlpha with
+    # its alpha 0.5) over a Lambert pass, in the viewer's RB_COLOR_DEPTH
+    sc, pro, _ = S.base(name="envmap" + ("_two_pass" if two_pass else ""))
+    pro = [("renderbuffer_type", GL_COLOR_ATTACHMENT0, scenes.RB_COLOR_DEPTH),
+           ("renderbuffer_type", GL_DEPTH_ATTACHMENT, scenes.RB_COLOR_DEPTH)] + pro[2:]
+    sc.arrays["spheremap"] = scenes.make_mipmap_stack(scenes.value_noise_rgba8(256, 29, lattice=8))
+    _vertex_colours(sc, "sphere", 9, -1.0, 1.0)   # face normals: weight 0 in the mix
+    env = [("use_program", scenes.PGM_ENVMAP),
+           ("bind_texture", 0, "@spheremap", 256, 256, 256, scenes.GL_LINEAR_MIPMAP_NEAREST)]
+    if two_pass:
+        body = S.lambert(sc) + S.draw(sc, "ground") + S.draw(sc, "sphere") + \
+            [("depth_func", scenes.GL_EQUAL), ("depth_write_mask", False), ("enable", GL_BLEND)] + env + \
+            S.draw(sc, "ground") + S.draw(sc, "sphere")
+    else:
+        body = env + S.draw(sc, "ground") + S.draw(sc, "sphere")
+    return _finish(sc, pro, body, False)
+
+
+def scene_iq_post(inorder=False, gamma=True, width=640, height=360, tile_blocks=(8, 8)):
+    # IQPostProgram on the store: per-channel gamma, vignette and hash of
+    # the canvas coordinate, which FilterTile steps from each tile's corner
+    sc, pro, _ = S.base(width, height, tile_blocks, name=f"iq_post_{width}x{height}_t{tile_blocks[0] * 8}" +
+                        ("" if gamma else "_linear"))
+    body = S.lambert(sc) + S.draw(sc, "ground") + S.draw(sc, "sphere")
+    return _finish(sc, pro, body, inorder, post=S.stores(post=scenes.PGM_IQ_POST, gamma=gamma))
+
+
 # scenes that need no input from another frame
 ALL = [scene_alpha_texture, lambda: scene_alpha_texture(True), scene_text, scene_depth_texture,
        lambda: scene_depth_texture(True), scene_pattern, lambda: scene_pattern(True), scene_pattern_clipped,
        lambda: scene_pattern_clipped(True), scene_obj1, lambda: scene_obj1(True), scene_obj2, lambda: scene_obj2(True),
-       scene_obj_white_exposure]
+       scene_obj_white_exposure, scene_envmap, lambda: scene_envmap(True), scene_iq_post, lambda: scene_iq_post(True), lambda: scene_iq_post(gamma=False),
+       lambda: scene_iq_post(False, True, 328, 184, (4, 4))]

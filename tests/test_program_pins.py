@@ -8,6 +8,7 @@ against a float64 evaluation with an exact matrix inverse."""
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 
@@ -411,3 +412,147 @@ def test_user_programs_header_builds_into_the_library(tmp_path):
     syms = subprocess.run([nm, "-C", out], capture_output=True, text=True, check=True).stdout
     for kern in ("k_vertex<rgl::ProgFlatScaled>", "k_clip<rgl::ProgFlatScaled>", "k_resolve<rgl::ShadeOne<rgl::ProgFlatScaled> >"):
         assert kern in syms, kern
+
+
+# ---------------------------------------------------------------------------
+# the IQ post program (viewer/shaders.hxx:56-67)
+# ---------------------------------------------------------------------------
+def test_sse_pow_accuracy_and_fixed_points():
+    # Fonseca's exp2 / log2 minimax fits (sse_pow.h): relative error of a
+    # few 1e-4 against pow; pow(1, y) = exp2(0) = the exp2 fit's constant
+    # term, 0.9999252
+    L = olib()
+    for x in np.geomspace(1e-4, 16.0, 400).astype(np.float32):
+        for y in (0.2, 0.45, 0.5, 0.55, 1.0, 2.0):
+            got = L.rglo_sse_pow(float(x), y)
+            want = float(x) ** float(np.float32(y))
+            assert abs(got / want - 1.0) < 3e-4, (x, y, got, want)
+    for y in (0.2, 0.45, 3.0):
+        assert L.rglo_sse_pow(1.0, y) == np.float32(9.9992520e-1)
+    assert L.rglo_sse_pow(0.0, 0.45) < 1e-16        # log2(0) = -127 in the fit
+    assert L.rglo_sse_pow(-0.25, 0.5) == L.rglo_sse_pow(0.25, 0.5)  # the sign bit is not read
+
+
+def test_approx_sin_accuracy_and_symmetry():
+    # the wrapped parabola y (3.1 + 3.6 |y|): |error| < 1.1e-3, odd, 2 pi
+    # periodic up to the wrap's rounding
+    L = olib()
+    xs = np.linspace(-60.0, 60.0, 20001).astype(np.float32)
+    got = np.array([L.rglo_approx_sin(float(x)) for x in xs])
+    assert np.max(np.abs(got - np.sin(xs.astype(np.float64)))) < 1.1e-3
+    for x in xs[::97]:
+        assert L.rglo_approx_sin(float(-x)) == -L.rglo_approx_sin(float(x))
+    assert L.rglo_approx_sin(0.0) == 0.0
+    assert abs(L.rglo_approx_sin(math.pi / 2) - 1.0) < 1e-3
+
+
+def test_fract_truncates_toward_zero():
+    L = olib()
+    for x, want in ((2.75, 0.75), (-2.75, -0.75), (0.0, 0.0), (12345.5, 0.5)):
+        assert L.rglo_fract(x) == want
+    assert L.rglo_fract(3e9) == np.float32(3e9) - np.float32(-2147483648.0)  # cvttps2dq overflow
+
+
+def _iq(c, qx, qy):
+    L = olib()
+    cin = np.ascontiguousarray(c, np.float32)
+    out = np.zeros(3, np.float32)
+    L.rglo_iq_post(cin.ctypes.data, qx, qy, out.ctypes.data)
+    return out
+
+
+def test_iq_post_composes_gamma_vignette_and_hash():
+    # built from the pinned blocks: pow per channel with (0.45, 0.5, 0.55),
+    # vignette 0.2 + 0.8 pow(16 x y (1 - x) (1 - y), 0.2), plus 1/255 of
+    # fract(sin(q.x + 13 q.y + k) 43758.5453123) for channel k (in (-1, 1))
+    L = olib()
+    rng = np.random.default_rng(41)
+    for _ in range(300):
+        c = rng.uniform(0.0, 1.5, 3).astype(np.float32)
+        qx, qy = (float(np.float32(v)) for v in rng.uniform(0.0, 1.0, 2))
+        got = _iq(c, qx, qy)
+        t = np.float32(np.float32(np.float32(np.float32(16.0) * np.float32(qx)) * np.float32(qy)) * np.float32(1.0 - np.float32(qx)))
+        t = np.float32(t * np.float32(np.float32(1.0) - np.float32(qy)))
+        v = np.float32(np.float32(0.2) + np.float32(0.8) * np.float32(L.rglo_sse_pow(float(t), 0.2)))
+        n = np.float32(np.float32(qx) + np.float32(13.0) * np.float32(qy))
+        for k, g in enumerate((0.45, 0.5, 0.55)):
+            base = np.float32(np.float32(L.rglo_sse_pow(float(c[k]), g)) * v)
+            h = np.float32(L.rglo_fract(float(np.float32(L.rglo_approx_sin(float(n + np.float32(k)))) * np.float32(43758.5453123))))
+            assert -1.0 < h < 1.0   # fract truncates toward zero: negative for a negative sine
+            want = np.float32(base + np.float32(1.0 / 255.0) * h)
+            assert got[k] == want, (c, qx, qy, k)
+    # the vignette is 1 (up to the exp2 fit) at the centre and 0.2 at the rim
+    ctr = _iq(np.ones(3, np.float32), 0.5, 0.5)
+    assert np.all(np.abs(ctr - 1.0) < 5e-3)
+    rim = _iq(np.ones(3, np.float32), 0.0, 0.7)
+    assert np.all((rim > 0.2 - 1 / 255 - 1e-4) & (rim < 0.2 + 1 / 255 + 1e-4))
+
+
+def _flat_iq_frame(w, h, tile_blocks, gamma):
+    sc = scenes.CallScene(name="iq_flat", width=w, height=h, tile_blocks=tile_blocks)
+    sc.calls = [("renderbuffer_type", GL_COLOR_ATTACHMENT0, RB_RGBF32), ("renderbuffer_type", GL_DEPTH_ATTACHMENT, RB_F32),
+                ("clear_color", 0.3, 0.5, 0.7), ("clear", CB | DB), ("use_program", scenes.PGM_IQ_POST),
+                ("store_color_tc", ("out", 0), gamma)]
+    sc.outputs = ["color"]
+    return oracle_calls(sc, threads=4)[0][0]
+
+
+def test_iq_post_frame_canvas_coordinates():
+    # FilterTile (rglr_algorithm.hxx:31-62) gives the post program
+    # q = ((x + 0.5) / W, (H - y - 0.5) / H) for pixel (x, y), stepped per
+    # quad from each tile's corner; on a flat frame the stored colour lies
+    # within 1/255 of the hash-free value computed with the exact q
+    L = olib()
+    w, h = 200, 96
+    out = _flat_iq_frame(w, h, (4, 4), False)           # linear store, 32 px tiles
+    ys, xs = np.mgrid[0:h, 0:w]
+    qx = (xs + 0.5) / w
+    qy = (h - ys - 0.5) / h
+    vig = 0.2 + 0.8 * (16 * qx * qy * (1 - qx) * (1 - qy)) ** 0.2
+    chans = _rgb(out)
+    for k, (c, g) in enumerate(((0.3, 0.45), (0.5, 0.5), (0.7, 0.55))):
+        c32 = float(np.float32(c))
+        base = c32 ** g * vig
+        lo = np.floor(np.clip(base * (1 - 1e-3) - 1 / 255, 0, 1) * 255)
+        hi = np.floor(np.clip(base * (1 + 1e-3) + 1 / 255, 0, 1) * 255)
+        got = chans[k]
+        assert np.all((got >= lo) & (got <= hi)), k
+        # the hash term spreads over (-1/255, 1/255): pixels land a code
+        # below, at and above the hash-free value
+        mid = np.floor(base * 255)
+        for frac in ((got < mid).mean(), (got == mid).mean(), (got > mid).mean()):
+            assert frac > 0.1, (k, frac)
+
+
+def test_envmap_vertex_sphere_map_uv():
+    # EnvmapProgram (shaders_envmap.hxx:100-112): e = unit view position,
+    # n = unit (transpose(inverse(vm)) normal), r = e - 2 (n . e) n,
+    # uv = r.xy / (2 sqrt(r.x^2 + r.y^2 + (r.z + 1)^2)) + 0.5; the face
+    # normal enters with weight 0
+    rng = np.random.default_rng(35)
+    pos, nrm, face, b = _mesh(rng, 12)
+    eye = scenes.mat4_rows(1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1)
+    # the closed form: a point straight ahead with its normal toward the eye
+    one = {0: scenes.pad4(np.array([0.0], np.float32)), 1: scenes.pad4(np.array([0.0], np.float32)),
+           2: scenes.pad4(np.array([-5.0], np.float32)), 3: scenes.pad4(np.array([0.0], np.float32)),
+           4: scenes.pad4(np.array([0.0], np.float32)), 5: scenes.pad4(np.array([1.0], np.float32))}
+    _, vary = shade_vertex(scenes.PGM_ENVMAP, 0, one, eye, eye)
+    assert vary[0] == 0.5 and vary[1] == 0.5
+    for trial in range(6):
+        vm = _view(rng)
+        V = vm.reshape(4, 4).T.astype(np.float64)
+        nm = np.linalg.inv(V).T
+        for i in range(12):
+            _, vary = shade_vertex(scenes.PGM_ENVMAP, i, b, vm, vm)
+            e = (V @ np.r_[pos[i], 1.0])[:3]
+            e /= np.linalg.norm(e)
+            n = nm[:3, :3] @ nrm[i]
+            n /= np.linalg.norm(n)
+            r = e - 2 * (n @ e) * n
+            m = 2 * np.sqrt(r[0] ** 2 + r[1] ** 2 + (r[2] + 1) ** 2)
+            np.testing.assert_allclose(vary[:2], r[:2] / m + 0.5, atol=2e-3)
+            # other face normals, same uv
+            b2 = dict(b)
+            for k in range(3):
+                b2[6 + k] = -b[6 + k]
+            assert np.array_equal(shade_vertex(scenes.PGM_ENVMAP, i, b2, vm, vm)[1][:2], vary[:2])
