@@ -1036,7 +1036,8 @@ struct rglo_gpu {
 
 	// ---- binning (GPU::BinImpl, rglv_gpu.cxx:119-260) ----
 	bool Bin() {
-		tiles.assign(static_cast<size_t>(tilesX) * tilesY, {});
+		tiles.resize(static_cast<size_t>(tilesX) * tilesY);
+		for (auto& t : tiles) t.clear();  // (keeps the lists' capacity from the previous frame)
 		clipped.clear();
 		const GLState* binState = nullptr;
 		int binStateIdx = -1;
@@ -1094,19 +1095,26 @@ struct rglo_gpu {
 		for (auto& t : tiles) t.push_back({ TileOp::TRI, -1, 0, 0, 0, 0, nullptr });  // draw marker
 		struct ClipItem { int iid; uint16_t i[3]; uint32_t prim; };
 		std::vector<ClipItem> clipQueue;
+		// each vertex of an instance shaded once (BinTriangles2P's vertex
+		// phase, rglv_gpu_impl.hxx:314-420): clip flags and device coordinates
+		int nVerts = 0;
+		for (int k = 0; k < numPrims * 3; ++k) nVerts = std::max(nVerts, (indices ? indices[k] : k) + 1);
+		std::vector<uint32_t> vcf(static_cast<size_t>(nVerts));
+		std::vector<float> vdx(static_cast<size_t>(nVerts)), vdy(static_cast<size_t>(nVerts));
 		for (int iid = 0; iid < instanceCnt; ++iid) {
+			for (int v = 0; v < nVerts; ++v) {
+				float pos[4]; VOut vo;
+				pgm->ShadeVertex(env, iid, v, pos, vo);
+				vcf[v] = frustum.Test(pos);
+				const float iw = oneover(pos[3]);  // pdiv (rglv_math.hxx:19-23)
+				vdx[v] = (pos[0] * iw) * DS[0] + DO[0];
+				vdy[v] = (pos[1] * iw) * DS[1] + DO[1]; }
 			for (int pi = 0; pi < numPrims; ++pi) {
 				++statSubmitted;
 				uint16_t ii[3];
 				for (int k = 0; k < 3; ++k) ii[k] = indices ? indices[pi*3 + k] : static_cast<uint16_t>(pi*3 + k);
 				uint32_t cf[3]; float dcx[3], dcy[3];
-				for (int k = 0; k < 3; ++k) {
-					float pos[4]; VOut vo;
-					pgm->ShadeVertex(env, iid, ii[k], pos, vo);
-					cf[k] = frustum.Test(pos);
-					const float iw = oneover(pos[3]);  // pdiv (rglv_math.hxx:19-23)
-					dcx[k] = (pos[0] * iw) * DS[0] + DO[0];
-					dcy[k] = (pos[1] * iw) * DS[1] + DO[1]; }
+				for (int k = 0; k < 3; ++k) { cf[k] = vcf[ii[k]]; dcx[k] = vdx[ii[k]]; dcy[k] = vdy[ii[k]]; }
 				const bool pointsOutside = (cf[0] | cf[1] | cf[2]) > 0;
 				const bool primsOutside = (cf[0] & cf[1] & cf[2]) > 0;
 				const uint32_t prim = primBase + static_cast<uint32_t>(iid) * numPrims + pi;
