@@ -390,7 +390,9 @@ struct rgl_gpu {
 		f.drawEnabled = 1;
 		f.opBase = static_cast<int>(ops.size());
 		const GLState* st = nullptr;
-		bool ordered = false;             // the frame needs the in-order tile path
+		// the frame needs the in-order tile path (the key path's
+		// visibility tiles are at most kMaxTileDim pixels square)
+		bool ordered = tileW > kMaxTileDim || tileH > kMaxTileDim;
 		int clears = 0, stores[3] = { 0, 0, 0 };
 		bool stored = false, drawn = false;
 		frameTriStart = triBase;
@@ -1178,7 +1180,10 @@ int rgl_gpu_install(rgl_gpu* g, int program_id, uint32_t state_key, int kind) {
 int rgl_gpu_reset(rgl_gpu* g, int width, int height, int tbx, int tby) {
 	if (check(g)) return RGL_E_ARG;
 	if (width <= 0 || height <= 0 || tbx <= 0 || tby <= 0) return g->fail(RGL_E_ARG, "bad size");
-	if (tbx * 8 > kMaxTileDim || tby * 8 > kMaxTileDim) return g->fail(RGL_E_UNSUPPORTED, "tiles larger than 64x64 pixels are not supported");
+	// the reference's fragment cursors step 64 quads a row: tiles of at
+	// most 128 x 128 pixels (rglr_fragmentcursor.hxx:14)
+	if (tbx * 8 > kMaxAnyTileDim || tby * 8 > kMaxAnyTileDim)
+		return g->fail(RGL_E_UNSUPPORTED, "tiles larger than 128x128 pixels are not supported");
 	if (width > 65535 * tbx * 8 || height > 65535 * tby * 8) return g->fail(RGL_E_ARG, "target too large");
 	if (g->pending) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
 	g->W = width; g->H = height; g->tileW = tbx * 8; g->tileH = tby * 8;

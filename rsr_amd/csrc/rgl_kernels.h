@@ -30,7 +30,8 @@ constexpr int kXcds = 8;             // workgroups are dealt round-robin over th
 // other in its list.
 constexpr int kCountGroups = kXcds;
 constexpr int kRasterThreads = 256;
-constexpr int kMaxTileDim = 64;     // tiles are at most 64 x 64 pixels
+constexpr int kMaxTileDim = 64;     // key-path tiles are at most 64 x 64 pixels
+constexpr int kMaxAnyTileDim = 128; // in-order tiles at most 128 x 128 (the reference's limit)
 constexpr int kSegPairs = 512;       // default (triangle, tile) pairs per visibility work item
 constexpr int kSplitQuads = 0;       // default quad work per quadrant from which visibility regions are halved
 constexpr int kSubDim = kMaxTileDim / 2;   // visibility quadrant: 32 x 32 pixels

@@ -118,3 +118,14 @@ def test_mid_frame_stores_snapshot_the_tile():
     # drawn, and its ids follow the ground's two triangles
     cov = (outs[3] != 0xFFFFFFFF) & (outs[3] >= 2)
     assert cov.sum() > 1000 and np.all(outs[2][cov] < 1.0)
+
+
+def test_tile_size_invariance_up_to_128px():
+    # the raster is invariant to the tile grid: 128, 96 and 64 px tiles
+    # store the same buffers (no scissor in the frame)
+    from test_tiles128_gpu import _plain
+    base = _full(_plain((8, 8)))
+    for tb in ((16, 16), (16, 8), (12, 12)):
+        got = _full(_plain(tb))
+        for a, b in zip(base, got):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), tb
