@@ -135,12 +135,32 @@ int rgl_gl_clear(rgl_gpu* gpu, int bits);                                       
 int rgl_gl_store_color_tc(rgl_gpu* gpu, uint32_t* dev_pixels, int enable_gamma);
 /* GL::StoreDepth rglv_gl.cxx:86-89: width*height float, row stride = width. */
 int rgl_gl_store_depth(rgl_gpu* gpu, float* dev_depth);
+/* GL::StoreColor(FloatingPointCanvas*, bool downsample) rglv_gl.cxx:64-70
+   (CMD_STORE_COLOR_FULL_LINEAR_FP / _HALF_LINEAR_FP, rglv_gpu.cxx:345-370):
+   dev_rgba = RGBA float pixels (alpha 0), row stride stride_px pixels (0:
+   width, or (width + 1) / 2 downsampled).  downsample = the 2 x 2 box mean
+   into pixel (x / 2, y / 2) (rglr_algorithm.cxx:118-165). */
+int rgl_gl_store_color_fp(rgl_gpu* gpu, float* dev_rgba, int stride_px, int downsample);
+/* GL::StoreColor(QFloat4Canvas*) rglv_gl.cxx:73-77 (CMD_STORE_COLOR_FULL_QUADS_FP,
+   rglv_gpu.cxx:371-384): the quad-swizzled canvas, quad (x / 2, y / 2) at
+   index (y / 2) * stride_quads + x / 2 (0: (width + 1) / 2), 16 floats
+   r[4] g[4] b[4] a[4], lane = (y & 1) * 2 + (x & 1); alpha 1 from
+   RB_RGBF32, the fourth channel otherwise (RB_COLOR_DEPTH: the depth). */
+int rgl_gl_store_color_quads(rgl_gpu* gpu, float* dev_quads, int stride_quads);
 /* No counterpart in the reference: stores, per pixel, the id of the
    triangle that last passed the depth test (instance * triangles per
    instance + triangle index, counted across the frame's draws in
    submission order), 0xffffffff where nothing was drawn. */
 int rgl_gl_store_primitive_id(rgl_gpu* gpu, uint32_t* dev_ids);
 int rgl_gl_reset(rgl_gpu* gpu);                                                            /* GL::Reset rglv_gl.cxx:92-97 */
+
+/* --- post effects -------------------------------------------------------- */
+/* rglr::KawaseBlurFilter rglr_kawase.cxx:21-85 over a whole canvas: one
+   blur pass of distance d from dev_src into dev_dst (RGBA float pixels,
+   width x height, row strides in pixels; src and dst must differ), queued
+   on hip_stream. */
+int rgl_kawase_blur(const float* dev_src, float* dev_dst, int width, int height, int src_stride, int dst_stride,
+                    int d, void* hip_stream);
 
 /* --- texture preparation ------------------------------------------------ */
 /* Texture::maybe_make_mipmap (rglr_texture.cxx:33-81) on host memory:

@@ -182,7 +182,7 @@ constexpr int GL_FRONT = 1, GL_BACK = 2;
 constexpr int GL_LESS = 0, GL_LEQUAL = 1, GL_EQUAL = 2;
 constexpr int GL_COLOR_BUFFER_BIT = 1, GL_DEPTH_BUFFER_BIT = 2;
 constexpr int GL_DEPTH_ATTACHMENT = 0, GL_COLOR_ATTACHMENT0 = 2;
-constexpr int RB_COLOR_DEPTH = 0;
+constexpr int RB_COLOR_DEPTH = 0, RB_RGBF32 = 1, RB_RGBAF32 = 2;
 constexpr int UNIFORM_BUFFER_SIZE = 32;
 
 struct TextureState { const float* ptr; int width, height, stride, filter; };
@@ -222,7 +222,7 @@ struct GLState {
 
 struct Cmd {
 	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
-	            STORE_TC, STORE_DEPTH, STORE_PRIM } kind;
+	            STORE_TC, STORE_DEPTH, STORE_PRIM, STORE_FP } kind;
 	int a, b, c;
 	const void* ptr; };
 
@@ -872,7 +872,7 @@ inline void iq_post(float c[3], float qx, float qy) {
 // per-tile command lists
 // ------------------------------------------------------------------------
 struct TileOp {
-	enum Op : uint8_t { STATE, CLEAR, TRI, CLIPPED_TRI, STORE_TC, STORE_DEPTH, STORE_PRIM } op;
+	enum Op : uint8_t { STATE, CLEAR, TRI, CLIPPED_TRI, STORE_TC, STORE_DEPTH, STORE_PRIM, STORE_FP } op;
 	int32_t a;      // state index | clear bits | iid | gamma
 	uint16_t i0, i1, i2;
 	uint32_t prim;
@@ -1062,6 +1062,9 @@ struct rglo_gpu {
 			case Cmd::STORE_PRIM:
 				for (auto& t : tiles) t.push_back({ TileOp::STORE_PRIM, 0, 0, 0, 0, 0, c.ptr });
 				break;
+			case Cmd::STORE_FP:  // a = the canvas stride, i0 = 0 full, 1 half (downsample), 2 quads
+				for (auto& t : tiles) t.push_back({ TileOp::STORE_FP, c.b, static_cast<uint16_t>(c.a), 0, 0, 0, c.ptr });
+				break;
 			default: {
 				if (binState == nullptr) { err = "draw before state"; return false; }
 				const uint32_t vkey = static_cast<uint32_t>(binState->programId) << 24;
@@ -1198,7 +1201,7 @@ struct rglo_gpu {
 
 	// ---- per-tile drawing (GPU::DrawImpl, rglv_gpu.cxx:263-432) ----
 	struct TileBuf {
-		std::vector<float> r, g, b, d;
+		std::vector<float> r, g, b, d, a;   // a: RB_RGBAF32's fourth channel (set by clears only)
 		std::vector<uint32_t> prim;
 		long long fragments = 0;
 		long long quads = 0; };
@@ -1397,7 +1400,7 @@ struct rglo_gpu {
 			~Rec() { g->tileQuads[t] = tb.quads; g->tilePairs[t] = pairs; } } rec{ this, t, tb, pairs };
 		const size_t area = static_cast<size_t>(tileW) * tileH;
 		tb.r.assign(area, 0.0F); tb.g.assign(area, 0.0F); tb.b.assign(area, 0.0F);
-		tb.d.assign(area, 0.0F); tb.prim.assign(area, 0xffffffffu);
+		tb.d.assign(area, 0.0F); tb.a.assign(area, 0.0F); tb.prim.assign(area, 0xffffffffu);
 		const GLState* st = nullptr;
 		DrawEnv env{};
 		const Program* pgm = nullptr;
@@ -1414,6 +1417,7 @@ struct rglo_gpu {
 					std::fill(tb.r.begin(), tb.r.end(), st->clearColor[0]);
 					std::fill(tb.g.begin(), tb.g.end(), st->clearColor[1]);
 					std::fill(tb.b.begin(), tb.b.end(), st->clearColor[2]);
+					std::fill(tb.a.begin(), tb.a.end(), st->clearColor[3]);
 					std::fill(tb.prim.begin(), tb.prim.end(), 0xffffffffu); }
 				if (op.a & GL_DEPTH_BUFFER_BIT) std::fill(tb.d.begin(), tb.d.end(), st->clearDepth);
 				break;
@@ -1493,6 +1497,40 @@ struct rglo_gpu {
 				for (int y = tr.y0; y < tr.y1; ++y)
 					for (int x = tr.x0; x < tr.x1; ++x)
 						out[static_cast<size_t>(y) * W + x] = tb.prim[(y - tr.y0) * tileW + (x - tr.x0)];
+				break; }
+			case TileOp::STORE_FP: {  // CMD_STORE_COLOR_{FULL,HALF}_LINEAR_FP, _FULL_QUADS_FP (rglv_gpu.cxx:345-384)
+				const int ct = st->color0AttachmentType;
+				if (ct != RB_COLOR_DEPTH && ct != RB_RGBF32 && ct != RB_RGBAF32) { err = "float store: attachment type"; return false; }
+				auto* out = static_cast<float*>(const_cast<void*>(op.ptr));
+				const long long stride = op.a;
+				auto px = [&](int x, int y) { return (y - tr.y0) * tileW + (x - tr.x0); };
+				if (op.i0 == 0) {  // Copy to a FloatingPointCanvas (rglr_algorithm.cxx:194-280): R, G, B, 0
+					for (int y = tr.y0; y < tr.y1; ++y)
+						for (int x = tr.x0; x < tr.x1; ++x) {
+							float* o = out + 4 * (static_cast<long long>(y) * stride + x);
+							const int p = px(x, y);
+							o[0] = tb.r[p]; o[1] = tb.g[p]; o[2] = tb.b[p]; o[3] = 0.0F; }}
+				else {
+					// per 2 x 2 quad with its top-left pixel in the tile rect; the
+					// quad's pixels in lane order (x, y), (x+1, y), (x, y+1),
+					// (x+1, y+1), read from the whole tile buffer
+					for (int y = tr.y0; y < tr.y1; y += 2)
+						for (int x = tr.x0; x < tr.x1; x += 2) {
+							int p[4];
+							for (int l = 0; l < 4; ++l) p[l] = px(x + (l & 1), y + (l >> 1));
+							const long long q = static_cast<long long>(y / 2) * stride + x / 2;
+							if (op.i0 == 1) {  // Downsample (rglr_algorithm.cxx:118-165)
+								float* o = out + 4 * q;
+								const std::vector<float>* ch[3] = { &tb.r, &tb.g, &tb.b };
+								for (int c = 0; c < 3; ++c) {
+									const auto& v = *ch[c];
+									o[c] = (((v[p[0]] + v[p[1]]) + v[p[2]]) + v[p[3]]) * 0.25F; }
+								o[3] = (((0.0F + 0.0F) + 0.0F) + 0.0F) * 0.25F; }
+							else {  // Copy to a QFloat4Canvas (rglr_algorithm.cxx:320-368)
+								float* o = out + 16 * q;
+								for (int l = 0; l < 4; ++l) {
+									o[l] = tb.r[p[l]]; o[4 + l] = tb.g[p[l]]; o[8 + l] = tb.b[p[l]];
+									o[12 + l] = ct == RB_RGBF32 ? 1.0F : ct == RB_COLOR_DEPTH ? tb.d[p[l]] : tb.a[p[l]]; }}}}
 				break; }}}
 		return true; }
 
@@ -1628,7 +1666,40 @@ void rglo_gl_store_color_tc(rglo_gpu* g, uint32_t* dst, int gamma) {
 	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_TC, gamma ? 1 : 0, 0, 0, dst }); }
 void rglo_gl_store_depth(rglo_gpu* g, float* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_DEPTH, 0, 0, 0, dst }); }
 void rglo_gl_store_primitive_id(rglo_gpu* g, uint32_t* dst) { g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_PRIM, 0, 0, 0, dst }); }
+void rglo_gl_store_color_fp(rglo_gpu* g, float* dst, int stride, int downsample) {
+	g->MaybeUpdateState();
+	g->commands.push_back({ Cmd::STORE_FP, downsample ? 1 : 0, stride > 0 ? stride : downsample ? (g->W + 1) / 2 : g->W, 0, dst }); }
+void rglo_gl_store_color_quads(rglo_gpu* g, float* dst, int stride) {
+	g->MaybeUpdateState();
+	g->commands.push_back({ Cmd::STORE_FP, 2, stride > 0 ? stride : (g->W + 1) / 2, 0, dst }); }
 void rglo_gl_reset(rglo_gpu* g) { g->ResetGL(); }
+
+// rglr::KawaseBlurFilter (rglr_kawase.cxx:21-85) over rows [0, height):
+// clamped sampling near the edges, direct in the interior, as the reference
+// splits it; RGBA float pixels, strides in pixels
+void rglo_kawase_blur(const float* src, float* dst, int width, int height, int srcStride, int dstStride, int d) {
+	const int ox[4] = { -d - 1, d, -d - 1, d }, oy[4] = { -d - 1, -d - 1, d, d };
+	auto at = [&](int x, int y) { return src + 4 * (static_cast<long long>(y) * srcStride + x); };
+	auto clampi = [](int a, int l, int h) { return std::min(std::max(a, l), h); };
+	const int margin = d + 2;
+	const int ySafeBegin = margin, ySafeEnd = height - margin, xSafeBegin = margin, xSafeEnd = width - margin;
+	for (int y = 0; y < height; ++y) {
+		for (int x = 0; x < width; ++x) {
+			const bool safe = y < ySafeBegin || ySafeEnd < y || x < xSafeBegin || x >= xSafeEnd;
+			float ax[4] = { 0.0F, 0.0F, 0.0F, 0.0F };
+			for (int k = 0; k < 4; ++k) {
+				const float* t[4];
+				const int x0 = x + ox[k], y0 = y + oy[k];
+				if (safe) {
+					t[0] = at(clampi(x0, 0, width - 1), clampi(y0, 0, height - 1));
+					t[1] = at(clampi(x0 + 1, 0, width - 1), clampi(y0, 0, height - 1));
+					t[2] = at(clampi(x0, 0, width - 1), clampi(y0 + 1, 0, height - 1));
+					t[3] = at(clampi(x0 + 1, 0, width - 1), clampi(y0 + 1, 0, height - 1)); }
+				else {
+					t[0] = at(x0, y0); t[1] = at(x0 + 1, y0); t[2] = at(x0, y0 + 1); t[3] = at(x0 + 1, y0 + 1); }
+				for (int c = 0; c < 4; ++c) ax[c] = ax[c] + (((t[0][c] + t[1][c]) + t[2][c]) + t[3][c]); }
+			float* o = dst + 4 * (static_cast<long long>(y) * dstStride + x);
+			for (int c = 0; c < 4; ++c) o[c] = ax[c] * (1.0F / 16.0F); }}}
 
 int rglo_gpu_tile_costs(rglo_gpu* g, long long* pairs, long long* quads, int n) {
 	const int m = std::min(n, static_cast<int>(g->tileQuads.size()));

@@ -16,6 +16,7 @@
 #include "../../include/rgl_hip.h"
 #include "rgl_mlp.h"
 #include "rgl_inorder.h"
+#include "rgl_postfx.h"
 
 using namespace rgl;
 
@@ -74,7 +75,7 @@ struct GLState {
 
 struct Cmd {
 	enum Kind { STATE, CLEAR, DRAW_ELEMENTS, DRAW_ARRAYS, DRAW_ELEMENTS_INSTANCED, DRAW_ARRAYS_INSTANCED,
-	            STORE_TC, STORE_DEPTH, STORE_PRIM, STORE_SAMPLES } kind;
+	            STORE_TC, STORE_DEPTH, STORE_PRIM, STORE_SAMPLES, STORE_FP, STORE_QUADS } kind;
 	int a, b;
 	const void* ptr; };
 
@@ -415,7 +416,7 @@ struct rgl_gpu {
 				if ((c.a & GL_DEPTH_BUFFER_BIT) && st->color0AttachmentType != RB_COLOR_DEPTH && st->depthAttachmentType != RB_F32)
 					return fail(RGL_E_UNSUPPORTED, "can't clear depth buffer of this type");
 				op.kind = kOpClear; op.bits = c.a & 3;
-				for (int k = 0; k < 3; ++k) op.color[k] = st->clearColor[k];
+				for (int k = 0; k < 4; ++k) op.color[k] = st->clearColor[k];
 				op.depth = st->clearDepth;
 				if (drawn || stored || clears > 0 || op.bits != 3) ordered = true;
 				++clears;
@@ -455,6 +456,21 @@ struct rgl_gpu {
 				stored = true;
 				ops.push_back(op);
 				break;
+			case Cmd::STORE_FP:      // CMD_STORE_COLOR_{FULL,HALF}_LINEAR_FP (rglv_gpu.cxx:345-370)
+			case Cmd::STORE_QUADS: { // CMD_STORE_COLOR_FULL_QUADS_FP (:371-384)
+				if (!st) return fail(RGL_E_STATE, "store without state");
+				if (st->color0AttachmentType != RB_COLOR_DEPTH && st->color0AttachmentType != RB_RGBF32 &&
+				    st->color0AttachmentType != RB_RGBAF32)
+					return fail(RGL_E_UNSUPPORTED, "float colour store not implemented for attachment type");
+				const bool half = c.kind == Cmd::STORE_FP && c.a != 0;
+				op.kind = c.kind == Cmd::STORE_QUADS ? kOpStoreColorQuads : half ? kOpStoreColorHalf : kOpStoreColorFP;
+				op.ptr = const_cast<void*>(c.ptr);
+				op.stride = c.b > 0 ? c.b : (c.kind == Cmd::STORE_QUADS || half) ? (W + 1) / 2 : W;
+				op.attach = st->color0AttachmentType;
+				ordered = true;   // drawn by the in-order tile path
+				stored = true;
+				ops.push_back(op);
+				break; }
 			case Cmd::STORE_SAMPLES:
 				stored = true; break;
 			default: {
@@ -1327,6 +1343,23 @@ int rgl_gl_store_color_tc(rgl_gpu* g, uint32_t* dst, int gamma) {
 int rgl_gl_store_depth(rgl_gpu* g, float* dst) {
 	if (check(g)) return RGL_E_ARG;
 	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_DEPTH, 0, 0, dst }); return 0; }
+int rgl_gl_store_color_fp(rgl_gpu* g, float* dst, int stride_px, int downsample) {
+	if (check(g)) return RGL_E_ARG;
+	if (dst == nullptr || stride_px < 0) return g->fail(RGL_E_ARG, "bad float colour target");
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_FP, downsample ? 1 : 0, stride_px, dst }); return 0; }
+int rgl_gl_store_color_quads(rgl_gpu* g, float* dst, int stride_quads) {
+	if (check(g)) return RGL_E_ARG;
+	if (dst == nullptr || stride_quads < 0) return g->fail(RGL_E_ARG, "bad quad canvas target");
+	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_QUADS, 0, stride_quads, dst }); return 0; }
+int rgl_kawase_blur(const float* src, float* dst, int width, int height, int src_stride, int dst_stride, int d,
+                    void* hip_stream) {
+	if (!src || !dst || src == dst || width <= 0 || height <= 0 || src_stride < width || dst_stride < width || d < 0)
+		return RGL_E_ARG;
+	const dim3 grid(static_cast<unsigned>((width + 63) / 64), static_cast<unsigned>((height + 3) / 4));
+	hipLaunchKernelGGL(k_kawase, grid, dim3(256), 0, static_cast<hipStream_t>(hip_stream),
+	                   reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), width, height, src_stride,
+	                   dst_stride, d);
+	return hipGetLastError() == hipSuccess ? 0 : RGL_E_HIP; }
 int rgl_gl_store_primitive_id(rgl_gpu* g, uint32_t* dst) {
 	if (check(g)) return RGL_E_ARG;
 	g->MaybeUpdateState(); g->commands.push_back({ Cmd::STORE_PRIM, 0, 0, dst }); return 0; }

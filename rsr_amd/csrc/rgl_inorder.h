@@ -30,17 +30,23 @@ namespace rgl {
 // order: CLEAR and the STORE_* commands that GPU::BinImpl broadcasts to all
 // tiles (rglv_gpu.cxx:139-191).  pos is the first visibility slot it
 // precedes: the triangles drawn before it have smaller slots.
-enum FrameOpKind { kOpClear = 0, kOpStoreColor = 1, kOpStoreDepth = 2, kOpStorePrim = 3 };
+enum FrameOpKind { kOpClear = 0, kOpStoreColor = 1, kOpStoreDepth = 2, kOpStorePrim = 3,
+	kOpStoreColorFP = 4,      // CMD_STORE_COLOR_FULL_LINEAR_FP: RGBA float pixels (A = 0)
+	kOpStoreColorHalf = 5,    // CMD_STORE_COLOR_HALF_LINEAR_FP: 2 x 2 box average (Downsample)
+	kOpStoreColorQuads = 6 }; // CMD_STORE_COLOR_FULL_QUADS_FP: the quad-swizzled qfloat4 canvas
 struct alignas(16) FrameOp {
 	uint32_t pos;
 	int kind;
 	int bits;            // clear: GL_COLOR_BUFFER_BIT (1) | GL_DEPTH_BUFFER_BIT (2)
 	int gamma;           // store colour: sRGB (1) or to_tc_basic (0)
-	float color[3];      // clear colour
+	float color[4];      // clear colour (alpha: RB_RGBAF32's fourth channel)
 	float depth;         // clear depth
 	int post;            // store colour: post program (1 default, 2 exposure, 3 IQ)
 	float exposure;
-	void* ptr;           // store target, row stride = the frame's width
+	int stride;          // float stores: the target's row stride, in pixels (quads for kOpStoreColorQuads)
+	int attach;          // float stores: the colour attachment type (the quad store's alpha source)
+	int pad[3];
+	void* ptr;           // store target; truecolor, depth and id stores have row stride = the frame's width
 };
 
 // the in-order frames of a batch: their first k_tile_order block (one per
@@ -52,6 +58,9 @@ struct alignas(16) IoFrame {
 	int regionsX, regionsY;
 	int pad[3];
 };
+
+// colour attachment types (rglv_gl.hxx RB_COLOR_DEPTH, RB_RGBF32)
+constexpr int kAttachColorDepth = 0, kAttachRGBF32 = 1;
 
 constexpr int kSortCap = 4096;       // records a tile sorts in LDS in one piece
 constexpr int kRegionDim = 32;       // k_inorder region: 32 x 32 pixels, 16 x 16 quads
@@ -199,6 +208,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	// yet cleared; the reference leaves the previous tile's contents)
 	float cr[4] = { 0.0F, 0.0F, 0.0F, 0.0F }, cg[4] = { 0.0F, 0.0F, 0.0F, 0.0F }, cb[4] = { 0.0F, 0.0F, 0.0F, 0.0F };
 	float dz[4] = { 0.0F, 0.0F, 0.0F, 0.0F };
+	float ca[4] = { 0.0F, 0.0F, 0.0F, 0.0F };   // RB_RGBAF32's alpha: set by clears, never drawn
 	uint32_t pid[4] = { 0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu };
 	uint32_t frags = 0;
 	const VisTile vt = tileDesc[tile];
@@ -210,10 +220,35 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 		if (op.kind == kOpClear) {
 			if (op.bits & 1) {
 #pragma unroll
-				for (int l = 0; l < 4; ++l) { cr[l] = op.color[0]; cg[l] = op.color[1]; cb[l] = op.color[2]; pid[l] = 0xffffffffu; }}
+				for (int l = 0; l < 4; ++l) {
+					cr[l] = op.color[0]; cg[l] = op.color[1]; cb[l] = op.color[2]; ca[l] = op.color[3]; pid[l] = 0xffffffffu; }}
 			if (op.bits & 2) {
 #pragma unroll
 				for (int l = 0; l < 4; ++l) dz[l] = op.depth; }}
+		else if (op.kind == kOpStoreColorHalf || op.kind == kOpStoreColorQuads) {
+			// one output per quad whose top-left pixel lies in the tile
+			// rect (the reference's loops step 2 x 2 over the rect)
+			if (!onScreen[0]) return;
+			const long long q = static_cast<long long>(qy >> 1) * op.stride + (qx >> 1);
+			if (op.kind == kOpStoreColorHalf) {
+				// Downsample (rglr_algorithm.cxx:118-165): the transposed
+				// quad's pixels summed in lane order, times 1/4; alpha 0
+				const float z = ((0.0F + 0.0F) + 0.0F) + 0.0F;
+				static_cast<float4*>(op.ptr)[q] = make_float4((((cr[0] + cr[1]) + cr[2]) + cr[3]) * 0.25F,
+				                                              (((cg[0] + cg[1]) + cg[2]) + cg[3]) * 0.25F,
+				                                              (((cb[0] + cb[1]) + cb[2]) + cb[3]) * 0.25F, z * 0.25F); }
+			else {
+				// Copy to a QFloat4Canvas (rglr_algorithm.cxx:320-368): alpha 1
+				// from RB_RGBF32, the fourth channel (RB_COLOR_DEPTH: the depth)
+				// otherwise
+				float4* o = static_cast<float4*>(op.ptr) + 4 * q;
+				float a[4];
+#pragma unroll
+				for (int l = 0; l < 4; ++l) a[l] = op.attach == kAttachRGBF32 ? 1.0F : op.attach == kAttachColorDepth ? dz[l] : ca[l];
+				o[0] = make_float4(cr[0], cr[1], cr[2], cr[3]);
+				o[1] = make_float4(cg[0], cg[1], cg[2], cg[3]);
+				o[2] = make_float4(cb[0], cb[1], cb[2], cb[3]);
+				o[3] = make_float4(a[0], a[1], a[2], a[3]); }}
 		else {
 #pragma unroll
 			for (int l = 0; l < 4; ++l) {
@@ -228,6 +263,9 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 						: ((to_linear8(c[0]) << 16) | (to_linear8(c[1]) << 8) | to_linear8(c[2]));
 					static_cast<uint32_t*>(op.ptr)[o] = px; }
 				else if (op.kind == kOpStoreDepth) static_cast<float*>(op.ptr)[o] = dz[l];
+				else if (op.kind == kOpStoreColorFP)   // Copy to a FloatingPointCanvas (rglr_algorithm.cxx:194-280): alpha 0
+					static_cast<float4*>(op.ptr)[static_cast<long long>(qy + (l >> 1)) * op.stride + qx + (l & 1)] =
+						make_float4(cr[l], cg[l], cb[l], 0.0F);
 				else static_cast<uint32_t*>(op.ptr)[o] = pid[l]; }}};
 
 	for (uint32_t base = 0; base < n; base += 256) {

@@ -76,6 +76,9 @@ def lib():
         L.rgl_gl_store_color_tc.argtypes = [vp, vp, i]
         L.rgl_gl_store_depth.argtypes = [vp, vp]
         L.rgl_gl_store_primitive_id.argtypes = [vp, vp]
+        L.rgl_gl_store_color_fp.argtypes = [vp, vp, i, i]
+        L.rgl_gl_store_color_quads.argtypes = [vp, vp, i]
+        L.rgl_kawase_blur.argtypes = [vp, vp, i, i, i, i, i, vp]
         L.rgl_gl_reset.argtypes = [vp]
         L.rgl_texture_make_mipmap.argtypes = [vp, i]
         L.rgl_gpu_run_batch.argtypes = [vp, ctypes.POINTER(vp), i, vp]
@@ -107,6 +110,7 @@ EXPORTS = [
     "rgl_gl_projection_matrix", "rgl_gl_normal_matrix", "rgl_gl_draw_elements",
     "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",
     "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
+    "rgl_gl_store_color_fp", "rgl_gl_store_color_quads", "rgl_kawase_blur",
     "rgl_gl_reset", "rgl_texture_make_mipmap",
     "rgl_gpu_run_batch", "rgl_gpu_set_graph", "rgl_gpu_graph_captures", "rgl_gpu_set_raster_params",
     "rgl_gpu_set_guard_band",
@@ -114,6 +118,19 @@ EXPORTS = [
     "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
     "rgl_mlp_train_noalloc",
 ]
+
+
+def kawase_blur(src, dst, d: int, stream=None) -> None:
+    """One Kawase pass (rglr_kawase.cxx:21-85) from src into dst: (H, W, 4)
+    float32 CUDA tensors, rows contiguous."""
+    import torch
+    assert src.shape == dst.shape and src.dim() == 3 and src.shape[2] == 4 and src.dtype == torch.float32
+    h, w = int(src.shape[0]), int(src.shape[1])
+    s = stream if stream is not None else torch.cuda.current_stream(src.device).cuda_stream
+    rc = lib().rgl_kawase_blur(src.data_ptr(), dst.data_ptr(), w, h, int(src.stride(0) // 4), int(dst.stride(0) // 4),
+                               int(d), s)
+    if rc != 0:
+        raise RglError(f"rgl_kawase_blur failed ({rc})")
 
 
 def make_mipmap(texels: np.ndarray, dim: int) -> np.ndarray:
@@ -325,16 +342,14 @@ def render_calls(sc, gpu: GPU | None = None, device="cuda:0"):
         gpu = GPU(int(str(device).split(":")[-1]) if ":" in str(device) else 0)
     dev = {k: torch.from_numpy(np.ascontiguousarray(a)).to(device) for k, a in sc.arrays.items()}
     ptr_of = {id(sc.arrays[k]): t.data_ptr() for k, t in dev.items()}
-    outs = []
-    for kind in sc.outputs:
-        dt = torch.float32 if kind == "depth" else torch.int32
-        # outputs start as a pattern the frame must overwrite
-        outs.append(torch.full((sc.height, sc.width), 0x5A5A5A5A if dt == torch.int32 else float("nan"), dtype=dt,
-                               device=device))
+    from .scenes import output_array
+    # outputs start as a pattern the frame must overwrite
+    host = [output_array(kind, sc.width, sc.height) for kind in sc.outputs]
+    outs = [torch.from_numpy(h.view(np.int32) if h.dtype == np.uint32 else h).to(device) for h in host]
     issue(gpu, sc, [o.data_ptr() for o in outs], ptr=lambda a: ptr_of[id(a)])
     stream = torch.cuda.current_stream(device).cuda_stream
     gpu.run_and_wait(stream)
-    res = [o.cpu().numpy().view(np.float32 if k == "depth" else np.uint32) for o, k in zip(outs, sc.outputs)]
+    res = [o.cpu().numpy().view(h.dtype) for o, h in zip(outs, host)]
     st = gpu.stats()
     if own:
         gpu.close()

@@ -668,7 +668,8 @@ class CallScene:
     method the GL member name in snake case ("enable", "draw_elements", ...);
     an argument "@name" is the array arrays[name] (a device pointer on the
     GPU), ("out", k) is output buffer k.  outputs: the kinds of the store
-    targets, "color" (u32 0x00RRGGBB), "depth" (f32) or "prim" (u32)."""
+    targets (output_array: "color", "depth", "prim", "rgba", "rgba_half",
+    "quads")."""
     name: str
     width: int
     height: int
@@ -677,6 +678,24 @@ class CallScene:
     calls: list = field(default_factory=list)
     outputs: list = field(default_factory=list)
     fixed_guard_band: bool = False
+
+
+def output_array(kind: str, width: int, height: int) -> np.ndarray:
+    """A store target of a CallScene output, filled with a pattern every
+    store must overwrite: "color" u32 0x00RRGGBB, "depth" f32, "prim" u32
+    (H x W); "rgba" (H x W x 4 f32, StoreColor(FloatingPointCanvas*,
+    false)); "rgba_half" (ceil(H/2) x ceil(W/2) x 4, the downsampled
+    store); "quads" (ceil(H/2) x ceil(W/2) x 16, StoreColor(QFloat4Canvas*))."""
+    hh, hw = (height + 1) // 2, (width + 1) // 2
+    if kind == "depth":
+        return np.full((height, width), np.nan, np.float32)
+    if kind == "rgba":
+        return np.full((height, width, 4), np.nan, np.float32)
+    if kind == "rgba_half":
+        return np.full((hh, hw, 4), np.nan, np.float32)
+    if kind == "quads":
+        return np.full((hh, hw, 16), np.nan, np.float32)
+    return np.full((height, width), 0x5A5A5A5A, np.uint32)
 
 
 def fragment_state_key_of(st: dict) -> int:

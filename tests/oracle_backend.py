@@ -66,6 +66,9 @@ def lib():
         L.rglo_gl_store_color_tc.argtypes = [vp, vp, i]
         L.rglo_gl_store_depth.argtypes = [vp, vp]
         L.rglo_gl_store_primitive_id.argtypes = [vp, vp]
+        L.rglo_gl_store_color_fp.argtypes = [vp, vp, i, i]
+        L.rglo_gl_store_color_quads.argtypes = [vp, vp, i]
+        L.rglo_kawase_blur.argtypes = [vp, vp, i, i, i, i, i]
         L.rglo_gl_reset.argtypes = [vp]
         L.rglo_oneover.argtypes = [f]
         L.rglo_oneover.restype = f
@@ -191,11 +194,9 @@ def render_calls(sc, threads: int = 4):
     from rsr_amd.scenes import render_calls as issue
     g = OracleGPU(threads)
     outs = []
+    from rsr_amd.scenes import output_array
     for kind in sc.outputs:
-        if kind == "depth":
-            outs.append(np.full((sc.height, sc.width), np.nan, np.float32))
-        else:
-            outs.append(np.full((sc.height, sc.width), 0x5A5A5A5A, np.uint32))
+        outs.append(output_array(kind, sc.width, sc.height))
     issue(g, sc, outs)
     g.run()
     st = g.stats()
