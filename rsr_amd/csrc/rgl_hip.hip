@@ -550,10 +550,11 @@ struct rgl_gpu {
 			for (int k = 0; k < 4; ++k)
 				if (s.buffers[11 + k] && s.bufferCounts[11 + k] < need[k])
 					return fail(RGL_E_ARG, "heavy-shader table buffer too short"); }
-		if (info.instanced) {
-			if (!s.buffers[15] || s.bufferCounts[15] < 16 * instances) return fail(RGL_E_ARG, "instance matrix buffer too short"); }
-		else if (instances != 1) {
-			return fail(RGL_E_UNSUPPORTED, "instanced draw with a non-instanced program"); }
+		// a program that reads no instance data (its LoadInstance does
+		// nothing, rglv_gpu_impl.hxx:363-366) draws every instance in place,
+		// as BaseProgram does in the $layer shadow pass of a $many node
+		if (info.instanced && (!s.buffers[15] || s.bufferCounts[15] < 16 * instances))
+			return fail(RGL_E_ARG, "instance matrix buffer too short");
 		d.verts = verts;
 		d.nvary = nv;
 		d.triBase = triBase;
