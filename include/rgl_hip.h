@@ -167,6 +167,19 @@ int rgl_kawase_blur(const float* dev_src, float* dev_dst, int width, int height,
    texels = dim*dim*2 RGBA floats with the base level in the first dim rows;
    fills the box-filtered mip chain below it. */
 int rgl_texture_make_mipmap(float* host_texels, int dim);
+/* PNGToTexture's texel loop (rglr_texture_load.cxx:39-66) on the device:
+   dev_rgba8 = width*height decoded RGBA8 texels -> dev_rgba = RGBA floats,
+   R, G, B through ryg::srgb8_to_float (ryg-srgb.h:33-56), A = a / 255,
+   RGB times A when premultiply; queued on hip_stream. */
+int rgl_texture_from_rgba8(const uint8_t* dev_rgba8, int width, int height, int premultiply, float* dev_rgba,
+                           void* hip_stream);
+/* rglr::ensurePowerOf2 (rglr_texture.cxx:13-31) on the device: the dim x
+   dim nearest resample (dim = the power of two at or above the long edge)
+   of a width x height RGBA float texture. */
+int rgl_texture_pow2(const float* dev_src, int width, int height, float* dev_dst, int dim, void* hip_stream);
+/* Texture::maybe_make_mipmap on the device: dev_texels = dim*dim*2 RGBA
+   floats, base level in the first dim rows; one launch per level. */
+int rgl_texture_mipmap(float* dev_texels, int dim, void* hip_stream);
 
 /* --- batches ----------------------------------------------------------- */
 /* No counterpart in the reference, which runs one GPU object per canvas

@@ -17,6 +17,7 @@
 #include "rgl_mlp.h"
 #include "rgl_inorder.h"
 #include "rgl_postfx.h"
+#include "rgl_assets.h"
 
 using namespace rgl;
 
@@ -1384,6 +1385,45 @@ int rgl_texture_make_mipmap(float* buf, int dim) {
 		src += srcSize * stride;
 		srcSize >>= 1; }
 	return 0; }
+
+/* ---- the asset path's texture preparation on the device ---- */
+static const Srgb8Table& srgb8_table() {
+	// ryg::srgb8_to_float (ryg-srgb.h:33-56): c = i / 255 in float, c / 12.92
+	// up to 0.04045, else pow of the float base and exponent, to float
+	static Srgb8Table tab = [] {
+		Srgb8Table t;
+		for (int i = 0; i < 256; ++i) {
+			const float c = static_cast<float>(i) * (1.0F / 255.0F);
+			t.v[i] = c <= 0.04045F ? c / 12.92F
+			                       : static_cast<float>(std::pow(static_cast<double>((c + 0.055F) / 1.055F), static_cast<double>(2.4F))); }
+		return t; }();
+	return tab; }
+
+int rgl_texture_from_rgba8(const uint8_t* src, int width, int height, int premultiply, float* dst, void* hip_stream) {
+	if (!src || !dst || width <= 0 || height <= 0) return RGL_E_ARG;
+	const long long n = static_cast<long long>(width) * height;
+	hipLaunchKernelGGL(k_texture_from_rgba8, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+	                   static_cast<hipStream_t>(hip_stream), reinterpret_cast<const uchar4*>(src), reinterpret_cast<float4*>(dst),
+	                   n, premultiply ? 1 : 0, srgb8_table());
+	return hipGetLastError() == hipSuccess ? 0 : RGL_E_HIP; }
+
+int rgl_texture_pow2(const float* src, int width, int height, float* dst, int dim, void* hip_stream) {
+	if (!src || !dst || width <= 0 || height <= 0 || dim < 2 || (dim & (dim - 1)) != 0 || dim > 16384 ||
+	    dim < width || dim < height) return RGL_E_ARG;
+	const long long n = static_cast<long long>(dim) * dim;
+	hipLaunchKernelGGL(k_texture_pow2, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+	                   static_cast<hipStream_t>(hip_stream), reinterpret_cast<const float4*>(src), width, height,
+	                   reinterpret_cast<float4*>(dst), dim);
+	return hipGetLastError() == hipSuccess ? 0 : RGL_E_HIP; }
+
+int rgl_texture_mipmap(float* buf, int dim, void* hip_stream) {
+	if (!buf || dim < 1 || (dim & (dim - 1)) != 0 || dim > 4096) return RGL_E_ARG;
+	int srcRow = 0, dstRow = dim;
+	for (int size = dim / 2; size >= 1; size >>= 1) {
+		hipLaunchKernelGGL(k_mip_level, dim3(static_cast<unsigned>((size * size + 255) / 256)), dim3(256), 0,
+		                   static_cast<hipStream_t>(hip_stream), reinterpret_cast<float4*>(buf), dim, srcRow, dstRow, size);
+		srcRow = dstRow; dstRow += size; }
+	return hipGetLastError() == hipSuccess ? 0 : RGL_E_HIP; }
 
 /* ---- batches, graph replay and raster parameters (no reference counterpart) ---- */
 int rgl_gpu_run_batch(rgl_gpu* g, rgl_gpu* const* frames, int n, void* hip_stream) {

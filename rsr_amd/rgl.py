@@ -81,6 +81,9 @@ def lib():
         L.rgl_kawase_blur.argtypes = [vp, vp, i, i, i, i, i, vp]
         L.rgl_gl_reset.argtypes = [vp]
         L.rgl_texture_make_mipmap.argtypes = [vp, i]
+        L.rgl_texture_from_rgba8.argtypes = [vp, i, i, i, vp, vp]
+        L.rgl_texture_pow2.argtypes = [vp, i, i, vp, i, vp]
+        L.rgl_texture_mipmap.argtypes = [vp, i, vp]
         L.rgl_gpu_run_batch.argtypes = [vp, ctypes.POINTER(vp), i, vp]
         L.rgl_gpu_set_graph.argtypes = [vp, i]
         L.rgl_gpu_graph_captures.argtypes = [vp]
@@ -111,13 +114,47 @@ EXPORTS = [
     "rgl_gl_draw_arrays", "rgl_gl_draw_elements_instanced", "rgl_gl_draw_arrays_instanced",
     "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
     "rgl_gl_store_color_fp", "rgl_gl_store_color_quads", "rgl_kawase_blur",
-    "rgl_gl_reset", "rgl_texture_make_mipmap",
+    "rgl_gl_reset", "rgl_texture_make_mipmap", "rgl_texture_from_rgba8", "rgl_texture_pow2", "rgl_texture_mipmap",
     "rgl_gpu_run_batch", "rgl_gpu_set_graph", "rgl_gpu_graph_captures", "rgl_gpu_set_raster_params",
     "rgl_gpu_set_guard_band",
     "rgl_mlp_param_count", "rgl_mlp_trainable_count", "rgl_gpu_set_learned_shading",
     "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
     "rgl_mlp_train_noalloc",
 ]
+
+
+def texture_from_rgba8(rgba8, premultiply: bool = False, mip: bool = True):
+    """A decoded RGBA8 image (H x W x 4 uint8, host or CUDA tensor or numpy)
+    to the pipeline's texture on the device, as rglr::LoadPNG,
+    ensurePowerOf2 and maybe_make_mipmap prepare it: returns (texels, dim)
+    with texels a (2 dim) x dim x 4 float32 CUDA tensor (base level on top,
+    the mip chain below) when mip, else the dim x dim level alone."""
+    import torch
+    src = torch.as_tensor(np.asarray(rgba8) if isinstance(rgba8, np.ndarray) else rgba8)
+    if not src.is_cuda:
+        src = src.cuda()
+    src = src.contiguous()
+    assert src.dtype == torch.uint8 and src.dim() == 3 and src.shape[2] == 4
+    h, w = int(src.shape[0]), int(src.shape[1])
+    s = torch.cuda.current_stream(src.device).cuda_stream
+    L = lib()
+    f = torch.empty((h, w, 4), dtype=torch.float32, device=src.device)
+    if L.rgl_texture_from_rgba8(src.data_ptr(), w, h, int(premultiply), f.data_ptr(), s) != 0:
+        raise RglError("rgl_texture_from_rgba8 failed")
+    dim = 1 << max(1, (max(w, h) - 1).bit_length())
+    if w == h and w == dim:
+        sq = f
+    else:
+        sq = torch.empty((dim, dim, 4), dtype=torch.float32, device=src.device)
+        if L.rgl_texture_pow2(f.data_ptr(), w, h, sq.data_ptr(), dim, s) != 0:
+            raise RglError("rgl_texture_pow2 failed")
+    if not mip:
+        return sq, dim
+    out = torch.zeros((2 * dim, dim, 4), dtype=torch.float32, device=src.device)
+    out[:dim] = sq
+    if L.rgl_texture_mipmap(out.data_ptr(), dim, s) != 0:
+        raise RglError("rgl_texture_mipmap failed")
+    return out, dim
 
 
 def kawase_blur(src, dst, d: int, stream=None) -> None:
