@@ -155,6 +155,28 @@ def test_config4_1080p_shipped_weights():
     gpu.close()
 
 
+def test_config5_view_learned_shading():
+    # the stress workload's learned leg on one full 3840x2160 view (2M
+    # triangles, depth complexity ~8): coverage, depth and ids identical to
+    # the plain heavy frame and >= 40 dB over covered pixels with fallback
+    import os
+    wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    gpu = rgl.GPU(0)
+    sc = scenes.scene_stress(1)
+    print(f"[{sc.name} learned]", flush=True)
+    pc, pd, pp, pst = _frame(gpu, sc)
+    fc, fd, fp, fst = _frame(gpu, sc, params)
+    cov = pd < 1.0
+    assert cov.sum() > 100000
+    assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
+    q = learned.psnr(pc, fc, cov)
+    print(f"4K stress view: PSNR covered {q:.2f} dB ({fst['fallback_tiles']} of {fst['tiles']} tiles re-shaded)")
+    assert q >= 40.0
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
+
+
 def test_learned_shading_keeps_linear_and_exposure_stores_plain():
     # the network predicts the default post program's sRGB colour: frames
     # stored linear or through the Exposure post shade with the plain
