@@ -942,21 +942,22 @@ struct rgl_gpu {
 	// per-program launches, generated from the program list
 	struct VertexLaunch {
 		rgl_gpu& g; const std::vector<DrawDesc>& draws; const BatchDesc& b; const ZeroArgs& za; unsigned vblocks; long long vtot;
-		int zeroHere;
+		int zeroHere; int varyStride;
 		template <class P> void run() {
 			bool present = false;
 			for (const auto& d : draws) present |= d.program == P::kId;
 			if (!present || vtot <= 0) return;
-			k_vertex<P><<<vblocks, 256, 0, g.stream>>>(g.dDraws.p, b.nDraws, vtot, g.vpos.p, g.vflags.p, g.vvary.p, za, zeroHere);
+			k_vertex<P><<<vblocks, 256, 0, g.stream>>>(g.dDraws.p, b.nDraws, vtot, g.vpos.p, g.vflags.p, g.vvary.p, za, zeroHere,
+			                                           varyStride);
 			zeroHere = 0; }};
 	struct ClipLaunch {
-		rgl_gpu& g; const std::vector<DrawDesc>& draws; const BatchDesc& b;
+		rgl_gpu& g; const std::vector<DrawDesc>& draws; const BatchDesc& b; int varyStride;
 		template <class P> void run() {
 			bool present = false;
 			for (const auto& d : draws) present |= d.program == P::kId;
 			if (!present) return;
 			k_clip<P><<<64, 64, 0, g.stream>>>(g.dDraws.p, b.nDraws, g.clipq.p, g.clipCap, g.cvpos.p, g.cvvary.p, g.fcount.p,
-				g.frect.p, g.clipHist.p, g.dCtr); }};
+				g.frect.p, g.clipHist.p, g.dCtr, varyStride); }};
 	struct ResolveLaunch {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;
 		template <class P> void run() {
@@ -974,8 +975,11 @@ struct rgl_gpu {
 		long long vtot = 0;
 		for (const auto& d : draws) vtot += static_cast<long long>(d.instances) * d.verts;
 		const unsigned vblocks = static_cast<unsigned>((vtot + 255) / 256);
+		// varyings are stored at the batch's largest varying count per vertex
+		int varyStride = 1;
+		for (const auto& d : draws) varyStride = std::max(varyStride, d.nvary);
 		// the first vertex launch also clears the frame's counters
-		VertexLaunch vl{ *this, draws, b, za, vblocks, vtot, 1 };
+		VertexLaunch vl{ *this, draws, b, za, vblocks, vtot, 1, varyStride };
 		for_each_program(vl, AllPrograms{});
 		int zeroHere = vl.zeroHere;
 		if (zeroHere) k_zero_counters<<<std::max(8, std::min((kCountGroups * b.nTiles + 255) / 256, 256)), 256, 0, stream>>>(za);
@@ -986,7 +990,7 @@ struct rgl_gpu {
 		Mark(2);
 		// clipping: the clip queue is complete after setup; the clipped
 		// per-tile counts feed the scan below
-		ClipLaunch cl{ *this, draws, b };
+		ClipLaunch cl{ *this, draws, b, varyStride };
 		for_each_program(cl, AllPrograms{});
 		Mark(3);
 		if (b.nTiles > kScanSplitTiles) {
@@ -1001,7 +1005,7 @@ struct rgl_gpu {
 		RasterArgs ra;
 		ra.draws = dDraws.p; ra.lite = dLite.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
 		ra.tvert = tvert.p; ra.vpos = vpos.p; ra.vvary = vvary.p;
-		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p;
+		ra.clipq = clipq.p; ra.clipqOf = clipqOf.p; ra.cvpos = cvpos.p; ra.cvvary = cvvary.p; ra.varyStride = varyStride;
 		ra.clipOff = clipOff.p; ra.clistCap = clistCap; ra.clipCap = clipCap; ra.ctr = dCtr;
 		ra.keys = keys.p; ra.tileCost = tileCost.p;
 		// plain frames end with k_resolve, which publishes the counters itself

@@ -336,17 +336,17 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 			const float* vary;
 			if (rel < nTri) {
 				srcTri = rel;
-				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr);
+				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr, a.varyStride);
 				vary = a.vvary; }
 			else {
 				const uint32_t fslot = rel - nTri;
 				srcTri = fslot / kFanMax;
 				const uint32_t jf = fslot - srcTri * kFanMax;
 				const uint32_t cq = a.clipqOf[dl.triBase + srcTri];
-				clipped_verts(cq * kFanMax + jf, a.cvpos, tr);
+				clipped_verts(cq * kFanMax + jf, a.cvpos, tr, a.varyStride);
 				vary = a.cvvary; }
 			float col[4][4];
-			ShadeQuad sq{ dd, tr.w, vary + tr.vary[0] * kMaxVary, vary + tr.vary[1] * kMaxVary, vary + tr.vary[2] * kMaxVary,
+			ShadeQuad sq{ dd, tr.w, vary + tr.vary[0], vary + tr.vary[1], vary + tr.vary[2],
 			              bx, by, bz, fd, qx, qy, f.H, tr.clipped, col, 0xfu, true };
 			dispatch_program(dd.program, sq, AllPrograms{});
 			mask &= sq.keep;   // gl_triMask after ShadeFragment

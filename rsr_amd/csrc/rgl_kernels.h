@@ -153,7 +153,7 @@ __device__ inline void zero_frame(const ZeroArgs& z) {
 // one transformed vertex: ShadeVertex, clip flags, pdiv, viewport
 template <class P>
 __device__ inline void vertex_one(const DrawDesc& d, long long gi, float4* __restrict__ vpos, uint8_t* __restrict__ vflags,
-                                  float* __restrict__ vvary) {
+                                  float* __restrict__ vvary, int varyStride) {
 	if (d.program != P::kId) return;
 	const long long i = gi - d.vBase;
 	const int iid = static_cast<int>(i / d.verts);
@@ -174,13 +174,13 @@ __device__ inline void vertex_one(const DrawDesc& d, long long gi, float4* __res
 	const float dcy = (pos[1] * iw) * d.DS[1] + d.DO[1];
 	vpos[gi] = make_float4(dcx, dcy, pos[2] * iw, iw);
 	vflags[gi] = static_cast<uint8_t>(f);
-	float* out = vvary + gi * kMaxVary;
+	float* out = vvary + gi * varyStride;
 	for (int k = 0; k < P::kVary; ++k) out[k] = vo[k]; }
 
 template <class P>
 __global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ draws, int nDraws, long long vtot,
                                                  float4* __restrict__ vpos, uint8_t* __restrict__ vflags, float* __restrict__ vvary,
-                                                 ZeroArgs z, int zeroHere) {
+                                                 ZeroArgs z, int zeroHere, int varyStride) {
 	if (zeroHere) zero_frame(z);
 	const long long gi = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
 	if (gi >= vtot) return;
@@ -189,8 +189,8 @@ __global__ __launch_bounds__(256) void k_vertex(const DrawDesc* __restrict__ dra
 	// a wave whose vertices all belong to one draw reads that draw's
 	// descriptor through scalar loads
 	const int d0 = __builtin_amdgcn_readfirstlane(lo);
-	if (__all(lo == d0)) vertex_one<P>(draws[d0], gi, vpos, vflags, vvary);
-	else vertex_one<P>(draws[lo], gi, vpos, vflags, vvary); }
+	if (__all(lo == d0)) vertex_one<P>(draws[d0], gi, vpos, vflags, vvary, varyStride);
+	else vertex_one<P>(draws[lo], gi, vpos, vflags, vvary, varyStride); }
 
 // ---------------------------------------------------------------------------
 // triangle setup + per-unit tile counts: the primitive loop of
@@ -576,7 +576,8 @@ __device__ inline float plane_dist(int plane, const float4& p) {
 template <class P>
 __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws, int nDraws,
 		const uint32_t* __restrict__ clipq, uint32_t clipCap, float4* __restrict__ cvpos, float* __restrict__ cvvary,
-		uint8_t* __restrict__ fcount, uint2* __restrict__ frect, uint32_t* __restrict__ clipHist, Counters* __restrict__ ctr) {
+		uint8_t* __restrict__ fcount, uint2* __restrict__ frect, uint32_t* __restrict__ clipHist, Counters* __restrict__ ctr,
+		int varyStride) {
 	const uint32_t nq = min(ctr->clipCount, clipCap);
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
 		const long long s = clipq[q];
@@ -639,7 +640,7 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 		for (int k = 0; k < na; ++k) {
 			const CV& src = backfacing ? A[na - 1 - k] : A[k];
 			cvpos[static_cast<long long>(q) * kClipVerts + k] = src.c;
-			float* o = cvvary + (static_cast<long long>(q) * kClipVerts + k) * kMaxVary;
+			float* o = cvvary + (static_cast<long long>(q) * kClipVerts + k) * varyStride;
 			for (int j = 0; j < P::kVary; ++j) o[j] = src.v[j]; }
 		const int nf = na - 2;
 		fcount[q] = static_cast<uint8_t>(nf);
@@ -755,23 +756,23 @@ __device__ inline Rect tile_rect(const FrameDesc& f, int tile) {
 struct TriVerts {
 	int32_t X[3], Y[3];
 	float z[3], w[3];
-	long long vary[3];   // index of each vertex's varyings (units of kMaxVary floats)
+	long long vary[3];   // offset of each vertex's varyings, in floats (index times the batch's varying stride)
 	bool clipped;
 	int draw;
 	uint32_t prim;
 };
 
-__device__ inline void inline_verts(const uint4 tv, const float4* __restrict__ vpos, TriVerts& tr) {
+__device__ inline void inline_verts(const uint4 tv, const float4* __restrict__ vpos, TriVerts& tr, int varyStride = 0) {
 	const uint32_t g[3] = { tv.x, tv.y, tv.z };
 	for (int k = 0; k < 3; ++k) {
 		const float4 p = vpos[g[k]];
 		tr.X[k] = cvtt(16.0F * p.x);   // GPUTileImpl::DrawTriangles (:885-886)
 		tr.Y[k] = cvtt(16.0F * p.y);
 		tr.z[k] = p.z; tr.w[k] = p.w;
-		tr.vary[k] = g[k]; }
+		tr.vary[k] = static_cast<long long>(g[k]) * varyStride; }
 	tr.clipped = false; }
 
-__device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cvpos, TriVerts& tr) {
+__device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cvpos, TriVerts& tr, int varyStride = 0) {
 	const uint32_t q = fan / kFanMax, j = fan - q * kFanMax;
 	const long long v[3] = { static_cast<long long>(q) * kClipVerts, static_cast<long long>(q) * kClipVerts + j + 1,
 	                         static_cast<long long>(q) * kClipVerts + j + 2 };
@@ -780,7 +781,7 @@ __device__ inline void clipped_verts(uint32_t fan, const float4* __restrict__ cv
 		tr.X[k] = cvtt(p.x * 16.0F);   // DrawClipped (:997-998)
 		tr.Y[k] = cvtt(p.y * 16.0F);
 		tr.z[k] = p.z; tr.w[k] = p.w;
-		tr.vary[k] = v[k]; }
+		tr.vary[k] = v[k] * varyStride; }
 	tr.clipped = true; }
 
 struct RasterArgs {
@@ -795,6 +796,7 @@ struct RasterArgs {
 	const uint32_t* clipqOf;
 	const float4* cvpos;
 	const float* cvvary;
+	int varyStride;             // floats per vertex in vvary and cvvary (the batch's largest varying count)
 	const uint32_t* clipOff;    // clipped records: tile t's fans are [clipOff[t], clipOff[t + 1])
 	uint32_t clistCap;
 	uint32_t clipCap;
@@ -1105,7 +1107,7 @@ struct ShadeOne {
 	__device__ static void load(const float* __restrict__ vary, const TriVerts& tr, Vary& out) {
 #pragma unroll
 		for (int i = 0; i < 3; ++i) {
-			const float* src = vary + tr.vary[i] * kMaxVary;
+			const float* src = vary + tr.vary[i];
 #pragma unroll
 			for (int k = 0; k < P::kVary; ++k) out.v[i][k] = src[k]; }}
 	__device__ static void shade(const DrawDesc& d, const TriSetup& t, const TriVerts& tr, const Vary& vy,
@@ -1116,7 +1118,7 @@ struct ShadeOne {
 struct ShadeAny {
 	struct Vary { const float* v[3]; };
 	__device__ static void load(const float* __restrict__ vary, const TriVerts& tr, Vary& out) {
-		for (int i = 0; i < 3; ++i) out.v[i] = vary + tr.vary[i] * kMaxVary; }
+		for (int i = 0; i < 3; ++i) out.v[i] = vary + tr.vary[i]; }
 	struct Call {
 		const DrawDesc& d; const TriSetup& t; const TriVerts& tr; const Vary& vy;
 		int kx, ky, lane; const uint32_t* rcpTab; float* color; float& depth; int h;
@@ -1157,7 +1159,7 @@ __device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, con
 	typename S::Vary vy;
 	if (rel < nTri) {
 		srcTri = rel;
-		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr);
+		inline_verts(a.tvert[d.triBase + rel], a.vpos, tr, a.varyStride);
 		S::load(a.vvary, tr, vy);
 		setup_vector(R, tr.X, tr.Y, t); }
 	else {
@@ -1165,7 +1167,7 @@ __device__ inline void resolve_draw(const FrameDesc& f, const RasterArgs& a, con
 		srcTri = fslot / kFanMax;
 		const uint32_t j = fslot - srcTri * kFanMax;
 		const uint32_t cq = a.clipqOf[d.triBase + srcTri];
-		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+		clipped_verts(cq * kFanMax + j, a.cvpos, tr, a.varyStride);
 		S::load(a.cvvary, tr, vy);
 		setup_scalar(R, tr.X, tr.Y, t); }
 	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;

@@ -282,7 +282,7 @@ __device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, co
 	const float* vary;
 	if (rel < nTri) {
 		srcTri = rel;
-		inline_verts(a.tvert[dd.triBase + rel], a.vpos, tr);
+		inline_verts(a.tvert[dd.triBase + rel], a.vpos, tr, a.varyStride);
 		setup_vector(R, tr.X, tr.Y, t);
 		vary = a.vvary; }
 	else {
@@ -290,7 +290,7 @@ __device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, co
 		srcTri = fslot / kFanMax;
 		const uint32_t j = fslot - srcTri * kFanMax;
 		const uint32_t cq = a.clipqOf[dd.triBase + srcTri];
-		clipped_verts(cq * kFanMax + j, a.cvpos, tr);
+		clipped_verts(cq * kFanMax + j, a.cvpos, tr, a.varyStride);
 		setup_scalar(R, tr.X, tr.Y, t);
 		vary = a.cvvary; }
 	const int kx = ((x & ~1) - t.minx) >> 1, ky = ((y & ~1) - t.miny) >> 1;
@@ -301,9 +301,9 @@ __device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, co
 	const int32_t q2 = wadd(wadd(t.c2, wmul(kx, wmul(t.dy23, 2))), wmul(ky, wmul(t.dx23, 2)));
 	const int32_t e1 = (lane == 0) ? q1 : (lane == 1) ? wadd(q1, t.dy12) : (lane == 2) ? wadd(q1, t.dx12) : wadd(wadd(q1, t.dx12), t.dy12);
 	const int32_t e2 = (lane == 0) ? q2 : (lane == 1) ? wadd(q2, t.dy23) : (lane == 2) ? wadd(q2, t.dx23) : wadd(wadd(q2, t.dx23), t.dy23);
-	const float* v0 = vary + tr.vary[0] * kMaxVary;
-	const float* v1 = vary + tr.vary[1] * kMaxVary;
-	const float* v2 = vary + tr.vary[2] * kMaxVary;
+	const float* v0 = vary + tr.vary[0];
+	const float* v1 = vary + tr.vary[1];
+	const float* v2 = vary + tr.vary[2];
 	const LaneBary own = lane_bary(t, tr, e1, e2, rcpTab);
 	dep = own.depth;
 	float vv[P::kVary];
