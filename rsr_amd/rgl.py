@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "librgl_hip.so")
+# RGL_HIP_LIB names another build of the library (for comparing builds)
+LIB_PATH = os.environ.get("RGL_HIP_LIB") or os.path.join(PKG, "librgl_hip.so")
 
 RGL_E_CAPACITY = -5
 
