@@ -73,8 +73,28 @@ int rgl_gpu_reset(rgl_gpu* gpu, int width, int height, int tile_blocks_x, int ti
    stream).  Completion is observed with rgl_gpu_wait. */
 int rgl_gpu_run(rgl_gpu* gpu, void* hip_stream);
 
-/* Blocks until the last Run has completed and checks its device counters. */
+/* Blocks until the last Run has completed and checks its device counters
+   (double-buffered: every Run in flight; the stats are the last one's). */
 int rgl_gpu_wait(rgl_gpu* gpu);
+
+/* rglv::doubleBuffer   rglv_gpu.cxx:16,96-116, rglv_gpu.hxx:176-192: with
+   enable = 1, frame N's binning runs beside frame N-1's drawing.  The
+   reference flag is global to every GPU and lags the image by one Run;
+   here it is per GPU and Run draws its own frame: consecutive Runs go
+   round two executors (enable = 1; 2 to 4: that many frames in flight),
+   each with its own per-frame buffers and a stream of the GPU's own.  A
+   frame starts after the work queued on hip_stream before its Run; its
+   first kernel that writes or samples caller memory waits for the previous
+   frame's end, so frames store in submission order.  The caller's stream
+   does not wait for the frame: read the outputs after rgl_gpu_wait.  What a
+   frame's vertex stage reads must not be written by the frame before it.
+   Reset does not wait for frames in flight.  A per-frame capacity overflow
+   is reported by the next rgl_gpu_wait (the capacities are grown; the
+   frame must be recorded and run again).  Learned shading, training
+   samples, timing, graph replay and batches run single-buffered
+   (RGL_E_UNSUPPORTED).  Switching waits for every frame in flight.
+   enable = 0 returns to single-buffered runs. */
+int rgl_gpu_set_double_buffer(rgl_gpu* gpu, int enable);
 
 /* GPUStats of the last completed Run. */
 int rgl_gpu_stats(rgl_gpu* gpu, rgl_stats* out);

@@ -291,6 +291,18 @@ struct rgl_gpu {
 	hipStream_t auxStream = nullptr;
 	hipEvent_t auxDone = nullptr;
 	float* scratchDev = nullptr;    // this device's address of `scratch` (set up by rgl_gpu_create)
+	// double-buffered runs (rgl_gpu_set_double_buffer): further executors
+	// with their own per-frame buffers; frames go round the executors, each
+	// on an engine stream, so that one frame's binning overlaps the previous
+	// frame's raster while the frames' stores stay in submission order
+	bool dbMode = false;
+	static constexpr int kMaxExec = 4;
+	int nExec = 1;
+	rgl_gpu* exec[kMaxExec] = {};   // exec[0] is this object
+	hipStream_t dbStream[kMaxExec] = {};
+	hipEvent_t dbIn = nullptr;      // recorded on the caller's stream at Run
+	int dbNext = 0, dbLast = -1;    // executor of the next and of the last frame
+	hipEvent_t startAfter = nullptr, rasterAfter = nullptr;   // waits of the frame being launched
 	rgl_stats stats{};
 	long long submitted = 0;
 	int lastCommands = 0;
@@ -845,7 +857,83 @@ struct rgl_gpu {
 		pending = true;
 		return 0; }
 
-	int Run(hipStream_t s) { rgl_gpu* self = this; return RunBatch(&self, 1, s); }
+	int Run(hipStream_t s) {
+		if (dbMode) return RunDouble(s);
+		rgl_gpu* self = this;
+		return RunBatch(&self, 1, s); }
+
+	// GPU::RunImpl with doubleBuffer (rglv_gpu.cxx:96-116): frame N's
+	// binning runs beside frame N-1's drawing.  Here frame N goes to the
+	// executor frame N-1 did not use: its kernels start after the work the
+	// caller queued on s before Run, and its first kernel that writes or
+	// samples caller memory (in-order tiles, resolve) waits for frame N-1's
+	// end.  The caller's stream does not wait for the frame: rgl_gpu_wait does.
+	int RunDouble(hipStream_t s) {
+		if (mlpParams || !sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "learned shading and training samples run single-buffered");
+		if (timing || graphMode) return fail(RGL_E_UNSUPPORTED, "double-buffered frames launch eagerly and untimed");
+		// a stream wait costs the host about as much as a kernel launch: none
+		// for a caller stream that has finished its work, or for a previous
+		// frame that has ended
+		hipError_t e = hipStreamQuery(s);
+		const bool callerBusy = e != hipSuccess;
+		if (callerBusy && e != hipErrorNotReady) return hipfail(e, "hipStreamQuery");
+		if (callerBusy && (e = hipEventRecord(dbIn, s)) != hipSuccess) return hipfail(e, "hipEventRecord");
+		rgl_gpu* x = exec[dbNext];
+		rgl_gpu* o = dbLast >= 0 ? exec[dbLast] : nullptr;
+		x->segPairs = segPairs; x->splitQuads = splitQuads;
+		x->startAfter = callerBusy ? dbIn : nullptr;
+		x->rasterAfter = o && o != x && o->pending && hipEventQuery(o->done) == hipErrorNotReady ? o->done : nullptr;
+		rgl_gpu* self = this;
+		const int r = x->RunBatch(&self, 1, dbStream[dbNext]);
+		x->startAfter = x->rasterAfter = nullptr;
+		if (r != 0) { if (x != this) err = x->err; return r; }
+		dbLast = dbNext;
+		dbNext = (dbNext + 1) % nExec;
+		return 0; }
+
+	// every frame in flight; the stats are the last frame's.  A capacity
+	// grown by one executor is grown in the other too.
+	int WaitAll() {
+		if (!dbMode) return Wait();
+		int rc = 0;
+		std::string e0;
+		for (int k = 0; k < nExec; ++k) {
+			const int r = exec[k]->Wait();
+			if (r != 0 && rc == 0) { rc = r; e0 = exec[k]->err; }}
+		if (dbLast > 0) { stats = exec[dbLast]->stats; lastWorkItems = exec[dbLast]->lastWorkItems; }
+		for (int k = 1; k < nExec; ++k) {
+			listCap = std::max(listCap, exec[k]->listCap);
+			clipCap = std::max(clipCap, exec[k]->clipCap);
+			clistCap = std::max(clistCap, exec[k]->clistCap); }
+		for (int k = 1; k < nExec; ++k) { exec[k]->listCap = listCap; exec[k]->clipCap = clipCap; exec[k]->clistCap = clistCap; }
+		if (rc != 0) err = e0;
+		return rc; }
+
+	int BeginDouble(int n) {
+		hipError_t e;
+		nExec = n;
+		exec[0] = this;
+		for (int k = 1; k < n; ++k) {
+			auto* t = new rgl_gpu();
+			t->device = device; t->guid = guid + "/" + std::to_string(k); t->scratchDev = scratchDev;
+			exec[k] = t; }
+		for (int k = 0; k < n; ++k)
+			if ((e = hipStreamCreateWithFlags(&dbStream[k], hipStreamNonBlocking)) != hipSuccess) { EndDouble(); return hipfail(e, "hipStreamCreate"); }
+		if ((e = hipEventCreateWithFlags(&dbIn, hipEventDisableTiming)) != hipSuccess) { EndDouble(); return hipfail(e, "hipEventCreate"); }
+		dbNext = 0; dbLast = -1;
+		dbMode = true;
+		return 0; }
+
+	// after WaitAll
+	void EndDouble() {
+		for (int k = 1; k < kMaxExec; ++k) { delete exec[k]; exec[k] = nullptr; }
+		exec[0] = nullptr;
+		nExec = 1;
+		for (hipStream_t& st : dbStream) { if (st) (void)hipStreamDestroy(st); st = nullptr; }
+		if (dbIn) (void)hipEventDestroy(dbIn);
+		dbIn = nullptr;
+		stream = nullptr;
+		dbMode = false; }
 
 	// The network entry points run on their caller's stream ms and share the
 	// network image and training scratch with nothing a frame writes except
@@ -970,6 +1058,9 @@ struct rgl_gpu {
 		hipError_t e;
 		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
 		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, tileCount.p, qCount.p };
+		// a double-buffered frame reads the caller's buffers only after the
+		// work the caller queued before Run
+		if (startAfter && (e = hipStreamWaitEvent(stream, startAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 		Mark(0);
 		// vertex and clip stages: one launch per program present in the batch
 		long long vtot = 0;
@@ -1024,13 +1115,18 @@ struct rgl_gpu {
 			Dispatch(0, k_vis, dim3(std::min(items, 1 << 20)), dim3(kRasterThreads), 0, b, ra,
 			         static_cast<const VisRec*>(rec.p), static_cast<const VisRec*>(crec.p), static_cast<const uint32_t*>(itemTile.p),
 			         static_cast<const VisTile*>(tileDesc.p), b.frames); }
-		if (nIo > 0) {
-			// frames drawn in order: the tiles' records sorted by slot, then
-			// every tile region walks them (before k_resolve, whose first
-			// block publishes the frame's counters)
+		// frames drawn in order: the tiles' records sorted by slot, then
+		// every tile region walks them (before k_resolve, whose first block
+		// publishes the frame's counters)
+		if (nIo > 0)
 			k_tile_order<<<nIoTiles, 256, 0, stream>>>(b, dIo.p, nIo, b.frames, tileDesc.p, rec.p, crec.p, order.p, skeys0.p, skeys1.p);
+		// everything above writes this executor's buffers alone; what
+		// follows writes or samples the caller's, after the previous
+		// double-buffered frame has finished
+		if (rasterAfter && (e = hipStreamWaitEvent(stream, rasterAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
+		if (nIo > 0)
 			k_inorder<<<nIoBlocks, 256, 0, stream>>>(b, ra, dIo.p, nIo, b.frames, dOps.p, dDraws.p, dLite.p, tileDesc.p, rec.p, crec.p,
-				order.p); }
+				order.p);
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		const DrawLite* cLite = dLite.p;
@@ -1185,7 +1281,8 @@ int rgl_gpu_create(int device_ordinal, int concurrency, const char* guid, rgl_gp
 
 int rgl_gpu_destroy(rgl_gpu* g) {
 	if (!g) return RGL_E_ARG;
-	if (g->pending) (void)g->Wait();
+	(void)g->WaitAll();
+	if (g->dbMode) g->EndDouble();
 	if (g->auxPending) (void)hipEventSynchronize(g->auxDone);
 	delete g;
 	return 0; }
@@ -1207,7 +1304,9 @@ int rgl_gpu_reset(rgl_gpu* g, int width, int height, int tbx, int tby) {
 	if (tbx * 8 > kMaxAnyTileDim || tby * 8 > kMaxAnyTileDim)
 		return g->fail(RGL_E_UNSUPPORTED, "tiles larger than 128x128 pixels are not supported");
 	if (width > 65535 * tbx * 8 || height > 65535 * tby * 8) return g->fail(RGL_E_ARG, "target too large");
-	if (g->pending) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
+	// a double-buffered frame in flight holds copies of what it draws: the
+	// next frame records while it runs
+	if (g->pending && !g->dbMode) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
 	g->W = width; g->H = height; g->tileW = tbx * 8; g->tileH = tby * 8;
 	g->tilesX = (width + g->tileW - 1) / g->tileW; g->tilesY = (height + g->tileH - 1) / g->tileH;
 	g->ResetGL();
@@ -1218,7 +1317,17 @@ int rgl_gpu_run(rgl_gpu* g, void* hip_stream) {
 	if (g->W <= 0) return g->fail(RGL_E_STATE, "Run before Reset");
 	return g->Run(static_cast<hipStream_t>(hip_stream)); }
 
-int rgl_gpu_wait(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; return g->Wait(); }
+int rgl_gpu_wait(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; return g->WaitAll(); }
+
+int rgl_gpu_set_double_buffer(rgl_gpu* g, int enable) {
+	if (check(g)) return RGL_E_ARG;
+	if (enable < 0 || enable > rgl_gpu::kMaxExec) return g->fail(RGL_E_ARG, "enable is 0, 1, or 2 to 4 frames in flight");
+	const int n = enable == 0 ? 1 : std::max(enable, 2);
+	if (n == g->nExec && g->dbMode == (n > 1)) return 0;
+	const int r = g->WaitAll();
+	if (r != 0 && r != RGL_E_CAPACITY) return r;
+	if (g->dbMode) g->EndDouble();
+	return n > 1 ? g->BeginDouble(n) : 0; }
 
 int rgl_gpu_stats(rgl_gpu* g, rgl_stats* out) {
 	if (check(g) || !out) return RGL_E_ARG;
@@ -1433,11 +1542,12 @@ int rgl_texture_mipmap(float* buf, int dim, void* hip_stream) {
 int rgl_gpu_run_batch(rgl_gpu* g, rgl_gpu* const* frames, int n, void* hip_stream) {
 	if (check(g)) return RGL_E_ARG;
 	if (!frames) return g->fail(RGL_E_ARG, "null frame list");
+	if (g->dbMode) return g->fail(RGL_E_UNSUPPORTED, "batches run single-buffered");
 	return g->RunBatch(frames, n, static_cast<hipStream_t>(hip_stream)); }
 
 int rgl_gpu_set_graph(rgl_gpu* g, int enable) {
 	if (check(g)) return RGL_E_ARG;
-	if (g->pending) { const int r = g->Wait(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
+	if (g->pending || g->dbMode) { const int r = g->WaitAll(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
 	g->graphMode = enable != 0;
 	g->graphValid = false;
 	return 0; }
