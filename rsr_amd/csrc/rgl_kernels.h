@@ -386,6 +386,7 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 	uint32_t carry = 0, ccarry = 0;
 #pragma unroll
 	for (int it = 0; it < kPer; ++it) {
+		if (it * kScanThreads >= nT) break;   // uniform: no tile left for any thread
 		const int t = it * kScanThreads + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
 		uint32_t grp[kCountGroups];
@@ -565,19 +566,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(BatchDesc b, const 
 // Sutherland-Hodgman against L, B, N, R, T, fan triangulation, per-tile
 // counts (ForEachCoveredTile, :795-817).
 // ---------------------------------------------------------------------------
-__device__ inline float plane_dist(int plane, const float4& p) {
-	switch (plane) {
-	case 0: return p.w + p.x;
-	case 1: return p.w + p.y;
-	case 2: return p.w + p.z;
-	case 3: return p.w - p.x;
-	default: return p.w - p.y; }}
-
 template <class P>
 __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws, int nDraws,
 		const uint32_t* __restrict__ clipq, uint32_t clipCap, float4* __restrict__ cvpos, float* __restrict__ cvvary,
 		uint8_t* __restrict__ fcount, uint2* __restrict__ frect, uint32_t* __restrict__ clipHist, Counters* __restrict__ ctr,
 		int varyStride) {
+	// the polygon being clipped and the next one, per lane in LDS (component
+	// c of vertex k at [k * kComp + c][lane]); dynamically indexed arrays
+	// in registers would live in scratch memory
+	constexpr int kComp = 4 + P::kVary;
+	__shared__ float poly[2][kClipVerts * kComp][64];
+	const int lane = threadIdx.x;
+	auto V = [&](int buf, int k, int c) -> float& { return poly[buf][k * kComp + c][lane]; };
+	auto dist = [&](int buf, int plane, int k) {
+		const float w = V(buf, k, 3);
+		switch (plane) {
+		case 0: return w + V(buf, k, 0);
+		case 1: return w + V(buf, k, 1);
+		case 2: return w + V(buf, k, 2);
+		case 3: return w - V(buf, k, 0);
+		default: return w - V(buf, k, 1); }};
 	const uint32_t nq = min(ctr->clipCount, clipCap);
 	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
 		const long long s = clipq[q];
@@ -588,68 +596,68 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 		const long long n = s - d.triBase;
 		const int iid = static_cast<int>(n / d.tris);
 		const int t = static_cast<int>(n - static_cast<long long>(iid) * d.tris);
-		struct CV { float4 c; float v[P::kVary]; };
-		CV A[kClipVerts], B[kClipVerts];
-		int na = 3;
 		for (int k = 0; k < 3; ++k) {
 			const int idx = d.elements ? d.indices[3 * t + k] : 3 * t + k;
-			float pos[4];
-			P::vertex(d, iid, idx, pos, A[k].v);
-			A[k].c = make_float4(pos[0], pos[1], pos[2], pos[3]); }
+			float pos[4], vo[P::kVary];
+			P::vertex(d, iid, idx, pos, vo);
+			for (int c = 0; c < 4; ++c) V(0, k, c) = pos[c];
+			for (int j = 0; j < P::kVary; ++j) V(0, k, 4 + j) = vo[j]; }
 		// all three vertices outside one clipping plane: Sutherland-Hodgman
 		// would emit an empty polygon for that plane
 		bool rejected = false;
 		for (int plane = 0; plane < 5; ++plane)
-			rejected |= plane_dist(plane, A[0].c) < 0.0F && plane_dist(plane, A[1].c) < 0.0F && plane_dist(plane, A[2].c) < 0.0F;
+			rejected |= dist(0, plane, 0) < 0.0F && dist(0, plane, 1) < 0.0F && dist(0, plane, 2) < 0.0F;
 		if (rejected) { fcount[q] = 0; continue; }
+		int cur = 0, na = 3;
 		for (int plane = 0; plane < 5 && na > 0; ++plane) {
-			bool hereIsInside = plane_dist(plane, A[0].c) >= 0.0F;
+			const int nxt = cur ^ 1;
+			bool hereIsInside = dist(cur, plane, 0) >= 0.0F;
 			int nb = 0;
 			for (int h = 0; h < na; ++h) {
-				const int nx = (h + 1) % na;
-				const bool nextIsInside = plane_dist(plane, A[nx].c) >= 0.0F;
-				if (hereIsInside) B[nb++] = A[h];
+				const int nx = h + 1 == na ? 0 : h + 1;
+				const bool nextIsInside = dist(cur, plane, nx) >= 0.0F;
+				if (hereIsInside) {
+					for (int c = 0; c < kComp; ++c) V(nxt, nb, c) = V(cur, h, c);
+					++nb; }
 				if (hereIsInside != nextIsInside) {
-					const CV& from = hereIsInside ? A[h] : A[nx];
-					const CV& to = hereIsInside ? A[nx] : A[h];
-					const float fa = plane_dist(plane, from.c), fb = plane_dist(plane, to.c);
+					const int from = hereIsInside ? h : nx;
+					const int to = hereIsInside ? nx : h;
+					const float fa = dist(cur, plane, from), fb = dist(cur, plane, to);
 					const float dd = fa / (fa - fb);
 					const float om = 1.0F - dd;
-					CV& o = B[nb++];
-					o.c = make_float4(om * from.c.x + dd * to.c.x, om * from.c.y + dd * to.c.y,
-					                  om * from.c.z + dd * to.c.z, om * from.c.w + dd * to.c.w);
-					for (int k = 0; k < P::kVary; ++k) o.v[k] = om * from.v[k] + dd * to.v[k];
+					for (int c = 0; c < kComp; ++c) V(nxt, nb, c) = om * V(cur, from, c) + dd * V(cur, to, c);
+					++nb;
 					hereIsInside = !hereIsInside; }}
 			na = nb;
-			for (int k = 0; k < na; ++k) A[k] = B[k]; }
+			cur = nxt; }
 		if (na == 0) { fcount[q] = 0; continue; }
 		for (int k = 0; k < na; ++k) {
-			const float iw = oneover(A[k].c.w);
-			float x = A[k].c.x * iw, y = A[k].c.y * iw;
-			const float z = A[k].c.z * iw;
+			const float iw = oneover(V(cur, k, 3));
+			float x = V(cur, k, 0) * iw, y = V(cur, k, 1) * iw;
+			const float z = V(cur, k, 2) * iw;
 			x = x * d.DS[0] + d.DO[0];
 			y = y * d.DS[1] + d.DO[1];
-			A[k].c = make_float4(x, y, z, iw); }
-		const float d31x = A[2].c.x - A[0].c.x, d31y = A[2].c.y - A[0].c.y;
-		const float d21x = A[1].c.x - A[0].c.x, d21y = A[1].c.y - A[0].c.y;
+			V(cur, k, 0) = x; V(cur, k, 1) = y; V(cur, k, 2) = z; V(cur, k, 3) = iw; }
+		const float d31x = V(cur, 2, 0) - V(cur, 0, 0), d31y = V(cur, 2, 1) - V(cur, 0, 1);
+		const float d21x = V(cur, 1, 0) - V(cur, 0, 0), d21y = V(cur, 1, 1) - V(cur, 0, 1);
 		const bool backfacing = (d31x*d21y - d31y*d21x) < 0.0F;
 		bool willCull = true;
 		if (backfacing) { if (!d.cullingEnabled || (d.cullFace & 2) == 0) willCull = false; }
 		else { if (!d.cullingEnabled || (d.cullFace & 1) == 0) willCull = false; }
 		if (willCull) { fcount[q] = 0; continue; }
+		// the fan's vertices in output order (reversed when backfacing)
+		auto src = [&](int k) { return backfacing ? na - 1 - k : k; };
 		for (int k = 0; k < na; ++k) {
-			const CV& src = backfacing ? A[na - 1 - k] : A[k];
-			cvpos[static_cast<long long>(q) * kClipVerts + k] = src.c;
+			const int sk = src(k);
+			cvpos[static_cast<long long>(q) * kClipVerts + k] = make_float4(V(cur, sk, 0), V(cur, sk, 1), V(cur, sk, 2), V(cur, sk, 3));
 			float* o = cvvary + (static_cast<long long>(q) * kClipVerts + k) * varyStride;
-			for (int j = 0; j < P::kVary; ++j) o[j] = src.v[j]; }
+			for (int j = 0; j < P::kVary; ++j) o[j] = V(cur, sk, 4 + j); }
 		const int nf = na - 2;
 		fcount[q] = static_cast<uint8_t>(nf);
 		for (int j = 0; j < nf; ++j) {
-			const float4 p0 = cvpos[static_cast<long long>(q) * kClipVerts + 0];
-			const float4 p1 = cvpos[static_cast<long long>(q) * kClipVerts + j + 1];
-			const float4 p2 = cvpos[static_cast<long long>(q) * kClipVerts + j + 2];
-			const int ix0 = cvtt(p0.x), ix1 = cvtt(p1.x), ix2 = cvtt(p2.x);
-			const int iy0 = cvtt(p0.y), iy1 = cvtt(p1.y), iy2 = cvtt(p2.y);
+			const int k0 = src(0), k1 = src(j + 1), k2 = src(j + 2);
+			const int ix0 = cvtt(V(cur, k0, 0)), ix1 = cvtt(V(cur, k1, 0)), ix2 = cvtt(V(cur, k2, 0));
+			const int iy0 = cvtt(V(cur, k0, 1)), iy1 = cvtt(V(cur, k1, 1)), iy2 = cvtt(V(cur, k2, 1));
 			const int vminx = imax(imin(ix0, imin(ix1, ix2)), d.scissor[0]);
 			const int vminy = imax(imin(iy0, imin(iy1, iy2)), d.scissor[1]);
 			const int vmaxx = imin(wadd(imax(ix0, imax(ix1, ix2)), 1), d.scissor[2] - 1);
