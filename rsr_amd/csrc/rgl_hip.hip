@@ -1057,7 +1057,10 @@ struct rgl_gpu {
 	           const std::vector<SampleReq>& reqs) {
 		hipError_t e;
 		const size_t dlBytes = sizeof(DrawLite) * static_cast<size_t>(std::max(b.nDraws, 1));
-		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, tileCount.p, qCount.p };
+		// small batches scan inside k_scatter (kFusedScanTiles)
+		const bool fusedScan = b.nTiles <= kFusedScanTiles && b.units > 0;
+		const ZeroArgs za{ dCtr, clipHist.p, b.nTiles, tileCount.p, qCount.p, fusedScan ? tileCursor.p : nullptr,
+		                   fusedScan ? clipCursor.p : nullptr };
 		// a double-buffered frame reads the caller's buffers only after the
 		// work the caller queued before Run
 		if (startAfter && (e = hipStreamWaitEvent(stream, startAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
@@ -1084,14 +1087,16 @@ struct rgl_gpu {
 		ClipLaunch cl{ *this, draws, b, varyStride };
 		for_each_program(cl, AllPrograms{});
 		Mark(3);
-		if (b.nTiles > kScanSplitTiles) {
+		const ScanOut so{ tileCursor.p, tileOff.p, clipOff.p, clipCursor.p, itemTile.p, itemCap, tileDesc.p, listCap, clistCap,
+		                  tileCost.p, dCtr };
+		if (fusedScan) {}
+		else if (b.nTiles > kScanSplitTiles) {
 			const int nb = (b.nTiles + kScanThreads - 1) / kScanThreads;
 			k_scan_local<<<nb, kScanThreads, 0, stream>>>(b, tileCount.p, clipHist.p, tileCost.p, tileDesc.p, scanPart.p);
 			k_scan_parts<<<1, kScanThreads, 0, stream>>>(scanPart.p, nb, b.nTiles, tileOff.p, clipOff.p, itemCap, dCtr);
 			k_scan_final<<<nb, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipOff.p, clipCursor.p,
 				itemTile.p, itemCap, tileDesc.p, listCap, clistCap, scanPart.p, nb); }
-		else k_col_scan<<<1, kScanThreads, 0, stream>>>(b, tileCount.p, tileCursor.p, tileOff.p, clipHist.p, clipOff.p, clipCursor.p,
-			itemTile.p, itemCap, tileDesc.p, listCap, clistCap, tileCost.p, dCtr);
+		else k_col_scan<<<1, kScanThreads, 0, stream>>>(b, tileCount.p, clipHist.p, so);
 		Mark(4);
 		RasterArgs ra;
 		ra.draws = dDraws.p; ra.lite = dLite.p; ra.tileOff = tileOff.p; ra.listCap = listCap;
@@ -1107,7 +1112,11 @@ struct rgl_gpu {
 			sa.clipq = clipq.p; sa.fcount = fcount.p; sa.frect = frect.p; sa.cvpos = cvpos.p; sa.clipCap = clipCap;
 			sa.clipCursor = clipCursor.p; sa.crec = crec.p; sa.clistCap = clistCap; sa.clipBlocks = kClipScatterBlocks;
 			sa.tileOff = tileOff.p; sa.clipOff = clipOff.p; sa.keys = keys.p; sa.ctr = dCtr;
-			k_scatter<<<b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, b.unitWindow * sizeof(uint32_t), stream>>>(b, dLite.p, sa); }
+			sa.fused = fusedScan ? 1 : 0; sa.tileCount = tileCount.p; sa.clipHist = clipHist.p;
+			sa.scan = so;
+			sa.scan.tileCursor = nullptr; sa.scan.clipCursor = nullptr;
+			k_scatter<<<(fusedScan ? 1 : 0) + b.units + kClipScatterBlocks + kResetBlocks, kUnitThreads, b.unitWindow * sizeof(uint32_t),
+				stream>>>(b, dLite.p, sa); }
 		Mark(5);
 		if (b.nTiles > 0) {
 			// one workgroup per work item, sized from the last completed frame;

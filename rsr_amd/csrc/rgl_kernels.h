@@ -140,6 +140,7 @@ __device__ inline void load_block_bases(int* fbb, const BatchDesc& b) {
 // or by k_zero_counters when a frame has no vertices)
 struct ZeroArgs {
 	Counters* ctr; uint32_t* clipHist; int nTiles; uint32_t* tileCount; uint32_t* qcount;
+	uint32_t* tileResv; uint32_t* clipResv;   // the fused scatter's reservation counters (null: none)
 };
 
 __device__ inline void zero_frame(const ZeroArgs& z) {
@@ -148,7 +149,10 @@ __device__ inline void zero_frame(const ZeroArgs& z) {
 	for (long long c = i; c < static_cast<long long>(kCountGroups) * z.nTiles; c += n) z.tileCount[c] = 0u;
 	for (long long c = i; c < static_cast<long long>(sizeof(Counters) / 4); c += n) reinterpret_cast<uint32_t*>(z.ctr)[c] = 0u;
 	for (long long c = i; c < 8 * 32; c += n) z.qcount[c] = 0u;
-	for (long long t = i; t < z.nTiles; t += n) z.clipHist[t] = 0u; }
+	for (long long t = i; t < z.nTiles; t += n) z.clipHist[t] = 0u;
+	if (z.tileResv) {
+		for (long long c = i; c < static_cast<long long>(kCountGroups) * z.nTiles; c += n) z.tileResv[c] = 0u;
+		for (long long t = i; t < z.nTiles; t += n) z.clipResv[t] = 0u; }}
 
 // one transformed vertex: ShadeVertex, clip flags, pdiv, viewport
 template <class P>
@@ -367,27 +371,33 @@ static_assert(sizeof(VisTile) == 32, "two 16-byte loads");
 
 // One block: per tile the list offset (exclusive scan of the tiles' pair
 // counts) and the count groups' cursors, the clipped-list offset and cursor
-// (scan of clipHist) and the tile's visibility work items.
-__global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const uint32_t* __restrict__ tileCount,
-		uint32_t* __restrict__ tileCursor, uint32_t* __restrict__ tileOff, const uint32_t* __restrict__ clipHist,
-		uint32_t* __restrict__ clipOff, uint32_t* __restrict__ clipCursor, uint32_t* __restrict__ itemTile, uint32_t itemCap,
-		VisTile* __restrict__ tileDesc, uint32_t listCap, uint32_t clistCap, uint32_t* __restrict__ tileCost,
-		Counters* __restrict__ ctr) {
-	__shared__ uint32_t sh[kScanThreads / 64 + 1];
+// (scan of clipHist) and the tile's visibility work items.  kThreads
+// threads, at most kTiles tiles; null cursors: the count groups' and
+// clipped cursors are not written (the fused scatter reserves its list
+// positions from zeroed counters and these offsets instead).
+struct ScanOut {
+	uint32_t* tileCursor; uint32_t* tileOff; uint32_t* clipOff; uint32_t* clipCursor;
+	uint32_t* itemTile; uint32_t itemCap; VisTile* tileDesc; uint32_t listCap, clistCap; uint32_t* tileCost; Counters* ctr;
+};
+
+template <int kThreads, int kTiles>
+__device__ inline void col_scan_block(const BatchDesc& b, const uint32_t* __restrict__ tileCount,
+		const uint32_t* __restrict__ clipHist, const ScanOut& o) {
+	__shared__ uint32_t sh[kThreads / 64 + 1];
 	__shared__ int ftb[kMaxFrames + 1];
 	load_tile_bases(ftb, b);
 	__syncthreads();
-	const int nT = b.nTiles;   // at most kScanSplitTiles: each thread keeps its tiles' descriptors in registers
-	constexpr int kPer = kScanSplitTiles / kScanThreads;
+	const int nT = b.nTiles;   // at most kTiles: each thread keeps its tiles' descriptors in registers
+	constexpr int kPer = kTiles / kThreads;
 	VisTile mine[kPer];
 	int mcls[kPer];
 	__shared__ uint32_t classItems[kCostClasses], classCursor[kCostClasses];
-	for (int c = threadIdx.x; c < kCostClasses; c += kScanThreads) classItems[c] = 0u;
+	for (int c = threadIdx.x; c < kCostClasses; c += kThreads) classItems[c] = 0u;
 	uint32_t carry = 0, ccarry = 0;
 #pragma unroll
 	for (int it = 0; it < kPer; ++it) {
-		if (it * kScanThreads >= nT) break;   // uniform: no tile left for any thread
-		const int t = it * kScanThreads + threadIdx.x;
+		if (it * kThreads >= nT) break;   // uniform: no tile left for any thread
+		const int t = it * kThreads + threadIdx.x;
 		uint32_t col = 0, v = 0, items = 0, nseg = 1, nreg = kQuadrants, cost = 0;
 		uint32_t grp[kCountGroups];
 #pragma unroll
@@ -396,28 +406,30 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 			col += grp[g]; }
 		if (t < nT) {
 			v = clipHist[t];
-			cost = tileCost[t];
+			cost = o.tileCost[t];
 			nseg = vis_segments(b, col + v);
 			nreg = vis_regions(b, cost, nseg);
 			items = nseg * nreg;
-			tileCost[t] = 0u; }
+			o.tileCost[t] = 0u; }
 		uint32_t tot, ctot;
 		const uint32_t ex = block_exclusive_scan(col, sh, tot);
 		const uint32_t cex = block_exclusive_scan(v, sh, ctot);
 		mcls[it] = cost_class(cost);
 		if (t < nT) {
-			tileOff[t] = carry + ex;
-			uint32_t run = carry + ex;
+			o.tileOff[t] = carry + ex;
+			if (o.tileCursor) {
+				uint32_t run = carry + ex;
 #pragma unroll
-			for (int g = 0; g < kCountGroups; ++g) { tileCursor[static_cast<long long>(g) * nT + t] = run; run += grp[g]; }
-			clipOff[t] = ccarry + cex; clipCursor[t] = ccarry + cex;
+				for (int g = 0; g < kCountGroups; ++g) { o.tileCursor[static_cast<long long>(g) * nT + t] = run; run += grp[g]; }}
+			o.clipOff[t] = ccarry + cex;
+			if (o.clipCursor) o.clipCursor[t] = ccarry + cex;
 			VisTile& d = mine[it];
 			d.item0 = items;   // the item count until the ordering below
 			d.frame = static_cast<uint32_t>(find_base(ftb, b.nFrames, t));
-			d.ib = min(carry + ex, listCap);
-			d.cb = min(ccarry + cex, clistCap);
-			d.ninl = min(carry + ex + col, listCap) - d.ib;
-			d.ntot = d.ninl + (min(ccarry + cex + v, clistCap) - d.cb);
+			d.ib = min(carry + ex, o.listCap);
+			d.cb = min(ccarry + cex, o.clistCap);
+			d.ninl = min(carry + ex + col, o.listCap) - d.ib;
+			d.ntot = d.ninl + (min(ccarry + cex + v, o.clistCap) - d.cb);
 			d.nreg = nreg;
 			d.multiSeg = nseg > 1 ? 1u : 0u;
 			atomicAdd(&classItems[mcls[it]], items); }
@@ -431,26 +443,30 @@ __global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const ui
 		const int lane = threadIdx.x, c = kCostClasses - 1 - lane;   // lane 0: the heaviest class
 		const uint32_t n = c >= 0 ? classItems[c] : 0u;
 		uint32_t x = n;
-		for (int o = 1; o < 64; o <<= 1) {
-			const uint32_t y = __shfl_up(x, o, 64);
-			if (lane >= o) x += y; }
+		for (int off = 1; off < 64; off <<= 1) {
+			const uint32_t y = __shfl_up(x, off, 64);
+			if (lane >= off) x += y; }
 		if (c >= 0) classCursor[c] = x - n;
 		if (lane == 63) classItems[0] = x; }   // the total, read after the barrier below
 	__syncthreads();
 	const uint32_t wcarry = classItems[0];
 #pragma unroll
 	for (int it = 0; it < kPer; ++it) {
-		const int t = it * kScanThreads + threadIdx.x;
+		const int t = it * kThreads + threadIdx.x;
 		if (t >= nT) continue;
 		VisTile& d = mine[it];
 		const uint32_t items = d.item0;
 		const uint32_t item0 = atomicAdd(&classCursor[mcls[it]], items);
-		for (uint32_t j = item0, je = min(j + items, itemCap); j < je; ++j) itemTile[j] = static_cast<uint32_t>(t);
+		for (uint32_t j = item0, je = min(j + items, o.itemCap); j < je; ++j) o.itemTile[j] = static_cast<uint32_t>(t);
 		d.item0 = item0;
-		tileDesc[t] = d; }
+		o.tileDesc[t] = d; }
 	if (threadIdx.x == 0) {
-		tileOff[nT] = carry; clipOff[nT] = ccarry;
-		ctr->pairs = carry; ctr->clipPairs = ccarry; ctr->workItems = min(wcarry, itemCap); ctr->workNext = 0; }}
+		o.tileOff[nT] = carry; o.clipOff[nT] = ccarry;
+		o.ctr->pairs = carry; o.ctr->clipPairs = ccarry; o.ctr->workItems = min(wcarry, o.itemCap); o.ctr->workNext = 0; }}
+
+__global__ __launch_bounds__(kScanThreads) void k_col_scan(BatchDesc b, const uint32_t* __restrict__ tileCount,
+		const uint32_t* __restrict__ clipHist, ScanOut o) {
+	col_scan_block<kScanThreads, kScanSplitTiles>(b, tileCount, clipHist, o); }
 
 // Large batches scan in three launches instead of one block: per block of
 // kScanThreads tiles the local prefixes and per-class item counts
@@ -879,9 +895,17 @@ __device__ inline Rect draw_tile_rect(const DrawLite& d, int tx, int ty) {
 	return r; }
 
 
+// Batches of at most kFusedScanTiles tiles scan inside k_scatter: its
+// block 0 lays out the tiles' lists and visibility work items (the scan
+// kernel's work), and every other block computes the list offsets it
+// needs from the pair counts itself, reserving positions from counters
+// zeroed at the frame's start.  That takes a launch and its latency off
+// the binning stage.
+constexpr int kFusedScanTiles = 512;
+
 struct ScatterArgs {
 	const uint2* trect; const uint4* tvert; const float4* vpos;
-	uint32_t* cursor;             // per (count group, tile) next free list position
+	uint32_t* cursor;             // per (count group, tile) next free list position (fused: position within the group's block)
 	VisRec* rec; uint32_t listCap;
 	// clipped fans (blocks [units, units + clipBlocks))
 	const uint32_t* clipq; const uint8_t* fcount; const uint2* frect; const float4* cvpos; uint32_t clipCap;
@@ -889,12 +913,33 @@ struct ScatterArgs {
 	// key reset of tiles cut into several segments (the remaining blocks)
 	const uint32_t* tileOff; const uint32_t* clipOff; unsigned long long* keys;
 	Counters* ctr;
+	// fused scan
+	int fused;
+	const uint32_t* tileCount; const uint32_t* clipHist;
+	ScanOut scan;
 };
+
+// the exclusive scan of v over the batch's tiles (at most
+// kFusedScanTiles) into out, by one block
+__device__ inline void block_tile_scan(int nT, const uint32_t* __restrict__ v, uint32_t* out, uint32_t* sh) {
+	uint32_t carry = 0;
+	for (int t0 = 0; t0 < nT; t0 += blockDim.x) {   // uniform
+		const int t = t0 + threadIdx.x;
+		const uint32_t x = t < nT ? v[t] : 0u;
+		uint32_t tot;
+		const uint32_t ex = block_exclusive_scan(x, sh, tot);
+		if (t < nT) out[t] = carry + ex;
+		carry += tot; }
+	__syncthreads(); }
 
 // clipped fans into their tiles' clipped lists (ForEachCoveredTile,
 // rglv_gpu_impl.hxx:795-817), positions taken from per-tile cursors
-__device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, const ScatterArgs& a) {
+__device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, const ScatterArgs& a, uint32_t* cbase,
+                                    uint32_t* sh) {
 	const uint32_t nq = min(a.ctr->clipCount, a.clipCap);
+	if (nq == 0) return;   // uniform
+	// fused: the tiles' clipped-list offsets, reserved within from zeroed counters
+	if (a.fused) block_tile_scan(b.nTiles, a.clipHist, cbase, sh);
 	for (uint32_t q = cb * blockDim.x + threadIdx.x; q < nq; q += a.clipBlocks * blockDim.x) {
 		const int nf = a.fcount[q];
 		const uint32_t s = a.clipq[q];
@@ -908,7 +953,8 @@ __device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int 
 			const uint32_t slot = d.slotBase + nTri + static_cast<uint32_t>(s - d.triBase) * kFanMax + static_cast<uint32_t>(j);
 			for (int ty = ty0; ty <= ty1; ++ty)
 				for (int tx = tx0; tx <= tx1; ++tx) {
-					const uint32_t pos = atomicAdd(&a.clipCursor[d.tileBase + ty * d.tilesX + tx], 1u);
+					const int tile = d.tileBase + ty * d.tilesX + tx;
+					const uint32_t pos = (a.fused ? cbase[tile] : 0u) + atomicAdd(&a.clipCursor[tile], 1u);
 					if (pos < a.clistCap) {
 						TriSetup t;
 						setup_scalar(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
@@ -920,7 +966,12 @@ __device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int 
 // segment are overwritten whole by plain stores
 __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, int nrb, const ScatterArgs& a) {
 	for (int tile = rb; tile < b.nTiles; tile += nrb) {
-		if (vis_segments(b, tile_pairs(a.tileOff, a.clipOff, tile)) == 1u) continue;
+		uint32_t pairs;
+		if (a.fused) {
+			pairs = a.clipHist[tile];
+			for (int g = 0; g < kCountGroups; ++g) pairs += a.tileCount[static_cast<long long>(g) * b.nTiles + tile]; }
+		else pairs = tile_pairs(a.tileOff, a.clipOff, tile);
+		if (vis_segments(b, pairs) == 1u) continue;
 		const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
 		const Rect R = tile_rect(f, tile - f.tileBase);
 		const int rw = R.x1 - R.x0, n = rw * (R.y1 - R.y0);
@@ -939,18 +990,42 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 	// draw table is read through the caches
 	extern __shared__ uint4 dyn_lds[];
 	__shared__ int ftb[kMaxFrames + 1];
+	__shared__ uint32_t fbase[kFusedScanTiles];   // fused: per tile, this block's list (or clipped-list) base
+	__shared__ uint32_t fsh[kUnitThreads / 64 + 1];
 	uint32_t* base = reinterpret_cast<uint32_t*>(dyn_lds);
-	const int u = blockIdx.x;
+	int u = blockIdx.x;
+	int nblk = static_cast<int>(gridDim.x);
+	if (a.fused) {
+		if (u == 0) { col_scan_block<kUnitThreads, kFusedScanTiles>(b, a.tileCount, a.clipHist, a.scan); return; }
+		--u; --nblk; }
 	if (u >= b.units + a.clipBlocks) {
 		load_tile_bases(ftb, b);
 		__syncthreads();
-		reset_keys(b, ftb, u - b.units - a.clipBlocks, static_cast<int>(gridDim.x) - b.units - a.clipBlocks, a);
+		reset_keys(b, ftb, u - b.units - a.clipBlocks, nblk - b.units - a.clipBlocks, a);
 		return; }
 	if (u >= b.units) {
-		clip_scatter(b, dl, u - b.units, a);
+		clip_scatter(b, dl, u - b.units, a, fbase, fsh);
 		return; }
 	const TileWindow win = unit_window(dl, b, u);
 	if (win.n > b.unitWindow) return;   // flagged by the setup
+	const int grp = u % kCountGroups;
+	if (a.fused) {
+		// per tile: its list offset (the scan of all groups' counts) plus
+		// the counts of the groups below this unit's
+		uint32_t carry = 0;
+		for (int t0 = 0; t0 < b.nTiles; t0 += kUnitThreads) {   // uniform
+			const int t = t0 + threadIdx.x;
+			uint32_t col = 0, below = 0;
+			if (t < b.nTiles) {
+#pragma unroll
+				for (int g = 0; g < kCountGroups; ++g) {
+					const uint32_t c = a.tileCount[static_cast<long long>(g) * b.nTiles + t];
+					col += c;
+					below += g < grp ? c : 0u; }}
+			uint32_t tot;
+			const uint32_t ex = block_exclusive_scan(col, fsh, tot);
+			if (t < b.nTiles) fbase[t] = carry + ex + below;
+			carry += tot; }}
 	for (int t = threadIdx.x; t < win.n; t += blockDim.x) base[t] = 0u;   // entry t: batch tile win.lo + t
 	__syncthreads();
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
@@ -964,10 +1039,10 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 		for (int ty = ty0; ty <= ty1; ++ty)
 			for (int tx = tx0; tx <= tx1; ++tx) atomicAdd(&base[d.tileBase - win.lo + ty * d.tilesX + tx], 1u); }
 	__syncthreads();
-	uint32_t* cur = a.cursor + static_cast<long long>(u % kCountGroups) * b.nTiles + win.lo;
+	uint32_t* cur = a.cursor + static_cast<long long>(grp) * b.nTiles + win.lo;
 	for (int t = threadIdx.x; t < win.n; t += blockDim.x) {
 		const uint32_t c = base[t];
-		if (c) base[t] = atomicAdd(&cur[t], c); }
+		if (c) base[t] = (a.fused ? fbase[win.lo + t] : 0u) + atomicAdd(&cur[t], c); }
 	__syncthreads();
 	bool ovf = false;
 	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
