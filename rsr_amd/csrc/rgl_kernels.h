@@ -19,7 +19,7 @@
 
 namespace rgl {
 
-constexpr int kUnitTris = 256;      // triangles per binning unit
+constexpr int kUnitTris = 512;      // triangles per binning unit
 constexpr int kUnitThreads = 256;   // threads per binning-unit workgroup (setup/count and scatter)
 constexpr int kXcds = 8;             // workgroups are dealt round-robin over the eight XCDs
 
