@@ -114,10 +114,17 @@ __device__ inline float mlp_enc_value(const float x[kMlpFeat], int e) {
 	for (int q = 0; q < kk; ++q) { const float s2 = 2.0F * sn * cs, c2 = (cs - sn) * (cs + sn); sn = s2; cs = c2; }
 	return (j & 1) ? sn : cs; }
 
+// orders the wave's own LDS writes before its later reads (no workgroup barrier)
+__device__ inline void wave_sync() {
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
+
 // one wave: 64 samples (one per lane) through the network.  xc: this
 // wave's [64][40] chunk of encoded inputs; hb: its [64][72] activations;
-// W: the LDS image.  Every wave of the workgroup calls it together (it
-// synchronises the workgroup between phases).
+// W: the LDS image.  The wave touches no other wave's slices, so its
+// phases are ordered within the wave alone (LDS operations of one wave
+// complete in order) and the workgroup's waves run through it independently.
 __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* xc, uint16_t* hb,
                                 const float x[kMlpFeat], float y[kMlpOut]) {
 	const int lane = threadIdx.x & 63, r16 = lane & 15, g = lane >> 4;
@@ -137,7 +144,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 		uint4* dst = reinterpret_cast<uint4*>(xc + lane * 40);
 #pragma unroll
 		for (int q = 0; q < 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-		__syncthreads();
+		wave_sync();
 		bf16x8_t a[4], b[4];
 #pragma unroll
 		for (int i = 0; i < 4; ++i) a[i] = lds_bf16x8(W + kImgW1 + (16 * i + r16) * kLd1 + 32 * c + g * 8);
@@ -147,7 +154,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 		for (int i = 0; i < 4; ++i)
 #pragma unroll
 			for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16(a[i], b[j], acc[i][j]);
-		__syncthreads(); }
+		wave_sync(); }
 	// layers 2 and 3: transposed products H^T = W H_prev^T, samples along N
 #pragma unroll
 	for (int layer = 0; layer < 3; ++layer) {
@@ -165,7 +172,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 					hv[r] = bf16_bits(v > 0.0F ? v : 0.0F); }
 				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) =
 					make_uint2(hv[0] | (static_cast<uint32_t>(hv[1]) << 16), hv[2] | (static_cast<uint32_t>(hv[3]) << 16)); }
-		__syncthreads();
+		wave_sync();
 		if (layer == 2) break;
 		const uint16_t* Wl = W + (layer == 0 ? kImgW2 : kImgW3);
 		bf16x8_t a[2][4], b[2][4];
@@ -182,7 +189,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 				f32x4_t s = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
 				s = mfma_bf16(a[0][i], b[0][j], s);
 				acc[i][j] = mfma_bf16(a[1][i], b[1][j], s); }
-		__syncthreads(); }
+		wave_sync(); }
 	// layer 4: 3 outputs in rows 0..2 of one 16-row tile
 	f32x4_t o[4];
 	{
@@ -198,7 +205,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 			s = mfma_bf16(a[0], b[0][j], s);
 			o[j] = mfma_bf16(a[1], b[1][j], s); }
 	}
-	__syncthreads();
+	wave_sync();
 	// lanes 0..15 hold rows 0..3 (channels) of samples 16 j + lane
 	float* yo = reinterpret_cast<float*>(xc);
 	if (g == 0) {
@@ -206,11 +213,11 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 		for (int j = 0; j < 4; ++j)
 #pragma unroll
 			for (int r = 0; r < 4; ++r) yo[(16 * j + r16) * 4 + r] = o[j][r]; }
-	__syncthreads();
+	wave_sync();
 	const float* b4 = bias + 3 * kMlpHid;
 #pragma unroll
 	for (int c = 0; c < kMlpOut; ++c) y[c] = yo[lane * 4 + c] + b4[c];
-	__syncthreads(); }
+	wave_sync(); }
 
 constexpr int kMlpWaves = 8;   // waves per inference workgroup (64 samples each)
 
