@@ -224,7 +224,10 @@ struct rgl_gpu {
 	float mlpMaxRms = 2.0F / 255.0F;
 	DevArray<uint32_t> mlpImg;      // the network image of learned frames
 	DevArray<uint32_t> fwdImg;      // the network image of rgl_mlp_forward
-	DevArray<float> tileErr, mlpQueue;
+	DevArray<float> mlpQueue;
+	DevArray<uint32_t> probeWant;   // the probes' plain colours, kMlpProbes per tile
+	hipStream_t probeStream = nullptr;   // learned frames: the probes beside the gather and the network
+	hipEvent_t probeFork = nullptr, probeJoin = nullptr;
 	DevArray<uint32_t> qCount;      // network queue counters, kQueueSegs x kQueueStride
 	int mlpShadeBlocks = 256;       // persistent network workgroups: one per CU
 	bool mlpShadeBlocksSet = false;
@@ -351,7 +354,8 @@ struct rgl_gpu {
 		for (auto* a : { &qCount, &tileCount, &tileCursor, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
 		tileDesc.release(); scanPart.release();
 		keys.release(); rec.release(); crec.release();
-		for (auto* a : { &tileErr, &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
+		probeWant.release();
+		for (auto* a : { &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release(); fwdImg.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
 		cvpos.release(); cvvary.release(); fcount.release(); frect.release(); dDraws.release(); dLite.release();
@@ -367,6 +371,9 @@ struct rgl_gpu {
 		if (auxDone) (void)hipEventDestroy(auxDone);
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
+		if (probeStream) (void)hipStreamDestroy(probeStream);
+		if (probeFork) (void)hipEventDestroy(probeFork);
+		if (probeJoin) (void)hipEventDestroy(probeJoin);
 		if (done) (void)hipEventDestroy(done);
 		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e);
 		for (auto& set : kev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
@@ -678,7 +685,11 @@ struct rgl_gpu {
 		if ((e = Grow(keys, px)) != hipSuccess) return hipfail(e, "hipMalloc keys");
 		if (mlpParams) {
 			if ((e = Grow(mlpImg, kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
-			if ((e = Grow(tileErr, 2LL * b.nTiles + 2)) != hipSuccess) return hipfail(e, "hipMalloc tile errors");
+			if ((e = Grow(probeWant, static_cast<size_t>(b.nTiles) * kMlpProbes + 1)) != hipSuccess) return hipfail(e, "hipMalloc probes");
+			if (!probeStream) {
+				if ((e = hipStreamCreateWithFlags(&probeStream, hipStreamNonBlocking)) != hipSuccess) return hipfail(e, "hipStreamCreate");
+				if ((e = hipEventCreateWithFlags(&probeFork, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate");
+				if ((e = hipEventCreateWithFlags(&probeJoin, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 			// each queue segment takes the resolve blocks with index % kQueueSegs == seg
 			mlpSegCap = static_cast<uint32_t>((static_cast<long long>(b.nBlocks) + kQueueSegs - 1) / kQueueSegs * 256);
 			if ((e = Grow(mlpQueue, static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
@@ -994,7 +1005,7 @@ struct rgl_gpu {
 		mix(lastIo.data(), lastIo.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
 		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, scanPart.p, tileCost.p,
-		                       keys.p, rec.p, crec.p, mlpImg.p, tileErr.p, mlpQueue.p, qCount.p, mlpParams,
+		                       keys.p, rec.p, crec.p, mlpImg.p, probeWant.p, mlpQueue.p, qCount.p, mlpParams,
 		                       dOps.p, dIo.p, order.p, skeys0.p, skeys1.p };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -1140,6 +1151,13 @@ struct rgl_gpu {
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		const DrawLite* cLite = dLite.p;
 		const DrawDesc* cDraws = dDraws.p;
+		if (learned && b.nTiles > 0) {
+			// the probes' plain colours (they need the keys, not the network)
+			// on a second stream, beside the gather and the network pass
+			if ((e = hipEventRecord(probeFork, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
+			if ((e = hipStreamWaitEvent(probeStream, probeFork, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
+			k_mlp_probe<ProgGGXEnv><<<b.nTiles, kMlpProbes, dlBytes, probeStream>>>(b, ra, probeWant.p);
+			if ((e = hipEventRecord(probeJoin, probeStream)) != hipSuccess) return hipfail(e, "hipEventRecord"); }
 		if (learned) {
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
 			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
@@ -1153,13 +1171,11 @@ struct rgl_gpu {
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);
 		if (learned && b.nTiles > 0) {
-			// per-tile error estimate at probe pixels, then plain re-shading of
-			// the tiles over the limit
-			k_fill_u32<<<8, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(tileErr.p), 2LL * b.nTiles, 0u);
-			const long long probes = static_cast<long long>(b.nTiles) * kMlpProbes;
-			k_mlp_probe<ProgGGXEnv><<<static_cast<int>((probes + 255) / 256), 256, dlBytes, stream>>>(b, ra, tileErr.p);
+			// plain re-shading of the tiles whose probes show an error over
+			// the limit, once the probes have joined
+			if ((e = hipStreamWaitEvent(stream, probeJoin, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 			const float lim = mlpMaxRms * 255.0F;
-			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, tileErr.p, lim * lim, &dCtr->fallbackTiles); }
+			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, probeWant.p, lim * lim, &dCtr->fallbackTiles); }
 		for (const SampleReq& q : reqs)
 			k_mlp_samples<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, q.out, q.cap, q.count, q.keep, q.seed);
 		Mark(9);

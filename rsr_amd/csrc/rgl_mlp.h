@@ -416,79 +416,104 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const
 			uint32_t* out = b.frames[f2u(r3.z)].outColor;
 			if (out) out[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}}
 
-// per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
-// each tile, squared 8-bit error against the network's output
+// Per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
+// each tile (one in each cell of an 8 x 8 grid), compared with the
+// network's colour there.  The probes' plain colours do not depend on the
+// network, so k_mlp_probe runs beside the gather and the network pass (on
+// a second stream); the comparison is made by the fallback pass, after the
+// network has stored its colours.
 constexpr int kMlpProbes = 64;
+constexpr uint32_t kNoProbe = 0xffffffffu;   // probe pixel not covered (or no colour output)
 
 __device__ inline uint32_t hash32(uint32_t v) {
 	v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16; return v; }
 
+// probe j of batch tile `tile`, whose pixel rect is R; false when the cell
+// falls outside the rect
+__device__ inline bool probe_pixel(const Rect& R, int tile, int j, int& x, int& y) {
+	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
+	const uint32_t id = static_cast<uint32_t>(tile) * kMlpProbes + static_cast<uint32_t>(j);
+	const uint32_t h = hash32(id * 0x9e3779b9u + 17u);
+	x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
+	y = R.y0 + (j >> 3) * ch + static_cast<int>((h >> 16) % static_cast<uint32_t>(ch));
+	return x < R.x1 && y < R.y1; }
+
+// one workgroup (one wave) per tile: the plain colour of each probe pixel,
+// or kNoProbe
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_probe(BatchDesc b, RasterArgs a, float* __restrict__ tileErr) {
+__global__ __launch_bounds__(kMlpProbes) void k_mlp_probe(BatchDesc b, RasterArgs a, uint32_t* __restrict__ want) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	__shared__ int ftb[kMaxFrames + 1];
 	load_lite(dl, a.draws, b.nDraws);
 	load_tile_bases(ftb, b);
 	__syncthreads();
-	const long long id = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-	if (id >= static_cast<long long>(b.nTiles) * kMlpProbes) return;
-	const int tile = static_cast<int>(id / kMlpProbes), j = static_cast<int>(id % kMlpProbes);
+	const int tile = static_cast<int>(blockIdx.x), j = static_cast<int>(threadIdx.x);
 	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
-	if (f.outColor == nullptr) return;
-	const Rect R = tile_rect(f, tile - f.tileBase);
-	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
-	const uint32_t h = hash32(static_cast<uint32_t>(id) * 0x9e3779b9u + 17u);
-	const int x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
-	const int y = R.y0 + (j >> 3) * ch + static_cast<int>((h >> 16) % static_cast<uint32_t>(ch));
-	if (x >= R.x1 || y >= R.y1) return;
-	float col[3], dep;
-	uint32_t prim;
-	if (!resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim)) return;
-	const uint32_t want = post_pixel(f, col, x, y, kSrgb8Tab);
-	const uint32_t got = f.outColor[static_cast<long long>(y) * f.W + x];
-	float se = 0.0F;
-	for (int c = 0; c < 3; ++c) {
-		const float dv = static_cast<float>(static_cast<int>((want >> (8 * c)) & 255u) - static_cast<int>((got >> (8 * c)) & 255u));
-		se += dv * dv; }
-	atomicAdd(&tileErr[2 * tile], se);
-	atomicAdd(&tileErr[2 * tile + 1], 3.0F); }
+	uint32_t w = kNoProbe;
+	int x, y;
+	if (f.outColor != nullptr && probe_pixel(tile_rect(f, tile - f.tileBase), tile, j, x, y)) {
+		float col[3], dep;
+		uint32_t prim;
+		if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim)) w = post_pixel(f, col, x, y, kSrgb8Tab); }
+	want[static_cast<long long>(tile) * kMlpProbes + j] = w; }
 
-// pixels of tiles whose estimated RMS error exceeds the limit are re-shaded
+// the fallback decision of one tile, by one wave: the mean squared 8-bit
+// error per channel at its probes (the network's stored colour against
+// the plain one) over the limit
+__device__ inline bool tile_over_limit(const FrameDesc& f, int tile, const uint32_t* __restrict__ want, float maxMse) {
+	const int j = threadIdx.x & 63;
+	int x, y;
+	float se = 0.0F, cnt = 0.0F;
+	const uint32_t w = want[static_cast<long long>(tile) * kMlpProbes + j];
+	if (w != kNoProbe && probe_pixel(tile_rect(f, tile - f.tileBase), tile, j, x, y)) {
+		const uint32_t got = f.outColor[static_cast<long long>(y) * f.W + x];
+		for (int c = 0; c < 3; ++c) {
+			const float dv = static_cast<float>(static_cast<int>((w >> (8 * c)) & 255u) - static_cast<int>((got >> (8 * c)) & 255u));
+			se += dv * dv; }
+		cnt = 3.0F; }
+	for (int o = 32; o > 0; o >>= 1) { se += __shfl_xor(se, o, 64); cnt += __shfl_xor(cnt, o, 64); }
+	return cnt > 0.0F && se / cnt > maxMse; }
+
+// pixels of tiles whose estimated error exceeds the limit are re-shaded
 // with the plain shader (colour only; depth and ids are already stored);
-// one thread per pixel, tiles under the limit exit at once
+// one thread per pixel.  Wave 0 decides the block's tiles; blocks of tiles
+// under the limit exit at once.
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a, const float* __restrict__ tileErr,
+__global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a, const uint32_t* __restrict__ want,
                                                       float maxMse, uint32_t* __restrict__ nFallback) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
 	__shared__ uint32_t rcpTab[2048];
 	__shared__ uint32_t srgbTab[104];
 	__shared__ int fbb[kMaxFrames + 1];
+	constexpr int kMaxBlockTiles = (kResolveW / 8 + 1) * (kResolveH / 8 + 1);   // tiles are at least 8 px
+	__shared__ uint32_t over[kMaxBlockTiles];
 	load_block_bases(fbb, b);
 	__syncthreads();
 	const BlockPixel bp = block_pixel(b, fbb);
 	const FrameDesc& f = b.frames[bp.frame];
+	if (f.outColor == nullptr) return;   // uniform over the block
 	const int x = bp.x, y = bp.y;
 	// the tiles the block's pixels lie in: rows ty0..ty1, columns tx0..tx1
 	const int lb = static_cast<int>(blockIdx.x) - f.blockBase, bx = lb % f.blocksX, by = lb / f.blocksX;
 	const int ty0 = (by * kResolveH) / f.tileH, ty1 = min(by * kResolveH + kResolveH - 1, f.H - 1) / f.tileH;
 	const int tx0 = (bx * kResolveW) / f.tileW, tx1 = min(bx * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
+	const int ntx = tx1 - tx0 + 1, nbt = (ty1 - ty0 + 1) * ntx;
+	if (threadIdx.x < 64)
+		for (int k = 0; k < nbt; ++k) {
+			const bool v = tile_over_limit(f, f.tileBase + (ty0 + k / ntx) * f.tilesX + tx0 + k % ntx, want, maxMse);
+			if (threadIdx.x == 0) over[k] = v ? 1u : 0u; }
+	__syncthreads();
 	bool any = false;
-	for (int ty = ty0; ty <= ty1; ++ty)
-		for (int tx = tx0; tx <= tx1; ++tx) {
-			const int t = f.tileBase + ty * f.tilesX + tx;
-			const float c = tileErr[2 * t + 1];
-			any = any || (c > 0.0F && tileErr[2 * t] / c > maxMse); }
-	if (!any || f.outColor == nullptr) return;   // uniform over the block
+	for (int k = 0; k < nbt; ++k) any = any || over[k] != 0u;
+	if (!any) return;   // uniform over the block
 	load_lite(dl, a.draws, b.nDraws);
 	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
 	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
 	if (x >= f.W || y >= f.H) return;
-	const int tile = f.tileBase + (y / f.tileH) * f.tilesX + x / f.tileW;
-	const float cnt = tileErr[2 * tile + 1];
-	if (!(cnt > 0.0F) || !(tileErr[2 * tile] / cnt > maxMse)) return;
+	if (!over[(y / f.tileH - ty0) * ntx + x / f.tileW - tx0]) return;
 	if (x % f.tileW == 0 && y % f.tileH == 0) atomicAdd(nFallback, 1u);
 	float col[3], dep;
 	uint32_t prim;

@@ -224,3 +224,38 @@ def test_network_call_on_another_stream_orders_against_frames():
     want, _ = learned.ref_forward(s, p)
     assert np.abs(out.cpu().numpy() - want).mean() < 2e-3
     gpu.close()
+
+
+def test_learned_frame_graph_replay_matches_eager():
+    # a learned frame with the fallback forks its probes onto a second
+    # stream and joins them before the fallback: replayed from a captured
+    # graph (5 times, with eager frames of another program between) it
+    # stores exactly the eager frame
+    import os
+    wpath = os.path.join(os.path.dirname(__file__), "..", "rsr_amd", "ggx_mlp_weights.npy")
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    sc = scenes.scene_heavy(640, 360)
+    gpu = rgl.GPU(0)
+    ec, ed, ep, est = _frame(gpu, sc, params)
+    print(f"eager: {est['fallback_tiles']} of {est['tiles']} tiles re-shaded", flush=True)
+    assert 0 < est["fallback_tiles"] < est["tiles"]
+    other = rgl.GPU(0)
+    oc, od, op, _ = rgl.render(scenes.scene_sphere_ground(), gpu=other)
+    gpu.set_graph(True)
+    ds = rgl.DeviceScene(sc, DEV, prim=True)
+    ds.record(gpu)
+    stream = torch.cuda.current_stream(DEV).cuda_stream
+    for k in range(5):
+        ds.color.zero_()
+        torch.cuda.synchronize()
+        gpu.run_and_wait(stream)
+        c, d, p = ds.outputs()
+        assert np.array_equal(c, ec) and np.array_equal(d.view(np.uint32), ed.view(np.uint32)) and np.array_equal(p, ep), k
+        assert gpu.stats()["fallback_tiles"] == est["fallback_tiles"]
+        c2, _, _, _ = rgl.render(scenes.scene_sphere_ground(), gpu=other)
+        assert np.array_equal(c2, oc)
+    assert gpu.graph_captures() == 1
+    gpu.set_graph(False)
+    gpu.set_learned_shading(None, 0.0)
+    other.close()
+    gpu.close()
