@@ -67,6 +67,13 @@ __device__ inline uint16_t bf16_bits(float v) {   // round to nearest even
 
 __device__ inline float bf16_float(uint16_t b) { return u2f(static_cast<uint32_t>(b) << 16); }
 
+// two values rounded to bf16 (nearest even) by one v_cvt_pk_bf16_f32, a in
+// the low half; the same bits as bf16_bits for every finite value
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t bf16_pack(float a, float b) {
+	return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2_t){ a, b }, bf16x2_t)); }
+
 __device__ inline f32x4_t mfma_bf16(const bf16x8_t& a, const bf16x8_t& b, const f32x4_t& c) {
 	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
@@ -139,8 +146,7 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 		uint32_t w[16];
 #pragma unroll
 		for (int q = 0; q < 16; ++q)
-			w[q] = static_cast<uint32_t>(bf16_bits(mlp_enc_value(x, 32 * c + 2 * q))) |
-			       (static_cast<uint32_t>(bf16_bits(mlp_enc_value(x, 32 * c + 2 * q + 1))) << 16);
+			w[q] = bf16_pack(mlp_enc_value(x, 32 * c + 2 * q), mlp_enc_value(x, 32 * c + 2 * q + 1));
 		uint4* dst = reinterpret_cast<uint4*>(xc + lane * 40);
 #pragma unroll
 		for (int q = 0; q < 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
@@ -165,13 +171,12 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 #pragma unroll
 			for (int j = 0; j < 4; ++j) {
 				const int n0 = 16 * i + 4 * g;
-				uint16_t hv[4];
+				float hv[4];
 #pragma unroll
 				for (int r = 0; r < 4; ++r) {
 					const float v = acc[i][j][r] + bl[n0 + r];
-					hv[r] = bf16_bits(v > 0.0F ? v : 0.0F); }
-				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) =
-					make_uint2(hv[0] | (static_cast<uint32_t>(hv[1]) << 16), hv[2] | (static_cast<uint32_t>(hv[3]) << 16)); }
+					hv[r] = v > 0.0F ? v : 0.0F; }
+				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) = make_uint2(bf16_pack(hv[0], hv[1]), bf16_pack(hv[2], hv[3])); }
 		wave_sync();
 		if (layer == 2) break;
 		const uint16_t* Wl = W + (layer == 0 ? kImgW2 : kImgW3);
