@@ -226,33 +226,44 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 	__syncthreads();
 	uint32_t binned = 0;
 	const long long s0 = static_cast<long long>(u) * b.unitTris;
-	for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {
-		const long long s = s0 + r;
-		if (s >= b.totalTris) break;
-		const int di = find_draw_ll(dl, b.nDraws, s);
-		const DrawLite& d = dl[di];
-		const long long n = s - d.triBase;
+	// A triangle in three stages: its vertex indices, then its vertices'
+	// positions and clip flags, then the work.  A unit of kUnitTris
+	// triangles (kPer per thread) runs each stage for all of a thread's
+	// triangles before the next, so their loads are in flight together.
+	struct Tri { long long s; long long vb; int di; bool live, bad; uint32_t ii[3]; uint32_t cf[3]; float dcx[3], dcy[3]; };
+	auto indices = [&](Tri& q) {
+		q.live = q.s < b.totalTris; q.bad = false; q.di = 0; q.vb = 0;
+		if (!q.live) return;
+		q.di = find_draw_ll(dl, b.nDraws, q.s);
+		const DrawLite& d = dl[q.di];
+		const long long n = q.s - d.triBase;
 		const int iid = static_cast<int>(n / d.tris);
 		const int t = static_cast<int>(n - static_cast<long long>(iid) * d.tris);
-		uint32_t ii[3];
-		bool bad = false;
+		for (int k = 0; k < 3; ++k) q.ii[k] = d.elements ? d.indices[3 * t + k] : static_cast<uint32_t>(3 * t + k);
+		q.vb = d.vBase + static_cast<long long>(iid) * d.verts; };
+	auto vertices = [&](Tri& q) {
+		if (!q.live) return;
+		const DrawLite& d = dl[q.di];
+		for (int k = 0; k < 3; ++k) q.bad |= q.ii[k] >= static_cast<uint32_t>(d.verts);
+		if (q.bad) return;
 		for (int k = 0; k < 3; ++k) {
-			ii[k] = d.elements ? d.indices[3 * t + k] : static_cast<uint32_t>(3 * t + k);
-			bad |= ii[k] >= static_cast<uint32_t>(d.verts); }
+			const float4 p = vpos[q.vb + q.ii[k]];
+			q.dcx[k] = p.x; q.dcy[k] = p.y;
+			q.cf[k] = vflags[q.vb + q.ii[k]]; }};
+	// every lane of the wave runs it (the clip queue takes a ballot)
+	auto work = [&](const Tri& q) {
 		uint2 rect = empty_rect();
 		uint4 tv = make_uint4(0, 0, 0, 0);
 		bool clips = false;
-		if (bad) {
-			atomicOr(&ctr->overflow, 8u); }
-		else {
-			const long long vb = d.vBase + static_cast<long long>(iid) * d.verts;
-			uint32_t cf[3]; float dcx[3], dcy[3];
-			for (int k = 0; k < 3; ++k) {
-				const float4 p = vpos[vb + ii[k]];
-				dcx[k] = p.x; dcy[k] = p.y;
-				cf[k] = vflags[vb + ii[k]]; }
-			const bool pointsOutside = (cf[0] | cf[1] | cf[2]) != 0u;
-			const bool primsOutside = (cf[0] & cf[1] & cf[2]) != 0u;
+		if (q.live && q.bad) atomicOr(&ctr->overflow, 8u);
+		if (q.live && !q.bad) {
+			const DrawLite& d = dl[q.di];
+			const uint32_t* ii = q.ii;
+			const float* dcx = q.dcx;
+			const float* dcy = q.dcy;
+			const long long vb = q.vb;
+			const bool pointsOutside = (q.cf[0] | q.cf[1] | q.cf[2]) != 0u;
+			const bool primsOutside = (q.cf[0] & q.cf[1] & q.cf[2]) != 0u;
 			clips = !primsOutside && pointsOutside;
 			if (!pointsOutside) {
 				const float d31x = dcx[2] - dcx[0], d31y = dcy[2] - dcy[0];
@@ -279,8 +290,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 					for (int ty = ty0; ty <= ty1; ++ty)
 						for (int tx = tx0; tx <= tx1; ++tx)
 							atomicAdd(&lhist[d.tileBase - win.lo + ty * d.tilesX + tx], 1u); }}}
-		trect[s] = rect;
-		tvert[s] = tv;
+		if (q.live) { trect[q.s] = rect; tvert[q.s] = tv; }
 		// the wave's clipped triangles take consecutive clip-queue entries
 		// from one atomic
 		const unsigned long long cm = __ballot(clips);
@@ -291,9 +301,25 @@ __global__ __launch_bounds__(kUnitThreads) void k_setup_count(const DrawDesc* __
 			if (lane == static_cast<unsigned>(leader)) q0 = atomicAdd(&ctr->clipCount, static_cast<uint32_t>(__popcll(cm)));
 			q0 = __shfl(q0, leader, 64);
 			if (clips) {
-				const uint32_t q = q0 + static_cast<uint32_t>(__popcll(cm & ((1ull << lane) - 1ull)));
-				if (q < clipCap) { clipq[q] = static_cast<uint32_t>(s); clipqOf[s] = q; }
-				else atomicOr(&ctr->overflow, 2u); }}}
+				const uint32_t qi = q0 + static_cast<uint32_t>(__popcll(cm & ((1ull << lane) - 1ull)));
+				if (qi < clipCap) { clipq[qi] = static_cast<uint32_t>(q.s); clipqOf[q.s] = qi; }
+				else atomicOr(&ctr->overflow, 2u); }}};
+	constexpr int kPer = kUnitTris / kUnitThreads;
+	if (b.unitTris == kUnitTris) {
+		Tri q[kPer];
+#pragma unroll
+		for (int k = 0; k < kPer; ++k) { q[k].s = s0 + threadIdx.x + k * kUnitThreads; indices(q[k]); }
+#pragma unroll
+		for (int k = 0; k < kPer; ++k) vertices(q[k]);
+#pragma unroll
+		for (int k = 0; k < kPer; ++k) work(q[k]); }
+	else for (int r = threadIdx.x; r < b.unitTris; r += blockDim.x) {   // grown units
+		Tri q;
+		q.s = s0 + r;
+		if (q.s >= b.totalTris) break;
+		indices(q);
+		vertices(q);
+		work(q); }
 	__syncthreads();
 	uint32_t* tc = tileCount + static_cast<long long>(u % kCountGroups) * b.nTiles;
 	for (int t = threadIdx.x; t < win.n; t += blockDim.x) {
