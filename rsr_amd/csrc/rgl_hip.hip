@@ -1575,6 +1575,13 @@ int rgl_gpu_set_graph(rgl_gpu* g, int enable) {
 	if (g->pending || g->dbMode) { const int r = g->WaitAll(); if (r != 0 && r != RGL_E_CAPACITY) return r; }
 	g->graphMode = enable != 0;
 	g->graphValid = false;
+	// leaving graph replay frees the capture stream: a process has few
+	// hardware queues (four by default), and streams beyond them share one
+	if (!g->graphMode) {
+		if (g->graphExec) (void)hipGraphExecDestroy(g->graphExec);
+		g->graphExec = nullptr;
+		if (g->capStream) (void)hipStreamDestroy(g->capStream);
+		g->capStream = nullptr; }
 	return 0; }
 
 int rgl_gpu_graph_captures(rgl_gpu* g) { return g ? g->captures : RGL_E_ARG; }
