@@ -96,6 +96,15 @@ int rgl_gpu_wait(rgl_gpu* gpu);
    enable = 0 returns to single-buffered runs. */
 int rgl_gpu_set_double_buffer(rgl_gpu* gpu, int enable);
 
+/* Double-buffered: blocks until the frame before the last Run has
+   completed (and with it every earlier frame: each frame's stores waited
+   for the previous frame's end), while the last frame may still run; the
+   stats are that frame's.  The reference's Run returns with the previous
+   frame drawn (rglv_gpu.cxx:96-116); a viewer that renders into two
+   canvases in turn displays the previous one after this call.
+   Single-buffered: rgl_gpu_wait. */
+int rgl_gpu_wait_previous(rgl_gpu* gpu);
+
 /* GPUStats of the last completed Run. */
 int rgl_gpu_stats(rgl_gpu* gpu, rgl_stats* out);
 

@@ -305,6 +305,8 @@ struct rgl_gpu {
 	hipStream_t dbStream[kMaxExec] = {};
 	hipEvent_t dbIn = nullptr;      // recorded on the caller's stream at Run
 	int dbNext = 0, dbLast = -1;    // executor of the next and of the last frame
+	int dbPrev = -1;                // executor of the frame before the last
+	rgl_stats dbStats{};            // double-buffered: the stats rgl_gpu_stats reports (the waited frame's)
 	hipEvent_t startAfter = nullptr, rasterAfter = nullptr;   // waits of the frame being launched
 	rgl_stats stats{};
 	long long submitted = 0;
@@ -898,9 +900,22 @@ struct rgl_gpu {
 		const int r = x->RunBatch(&self, 1, dbStream[dbNext]);
 		x->startAfter = x->rasterAfter = nullptr;
 		if (r != 0) { if (x != this) err = x->err; return r; }
+		dbPrev = dbLast;
 		dbLast = dbNext;
 		dbNext = (dbNext + 1) % nExec;
 		return 0; }
+
+	// the frame before the last one queued (double-buffered), and so every
+	// frame before it: each frame's stores waited for the previous frame's
+	// end.  The stats are that frame's.
+	int WaitPrevious() {
+		if (!dbMode) return WaitAll();
+		if (dbPrev < 0) return 0;
+		rgl_gpu* p = exec[dbPrev];
+		const int r = p->Wait();
+		dbStats = p->stats;
+		if (p != this && r != 0) err = p->err;
+		return r; }
 
 	// every frame in flight; the stats are the last frame's.  A capacity
 	// grown by one executor is grown in the other too.
@@ -911,7 +926,7 @@ struct rgl_gpu {
 		for (int k = 0; k < nExec; ++k) {
 			const int r = exec[k]->Wait();
 			if (r != 0 && rc == 0) { rc = r; e0 = exec[k]->err; }}
-		if (dbLast > 0) { stats = exec[dbLast]->stats; lastWorkItems = exec[dbLast]->lastWorkItems; }
+		if (dbLast >= 0) dbStats = exec[dbLast]->stats;
 		for (int k = 1; k < nExec; ++k) {
 			listCap = std::max(listCap, exec[k]->listCap);
 			clipCap = std::max(clipCap, exec[k]->clipCap);
@@ -931,7 +946,8 @@ struct rgl_gpu {
 		for (int k = 0; k < n; ++k)
 			if ((e = hipStreamCreateWithFlags(&dbStream[k], hipStreamNonBlocking)) != hipSuccess) { EndDouble(); return hipfail(e, "hipStreamCreate"); }
 		if ((e = hipEventCreateWithFlags(&dbIn, hipEventDisableTiming)) != hipSuccess) { EndDouble(); return hipfail(e, "hipEventCreate"); }
-		dbNext = 0; dbLast = -1;
+		dbNext = 0; dbLast = -1; dbPrev = -1;
+		dbStats = stats;
 		dbMode = true;
 		return 0; }
 
@@ -944,6 +960,7 @@ struct rgl_gpu {
 		if (dbIn) (void)hipEventDestroy(dbIn);
 		dbIn = nullptr;
 		stream = nullptr;
+		if (dbMode && dbLast >= 0) stats = dbStats;
 		dbMode = false; }
 
 	// The network entry points run on their caller's stream ms and share the
@@ -1344,6 +1361,8 @@ int rgl_gpu_run(rgl_gpu* g, void* hip_stream) {
 
 int rgl_gpu_wait(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; return g->WaitAll(); }
 
+int rgl_gpu_wait_previous(rgl_gpu* g) { if (check(g)) return RGL_E_ARG; return g->WaitPrevious(); }
+
 int rgl_gpu_set_double_buffer(rgl_gpu* g, int enable) {
 	if (check(g)) return RGL_E_ARG;
 	if (enable < 0 || enable > rgl_gpu::kMaxExec) return g->fail(RGL_E_ARG, "enable is 0, 1, or 2 to 4 frames in flight");
@@ -1356,7 +1375,7 @@ int rgl_gpu_set_double_buffer(rgl_gpu* g, int enable) {
 
 int rgl_gpu_stats(rgl_gpu* g, rgl_stats* out) {
 	if (check(g) || !out) return RGL_E_ARG;
-	*out = g->stats;
+	*out = g->dbMode ? g->dbStats : g->stats;
 	return 0; }
 
 int rgl_gpu_set_timing(rgl_gpu* g, int enable) {

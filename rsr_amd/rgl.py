@@ -88,6 +88,7 @@ def lib():
         L.rgl_gpu_run_batch.argtypes = [vp, ctypes.POINTER(vp), i, vp]
         L.rgl_gpu_set_graph.argtypes = [vp, i]
         L.rgl_gpu_set_double_buffer.argtypes = [vp, i]
+        L.rgl_gpu_wait_previous.argtypes = [vp]
         L.rgl_gpu_graph_captures.argtypes = [vp]
         L.rgl_gpu_set_raster_params.argtypes = [vp, i, i]
         L.rgl_gpu_set_guard_band.argtypes = [vp, i]
@@ -117,7 +118,7 @@ EXPORTS = [
     "rgl_gl_clear", "rgl_gl_store_color_tc", "rgl_gl_store_depth", "rgl_gl_store_primitive_id",
     "rgl_gl_store_color_fp", "rgl_gl_store_color_quads", "rgl_kawase_blur",
     "rgl_gl_reset", "rgl_texture_make_mipmap", "rgl_texture_from_rgba8", "rgl_texture_pow2", "rgl_texture_mipmap",
-    "rgl_gpu_run_batch", "rgl_gpu_set_graph", "rgl_gpu_set_double_buffer", "rgl_gpu_graph_captures", "rgl_gpu_set_raster_params",
+    "rgl_gpu_run_batch", "rgl_gpu_set_graph", "rgl_gpu_set_double_buffer", "rgl_gpu_wait_previous", "rgl_gpu_graph_captures", "rgl_gpu_set_raster_params",
     "rgl_gpu_set_guard_band",
     "rgl_mlp_param_count", "rgl_mlp_trainable_count", "rgl_gpu_set_learned_shading",
     "rgl_gl_store_training_samples", "rgl_mlp_forward", "rgl_mlp_gradient", "rgl_mlp_train",
@@ -264,6 +265,10 @@ class GPU:
         """rglv::doubleBuffer (rglv_gpu.cxx:16): frame N's binning beside
         frame N-1's drawing; read the outputs after wait()."""
         self._check(self.L.rgl_gpu_set_double_buffer(self.h, int(enable)))
+
+    def wait_previous(self):
+        """Double-buffered: wait for the frame before the last Run."""
+        return self._check(self.L.rgl_gpu_wait_previous(self.h))
 
     def graph_captures(self) -> int:
         return self._check(self.L.rgl_gpu_graph_captures(self.h))

@@ -131,3 +131,43 @@ def test_refused_modes_and_switching_back():
     for a, b in zip(single, d.outputs()):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     g.close()
+
+
+def test_wait_previous_returns_the_frame_before_the_last():
+    # two canvases in turn, as a viewer would: after each Run, wait for the
+    # frame before it and check that frame's canvas and stats; the last
+    # frame is checked after the final wait
+    from rsr_amd import rgl
+    light = scenes.scene_sphere_ground(width=1920, height=1080)
+    heavy = scenes.scene_icospheres()
+    g = rgl.GPU(0)
+    g.set_double_buffer(True)
+    ref = {}
+    one = rgl.GPU(0)
+    for sc in (light, heavy):
+        r = rgl.DeviceScene(sc, DEV, prim=True)
+        r.record(one)
+        one.run_and_wait(_stream())
+        ref[sc.name] = (r.outputs(), one.stats())
+    one.close()
+    canv = [rgl.DeviceScene(sc, DEV, prim=True) for sc in (light, heavy)]
+    torch.cuda.synchronize()
+    seq = [0, 1, 0, 1, 1, 0]
+    for k, c in enumerate(seq):
+        print(f"[double-buffered] frame {k}: {canv[c].scene.name}", flush=True)
+        canv[c].record(g)
+        g.run(_stream())
+        if k == 0:
+            continue
+        g.wait_previous()
+        prev = canv[seq[k - 1]]
+        outs, st = ref[prev.scene.name]
+        if seq[k - 1] != c:   # the previous frame's canvas is not being redrawn
+            for a, b in zip(outs, prev.outputs()):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (k, prev.scene.name)
+        assert g.stats()["primitives_submitted"] == st["primitives_submitted"], k
+    g.wait()
+    last = canv[seq[-1]]
+    for a, b in zip(ref[last.scene.name][0], last.outputs()):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    g.close()
