@@ -442,10 +442,28 @@ def main():
             raise SystemExit("capacity overflow inside the double-buffered timed region")
         gpu.set_double_buffer(0)
         return r
-    db_ms = tb_ms = None
+    def viewer_loop_ms():
+        # a viewer's loop: after each Run, wait for the frame before it
+        gpu.set_double_buffer(1)
+        for _ in range(4):
+            gpu.run(stream)
+            gpu.wait_previous()
+        gpu.wait()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            gpu.run(stream)
+            gpu.wait_previous()
+        gpu.wait()
+        torch.cuda.synchronize()
+        r = (time.perf_counter() - t2) * 1e3 / args.steps
+        gpu.set_double_buffer(0)
+        return r
+    db_ms = tb_ms = vl_ms = None
     if not args.no_double_buffer:
         db_ms = in_flight_ms(1)      # two executors: the reference's double buffer
         tb_ms = in_flight_ms(3)      # three frames in flight
+        vl_ms = viewer_loop_ms()
     # per-kernel durations: a second pass with HIP events recorded on the
     # frame's stream between the kernels
     _, kt = timed_frames(gpu, stream, max(20, args.steps // 2), 2)
@@ -525,6 +543,7 @@ def main():
             "frame_ms": round(db_ms, 4), "mtris_per_s": round(tris / (db_ms * 1e-3) / 1e6, 3),
             "gfrag_per_s": round(st["fragments"] / (db_ms * 1e-3) / 1e9, 4), "speedup_vs_single": round(ms / db_ms, 3),
             "three_in_flight_frame_ms": round(tb_ms, 4),
+            "viewer_loop_frame_ms": round(vl_ms, 4),
             "mode": "rgl_gpu_set_double_buffer: two executors, frame N binning beside frame N-1 raster, "
                     "stores in submission order"}
     if not args.no_learned:
