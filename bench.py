@@ -244,6 +244,47 @@ def bench_batch(steps: int):
     return out
 
 
+def bench_small(steps: int, with_cpu: bool):
+    """Config 1, the reference's CPU-runnable case: the 640x360 UV sphere
+    over a ground quad (962 triangles) with the Lambert texture program;
+    frames queued back to back, and the oracle's frame on the host beside
+    it."""
+    import torch
+    from rsr_amd import rgl, scenes
+    dev = "cuda:0"
+    sc = scenes.scene_sphere_ground()
+    gpu = rgl.GPU(0)
+    ds = rgl.DeviceScene(sc, dev, prim=True)
+    ds.record(gpu)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ms = untimed_frames(gpu, stream, steps, 10)
+    st = gpu.stats()
+    gpu.close()
+    out = {"workload": sc.name, "triangles": sc.triangles, "frame_ms": round(ms, 4),
+           "mtris_per_s": round(sc.triangles / (ms * 1e-3) / 1e6, 3), "fragments": st["fragments"]}
+    if with_cpu:
+        # the oracle is the checker and the CPU baseline, never the product
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_backend import OracleGPU
+        from rsr_amd.scenes import render_scene
+        g = OracleGPU(host_cpu()["share"])
+        color = np.zeros((sc.height, sc.width), np.uint32)
+        depth = np.zeros((sc.height, sc.width), np.float32)
+        prim = np.zeros((sc.height, sc.width), np.uint32)
+        frame = []
+        for i in range(50 + 500):
+            render_scene(g, sc, color, depth, prim)
+            t0 = time.perf_counter()
+            g.run()
+            if i >= 50:
+                frame.append((time.perf_counter() - t0) * 1e3)
+        g.close()
+        cpu_ms = _stats_ms(frame)["mean"]
+        out["cpu_oracle_frame_ms"] = round(cpu_ms, 4)
+        out["speedup_vs_cpu"] = round(cpu_ms / ms, 2)
+    return out
+
+
 def bench_stress(steps: int):
     """The stress workload: 3840x2160, 2M triangles (6250 instanced
     subdivision-2 icospheres, projected areas ~0.25-16 px^2, clustered to a
@@ -552,6 +593,7 @@ def main():
         out["batched"] = bench_batch(max(10, args.steps // 10))
     if not args.no_stress:
         out["stress"] = bench_stress(max(5, args.steps // 20))
+    out["small"] = bench_small(args.steps, not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.cpu_priming, args.cpu_frames, args.cpu_seconds)
         out["cpu_baseline"] = cb

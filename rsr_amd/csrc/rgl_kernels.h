@@ -958,34 +958,60 @@ __device__ inline void block_tile_scan(int nT, const uint32_t* __restrict__ v, u
 		carry += tot; }
 	__syncthreads(); }
 
+// the tiles of a packed tile rect (none when empty)
+__device__ inline uint32_t rect_tiles(uint2 rc) {
+	const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+	return tx0 <= tx1 && ty0 <= ty1 ? static_cast<uint32_t>((tx1 - tx0 + 1) * (ty1 - ty0 + 1)) : 0u; }
+
 // clipped fans into their tiles' clipped lists (ForEachCoveredTile,
-// rglv_gpu_impl.hxx:795-817), positions taken from per-tile cursors
+// rglv_gpu_impl.hxx:795-817).  A batch of the block's clipped triangles,
+// one per thread, is expanded into its (fan triangle, tile) pairs, which
+// the block's threads share out: a large clipped triangle's pairs take
+// their list positions (an atomic each) in parallel.
 __device__ inline void clip_scatter(const BatchDesc& b, const DrawLite* dl, int cb, const ScatterArgs& a, uint32_t* cbase,
-                                    uint32_t* sh) {
+                                    uint32_t* sh, uint32_t* pre) {
 	const uint32_t nq = min(a.ctr->clipCount, a.clipCap);
 	if (nq == 0) return;   // uniform
 	// fused: the tiles' clipped-list offsets, reserved within from zeroed counters
 	if (a.fused) block_tile_scan(b.nTiles, a.clipHist, cbase, sh);
-	for (uint32_t q = cb * blockDim.x + threadIdx.x; q < nq; q += a.clipBlocks * blockDim.x) {
-		const int nf = a.fcount[q];
-		const uint32_t s = a.clipq[q];
-		const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
-		const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
-		for (int j = 0; j < nf; ++j) {
-			const uint2 rc = a.frect[static_cast<long long>(q) * kFanMax + j];
-			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
-			TriVerts tr;
-			clipped_verts(q * kFanMax + j, a.cvpos, tr);
+	const uint32_t step = static_cast<uint32_t>(a.clipBlocks) * blockDim.x;
+	for (uint32_t q0 = static_cast<uint32_t>(cb) * blockDim.x; q0 < nq; q0 += step) {   // uniform
+		const uint32_t q = q0 + threadIdx.x;
+		uint32_t n = 0;
+		if (q < nq) {
+			const int nf = a.fcount[q];
+			for (int j = 0; j < nf; ++j) n += rect_tiles(a.frect[static_cast<long long>(q) * kFanMax + j]); }
+		uint32_t tot;
+		const uint32_t ex = block_exclusive_scan(n, sh, tot);
+		pre[threadIdx.x] = ex;
+		__syncthreads();
+		for (uint32_t i = threadIdx.x; i < tot; i += blockDim.x) {
+			// the pair's triangle: the last thread whose prefix is at or below i
+			int lo = 0, hi = static_cast<int>(blockDim.x) - 1;
+			while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (pre[mid] <= i) lo = mid; else hi = mid - 1; }
+			const uint32_t qq = q0 + static_cast<uint32_t>(lo);
+			uint32_t r = i - pre[lo];
+			// its fan triangle and tile
+			int j = 0;
+			uint2 rc = a.frect[static_cast<long long>(qq) * kFanMax];
+			for (uint32_t nt = rect_tiles(rc); r >= nt; nt = rect_tiles(rc)) { r -= nt; ++j; rc = a.frect[static_cast<long long>(qq) * kFanMax + j]; }
+			const int tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
+			const int w = tx1 - tx0 + 1;
+			const int ty = ty0 + static_cast<int>(r) / w, tx = tx0 + static_cast<int>(r) % w;
+			const uint32_t s = a.clipq[qq];
+			const DrawLite& d = dl[find_draw_ll(dl, b.nDraws, s)];
+			const uint32_t nTri = static_cast<uint32_t>(d.tris) * static_cast<uint32_t>(d.instances);
 			const uint32_t slot = d.slotBase + nTri + static_cast<uint32_t>(s - d.triBase) * kFanMax + static_cast<uint32_t>(j);
-			for (int ty = ty0; ty <= ty1; ++ty)
-				for (int tx = tx0; tx <= tx1; ++tx) {
-					const int tile = d.tileBase + ty * d.tilesX + tx;
-					const uint32_t pos = (a.fused ? cbase[tile] : 0u) + atomicAdd(&a.clipCursor[tile], 1u);
-					if (pos < a.clistCap) {
-						TriSetup t;
-						setup_scalar(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
-						a.crec[pos] = make_rec(t, tr, slot); }
-					else atomicOr(&a.ctr->overflow, 4u); }}}}
+			TriVerts tr;
+			clipped_verts(qq * kFanMax + j, a.cvpos, tr);
+			const int tile = d.tileBase + ty * d.tilesX + tx;
+			const uint32_t pos = (a.fused ? cbase[tile] : 0u) + atomicAdd(&a.clipCursor[tile], 1u);
+			if (pos < a.clistCap) {
+				TriSetup t;
+				setup_scalar(draw_tile_rect(d, tx, ty), tr.X, tr.Y, t);
+				a.crec[pos] = make_rec(t, tr, slot); }
+			else atomicOr(&a.ctr->overflow, 4u); }
+		__syncthreads(); }}
 
 // tiles cut into several segments merge their regions with a 64-bit
 // atomic min, so their keys start the frame at the sentinel; tiles of one
@@ -1018,6 +1044,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 	__shared__ int ftb[kMaxFrames + 1];
 	__shared__ uint32_t fbase[kFusedScanTiles];   // fused: per tile, this block's list (or clipped-list) base
 	__shared__ uint32_t fsh[kUnitThreads / 64 + 1];
+	__shared__ uint32_t cpre[kUnitThreads];      // clip blocks: the batch's pair prefixes
 	uint32_t* base = reinterpret_cast<uint32_t*>(dyn_lds);
 	int u = blockIdx.x;
 	int nblk = static_cast<int>(gridDim.x);
@@ -1030,7 +1057,7 @@ __global__ __launch_bounds__(kUnitThreads) void k_scatter(BatchDesc b, const Dra
 		reset_keys(b, ftb, u - b.units - a.clipBlocks, nblk - b.units - a.clipBlocks, a);
 		return; }
 	if (u >= b.units) {
-		clip_scatter(b, dl, u - b.units, a, fbase, fsh);
+		clip_scatter(b, dl, u - b.units, a, fbase, fsh, cpre);
 		return; }
 	const TileWindow win = unit_window(dl, b, u);
 	if (win.n > b.unitWindow) return;   // flagged by the setup
