@@ -1072,7 +1072,8 @@ struct rgl_gpu {
 			bool present = false;
 			for (const auto& d : draws) present |= d.program == P::kId;
 			if (!present) return;
-			k_clip<P><<<64, 64, 0, g.stream>>>(g.dDraws.p, b.nDraws, g.clipq.p, g.clipCap, g.cvpos.p, g.cvvary.p, g.fcount.p,
+			const unsigned waves = std::max(1u, std::min<unsigned>(kClipWaves, g.clipCap));
+			k_clip<P><<<waves, 64, 0, g.stream>>>(g.dDraws.p, b.nDraws, g.clipq.p, g.clipCap, g.cvpos.p, g.cvvary.p, g.fcount.p,
 				g.frect.p, g.clipHist.p, g.dCtr, varyStride); }};
 	struct ResolveLaunch {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;

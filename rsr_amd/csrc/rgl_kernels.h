@@ -608,48 +608,56 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_final(BatchDesc b, const 
 // Sutherland-Hodgman against L, B, N, R, T, fan triangulation, per-tile
 // counts (ForEachCoveredTile, :795-817).
 // ---------------------------------------------------------------------------
+// One wave per clipped triangle: lane c holds component c (x, y, z, w,
+// then the varyings) of every vertex of the polygon, in an LDS column of
+// its own, so each copy or interpolation of a vertex is one operation per
+// lane; the wave's control flow, driven by the plane distances every lane
+// reads alike, is uniform.  The arithmetic per component is the serial
+// clipper's.
+constexpr int kClipWaves = 1024;   // workgroups (one wave each) of the clipping launch
+
 template <class P>
 __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws, int nDraws,
 		const uint32_t* __restrict__ clipq, uint32_t clipCap, float4* __restrict__ cvpos, float* __restrict__ cvvary,
 		uint8_t* __restrict__ fcount, uint2* __restrict__ frect, uint32_t* __restrict__ clipHist, Counters* __restrict__ ctr,
 		int varyStride) {
-	// the polygon being clipped and the next one, per lane in LDS (component
-	// c of vertex k at [k * kComp + c][lane]); dynamically indexed arrays
-	// in registers would live in scratch memory
 	constexpr int kComp = 4 + P::kVary;
-	__shared__ float poly[2][kClipVerts * kComp][64];
+	static_assert(kComp <= 64, "one lane per component");
+	__shared__ float poly[2][kClipVerts][64];
 	const int lane = threadIdx.x;
-	auto V = [&](int buf, int k, int c) -> float& { return poly[buf][k * kComp + c][lane]; };
-	auto dist = [&](int buf, int plane, int k) {
-		const float w = V(buf, k, 3);
-		switch (plane) {
-		case 0: return w + V(buf, k, 0);
-		case 1: return w + V(buf, k, 1);
-		case 2: return w + V(buf, k, 2);
-		case 3: return w - V(buf, k, 0);
-		default: return w - V(buf, k, 1); }};
 	const uint32_t nq = min(ctr->clipCount, clipCap);
-	for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += gridDim.x * blockDim.x) {
+	for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+		__syncthreads();
 		const long long s = clipq[q];
 		int lo = 0, hi = nDraws - 1;
 		while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (draws[mid].triBase <= s) lo = mid; else hi = mid - 1; }
 		const DrawDesc& d = draws[lo];
-		if (d.program != P::kId) continue;   // another program's launch clips it
+		if (d.program != P::kId) continue;   // another program's launch clips it (uniform)
 		const long long n = s - d.triBase;
 		const int iid = static_cast<int>(n / d.tris);
 		const int t = static_cast<int>(n - static_cast<long long>(iid) * d.tris);
-		for (int k = 0; k < 3; ++k) {
-			const int idx = d.elements ? d.indices[3 * t + k] : 3 * t + k;
+		// lanes 0..2 shade the triangle's vertices into rows 0..2
+		if (lane < 3) {
+			const int idx = d.elements ? d.indices[3 * t + lane] : 3 * t + lane;
 			float pos[4], vo[P::kVary];
 			P::vertex(d, iid, idx, pos, vo);
-			for (int c = 0; c < 4; ++c) V(0, k, c) = pos[c];
-			for (int j = 0; j < P::kVary; ++j) V(0, k, 4 + j) = vo[j]; }
+			for (int c = 0; c < 4; ++c) poly[0][lane][c] = pos[c];
+			for (int j = 0; j < P::kVary; ++j) poly[0][lane][4 + j] = vo[j]; }
+		__syncthreads();
+		auto dist = [&](int buf, int plane, int k) {
+			const float w = poly[buf][k][3];
+			switch (plane) {
+			case 0: return w + poly[buf][k][0];
+			case 1: return w + poly[buf][k][1];
+			case 2: return w + poly[buf][k][2];
+			case 3: return w - poly[buf][k][0];
+			default: return w - poly[buf][k][1]; }};
 		// all three vertices outside one clipping plane: Sutherland-Hodgman
 		// would emit an empty polygon for that plane
 		bool rejected = false;
 		for (int plane = 0; plane < 5; ++plane)
 			rejected |= dist(0, plane, 0) < 0.0F && dist(0, plane, 1) < 0.0F && dist(0, plane, 2) < 0.0F;
-		if (rejected) { fcount[q] = 0; continue; }
+		if (rejected) { if (lane == 0) fcount[q] = 0; continue; }
 		int cur = 0, na = 3;
 		for (int plane = 0; plane < 5 && na > 0; ++plane) {
 			const int nxt = cur ^ 1;
@@ -658,48 +666,47 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 			for (int h = 0; h < na; ++h) {
 				const int nx = h + 1 == na ? 0 : h + 1;
 				const bool nextIsInside = dist(cur, plane, nx) >= 0.0F;
-				if (hereIsInside) {
-					for (int c = 0; c < kComp; ++c) V(nxt, nb, c) = V(cur, h, c);
-					++nb; }
+				if (hereIsInside) { poly[nxt][nb][lane] = poly[cur][h][lane]; ++nb; }
 				if (hereIsInside != nextIsInside) {
 					const int from = hereIsInside ? h : nx;
 					const int to = hereIsInside ? nx : h;
 					const float fa = dist(cur, plane, from), fb = dist(cur, plane, to);
 					const float dd = fa / (fa - fb);
 					const float om = 1.0F - dd;
-					for (int c = 0; c < kComp; ++c) V(nxt, nb, c) = om * V(cur, from, c) + dd * V(cur, to, c);
+					poly[nxt][nb][lane] = om * poly[cur][from][lane] + dd * poly[cur][to][lane];
 					++nb;
 					hereIsInside = !hereIsInside; }}
 			na = nb;
 			cur = nxt; }
-		if (na == 0) { fcount[q] = 0; continue; }
+		if (na == 0) { if (lane == 0) fcount[q] = 0; continue; }
+		// pdiv and viewport: lanes 0..3 (x, y, z, 1/w)
 		for (int k = 0; k < na; ++k) {
-			const float iw = oneover(V(cur, k, 3));
-			float x = V(cur, k, 0) * iw, y = V(cur, k, 1) * iw;
-			const float z = V(cur, k, 2) * iw;
-			x = x * d.DS[0] + d.DO[0];
-			y = y * d.DS[1] + d.DO[1];
-			V(cur, k, 0) = x; V(cur, k, 1) = y; V(cur, k, 2) = z; V(cur, k, 3) = iw; }
-		const float d31x = V(cur, 2, 0) - V(cur, 0, 0), d31y = V(cur, 2, 1) - V(cur, 0, 1);
-		const float d21x = V(cur, 1, 0) - V(cur, 0, 0), d21y = V(cur, 1, 1) - V(cur, 0, 1);
+			const float iw = oneover(poly[cur][k][3]);
+			float v = poly[cur][k][lane < 4 ? lane : 0];
+			if (lane == 0) v = (v * iw) * d.DS[0] + d.DO[0];
+			else if (lane == 1) v = (v * iw) * d.DS[1] + d.DO[1];
+			else if (lane == 2) v = v * iw;
+			else if (lane == 3) v = iw;
+			if (lane < 4) poly[cur][k][lane] = v; }
+		const float d31x = poly[cur][2][0] - poly[cur][0][0], d31y = poly[cur][2][1] - poly[cur][0][1];
+		const float d21x = poly[cur][1][0] - poly[cur][0][0], d21y = poly[cur][1][1] - poly[cur][0][1];
 		const bool backfacing = (d31x*d21y - d31y*d21x) < 0.0F;
 		bool willCull = true;
 		if (backfacing) { if (!d.cullingEnabled || (d.cullFace & 2) == 0) willCull = false; }
 		else { if (!d.cullingEnabled || (d.cullFace & 1) == 0) willCull = false; }
-		if (willCull) { fcount[q] = 0; continue; }
+		if (willCull) { if (lane == 0) fcount[q] = 0; continue; }
 		// the fan's vertices in output order (reversed when backfacing)
 		auto src = [&](int k) { return backfacing ? na - 1 - k : k; };
 		for (int k = 0; k < na; ++k) {
-			const int sk = src(k);
-			cvpos[static_cast<long long>(q) * kClipVerts + k] = make_float4(V(cur, sk, 0), V(cur, sk, 1), V(cur, sk, 2), V(cur, sk, 3));
-			float* o = cvvary + (static_cast<long long>(q) * kClipVerts + k) * varyStride;
-			for (int j = 0; j < P::kVary; ++j) o[j] = V(cur, sk, 4 + j); }
+			const float v = poly[cur][src(k)][lane];
+			if (lane < 4) reinterpret_cast<float*>(cvpos)[(static_cast<long long>(q) * kClipVerts + k) * 4 + lane] = v;
+			else if (lane < kComp) cvvary[(static_cast<long long>(q) * kClipVerts + k) * varyStride + (lane - 4)] = v; }
 		const int nf = na - 2;
-		fcount[q] = static_cast<uint8_t>(nf);
+		if (lane == 0) fcount[q] = static_cast<uint8_t>(nf);
 		for (int j = 0; j < nf; ++j) {
 			const int k0 = src(0), k1 = src(j + 1), k2 = src(j + 2);
-			const int ix0 = cvtt(V(cur, k0, 0)), ix1 = cvtt(V(cur, k1, 0)), ix2 = cvtt(V(cur, k2, 0));
-			const int iy0 = cvtt(V(cur, k0, 1)), iy1 = cvtt(V(cur, k1, 1)), iy2 = cvtt(V(cur, k2, 1));
+			const int ix0 = cvtt(poly[cur][k0][0]), ix1 = cvtt(poly[cur][k1][0]), ix2 = cvtt(poly[cur][k2][0]);
+			const int iy0 = cvtt(poly[cur][k0][1]), iy1 = cvtt(poly[cur][k1][1]), iy2 = cvtt(poly[cur][k2][1]);
 			const int vminx = imax(imin(ix0, imin(ix1, ix2)), d.scissor[0]);
 			const int vminy = imax(imin(iy0, imin(iy1, iy2)), d.scissor[1]);
 			const int vmaxx = imin(wadd(imax(ix0, imax(ix1, ix2)), 1), d.scissor[2] - 1);
@@ -708,11 +715,12 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 			int tx1 = vmaxx / d.tileW, ty1 = vmaxy / d.tileH;
 			tx1 = imin(tx1, d.tilesX - 1); ty1 = imin(ty1, d.tilesY - 1);
 			tx0 = imax(tx0, 0); ty0 = imax(ty0, 0);
-			if (tx0 > tx1 || ty0 > ty1) { frect[static_cast<long long>(q) * kFanMax + j] = empty_rect(); continue; }
-			frect[static_cast<long long>(q) * kFanMax + j] = pack_rect(tx0, ty0, tx1, ty1);
-			for (int ty = ty0; ty <= ty1; ++ty)
-				for (int tx = tx0; tx <= tx1; ++tx)
-					atomicAdd(&clipHist[d.tileBase + ty * d.tilesX + tx], 1u); }}}
+			if (tx0 > tx1 || ty0 > ty1) { if (lane == 0) frect[static_cast<long long>(q) * kFanMax + j] = empty_rect(); continue; }
+			if (lane == 0) frect[static_cast<long long>(q) * kFanMax + j] = pack_rect(tx0, ty0, tx1, ty1);
+			// the fan triangle's tiles, shared out over the lanes
+			const int w = tx1 - tx0 + 1, ntl = w * (ty1 - ty0 + 1);
+			for (int i = lane; i < ntl; i += 64)
+				atomicAdd(&clipHist[d.tileBase + (ty0 + i / w) * d.tilesX + tx0 + i % w], 1u); }}}
 
 // ---------------------------------------------------------------------------
 // per-tile raster + resolve
