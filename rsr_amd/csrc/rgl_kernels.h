@@ -644,33 +644,40 @@ __global__ __launch_bounds__(64) void k_clip(const DrawDesc* __restrict__ draws,
 			for (int c = 0; c < 4; ++c) poly[0][lane][c] = pos[c];
 			for (int j = 0; j < P::kVary; ++j) poly[0][lane][4 + j] = vo[j]; }
 		__syncthreads();
-		auto dist = [&](int buf, int plane, int k) {
-			const float w = poly[buf][k][3];
+		// lane k's distance of polygon vertex k to a clipping plane; every
+		// lane reads vertex k's with readlane (a uniform value)
+		auto lane_dist = [&](int buf, int plane) {
+			if (lane >= kClipVerts) return 0.0F;
+			const float w = poly[buf][lane][3];
 			switch (plane) {
-			case 0: return w + poly[buf][k][0];
-			case 1: return w + poly[buf][k][1];
-			case 2: return w + poly[buf][k][2];
-			case 3: return w - poly[buf][k][0];
-			default: return w - poly[buf][k][1]; }};
+			case 0: return w + poly[buf][lane][0];
+			case 1: return w + poly[buf][lane][1];
+			case 2: return w + poly[buf][lane][2];
+			case 3: return w - poly[buf][lane][0];
+			default: return w - poly[buf][lane][1]; }};
+		auto at = [](float v, int k) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k)); };
 		// all three vertices outside one clipping plane: Sutherland-Hodgman
 		// would emit an empty polygon for that plane
 		bool rejected = false;
-		for (int plane = 0; plane < 5; ++plane)
-			rejected |= dist(0, plane, 0) < 0.0F && dist(0, plane, 1) < 0.0F && dist(0, plane, 2) < 0.0F;
+		for (int plane = 0; plane < 5; ++plane) {
+			const float dk = lane_dist(0, plane);
+			rejected |= at(dk, 0) < 0.0F && at(dk, 1) < 0.0F && at(dk, 2) < 0.0F; }
 		if (rejected) { if (lane == 0) fcount[q] = 0; continue; }
 		int cur = 0, na = 3;
 		for (int plane = 0; plane < 5 && na > 0; ++plane) {
 			const int nxt = cur ^ 1;
-			bool hereIsInside = dist(cur, plane, 0) >= 0.0F;
+			const float dk = lane_dist(cur, plane);
+			auto dist = [&](int k) { return at(dk, k); };
+			bool hereIsInside = dist(0) >= 0.0F;
 			int nb = 0;
 			for (int h = 0; h < na; ++h) {
 				const int nx = h + 1 == na ? 0 : h + 1;
-				const bool nextIsInside = dist(cur, plane, nx) >= 0.0F;
+				const bool nextIsInside = dist(nx) >= 0.0F;
 				if (hereIsInside) { poly[nxt][nb][lane] = poly[cur][h][lane]; ++nb; }
 				if (hereIsInside != nextIsInside) {
 					const int from = hereIsInside ? h : nx;
 					const int to = hereIsInside ? nx : h;
-					const float fa = dist(cur, plane, from), fb = dist(cur, plane, to);
+					const float fa = dist(from), fb = dist(to);
 					const float dd = fa / (fa - fb);
 					const float om = 1.0F - dd;
 					poly[nxt][nb][lane] = om * poly[cur][from][lane] + dd * poly[cur][to][lane];
