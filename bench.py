@@ -259,9 +259,13 @@ def bench_small(steps: int, with_cpu: bool):
     stream = torch.cuda.current_stream(dev).cuda_stream
     ms = untimed_frames(gpu, stream, steps, 10)
     st = gpu.stats()
+    gpu.set_double_buffer(1)
+    db_ms = untimed_frames(gpu, stream, steps, 10)
+    gpu.set_double_buffer(0)
     gpu.close()
     out = {"workload": sc.name, "triangles": sc.triangles, "frame_ms": round(ms, 4),
-           "mtris_per_s": round(sc.triangles / (ms * 1e-3) / 1e6, 3), "fragments": st["fragments"]}
+           "mtris_per_s": round(sc.triangles / (ms * 1e-3) / 1e6, 3), "fragments": st["fragments"],
+           "double_buffered_frame_ms": round(db_ms, 4)}
     if with_cpu:
         # the oracle is the checker and the CPU baseline, never the product
         sys.path.insert(0, os.path.join(ROOT, "tests"))
