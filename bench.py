@@ -389,6 +389,7 @@ def main():
     ap.add_argument("--no-stress", action="store_true", help="skip the 4K stress measurement")
     ap.add_argument("--graph", action="store_true", help="time graph replay instead of eager launches")
     ap.add_argument("--no-double-buffer", action="store_true", help="skip the double-buffered timing")
+    ap.add_argument("--no-small", action="store_true", help="skip the 640x360 sphere-and-ground measurement")
     ap.add_argument("--no-tune", action="store_true", help="default tile size and pairs per work item, no autotuner")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
@@ -597,7 +598,8 @@ def main():
         out["batched"] = bench_batch(max(10, args.steps // 10))
     if not args.no_stress:
         out["stress"] = bench_stress(max(5, args.steps // 20))
-    out["small"] = bench_small(args.steps, not args.no_cpu)
+    if not args.no_small:
+        out["small"] = bench_small(args.steps, not args.no_cpu)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.cpu_priming, args.cpu_frames, args.cpu_seconds)
         out["cpu_baseline"] = cb
