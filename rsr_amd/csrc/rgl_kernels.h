@@ -1046,7 +1046,8 @@ __device__ inline void reset_keys(const BatchDesc& b, const int* ftb, int rb, in
 			const int ly = i / rw, lx = i - ly * rw;
 			a.keys[f.keyBase + static_cast<long long>(R.y0 + ly) * f.W + R.x0 + lx] = kKeySentinel; }}}
 
-// blocks [0, units): one binning unit each.  The unit counts its pairs per
+// With the fused scan, block 0 is the scan's and the others shift by one.
+// Blocks [0, units): one binning unit each.  The unit counts its pairs per
 // tile in LDS, reserves one block of list positions per tile it touches
 // from its count group's cursor, then each lane walks its triangle's
 // tiles, takes a position from the unit's LDS cursor of the tile and
