@@ -257,8 +257,9 @@ int rgl_mlp_trainable_count(void);
 /* Frames drawn with program 82 shade with the network from the next Run on:
    the plain shader checks 64 probe pixels per tile and re-shades every tile
    whose RMS error (8-bit, over covered probes) exceeds max_rms_err (in
-   [0,1] units, e.g. 2/255).  Coverage, depth and primitive ids are never
-   affected.  dev_params = NULL switches back to the plain shader. */
+   [0,1] units, e.g. 2/255); max_rms_err >= 1 checks nothing and keeps the
+   network's colours everywhere.  Coverage, depth and primitive ids are
+   never affected.  dev_params = NULL switches back to the plain shader. */
 int rgl_gpu_set_learned_shading(rgl_gpu* gpu, const float* dev_params, float max_rms_err);
 
 /* Training data: covered pixels of the frame, each kept with probability

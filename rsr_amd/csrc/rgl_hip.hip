@@ -146,7 +146,7 @@ struct DevArray {
 	void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }};
 
 const char* kKernelNames[] = { "k_vertex", "k_setup_count", "k_clip", "k_scan", "k_scatter", "k_vis", "k_resolve",
-                               "k_mlp_shade", "k_mlp_check" };
+                               "k_mlp_shade", "k_mlp_fallback" };
 constexpr int kNumTimed = 9;
 
 // a StoreTrainingSamples request (no reference counterpart)
@@ -226,8 +226,6 @@ struct rgl_gpu {
 	DevArray<uint32_t> fwdImg;      // the network image of rgl_mlp_forward
 	DevArray<float> mlpQueue;
 	DevArray<uint32_t> probeWant;   // the probes' plain colours, kMlpProbes per tile
-	hipStream_t probeStream = nullptr;   // learned frames: the probes beside the gather and the network
-	hipEvent_t probeFork = nullptr, probeJoin = nullptr;
 	DevArray<uint32_t> qCount;      // network queue counters, kQueueSegs x kQueueStride
 	int mlpShadeBlocks = 256;       // persistent network workgroups: one per CU
 	bool mlpShadeBlocksSet = false;
@@ -373,9 +371,6 @@ struct rgl_gpu {
 		if (auxDone) (void)hipEventDestroy(auxDone);
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
-		if (probeStream) (void)hipStreamDestroy(probeStream);
-		if (probeFork) (void)hipEventDestroy(probeFork);
-		if (probeJoin) (void)hipEventDestroy(probeJoin);
 		if (done) (void)hipEventDestroy(done);
 		for (auto& set : ev) for (auto& e : set) if (e) (void)hipEventDestroy(e);
 		for (auto& set : kev) for (auto& e : set) if (e) (void)hipEventDestroy(e); }
@@ -688,13 +683,9 @@ struct rgl_gpu {
 		if (mlpParams) {
 			if ((e = Grow(mlpImg, kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
 			if ((e = Grow(probeWant, static_cast<size_t>(b.nTiles) * kMlpProbes + 1)) != hipSuccess) return hipfail(e, "hipMalloc probes");
-			if (!probeStream) {
-				if ((e = hipStreamCreateWithFlags(&probeStream, hipStreamNonBlocking)) != hipSuccess) return hipfail(e, "hipStreamCreate");
-				if ((e = hipEventCreateWithFlags(&probeFork, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate");
-				if ((e = hipEventCreateWithFlags(&probeJoin, hipEventDisableTiming)) != hipSuccess) return hipfail(e, "hipEventCreate"); }
 			// each queue segment takes the resolve blocks with index % kQueueSegs == seg
 			mlpSegCap = static_cast<uint32_t>((static_cast<long long>(b.nBlocks) + kQueueSegs - 1) / kQueueSegs * 256);
-			if ((e = Grow(mlpQueue, static_cast<size_t>(mlpSegCap) * kQueueSegs * kMlpSample)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
+			if ((e = Grow(mlpQueue, static_cast<size_t>(mlpSegCap) * kQueueSegs * kQueueWords)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
 			if (!mlpShadeBlocksSet) {
 				int cus = 0;
 				if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0) mlpShadeBlocks = cus;
@@ -1169,16 +1160,15 @@ struct rgl_gpu {
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		const DrawLite* cLite = dLite.p;
 		const DrawDesc* cDraws = dDraws.p;
-		if (learned && b.nTiles > 0) {
-			// the probes' plain colours (they need the keys, not the network)
-			// on a second stream, beside the gather and the network pass
-			if ((e = hipEventRecord(probeFork, stream)) != hipSuccess) return hipfail(e, "hipEventRecord");
-			if ((e = hipStreamWaitEvent(probeStream, probeFork, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
-			k_mlp_probe<ProgGGXEnv><<<b.nTiles, kMlpProbes, dlBytes, probeStream>>>(b, ra, probeWant.p);
-			if ((e = hipEventRecord(probeJoin, probeStream)) != hipSuccess) return hipfail(e, "hipEventRecord"); }
+		// a limit of 1 or more (255 per channel) re-shades no tile: no probes
+		const bool fallback = learned && b.nTiles > 0 && mlpMaxRms < 1.0F;
 		if (learned) {
+			// the gather's leading workgroups compute the tiles' probes (their
+			// plain colours need the keys, not the network)
+			const int probeBlocks = fallback ? (b.nTiles + kProbeTilesPerBlock - 1) / kProbeTilesPerBlock : 0;
 			k_mlp_pack<<<16, 256, 0, stream>>>(mlpParams, mlpImg.p);
-			k_mlp_gather<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap); }
+			k_mlp_gather<ProgGGXEnv><<<rblocks + probeBlocks, 256, dlBytes, stream>>>(b, ra, mlpQueue.p, qCount.p, mlpSegCap,
+				probeWant.p, probeBlocks, mlpImg.p); }
 		else {
 			// a batch of one program resolves with it compiled in, others
 			// dispatch over the program list per pixel
@@ -1188,10 +1178,9 @@ struct rgl_gpu {
 		Mark(7);
 		if (learned) k_mlp_shade<<<mlpShadeBlocks, 64 * kMlpWaves, 0, stream>>>(b, mlpQueue.p, qCount.p, mlpSegCap, mlpImg.p);
 		Mark(8);
-		if (learned && b.nTiles > 0) {
+		if (fallback) {
 			// plain re-shading of the tiles whose probes show an error over
-			// the limit, once the probes have joined
-			if ((e = hipStreamWaitEvent(stream, probeJoin, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
+			// the limit
 			const float lim = mlpMaxRms * 255.0F;
 			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, probeWant.p, lim * lim, &dCtr->fallbackTiles); }
 		for (const SampleReq& q : reqs)

@@ -1389,8 +1389,7 @@ __device__ inline void resolve_store(const BatchDesc& b, const FrameDesc& f, con
 // each frame's blocks row-major over kResolveW x kResolveH pixel blocks
 struct BlockPixel { int frame, x, y; };
 
-__device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb) {
-	const int blk = static_cast<int>(blockIdx.x);
+__device__ inline BlockPixel block_pixel(const BatchDesc& b, const int* fbb, int blk = static_cast<int>(blockIdx.x)) {
 	BlockPixel r;
 	r.frame = find_base(fbb, b.nFrames, blk);
 	const FrameDesc& f = b.frames[r.frame];

@@ -77,6 +77,12 @@ __device__ inline uint32_t bf16_pack(float a, float b) {
 __device__ inline f32x4_t mfma_bf16(const bf16x8_t& a, const bf16x8_t& b, const f32x4_t& c) {
 	return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
+// ReLU of two packed bf16 values: as 16-bit integers, negative values
+// (sign bit set, -0 included) are below 0 and positive ones keep their bits
+typedef short i16x2_t __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t relu_bf16x2(uint32_t v) {
+	return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2_t, v), (i16x2_t){ 0, 0 })); }
+
 __device__ inline bf16x8_t lds_bf16x8(const uint16_t* p) { return *reinterpret_cast<const bf16x8_t*>(p); }
 
 // fp32 parameters -> bf16 LDS image
@@ -135,11 +141,14 @@ __device__ inline void wave_sync() {
 __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* xc, uint16_t* hb,
                                 const float x[kMlpFeat], float y[kMlpOut]) {
 	const int lane = threadIdx.x & 63, r16 = lane & 15, g = lane >> 4;
+	// each product starts from its layer's biases (the accumulator rows of
+	// tile i are neurons 16 i + 4 g .. + 3), so no epilogue adds them
 	f32x4_t acc[4][4];
 #pragma unroll
-	for (int i = 0; i < 4; ++i)
+	for (int i = 0; i < 4; ++i) {
+		const f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bias + 16 * i + 4 * g);
 #pragma unroll
-		for (int j = 0; j < 4; ++j) acc[i][j] = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+		for (int j = 0; j < 4; ++j) acc[i][j] = bv; }
 	// layer 1, K = 128 in four chunks of 32 encoded inputs
 #pragma unroll
 	for (int c = 0; c < kMlpK1 / 32; ++c) {
@@ -164,22 +173,19 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 	// layers 2 and 3: transposed products H^T = W H_prev^T, samples along N
 #pragma unroll
 	for (int layer = 0; layer < 3; ++layer) {
-		const float* bl = bias + layer * kMlpHid;
-		// epilogue of the previous product: bias, ReLU, bf16, H[sample][neuron]
+		// epilogue of the previous product: bf16, ReLU, H[sample][neuron]
 #pragma unroll
 		for (int i = 0; i < 4; ++i)
 #pragma unroll
 			for (int j = 0; j < 4; ++j) {
 				const int n0 = 16 * i + 4 * g;
-				float hv[4];
-#pragma unroll
-				for (int r = 0; r < 4; ++r) {
-					const float v = acc[i][j][r] + bl[n0 + r];
-					hv[r] = v > 0.0F ? v : 0.0F; }
-				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) = make_uint2(bf16_pack(hv[0], hv[1]), bf16_pack(hv[2], hv[3])); }
+				const f32x4_t v = acc[i][j];
+				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) =
+					make_uint2(relu_bf16x2(bf16_pack(v[0], v[1])), relu_bf16x2(bf16_pack(v[2], v[3]))); }
 		wave_sync();
 		if (layer == 2) break;
 		const uint16_t* Wl = W + (layer == 0 ? kImgW2 : kImgW3);
+		const float* bl = bias + (layer + 1) * kMlpHid;
 		bf16x8_t a[2][4], b[2][4];
 #pragma unroll
 		for (int c = 0; c < 2; ++c) {
@@ -188,12 +194,10 @@ __device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* 
 #pragma unroll
 			for (int j = 0; j < 4; ++j) b[c][j] = lds_bf16x8(hb + (16 * j + r16) * kLdH + 32 * c + g * 8); }
 #pragma unroll
-		for (int i = 0; i < 4; ++i)
+		for (int i = 0; i < 4; ++i) {
+			const f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bl + 16 * i + 4 * g);
 #pragma unroll
-			for (int j = 0; j < 4; ++j) {
-				f32x4_t s = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
-				s = mfma_bf16(a[0][i], b[0][j], s);
-				acc[i][j] = mfma_bf16(a[1][i], b[1][j], s); }
+			for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16(a[1][i], b[1][j], mfma_bf16(a[0][i], b[0][j], bv)); }
 		wave_sync(); }
 	// layer 4: 3 outputs in rows 0..2 of one 16-row tile
 	f32x4_t o[4];
@@ -329,34 +333,97 @@ __device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, co
 	prim = dd.primBase + srcTri;
 	return true; }
 
+// Per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
+// each tile (one in each cell of an 8 x 8 grid), compared with the
+// network's colour there.  The probes' plain colours do not depend on the
+// network: the gather pass computes them in leading workgroups of its own
+// (one wave per tile) while its other workgroups queue the fragments; the
+// comparison is made by the fallback pass, after the network has stored
+// its colours.
+constexpr int kMlpProbes = 64;
+constexpr int kProbeTilesPerBlock = 256 / kMlpProbes;   // tiles per gather workgroup of probes
+constexpr uint32_t kNoProbe = 0xffffffffu;   // probe pixel not covered (or no colour output)
+
+__device__ inline uint32_t hash32(uint32_t v) {
+	v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16; return v; }
+
+// probe j of batch tile `tile`, whose pixel rect is R; false when the cell
+// falls outside the rect
+__device__ inline bool probe_pixel(const Rect& R, int tile, int j, int& x, int& y) {
+	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
+	const uint32_t id = static_cast<uint32_t>(tile) * kMlpProbes + static_cast<uint32_t>(j);
+	const uint32_t h = hash32(id * 0x9e3779b9u + 17u);
+	x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
+	y = R.y0 + (j >> 3) * ch + static_cast<int>((h >> 16) % static_cast<uint32_t>(ch));
+	return x < R.x1 && y < R.y1; }
+
+// one wave: the plain colour of each probe pixel of tile `tile` (lane j:
+// probe j), or kNoProbe
+template <class P>
+__device__ inline void tile_probes(const BatchDesc& b, const RasterArgs& a, const DrawLite* dl, const int* ftb, int tile,
+                                   const uint32_t* rcpTab, uint32_t* __restrict__ want) {
+	const int j = static_cast<int>(threadIdx.x & 63);
+	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+	uint32_t w = kNoProbe;
+	int x, y;
+	if (f.outColor != nullptr && probe_pixel(tile_rect(f, tile - f.tileBase), tile, j, x, y)) {
+		float col[3], dep;
+		uint32_t prim;
+		if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, rcpTab, col, dep, prim)) w = post_pixel(f, col, x, y, kSrgb8Tab); }
+	want[static_cast<long long>(tile) * kMlpProbes + j] = w; }
+
+// A queued fragment, 32 bytes: its 13 network inputs (normalised as the
+// network reads them) in 16-bit fixed point over [-2, 2] (steps of 6.1e-5,
+// well inside bf16's rounding of the encoded values; the inputs of a
+// fragment inside the normalisation box lie in [-1, 1]), its frame, and
+// its pixel index.
+constexpr int kQueueWords = 8;
+constexpr float kQueueScale = 16383.5F;   // 32767 / 2
+
+__device__ inline uint32_t queue_q16(float v) {
+	v = v > 2.0F ? 2.0F : (v < -2.0F ? -2.0F : v);
+	return static_cast<uint32_t>(static_cast<int>(__builtin_rintf(v * kQueueScale))) & 0xffffu; }
+
+__device__ inline float queue_f16(uint32_t h) {
+	return static_cast<float>(static_cast<int>(static_cast<int16_t>(static_cast<uint16_t>(h)))) * (1.0F / kQueueScale); }
+
 // learned resolve, pass 1: like k_resolve for depth, ids and the clear
-// colour; each covered pixel's shading inputs are queued (16 floats: the 13
-// inputs, then the pixel index) for the dense network pass.  The queue is
+// colour; each covered pixel's network inputs are queued (the 32-byte
+// record above) for the dense network pass.  The queue is
 // cut into kQueueSegs segments with a counter each, picked by block index,
 // so that consecutive workgroups (dealt round-robin to the XCDs) append
-// through different counters.
+// through different counters.  The first probeBlocks workgroups compute
+// the tiles' probes instead (kProbeTilesPerBlock tiles each), beside the
+// pixel workgroups.
 constexpr int kQueueSegs = 8;
 constexpr int kQueueStride = 32;   // counters 128 bytes apart
 
 template <class P>
 __global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, float* __restrict__ queue, uint32_t* __restrict__ count,
-                                                    uint32_t segCap) {
+                                                    uint32_t segCap, uint32_t* __restrict__ want, int probeBlocks,
+                                                    const uint32_t* __restrict__ img) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ uint32_t rcpTab[2048];
 	__shared__ int fbb[kMaxFrames + 1];
+	const int blk = static_cast<int>(blockIdx.x);
 	load_lite(dl, a.draws, b.nDraws);
-	load_block_bases(fbb, b);
-	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
+	if (blk < probeBlocks) load_tile_bases(fbb, b);
+	else load_block_bases(fbb, b);
 	__syncthreads();
-	const BlockPixel bp = block_pixel(b, fbb);
+	// the rcpps and sRGB tables are read through the caches, as the plain
+	// resolve reads them
+	if (blk < probeBlocks) {   // uniform over the block
+		const int tile = blk * kProbeTilesPerBlock + static_cast<int>(threadIdx.x >> 6);
+		if (tile < b.nTiles) tile_probes<P>(b, a, dl, fbb, tile, kRcpMantissa, want);
+		return; }
+	const BlockPixel bp = block_pixel(b, fbb, blk - probeBlocks);
 	const FrameDesc& f = b.frames[bp.frame];
 	const int x = bp.x, y = bp.y;
 	const bool on = x < f.W && y < f.H;
 	ShadeInput s;
 	float dep = f.clearDepth;
 	uint32_t prim = 0xffffffffu;
-	const bool hit = on && resolve_inputs<P>(b, f, a, dl, x, y, rcpTab, s, dep, prim);
+	const bool hit = on && resolve_inputs<P>(b, f, a, dl, x, y, kRcpMantissa, s, dep, prim);
 	const long long o = static_cast<long long>(y) * f.W + x;
 	// one queue reservation per workgroup
 	__shared__ uint32_t wcnt[5];
@@ -364,7 +431,7 @@ __global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, f
 	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 	if (lane == 0) wcnt[wv] = static_cast<uint32_t>(__popcll(m));
 	__syncthreads();
-	const int seg = static_cast<int>(blockIdx.x % kQueueSegs);
+	const int seg = static_cast<int>((blk - probeBlocks) % kQueueSegs);
 	if (threadIdx.x == 0) {
 		const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
 		wcnt[4] = tot ? atomicAdd(&count[seg * kQueueStride], tot) : 0u; }
@@ -374,11 +441,15 @@ __global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, f
 	if (hit) {
 		const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << lane) - 1ull)));
 		if (slot < segCap) {
-			float4* r = reinterpret_cast<float4*>(queue + (static_cast<long long>(seg) * segCap + slot) * kMlpSample);
-			r[0] = make_float4(s.n[0], s.n[1], s.n[2], s.v[0]);
-			r[1] = make_float4(s.v[1], s.v[2], s.p[0], s.p[1]);
-			r[2] = make_float4(s.p[2], s.albedo[0], s.albedo[1], s.albedo[2]);
-			r[3] = make_float4(s.rough, u2f(static_cast<uint32_t>(o)), u2f(static_cast<uint32_t>(bp.frame)), 0.0F); }}
+			// the position's normalisation sits after the biases of the image
+			float xin[kMlpFeat];
+			mlp_inputs(s, reinterpret_cast<const float*>(img) + kImgBiasF + 3 * kMlpHid + 4, xin);
+			uint32_t w[6];
+#pragma unroll
+			for (int c = 0; c < 6; ++c) w[c] = queue_q16(xin[2 * c]) | (queue_q16(xin[2 * c + 1]) << 16);
+			uint4* r = reinterpret_cast<uint4*>(queue + (static_cast<long long>(seg) * segCap + slot) * kQueueWords);
+			r[0] = make_uint4(w[0], w[1], w[2], w[3]);
+			r[1] = make_uint4(w[4], w[5], queue_q16(xin[12]) | (static_cast<uint32_t>(bp.frame) << 16), static_cast<uint32_t>(o)); }}
 	if (!on) return;
 	if (f.outColor && !hit) f.outColor[o] = post_pixel(f, f.clearColor, x, y, kSrgb8Tab);
 	if (f.outDepth) f.outDepth[o] = dep;
@@ -406,62 +477,18 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const
 		const uint32_t base = (item - first[seg]) * kRound;
 		const uint32_t i = base + threadIdx.x;
 		const bool ok = i < cnt[seg];
-		const float4* r = reinterpret_cast<const float4*>(queue + (static_cast<long long>(seg) * segCap + (ok ? i : base)) * kMlpSample);
-		const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-		ShadeInput s;
-		s.n[0] = r0.x; s.n[1] = r0.y; s.n[2] = r0.z; s.v[0] = r0.w; s.v[1] = r1.x; s.v[2] = r1.y;
-		s.p[0] = r1.z; s.p[1] = r1.w; s.p[2] = r2.x; s.albedo[0] = r2.y; s.albedo[1] = r2.z; s.albedo[2] = r2.w;
-		s.rough = r3.x;
-		const uint32_t o = f2u(r3.y);
+		const uint4* r = reinterpret_cast<const uint4*>(queue + (static_cast<long long>(seg) * segCap + (ok ? i : base)) * kQueueWords);
+		const uint4 r0 = r[0], r1 = r[1];
+		const uint32_t w[7] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z };
 		float x[kMlpFeat];
-		mlp_inputs(s, sh.norm, x);
+#pragma unroll
+		for (int c = 0; c < kMlpFeat; ++c) x[c] = queue_f16(w[c >> 1] >> (16 * (c & 1)));
+		const uint32_t o = r1.w;
 		float y[kMlpOut];
 		mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
 		if (ok) {
-			uint32_t* out = b.frames[f2u(r3.z)].outColor;
+			uint32_t* out = b.frames[r1.z >> 16].outColor;
 			if (out) out[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}}
-
-// Per-tile error estimate: the plain shader at kMlpProbes jittered pixels of
-// each tile (one in each cell of an 8 x 8 grid), compared with the
-// network's colour there.  The probes' plain colours do not depend on the
-// network, so k_mlp_probe runs beside the gather and the network pass (on
-// a second stream); the comparison is made by the fallback pass, after the
-// network has stored its colours.
-constexpr int kMlpProbes = 64;
-constexpr uint32_t kNoProbe = 0xffffffffu;   // probe pixel not covered (or no colour output)
-
-__device__ inline uint32_t hash32(uint32_t v) {
-	v ^= v >> 16; v *= 0x7feb352dU; v ^= v >> 15; v *= 0x846ca68bU; v ^= v >> 16; return v; }
-
-// probe j of batch tile `tile`, whose pixel rect is R; false when the cell
-// falls outside the rect
-__device__ inline bool probe_pixel(const Rect& R, int tile, int j, int& x, int& y) {
-	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;   // 8 x 8 cells
-	const uint32_t id = static_cast<uint32_t>(tile) * kMlpProbes + static_cast<uint32_t>(j);
-	const uint32_t h = hash32(id * 0x9e3779b9u + 17u);
-	x = R.x0 + (j & 7) * cw + static_cast<int>(h % static_cast<uint32_t>(cw));
-	y = R.y0 + (j >> 3) * ch + static_cast<int>((h >> 16) % static_cast<uint32_t>(ch));
-	return x < R.x1 && y < R.y1; }
-
-// one workgroup (one wave) per tile: the plain colour of each probe pixel,
-// or kNoProbe
-template <class P>
-__global__ __launch_bounds__(kMlpProbes) void k_mlp_probe(BatchDesc b, RasterArgs a, uint32_t* __restrict__ want) {
-	extern __shared__ uint4 dyn_lds[];   // the batch's draws
-	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ int ftb[kMaxFrames + 1];
-	load_lite(dl, a.draws, b.nDraws);
-	load_tile_bases(ftb, b);
-	__syncthreads();
-	const int tile = static_cast<int>(blockIdx.x), j = static_cast<int>(threadIdx.x);
-	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
-	uint32_t w = kNoProbe;
-	int x, y;
-	if (f.outColor != nullptr && probe_pixel(tile_rect(f, tile - f.tileBase), tile, j, x, y)) {
-		float col[3], dep;
-		uint32_t prim;
-		if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim)) w = post_pixel(f, col, x, y, kSrgb8Tab); }
-	want[static_cast<long long>(tile) * kMlpProbes + j] = w; }
 
 // the fallback decision of one tile, by one wave: the mean squared 8-bit
 // error per channel at its probes (the network's stored colour against
@@ -489,8 +516,6 @@ __global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a,
                                                       float maxMse, uint32_t* __restrict__ nFallback) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ uint32_t rcpTab[2048];
-	__shared__ uint32_t srgbTab[104];
 	__shared__ int fbb[kMaxFrames + 1];
 	constexpr int kMaxBlockTiles = (kResolveW / 8 + 1) * (kResolveH / 8 + 1);   // tiles are at least 8 px
 	__shared__ uint32_t over[kMaxBlockTiles];
@@ -514,16 +539,14 @@ __global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a,
 	for (int k = 0; k < nbt; ++k) any = any || over[k] != 0u;
 	if (!any) return;   // uniform over the block
 	load_lite(dl, a.draws, b.nDraws);
-	for (int i = threadIdx.x; i < 2048; i += 256) rcpTab[i] = kRcpMantissa[i];
-	for (int i = threadIdx.x; i < 104; i += 256) srgbTab[i] = kSrgb8Tab[i];
 	__syncthreads();
 	if (x >= f.W || y >= f.H) return;
 	if (!over[(y / f.tileH - ty0) * ntx + x / f.tileW - tx0]) return;
 	if (x % f.tileW == 0 && y % f.tileH == 0) atomicAdd(nFallback, 1u);
 	float col[3], dep;
 	uint32_t prim;
-	if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, rcpTab, col, dep, prim))
-		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, x, y, srgbTab); }
+	if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim))
+		f.outColor[static_cast<long long>(y) * f.W + x] = post_pixel(f, col, x, y, kSrgb8Tab); }
 
 // training samples from a frame: covered pixels kept with probability
 // `keep` (hash of pixel and seed), appended as 13 inputs + the plain
