@@ -10,8 +10,10 @@
 //
 // Inference follows the resolve: a gather pass queues the shading inputs
 // of covered pixels, a dense persistent pass runs them through the network,
-// 64 per wave; layer activations go through LDS in the operand layouts MFMA
-// reads, weights sit in LDS for the whole workgroup.  Training uses a generic bf16 MFMA
+// 64 per wave; the weights sit in LDS for the whole workgroup, and the
+// encoded inputs and layer activations stay in registers, in the lanes the
+// next product reads them from (the weight image's K columns are ordered
+// to match).  Training uses a generic bf16 MFMA
 // GEMM (operands rounded to bf16 when staged, FP32 accumulation) for the
 // forward and backward passes, split-K weight gradients and an Adam update.
 // No part of this has a counterpart in the reference, whose shaders are
@@ -85,7 +87,26 @@ __device__ inline uint32_t relu_bf16x2(uint32_t v) {
 
 __device__ inline bf16x8_t lds_bf16x8(const uint16_t* p) { return *reinterpret_cast<const bf16x8_t*>(p); }
 
-// fp32 parameters -> bf16 LDS image
+// K slot order of the weight image.  A 16x16x32 MFMA reads K slots
+// 8 g .. 8 g + 7 of B from lane group g (= lane / 16), and leaves lane
+// group g the result rows 4 g .. 4 g + 3.
+// Layer 1: lane group g encodes features 3 g .. 3 g + 2 (slots q = 0..26
+// of its 32, q = 8 c + t over the four K chunks c) and part of feature 12
+// (group 0: its value and bands 0 and 1; group 1: bands 2 and 3).  Slot
+// 32 c + 8 g + t therefore holds encoded input w1_slot_input (-1: none).
+__device__ inline int w1_slot_input(int col) {
+	const int c = col >> 5, g = (col >> 3) & 3, q = 8 * c + (col & 7);
+	if (q < 27) return 27 * g + q;
+	if (g == 0) return 108 + q - 27;
+	if (g == 1 && q < 31) return 113 + q - 27;
+	return -1; }
+// Hidden layers: K chunk c of the next product takes, in lane group g, the
+// rows 4 g .. 4 g + 3 of result tiles 2 c and 2 c + 1.
+__device__ inline int hid_slot_input(int col) {
+	const int c = col >> 5, g = (col >> 3) & 3, t = col & 7;
+	return 32 * c + (t < 4 ? 4 * g + t : 16 + 4 * g + t - 4); }
+
+// fp32 parameters -> bf16 LDS image, K columns in slot order
 __global__ void k_mlp_pack(const float* __restrict__ prm, uint32_t* __restrict__ img) {
 	uint16_t* h = reinterpret_cast<uint16_t*>(img);
 	float* fl = reinterpret_cast<float*>(img);
@@ -94,10 +115,15 @@ __global__ void k_mlp_pack(const float* __restrict__ prm, uint32_t* __restrict__
 			for (int half = 0; half < 2; ++half) {
 				const int e = 2 * i + half;
 				float v = 0.0F;
-				if (e < kImgW2) { const int r = e / kLd1, c = e - r * kLd1; if (c < kMlpK1) v = prm[kW1 + r * kMlpK1 + c]; }
-				else if (e < kImgW3) { const int q = e - kImgW2, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW2 + r * kMlpHid + c]; }
-				else if (e < kImgW4) { const int q = e - kImgW3, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW3 + r * kMlpHid + c]; }
-				else { const int q = e - kImgW4, r = q / kLdH, c = q - r * kLdH; if (r < kMlpOut && c < kMlpHid) v = prm[kW4 + r * kMlpHid + c]; }
+				if (e < kImgW2) {
+					const int r = e / kLd1, c = e - r * kLd1;
+					const int src = c < kMlpK1 ? w1_slot_input(c) : -1;
+					if (src >= 0) v = prm[kW1 + r * kMlpK1 + src]; }
+				else if (e < kImgW3) { const int q = e - kImgW2, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW2 + r * kMlpHid + hid_slot_input(c)]; }
+				else if (e < kImgW4) { const int q = e - kImgW3, r = q / kLdH, c = q - r * kLdH; if (c < kMlpHid) v = prm[kW3 + r * kMlpHid + hid_slot_input(c)]; }
+				else {
+					const int q = e - kImgW4, r = q / kLdH, c = q - r * kLdH;
+					if (r < kMlpOut && c < kMlpHid) v = prm[kW4 + r * kMlpHid + hid_slot_input(c)]; }
 				h[e] = bf16_bits(v); }}
 		else {
 			const int q = i - kImgBiasF;
@@ -127,115 +153,120 @@ __device__ inline float mlp_enc_value(const float x[kMlpFeat], int e) {
 	for (int q = 0; q < kk; ++q) { const float s2 = 2.0F * sn * cs, c2 = (cs - sn) * (cs + sn); sn = s2; cs = c2; }
 	return (j & 1) ? sn : cs; }
 
-// orders the wave's own LDS writes before its later reads (no workgroup barrier)
-__device__ inline void wave_sync() {
-	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront"); }
+// sin and cos of 2^k pi v, k = 0..3, by the double-angle recurrence from
+// sin(pi v), cos(pi v) (the operations of mlp_enc_value)
+__device__ inline void enc_bands(float v, float sn[4], float cs[4]) {
+	sn[0] = __builtin_amdgcn_sinf(0.5F * v); cs[0] = __builtin_amdgcn_cosf(0.5F * v);   // revolutions
+#pragma unroll
+	for (int k = 1; k < kMlpBands; ++k) { sn[k] = 2.0F * sn[k - 1] * cs[k - 1]; cs[k] = (cs[k - 1] - sn[k - 1]) * (cs[k - 1] + sn[k - 1]); }}
 
-// one wave: 64 samples (one per lane) through the network.  xc: this
-// wave's [64][40] chunk of encoded inputs; hb: its [64][72] activations;
-// W: the LDS image.  The wave touches no other wave's slices, so its
-// phases are ordered within the wave alone (LDS operations of one wave
-// complete in order) and the workgroup's waves run through it independently.
-__device__ inline void mlp_wave(const uint16_t* W, const float* bias, uint16_t* xc, uint16_t* hb,
-                                const float x[kMlpFeat], float y[kMlpOut]) {
+// one feature's 9 encoded values: x, then (sin, cos)(2^k pi x), k = 0..3,
+// as bf16 into slot pairs of out starting at slot q (q + 9 <= 32)
+__device__ inline void enc_feature(float x, float (&v)[32], int q) {
+	float sn[4], cs[4];
+	enc_bands(x, sn, cs);
+	v[q] = x;
+#pragma unroll
+	for (int b = 0; b < kMlpBands; ++b) { v[q + 1 + 2 * b] = sn[b]; v[q + 2 + 2 * b] = cs[b]; }}
+
+// lane group g's 32 layer-1 K slots of one sample (w1_slot_input's order)
+// from the sample's inputs 3 g .. 3 g + 2 (fa, fb, fc) and 12 (fr), as 16
+// bf16 pairs.  The inputs come as scalars picked by the caller: an array
+// indexed by g would be kept in scratch memory.
+__device__ inline void enc_group(int g, float fa, float fb, float fc, float fr, uint32_t out[16]) {
+	float v[32];
+	enc_feature(fa, v, 0);
+	enc_feature(fb, v, 9);
+	enc_feature(fc, v, 18);
+	float sn[4], cs[4];
+	enc_bands(fr, sn, cs);
+	const bool g0 = g == 0, g1 = g == 1;
+	v[27] = g0 ? fr : (g1 ? sn[2] : 0.0F);
+	v[28] = g0 ? sn[0] : (g1 ? cs[2] : 0.0F);
+	v[29] = g0 ? cs[0] : (g1 ? sn[3] : 0.0F);
+	v[30] = g0 ? sn[1] : (g1 ? cs[3] : 0.0F);
+	v[31] = g0 ? cs[1] : 0.0F;
+#pragma unroll
+	for (int u = 0; u < 16; ++u) out[u] = bf16_pack(v[2 * u], v[2 * u + 1]); }
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ inline bf16x8_t as_bf16x8(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+	return __builtin_bit_cast(bf16x8_t, (u32x4_t){ a, b, c, d }); }
+
+// one wave: 64 samples through the network.  Lane (g, r) = (lane / 16,
+// lane % 16) holds samples 16 j + r, j = 0..3: enc[j] is sample j's
+// layer-1 K slots of group g, and lanes of group 0 get y[j], its three
+// outputs.  Every product starts from its layer's biases; activations are
+// ReLU'd to bf16 in registers and read by the next product where they lie;
+// W (the LDS image) is the only operand read from memory.
+__device__ inline void mlp_wave(const uint16_t* W, const float* bias, const uint32_t enc[4][16], float y[4][kMlpOut]) {
 	const int lane = threadIdx.x & 63, r16 = lane & 15, g = lane >> 4;
-	// each product starts from its layer's biases (the accumulator rows of
-	// tile i are neurons 16 i + 4 g .. + 3), so no epilogue adds them
 	f32x4_t acc[4][4];
 #pragma unroll
 	for (int i = 0; i < 4; ++i) {
 		const f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bias + 16 * i + 4 * g);
 #pragma unroll
 		for (int j = 0; j < 4; ++j) acc[i][j] = bv; }
-	// layer 1, K = 128 in four chunks of 32 encoded inputs
+	// layer 1, K = 128 in four chunks of 32 slots
 #pragma unroll
 	for (int c = 0; c < kMlpK1 / 32; ++c) {
-		uint32_t w[16];
-#pragma unroll
-		for (int q = 0; q < 16; ++q)
-			w[q] = bf16_pack(mlp_enc_value(x, 32 * c + 2 * q), mlp_enc_value(x, 32 * c + 2 * q + 1));
-		uint4* dst = reinterpret_cast<uint4*>(xc + lane * 40);
-#pragma unroll
-		for (int q = 0; q < 4; ++q) dst[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-		wave_sync();
-		bf16x8_t a[4], b[4];
+		bf16x8_t a[4];
 #pragma unroll
 		for (int i = 0; i < 4; ++i) a[i] = lds_bf16x8(W + kImgW1 + (16 * i + r16) * kLd1 + 32 * c + g * 8);
 #pragma unroll
-		for (int j = 0; j < 4; ++j) b[j] = lds_bf16x8(xc + (16 * j + r16) * 40 + g * 8);
+		for (int j = 0; j < 4; ++j) {
+			const bf16x8_t bj = as_bf16x8(enc[j][4 * c], enc[j][4 * c + 1], enc[j][4 * c + 2], enc[j][4 * c + 3]);
 #pragma unroll
-		for (int i = 0; i < 4; ++i)
-#pragma unroll
-			for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16(a[i], b[j], acc[i][j]);
-		wave_sync(); }
-	// layers 2 and 3: transposed products H^T = W H_prev^T, samples along N
+			for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(a[i], bj, acc[i][j]); }}
+	// layers 2 and 3, then the output layer; h[c][j]: K chunk c of tile j
+	bf16x8_t h[2][4];
 #pragma unroll
 	for (int layer = 0; layer < 3; ++layer) {
-		// epilogue of the previous product: bf16, ReLU, H[sample][neuron]
 #pragma unroll
-		for (int i = 0; i < 4; ++i)
+		for (int c = 0; c < 2; ++c)
 #pragma unroll
 			for (int j = 0; j < 4; ++j) {
-				const int n0 = 16 * i + 4 * g;
-				const f32x4_t v = acc[i][j];
-				*reinterpret_cast<uint2*>(hb + (16 * j + r16) * kLdH + n0) =
-					make_uint2(relu_bf16x2(bf16_pack(v[0], v[1])), relu_bf16x2(bf16_pack(v[2], v[3]))); }
-		wave_sync();
+				const f32x4_t lo = acc[2 * c][j], hi = acc[2 * c + 1][j];
+				h[c][j] = as_bf16x8(relu_bf16x2(bf16_pack(lo[0], lo[1])), relu_bf16x2(bf16_pack(lo[2], lo[3])),
+				                    relu_bf16x2(bf16_pack(hi[0], hi[1])), relu_bf16x2(bf16_pack(hi[2], hi[3]))); }
 		if (layer == 2) break;
 		const uint16_t* Wl = W + (layer == 0 ? kImgW2 : kImgW3);
 		const float* bl = bias + (layer + 1) * kMlpHid;
-		bf16x8_t a[2][4], b[2][4];
-#pragma unroll
-		for (int c = 0; c < 2; ++c) {
-#pragma unroll
-			for (int i = 0; i < 4; ++i) a[c][i] = lds_bf16x8(Wl + (16 * i + r16) * kLdH + 32 * c + g * 8);
-#pragma unroll
-			for (int j = 0; j < 4; ++j) b[c][j] = lds_bf16x8(hb + (16 * j + r16) * kLdH + 32 * c + g * 8); }
 #pragma unroll
 		for (int i = 0; i < 4; ++i) {
 			const f32x4_t bv = *reinterpret_cast<const f32x4_t*>(bl + 16 * i + 4 * g);
 #pragma unroll
-			for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16(a[1][i], b[1][j], mfma_bf16(a[0][i], b[0][j], bv)); }
-		wave_sync(); }
-	// layer 4: 3 outputs in rows 0..2 of one 16-row tile
-	f32x4_t o[4];
-	{
-		bf16x8_t a[2], b[2][4];
+			for (int j = 0; j < 4; ++j) acc[i][j] = bv; }
 #pragma unroll
 		for (int c = 0; c < 2; ++c) {
-			a[c] = lds_bf16x8(W + kImgW4 + r16 * kLdH + 32 * c + g * 8);
+			bf16x8_t a[4];
 #pragma unroll
-			for (int j = 0; j < 4; ++j) b[c][j] = lds_bf16x8(hb + (16 * j + r16) * kLdH + 32 * c + g * 8); }
+			for (int i = 0; i < 4; ++i) a[i] = lds_bf16x8(Wl + (16 * i + r16) * kLdH + 32 * c + g * 8);
 #pragma unroll
-		for (int j = 0; j < 4; ++j) {
-			f32x4_t s = f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
-			s = mfma_bf16(a[0], b[0][j], s);
-			o[j] = mfma_bf16(a[1], b[1][j], s); }
-	}
-	wave_sync();
-	// lanes 0..15 hold rows 0..3 (channels) of samples 16 j + lane
-	float* yo = reinterpret_cast<float*>(xc);
-	if (g == 0) {
+			for (int j = 0; j < 4; ++j)
 #pragma unroll
-		for (int j = 0; j < 4; ++j)
+				for (int i = 0; i < 4; ++i) acc[i][j] = mfma_bf16(a[i], h[c][j], acc[i][j]); }}
+	// output layer: 3 outputs in rows 0..2 of one 16-row tile (lane group 0)
+	const f32x4_t b4 = g == 0 ? *reinterpret_cast<const f32x4_t*>(bias + 3 * kMlpHid) : f32x4_t{ 0.0F, 0.0F, 0.0F, 0.0F };
+	f32x4_t o[4] = { b4, b4, b4, b4 };
 #pragma unroll
-			for (int r = 0; r < 4; ++r) yo[(16 * j + r16) * 4 + r] = o[j][r]; }
-	wave_sync();
-	const float* b4 = bias + 3 * kMlpHid;
+	for (int c = 0; c < 2; ++c) {
+		const bf16x8_t a = lds_bf16x8(W + kImgW4 + r16 * kLdH + 32 * c + g * 8);
 #pragma unroll
-	for (int c = 0; c < kMlpOut; ++c) y[c] = yo[lane * 4 + c] + b4[c];
-	wave_sync(); }
+		for (int j = 0; j < 4; ++j) o[j] = mfma_bf16(a, h[c][j], o[j]); }
+#pragma unroll
+	for (int j = 0; j < 4; ++j)
+#pragma unroll
+		for (int ch = 0; ch < kMlpOut; ++ch) y[j][ch] = o[j][ch]; }
 
-constexpr int kMlpWaves = 8;   // waves per inference workgroup (64 samples each)
+// waves per inference workgroup (64 samples each); the workgroup's LDS
+// holds only the weight image, and 8 waves leave each up to 256 VGPRs
+constexpr int kMlpWaves = 8;
 
 struct MlpShared {
 	uint16_t w[kImgEnd];
 	float bias[3 * kMlpHid + 4];
 	float norm[4];
-	uint16_t xc[kMlpWaves][64 * 40];
-	uint16_t hb[kMlpWaves][64 * kLdH];
 };
 
 __device__ inline void mlp_load_image(MlpShared& sh, const uint32_t* __restrict__ img) {
@@ -251,25 +282,32 @@ __device__ inline uint32_t srgb_unit_to8(float v) {   // sRGB-space [0,1] -> byt
 	v = v < 1.0F ? v : 1.0F;
 	return static_cast<uint32_t>(v * 255.0F + 0.5F); }
 
-// dense inference: n samples of kMlpSample floats (13 inputs used) -> n x 3
+// dense inference: n samples of kMlpSample floats (13 inputs used) -> n x 3;
+// wave w of block k takes samples 64 (k kMlpWaves + w) .. + 63
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(const float* __restrict__ in, int n,
 		const uint32_t* __restrict__ img, float* __restrict__ out) {
 	__shared__ __attribute__((aligned(16))) MlpShared sh;
 	mlp_load_image(sh, img);
 	__syncthreads();
-	const int wv = threadIdx.x >> 6;
-	const long long i = static_cast<long long>(blockIdx.x) * (64 * kMlpWaves) + threadIdx.x;
-	float x[kMlpFeat];
-	ShadeInput s;
-	const bool ok = i < n;
-	const float* r = in + (ok ? i : 0) * kMlpSample;
-	s.n[0] = r[0]; s.n[1] = r[1]; s.n[2] = r[2]; s.v[0] = r[3]; s.v[1] = r[4]; s.v[2] = r[5];
-	s.p[0] = r[6]; s.p[1] = r[7]; s.p[2] = r[8]; s.albedo[0] = r[9]; s.albedo[1] = r[10]; s.albedo[2] = r[11];
-	s.rough = r[12];
-	mlp_inputs(s, sh.norm, x);
-	float y[kMlpOut];
-	mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
-	if (ok) { out[i * 3 + 0] = y[0]; out[i * 3 + 1] = y[1]; out[i * 3 + 2] = y[2]; }}
+	const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, g = lane >> 4;
+	const long long base = (static_cast<long long>(blockIdx.x) * kMlpWaves + wv) * 64;
+	uint32_t enc[4][16];
+#pragma unroll
+	for (int j = 0; j < 4; ++j) {
+		// lane group g reads inputs 3 g .. 3 g + 2 and 12 of its sample;
+		// group 2's are the position, normalised as mlp_inputs does
+		const long long i = base + 16 * j + r16;
+		const float* r = in + (i < n ? i : 0) * kMlpSample;
+		float fa = r[3 * g], fb = r[3 * g + 1], fc = r[3 * g + 2];
+		if (g == 2) { fa = (fa - sh.norm[0]) * sh.norm[3]; fb = (fb - sh.norm[1]) * sh.norm[3]; fc = (fc - sh.norm[2]) * sh.norm[3]; }
+		enc_group(g, fa, fb, fc, r[12], enc[j]); }
+	float y[4][kMlpOut];
+	mlp_wave(sh.w, sh.bias, enc, y);
+	if (g == 0)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const long long i = base + 16 * j + r16;
+			if (i < n) { out[i * 3 + 0] = y[j][0]; out[i * 3 + 1] = y[j][1]; out[i * 3 + 2] = y[j][2]; }}}
 
 // ---------------------------------------------------------------------------
 // learned resolve: like k_resolve, but the winner's shading inputs go
@@ -455,40 +493,56 @@ __global__ __launch_bounds__(256) void k_mlp_gather(BatchDesc b, RasterArgs a, f
 	if (f.outDepth) f.outDepth[o] = dep;
 	if (f.outPrim) f.outPrim[o] = prim; }
 
-// learned resolve, pass 2: the queued fragments through the network, 512
-// per round; persistent workgroups keep the weights in LDS across rounds
+// learned resolve, pass 2: the queued fragments through the network, 64
+// per wave batch; persistent workgroups keep the weights in LDS, and each
+// wave takes batches on its own (no workgroup barrier after the weights)
 __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const float* __restrict__ queue,
 		const uint32_t* __restrict__ count, uint32_t segCap, const uint32_t* __restrict__ img) {
 	__shared__ __attribute__((aligned(16))) MlpShared sh;
 	mlp_load_image(sh, img);
 	__syncthreads();
-	constexpr uint32_t kRound = 64 * kMlpWaves;
 	uint32_t cnt[kQueueSegs], first[kQueueSegs + 1];
 	first[0] = 0;
 #pragma unroll
 	for (int q = 0; q < kQueueSegs; ++q) {
 		cnt[q] = min(count[q * kQueueStride], segCap);
-		first[q + 1] = first[q] + (cnt[q] + kRound - 1) / kRound; }
-	const int wv = threadIdx.x >> 6;
-	for (uint32_t item = blockIdx.x; item < first[kQueueSegs]; item += gridDim.x) {
+		first[q + 1] = first[q] + (cnt[q] + 63u) / 64u; }
+	// the wave's index is uniform: its batch, segment and counts sit in
+	// scalar registers
+	const int wv = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6)), lane = threadIdx.x & 63, g = lane >> 4;
+	const uint32_t stride = gridDim.x * kMlpWaves;
+	for (uint32_t item = blockIdx.x * kMlpWaves + wv; item < first[kQueueSegs]; item += stride) {   // uniform over the wave
+		// the weights' LDS reads stay in the loop: hoisted, the 34 KB image
+		// would not fit the registers (this emits no instruction)
+		__asm__ volatile("" ::: "memory");
 		int seg = 0;
 #pragma unroll
 		for (int q = 1; q < kQueueSegs; ++q) if (item >= first[q]) seg = q;
-		const uint32_t base = (item - first[seg]) * kRound;
-		const uint32_t i = base + threadIdx.x;
-		const bool ok = i < cnt[seg];
-		const uint4* r = reinterpret_cast<const uint4*>(queue + (static_cast<long long>(seg) * segCap + (ok ? i : base)) * kQueueWords);
-		const uint4 r0 = r[0], r1 = r[1];
-		const uint32_t w[7] = { r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z };
-		float x[kMlpFeat];
+		const uint32_t base = (item - first[seg]) * 64u;
+		// lane (g, r) encodes its slots of samples base + 16 j + r
+		uint32_t enc[4][16], po[4], pf[4];
+		bool pok[4];
 #pragma unroll
-		for (int c = 0; c < kMlpFeat; ++c) x[c] = queue_f16(w[c >> 1] >> (16 * (c & 1)));
-		const uint32_t o = r1.w;
-		float y[kMlpOut];
-		mlp_wave(sh.w, sh.bias, sh.xc[wv], sh.hb[wv], x, y);
-		if (ok) {
-			uint32_t* out = b.frames[r1.z >> 16].outColor;
-			if (out) out[o] = (srgb_unit_to8(y[0]) << 16) | (srgb_unit_to8(y[1]) << 8) | srgb_unit_to8(y[2]); }}}
+		for (int j = 0; j < 4; ++j) {
+			const uint32_t i = base + 16u * j + (lane & 15);
+			pok[j] = i < cnt[seg];
+			const uint4* r = reinterpret_cast<const uint4*>(queue + (static_cast<long long>(seg) * segCap + (pok[j] ? i : base)) * kQueueWords);
+			const uint4 r0 = r[0], r1 = r[1];
+			// input f is 16-bit slot f of the record: lane group g's inputs
+			// 3 g, 3 g + 1, 3 g + 2 are picked from the words that hold them
+			const uint32_t wa = g < 2 ? (g == 0 ? r0.x : r0.y) : (g == 2 ? r0.w : r1.x);
+			const uint32_t wb = g < 2 ? (g == 0 ? r0.x : r0.z) : (g == 2 ? r0.w : r1.y);
+			const uint32_t wc = g < 2 ? (g == 0 ? r0.y : r0.z) : (g == 2 ? r1.x : r1.y);
+			const int odd = (g & 1) * 16;   // 3 g is odd for g = 1, 3
+			enc_group(g, queue_f16(wa >> odd), queue_f16(wb >> (16 - odd)), queue_f16(wc >> odd), queue_f16(r1.z), enc[j]);
+			po[j] = r1.w; pf[j] = r1.z >> 16; }
+		float y[4][kMlpOut];
+		mlp_wave(sh.w, sh.bias, enc, y);
+		if (lane < 16)
+#pragma unroll
+			for (int j = 0; j < 4; ++j) {
+				uint32_t* out = b.frames[pf[j]].outColor;
+				if (pok[j] && out) out[po[j]] = (srgb_unit_to8(y[j][0]) << 16) | (srgb_unit_to8(y[j][1]) << 8) | srgb_unit_to8(y[j][2]); }}}
 
 // the fallback decision of one tile, by one wave: the mean squared 8-bit
 // error per channel at its probes (the network's stored colour against

@@ -132,6 +132,64 @@ def test_learned_frame_and_training_gain():
     gpu.close()
 
 
+def _hash32(v):
+    v &= 0xffffffff
+    v ^= v >> 16; v = (v * 0x7feb352d) & 0xffffffff
+    v ^= v >> 15; v = (v * 0x846ca68b) & 0xffffffff
+    v ^= v >> 16
+    return v
+
+
+def _probe(x0, y0, x1, y1, tile, j):
+    # probe j of a tile: one jittered pixel in each cell of an 8 x 8 grid
+    cw, ch = (x1 - x0 + 7) // 8, (y1 - y0 + 7) // 8
+    h = _hash32((((tile * 64 + j) * 0x9e3779b9) & 0xffffffff) + 17)
+    x = x0 + (j & 7) * cw + h % cw
+    y = y0 + (j >> 3) * ch + (h >> 16) % ch
+    return (x, y) if x < x1 and y < y1 else None
+
+
+@pytest.mark.parametrize("max_rms", [2.0 / 255.0, 0.01, 6.0 / 255.0])
+def test_fallback_tiles_follow_the_probes(max_rms):
+    # every tile whose covered probes show a mean squared 8-bit error over
+    # the limit (the network's colours against the plain shader's) is the
+    # plain frame's tile, every other tile the network's, bit for bit
+    import os
+    wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
+    params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    gpu = rgl.GPU(0)
+    sc = scenes.scene_heavy(640, 360)
+    tw, th = sc.tile_blocks[0] * 8, sc.tile_blocks[1] * 8
+    pc, pd, _, _ = _frame(gpu, sc)
+    nc, _, _, _ = _frame(gpu, sc, params, max_rms=1.0)
+    fc, _, _, fst = _frame(gpu, sc, params, max_rms=max_rms)
+    lim = np.float32(max_rms) * np.float32(255.0)
+    max_mse = lim * lim
+    ch = lambda c: np.array([(int(c) >> s) & 255 for s in (0, 8, 16)], np.int64)
+    tiles_x = (sc.width + tw - 1) // tw
+    over_tiles = 0
+    for ty in range(0, (sc.height + th - 1) // th):
+        for tx in range(tiles_x):
+            t = ty * tiles_x + tx
+            x0, y0 = tx * tw, ty * th
+            x1, y1 = min(x0 + tw, sc.width), min(y0 + th, sc.height)
+            se = cnt = 0
+            for j in range(64):
+                p = _probe(x0, y0, x1, y1, t, j)
+                if p is None or not pd[p[1], p[0]] < 1.0:
+                    continue
+                se += int(((ch(pc[p[1], p[0]]) - ch(nc[p[1], p[0]])) ** 2).sum())
+                cnt += 3
+            over = cnt > 0 and np.float32(se) / np.float32(cnt) > max_mse
+            over_tiles += int(over)
+            want = (pc if over else nc)[y0:y1, x0:x1]
+            assert np.array_equal(fc[y0:y1, x0:x1], want), (tx, ty, over, se, cnt)
+    print(f"limit {max_rms * 255:.2f}/255: {over_tiles} tiles re-shaded")
+    assert fst["fallback_tiles"] == over_tiles
+    gpu.set_learned_shading(None, 0.0)
+    gpu.close()
+
+
 def test_config4_1080p_shipped_weights():
     # the learned-shading workload at its full size with the shipped
     # network: coverage, depth and ids identical to the plain heavy frame,
