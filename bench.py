@@ -125,9 +125,9 @@ def cpu_baseline(scene, priming: int = 100, frames: int = 1000, seconds: float =
 
 MLP_FLOP_PER_FRAGMENT = 2 * (117 * 64 + 64 * 64 + 64 * 64 + 64 * 3)   # 31,744
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-# the learned frames' per-tile fallback limit: the RMS error (in [0,1]
-# units) of the 40 dB PSNR target, 10^(-40/20) = 0.01 = 2.55/255; tiles
-# whose probes show more are re-shaded with the plain shader
+# the learned frames' fallback limit: the RMS error (in [0,1] units) of the
+# 40 dB PSNR target, 10^(-40/20) = 0.01 = 2.55/255; tile quadrants whose
+# probes show more are re-shaded with the plain shader
 LEARNED_TILE_RMS = 0.01
 
 
@@ -208,7 +208,7 @@ def bench_learned(steps: int, warmup: int):
         "psnr_db_covered": round(learned.psnr(pc, fc, cov), 2),
         "psnr_db_frame": round(learned.psnr(pc, fc), 2),
         "psnr_db_network_only_covered": round(learned.psnr(pc, nc, cov), 2),
-        "fallback_tiles": st["fallback_tiles"], "tiles": st["tiles"], "fallback_tile_rms": LEARNED_TILE_RMS,
+        "fallback_quads": st["fallback_quads"], "tile_quads": 4 * st["tiles"], "fallback_rms": LEARNED_TILE_RMS,
         "coverage_depth_ids_identical": same_geometry,
         "resolve_ms": {"plain_heavy": round(plain_kt.get("k_resolve", 0.0), 5),
                        "gather": round(net_kt.get("k_resolve", 0.0), 5),
@@ -370,7 +370,7 @@ def bench_stress(steps: int):
         tfl = frags * MLP_FLOP_PER_FRAGMENT / (net_ms * 1e-3) / 1e12 if net_ms else 0.0
         out["learned"] = {"learned_ms": round(mlp_ms, 4), "speedup": round(plain_ms / mlp_ms, 3),
                           "psnr_db_covered_per_view": [round(x, 2) for x in q],
-                          "fallback_tiles": lst["fallback_tiles"], "tiles": lst["tiles"],
+                          "fallback_quads": lst["fallback_quads"], "tile_quads": 4 * lst["tiles"],
                           "mfma": {"kernel": "k_mlp_shade", "achieved_tflops": round(tfl, 2),
                                    "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "frac": round(tfl / MFMA_BF16_PEAK_TFLOPS, 5)},
                           "kernel_ms": {k: round(v, 5) for k, v in mlp_kt.items()}}

@@ -226,6 +226,7 @@ struct rgl_gpu {
 	DevArray<uint32_t> fwdImg;      // the network image of rgl_mlp_forward
 	DevArray<float> mlpQueue;
 	DevArray<uint32_t> probeWant;   // the probes' plain colours, kMlpProbes per tile
+	DevArray<uint32_t> quadList;    // tile quadrants the fallback re-shades
 	DevArray<uint32_t> qCount;      // network queue counters, kQueueSegs x kQueueStride
 	int mlpShadeBlocks = 256;       // persistent network workgroups: one per CU
 	bool mlpShadeBlocksSet = false;
@@ -354,7 +355,7 @@ struct rgl_gpu {
 		for (auto* a : { &qCount, &tileCount, &tileCursor, &tileOff, &clipq, &clipqOf, &clipHist, &clipOff, &clipCursor, &itemTile, &tileCost }) a->release();
 		tileDesc.release(); scanPart.release();
 		keys.release(); rec.release(); crec.release();
-		probeWant.release();
+		probeWant.release(); quadList.release();
 		for (auto* a : { &mlpQueue, &tX, &tT, &tH1, &tH2, &tH3, &tY, &tdY, &tdA, &tdB, &tPart, &tLoss }) a->release();
 		mlpImg.release(); fwdImg.release();
 		vpos.release(); vflags.release(); vvary.release(); trect.release(); tvert.release();
@@ -683,6 +684,7 @@ struct rgl_gpu {
 		if (mlpParams) {
 			if ((e = Grow(mlpImg, kImgFloats)) != hipSuccess) return hipfail(e, "hipMalloc network image");
 			if ((e = Grow(probeWant, static_cast<size_t>(b.nTiles) * kMlpProbes + 1)) != hipSuccess) return hipfail(e, "hipMalloc probes");
+			if ((e = Grow(quadList, static_cast<size_t>(b.nTiles) * 4 + 1)) != hipSuccess) return hipfail(e, "hipMalloc fallback list");
 			// each queue segment takes the resolve blocks with index % kQueueSegs == seg
 			mlpSegCap = static_cast<uint32_t>((static_cast<long long>(b.nBlocks) + kQueueSegs - 1) / kQueueSegs * 256);
 			if ((e = Grow(mlpQueue, static_cast<size_t>(mlpSegCap) * kQueueSegs * kQueueWords)) != hipSuccess) return hipfail(e, "hipMalloc network queue");
@@ -1013,7 +1015,7 @@ struct rgl_gpu {
 		mix(lastIo.data(), lastIo.size());
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
 		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, scanPart.p, tileCost.p,
-		                       keys.p, rec.p, crec.p, mlpImg.p, probeWant.p, mlpQueue.p, qCount.p, mlpParams,
+		                       keys.p, rec.p, crec.p, mlpImg.p, probeWant.p, quadList.p, mlpQueue.p, qCount.p, mlpParams,
 		                       dOps.p, dIo.p, order.p, skeys0.p, skeys1.p };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
@@ -1182,7 +1184,14 @@ struct rgl_gpu {
 			// plain re-shading of the tiles whose probes show an error over
 			// the limit
 			const float lim = mlpMaxRms * 255.0F;
-			k_mlp_fallback<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, probeWant.p, lim * lim, &dCtr->fallbackTiles); }
+			k_mlp_decide<<<(b.nTiles + 3) / 4, 256, 0, stream>>>(b, probeWant.p, lim * lim, quadList.p, &dCtr->fallbackQuads);
+			// 16 x 16 pixel workgroups per listed quadrant: a quadrant is at
+			// most 4 probe cells of ceil(tile / 8) pixels across
+			int tw = 8, th = 8;
+			for (int k = 0; k < b.nFrames; ++k) { tw = std::max(tw, hFrames[k].tileW); th = std::max(th, hFrames[k].tileH); }
+			const int subX = (4 * ((tw + 7) / 8) + 15) / 16, subY = (4 * ((th + 7) / 8) + 15) / 16;
+			k_mlp_fallback<ProgGGXEnv><<<4 * b.nTiles * subX * subY, 256, dlBytes, stream>>>(b, ra, quadList.p, &dCtr->fallbackQuads,
+				subX, subY); }
 		for (const SampleReq& q : reqs)
 			k_mlp_samples<ProgGGXEnv><<<rblocks, 256, dlBytes, stream>>>(b, ra, q.out, q.cap, q.count, q.keep, q.seed);
 		Mark(9);
@@ -1269,7 +1278,7 @@ struct rgl_gpu {
 		stats.clipped_tile_pairs = hCtr->clipPairs;
 		stats.fragments = static_cast<int64_t>(frags);
 		stats.overflow = hCtr->overflow;
-		stats.fallback_tiles = hCtr->fallbackTiles;
+		stats.fallback_quads = hCtr->fallbackQuads;
 		lastWorkItems = static_cast<int>(std::min<uint32_t>(hCtr->workItems, 1u << 20));
 		for (int k = 0; k < kRing; ++k) Accumulate(k);
 		if (hCtr->overflow & 8u) return fail(RGL_E_ARG, "vertex index outside the position buffer");

@@ -311,8 +311,9 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_forward(const float* __r
 
 // ---------------------------------------------------------------------------
 // learned resolve: like k_resolve, but the winner's shading inputs go
-// through the network; 64 x kMlpWaves pixels per workgroup (one row per
-// wave).  Depth and primitive id are the same stores as the plain resolve.
+// through the network (a gather pass, then the network pass over the
+// queued fragments).  Depth and primitive id are the same stores as the
+// plain resolve.
 // ---------------------------------------------------------------------------
 template <class P>
 __device__ inline bool resolve_inputs(const BatchDesc& b, const FrameDesc& f, const RasterArgs& a, const DrawLite* dl, int x, int y,
@@ -544,10 +545,12 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const
 				uint32_t* out = b.frames[pf[j]].outColor;
 				if (pok[j] && out) out[po[j]] = (srgb_unit_to8(y[j][0]) << 16) | (srgb_unit_to8(y[j][1]) << 8) | srgb_unit_to8(y[j][2]); }}}
 
-// the fallback decision of one tile, by one wave: the mean squared 8-bit
-// error per channel at its probes (the network's stored colour against
-// the plain one) over the limit
-__device__ inline bool tile_over_limit(const FrameDesc& f, int tile, const uint32_t* __restrict__ want, float maxMse) {
+// the fallback decision of one tile's four quadrants, by one wave: per
+// quadrant (the probes of its 4 x 4 of the tile's 8 x 8 probe cells), the
+// mean squared 8-bit error per channel at its covered probes (the
+// network's stored colour against the plain one) over the limit; bit q of
+// the result is quadrant q = (right half) + 2 (bottom half)
+__device__ inline uint32_t tile_over_limit(const FrameDesc& f, int tile, const uint32_t* __restrict__ want, float maxMse) {
 	const int j = threadIdx.x & 63;
 	int x, y;
 	float se = 0.0F, cnt = 0.0F;
@@ -558,45 +561,61 @@ __device__ inline bool tile_over_limit(const FrameDesc& f, int tile, const uint3
 			const float dv = static_cast<float>(static_cast<int>((w >> (8 * c)) & 255u) - static_cast<int>((got >> (8 * c)) & 255u));
 			se += dv * dv; }
 		cnt = 3.0F; }
-	for (int o = 32; o > 0; o >>= 1) { se += __shfl_xor(se, o, 64); cnt += __shfl_xor(cnt, o, 64); }
-	return cnt > 0.0F && se / cnt > maxMse; }
+	const int q = ((j & 7) >> 2) | (((j >> 3) >> 2) << 1);
+	uint32_t over = 0;
+#pragma unroll
+	for (int qq = 0; qq < 4; ++qq) {
+		float s = q == qq ? se : 0.0F, c = q == qq ? cnt : 0.0F;
+		for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o, 64); c += __shfl_xor(c, o, 64); }
+		if (c > 0.0F && s / c > maxMse) over |= 1u << qq; }
+	return over; }
 
-// pixels of tiles whose estimated error exceeds the limit are re-shaded
-// with the plain shader (colour only; depth and ids are already stored);
-// one thread per pixel.  Wave 0 decides the block's tiles; blocks of tiles
-// under the limit exit at once.
+// the fallback decision: one wave per tile appends the tile's quadrants
+// over the limit (tile * 4 + quadrant) to `list`, counting them in `count`
+// (zeroed at the frame's start)
+__global__ __launch_bounds__(256) void k_mlp_decide(BatchDesc b, const uint32_t* __restrict__ want, float maxMse,
+                                                    uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+	__shared__ int ftb[kMaxFrames + 1];
+	load_tile_bases(ftb, b);
+	__syncthreads();
+	const int tile = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+	if (tile >= b.nTiles) return;   // uniform over the wave
+	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+	if (f.outColor == nullptr) return;
+	const uint32_t over = tile_over_limit(f, tile, want, maxMse);
+	if ((threadIdx.x & 63) == 0 && over) {
+		uint32_t at = atomicAdd(count, static_cast<uint32_t>(__popc(over)));
+		for (int q = 0; q < 4; ++q)
+			if ((over >> q) & 1u) list[at++] = static_cast<uint32_t>(tile) * 4u + static_cast<uint32_t>(q); }}
+
+// the listed tile quadrants re-shaded with the plain shader (colour only;
+// depth and ids are already stored): subX x subY workgroups of 16 x 16
+// pixels per list entry, one thread per pixel; workgroups past the list's
+// end exit at once
 template <class P>
-__global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a, const uint32_t* __restrict__ want,
-                                                      float maxMse, uint32_t* __restrict__ nFallback) {
+__global__ __launch_bounds__(256) void k_mlp_fallback(BatchDesc b, RasterArgs a, const uint32_t* __restrict__ list,
+                                                      const uint32_t* __restrict__ count, int subX, int subY) {
 	extern __shared__ uint4 dyn_lds[];   // the batch's draws
 	DrawLite* dl = reinterpret_cast<DrawLite*>(dyn_lds);
-	__shared__ int fbb[kMaxFrames + 1];
-	constexpr int kMaxBlockTiles = (kResolveW / 8 + 1) * (kResolveH / 8 + 1);   // tiles are at least 8 px
-	__shared__ uint32_t over[kMaxBlockTiles];
-	load_block_bases(fbb, b);
-	__syncthreads();
-	const BlockPixel bp = block_pixel(b, fbb);
-	const FrameDesc& f = b.frames[bp.frame];
-	if (f.outColor == nullptr) return;   // uniform over the block
-	const int x = bp.x, y = bp.y;
-	// the tiles the block's pixels lie in: rows ty0..ty1, columns tx0..tx1
-	const int lb = static_cast<int>(blockIdx.x) - f.blockBase, bx = lb % f.blocksX, by = lb / f.blocksX;
-	const int ty0 = (by * kResolveH) / f.tileH, ty1 = min(by * kResolveH + kResolveH - 1, f.H - 1) / f.tileH;
-	const int tx0 = (bx * kResolveW) / f.tileW, tx1 = min(bx * kResolveW + kResolveW - 1, f.W - 1) / f.tileW;
-	const int ntx = tx1 - tx0 + 1, nbt = (ty1 - ty0 + 1) * ntx;
-	if (threadIdx.x < 64)
-		for (int k = 0; k < nbt; ++k) {
-			const bool v = tile_over_limit(f, f.tileBase + (ty0 + k / ntx) * f.tilesX + tx0 + k % ntx, want, maxMse);
-			if (threadIdx.x == 0) over[k] = v ? 1u : 0u; }
-	__syncthreads();
-	bool any = false;
-	for (int k = 0; k < nbt; ++k) any = any || over[k] != 0u;
-	if (!any) return;   // uniform over the block
+	__shared__ int ftb[kMaxFrames + 1];
+	const int per = subX * subY;
+	const uint32_t item = blockIdx.x / static_cast<uint32_t>(per);
+	if (item >= *count) return;   // uniform over the block
 	load_lite(dl, a.draws, b.nDraws);
+	load_tile_bases(ftb, b);
 	__syncthreads();
-	if (x >= f.W || y >= f.H) return;
-	if (!over[(y / f.tileH - ty0) * ntx + x / f.tileW - tx0]) return;
-	if (x % f.tileW == 0 && y % f.tileH == 0) atomicAdd(nFallback, 1u);
+	const uint32_t id = list[item];
+	const int tile = static_cast<int>(id >> 2), q = static_cast<int>(id & 3u);
+	const FrameDesc& f = b.frames[find_base(ftb, b.nFrames, tile)];
+	// the quadrant: the tile split where its probe cells split
+	const Rect R = tile_rect(f, tile - f.tileBase);
+	const int cw = (R.x1 - R.x0 + 7) / 8, ch = (R.y1 - R.y0 + 7) / 8;
+	const int qx0 = (q & 1) ? R.x0 + 4 * cw : R.x0, qx1 = (q & 1) ? R.x1 : min(R.x0 + 4 * cw, R.x1);
+	const int qy0 = (q >> 1) ? R.y0 + 4 * ch : R.y0, qy1 = (q >> 1) ? R.y1 : min(R.y0 + 4 * ch, R.y1);
+	const int sub = static_cast<int>(blockIdx.x % static_cast<uint32_t>(per));
+	const int x = qx0 + (sub % subX) * 16 + static_cast<int>(threadIdx.x & 15);
+	const int y = qy0 + (sub / subX) * 16 + static_cast<int>(threadIdx.x >> 4);
+	if (x >= qx1 || y >= qy1) return;
 	float col[3], dep;
 	uint32_t prim;
 	if (resolve_pixel<ShadeOne<P>>(b, f, a, dl, x, y, kRcpMantissa, col, dep, prim))

@@ -26,7 +26,7 @@ class RglError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "tiles", "commands", "primitives_submitted", "primitives_clipped", "primitives_binned",
-        "tile_pairs", "clipped_tile_pairs", "fragments", "overflow", "fallback_tiles")]
+        "tile_pairs", "clipped_tile_pairs", "fragments", "overflow", "fallback_quads")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

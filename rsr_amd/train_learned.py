@@ -72,7 +72,7 @@ def main():
     fc, _, _, st = rgl.render(sc, gpu=gpu)
     cov = pd < 1.0
     print(f"evaluation PSNR (covered pixels): network {learned.psnr(pc, lc, cov):.2f} dB, "
-          f"with 2/255 fallback {learned.psnr(pc, fc, cov):.2f} dB ({st['fallback_tiles']}/{st['tiles']} tiles); "
+          f"with 2/255 fallback {learned.psnr(pc, fc, cov):.2f} dB ({st['fallback_quads']}/{4 * st['tiles']} tile quadrants); "
           f"saved {args.out}", flush=True)
     gpu.set_learned_shading(None, 0.0)
     gpu.close()

@@ -110,26 +110,41 @@ def test_learned_frame_and_training_gain():
     cov = pd < 1.0
     lc, ld, lp, lst = _frame(gpu, sc, params, max_rms=1.0)     # network everywhere (no fallback)
     assert np.array_equal(pd.view(np.uint32), ld.view(np.uint32)) and np.array_equal(pp, lp)
-    assert lst["fallback_tiles"] == 0
+    assert lst["fallback_quads"] == 0
     uc, _, _, _ = _frame(gpu, sc, p0, max_rms=1.0)             # the untrained network
-    fc, fd, fp, fst = _frame(gpu, sc, params)                  # with the 2/255 per-tile fallback
+    fc, fd, fp, fst = _frame(gpu, sc, params)                  # with the 2/255 fallback
     assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
     q_trained, q_untrained, q_fallback = learned.psnr(pc, lc, cov), learned.psnr(pc, uc, cov), learned.psnr(pc, fc, cov)
     print(f"PSNR over covered pixels: untrained {q_untrained:.2f} dB, trained {q_trained:.2f} dB, "
-          f"trained + fallback {q_fallback:.2f} dB ({fst['fallback_tiles']} of {fst['tiles']} tiles re-shaded)")
+          f"trained + fallback {q_fallback:.2f} dB ({fst['fallback_quads']} of {4 * fst['tiles']} tile quadrants re-shaded)")
     assert q_trained > q_untrained + 10.0
     assert q_fallback >= q_trained
     # background pixels are the clear colour in every variant
     assert np.array_equal(pc[~cov], lc[~cov]) and np.array_equal(pc[~cov], fc[~cov])
-    # a zero error budget re-shades every tile whose probes saw any error
+    # a zero error budget re-shades every tile quadrant whose probes saw any
+    # error
     zc, _, _, zst = _frame(gpu, sc, params, max_rms=0.0)
-    th, tw = 64, 64
-    same = sum(np.array_equal(zc[y:y + th, x:x + tw], pc[y:y + th, x:x + tw])
-               for y in range(0, h, th) for x in range(0, w, tw))
-    print(f"zero budget: {zst['fallback_tiles']} tiles re-shaded, {same} tiles equal to the plain frame")
-    assert zst["fallback_tiles"] > 0 and same >= zst["fallback_tiles"]
+    same = sum(np.array_equal(zc[y0:y1, x0:x1], pc[y0:y1, x0:x1])
+               for _, _, (x0, y0, x1, y1) in _quadrants(w, h, sc.tile_blocks[0] * 8, sc.tile_blocks[1] * 8))
+    print(f"zero budget: {zst['fallback_quads']} tile quadrants re-shaded, {same} equal to the plain frame")
+    assert zst["fallback_quads"] > 0 and same >= zst["fallback_quads"]
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
+
+
+def _quadrants(w, h, tw, th):
+    # (tile, quadrant, pixel rect) of every tile quadrant of a w x h target:
+    # a tile splits where its 8 x 8 probe cells do
+    tiles_x = (w + tw - 1) // tw
+    for ty in range((h + th - 1) // th):
+        for tx in range(tiles_x):
+            x0, y0 = tx * tw, ty * th
+            x1, y1 = min(x0 + tw, w), min(y0 + th, h)
+            cw, ch = (x1 - x0 + 7) // 8, (y1 - y0 + 7) // 8
+            for q in range(4):
+                qx0, qx1 = (x0 + 4 * cw, x1) if q & 1 else (x0, min(x0 + 4 * cw, x1))
+                qy0, qy1 = (y0 + 4 * ch, y1) if q >> 1 else (y0, min(y0 + 4 * ch, y1))
+                yield ty * tiles_x + tx, q, (qx0, qy0, qx1, qy1)
 
 
 def _hash32(v):
@@ -150,10 +165,11 @@ def _probe(x0, y0, x1, y1, tile, j):
 
 
 @pytest.mark.parametrize("max_rms", [2.0 / 255.0, 0.01, 6.0 / 255.0])
-def test_fallback_tiles_follow_the_probes(max_rms):
-    # every tile whose covered probes show a mean squared 8-bit error over
-    # the limit (the network's colours against the plain shader's) is the
-    # plain frame's tile, every other tile the network's, bit for bit
+def test_fallback_quadrants_follow_the_probes(max_rms):
+    # every tile quadrant whose covered probes (the 16 of its 4 x 4 probe
+    # cells) show a mean squared 8-bit error over the limit (the network's
+    # colours against the plain shader's) is the plain frame's quadrant,
+    # every other quadrant the network's, bit for bit
     import os
     wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
     params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
@@ -167,25 +183,30 @@ def test_fallback_tiles_follow_the_probes(max_rms):
     max_mse = lim * lim
     ch = lambda c: np.array([(int(c) >> s) & 255 for s in (0, 8, 16)], np.int64)
     tiles_x = (sc.width + tw - 1) // tw
-    over_tiles = 0
+    over_quads = 0
     for ty in range(0, (sc.height + th - 1) // th):
         for tx in range(tiles_x):
             t = ty * tiles_x + tx
             x0, y0 = tx * tw, ty * th
             x1, y1 = min(x0 + tw, sc.width), min(y0 + th, sc.height)
-            se = cnt = 0
+            cw, chh = (x1 - x0 + 7) // 8, (y1 - y0 + 7) // 8
+            se, cnt = [0] * 4, [0] * 4
             for j in range(64):
                 p = _probe(x0, y0, x1, y1, t, j)
                 if p is None or not pd[p[1], p[0]] < 1.0:
                     continue
-                se += int(((ch(pc[p[1], p[0]]) - ch(nc[p[1], p[0]])) ** 2).sum())
-                cnt += 3
-            over = cnt > 0 and np.float32(se) / np.float32(cnt) > max_mse
-            over_tiles += int(over)
-            want = (pc if over else nc)[y0:y1, x0:x1]
-            assert np.array_equal(fc[y0:y1, x0:x1], want), (tx, ty, over, se, cnt)
-    print(f"limit {max_rms * 255:.2f}/255: {over_tiles} tiles re-shaded")
-    assert fst["fallback_tiles"] == over_tiles
+                q = int((j & 7) >= 4) + 2 * int((j >> 3) >= 4)
+                se[q] += int(((ch(pc[p[1], p[0]]) - ch(nc[p[1], p[0]])) ** 2).sum())
+                cnt[q] += 3
+            for q in range(4):
+                qx0, qx1 = (x0 + 4 * cw, x1) if q & 1 else (x0, min(x0 + 4 * cw, x1))
+                qy0, qy1 = (y0 + 4 * chh, y1) if q >> 1 else (y0, min(y0 + 4 * chh, y1))
+                over = cnt[q] > 0 and np.float32(se[q]) / np.float32(cnt[q]) > max_mse
+                over_quads += int(over)
+                want = (pc if over else nc)[qy0:qy1, qx0:qx1]
+                assert np.array_equal(fc[qy0:qy1, qx0:qx1], want), (tx, ty, q, over, se[q], cnt[q])
+    print(f"limit {max_rms * 255:.2f}/255: {over_quads} tile quadrants re-shaded")
+    assert fst["fallback_quads"] == over_quads
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
 
@@ -206,7 +227,7 @@ def test_config4_1080p_shipped_weights():
     assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
     assert np.array_equal(pd.view(np.uint32), nd.view(np.uint32)) and np.array_equal(pp, npid)
     q, qn = learned.psnr(pc, fc, cov), learned.psnr(pc, nc, cov)
-    print(f"1080p heavy: PSNR covered {q:.2f} dB with fallback ({fst['fallback_tiles']} of {fst['tiles']} tiles), "
+    print(f"1080p heavy: PSNR covered {q:.2f} dB with fallback ({fst['fallback_quads']} of {4 * fst['tiles']} tile quadrants), "
           f"network alone {qn:.2f} dB")
     assert q >= 40.0
     gpu.set_learned_shading(None, 0.0)
@@ -229,7 +250,7 @@ def test_config5_view_learned_shading():
     assert cov.sum() > 100000
     assert np.array_equal(pd.view(np.uint32), fd.view(np.uint32)) and np.array_equal(pp, fp)
     q = learned.psnr(pc, fc, cov)
-    print(f"4K stress view: PSNR covered {q:.2f} dB ({fst['fallback_tiles']} of {fst['tiles']} tiles re-shaded)")
+    print(f"4K stress view: PSNR covered {q:.2f} dB ({fst['fallback_quads']} of {4 * fst['tiles']} tile quadrants re-shaded)")
     assert q >= 40.0
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
@@ -250,7 +271,7 @@ def test_learned_shading_keeps_linear_and_exposure_stores_plain():
     plain = _frame(gpu, sc)
     assert (plain[1] < 1.0).sum() > 1000 and len(np.unique(plain[0])) > 100
     lrn = _frame(gpu, sc, params, max_rms=1.0)
-    assert lrn[3]["fallback_tiles"] == 0
+    assert lrn[3]["fallback_quads"] == 0
     assert np.array_equal(plain[0], lrn[0])
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
@@ -295,8 +316,8 @@ def test_learned_frame_graph_replay_matches_eager():
     sc = scenes.scene_heavy(640, 360)
     gpu = rgl.GPU(0)
     ec, ed, ep, est = _frame(gpu, sc, params)
-    print(f"eager: {est['fallback_tiles']} of {est['tiles']} tiles re-shaded", flush=True)
-    assert 0 < est["fallback_tiles"] < est["tiles"]
+    print(f"eager: {est['fallback_quads']} of {4 * est['tiles']} tile quadrants re-shaded", flush=True)
+    assert 0 < est["fallback_quads"] < 4 * est["tiles"]
     other = rgl.GPU(0)
     oc, od, op, _ = rgl.render(scenes.scene_sphere_ground(), gpu=other)
     gpu.set_graph(True)
@@ -309,7 +330,7 @@ def test_learned_frame_graph_replay_matches_eager():
         gpu.run_and_wait(stream)
         c, d, p = ds.outputs()
         assert np.array_equal(c, ec) and np.array_equal(d.view(np.uint32), ed.view(np.uint32)) and np.array_equal(p, ep), k
-        assert gpu.stats()["fallback_tiles"] == est["fallback_tiles"]
+        assert gpu.stats()["fallback_quads"] == est["fallback_quads"]
         c2, _, _, _ = rgl.render(scenes.scene_sphere_ground(), gpu=other)
         assert np.array_equal(c2, oc)
     assert gpu.graph_captures() == 1
