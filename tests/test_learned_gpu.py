@@ -164,8 +164,8 @@ def _probe(x0, y0, x1, y1, tile, j):
     return (x, y) if x < x1 and y < y1 else None
 
 
-@pytest.mark.parametrize("max_rms", [2.0 / 255.0, 0.01, 6.0 / 255.0])
-def test_fallback_quadrants_follow_the_probes(max_rms):
+@pytest.mark.parametrize("max_rms,tile", [(2.0 / 255.0, 8), (0.01, 8), (6.0 / 255.0, 8), (0.01, 4), (2.0 / 255.0, 4)])
+def test_fallback_quadrants_follow_the_probes(max_rms, tile):
     # every tile quadrant whose covered probes (the 16 of its 4 x 4 probe
     # cells) show a mean squared 8-bit error over the limit (the network's
     # colours against the plain shader's) is the plain frame's quadrant,
@@ -173,8 +173,11 @@ def test_fallback_quadrants_follow_the_probes(max_rms):
     import os
     wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
     params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
+    import dataclasses
     gpu = rgl.GPU(0)
-    sc = scenes.scene_heavy(640, 360)
+    # 64 and 32 px tiles (360 rows leave 40 and 8 px tiles at the bottom,
+    # whose quadrants differ in size)
+    sc = dataclasses.replace(scenes.scene_heavy(640, 360), tile_blocks=(tile, tile))
     tw, th = sc.tile_blocks[0] * 8, sc.tile_blocks[1] * 8
     pc, pd, _, _ = _frame(gpu, sc)
     nc, _, _, _ = _frame(gpu, sc, params, max_rms=1.0)
