@@ -250,6 +250,31 @@ def bench_batch(steps: int):
     return out
 
 
+def bench_inorder(steps: int, key_ms: float):
+    """Config 2 with alpha blending on: the same 327,680 triangles drawn by
+    the in-order tile path in every tile (submission order per tile, late
+    depth test, blend into the tile's colour), frames queued back to back,
+    beside the key path's frame.  The program writes alpha 1, so the frame
+    equals the key path's."""
+    import dataclasses
+    import torch
+    from rsr_amd import rgl, scenes
+    dev = "cuda:0"
+    base = scenes.scene_icospheres()
+    sc = dataclasses.replace(base, blend=True, name=base.name + "_blend")
+    gpu = rgl.GPU(0)
+    ds = rgl.DeviceScene(sc, dev, prim=True)
+    ds.record(gpu)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    ms = untimed_frames(gpu, stream, steps, 10)
+    st = gpu.stats()
+    gpu.close()
+    return {"workload": sc.name, "path": "in-order tiles (GL_BLEND, BlendAlpha)", "frame_ms": round(ms, 4),
+            "mtris_per_s": round(sc.triangles / (ms * 1e-3) / 1e6, 3),
+            "gfrag_per_s": round(st["fragments"] / (ms * 1e-3) / 1e9, 4), "fragments": st["fragments"],
+            "key_path_frame_ms": round(key_ms, 4), "ratio_to_key_path": round(ms / key_ms, 3)}
+
+
 def bench_small(steps: int, with_cpu: bool):
     """Config 1, the reference's CPU-runnable case: the 640x360 UV sphere
     over a ground quad (962 triangles) with the Lambert texture program;
@@ -396,6 +421,7 @@ def main():
     ap.add_argument("--graph", action="store_true", help="time graph replay instead of eager launches")
     ap.add_argument("--no-double-buffer", action="store_true", help="skip the double-buffered timing")
     ap.add_argument("--no-small", action="store_true", help="skip the 640x360 sphere-and-ground measurement")
+    ap.add_argument("--no-inorder", action="store_true", help="skip the blended (in-order path) config-2 measurement")
     ap.add_argument("--no-tune", action="store_true", help="default tile size and pairs per work item, no autotuner")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run one 64 MiB device-to-device copy (a known-size kernel that "
@@ -603,6 +629,8 @@ def main():
             "viewer_loop_frame_ms": round(vl_ms, 4),
             "mode": "rgl_gpu_set_double_buffer: two executors, frame N binning beside frame N-1 raster, "
                     "stores in submission order"}
+    if not args.no_inorder:
+        out["inorder"] = bench_inorder(max(20, args.steps // 4), ms)
     if not args.no_learned:
         out["learned"] = bench_learned(max(20, args.steps // 4), max(5, args.warmup // 4))
     if not args.no_batch:

@@ -1068,6 +1068,13 @@ struct rgl_gpu {
 			const unsigned waves = std::max(1u, std::min<unsigned>(kClipWaves, g.clipCap));
 			k_clip<P><<<waves, 64, 0, g.stream>>>(g.dDraws.p, b.nDraws, g.clipq.p, g.clipCap, g.cvpos.p, g.cvvary.p, g.fcount.p,
 				g.frect.p, g.clipHist.p, g.dCtr, varyStride); }};
+	// k_inorder compiled for the batch's one program (or for all of them)
+	struct InorderLaunch {
+		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; size_t lds;
+		template <class P> void run() { launch<ProgramList<P>>(); }
+		template <class L> void launch() {
+			k_inorder<L><<<g.nIoBlocks, 256, lds, g.stream>>>(b, ra, g.dIo.p, g.nIo, b.frames, g.dOps.p, g.dDraws.p, g.dLite.p, g.tileDesc.p,
+				g.rec.p, g.crec.p, g.order.p); }};
 	struct ResolveLaunch {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;
 		template <class P> void run() {
@@ -1155,9 +1162,9 @@ struct rgl_gpu {
 		// follows writes or samples the caller's, after the previous
 		// double-buffered frame has finished
 		if (rasterAfter && (e = hipStreamWaitEvent(stream, rasterAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
-		if (nIo > 0)
-			k_inorder<<<nIoBlocks, 256, 0, stream>>>(b, ra, dIo.p, nIo, b.frames, dOps.p, dDraws.p, dLite.p, tileDesc.p, rec.p, crec.p,
-				order.p);
+		if (nIo > 0) {
+			InorderLaunch il{ *this, b, ra, sizeof(float) * 256 * 3 * static_cast<size_t>(varyStride) };
+			if (pid < 0 || !dispatch_program(pid, il, AllPrograms{})) il.launch<AllPrograms>(); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));
 		const DrawLite* cLite = dLite.p;

@@ -181,6 +181,10 @@ struct ShadeQuad {
 __device__ inline bool depth_pass(int func, float frag, float dest) {
 	return func == 0 ? frag < dest : func == 1 ? frag <= dest : frag == dest; }
 
+// L: the programs the draws may use (a frame of one program compiles only
+// that program's shading, which keeps the registers, and so the resident
+// waves that hide each triangle's texture fetches, of that program alone)
+template <class L>
 __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, const IoFrame* __restrict__ io, int nIo,
 		const FrameDesc* __restrict__ frames, const FrameOp* __restrict__ ops, const DrawDesc* __restrict__ draws,
 		const DrawLite* __restrict__ lite, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
@@ -188,6 +192,15 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	__shared__ VisRec srec[256];
 	__shared__ int sdraw[256];
 	__shared__ uint32_t sfrag[4];
+	// each batch's triangles: 1/w of the three vertices, the source
+	// triangle (bit 31: a clipped fan), and the vertices' varyings at the
+	// batch's stride (dynamic LDS: 256 x 3 x varyStride floats), loaded by
+	// one thread per triangle before the workgroup steps through them
+	__shared__ float sw[256][3];
+	__shared__ uint32_t ssrc[256];
+	__shared__ uint32_t scan_sh[256 / 64 + 1];
+	extern __shared__ float svary[];
+	const int vs = a.varyStride;
 	const int blk = static_cast<int>(blockIdx.x);
 	const IoFrame e = io[find_io(io, nIo, blk, false)];
 	const FrameDesc& f = frames[e.frame];
@@ -268,15 +281,50 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 						make_float4(cr[l], cg[l], cb[l], 0.0F);
 				else static_cast<uint32_t*>(op.ptr)[o] = pid[l]; }}};
 
+	// the region's pixel rect: records whose quads miss it are dropped when
+	// a batch is loaded (the rest keep their order), so the workgroup steps
+	// only through the triangles that can touch its pixels
+	const int rx0 = R.x0 + rgx * kRegionDim, ry0 = R.y0 + rgy * kRegionDim;
+	const int rx1 = min(rx0 + kRegionDim, R.x1), ry1 = min(ry0 + kRegionDim, R.y1);
 	for (uint32_t base = 0; base < n; base += 256) {
 		__syncthreads();
+		VisRec rr;
+		bool hit = false;
 		if (base + static_cast<uint32_t>(t) < n) {
 			const uint32_t p = order[ob + base + t];
-			const VisRec r = p < ninl ? rec[vt.ib + p] : crec[vt.cb + (p - ninl)];
-			srec[t] = r;
-			sdraw[t] = find_draw_slot(lite, b.nDraws, r.slot); }
+			rr = p < ninl ? rec[vt.ib + p] : crec[vt.cb + (p - ninl)];
+			const int minx = static_cast<int>(rr.minxy & 0xffffu), miny = static_cast<int>(rr.minxy >> 16);
+			const int nqx = static_cast<int>(rr.nq & 0xffffu), nqy = static_cast<int>(rr.nq >> 16);
+			hit = minx < rx1 && minx + 2 * nqx > rx0 && miny < ry1 && miny + 2 * nqy > ry0; }
+		uint32_t cnt;
+		const uint32_t at = block_exclusive_scan(hit ? 1u : 0u, scan_sh, cnt);
+		if (hit) {
+			const uint32_t t2 = at;   // the record's place among the batch's hits
+			srec[t2] = rr;
+			const int di = find_draw_slot(lite, b.nDraws, rr.slot);
+			sdraw[t2] = di;
+			const DrawLite& dl = lite[di];
+			const uint32_t rel = rr.slot - dl.slotBase;
+			const uint32_t nTri = static_cast<uint32_t>(dl.tris) * static_cast<uint32_t>(dl.instances);
+			TriVerts tr;
+			uint32_t srcTri;
+			const float* vary;
+			if (rel < nTri) {
+				srcTri = rel;
+				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr, vs);
+				vary = a.vvary; }
+			else {
+				const uint32_t fslot = rel - nTri;
+				srcTri = fslot / kFanMax;
+				const uint32_t jf = fslot - srcTri * kFanMax;
+				clipped_verts(a.clipqOf[dl.triBase + srcTri] * kFanMax + jf, a.cvpos, tr, vs);
+				vary = a.cvvary; }
+			ssrc[t2] = srcTri | (tr.clipped ? 0x80000000u : 0u);
+			float* sv = svary + static_cast<long long>(t2) * 3 * vs;
+			for (int k = 0; k < 3; ++k) {
+				sw[t2][k] = tr.w[k];
+				for (int c = 0; c < vs; ++c) sv[k * vs + c] = vary[tr.vary[k] + c]; }}
 		__syncthreads();
-		const uint32_t cnt = min(256u, n - base);
 		for (uint32_t j = 0; j < cnt; ++j) {
 			const uint32_t slot = srec[j].slot;
 			while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
@@ -328,27 +376,14 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 			if (dd.depthWrite && earlyZ) {
 #pragma unroll
 				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
-			// the winning triangle's vertices: 1/w and varyings
-			const uint32_t rel = slot - dl.slotBase;
-			const uint32_t nTri = static_cast<uint32_t>(dl.tris) * static_cast<uint32_t>(dl.instances);
-			TriVerts tr;
-			uint32_t srcTri;
-			const float* vary;
-			if (rel < nTri) {
-				srcTri = rel;
-				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr, a.varyStride);
-				vary = a.vvary; }
-			else {
-				const uint32_t fslot = rel - nTri;
-				srcTri = fslot / kFanMax;
-				const uint32_t jf = fslot - srcTri * kFanMax;
-				const uint32_t cq = a.clipqOf[dl.triBase + srcTri];
-				clipped_verts(cq * kFanMax + jf, a.cvpos, tr, a.varyStride);
-				vary = a.cvvary; }
+			// the triangle's vertices (1/w, varyings) from LDS
+			const uint32_t src = ssrc[j], srcTri = src & 0x7fffffffu;
+			const float w3[3] = { sw[j][0], sw[j][1], sw[j][2] };
+			const float* sv = svary + static_cast<long long>(j) * 3 * vs;
 			float col[4][4];
-			ShadeQuad sq{ dd, tr.w, vary + tr.vary[0], vary + tr.vary[1], vary + tr.vary[2],
-			              bx, by, bz, fd, qx, qy, f.H, tr.clipped, col, 0xfu, true };
-			dispatch_program(dd.program, sq, AllPrograms{});
+			ShadeQuad sq{ dd, w3, sv, sv + vs, sv + 2 * vs,
+			              bx, by, bz, fd, qx, qy, f.H, (src >> 31) != 0u, col, 0xfu, true };
+			dispatch_program(dd.program, sq, L{});
 			mask &= sq.keep;   // gl_triMask after ShadeFragment
 			if (dd.depthTest && !earlyZ) {
 #pragma unroll

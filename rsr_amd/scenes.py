@@ -293,6 +293,7 @@ class Scene:
     depth_func: int = GL_LESS
     post_program: int = PGM_DEFAULT_POST
     fixed_guard_band: bool = False   # guard band factor raised to >= 1 (large targets)
+    blend: bool = False              # GL_BLEND (BlendAlpha) for every draw: the in-order tile path
 
     @property
     def triangles(self) -> int:
@@ -594,9 +595,10 @@ def scene_stress(view: int = 0, width: int = 3840, height: int = 2160, instances
 
 def fragment_state_key(scene: Scene) -> int:
     """GLState::FragmentStateKey of the scene's draw state without the
-    program id (rglv_gl.hxx:136-146): depth test on, depth and colour
-    writes on, RB_RGBF32 colour, RB_F32 depth."""
-    return (1 << 1) | (scene.depth_func << 2) | (1 << 5) | (1 << 6) | (RB_RGBF32 << 7) | (RB_F32 << 9)
+    program id (rglv_gl.hxx:136-146): depth test on, blending as the scene
+    sets it, depth and colour writes on, RB_RGBF32 colour, RB_F32 depth."""
+    return ((1 << 1) | (scene.depth_func << 2) | (int(scene.blend) << 4) | (1 << 5) | (1 << 6) | (RB_RGBF32 << 7)
+            | (RB_F32 << 9))
 
 
 def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers=None) -> None:
@@ -624,6 +626,8 @@ def render_scene(gpu, scene: Scene, color_out, depth_out, prim_out=None, buffers
         gpu.gl_enable(GL_CULL_FACE)
         gpu.gl_cull_face(scene.cull_face)
     gpu.gl_depth_func(scene.depth_func)
+    if scene.blend:
+        gpu.gl_enable(GL_BLEND)
     if scene.texture is not None:
         gpu.gl_bind_texture(0, ptr(scene.texture), scene.texture_dim, scene.texture_dim,
                             scene.texture_dim, scene.texture_filter)
