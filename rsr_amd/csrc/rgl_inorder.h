@@ -199,6 +199,13 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	__shared__ float sw[256][3];
 	__shared__ uint32_t ssrc[256];
 	__shared__ uint32_t scan_sh[256 / 64 + 1];
+	// a window of 256 (triangle, quad) pairs, shaded in parallel: the
+	// lanes' depths and colours, the live and kept lanes; qoff[j]: the first
+	// pair of batch triangle j
+	__shared__ float wfd[256][4];
+	__shared__ float4 wcol[256][4];
+	__shared__ uint32_t wmask[256];
+	__shared__ uint32_t qoff[257];
 	extern __shared__ float svary[];
 	const int vs = a.varyStride;
 	const int blk = static_cast<int>(blockIdx.x);
@@ -325,86 +332,145 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 				sw[t2][k] = tr.w[k];
 				for (int c = 0; c < vs; ++c) sv[k * vs + c] = vary[tr.vary[k] + c]; }}
 		__syncthreads();
-		for (uint32_t j = 0; j < cnt; ++j) {
-			const uint32_t slot = srec[j].slot;
-			while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
+		// the batch's (triangle, quad) pairs inside the region, triangle by
+		// triangle in order: triangle j's quads of its quad rect that lie in
+		// the region's 16 x 16 quads (both grids step 2 from even origins)
+		auto pair_rect = [&](int j, int& x0, int& y0, int& cx, int& cy) {
 			const int minx = static_cast<int>(srec[j].minxy & 0xffffu), miny = static_cast<int>(srec[j].minxy >> 16);
 			const int nqx = static_cast<int>(srec[j].nq & 0xffffu), nqy = static_cast<int>(srec[j].nq >> 16);
-			if (qx < minx || qy < miny) continue;
-			const int kx = (qx - minx) >> 1, ky = (qy - miny) >> 1;
-			if (kx >= nqx || ky >= nqy) continue;
-			const VisRec& r = srec[j];
-			// the quad's lanes (c, c+dy, c+dx, c+dx+dy), stepped as the
-			// rasteriser loops step (rglv_triangle.hxx:253-304)
-			const int32_t q1 = wadd(wadd(r.c1, wmul(kx, wmul(r.dy12, 2))), wmul(ky, wmul(r.dx12, 2)));
-			const int32_t q2 = wadd(wadd(r.c2, wmul(kx, wmul(r.dy23, 2))), wmul(ky, wmul(r.dx23, 2)));
-			const int32_t q3 = wadd(wadd(r.c3, wmul(kx, wmul(r.dy31, 2))), wmul(ky, wmul(r.dx31, 2)));
-			const int32_t e1[4] = { q1, wadd(q1, r.dy12), wadd(q1, r.dx12), wadd(wadd(q1, r.dx12), r.dy12) };
-			const int32_t e2[4] = { q2, wadd(q2, r.dy23), wadd(q2, r.dx23), wadd(wadd(q2, r.dx23), r.dy23) };
-			const int32_t e3[4] = { q3, wadd(q3, r.dy31), wadd(q3, r.dx31), wadd(wadd(q3, r.dx31), r.dy31) };
-			uint32_t live = 0;
+			x0 = max(minx, rx0); y0 = max(miny, ry0);
+			const int x1 = min(minx + 2 * nqx, rx0 + kRegionDim), y1 = min(miny + 2 * nqy, ry0 + kRegionDim);
+			cx = x1 > x0 ? (x1 - x0 + 1) >> 1 : 0;
+			cy = y1 > y0 ? (y1 - y0 + 1) >> 1 : 0; };
+		uint32_t np = 0;
+		if (static_cast<uint32_t>(t) < cnt) { int x0, y0, cx, cy; pair_rect(t, x0, y0, cx, cy); np = static_cast<uint32_t>(cx * cy); }
+		uint32_t total;
+		const uint32_t poff = block_exclusive_scan(np, scan_sh, total);
+		if (static_cast<uint32_t>(t) < cnt) qoff[t] = poff;
+		if (t == 0) qoff[cnt] = total;
+		__syncthreads();
+		for (uint32_t w0 = 0; w0 < total; w0 += 256) {
+			// shade: thread t takes pair w0 + t (coverage, scissor, the
+			// program), in parallel over the window
+			{
+				const uint32_t p = w0 + static_cast<uint32_t>(t);
+				uint32_t pm = 0;   // bits 0-3: live lanes the program keeps, bits 4-7: live lanes
+				if (p < total) {
+					int lo = 0, hi = static_cast<int>(cnt) - 1;   // the last triangle whose pairs start at or before p
+					while (lo < hi) { const int mid = (lo + hi + 1) >> 1; if (qoff[mid] <= p) lo = mid; else hi = mid - 1; }
+					const int j = lo;
+					int x0, y0, cx, cy;
+					pair_rect(j, x0, y0, cx, cy);
+					const int k = static_cast<int>(p - qoff[j]), iy = k / cx, ix = k - iy * cx;
+					const int px = x0 + 2 * ix, py = y0 + 2 * iy;
+					const VisRec& r = srec[j];
+					const int kx = (px - static_cast<int>(r.minxy & 0xffffu)) >> 1, ky = (py - static_cast<int>(r.minxy >> 16)) >> 1;
+					// the quad's lanes (c, c+dy, c+dx, c+dx+dy), stepped as the
+					// rasteriser loops step (rglv_triangle.hxx:253-304)
+					const int32_t q1 = wadd(wadd(r.c1, wmul(kx, wmul(r.dy12, 2))), wmul(ky, wmul(r.dx12, 2)));
+					const int32_t q2 = wadd(wadd(r.c2, wmul(kx, wmul(r.dy23, 2))), wmul(ky, wmul(r.dx23, 2)));
+					const int32_t q3 = wadd(wadd(r.c3, wmul(kx, wmul(r.dy31, 2))), wmul(ky, wmul(r.dx31, 2)));
+					const int32_t e1[4] = { q1, wadd(q1, r.dy12), wadd(q1, r.dx12), wadd(wadd(q1, r.dx12), r.dy12) };
+					const int32_t e2[4] = { q2, wadd(q2, r.dy23), wadd(q2, r.dx23), wadd(wadd(q2, r.dx23), r.dy23) };
+					const int32_t e3[4] = { q3, wadd(q3, r.dy31), wadd(q3, r.dx31), wadd(wadd(q3, r.dx31), r.dy31) };
+					uint32_t live = 0;
 #pragma unroll
-			for (int l = 0; l < 4; ++l) live |= ((e1[l] | e2[l] | e3[l]) >= 0 ? 1u : 0u) << l;
-			if (!live) continue;
-			const int di = __builtin_amdgcn_readfirstlane(sdraw[j]);
-			const DrawDesc& dd = draws[di];
-			const DrawLite& dl = lite[di];
-			if (dd.scissorOn) {
-				// the scissor flags of the rasteriser loops mask exactly the
-				// lanes outside the clipped raster rect (tile origins are even)
-				const Rect S = draw_tile_rect(dl, (R.x0 / f.tileW), (R.y0 / f.tileH));
+					for (int l = 0; l < 4; ++l) live |= ((e1[l] | e2[l] | e3[l]) >= 0 ? 1u : 0u) << l;
+					const int di = sdraw[j];
+					const DrawDesc& dd = draws[di];
+					if (live && dd.scissorOn) {
+						// the scissor flags of the rasteriser loops mask exactly
+						// the lanes outside the clipped raster rect (tile origins
+						// are even)
+						const Rect S = draw_tile_rect(lite[di], (R.x0 / f.tileW), (R.y0 / f.tileH));
 #pragma unroll
-				for (int l = 0; l < 4; ++l) {
-					const int x = qx + (l & 1), y = qy + (l >> 1);
-					if (x < S.x0 || x >= S.x1 || y < S.y0 || y >= S.y1) live &= ~(1u << l); }}
-			frags += static_cast<uint32_t>(__popc(live));
-			float bx[4], by[4], bz[4], fd[4];
+						for (int l = 0; l < 4; ++l) {
+							const int x = px + (l & 1), y = py + (l >> 1);
+							if (x < S.x0 || x >= S.x1 || y < S.y0 || y >= S.y1) live &= ~(1u << l); }}
+					if (live) {
+						frags += static_cast<uint32_t>(__popc(live));
+						float bx[4], by[4], bz[4], fd[4];
 #pragma unroll
-			for (int l = 0; l < 4; ++l) {
-				bx[l] = itof(e2[l]) * r.scale;
-				bz[l] = itof(e1[l]) * r.scale;
-				by[l] = (1.0F - bx[l]) - bz[l];
-				fd[l] = (bx[l] * r.z0 + by[l] * r.z1) + bz[l] * r.z2; }
-			// Render: early or late depth test and depth write around the
-			// program, then blend and the masked colour store
-			const bool earlyZ = program_info(dd.program).earlyZ;
-			uint32_t mask = live;
-			if (dd.depthTest && earlyZ) {
+						for (int l = 0; l < 4; ++l) {
+							bx[l] = itof(e2[l]) * r.scale;
+							bz[l] = itof(e1[l]) * r.scale;
+							by[l] = (1.0F - bx[l]) - bz[l];
+							fd[l] = (bx[l] * r.z0 + by[l] * r.z1) + bz[l] * r.z2; }
+						// the program on the triangle's vertices (1/w, varyings) from
+						// LDS: it has no side effects, so it runs before the quad's
+						// depth test, which the ordered pass below applies
+						const uint32_t src = ssrc[j];
+						const float w3[3] = { sw[j][0], sw[j][1], sw[j][2] };
+						const float* sv = svary + static_cast<long long>(j) * 3 * vs;
+						float col[4][4];
+						ShadeQuad sq{ dd, w3, sv, sv + vs, sv + 2 * vs,
+						              bx, by, bz, fd, px, py, f.H, (src >> 31) != 0u, col, 0xfu, true };
+						dispatch_program(dd.program, sq, L{});
+						pm = (live & sq.keep) | (live << 4);
 #pragma unroll
-				for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
-				if (!mask) continue; }
-			if (dd.depthWrite && earlyZ) {
+						for (int l = 0; l < 4; ++l) {
+							wfd[t][l] = fd[l];
+							wcol[t][l] = make_float4(col[l][0], col[l][1], col[l][2], col[l][3]); }}}
+				wmask[t] = pm;
+			}
+			__syncthreads();
+			// apply: each thread walks its quad's pairs of the window in
+			// triangle order with the frame commands between them:
+			// TriangleProgram::Render's early or late depth test and depth
+			// write around the program, then blend and the masked colour store
+			int jb = 0, je = static_cast<int>(cnt) - 1;   // the first triangle whose pairs end after w0
+			while (jb < je) { const int mid = (jb + je) >> 1; if (qoff[mid + 1] > w0) je = mid; else jb = mid + 1; }
+			for (int j = jb; j < static_cast<int>(cnt) && qoff[j] < w0 + 256u; ++j) {
+				const uint32_t slot = srec[j].slot;
+				while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
+				int x0, y0, cx, cy;
+				pair_rect(j, x0, y0, cx, cy);
+				if (qx < x0 || qy < y0) continue;
+				const int ix = (qx - x0) >> 1, iy = (qy - y0) >> 1;
+				if (ix >= cx || iy >= cy) continue;
+				const uint32_t p = qoff[j] + static_cast<uint32_t>(iy * cx + ix);
+				if (p < w0 || p >= w0 + 256u) continue;
+				const int wi = static_cast<int>(p - w0);
+				const uint32_t pm = wmask[wi];
+				uint32_t mask = pm >> 4;
+				if (!mask) continue;
+				const int di = __builtin_amdgcn_readfirstlane(sdraw[j]);
+				const DrawDesc& dd = draws[di];
+				const DrawLite& dl = lite[di];
+				float fd[4];
 #pragma unroll
-				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
-			// the triangle's vertices (1/w, varyings) from LDS
-			const uint32_t src = ssrc[j], srcTri = src & 0x7fffffffu;
-			const float w3[3] = { sw[j][0], sw[j][1], sw[j][2] };
-			const float* sv = svary + static_cast<long long>(j) * 3 * vs;
-			float col[4][4];
-			ShadeQuad sq{ dd, w3, sv, sv + vs, sv + 2 * vs,
-			              bx, by, bz, fd, qx, qy, f.H, (src >> 31) != 0u, col, 0xfu, true };
-			dispatch_program(dd.program, sq, L{});
-			mask &= sq.keep;   // gl_triMask after ShadeFragment
-			if (dd.depthTest && !earlyZ) {
+				for (int l = 0; l < 4; ++l) fd[l] = wfd[wi][l];
+				const bool earlyZ = program_info(dd.program).earlyZ;
+				if (dd.depthTest && earlyZ) {
 #pragma unroll
-				for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
-				if (!mask) continue; }
-			if (dd.depthWrite && !earlyZ) {
+					for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+					if (!mask) continue; }
+				if (dd.depthWrite && earlyZ) {
 #pragma unroll
-				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
-			if (dd.colorWrite) {
+					for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
+				mask &= pm & 0xfu;   // gl_triMask after ShadeFragment
+				if (dd.depthTest && !earlyZ) {
 #pragma unroll
-				for (int l = 0; l < 4; ++l) {
-					if (!(mask & (1u << l))) continue;
-					if (dd.blend) {   // BlendAlpha (rglv_gpu_impl.hxx:80-84)
-						const float al = col[l][3], om = 1.0F - al;
-						cr[l] = col[l][0] * al + cr[l] * om;
-						cg[l] = col[l][1] * al + cg[l] * om;
-						cb[l] = col[l][2] * al + cb[l] * om; }
-					else { cr[l] = col[l][0]; cg[l] = col[l][1]; cb[l] = col[l][2]; }}}
-			const uint32_t prim = dl.primBase + srcTri;
+					for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+					if (!mask) continue; }
+				if (dd.depthWrite && !earlyZ) {
 #pragma unroll
-			for (int l = 0; l < 4; ++l) if (mask & (1u << l)) pid[l] = prim; }}
+					for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
+				if (dd.colorWrite) {
+#pragma unroll
+					for (int l = 0; l < 4; ++l) {
+						if (!(mask & (1u << l))) continue;
+						const float4 c = wcol[wi][l];
+						if (dd.blend) {   // BlendAlpha (rglv_gpu_impl.hxx:80-84)
+							const float al = c.w, om = 1.0F - al;
+							cr[l] = c.x * al + cr[l] * om;
+							cg[l] = c.y * al + cg[l] * om;
+							cb[l] = c.z * al + cb[l] * om; }
+						else { cr[l] = c.x; cg[l] = c.y; cb[l] = c.z; }}}
+				const uint32_t prim = dl.primBase + (ssrc[j] & 0x7fffffffu);
+#pragma unroll
+				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) pid[l] = prim; }
+			__syncthreads(); }}
 	while (opi < ope) { apply_op(ops[opi]); ++opi; }
 	// fragments that reached the depth test, one atomic per workgroup
 	for (int o = 32; o > 0; o >>= 1) frags += __shfl_xor(frags, o, 64);
