@@ -1073,7 +1073,7 @@ struct rgl_gpu {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; size_t lds;
 		template <class P> void run() { launch<ProgramList<P>>(); }
 		template <class L> void launch() {
-			k_inorder<L><<<g.nIoBlocks, 256, lds, g.stream>>>(b, ra, g.dIo.p, g.nIo, b.frames, g.dOps.p, g.dDraws.p, g.dLite.p, g.tileDesc.p,
+			k_inorder<L><<<g.nIoBlocks, kIoThreads, lds, g.stream>>>(b, ra, g.dIo.p, g.nIo, b.frames, g.dOps.p, g.dDraws.p, g.dLite.p, g.tileDesc.p,
 				g.rec.p, g.crec.p, g.order.p); }};
 	struct ResolveLaunch {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;
@@ -1163,7 +1163,7 @@ struct rgl_gpu {
 		// double-buffered frame has finished
 		if (rasterAfter && (e = hipStreamWaitEvent(stream, rasterAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 		if (nIo > 0) {
-			InorderLaunch il{ *this, b, ra, sizeof(float) * 256 * 3 * static_cast<size_t>(varyStride) };
+			InorderLaunch il{ *this, b, ra, sizeof(float) * kIoThreads * 3 * static_cast<size_t>(varyStride) };
 			if (pid < 0 || !dispatch_program(pid, il, AllPrograms{})) il.launch<AllPrograms>(); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));

@@ -12,10 +12,10 @@
 //                 records, inline and clipped, sorted by submission slot (an
 //                 LDS bitonic sort; above kSortCap records, LDS-sorted runs
 //                 merged in global memory).
-//   k_inorder     one workgroup per 32 x 32-pixel region of a tile, one
+//   k_inorder     one workgroup per 16 x 16-pixel region of a tile, one
 //                 thread per 2x2 quad: the quad's colour, depth and
 //                 primitive id stay in registers while the workgroup steps
-//                 through the tile's triangles (from LDS, 256 at a time) and
+//                 through the tile's triangles (from LDS, 64 at a time) and
 //                 frame commands in slot order, applying
 //                 TriangleProgram::Render (rglv_gpu_impl.hxx:166-222) to the
 //                 quad: coverage, scissor, early or late depth test, depth
@@ -63,7 +63,8 @@ struct alignas(16) IoFrame {
 constexpr int kAttachColorDepth = 0, kAttachRGBF32 = 1;
 
 constexpr int kSortCap = 4096;       // records a tile sorts in LDS in one piece
-constexpr int kRegionDim = 32;       // k_inorder region: 32 x 32 pixels, 16 x 16 quads
+constexpr int kRegionDim = 16;       // k_inorder region: 16 x 16 pixels, 8 x 8 quads
+constexpr int kIoThreads = (kRegionDim / 2) * (kRegionDim / 2);   // one thread per quad of the region
 
 __device__ inline int find_io(const IoFrame* io, int n, int blk, bool tiles) {
 	int lo = 0, hi = n - 1;
@@ -185,27 +186,27 @@ __device__ inline bool depth_pass(int func, float frag, float dest) {
 // that program's shading, which keeps the registers, and so the resident
 // waves that hide each triangle's texture fetches, of that program alone)
 template <class L>
-__global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, const IoFrame* __restrict__ io, int nIo,
+__global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs a, const IoFrame* __restrict__ io, int nIo,
 		const FrameDesc* __restrict__ frames, const FrameOp* __restrict__ ops, const DrawDesc* __restrict__ draws,
 		const DrawLite* __restrict__ lite, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
 		const VisRec* __restrict__ crec, const uint32_t* __restrict__ order) {
-	__shared__ VisRec srec[256];
-	__shared__ int sdraw[256];
-	__shared__ uint32_t sfrag[4];
+	__shared__ VisRec srec[kIoThreads];
+	__shared__ int sdraw[kIoThreads];
+	__shared__ uint32_t sfrag[kIoThreads / 64];
 	// each batch's triangles: 1/w of the three vertices, the source
 	// triangle (bit 31: a clipped fan), and the vertices' varyings at the
-	// batch's stride (dynamic LDS: 256 x 3 x varyStride floats), loaded by
+	// batch's stride (dynamic LDS: kIoThreads x 3 x varyStride floats), loaded by
 	// one thread per triangle before the workgroup steps through them
-	__shared__ float sw[256][3];
-	__shared__ uint32_t ssrc[256];
-	__shared__ uint32_t scan_sh[256 / 64 + 1];
-	// a window of 256 (triangle, quad) pairs, shaded in parallel: the
+	__shared__ float sw[kIoThreads][3];
+	__shared__ uint32_t ssrc[kIoThreads];
+	__shared__ uint32_t scan_sh[kIoThreads / 64 + 1];
+	// a window of kIoThreads (triangle, quad) pairs, shaded in parallel: the
 	// lanes' depths and colours, the live and kept lanes; qoff[j]: the first
 	// pair of batch triangle j
-	__shared__ float wfd[256][4];
-	__shared__ float4 wcol[256][4];
-	__shared__ uint32_t wmask[256];
-	__shared__ uint32_t qoff[257];
+	__shared__ float wfd[kIoThreads][4];
+	__shared__ float4 wcol[kIoThreads][4];
+	__shared__ uint32_t wmask[kIoThreads];
+	__shared__ uint32_t qoff[kIoThreads + 1];
 	extern __shared__ float svary[];
 	const int vs = a.varyStride;
 	const int blk = static_cast<int>(blockIdx.x);
@@ -219,8 +220,8 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	const int rgy = reg / e.regionsX, rgx = reg - rgy * e.regionsX;
 	const int t = static_cast<int>(threadIdx.x);
 	// this thread's quad: top-left pixel (qx, qy); lane l = (l & 1, l >> 1)
-	const int qx = R.x0 + rgx * kRegionDim + 2 * (t & 15);
-	const int qy = R.y0 + rgy * kRegionDim + 2 * (t >> 4);
+	const int qx = R.x0 + rgx * kRegionDim + 2 * (t % (kRegionDim / 2));
+	const int qy = R.y0 + rgy * kRegionDim + 2 * (t / (kRegionDim / 2));
 	bool onScreen[4];
 #pragma unroll
 	for (int l = 0; l < 4; ++l) onScreen[l] = qx + (l & 1) < R.x1 && qy + (l >> 1) < R.y1;
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	// only through the triangles that can touch its pixels
 	const int rx0 = R.x0 + rgx * kRegionDim, ry0 = R.y0 + rgy * kRegionDim;
 	const int rx1 = min(rx0 + kRegionDim, R.x1), ry1 = min(ry0 + kRegionDim, R.y1);
-	for (uint32_t base = 0; base < n; base += 256) {
+	for (uint32_t base = 0; base < n; base += kIoThreads) {
 		__syncthreads();
 		VisRec rr;
 		bool hit = false;
@@ -334,7 +335,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 		__syncthreads();
 		// the batch's (triangle, quad) pairs inside the region, triangle by
 		// triangle in order: triangle j's quads of its quad rect that lie in
-		// the region's 16 x 16 quads (both grids step 2 from even origins)
+		// the region's quads (both grids step 2 from even origins)
 		auto pair_rect = [&](int j, int& x0, int& y0, int& cx, int& cy) {
 			const int minx = static_cast<int>(srec[j].minxy & 0xffffu), miny = static_cast<int>(srec[j].minxy >> 16);
 			const int nqx = static_cast<int>(srec[j].nq & 0xffffu), nqy = static_cast<int>(srec[j].nq >> 16);
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 		if (static_cast<uint32_t>(t) < cnt) qoff[t] = poff;
 		if (t == 0) qoff[cnt] = total;
 		__syncthreads();
-		for (uint32_t w0 = 0; w0 < total; w0 += 256) {
+		for (uint32_t w0 = 0; w0 < total; w0 += kIoThreads) {
 			// shade: thread t takes pair w0 + t (coverage, scissor, the
 			// program), in parallel over the window
 			{
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 			// write around the program, then blend and the masked colour store
 			int jb = 0, je = static_cast<int>(cnt) - 1;   // the first triangle whose pairs end after w0
 			while (jb < je) { const int mid = (jb + je) >> 1; if (qoff[mid + 1] > w0) je = mid; else jb = mid + 1; }
-			for (int j = jb; j < static_cast<int>(cnt) && qoff[j] < w0 + 256u; ++j) {
+			for (int j = jb; j < static_cast<int>(cnt) && qoff[j] < w0 + kIoThreads; ++j) {
 				const uint32_t slot = srec[j].slot;
 				while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
 				int x0, y0, cx, cy;
@@ -429,7 +430,7 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 				const int ix = (qx - x0) >> 1, iy = (qy - y0) >> 1;
 				if (ix >= cx || iy >= cy) continue;
 				const uint32_t p = qoff[j] + static_cast<uint32_t>(iy * cx + ix);
-				if (p < w0 || p >= w0 + 256u) continue;
+				if (p < w0 || p >= w0 + kIoThreads) continue;
 				const int wi = static_cast<int>(p - w0);
 				const uint32_t pm = wmask[wi];
 				uint32_t mask = pm >> 4;
@@ -477,7 +478,8 @@ __global__ __launch_bounds__(256) void k_inorder(BatchDesc b, RasterArgs a, cons
 	if ((t & 63) == 0) sfrag[t >> 6] = frags;
 	__syncthreads();
 	if (t == 0) {
-		const unsigned long long sum = static_cast<unsigned long long>(sfrag[0]) + sfrag[1] + sfrag[2] + sfrag[3];
+		unsigned long long sum = 0;
+		for (int w = 0; w < kIoThreads / 64; ++w) sum += sfrag[w];
 		if (sum) atomicAdd(&a.ctr->slot[blockIdx.x % kCtrSlots].fragments, sum); }}
 
 }  // namespace rgl
