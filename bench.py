@@ -125,9 +125,9 @@ def cpu_baseline(scene, priming: int = 100, frames: int = 1000, seconds: float =
 
 MLP_FLOP_PER_FRAGMENT = 2 * (117 * 64 + 64 * 64 + 64 * 64 + 64 * 3)   # 31,744
 MFMA_BF16_PEAK_TFLOPS = 2500.0
-# the learned frames' fallback limit: the RMS error (in [0,1] units) of the
-# 40 dB PSNR target, 10^(-40/20) = 0.01 = 2.55/255; tile quadrants whose
-# probes show more are re-shaded with the plain shader
+# the learned frames' per-tile fallback limit: the RMS error (in [0,1]
+# units) of the 40 dB PSNR target, 10^(-40/20) = 0.01 = 2.55/255; tiles
+# whose probes show more are re-shaded with the plain shader
 LEARNED_TILE_RMS = 0.01
 
 

@@ -545,11 +545,13 @@ __global__ __launch_bounds__(64 * kMlpWaves) void k_mlp_shade(BatchDesc b, const
 				uint32_t* out = b.frames[pf[j]].outColor;
 				if (pok[j] && out) out[po[j]] = (srgb_unit_to8(y[j][0]) << 16) | (srgb_unit_to8(y[j][1]) << 8) | srgb_unit_to8(y[j][2]); }}}
 
-// the fallback decision of one tile's four quadrants, by one wave: per
-// quadrant (the probes of its 4 x 4 of the tile's 8 x 8 probe cells), the
-// mean squared 8-bit error per channel at its covered probes (the
-// network's stored colour against the plain one) over the limit; bit q of
-// the result is quadrant q = (right half) + 2 (bottom half)
+// the fallback decision of one tile, by one wave: the mean squared 8-bit
+// error per channel at the tile's covered probes (the network's stored
+// colour against the plain one) over the limit.  The result has a bit per
+// tile quadrant, q = (right half) + 2 (bottom half), the unit the
+// re-shading is dealt out in: all four or none.  (A decision per quadrant,
+// from its 16 probes, missed quadrants whose probes all fell on good
+// pixels: a 4K view then measured 39 dB however low the limit.)
 __device__ inline uint32_t tile_over_limit(const FrameDesc& f, int tile, const uint32_t* __restrict__ want, float maxMse) {
 	const int j = threadIdx.x & 63;
 	int x, y;
@@ -561,16 +563,10 @@ __device__ inline uint32_t tile_over_limit(const FrameDesc& f, int tile, const u
 			const float dv = static_cast<float>(static_cast<int>((w >> (8 * c)) & 255u) - static_cast<int>((got >> (8 * c)) & 255u));
 			se += dv * dv; }
 		cnt = 3.0F; }
-	const int q = ((j & 7) >> 2) | (((j >> 3) >> 2) << 1);
-	uint32_t over = 0;
-#pragma unroll
-	for (int qq = 0; qq < 4; ++qq) {
-		float s = q == qq ? se : 0.0F, c = q == qq ? cnt : 0.0F;
-		for (int o = 32; o > 0; o >>= 1) { s += __shfl_xor(s, o, 64); c += __shfl_xor(c, o, 64); }
-		if (c > 0.0F && s / c > maxMse) over |= 1u << qq; }
-	return over; }
+	for (int o = 32; o > 0; o >>= 1) { se += __shfl_xor(se, o, 64); cnt += __shfl_xor(cnt, o, 64); }
+	return cnt > 0.0F && se / cnt > maxMse ? 0xfu : 0u; }
 
-// the fallback decision: one wave per tile appends the tile's quadrants
+// the fallback decision: one wave per tile appends the quadrants of a tile
 // over the limit (tile * 4 + quadrant) to `list`, counting them in `count`
 // (zeroed at the frame's start)
 __global__ __launch_bounds__(256) void k_mlp_decide(BatchDesc b, const uint32_t* __restrict__ want, float maxMse,

@@ -121,8 +121,7 @@ def test_learned_frame_and_training_gain():
     assert q_fallback >= q_trained
     # background pixels are the clear colour in every variant
     assert np.array_equal(pc[~cov], lc[~cov]) and np.array_equal(pc[~cov], fc[~cov])
-    # a zero error budget re-shades every tile quadrant whose probes saw any
-    # error
+    # a zero error budget re-shades every tile whose probes saw any error
     zc, _, _, zst = _frame(gpu, sc, params, max_rms=0.0)
     same = sum(np.array_equal(zc[y0:y1, x0:x1], pc[y0:y1, x0:x1])
                for _, _, (x0, y0, x1, y1) in _quadrants(w, h, sc.tile_blocks[0] * 8, sc.tile_blocks[1] * 8))
@@ -165,11 +164,11 @@ def _probe(x0, y0, x1, y1, tile, j):
 
 
 @pytest.mark.parametrize("max_rms,tile", [(2.0 / 255.0, 8), (0.01, 8), (6.0 / 255.0, 8), (0.01, 4), (2.0 / 255.0, 4)])
-def test_fallback_quadrants_follow_the_probes(max_rms, tile):
-    # every tile quadrant whose covered probes (the 16 of its 4 x 4 probe
-    # cells) show a mean squared 8-bit error over the limit (the network's
-    # colours against the plain shader's) is the plain frame's quadrant,
-    # every other quadrant the network's, bit for bit
+def test_fallback_tiles_follow_the_probes(max_rms, tile):
+    # every tile whose covered probes show a mean squared 8-bit error over
+    # the limit (the network's colours against the plain shader's) is the
+    # plain frame's tile, every other tile the network's, bit for bit; the
+    # count is of re-shaded tile quadrants, four per tile
     import os
     wpath = os.path.join(os.path.dirname(learned.__file__), "ggx_mlp_weights.npy")
     params = torch.from_numpy(np.load(wpath, allow_pickle=False).astype(np.float32)).to(DEV)
@@ -192,23 +191,18 @@ def test_fallback_quadrants_follow_the_probes(max_rms, tile):
             t = ty * tiles_x + tx
             x0, y0 = tx * tw, ty * th
             x1, y1 = min(x0 + tw, sc.width), min(y0 + th, sc.height)
-            cw, chh = (x1 - x0 + 7) // 8, (y1 - y0 + 7) // 8
-            se, cnt = [0] * 4, [0] * 4
+            se = cnt = 0
             for j in range(64):
                 p = _probe(x0, y0, x1, y1, t, j)
                 if p is None or not pd[p[1], p[0]] < 1.0:
                     continue
-                q = int((j & 7) >= 4) + 2 * int((j >> 3) >= 4)
-                se[q] += int(((ch(pc[p[1], p[0]]) - ch(nc[p[1], p[0]])) ** 2).sum())
-                cnt[q] += 3
-            for q in range(4):
-                qx0, qx1 = (x0 + 4 * cw, x1) if q & 1 else (x0, min(x0 + 4 * cw, x1))
-                qy0, qy1 = (y0 + 4 * chh, y1) if q >> 1 else (y0, min(y0 + 4 * chh, y1))
-                over = cnt[q] > 0 and np.float32(se[q]) / np.float32(cnt[q]) > max_mse
-                over_quads += int(over)
-                want = (pc if over else nc)[qy0:qy1, qx0:qx1]
-                assert np.array_equal(fc[qy0:qy1, qx0:qx1], want), (tx, ty, q, over, se[q], cnt[q])
-    print(f"limit {max_rms * 255:.2f}/255: {over_quads} tile quadrants re-shaded")
+                se += int(((ch(pc[p[1], p[0]]) - ch(nc[p[1], p[0]])) ** 2).sum())
+                cnt += 3
+            over = cnt > 0 and np.float32(se) / np.float32(cnt) > max_mse
+            over_quads += 4 * int(over)
+            want = (pc if over else nc)[y0:y1, x0:x1]
+            assert np.array_equal(fc[y0:y1, x0:x1], want), (tx, ty, over, se, cnt)
+    print(f"limit {max_rms * 255:.2f}/255: {over_quads // 4} tiles re-shaded")
     assert fst["fallback_quads"] == over_quads
     gpu.set_learned_shading(None, 0.0)
     gpu.close()
