@@ -48,7 +48,7 @@ typedef struct rgl_stats {
 	int64_t clipped_tile_pairs;  /* (clipped triangle, tile) pairs */
 	int64_t fragments;           /* covered pixels that reached the depth test */
 	int64_t overflow;            /* nonzero: a capacity was exceeded, frame invalid */
-	int64_t fallback_quads;      /* learned shading: tile quadrants re-shaded by the plain shader */
+	int64_t fallback_quads;      /* learned shading: tile quadrants re-shaded by the plain shader (4 per tile) */
 } rgl_stats;
 
 /* --- rglv::GPU ---------------------------------------------------------- */
@@ -255,9 +255,9 @@ int rgl_mlp_param_count(void);
 int rgl_mlp_trainable_count(void);
 
 /* Frames drawn with program 82 shade with the network from the next Run on:
-   the plain shader checks 64 probe pixels per tile (16 per tile quadrant)
-   and re-shades every tile quadrant whose RMS error (8-bit, over its
-   covered probes) exceeds max_rms_err (in
+   the plain shader checks 64 probe pixels per tile and re-shades every
+   tile whose RMS error (8-bit, over its covered probes) exceeds
+   max_rms_err (in
    [0,1] units, e.g. 2/255); max_rms_err >= 1 checks nothing and keeps the
    network's colours everywhere.  Coverage, depth and primitive ids are
    never affected.  dev_params = NULL switches back to the plain shader. */

@@ -59,7 +59,7 @@ struct alignas(128) Counters {
 	uint32_t overflow;       // bit 0 list, 1 clip queue, 2 clipped list, 3 bad index
 	uint32_t workItems;      // visibility work items of the frame
 	uint32_t workNext;       // next work item to hand out
-	uint32_t fallbackQuads;  // learned shading: tile quadrants re-shaded by the plain shader
+	uint32_t fallbackQuads;  // learned shading: tile quadrants re-shaded by the plain shader (4 per tile)
 	uint32_t pad[25];
 	CtrSlot slot[kCtrSlots];
 };
