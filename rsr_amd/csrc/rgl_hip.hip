@@ -746,7 +746,7 @@ struct rgl_gpu {
 		if (n > 1) for (int i = 0; i < n; ++i)
 			if (!recs[i]->sampleReqs.empty()) return fail(RGL_E_UNSUPPORTED, "training samples are stored from single-frame runs");
 		// the in-order frames: one k_tile_order block per tile, one k_inorder
-		// block per 32 x 32 region of a tile
+		// block per kRegionDim x kRegionDim region of a tile
 		std::vector<IoFrame> io;
 		int ioTiles = 0, ioBlocks = 0;
 		for (int i = 0; i < n; ++i) {
