@@ -271,6 +271,7 @@ struct rgl_gpu {
 	std::vector<uint8_t> lastIo;
 	int nIo = 0, nIoTiles = 0, nIoBlocks = 0;
 	DevArray<uint32_t> order;
+	DevArray<int4> ioBoxes;         // the pixel rect of every 64 sorted records of an in-order tile
 	DevArray<unsigned long long> skeys0, skeys1;
 	int segPairs = kSegPairs;       // (triangle, tile) pairs per visibility work item
 	int splitQuads = kSplitQuads;   // quad work per quadrant from which visibility regions are halved
@@ -368,7 +369,7 @@ struct rgl_gpu {
 		dFrames.release();
 		if (hOps) (void)hipHostFree(hOps);
 		if (hIo) (void)hipHostFree(hIo);
-		dOps.release(); dIo.release(); order.release(); skeys0.release(); skeys1.release();
+		dOps.release(); dIo.release(); order.release(); ioBoxes.release(); skeys0.release(); skeys1.release();
 		if (auxDone) (void)hipEventDestroy(auxDone);
 		if (graphExec) (void)hipGraphExecDestroy(graphExec);
 		if (capStream) (void)hipStreamDestroy(capStream);
@@ -789,6 +790,7 @@ struct rgl_gpu {
 		if (!io.empty()) {
 			const size_t nl = static_cast<size_t>(listCap) + clistCap;
 			if ((e = Grow(order, nl)) != hipSuccess) return hipfail(e, "hipMalloc tile order");
+			if ((e = Grow(ioBoxes, nl)) != hipSuccess) return hipfail(e, "hipMalloc tile order rects");
 			if ((e = Grow(skeys0, nl)) != hipSuccess) return hipfail(e, "hipMalloc sort keys");
 			if ((e = Grow(skeys1, nl)) != hipSuccess) return hipfail(e, "hipMalloc sort keys"); }
 		if (!dOps.p) {
@@ -1016,7 +1018,7 @@ struct rgl_gpu {
 		const void* ptrs[] = { vpos.p, vflags.p, vvary.p, trect.p, tvert.p, tileCount.p, tileCursor.p, tileOff.p, clipq.p, clipqOf.p,
 		                       cvpos.p, cvvary.p, fcount.p, frect.p, clipHist.p, clipOff.p, clipCursor.p, itemTile.p, tileDesc.p, scanPart.p, tileCost.p,
 		                       keys.p, rec.p, crec.p, mlpImg.p, probeWant.p, quadList.p, mlpQueue.p, qCount.p, mlpParams,
-		                       dOps.p, dIo.p, order.p, skeys0.p, skeys1.p };
+		                       dOps.p, dIo.p, order.p, ioBoxes.p, skeys0.p, skeys1.p };
 		mix(ptrs, sizeof(ptrs));
 		const uint32_t caps[] = { listCap, clipCap, clistCap, mlpSegCap, static_cast<uint32_t>(items), learned ? 1u : 0u,
 		                          static_cast<uint32_t>(mlpShadeBlocks) };
@@ -1074,7 +1076,7 @@ struct rgl_gpu {
 		template <class P> void run() { launch<ProgramList<P>>(); }
 		template <class L> void launch() {
 			k_inorder<L><<<g.nIoBlocks, kIoThreads, lds, g.stream>>>(b, ra, g.dIo.p, g.nIo, b.frames, g.dOps.p, g.dDraws.p, g.dLite.p, g.tileDesc.p,
-				g.rec.p, g.crec.p, g.order.p); }};
+				g.rec.p, g.crec.p, g.order.p, g.ioBoxes.p); }};
 	struct ResolveLaunch {
 		rgl_gpu& g; const BatchDesc& b; const RasterArgs& ra; unsigned rblocks; const DrawLite* lite; const DrawDesc* dd;
 		template <class P> void run() {
@@ -1157,7 +1159,8 @@ struct rgl_gpu {
 		// every tile region walks them (before k_resolve, whose first block
 		// publishes the frame's counters)
 		if (nIo > 0)
-			k_tile_order<<<nIoTiles, 256, 0, stream>>>(b, dIo.p, nIo, b.frames, tileDesc.p, rec.p, crec.p, order.p, skeys0.p, skeys1.p);
+			k_tile_order<<<nIoTiles, 256, 0, stream>>>(b, dIo.p, nIo, b.frames, tileDesc.p, rec.p, crec.p, order.p, skeys0.p, skeys1.p,
+				ioBoxes.p);
 		// everything above writes this executor's buffers alone; what
 		// follows writes or samples the caller's, after the previous
 		// double-buffered frame has finished

@@ -84,6 +84,28 @@ __device__ inline void bitonic_sort(unsigned long long* s, uint32_t m) {
 					if ((x > y) == ((i & k) == 0)) { s[i] = y; s[ixj] = x; }}}
 			__syncthreads(); }}
 
+// The pixel rect of the quads of every kIoThreads consecutive records of a
+// tile's sorted list (one wave each): a k_inorder region skips the batches
+// whose rect misses it.  boxes[ob + k] holds the rect of records k ..
+// k + kIoThreads - 1 as (x0, y0, x1, y1), x1 and y1 exclusive.
+static_assert(kIoThreads == 64, "a batch's rect is one wave's reduction");
+__device__ inline void batch_boxes(uint32_t n, uint32_t ninl, const VisTile& d, uint32_t ob, const VisRec* __restrict__ rec,
+                                   const VisRec* __restrict__ crec, const uint32_t* __restrict__ order, int4* __restrict__ boxes) {
+	__syncthreads();   // the workgroup's order[] stores
+	const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	for (uint32_t k0 = wv * 64u; k0 < n; k0 += nw * 64u) {   // uniform over the wave
+		int x0 = 0x7fffffff, y0 = 0x7fffffff, x1 = -0x7fffffff, y1 = -0x7fffffff;
+		if (k0 + lane < n) {
+			const uint32_t p = order[ob + k0 + lane];
+			const VisRec& r = p < ninl ? rec[d.ib + p] : crec[d.cb + (p - ninl)];
+			const int minx = static_cast<int>(r.minxy & 0xffffu), miny = static_cast<int>(r.minxy >> 16);
+			x0 = minx; y0 = miny;
+			x1 = minx + 2 * static_cast<int>(r.nq & 0xffffu); y1 = miny + 2 * static_cast<int>(r.nq >> 16); }
+		for (int o = 32; o > 0; o >>= 1) {
+			x0 = min(x0, __shfl_xor(x0, o, 64)); y0 = min(y0, __shfl_xor(y0, o, 64));
+			x1 = max(x1, __shfl_xor(x1, o, 64)); y1 = max(y1, __shfl_xor(y1, o, 64)); }
+		if (lane == 0) boxes[ob + k0] = make_int4(x0, y0, x1, y1); }}
+
 // A tile's records in submission order.  Keys are (slot << 32 | p), p the
 // record's position in the tile (inline records first, then clipped); every
 // (triangle, tile) pair has its own slot, so keys are distinct.  order[ob +
@@ -92,7 +114,7 @@ __device__ inline void bitonic_sort(unsigned long long* s, uint32_t m) {
 __global__ __launch_bounds__(256) void k_tile_order(BatchDesc b, const IoFrame* __restrict__ io, int nIo,
 		const FrameDesc* __restrict__ frames, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
 		const VisRec* __restrict__ crec, uint32_t* __restrict__ order, unsigned long long* __restrict__ skeys0,
-		unsigned long long* __restrict__ skeys1) {
+		unsigned long long* __restrict__ skeys1, int4* __restrict__ boxes) {
 	__shared__ unsigned long long s[kSortCap];
 	const int blk = static_cast<int>(blockIdx.x);
 	const IoFrame e = io[find_io(io, nIo, blk, true)];
@@ -111,6 +133,7 @@ __global__ __launch_bounds__(256) void k_tile_order(BatchDesc b, const IoFrame* 
 		__syncthreads();
 		bitonic_sort(s, m);
 		for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(s[i]);
+		batch_boxes(n, ninl, d, ob, rec, crec, order, boxes);
 		return; }
 	// runs of kSortCap sorted in LDS, then merged pairwise: each key's place
 	// in the merged run is its index in its own run plus its rank (binary
@@ -138,7 +161,8 @@ __global__ __launch_bounds__(256) void k_tile_order(BatchDesc b, const IoFrame* 
 			dst[a0 + own + (lo - base)] = key; }
 		__syncthreads();
 		unsigned long long* x = src; src = dst; dst = x; }
-	for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(src[i]); }
+	for (uint32_t i = t; i < n; i += blockDim.x) order[ob + i] = static_cast<uint32_t>(src[i]);
+	batch_boxes(n, ninl, d, ob, rec, crec, order, boxes); }
 
 // ShadeFragment of program P for a quad at (qx, qy): perspective-correct BP
 // from BS and the vertices' 1/w (Render, rglv_gpu_impl.hxx:187-191),
@@ -189,7 +213,7 @@ template <class L>
 __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs a, const IoFrame* __restrict__ io, int nIo,
 		const FrameDesc* __restrict__ frames, const FrameOp* __restrict__ ops, const DrawDesc* __restrict__ draws,
 		const DrawLite* __restrict__ lite, const VisTile* __restrict__ tileDesc, const VisRec* __restrict__ rec,
-		const VisRec* __restrict__ crec, const uint32_t* __restrict__ order) {
+		const VisRec* __restrict__ crec, const uint32_t* __restrict__ order, const int4* __restrict__ boxes) {
 	__shared__ VisRec srec[kIoThreads];
 	__shared__ int sdraw[kIoThreads];
 	__shared__ uint32_t sfrag[kIoThreads / 64];
@@ -295,6 +319,9 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 	const int rx0 = R.x0 + rgx * kRegionDim, ry0 = R.y0 + rgy * kRegionDim;
 	const int rx1 = min(rx0 + kRegionDim, R.x1), ry1 = min(ry0 + kRegionDim, R.y1);
 	for (uint32_t base = 0; base < n; base += kIoThreads) {
+		// a batch whose records' quads all miss the region (uniform)
+		const int4 bb = boxes[ob + base];
+		if (bb.x >= rx1 || bb.z <= rx0 || bb.y >= ry1 || bb.w <= ry0) continue;
 		__syncthreads();
 		VisRec rr;
 		bool hit = false;
