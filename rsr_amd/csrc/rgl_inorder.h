@@ -231,6 +231,11 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 	__shared__ float4 wcol[kIoThreads][4];
 	__shared__ uint32_t wmask[kIoThreads];
 	__shared__ uint32_t qoff[kIoThreads + 1];
+	// per batch triangle, for the ordered pass: its draw's fragment state
+	// (bit 0 depth test, 1 depth write, 2-3 depth function, 4 colour write,
+	// 5 blend, 6 early depth test), its primitive id, and its pair rect in
+	// the region (x0 | y0 << 16, cx | cy << 16)
+	__shared__ uint32_t sstate[kIoThreads], sprim[kIoThreads], sxy[kIoThreads], scxy[kIoThreads];
 	extern __shared__ float svary[];
 	const int vs = a.varyStride;
 	const int blk = static_cast<int>(blockIdx.x);
@@ -260,6 +265,7 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 	const uint32_t n = vt.ntot, ninl = vt.ninl, ob = vt.ib + vt.cb;
 	uint32_t opi = static_cast<uint32_t>(f.opBase);
 	const uint32_t ope = static_cast<uint32_t>(f.opBase + f.nOps);
+	uint32_t nextOp = opi < ope ? ops[opi].pos : 0xffffffffu;   // the position of the next frame command
 
 	auto apply_op = [&](const FrameOp& op) {
 		if (op.kind == kOpClear) {
@@ -355,6 +361,10 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 				clipped_verts(a.clipqOf[dl.triBase + srcTri] * kFanMax + jf, a.cvpos, tr, vs);
 				vary = a.cvvary; }
 			ssrc[t2] = srcTri | (tr.clipped ? 0x80000000u : 0u);
+			const DrawDesc& dd = draws[di];
+			sstate[t2] = (dd.depthTest ? 1u : 0u) | (dd.depthWrite ? 2u : 0u) | (static_cast<uint32_t>(dd.depthFunc) << 2) |
+			             (dd.colorWrite ? 16u : 0u) | (dd.blend ? 32u : 0u) | (program_info(dd.program).earlyZ ? 64u : 0u);
+			sprim[t2] = dl.primBase + srcTri;
 			float* sv = svary + static_cast<long long>(t2) * 3 * vs;
 			for (int k = 0; k < 3; ++k) {
 				sw[t2][k] = tr.w[k];
@@ -371,7 +381,12 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 			cx = x1 > x0 ? (x1 - x0 + 1) >> 1 : 0;
 			cy = y1 > y0 ? (y1 - y0 + 1) >> 1 : 0; };
 		uint32_t np = 0;
-		if (static_cast<uint32_t>(t) < cnt) { int x0, y0, cx, cy; pair_rect(t, x0, y0, cx, cy); np = static_cast<uint32_t>(cx * cy); }
+		if (static_cast<uint32_t>(t) < cnt) {
+			int x0, y0, cx, cy;
+			pair_rect(t, x0, y0, cx, cy);
+			np = static_cast<uint32_t>(cx * cy);
+			sxy[t] = static_cast<uint32_t>(x0) | (static_cast<uint32_t>(y0) << 16);
+			scxy[t] = static_cast<uint32_t>(cx) | (static_cast<uint32_t>(cy) << 16); }
 		uint32_t total;
 		const uint32_t poff = block_exclusive_scan(np, scan_sh, total);
 		if (static_cast<uint32_t>(t) < cnt) qoff[t] = poff;
@@ -450,9 +465,10 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 			while (jb < je) { const int mid = (jb + je) >> 1; if (qoff[mid + 1] > w0) je = mid; else jb = mid + 1; }
 			for (int j = jb; j < static_cast<int>(cnt) && qoff[j] < w0 + kIoThreads; ++j) {
 				const uint32_t slot = srec[j].slot;
-				while (opi < ope && ops[opi].pos <= slot) { apply_op(ops[opi]); ++opi; }
-				int x0, y0, cx, cy;
-				pair_rect(j, x0, y0, cx, cy);
+				while (nextOp <= slot) { apply_op(ops[opi]); ++opi; nextOp = opi < ope ? ops[opi].pos : 0xffffffffu; }
+				const uint32_t xy = sxy[j], cxy = scxy[j];
+				const int x0 = static_cast<int>(xy & 0xffffu), y0 = static_cast<int>(xy >> 16);
+				const int cx = static_cast<int>(cxy & 0xffffu), cy = static_cast<int>(cxy >> 16);
 				if (qx < x0 || qy < y0) continue;
 				const int ix = (qx - x0) >> 1, iy = (qy - y0) >> 1;
 				if (ix >= cx || iy >= cy) continue;
@@ -462,40 +478,39 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 				const uint32_t pm = wmask[wi];
 				uint32_t mask = pm >> 4;
 				if (!mask) continue;
-				const int di = __builtin_amdgcn_readfirstlane(sdraw[j]);
-				const DrawDesc& dd = draws[di];
-				const DrawLite& dl = lite[di];
+				const uint32_t st = sstate[j];
+				const bool depthTest = (st & 1u) != 0u, depthWrite = (st & 2u) != 0u, earlyZ = (st & 64u) != 0u;
+				const int depthFunc = static_cast<int>((st >> 2) & 3u);
 				float fd[4];
 #pragma unroll
 				for (int l = 0; l < 4; ++l) fd[l] = wfd[wi][l];
-				const bool earlyZ = program_info(dd.program).earlyZ;
-				if (dd.depthTest && earlyZ) {
+				if (depthTest && earlyZ) {
 #pragma unroll
-					for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+					for (int l = 0; l < 4; ++l) if (!depth_pass(depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
 					if (!mask) continue; }
-				if (dd.depthWrite && earlyZ) {
+				if (depthWrite && earlyZ) {
 #pragma unroll
 					for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
 				mask &= pm & 0xfu;   // gl_triMask after ShadeFragment
-				if (dd.depthTest && !earlyZ) {
+				if (depthTest && !earlyZ) {
 #pragma unroll
-					for (int l = 0; l < 4; ++l) if (!depth_pass(dd.depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
+					for (int l = 0; l < 4; ++l) if (!depth_pass(depthFunc, fd[l], dz[l])) mask &= ~(1u << l);
 					if (!mask) continue; }
-				if (dd.depthWrite && !earlyZ) {
+				if (depthWrite && !earlyZ) {
 #pragma unroll
 					for (int l = 0; l < 4; ++l) if (mask & (1u << l)) dz[l] = fd[l]; }
-				if (dd.colorWrite) {
+				if (st & 16u) {
 #pragma unroll
 					for (int l = 0; l < 4; ++l) {
 						if (!(mask & (1u << l))) continue;
 						const float4 c = wcol[wi][l];
-						if (dd.blend) {   // BlendAlpha (rglv_gpu_impl.hxx:80-84)
+						if (st & 32u) {   // BlendAlpha (rglv_gpu_impl.hxx:80-84)
 							const float al = c.w, om = 1.0F - al;
 							cr[l] = c.x * al + cr[l] * om;
 							cg[l] = c.y * al + cg[l] * om;
 							cb[l] = c.z * al + cb[l] * om; }
 						else { cr[l] = c.x; cg[l] = c.y; cb[l] = c.z; }}}
-				const uint32_t prim = dl.primBase + (ssrc[j] & 0x7fffffffu);
+				const uint32_t prim = sprim[j];
 #pragma unroll
 				for (int l = 0; l < 4; ++l) if (mask & (1u << l)) pid[l] = prim; }
 			__syncthreads(); }}
