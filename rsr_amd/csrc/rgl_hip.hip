@@ -1166,7 +1166,7 @@ struct rgl_gpu {
 		// double-buffered frame has finished
 		if (rasterAfter && (e = hipStreamWaitEvent(stream, rasterAfter, 0)) != hipSuccess) return hipfail(e, "hipStreamWaitEvent");
 		if (nIo > 0) {
-			InorderLaunch il{ *this, b, ra, sizeof(float) * kIoThreads * 3 * static_cast<size_t>(varyStride) };
+			InorderLaunch il{ *this, b, ra, 0 };
 			if (pid < 0 || !dispatch_program(pid, il, AllPrograms{})) il.launch<AllPrograms>(); }
 		Mark(6);
 		const unsigned rblocks = static_cast<unsigned>(std::max(b.nBlocks, 1));

@@ -217,11 +217,7 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 	__shared__ VisRec srec[kIoThreads];
 	__shared__ int sdraw[kIoThreads];
 	__shared__ uint32_t sfrag[kIoThreads / 64];
-	// each batch's triangles: 1/w of the three vertices, the source
-	// triangle (bit 31: a clipped fan), and the vertices' varyings at the
-	// batch's stride (dynamic LDS: kIoThreads x 3 x varyStride floats), loaded by
-	// one thread per triangle before the workgroup steps through them
-	__shared__ float sw[kIoThreads][3];
+	// each batch triangle's source triangle (bit 31: a clipped fan)
 	__shared__ uint32_t ssrc[kIoThreads];
 	__shared__ uint32_t scan_sh[kIoThreads / 64 + 1];
 	// a window of kIoThreads (triangle, quad) pairs, shaded in parallel: the
@@ -236,7 +232,6 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 	// 5 blend, 6 early depth test), its primitive id, and its pair rect in
 	// the region (x0 | y0 << 16, cx | cy << 16)
 	__shared__ uint32_t sstate[kIoThreads], sprim[kIoThreads], sxy[kIoThreads], scxy[kIoThreads];
-	extern __shared__ float svary[];
 	const int vs = a.varyStride;
 	const int blk = static_cast<int>(blockIdx.x);
 	const IoFrame e = io[find_io(io, nIo, blk, false)];
@@ -347,28 +342,12 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 			const DrawLite& dl = lite[di];
 			const uint32_t rel = rr.slot - dl.slotBase;
 			const uint32_t nTri = static_cast<uint32_t>(dl.tris) * static_cast<uint32_t>(dl.instances);
-			TriVerts tr;
-			uint32_t srcTri;
-			const float* vary;
-			if (rel < nTri) {
-				srcTri = rel;
-				inline_verts(a.tvert[dl.triBase + rel], a.vpos, tr, vs);
-				vary = a.vvary; }
-			else {
-				const uint32_t fslot = rel - nTri;
-				srcTri = fslot / kFanMax;
-				const uint32_t jf = fslot - srcTri * kFanMax;
-				clipped_verts(a.clipqOf[dl.triBase + srcTri] * kFanMax + jf, a.cvpos, tr, vs);
-				vary = a.cvvary; }
-			ssrc[t2] = srcTri | (tr.clipped ? 0x80000000u : 0u);
+			const uint32_t srcTri = rel < nTri ? rel : (rel - nTri) / kFanMax;
+			ssrc[t2] = srcTri | (rel < nTri ? 0u : 0x80000000u);
 			const DrawDesc& dd = draws[di];
 			sstate[t2] = (dd.depthTest ? 1u : 0u) | (dd.depthWrite ? 2u : 0u) | (static_cast<uint32_t>(dd.depthFunc) << 2) |
 			             (dd.colorWrite ? 16u : 0u) | (dd.blend ? 32u : 0u) | (program_info(dd.program).earlyZ ? 64u : 0u);
-			sprim[t2] = dl.primBase + srcTri;
-			float* sv = svary + static_cast<long long>(t2) * 3 * vs;
-			for (int k = 0; k < 3; ++k) {
-				sw[t2][k] = tr.w[k];
-				for (int c = 0; c < vs; ++c) sv[k * vs + c] = vary[tr.vary[k] + c]; }}
+			sprim[t2] = dl.primBase + srcTri; }
 		__syncthreads();
 		// the batch's (triangle, quad) pairs inside the region, triangle by
 		// triangle in order: triangle j's quads of its quad rect that lie in
@@ -442,12 +421,20 @@ __global__ __launch_bounds__(kIoThreads) void k_inorder(BatchDesc b, RasterArgs 
 						// the program on the triangle's vertices (1/w, varyings) from
 						// LDS: it has no side effects, so it runs before the quad's
 						// depth test, which the ordered pass below applies
-						const uint32_t src = ssrc[j];
-						const float w3[3] = { sw[j][0], sw[j][1], sw[j][2] };
-						const float* sv = svary + static_cast<long long>(j) * 3 * vs;
+						const DrawLite& dl = lite[di];
+						const uint32_t src = ssrc[j], srcTri = src & 0x7fffffffu;
+						TriVerts tr;
+						const float* vary;
+						if (!(src >> 31)) {
+							inline_verts(a.tvert[dl.triBase + srcTri], a.vpos, tr, vs);
+							vary = a.vvary; }
+						else {
+							const uint32_t fslot = r.slot - dl.slotBase - static_cast<uint32_t>(dl.tris) * static_cast<uint32_t>(dl.instances);
+							clipped_verts(a.clipqOf[dl.triBase + srcTri] * kFanMax + (fslot - srcTri * kFanMax), a.cvpos, tr, vs);
+							vary = a.cvvary; }
 						float col[4][4];
-						ShadeQuad sq{ dd, w3, sv, sv + vs, sv + 2 * vs,
-						              bx, by, bz, fd, px, py, f.H, (src >> 31) != 0u, col, 0xfu, true };
+						ShadeQuad sq{ dd, tr.w, vary + tr.vary[0], vary + tr.vary[1], vary + tr.vary[2],
+						              bx, by, bz, fd, px, py, f.H, tr.clipped, col, 0xfu, true };
 						dispatch_program(dd.program, sq, L{});
 						pm = (live & sq.keep) | (live << 4);
 #pragma unroll
